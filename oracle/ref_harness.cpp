@@ -1,0 +1,220 @@
+// Test infrastructure only: a C entry layer over the reference decoder, built
+// from the reference's own source file where it lies (the Makefile in this
+// directory passes its directory with -I and renames its main()).  Output goes
+// to oracle/_ref/ only.  Nothing in the product links or loads this.
+//
+// Every function below calls the reference's own routines:
+//   read_params / build_block_matrix_hx/hz / build_check_neighbors /
+//   build_var_adjacency / build_row_basis   jointbp_ets.cpp:98-180, 628-648, 845-935
+//   trial_seed_from_base + mt19937_64 + uniform_real_distribution sampling
+//                                           jointbp_ets.cpp:4408-4411, 5931-5946
+//   compute_syndrome                        jointbp_ets.cpp:1834-1849
+//   joint_bp_decode (PP off)                jointbp_ets.cpp:1943-3079
+//   fill_diff_bits / in_row_space           jointbp_ets.cpp:887-920, 6199-6207
+//   compute_girth / orthogonality           jointbp_ets.cpp:1018-1171
+// The only code written here is the loop glue that main() has inline
+// (jointbp_ets.cpp:5931-5948, 6199-6207), restated line for line.
+#include "jointbp_ets.cpp"
+
+#include <atomic>
+#include <thread>
+
+namespace {
+
+struct RefState {
+    std::vector<std::vector<int>> x_checks, z_checks;
+    std::vector<std::vector<EdgeRef>> var_to_x, var_to_z;
+    RowBasis hx_basis, hz_basis;
+    int nvars = 0;
+    bool ready = false;
+};
+
+RefState g_ref;
+
+void finish_code(int nvars) {
+    g_ref.nvars = nvars;
+    g_ref.var_to_x = build_var_adjacency(g_ref.x_checks, nvars);
+    g_ref.var_to_z = build_var_adjacency(g_ref.z_checks, nvars);
+    g_ref.hx_basis = build_row_basis(g_ref.x_checks, nvars);
+    g_ref.hz_basis = build_row_basis(g_ref.z_checks, nvars);
+    g_ref.ready = true;
+}
+
+Msg make_prior(double p_err) {
+    // jointbp_ets.cpp:5484-5485
+    Msg prior{1.0 - p_err, p_err / 3.0, p_err / 3.0, p_err / 3.0};
+    normalize_msg(prior);
+    return prior;
+}
+
+void sample_trial(uint64_t base_seed, long long trial_idx, double p_err, std::vector<int> &err) {
+    // jointbp_ets.cpp:5933-5946
+    std::uniform_real_distribution<double> dist(0.0, 1.0);
+    uint64_t trial_seed = trial_seed_from_base(base_seed, static_cast<uint64_t>(trial_idx));
+    std::mt19937_64 rng(trial_seed);
+    err.assign(g_ref.nvars, 0);
+    for (int i = 0; i < g_ref.nvars; ++i) {
+        double u = dist(rng);
+        if (u < (1.0 - p_err)) {
+            err[i] = 0;
+        } else {
+            double v = (u - (1.0 - p_err)) / p_err;
+            if (v < 1.0 / 3.0) err[i] = 1;
+            else if (v < 2.0 / 3.0) err[i] = 2;
+            else err[i] = 3;
+        }
+    }
+}
+
+JointBPResult decode_plain(const std::vector<int> &sx, const std::vector<int> &sz,
+                           const Msg &prior, int max_iter, double damping, bool freeze_syn) {
+    return joint_bp_decode(g_ref.x_checks, g_ref.z_checks, g_ref.var_to_x, g_ref.var_to_z,
+                           sx, sz, prior, max_iter, /*flip_hist_window=*/5, freeze_syn, damping,
+                           false, false, false, nullptr, nullptr,
+                           &g_ref.hx_basis, &g_ref.hz_basis, nullptr, nullptr);
+}
+
+bool stab_equiv(const std::vector<int> &err, const std::vector<int> &est) {
+    std::vector<uint64_t> dx(g_ref.hz_basis.words, 0), dz(g_ref.hx_basis.words, 0);
+    fill_diff_bits(dx, err, est, true);
+    fill_diff_bits(dz, err, est, false);
+    return in_row_space(g_ref.hz_basis, dx) && in_row_space(g_ref.hx_basis, dz);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Build the APM code from a params file with the reference's own constructors.
+int ref_load_params(const char *path) {
+    Params params;
+    if (!read_params(path, params)) return -1;
+    auto hx_blocks = build_block_matrix_hx(params.P, params.J, params.L2, params.a, params.b, params.c, params.d);
+    auto hz_blocks = build_block_matrix_hz(params.P, params.J, params.L2, params.a, params.b, params.c, params.d);
+    g_ref.x_checks = build_check_neighbors(hx_blocks, params.P);
+    g_ref.z_checks = build_check_neighbors(hz_blocks, params.P);
+    finish_code(params.L * static_cast<int>(params.P));
+    return 0;
+}
+
+// Install an arbitrary code from flat check lists (row-major, `deg` entries
+// per check, -1 = absent), e.g. a hypergraph-product code or a tiny code.
+int ref_set_checks(int nvars, int mx, int xdeg, const int *xflat, int mz, int zdeg, const int *zflat) {
+    g_ref.x_checks.assign(mx, std::vector<int>(xdeg));
+    g_ref.z_checks.assign(mz, std::vector<int>(zdeg));
+    for (int c = 0; c < mx; ++c)
+        for (int i = 0; i < xdeg; ++i) g_ref.x_checks[c][i] = xflat[c * xdeg + i];
+    for (int c = 0; c < mz; ++c)
+        for (int i = 0; i < zdeg; ++i) g_ref.z_checks[c][i] = zflat[c * zdeg + i];
+    finish_code(nvars);
+    return 0;
+}
+
+// out[0..8] = n, mx, mz, rank_x, rank_z, edges_x, edges_z, orthogonal, odd_pairs
+int ref_code_info(long long *out, int with_girth, int *girth_out) {
+    if (!g_ref.ready) return -1;
+    OrthCheckResult orth = check_hx_hz_orthogonality(g_ref.x_checks, g_ref.z_checks, g_ref.nvars);
+    DegreeStats xs = compute_var_degree_stats(g_ref.var_to_x);
+    DegreeStats zs = compute_var_degree_stats(g_ref.var_to_z);
+    out[0] = g_ref.nvars;
+    out[1] = static_cast<long long>(g_ref.x_checks.size());
+    out[2] = static_cast<long long>(g_ref.z_checks.size());
+    out[3] = static_cast<long long>(g_ref.hx_basis.rows.size());
+    out[4] = static_cast<long long>(g_ref.hz_basis.rows.size());
+    out[5] = xs.edges;
+    out[6] = zs.edges;
+    out[7] = orth.orthogonal ? 1 : 0;
+    out[8] = orth.odd_pairs;
+    if (with_girth && girth_out) {
+        girth_out[0] = compute_girth(g_ref.x_checks, g_ref.var_to_x);
+        girth_out[1] = compute_girth(g_ref.z_checks, g_ref.var_to_z);
+    }
+    return 0;
+}
+
+// Flat check lists of the loaded code (for comparing code construction).
+int ref_get_checks(int side, int *out_flat) {
+    const auto &ch = side == 0 ? g_ref.x_checks : g_ref.z_checks;
+    size_t k = 0;
+    for (const auto &row : ch)
+        for (int v : row) out_flat[k++] = v;
+    return static_cast<int>(k);
+}
+
+uint64_t ref_trial_seed(uint64_t base, uint64_t idx) { return trial_seed_from_base(base, idx); }
+
+void ref_sample(uint64_t base, long long idx, double p, int *err_out) {
+    std::vector<int> err;
+    sample_trial(base, idx, p, err);
+    std::copy(err.begin(), err.end(), err_out);
+}
+
+void ref_syndrome(const int *err_in, int *sx_out, int *sz_out) {
+    std::vector<int> err(err_in, err_in + g_ref.nvars);
+    auto sx = compute_syndrome(g_ref.x_checks, err, true);
+    auto sz = compute_syndrome(g_ref.z_checks, err, false);
+    std::copy(sx.begin(), sx.end(), sx_out);
+    std::copy(sz.begin(), sz.end(), sz_out);
+}
+
+// Decode one syndrome pair with PP off. Returns iterations; writes est and match flag.
+int ref_decode(const int *sx_in, const int *sz_in, const double *prior4, int max_iter,
+               double damping, int freeze_syn, int *est_out, int *syn_match) {
+    std::vector<int> sx(sx_in, sx_in + g_ref.x_checks.size());
+    std::vector<int> sz(sz_in, sz_in + g_ref.z_checks.size());
+    Msg prior{prior4[0], prior4[1], prior4[2], prior4[3]};
+    JointBPResult res = decode_plain(sx, sz, prior, max_iter, damping, freeze_syn != 0);
+    std::copy(res.est.begin(), res.est.end(), est_out);
+    *syn_match = res.syndrome_match ? 1 : 0;
+    return res.iterations;
+}
+
+int ref_stab_equiv(const int *err_in, const int *est_in) {
+    std::vector<int> err(err_in, err_in + g_ref.nvars), est(est_in, est_in + g_ref.nvars);
+    return stab_equiv(err, est) ? 1 : 0;
+}
+
+void ref_prior(double p, double *out4) {
+    Msg m = make_prior(p);
+    for (int i = 0; i < 4; ++i) out4[i] = m[i];
+}
+
+// The simulate loop (jointbp_ets.cpp:5931-6207, PP off) over trial indices
+// first_idx .. first_idx+count-1, spread over `threads` std::threads (the
+// reference runs them one after another; trials are independent).
+// Per trial: iterations, syndrome_match, success (= match && stabilizer
+// residual), exact (est == err).  Returns wall seconds of the loop.
+double ref_simulate(uint64_t base, long long first_idx, long long count, double p, int max_iter,
+                    double damping, int threads, int *iters, unsigned char *syn_ok,
+                    unsigned char *success, unsigned char *exact) {
+    Msg prior = make_prior(p);
+    std::atomic<long long> next{0};
+    auto worker = [&]() {
+        std::vector<int> err;
+        for (;;) {
+            long long t = next.fetch_add(1);
+            if (t >= count) break;
+            sample_trial(base, first_idx + t, p, err);
+            auto sx = compute_syndrome(g_ref.x_checks, err, true);
+            auto sz = compute_syndrome(g_ref.z_checks, err, false);
+            JointBPResult res = decode_plain(sx, sz, prior, max_iter, damping, false);
+            bool ok = res.syndrome_match;
+            bool eq = ok && stab_equiv(err, res.est);
+            iters[t] = res.iterations;
+            syn_ok[t] = ok ? 1 : 0;
+            success[t] = (ok && eq) ? 1 : 0;
+            exact[t] = (res.est == err) ? 1 : 0;
+        }
+    };
+    auto t0 = std::chrono::steady_clock::now();
+    if (threads <= 1) {
+        worker();
+    } else {
+        std::vector<std::thread> pool;
+        for (int i = 0; i < threads; ++i) pool.emplace_back(worker);
+        for (auto &th : pool) th.join();
+    }
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // extern "C"
