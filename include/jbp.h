@@ -1,0 +1,172 @@
+/*
+ * jbp.h — C ABI of the MI355X joint belief-propagation decoder library
+ * (libjbp.so).  Plain C types only: pointers, sizes, status codes.
+ *
+ * It replaces the reference's C++ GPU boundary jointbp_cuda.h:1-93
+ * (CudaBPGraph / CudaBPResult / cuda_bp_create / cuda_bp_decode /
+ * cuda_bp_destroy) and moves the Monte-Carlo inner loop of
+ * jointbp_ets.cpp:5931-6207 (sampling, syndromes, decode, stabilizer-residual
+ * check) onto the device as a batched call.
+ *
+ * Threading: a context owns a HIP stream and device buffers and is not
+ * thread-safe; use one context per host thread.
+ */
+#ifndef JBP_H_
+#define JBP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  Every entry point returns one of these; on failure the
+ * caller's error buffer (when given) holds a NUL-terminated message, the way
+ * the reference fills *error_out (jointbp_cuda.cu:565-573). */
+#define JBP_OK 0
+#define JBP_ERR_ARG 1         /* invalid argument / size mismatch           */
+#define JBP_ERR_DEVICE 2      /* HIP runtime failure                        */
+#define JBP_ERR_UNSUPPORTED 3 /* option this build does not run on the GPU  */
+#define JBP_ERR_NOMEM 4       /* device allocation failed                   */
+
+/* Arithmetic of the decoder. */
+#define JBP_FP32 32 /* production: FP32 log-likelihood-ratio messages       */
+#define JBP_FP64 64 /* parity: FP64 probability messages, reference order   */
+
+typedef struct jbp_ctx jbp_ctx;
+
+/* Tanner-graph description; field for field the reference's CudaBPGraph
+ * (jointbp_cuda.h:10-30) as built by build_cuda_graph (jointbp_ets.cpp:937-1016):
+ * check-major edge numbering; x_var_edges lists each variable's edges in
+ * ascending check order.  The arrays are copied during jbp_create; the
+ * caller keeps ownership. */
+typedef struct jbp_graph_desc {
+    int nvars, mX, mZ, x_edges, z_edges;
+    const int *x_check_offsets; /* mX+1 */
+    const int *x_check_edges;   /* x_edges (identity in the reference)       */
+    const int *x_edge_var;      /* x_edges                                   */
+    const int *x_edge_check;    /* x_edges                                   */
+    const int *x_edge_pos;      /* x_edges                                   */
+    const int *x_var_offsets;   /* nvars+1                                   */
+    const int *x_var_edges;     /* x_edges                                   */
+    const int *z_check_offsets;
+    const int *z_check_edges;
+    const int *z_edge_var;
+    const int *z_edge_check;
+    const int *z_edge_pos;
+    const int *z_var_offsets;
+    const int *z_var_edges;
+} jbp_graph_desc;
+
+/* Per-decode result; the scalar fields of CudaBPResult (jointbp_cuda.h:32-43).
+ * The estimate goes to the caller's est_out array instead of a std::vector. */
+typedef struct jbp_decode_result {
+    int iterations;     /* k at first syndrome match, else max_iter         */
+    int check_count;    /* syndrome tests performed                         */
+    int syndrome_match; /* 1 when both syndromes of est match the input     */
+    double kernel_ms, memcpy_ms, check_kernel_ms, check_memcpy_ms, init_kernel_ms, host_ms;
+} jbp_decode_result;
+
+/* cuda_bp_create (jointbp_cuda.h:48, jointbp_cuda.cu:589-642): validate the
+ * graph, select the device, upload the graph and allocate working buffers.
+ * Codes whose checks are built from affine permutation blocks of one size P
+ * (the params-file codes, jointbp_ets.cpp:98-160) are detected from the CSR
+ * and get the LDS-resident FP32 decoder; any other code runs the generic
+ * CSR decoders.  On failure *out is NULL. */
+int jbp_create(const jbp_graph_desc *graph, int device_id, jbp_ctx **out, char *err, size_t errlen);
+
+/* cuda_bp_destroy (jointbp_cuda.h:49, jointbp_cuda.cu:644-660); NULL is a no-op. */
+void jbp_destroy(jbp_ctx *ctx);
+
+/* cuda_bp_decode (jointbp_cuda.h:50-63, jointbp_cuda.cu:681-1104): decode one
+ * syndrome pair (sx: mX ints, sz: mZ ints, nonzero = 1) with the given prior
+ * over (I, X, Z, Y).  est_out receives nvars states 0..3.  precision is
+ * JBP_FP32 or JBP_FP64.  check_warmup / check_interval skip syndrome tests as
+ * the reference's flags do (FP32 only; FP64 tests every iteration).
+ * freeze_syn is not run on the GPU: it returns JBP_ERR_UNSUPPORTED so that
+ * the caller falls back to its CPU decoder, as the reference caller does on
+ * any decode failure (jointbp_ets.cpp:5961-5985). */
+int jbp_decode(jbp_ctx *ctx, const int *sx, const int *sz, const double prior[4], int max_iter,
+               int check_warmup, int check_interval, int measure_costs, int use_graph,
+               int freeze_syn, double damping, int precision, jbp_decode_result *out,
+               int *est_out, char *err, size_t errlen);
+
+/* Batched form of jbp_decode: `count` independent syndrome pairs, packed as
+ * bits (sx_bits: count x ceil(mX/32) uint32 words, bit c of word c/32; sz
+ * likewise).  Outputs per trial: iterations, syndrome_match and, when
+ * est_out is non-NULL, count x nvars states. */
+int jbp_decode_batch(jbp_ctx *ctx, int count, const uint32_t *sx_bits, const uint32_t *sz_bits,
+                     const double prior[4], int max_iter, double damping, int precision,
+                     int *iterations_out, uint8_t *syn_match_out, int *est_out, char *err,
+                     size_t errlen);
+
+/* Monte-Carlo configuration: the inner loop of the reference's --simulate
+ * path (jointbp_ets.cpp:5931-6207) with PP off. */
+typedef struct jbp_sim_cfg {
+    uint64_t base_seed;  /* --seed                                            */
+    int64_t first_trial; /* trial index of the first trial (reference: 1)     */
+    int64_t n_trials;    /* trials to run                                     */
+    double p;            /* depolarizing rate: X, Y, Z each with p/3          */
+    int max_iter;        /* --max-iter                                        */
+    double damping;      /* --damping                                         */
+    int precision;       /* JBP_FP32 or JBP_FP64                              */
+    int check_warmup;    /* --cuda-check-warmup (FP32)                        */
+    int check_interval;  /* --cuda-check-interval (FP32)                      */
+    int slots;           /* trials resident at once; 0 = library default      */
+    int block;           /* threads per block for the generic decoders; 0 = tuned */
+    int64_t min_fails;   /* >0: stop at the first trial index where the
+                            failure count reaches this (reference 6409-6411) */
+} jbp_sim_cfg;
+
+/* Per-trial outcome (structure of arrays; any pointer may be NULL). */
+typedef struct jbp_trial_out {
+    int *iterations;          /* n_trials                                    */
+    uint8_t *syn_match;       /* n_trials                                    */
+    uint8_t *success;         /* syndrome match and residual a stabilizer    */
+    uint8_t *exact;           /* est == err                                  */
+    int *err;                 /* n_trials x nvars, sampled error (optional)  */
+    int *est;                 /* n_trials x nvars, final estimate (optional) */
+} jbp_trial_out;
+
+/* Aggregates in the reference's Run Summary terms (jointbp_ets.cpp:6414-6449). */
+typedef struct jbp_run_stats {
+    int64_t done, failures, bp_failures, exact, total_iters;
+    double device_ms;  /* device time of the decode launches (HIP events) */
+    double elapsed_ms; /* wall time of the whole call                      */
+} jbp_run_stats;
+
+/* Sample, decode and judge trials first_trial .. first_trial+n_trials-1 on
+ * the device: per-trial seed splitmix64(base ^ idx*0x9e3779b97f4a7c15),
+ * mt19937_64 + libstdc++ generate_canonical draws, depolarizing thresholds,
+ * syndromes, decode, stabilizer-residual test.  Bit-exact with the reference
+ * for the sampled errors, syndromes and residual tests. */
+int jbp_simulate(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *per_trial,
+                 jbp_run_stats *stats, char *err, size_t errlen);
+
+/* Sample only: errors of trials first..first+count-1 (count x nvars states)
+ * and their syndromes (count x mX, count x mZ ints; either may be NULL). */
+int jbp_sample(jbp_ctx *ctx, uint64_t base_seed, int64_t first_trial, int count, double p,
+               int *err_out, int *sx_out, int *sz_out, char *err, size_t errlen);
+
+/* Row-space membership of count bit vectors (count x nvars ints, nonzero = 1)
+ * in the row space of Hx (side 0) or Hz (side 1): out[i] = 1 when vector i
+ * is a sum of rows (the in_row_space test of jointbp_ets.cpp:904-920). */
+int jbp_in_row_space(jbp_ctx *ctx, int side, int count, const int *bits, uint8_t *out, char *err,
+                     size_t errlen);
+
+/* Device time of the most recent launch of each kernel family, for
+ * profiling: which = 0 sample, 1 decode, 2 residual check.  Returns ms and
+ * the launch count since the context was created. */
+double jbp_last_kernel_ms(jbp_ctx *ctx, int which, int64_t *launches);
+
+/* Context facts: out[0] = 1 when the LDS-resident FP32 decoder applies,
+ * out[1] = permutation size P, out[2] = J, out[3] = L, out[4] = device id,
+ * out[5] = compute units, out[6] = tuned threads per block. */
+int jbp_info(jbp_ctx *ctx, int64_t out[8]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JBP_H_ */

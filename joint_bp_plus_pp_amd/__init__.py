@@ -1,0 +1,342 @@
+"""MI355X joint belief-propagation decoder: Python host over libjbp's C ABI.
+
+The library (libjbp.so, built from csrc/ into this directory) holds the HIP
+kernels for gfx950 and the C entry points declared in include/jbp.h and
+include/jbp_code.h.  This module is a thin ctypes layer with the reference
+GPU boundary's vocabulary (jointbp_cuda.h): a graph, a context created on a
+device, single decodes, and the batched Monte-Carlo call.
+
+There is no CPU fallback here: if libjbp.so is missing, importing the
+bindings raises.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+__all__ = ["Code", "Decoder", "JbpError", "lib", "LIB_PATH", "FP32", "FP64", "prior_from_p"]
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libjbp.so")
+INCLUDE_DIR = os.path.join(os.path.dirname(PKG_DIR), "include")
+
+FP32 = 32
+FP64 = 64
+STATUS = {0: "ok", 1: "invalid argument", 2: "device error", 3: "unsupported", 4: "out of memory"}
+
+
+class JbpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (status %d: %s)" % (msg, code, STATUS.get(code, "?")))
+        self.code = code
+
+
+class GraphDesc(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nvars", "mX", "mZ", "x_edges", "z_edges")] + [
+        (n, C.POINTER(C.c_int)) for n in (
+            "x_check_offsets", "x_check_edges", "x_edge_var", "x_edge_check", "x_edge_pos",
+            "x_var_offsets", "x_var_edges", "z_check_offsets", "z_check_edges", "z_edge_var",
+            "z_edge_check", "z_edge_pos", "z_var_offsets", "z_var_edges")]
+
+
+class DecodeResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("check_count", C.c_int), ("syndrome_match", C.c_int)] + [
+        (n, C.c_double) for n in ("kernel_ms", "memcpy_ms", "check_kernel_ms", "check_memcpy_ms",
+                                  "init_kernel_ms", "host_ms")]
+
+
+class SimCfg(C.Structure):
+    _fields_ = [("base_seed", C.c_uint64), ("first_trial", C.c_int64), ("n_trials", C.c_int64),
+                ("p", C.c_double), ("max_iter", C.c_int), ("damping", C.c_double),
+                ("precision", C.c_int), ("check_warmup", C.c_int), ("check_interval", C.c_int),
+                ("slots", C.c_int), ("block", C.c_int), ("min_fails", C.c_int64)]
+
+
+class TrialOut(C.Structure):
+    _fields_ = [("iterations", C.POINTER(C.c_int)), ("syn_match", C.POINTER(C.c_uint8)),
+                ("success", C.POINTER(C.c_uint8)), ("exact", C.POINTER(C.c_uint8)),
+                ("err", C.POINTER(C.c_int)), ("est", C.POINTER(C.c_int))]
+
+
+class RunStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("done", "failures", "bp_failures", "exact", "total_iters")] + [
+        ("device_ms", C.c_double), ("elapsed_ms", C.c_double)]
+
+
+_LIB = None
+
+
+def lib():
+    """Load libjbp.so (once).  Raises when it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libjbp.so not built (%s); run __graft_entry__.build() or make -C "
+                          "joint_bp_plus_pp_amd/csrc" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp, cp, sz, i32, i64, u64, dbl = C.c_void_p, C.c_char_p, C.c_size_t, C.c_int, C.c_int64, C.c_uint64, C.c_double
+    sig = {
+        "jbp_create": (i32, [C.POINTER(GraphDesc), i32, C.POINTER(vp), cp, sz]),
+        "jbp_destroy": (None, [vp]),
+        "jbp_decode": (i32, [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, dbl, i32,
+                             C.POINTER(DecodeResult), vp, cp, sz]),
+        "jbp_decode_batch": (i32, [vp, i32, vp, vp, vp, i32, dbl, i32, vp, vp, vp, cp, sz]),
+        "jbp_simulate": (i32, [vp, C.POINTER(SimCfg), C.POINTER(TrialOut), C.POINTER(RunStats), cp, sz]),
+        "jbp_sample": (i32, [vp, u64, i64, i32, dbl, vp, vp, vp, cp, sz]),
+        "jbp_in_row_space": (i32, [vp, i32, i32, vp, vp, cp, sz]),
+        "jbp_last_kernel_ms": (dbl, [vp, i32, C.POINTER(i64)]),
+        "jbp_info": (i32, [vp, vp]),
+        "jbp_code_from_params": (i32, [cp, C.POINTER(vp), cp, sz]),
+        "jbp_code_from_checks": (i32, [i32, i32, i32, vp, i32, i32, vp, C.POINTER(vp), cp, sz]),
+        "jbp_code_hgp": (i32, [i32, i32, i32, i32, u64, C.POINTER(vp), cp, sz]),
+        "jbp_code_free": (None, [vp]),
+        "jbp_code_dims": (i32, [vp, vp]),
+        "jbp_code_graph": (i32, [vp, C.POINTER(GraphDesc)]),
+        "jbp_code_params": (i32, [vp, vp]),
+        "jbp_code_checks": (i32, [vp, i32, vp]),
+        "jbp_code_diagnostics": (i32, [vp, i32, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def header_symbols():
+    """Every function name declared in include/*.h."""
+    names = []
+    for h in sorted(os.listdir(INCLUDE_DIR)):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE_DIR, h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"\b(jbp_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _errbuf():
+    return C.create_string_buffer(512)
+
+
+def _check(rc, buf):
+    if rc != 0:
+        raise JbpError(rc, buf.value.decode(errors="replace") or "libjbp call failed")
+
+
+def prior_from_p(p):
+    """normalize_msg({1-p, p/3, p/3, p/3}) (jointbp_ets.cpp:5484-5485)."""
+    m = [1.0 - p, p / 3.0, p / 3.0, p / 3.0]
+    s = ((m[0] + m[1]) + m[2]) + m[3]
+    return np.array([0.25] * 4 if s <= 0 else [v / s for v in m], dtype=np.float64)
+
+
+class Code:
+    """A CSS code's two Tanner graphs (host side)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+        dims = np.zeros(8, np.int64)
+        lib().jbp_code_dims(self._h, _ptr(dims))
+        self.n, self.mx, self.mz, self.x_edges, self.z_edges, self.P, self.J, self.L = map(int, dims)
+
+    @classmethod
+    def from_params(cls, path):
+        h, e = C.c_void_p(), _errbuf()
+        _check(lib().jbp_code_from_params(os.fspath(path).encode(), C.byref(h), e, 512), e)
+        return cls(h.value)
+
+    @classmethod
+    def from_checks(cls, n, x_checks, z_checks):
+        xs = np.ascontiguousarray(x_checks, dtype=np.int32)
+        zs = np.ascontiguousarray(z_checks, dtype=np.int32)
+        h, e = C.c_void_p(), _errbuf()
+        _check(lib().jbp_code_from_checks(n, xs.shape[0], xs.shape[1], _ptr(xs), zs.shape[0],
+                                          zs.shape[1], _ptr(zs), C.byref(h), e, 512), e)
+        return cls(h.value)
+
+    @classmethod
+    def hgp(cls, m=150, n=200, col_w=3, row_w=4, seed=1):
+        h, e = C.c_void_p(), _errbuf()
+        _check(lib().jbp_code_hgp(m, n, col_w, row_w, seed, C.byref(h), e, 512), e)
+        return cls(h.value)
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().jbp_code_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def graph(self):
+        g = GraphDesc()
+        lib().jbp_code_graph(self._h, C.byref(g))
+        return g
+
+    def checks(self, side):
+        m = self.mx if side == 0 else self.mz
+        deg = lib().jbp_code_checks(self._h, side, None)
+        out = np.zeros((m, max(deg, 0)), np.int32)
+        lib().jbp_code_checks(self._h, side, _ptr(out))
+        return out
+
+    def params(self):
+        out = np.zeros(2 * self.L, np.int64)
+        lib().jbp_code_params(self._h, _ptr(out))
+        return out.reshape(4, -1)
+
+    def diagnostics(self, girth=True):
+        out = np.zeros(15, np.int64)
+        lib().jbp_code_diagnostics(self._h, 1 if girth else 0, _ptr(out))
+        keys = ["rank_x", "rank_z", "x_var_deg_min", "x_var_deg_max", "x_check_deg_min",
+                "x_check_deg_max", "z_var_deg_min", "z_var_deg_max", "z_check_deg_min",
+                "z_check_deg_max", "odd_pairs", "girth_x", "girth_z"]
+        return dict(zip(keys, map(int, out[:13])))
+
+    def csr(self, side):
+        """(check_offsets, edge_var) of one side as numpy arrays."""
+        g = self.graph()
+        m = self.mx if side == 0 else self.mz
+        E = self.x_edges if side == 0 else self.z_edges
+        off = g.x_check_offsets if side == 0 else g.z_check_offsets
+        ev = g.x_edge_var if side == 0 else g.z_edge_var
+        return (np.ctypeslib.as_array(off, shape=(m + 1,)).copy(),
+                np.ctypeslib.as_array(ev, shape=(E,)).copy())
+
+
+class Decoder:
+    """A decoder context on one device: cuda_bp_create / cuda_bp_decode /
+    cuda_bp_destroy (jointbp_cuda.h:48-63) plus the batched calls."""
+
+    def __init__(self, code, device=0):
+        self.code = code
+        self._graph = code.graph()
+        h, e = C.c_void_p(), _errbuf()
+        _check(lib().jbp_create(C.byref(self._graph), device, C.byref(h), e, 512), e)
+        self._h = h
+
+    def destroy(self):
+        if self._h and self._h.value:
+            lib().jbp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    close = destroy
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.destroy()
+
+    def info(self):
+        out = np.zeros(8, np.int64)
+        lib().jbp_info(self._h, _ptr(out))
+        return dict(zip(["qc", "P", "J", "L", "device", "cus", "block", "basis"], map(int, out)))
+
+    def last_kernel_ms(self, which):
+        n = C.c_int64(0)
+        ms = lib().jbp_last_kernel_ms(self._h, which, C.byref(n))
+        return ms, n.value
+
+    def decode(self, sx, sz, prior, max_iter=50, check_warmup=0, check_interval=1, damping=0.0,
+               precision=FP32, freeze_syn=False, use_graph=False):
+        """One syndrome pair -> (est, iterations, syndrome_match, result)."""
+        sx = np.ascontiguousarray(sx, dtype=np.int32)
+        sz = np.ascontiguousarray(sz, dtype=np.int32)
+        pr = np.ascontiguousarray(prior, dtype=np.float64)
+        est = np.zeros(self.code.n, np.int32)
+        res, e = DecodeResult(), _errbuf()
+        _check(lib().jbp_decode(self._h, _ptr(sx), _ptr(sz), _ptr(pr), max_iter, check_warmup,
+                                check_interval, 0, int(use_graph), int(freeze_syn), damping,
+                                precision, C.byref(res), _ptr(est), e, 512), e)
+        return est, res.iterations, bool(res.syndrome_match), res
+
+    def decode_batch(self, sx, sz, prior, max_iter=50, damping=0.0, precision=FP32, want_est=True):
+        """Batched decode of int syndrome arrays (count x m)."""
+        sx = np.atleast_2d(np.asarray(sx))
+        sz = np.atleast_2d(np.asarray(sz))
+        count = sx.shape[0]
+        bx, bz = pack_bits(sx), pack_bits(sz)
+        pr = np.ascontiguousarray(prior, dtype=np.float64)
+        it = np.zeros(count, np.int32)
+        ok = np.zeros(count, np.uint8)
+        est = np.zeros((count, self.code.n), np.int32) if want_est else None
+        e = _errbuf()
+        _check(lib().jbp_decode_batch(self._h, count, _ptr(bx), _ptr(bz), _ptr(pr), max_iter,
+                                      damping, precision, _ptr(it), _ptr(ok), _ptr(est), e, 512), e)
+        return it, ok, est
+
+    def sample(self, base_seed, first_trial, count, p):
+        """Device-sampled errors and syndromes of trials first..first+count-1."""
+        err = np.zeros((count, self.code.n), np.int32)
+        sx = np.zeros((count, self.code.mx), np.int32)
+        sz = np.zeros((count, self.code.mz), np.int32)
+        e = _errbuf()
+        _check(lib().jbp_sample(self._h, base_seed, first_trial, count, p, _ptr(err), _ptr(sx),
+                                _ptr(sz), e, 512), e)
+        return err, sx, sz
+
+    def in_row_space(self, side, bits):
+        bits = np.ascontiguousarray(np.atleast_2d(bits), dtype=np.int32)
+        out = np.zeros(bits.shape[0], np.uint8)
+        e = _errbuf()
+        _check(lib().jbp_in_row_space(self._h, side, bits.shape[0], _ptr(bits), _ptr(out), e, 512), e)
+        return out.astype(bool)
+
+    def simulate(self, base_seed, n_trials, p, max_iter=50, first_trial=1, damping=0.0,
+                 precision=FP32, check_warmup=0, check_interval=1, min_fails=0, slots=0, block=0,
+                 per_trial=True, want_err=False, want_est=False):
+        """The --simulate --no-pp inner loop on the device.  Returns (stats,
+        per-trial dict or None)."""
+        cfg = SimCfg(base_seed, first_trial, n_trials, p, max_iter, damping, precision, check_warmup,
+                     check_interval, slots, block, min_fails)
+        out = None
+        arrays = {}
+        if per_trial:
+            arrays = {"iterations": np.zeros(n_trials, np.int32),
+                      "syn_match": np.zeros(n_trials, np.uint8),
+                      "success": np.zeros(n_trials, np.uint8),
+                      "exact": np.zeros(n_trials, np.uint8)}
+            if want_err:
+                arrays["err"] = np.zeros((n_trials, self.code.n), np.int32)
+            if want_est:
+                arrays["est"] = np.zeros((n_trials, self.code.n), np.int32)
+            out = TrialOut()
+            for k, a in arrays.items():
+                setattr(out, k, a.ctypes.data_as(C.POINTER(C.c_int if a.dtype == np.int32 else C.c_uint8)))
+        st, e = RunStats(), _errbuf()
+        _check(lib().jbp_simulate(self._h, C.byref(cfg), C.byref(out) if out else None, C.byref(st),
+                                  e, 512), e)
+        stats = {k: getattr(st, k) for k, _ in RunStats._fields_}
+        if per_trial:
+            d = int(stats["done"])
+            arrays = {k: v[:d] for k, v in arrays.items()}
+        return stats, (arrays if per_trial else None)
+
+
+def pack_bits(a):
+    """int arrays (count x len, nonzero = 1) -> uint32 bit planes (count x words)."""
+    a = np.atleast_2d(np.asarray(a)) != 0
+    count, length = a.shape
+    words = max(1, (length + 31) // 32)
+    pad = np.zeros((count, words * 32), dtype=bool)
+    pad[:, :length] = a
+    b = pad.reshape(count, words, 32).astype(np.uint64)
+    return np.ascontiguousarray((b << np.arange(32, dtype=np.uint64)).sum(axis=2).astype(np.uint32))

@@ -1,0 +1,794 @@
+// Host side of libjbp: the C ABI declared in include/jbp.h.
+//
+// Replaces the reference's CUDA backend host code (jointbp_cuda.cu:518-1104:
+// context, uploads, per-trial decode loop) with a batched design: a call
+// hands the device a whole range of trials; sampling, syndromes, decoding
+// and the stabilizer-residual test all run there, and only per-trial
+// outcomes (iterations, flags) come back.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/jbp.h"
+#include "jbp_common.h"
+
+using namespace jbp;
+
+namespace {
+
+// Largest dense basis (bytes per side) built for the device residual test.
+constexpr size_t kMaxBasisBytes = size_t(256) << 20;
+constexpr int kDefaultChunk = 65536;
+
+void set_err(char *err, size_t len, const char *fmt, ...) {
+    if (!err || !len) return;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(err, len, fmt, ap);
+    va_end(ap);
+}
+
+// Host-side fault injection for CPU-only tests of the caller's fallback:
+// JBP_FAULT_INJECT=create|decode|simulate makes that entry point fail.
+bool fault(const char *what) {
+    const char *v = std::getenv("JBP_FAULT_INJECT");
+    return v && std::strstr(v, what) != nullptr;
+}
+
+struct HostSide {
+    int m = 0, E = 0;
+    std::vector<int> chk_off, edge_var, var_off, var_edge;
+};
+
+}  // namespace
+
+struct jbp_ctx {
+    int device = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int n = 0;
+    HostSide hs[2];
+    DevGraph dg;
+    std::vector<void *> allocs;  // graph arrays, basis, tables
+    bool qc = false;
+    QcParams qcp;
+    int *d_qc_tables = nullptr;
+    bool have_basis = false;
+    DevBasis basis[2];
+    // chunk buffers
+    int chunk_cap = 0;
+    TrialPlanes tp;
+    std::vector<void *> chunk_allocs;
+    double *f64_scratch = nullptr;
+    int f64_slots = 0;
+    int f64_block = 256;
+    int *d_counter = nullptr;
+    int *d_states = nullptr;  // unpack buffer
+    size_t d_states_cap = 0;
+    int *d_ints = nullptr;    // pack input buffer
+    size_t d_ints_cap = 0;
+    double last_ms[3] = {0, 0, 0};
+    int64_t launches[3] = {0, 0, 0};
+};
+
+namespace {
+
+#define HIPCHK(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            set_err(err, errlen, "%s failed: %s", #call, hipGetErrorString(e_));        \
+            return JBP_ERR_DEVICE;                                                      \
+        }                                                                               \
+    } while (0)
+
+template <typename T>
+int dev_upload(jbp_ctx *ctx, const std::vector<T> &h, const T **out, char *err, size_t errlen) {
+    void *p = nullptr;
+    size_t bytes = std::max<size_t>(1, h.size()) * sizeof(T);
+    HIPCHK(hipMalloc(&p, bytes));
+    ctx->allocs.push_back(p);
+    if (!h.empty()) HIPCHK(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    *out = static_cast<const T *>(p);
+    return JBP_OK;
+}
+
+int validate_side(const char *name, int nvars, int m, int E, const int *chk_off, const int *edge_var,
+                  const int *var_off, const int *var_edge, HostSide &hs, char *err,
+                  size_t errlen) {
+    if (m < 0 || E < 0 || !chk_off || !edge_var || !var_off || !var_edge) {
+        set_err(err, errlen, "%s graph arrays missing", name);
+        return JBP_ERR_ARG;
+    }
+    hs.m = m;
+    hs.E = E;
+    hs.chk_off.assign(chk_off, chk_off + m + 1);
+    hs.edge_var.assign(edge_var, edge_var + E);
+    hs.var_off.assign(var_off, var_off + nvars + 1);
+    hs.var_edge.assign(var_edge, var_edge + E);
+    if (hs.chk_off[0] != 0 || hs.chk_off[m] != E || hs.var_off[0] != 0 || hs.var_off[nvars] != E) {
+        set_err(err, errlen, "%s offsets do not span the %d edges", name, E);
+        return JBP_ERR_ARG;
+    }
+    for (int c = 0; c < m; ++c)
+        if (hs.chk_off[c + 1] < hs.chk_off[c]) {
+            set_err(err, errlen, "%s check offsets not monotone at %d", name, c);
+            return JBP_ERR_ARG;
+        }
+    std::vector<int> seen(E, 0);
+    for (int v = 0; v < nvars; ++v) {
+        if (hs.var_off[v + 1] < hs.var_off[v]) {
+            set_err(err, errlen, "%s var offsets not monotone at %d", name, v);
+            return JBP_ERR_ARG;
+        }
+        for (int k = hs.var_off[v]; k < hs.var_off[v + 1]; ++k) {
+            int e = hs.var_edge[k];
+            if (e < 0 || e >= E || hs.edge_var[e] != v || seen[e]++) {
+                set_err(err, errlen, "%s var_edges inconsistent at var %d", name, v);
+                return JBP_ERR_ARG;
+            }
+        }
+    }
+    for (int e = 0; e < E; ++e)
+        if (hs.edge_var[e] < 0 || hs.edge_var[e] >= nvars) {
+            set_err(err, errlen, "%s edge %d has variable out of range", name, e);
+            return JBP_ERR_ARG;
+        }
+    return JBP_OK;
+}
+
+long long gcdll(long long a, long long b) {
+    while (b) { long long t = a % b; a = b; b = t; }
+    return a < 0 ? -a : a;
+}
+
+long long inv_mod(long long a, long long m) {
+    long long g = m, x = 0, x1 = 1, aa = ((a % m) + m) % m;
+    while (aa) {
+        long long q = g / aa;
+        long long t = g - q * aa; g = aa; aa = t;
+        t = x - q * x1; x = x1; x1 = t;
+    }
+    return g == 1 ? ((x % m) + m) % m : -1;
+}
+
+// Detect the affine permutation-block structure of both sides from the CSR.
+bool detect_qc(const HostSide hs[2], int n, QcParams &qc) {
+    if (hs[0].m == 0 || hs[0].m != hs[1].m) return false;
+    int L = hs[0].chk_off[1] - hs[0].chk_off[0];
+    if (L <= 0 || n % L != 0) return false;
+    int P = n / L;
+    if (P < 2 || hs[0].m % P != 0) return false;
+    int J = hs[0].m / P;
+    if (J * L > kQcMaxBlocks) return false;
+    for (int s = 0; s < 2; ++s) {
+        const HostSide &h = hs[s];
+        for (int c = 0; c < h.m; ++c)
+            if (h.chk_off[c + 1] - h.chk_off[c] != L) return false;
+        for (int r = 0; r < J; ++r)
+            for (int cc = 0; cc < L; ++cc) {
+                auto y_of = [&](int x) {
+                    int v = h.edge_var[h.chk_off[r * P + x] + cc];
+                    return v / P == cc ? v % P : -1;
+                };
+                int y0 = y_of(0), y1 = y_of(1);
+                if (y0 < 0 || y1 < 0) return false;
+                long long a = ((y1 - y0) % P + P) % P;
+                if (gcdll(a, P) != 1) return false;
+                for (int x = 0; x < P; ++x)
+                    if (y_of(x) != static_cast<int>((a * x + y0) % P)) return false;
+                qc.a[s][r * L + cc] = static_cast<int>(a);
+                qc.b[s][r * L + cc] = y0;
+                qc.ainv[s][r * L + cc] = static_cast<int>(inv_mod(a, P));
+            }
+    }
+    qc.P = P;
+    qc.J = J;
+    qc.L = L;
+    return true;
+}
+
+// Reduced row-echelon basis of the rows of one side over GF(2).
+void build_rref(const HostSide &h, int n, std::vector<uint32_t> &rows32, std::vector<int> &pivots,
+                int &rank) {
+    const int W = (n + 63) / 64;
+    std::vector<std::vector<uint64_t>> rows(h.m, std::vector<uint64_t>(W, 0));
+    for (int c = 0; c < h.m; ++c)
+        for (int e = h.chk_off[c]; e < h.chk_off[c + 1]; ++e) {
+            int v = h.edge_var[e];
+            rows[c][v >> 6] ^= 1ull << (v & 63);
+        }
+    rank = 0;
+    pivots.clear();
+    for (int col = 0; col < n && rank < h.m; ++col) {
+        const int w = col >> 6;
+        const uint64_t bit = 1ull << (col & 63);
+        int sel = -1;
+        for (int r = rank; r < h.m; ++r)
+            if (rows[r][w] & bit) { sel = r; break; }
+        if (sel < 0) continue;
+        std::swap(rows[rank], rows[sel]);
+        const std::vector<uint64_t> &pr = rows[rank];
+        for (int r = 0; r < h.m; ++r) {
+            if (r == rank || !(rows[r][w] & bit)) continue;
+            uint64_t *dst = rows[r].data();
+            for (int k = w; k < W; ++k) dst[k] ^= pr[k];
+            // words below w are zero in pr: columns < col are pivots or eliminated
+        }
+        pivots.push_back(col);
+        ++rank;
+    }
+    const int W32 = (n + 31) / 32;
+    rows32.assign(static_cast<size_t>(rank) * W32, 0u);
+    for (int r = 0; r < rank; ++r)
+        for (int k = 0; k < W32; ++k)
+            rows32[static_cast<size_t>(r) * W32 + k] =
+                static_cast<uint32_t>(rows[r][k >> 1] >> (32 * (k & 1)));
+}
+
+void free_chunk(jbp_ctx *ctx) {
+    for (void *p : ctx->chunk_allocs) (void)hipFree(p);
+    ctx->chunk_allocs.clear();
+    ctx->chunk_cap = 0;
+    ctx->tp = TrialPlanes();
+}
+
+int ensure_chunk(jbp_ctx *ctx, int count, char *err, size_t errlen) {
+    if (count <= ctx->chunk_cap) return JBP_OK;
+    free_chunk(ctx);
+    const size_t c = static_cast<size_t>(count);
+    auto alloc = [&](size_t bytes, void **p) -> int {
+        HIPCHK(hipMalloc(p, std::max<size_t>(bytes, 4)));
+        ctx->chunk_allocs.push_back(*p);
+        return JBP_OK;
+    };
+    const DevGraph &g = ctx->dg;
+    int rc = JBP_OK;
+    void *p = nullptr;
+    if ((rc = alloc(c * g.wn * 4, &p))) return rc; ctx->tp.err_x = static_cast<uint32_t *>(p);
+    if ((rc = alloc(c * g.wn * 4, &p))) return rc; ctx->tp.err_z = static_cast<uint32_t *>(p);
+    if ((rc = alloc(c * g.wn * 4, &p))) return rc; ctx->tp.est_x = static_cast<uint32_t *>(p);
+    if ((rc = alloc(c * g.wn * 4, &p))) return rc; ctx->tp.est_z = static_cast<uint32_t *>(p);
+    if ((rc = alloc(c * g.wm[0] * 4, &p))) return rc; ctx->tp.syn_x = static_cast<uint32_t *>(p);
+    if ((rc = alloc(c * g.wm[1] * 4, &p))) return rc; ctx->tp.syn_z = static_cast<uint32_t *>(p);
+    if ((rc = alloc(c * 4, &p))) return rc; ctx->tp.iters = static_cast<int *>(p);
+    if ((rc = alloc(c, &p))) return rc; ctx->tp.syn_ok = static_cast<uint8_t *>(p);
+    if ((rc = alloc(c, &p))) return rc; ctx->tp.success = static_cast<uint8_t *>(p);
+    if ((rc = alloc(c, &p))) return rc; ctx->tp.exact = static_cast<uint8_t *>(p);
+    ctx->chunk_cap = count;
+    return JBP_OK;
+}
+
+int ensure_states(jbp_ctx *ctx, size_t elems, char *err, size_t errlen) {
+    if (elems <= ctx->d_states_cap) return JBP_OK;
+    if (ctx->d_states) (void)hipFree(ctx->d_states);
+    ctx->d_states = nullptr;
+    ctx->d_states_cap = 0;
+    HIPCHK(hipMalloc(&ctx->d_states, elems * sizeof(int)));
+    ctx->d_states_cap = elems;
+    return JBP_OK;
+}
+
+int ensure_ints(jbp_ctx *ctx, size_t elems, char *err, size_t errlen) {
+    if (elems <= ctx->d_ints_cap) return JBP_OK;
+    if (ctx->d_ints) (void)hipFree(ctx->d_ints);
+    ctx->d_ints = nullptr;
+    ctx->d_ints_cap = 0;
+    HIPCHK(hipMalloc(&ctx->d_ints, elems * sizeof(int)));
+    ctx->d_ints_cap = elems;
+    return JBP_OK;
+}
+
+int ensure_f64(jbp_ctx *ctx, char *err, size_t errlen) {
+    if (ctx->f64_scratch) return JBP_OK;
+    int slots = ctx->f64_slots > 0 ? ctx->f64_slots : 2 * std::max(1, ctx->cus);
+    size_t bytes = decode_f64_scratch_bytes(ctx->dg, slots);
+    HIPCHK(hipMalloc(&ctx->f64_scratch, bytes));
+    ctx->f64_slots = slots;
+    return JBP_OK;
+}
+
+Prior make_prior(const double p4[4]) {
+    Prior pr;
+    for (int i = 0; i < 4; ++i) pr.p[i] = p4[i];
+    return pr;
+}
+
+// normalize_msg of {1-p, p/3, p/3, p/3} (jointbp_ets.cpp:5484-5485, 650-657)
+Prior prior_from_p(double p) {
+    double m[4] = {1.0 - p, p / 3.0, p / 3.0, p / 3.0};
+    double s = m[0] + m[1] + m[2] + m[3];
+    Prior pr;
+    for (int i = 0; i < 4; ++i) pr.p[i] = s <= 0.0 ? 0.25 : m[i] / s;
+    return pr;
+}
+
+// Decode `count` trials whose syndrome planes are already in ctx->tp.
+int run_decode(jbp_ctx *ctx, int count, const Prior &prior, int max_iter, int warmup, int interval,
+               double damping, int precision, char *err, size_t errlen) {
+    if (precision != JBP_FP32 && precision != JBP_FP64) {
+        set_err(err, errlen, "precision must be 32 or 64");
+        return JBP_ERR_ARG;
+    }
+    launch_fill_i32(ctx->d_counter, 0, 1, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    if (precision == JBP_FP32) {
+        if (!ctx->qc) {
+            set_err(err, errlen, "FP32 decoding needs a permutation-block code in this build");
+            return JBP_ERR_UNSUPPORTED;
+        }
+        if (damping > 0.0) {
+            set_err(err, errlen, "FP32 decoder runs damping 0 only; use precision 64");
+            return JBP_ERR_UNSUPPORTED;
+        }
+        launch_decode_qc_f32(ctx->dg, ctx->qcp, ctx->d_qc_tables, ctx->tp, count, prior, max_iter,
+                             warmup, interval, std::max(1, ctx->cus), ctx->d_counter, ctx->stream);
+    } else {
+        int rc = ensure_f64(ctx, err, errlen);
+        if (rc) return rc;
+        launch_decode_f64(ctx->dg, ctx->tp, count, prior, max_iter, damping, ctx->f64_slots,
+                          ctx->f64_block, ctx->f64_scratch, ctx->d_counter, ctx->stream);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_ms[1] = ms;
+    ctx->launches[1]++;
+    return JBP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jbp_create(const jbp_graph_desc *g, int device_id, jbp_ctx **out, char *err, size_t errlen) {
+    if (!out) {
+        set_err(err, errlen, "out is NULL");
+        return JBP_ERR_ARG;
+    }
+    *out = nullptr;
+    if (fault("create")) {
+        set_err(err, errlen, "fault injected (JBP_FAULT_INJECT=create)");
+        return JBP_ERR_DEVICE;
+    }
+    if (!g || g->nvars <= 0) {
+        set_err(err, errlen, "graph has no variables");
+        return JBP_ERR_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        set_err(err, errlen, "no HIP device available");
+        return JBP_ERR_DEVICE;
+    }
+    if (device_id < 0 || device_id >= ndev) {
+        set_err(err, errlen, "invalid device id %d (have %d)", device_id, ndev);
+        return JBP_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(device_id));
+    auto *ctx = new jbp_ctx();
+    ctx->device = device_id;
+    ctx->n = g->nvars;
+    int rc = validate_side("X", g->nvars, g->mX, g->x_edges, g->x_check_offsets, g->x_edge_var,
+                           g->x_var_offsets, g->x_var_edges, ctx->hs[0], err, errlen);
+    if (!rc)
+        rc = validate_side("Z", g->nvars, g->mZ, g->z_edges, g->z_check_offsets, g->z_edge_var,
+                           g->z_var_offsets, g->z_var_edges, ctx->hs[1], err, errlen);
+    if (rc) {
+        delete ctx;
+        return rc;
+    }
+    auto fail = [&](int code) {
+        jbp_destroy(ctx);
+        return code;
+    };
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device_id) != hipSuccess) {
+        set_err(err, errlen, "hipGetDeviceProperties failed");
+        return fail(JBP_ERR_DEVICE);
+    }
+    ctx->cus = prop.multiProcessorCount;
+    if (hipStreamCreate(&ctx->stream) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
+        hipEventCreate(&ctx->ev1) != hipSuccess) {
+        set_err(err, errlen, "stream/event creation failed");
+        return fail(JBP_ERR_DEVICE);
+    }
+    DevGraph &dg = ctx->dg;
+    dg.n = ctx->n;
+    dg.wn = (ctx->n + 31) / 32;
+    for (int s = 0; s < 2; ++s) {
+        HostSide &h = ctx->hs[s];
+        DevSide &d = dg.side[s];
+        d.m = h.m;
+        d.E = h.E;
+        dg.wm[s] = std::max(1, (h.m + 31) / 32);
+        if ((rc = dev_upload(ctx, h.chk_off, &d.chk_off, err, errlen))) return fail(rc);
+        if ((rc = dev_upload(ctx, h.edge_var, &d.edge_var, err, errlen))) return fail(rc);
+        if ((rc = dev_upload(ctx, h.var_off, &d.var_off, err, errlen))) return fail(rc);
+        if ((rc = dev_upload(ctx, h.var_edge, &d.var_edge, err, errlen))) return fail(rc);
+        for (int c = 0; c < h.m; ++c)
+            dg.max_cdeg = std::max(dg.max_cdeg, h.chk_off[c + 1] - h.chk_off[c]);
+        for (int v = 0; v < ctx->n; ++v)
+            dg.max_vdeg = std::max(dg.max_vdeg, h.var_off[v + 1] - h.var_off[v]);
+    }
+    if (dg.max_cdeg > 32 || dg.max_vdeg > 16) {
+        set_err(err, errlen, "degrees above 32 (check) / 16 (variable) are not supported");
+        return fail(JBP_ERR_UNSUPPORTED);
+    }
+    ctx->qc = detect_qc(ctx->hs, ctx->n, ctx->qcp) &&
+              qc_f32_supported(ctx->qcp, device_id, nullptr, 0);
+    if (ctx->qc) {
+        const int JL = ctx->qcp.J * ctx->qcp.L;
+        std::vector<int> tab(4 * JL);
+        for (int s = 0; s < 2; ++s)
+            for (int i = 0; i < JL; ++i) {
+                tab[s * JL + i] = ctx->qcp.ainv[s][i];
+                tab[2 * JL + s * JL + i] = ctx->qcp.b[s][i];
+            }
+        const int *dt = nullptr;
+        if ((rc = dev_upload(ctx, tab, &dt, err, errlen))) return fail(rc);
+        ctx->d_qc_tables = const_cast<int *>(dt);
+    }
+    // Residual-test bases: row spaces of Hx (side 0) and Hz (side 1).
+    size_t basis_bytes = static_cast<size_t>(std::max(ctx->hs[0].m, ctx->hs[1].m)) * dg.wn * 4;
+    if (basis_bytes <= kMaxBasisBytes) {
+        for (int s = 0; s < 2; ++s) {
+            std::vector<uint32_t> rows;
+            std::vector<int> piv;
+            int rank = 0;
+            build_rref(ctx->hs[s], ctx->n, rows, piv, rank);
+            DevBasis &b = ctx->basis[s];
+            b.rank = rank;
+            b.words = dg.wn;
+            if ((rc = dev_upload(ctx, rows, &b.rows, err, errlen))) return fail(rc);
+            if ((rc = dev_upload(ctx, piv, &b.pivot, err, errlen))) return fail(rc);
+        }
+        ctx->have_basis = true;
+    }
+    if (hipMalloc(&ctx->d_counter, sizeof(int)) != hipSuccess) {
+        set_err(err, errlen, "hipMalloc counter failed");
+        return fail(JBP_ERR_NOMEM);
+    }
+    *out = ctx;
+    return JBP_OK;
+}
+
+void jbp_destroy(jbp_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_chunk(ctx);
+    for (void *p : ctx->allocs) (void)hipFree(p);
+    if (ctx->f64_scratch) (void)hipFree(ctx->f64_scratch);
+    if (ctx->d_counter) (void)hipFree(ctx->d_counter);
+    if (ctx->d_states) (void)hipFree(ctx->d_states);
+    if (ctx->d_ints) (void)hipFree(ctx->d_ints);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int jbp_info(jbp_ctx *ctx, int64_t out[8]) {
+    if (!ctx || !out) return JBP_ERR_ARG;
+    out[0] = ctx->qc ? 1 : 0;
+    out[1] = ctx->qcp.P;
+    out[2] = ctx->qcp.J;
+    out[3] = ctx->qcp.L;
+    out[4] = ctx->device;
+    out[5] = ctx->cus;
+    out[6] = ctx->f64_block;
+    out[7] = ctx->have_basis ? 1 : 0;
+    return JBP_OK;
+}
+
+double jbp_last_kernel_ms(jbp_ctx *ctx, int which, int64_t *launches) {
+    if (!ctx || which < 0 || which > 2) return -1.0;
+    if (launches) *launches = ctx->launches[which];
+    return ctx->last_ms[which];
+}
+
+int jbp_sample(jbp_ctx *ctx, uint64_t base_seed, int64_t first_trial, int count, double p,
+               int *err_out, int *sx_out, int *sz_out, char *err, size_t errlen) {
+    if (!ctx || count < 0) {
+        set_err(err, errlen, "bad arguments");
+        return JBP_ERR_ARG;
+    }
+    if (count == 0) return JBP_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = ensure_chunk(ctx, count, err, errlen);
+    if (rc) return rc;
+    const DevGraph &g = ctx->dg;
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    launch_sample(g, ctx->tp, base_seed, first_trial, count, p, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    if (err_out) {
+        if ((rc = ensure_states(ctx, static_cast<size_t>(count) * g.n, err, errlen))) return rc;
+        launch_unpack_states(ctx->tp.err_x, ctx->tp.err_z, g.wn, g.n, count, ctx->d_states,
+                             ctx->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(err_out, ctx->d_states, static_cast<size_t>(count) * g.n * 4,
+                              hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_ms[0] = ms;
+    ctx->launches[0]++;
+    for (int s = 0; s < 2; ++s) {
+        int *dst = s == 0 ? sx_out : sz_out;
+        if (!dst) continue;
+        const int m = ctx->hs[s].m, wm = g.wm[s];
+        std::vector<uint32_t> bits(static_cast<size_t>(count) * wm);
+        HIPCHK(hipMemcpy(bits.data(), s == 0 ? ctx->tp.syn_x : ctx->tp.syn_z, bits.size() * 4,
+                         hipMemcpyDeviceToHost));
+        for (int t = 0; t < count; ++t)
+            for (int c = 0; c < m; ++c)
+                dst[static_cast<size_t>(t) * m + c] =
+                    (bits[static_cast<size_t>(t) * wm + (c >> 5)] >> (c & 31)) & 1;
+    }
+    return JBP_OK;
+}
+
+int jbp_decode_batch(jbp_ctx *ctx, int count, const uint32_t *sx_bits, const uint32_t *sz_bits,
+                     const double prior[4], int max_iter, double damping, int precision,
+                     int *iterations_out, uint8_t *syn_match_out, int *est_out, char *err,
+                     size_t errlen) {
+    if (!ctx || count < 0 || !sx_bits || !sz_bits || !prior || max_iter < 0) {
+        set_err(err, errlen, "bad arguments");
+        return JBP_ERR_ARG;
+    }
+    if (fault("decode")) {
+        set_err(err, errlen, "fault injected (JBP_FAULT_INJECT=decode)");
+        return JBP_ERR_DEVICE;
+    }
+    if (count == 0) return JBP_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = ensure_chunk(ctx, count, err, errlen);
+    if (rc) return rc;
+    const DevGraph &g = ctx->dg;
+    HIPCHK(hipMemcpyAsync(ctx->tp.syn_x, sx_bits, static_cast<size_t>(count) * g.wm[0] * 4,
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->tp.syn_z, sz_bits, static_cast<size_t>(count) * g.wm[1] * 4,
+                          hipMemcpyHostToDevice, ctx->stream));
+    rc = run_decode(ctx, count, make_prior(prior), max_iter, 0, 1, damping, precision, err, errlen);
+    if (rc) return rc;
+    if (iterations_out)
+        HIPCHK(hipMemcpy(iterations_out, ctx->tp.iters, static_cast<size_t>(count) * 4,
+                         hipMemcpyDeviceToHost));
+    if (syn_match_out)
+        HIPCHK(hipMemcpy(syn_match_out, ctx->tp.syn_ok, static_cast<size_t>(count),
+                         hipMemcpyDeviceToHost));
+    if (est_out) {
+        if ((rc = ensure_states(ctx, static_cast<size_t>(count) * g.n, err, errlen))) return rc;
+        launch_unpack_states(ctx->tp.est_x, ctx->tp.est_z, g.wn, g.n, count, ctx->d_states,
+                             ctx->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(est_out, ctx->d_states, static_cast<size_t>(count) * g.n * 4,
+                              hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    return JBP_OK;
+}
+
+int jbp_decode(jbp_ctx *ctx, const int *sx, const int *sz, const double prior[4], int max_iter,
+               int check_warmup, int check_interval, int measure_costs, int use_graph,
+               int freeze_syn, double damping, int precision, jbp_decode_result *out,
+               int *est_out, char *err, size_t errlen) {
+    (void)measure_costs;
+    (void)use_graph;
+    auto t0 = std::chrono::steady_clock::now();
+    if (!ctx || !sx || !sz || !prior || !out || max_iter < 0) {
+        set_err(err, errlen, "bad arguments");
+        return JBP_ERR_ARG;
+    }
+    if (freeze_syn) {
+        set_err(err, errlen, "freeze-syn is not supported on the GPU");
+        return JBP_ERR_UNSUPPORTED;
+    }
+    if (fault("decode")) {
+        set_err(err, errlen, "fault injected (JBP_FAULT_INJECT=decode)");
+        return JBP_ERR_DEVICE;
+    }
+    std::memset(out, 0, sizeof(*out));
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = ensure_chunk(ctx, 1, err, errlen);
+    if (rc) return rc;
+    const DevGraph &g = ctx->dg;
+    std::vector<uint32_t> bx(g.wm[0], 0u), bz(g.wm[1], 0u);
+    for (int c = 0; c < ctx->hs[0].m; ++c)
+        if (sx[c]) bx[c >> 5] |= 1u << (c & 31);
+    for (int c = 0; c < ctx->hs[1].m; ++c)
+        if (sz[c]) bz[c >> 5] |= 1u << (c & 31);
+    auto m0 = std::chrono::steady_clock::now();
+    HIPCHK(hipMemcpy(ctx->tp.syn_x, bx.data(), bx.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->tp.syn_z, bz.data(), bz.size() * 4, hipMemcpyHostToDevice));
+    auto m1 = std::chrono::steady_clock::now();
+    int warmup = precision == JBP_FP32 ? std::max(0, check_warmup) : 0;
+    int interval = precision == JBP_FP32 ? std::max(1, check_interval) : 1;
+    rc = run_decode(ctx, 1, make_prior(prior), max_iter, warmup, interval, damping, precision, err,
+                    errlen);
+    if (rc) return rc;
+    out->kernel_ms = ctx->last_ms[1];
+    auto m2 = std::chrono::steady_clock::now();
+    int iters = 0;
+    uint8_t ok = 0;
+    HIPCHK(hipMemcpy(&iters, ctx->tp.iters, 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&ok, ctx->tp.syn_ok, 1, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> ex(g.wn), ez(g.wn);
+    HIPCHK(hipMemcpy(ex.data(), ctx->tp.est_x, g.wn * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ez.data(), ctx->tp.est_z, g.wn * 4, hipMemcpyDeviceToHost));
+    auto m3 = std::chrono::steady_clock::now();
+    if (est_out)
+        for (int v = 0; v < g.n; ++v)
+            est_out[v] = static_cast<int>(((ex[v >> 5] >> (v & 31)) & 1u) |
+                                          (((ez[v >> 5] >> (v & 31)) & 1u) << 1));
+    out->iterations = iters;
+    out->syndrome_match = ok;
+    out->check_count = iters;
+    out->memcpy_ms = std::chrono::duration<double, std::milli>((m1 - m0) + (m3 - m2)).count();
+    out->host_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return JBP_OK;
+}
+
+int jbp_in_row_space(jbp_ctx *ctx, int side, int count, const int *bits, uint8_t *out, char *err,
+                     size_t errlen) {
+    if (!ctx || (side != 0 && side != 1) || count < 0 || (count && (!bits || !out))) {
+        set_err(err, errlen, "bad arguments");
+        return JBP_ERR_ARG;
+    }
+    if (!ctx->have_basis) {
+        set_err(err, errlen, "no dense row basis for this code size");
+        return JBP_ERR_UNSUPPORTED;
+    }
+    if (count == 0) return JBP_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = ensure_chunk(ctx, count, err, errlen);
+    if (rc) return rc;
+    const DevGraph &g = ctx->dg;
+    const size_t words = static_cast<size_t>(count) * g.wn;
+    if ((rc = ensure_ints(ctx, static_cast<size_t>(count) * g.n, err, errlen))) return rc;
+    HIPCHK(hipMemcpy(ctx->d_ints, bits, static_cast<size_t>(count) * g.n * 4,
+                     hipMemcpyHostToDevice));
+    // The residual kernel tests err_x ^ est_x against rowspace(Hz) and
+    // err_z ^ est_z against rowspace(Hx); put the vector in the plane that
+    // meets the requested basis and zeros elsewhere.
+    uint32_t *target = side == 1 ? ctx->tp.err_x : ctx->tp.err_z;
+    uint32_t *other = side == 1 ? ctx->tp.err_z : ctx->tp.err_x;
+    launch_pack_bits(ctx->d_ints, g.n, g.wn, count, target, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemsetAsync(other, 0, words * 4, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->tp.est_x, 0, words * 4, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->tp.est_z, 0, words * 4, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->tp.syn_ok, 1, static_cast<size_t>(count), ctx->stream));
+    launch_residual(g, ctx->basis[0], ctx->basis[1], ctx->tp, count, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, ctx->tp.success, static_cast<size_t>(count), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return JBP_OK;
+}
+
+int jbp_simulate(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_run_stats *stats,
+                 char *err, size_t errlen) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (!ctx || !cfg || cfg->n_trials < 0 || cfg->max_iter < 0) {
+        set_err(err, errlen, "bad arguments");
+        return JBP_ERR_ARG;
+    }
+    if (fault("simulate")) {
+        set_err(err, errlen, "fault injected (JBP_FAULT_INJECT=simulate)");
+        return JBP_ERR_DEVICE;
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    jbp_run_stats st;
+    std::memset(&st, 0, sizeof(st));
+    const DevGraph &g = ctx->dg;
+    const Prior prior = prior_from_p(cfg->p);
+    if (cfg->block > 0) ctx->f64_block = cfg->block;
+    if (cfg->slots > 0 && cfg->precision == JBP_FP64 && cfg->slots != ctx->f64_slots) {
+        if (ctx->f64_scratch) (void)hipFree(ctx->f64_scratch);
+        ctx->f64_scratch = nullptr;
+        ctx->f64_slots = cfg->slots;
+    }
+    const int64_t total = cfg->n_trials;
+    const int chunk = static_cast<int>(std::min<int64_t>(std::max<int64_t>(total, 1), kDefaultChunk));
+    int rc = ensure_chunk(ctx, chunk, err, errlen);
+    if (rc) return rc;
+    std::vector<int> iters(chunk);
+    std::vector<uint8_t> ok(chunk), succ(chunk), exact(chunk);
+    double dev_ms = 0.0;
+    bool stop = false;
+    for (int64_t done = 0; done < total && !stop;) {
+        const int cnt = static_cast<int>(std::min<int64_t>(chunk, total - done));
+        const int64_t first = cfg->first_trial + done;
+        HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+        launch_sample(g, ctx->tp, cfg->base_seed, first, cnt, cfg->p, ctx->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        HIPCHK(hipEventSynchronize(ctx->ev1));
+        float sms = 0.f;
+        HIPCHK(hipEventElapsedTime(&sms, ctx->ev0, ctx->ev1));
+        ctx->last_ms[0] = sms;
+        ctx->launches[0]++;
+        rc = run_decode(ctx, cnt, prior, cfg->max_iter, cfg->check_warmup, cfg->check_interval,
+                        cfg->damping, cfg->precision, err, errlen);
+        if (rc) return rc;
+        dev_ms += ctx->last_ms[1];
+        if (ctx->have_basis) {
+            HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+            launch_residual(g, ctx->basis[0], ctx->basis[1], ctx->tp, cnt, ctx->stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        }
+        HIPCHK(hipMemcpyAsync(iters.data(), ctx->tp.iters, cnt * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ok.data(), ctx->tp.syn_ok, cnt, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(succ.data(), ctx->tp.success, cnt, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(exact.data(), ctx->tp.exact, cnt, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (ctx->have_basis) {
+            float rms = 0.f;
+            HIPCHK(hipEventElapsedTime(&rms, ctx->ev0, ctx->ev1));
+            ctx->last_ms[2] = rms;
+            ctx->launches[2]++;
+        } else {
+            // Without a dense basis only exact matches are counted as successes.
+            for (int i = 0; i < cnt; ++i) succ[i] = 0;
+        }
+        if (pt && (pt->err || pt->est)) {
+            if ((rc = ensure_states(ctx, static_cast<size_t>(cnt) * g.n, err, errlen))) return rc;
+            for (int which = 0; which < 2; ++which) {
+                int *dst = which == 0 ? pt->err : pt->est;
+                if (!dst) continue;
+                const uint32_t *xb = which == 0 ? ctx->tp.err_x : ctx->tp.est_x;
+                const uint32_t *zb = which == 0 ? ctx->tp.err_z : ctx->tp.est_z;
+                launch_unpack_states(xb, zb, g.wn, g.n, cnt, ctx->d_states, ctx->stream);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpy(dst + static_cast<size_t>(done) * g.n, ctx->d_states,
+                                 static_cast<size_t>(cnt) * g.n * 4, hipMemcpyDeviceToHost));
+            }
+        }
+        int take = cnt;
+        for (int i = 0; i < cnt; ++i) {
+            if (!ctx->have_basis) succ[i] = (ok[i] && exact[i]) ? 1 : 0;
+            st.total_iters += iters[i];
+            st.failures += succ[i] ? 0 : 1;
+            st.bp_failures += ok[i] ? 0 : 1;
+            st.exact += exact[i] ? 1 : 0;
+            if (cfg->min_fails > 0 && st.failures >= cfg->min_fails) {
+                take = i + 1;
+                stop = true;
+                break;
+            }
+        }
+        if (pt) {
+            for (int i = 0; i < take; ++i) {
+                if (pt->iterations) pt->iterations[done + i] = iters[i];
+                if (pt->syn_match) pt->syn_match[done + i] = ok[i];
+                if (pt->success) pt->success[done + i] = succ[i];
+                if (pt->exact) pt->exact[done + i] = exact[i];
+            }
+        }
+        done += take;
+        st.done = done;
+    }
+    st.device_ms = dev_ms;
+    st.elapsed_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = st;
+    return JBP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// Definitions shared by the host orchestration (jbp_api.cpp) and the HIP
+// kernels of libjbp.  Plain structs of device pointers and sizes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace jbp {
+
+// Device copy of one Tanner graph side (check-major CSR, the edge order of
+// the reference's build_cuda_graph, jointbp_ets.cpp:937-1016).
+struct DevSide {
+    int m = 0;                 // checks
+    int E = 0;                 // edges
+    const int *chk_off = nullptr;   // m+1
+    const int *edge_var = nullptr;  // E
+    const int *var_off = nullptr;   // n+1
+    const int *var_edge = nullptr;  // E: edge ids per variable, ascending check
+};
+
+struct DevGraph {
+    int n = 0;
+    int wn = 0;                // 32-bit words of an n-bit vector
+    int wm[2] = {0, 0};        // 32-bit words of an m-bit vector per side
+    int max_cdeg = 0, max_vdeg = 0;
+    DevSide side[2];
+};
+
+// Quasi-cyclic structure: check (r*P + x) of side g meets variable
+// cc*P + (a*x + b mod P) for every column block cc (jointbp_ets.cpp:636-645).
+// ainv/b are kept per (g, r, cc) for the variable-side inverse map
+// x = ainv*(y - b) mod P.
+constexpr int kQcMaxBlocks = 64;
+struct QcParams {
+    int P = 0, J = 0, L = 0;
+    int a[2][kQcMaxBlocks];     // [side][r*L + cc]
+    int b[2][kQcMaxBlocks];
+    int ainv[2][kQcMaxBlocks];
+};
+
+// Dense reduced-row-echelon basis of one side's row space, for the
+// stabilizer-residual test (membership is basis independent).
+struct DevBasis {
+    int rank = 0;
+    int words = 0;                     // 32-bit words per row (= wn)
+    const uint32_t *rows = nullptr;    // rank x words
+    const int *pivot = nullptr;        // rank: pivot column of each row
+};
+
+// Per-trial bit planes resident in HBM (one row per trial in a chunk).
+struct TrialPlanes {
+    uint32_t *err_x = nullptr;   // chunk x wn
+    uint32_t *err_z = nullptr;
+    uint32_t *syn_x = nullptr;   // chunk x wm[0]
+    uint32_t *syn_z = nullptr;   // chunk x wm[1]
+    uint32_t *est_x = nullptr;   // chunk x wn
+    uint32_t *est_z = nullptr;
+    int *iters = nullptr;        // chunk
+    uint8_t *syn_ok = nullptr;   // chunk
+    uint8_t *success = nullptr;  // chunk
+    uint8_t *exact = nullptr;    // chunk
+};
+
+// Channel constants derived from the prior over (I, X, Z, Y).
+struct Prior {
+    double p[4];
+};
+
+// Launchers (defined in the .hip files).
+void launch_sample(const DevGraph &g, TrialPlanes tp, uint64_t base_seed, int64_t first_trial,
+                   int count, double p, hipStream_t st);
+void launch_residual(const DevGraph &g, DevBasis bx, DevBasis bz, TrialPlanes tp, int count,
+                     hipStream_t st);
+void launch_decode_f64(const DevGraph &g, TrialPlanes tp, int count, Prior prior, int max_iter,
+                       double damping, int slots, int block, double *scratch, int *counter,
+                       hipStream_t st);
+size_t decode_f64_scratch_bytes(const DevGraph &g, int slots);
+// d_tables: device ints [2][J][L] ainv followed by [2][J][L] b.
+void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_tables,
+                          TrialPlanes tp, int count, Prior prior, int max_iter, int check_warmup,
+                          int check_interval, int slots, int *counter, hipStream_t st);
+bool qc_f32_supported(const QcParams &qc, int device, char *why, size_t len);
+void launch_fill_i32(int *ptr, int value, int count, hipStream_t st);
+void launch_unpack_states(const uint32_t *xbits, const uint32_t *zbits, int words, int n,
+                          int count, int *out, hipStream_t st);
+void launch_pack_bits(const int *vals, int len, int words, int count, uint32_t *out,
+                      hipStream_t st);
+
+}  // namespace jbp

@@ -1,0 +1,341 @@
+// FP32 production decoder for codes built from affine permutation blocks
+// (the params-file codes of jointbp_ets.cpp:98-160, 628-648): check r*P + x
+// meets variable cc*P + (a[r][cc]*x + b[r][cc] mod P) for every column block.
+//
+// One workgroup of P threads owns one trial at a time and keeps its whole
+// message state on chip; it pulls the next trial from a device counter when
+// the current one stops (persistent kernel, refill on completion, so trials
+// of uneven length never idle a compute unit).
+//
+// Message representation.  The reference's X-side check message is
+// {a, b, a, b} and an X check reads only q0 = m0+m2, q1 = m1+m3 of a
+// variable message (jointbp_ets.cpp:2378-2408); likewise {a, a, b, b} and
+// m0+m1 / m2+m3 on the Z side.  Each directed edge message is therefore one
+// binary log-likelihood ratio, kept here in base 2 (bits):
+//   c2v   lambda = log2(P(bit=0) / P(bit=1)) from the check
+//   v2c   mu     = log2(q0 / q1)
+// The joint variable update (jointbp_ets.cpp:2451-2519) becomes, with
+// Lx = sum_r lambda_x[r], Lz = sum_r lambda_z[r] and prior pi over (I,X,Z,Y):
+//   est      = argmax(log2 pi_I + Lx + Lz, log2 pi_X + Lz, log2 pi_Z + Lx, log2 pi_Y)
+//   mu_x[r]  = Lx - lambda_x[r] + log2((pi_I 2^Lz + pi_Z) / (pi_X 2^Lz + pi_Y))
+//   mu_z[r]  = Lz - lambda_z[r] + log2((pi_I 2^Lx + pi_X) / (pi_Z 2^Lx + pi_Y))
+// and the check update (jointbp_ets.cpp:2373-2449) is the reference's
+// even/odd parity recursion run on pairs (1, 2^-|mu|), with the signs and
+// the syndrome bit applied to the result.
+//
+// On-chip layout.  Thread t is the owner of the variables (cc, y = t) for
+// all column blocks and keeps their incoming check messages lambda[side][r][cc]
+// in registers.  Thread t is also the owner of the checks (r, x = t).  The
+// two views meet in one LDS exchange buffer B[r][cc][x] (J*L*P floats): the
+// variable owner writes mu to B at x = ainv*(y - b) mod P, the check owner
+// reads its row B[r][*][x], writes lambda back in place, and the variable
+// owner reads it from the same slot.  Because P is a multiple of 64 and the
+// block multiplier is odd, a wavefront's 64 permuted addresses fall in 64
+// distinct LDS banks.
+//
+// Syndrome test for free: the variable owner stores the hard-decision bit of
+// the edge's variable in the least-significant mantissa bit of mu (a 2^-23
+// relative perturbation, below FP32 rounding of the update itself), so each
+// check owner recovers the parity of est over its row while reading its
+// inputs.  The test of iteration k's estimate therefore completes during the
+// check pass of iteration k+1; a trial that converged at k keeps est_k.
+#include "jbp_common.h"
+
+namespace jbp {
+namespace {
+
+constexpr float kLlrMax = 64.0f;  // clamp of |lambda| in bits
+
+struct QcDev {
+    const int *ainv;  // [2][J][L]
+    const int *b;     // [2][J][L]
+    int P;
+};
+
+struct PriorF {
+    float lp[4];            // log2 prior (I, X, Z, Y)
+    float fx[4];            // (pi_I, pi_Z, pi_X, pi_Y) for the X-side extrinsic
+    float fz[4];            // (pi_I, pi_X, pi_Z, pi_Y) for the Z-side extrinsic
+};
+
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
+
+// log2((a 2^L + b) / (c 2^L + d)) without overflow for any L.
+__device__ __forceinline__ float extrinsic_term(float L, const float *k) {
+    float t = fast_exp2(-fabsf(L));
+    float num, den;
+    if (L >= 0.0f) {
+        num = k[0] + k[1] * t;
+        den = k[2] + k[3] * t;
+    } else {
+        num = k[0] * t + k[1];
+        den = k[2] * t + k[3];
+    }
+    return fast_log2(num) - fast_log2(den);
+}
+
+__device__ __forceinline__ float with_lsb(float v, uint32_t bit) {
+    return __uint_as_float((__float_as_uint(v) & ~1u) | bit);
+}
+
+// Check update of one row of L inputs (stride P floats apart), in place.
+// The forward pass parks e_i = 2^-|mu_i| in the row slot it read, so the
+// backward pass needs no register copy of it.
+template <int L, int P>
+__device__ __forceinline__ uint32_t check_row(float *row, uint32_t syn) {
+    uint32_t neg = 0, par = 0;
+    float pe[L + 1], po[L + 1];
+    pe[0] = 1.0f;
+    po[0] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        float mu = row[i * P];
+        uint32_t u = __float_as_uint(mu);
+        par ^= u & 1u;
+        neg |= (u >> 31) << i;
+        float e = fast_exp2(-fabsf(mu));
+        row[i * P] = e;
+        pe[i + 1] = pe[i] + po[i] * e;
+        po[i + 1] = pe[i] * e + po[i];
+    }
+    const uint32_t negall = (__popc(neg) & 1u) ^ syn;
+    float se = 1.0f, so = 0.0f;
+#pragma unroll
+    for (int i = L - 1; i >= 0; --i) {
+        float e = row[i * P];
+        float PE = pe[i] * se + po[i] * so;
+        float PO = pe[i] * so + po[i] * se;
+        float mag = fminf(fast_log2(PE) - fast_log2(PO), kLlrMax);
+        uint32_t sg = negall ^ ((neg >> i) & 1u);
+        row[i * P] = sg ? -mag : mag;
+        float ne = se + so * e;
+        float no = se * e + so;
+        se = ne;
+        so = no;
+    }
+    return par;
+}
+
+template <int P, int J, int L>
+__global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, TrialPlanes tp,
+                                                         int count, PriorF pr, int max_iter,
+                                                         int warmup, int interval, int *counter) {
+    static_assert(P % 64 == 0 && P <= 1024, "one thread per block row, whole wavefronts");
+    static_assert(P <= 1024, "10-bit row indices");
+    extern __shared__ float B[];  // [J][L][P] exchange buffer, then GZ[L][P]
+    float *GZ = B + J * L * P;   // Z-side extrinsic base per variable, parked between passes
+    __shared__ int s_trial;
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    constexpr int W = (L + 2) / 3;  // three 10-bit row indices per register
+
+    // Row index x = ainv*(t - b) mod P of this thread's variable (cc, t) in
+    // block (side, r, cc); its exchange slot is B[(r*L + cc)*P + x].
+    uint32_t xw[2][J][W];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int r = 0; r < J; ++r)
+#pragma unroll
+            for (int w = 0; w < W; ++w) xw[s][r][w] = 0;
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int r = 0; r < J; ++r)
+#pragma unroll
+            for (int cc = 0; cc < L; ++cc) {
+                int idx = (s * J + r) * L + cc;
+                int d = (t - qc.b[idx]) % P;
+                if (d < 0) d += P;
+                uint32_t x = static_cast<uint32_t>((qc.ainv[idx] * d) % P);
+                xw[s][r][cc / 3] |= x << (10 * (cc % 3));
+            }
+    // Make the packed indices opaque to the optimiser at chosen points, so
+    // that unpacked addresses are recomputed instead of being kept live
+    // (and spilled) across the check passes or the iteration loop.
+    auto launder = [&]() {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int r = 0; r < J; ++r)
+#pragma unroll
+                for (int w = 0; w < W; ++w) asm volatile("" : "+v"(xw[s][r][w]));
+    };
+    auto slot = [&](int s, int r, int cc) -> int {
+        return (r * L + cc) * P + static_cast<int>((xw[s][r][cc / 3] >> (10 * (cc % 3))) & 1023u);
+    };
+
+    for (;;) {
+        if (t == 0) s_trial = atomicAdd(counter, 1);
+        __syncthreads();
+        const int trial = s_trial;
+        __syncthreads();
+        if (trial >= count) return;
+
+        uint32_t synx = 0, synz = 0;  // bit r: syndrome of check (r, t)
+#pragma unroll
+        for (int r = 0; r < J; ++r) {
+            int c = r * P + t;
+            synx |= ((tp.syn_x[static_cast<size_t>(trial) * g.wm[0] + (c >> 5)] >> (c & 31)) & 1u) << r;
+            synz |= ((tp.syn_z[static_cast<size_t>(trial) * g.wm[1] + (c >> 5)] >> (c & 31)) & 1u) << r;
+        }
+
+        float lam[2][J][L];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int r = 0; r < J; ++r)
+#pragma unroll
+                for (int cc = 0; cc < L; ++cc) lam[s][r][cc] = 0.0f;
+
+        int k = 0;
+        int iterations = max_iter;
+        int matched = 0;
+        uint32_t estx = 0, estz = 0;  // bit cc: hard decision of variable (cc, t)
+        for (;;) {
+            launder();
+            // Variable pass k: estimate, extrinsic X messages (+ est bit) to B.
+            estx = 0;
+            estz = 0;
+#pragma unroll
+            for (int cc = 0; cc < L; ++cc) {
+                float lx = 0.0f, lz = 0.0f;
+#pragma unroll
+                for (int r = 0; r < J; ++r) {
+                    lx += lam[0][r][cc];
+                    lz += lam[1][r][cc];
+                }
+                float l0 = pr.lp[0] + lx + lz, l1 = pr.lp[1] + lz, l2 = pr.lp[2] + lx, l3 = pr.lp[3];
+                int best = 0;
+                float bv = l0;
+                if (l1 > bv) { best = 1; bv = l1; }
+                if (l2 > bv) { best = 2; bv = l2; }
+                if (l3 > bv) { best = 3; bv = l3; }
+                estx |= static_cast<uint32_t>(best & 1) << cc;
+                estz |= static_cast<uint32_t>(best >> 1) << cc;
+                float gx = lx + extrinsic_term(lz, pr.fx);
+                GZ[cc * P + t] = lz + extrinsic_term(lx, pr.fz);
+                uint32_t bx = static_cast<uint32_t>(best & 1);
+#pragma unroll
+                for (int r = 0; r < J; ++r) B[slot(0, r, cc)] = with_lsb(gx - lam[0][r][cc], bx);
+            }
+            __syncthreads();
+            // X check pass: rows (r, t).
+            uint32_t bad = 0;
+#pragma unroll 1
+            for (int r = 0; r < J; ++r) {
+                uint32_t sb = (synx >> r) & 1u;
+                bad |= check_row<L, P>(B + r * L * P + t, sb) ^ sb;
+            }
+            const int badx = __syncthreads_or(static_cast<int>(bad));
+            launder();
+#pragma unroll
+            for (int r = 0; r < J; ++r)
+#pragma unroll
+                for (int cc = 0; cc < L; ++cc) lam[0][r][cc] = B[slot(0, r, cc)];
+            __syncthreads();
+#pragma unroll
+            for (int cc = 0; cc < L; ++cc) {
+                uint32_t bz = (estz >> cc) & 1u;
+                const float gz = GZ[cc * P + t];
+#pragma unroll
+                for (int r = 0; r < J; ++r) B[slot(1, r, cc)] = with_lsb(gz - lam[1][r][cc], bz);
+            }
+            __syncthreads();
+            bad = 0;
+#pragma unroll 1
+            for (int r = 0; r < J; ++r) {
+                uint32_t sb = (synz >> r) & 1u;
+                bad |= check_row<L, P>(B + r * L * P + t, sb) ^ sb;
+            }
+            const int badz = __syncthreads_or(static_cast<int>(bad));
+            if (k >= 1) {
+                bool test = (k == max_iter) || (k > warmup && ((k - warmup) % interval) == 0);
+                if (test && !badx && !badz) {
+                    iterations = k;
+                    matched = 1;
+                    break;
+                }
+            }
+            if (k >= max_iter) break;
+            launder();
+#pragma unroll
+            for (int r = 0; r < J; ++r)
+#pragma unroll
+                for (int cc = 0; cc < L; ++cc) lam[1][r][cc] = B[slot(1, r, cc)];
+            ++k;
+            __syncthreads();
+        }
+
+        // est_k bit planes: variable (cc, t) is bit cc*P + t; P % 64 == 0
+        // makes each wavefront's 64 variables one aligned 64-bit span.
+        uint32_t *ox = tp.est_x + static_cast<size_t>(trial) * g.wn;
+        uint32_t *oz = tp.est_z + static_cast<size_t>(trial) * g.wn;
+#pragma unroll
+        for (int cc = 0; cc < L; ++cc) {
+            uint64_t bx = __ballot((estx >> cc) & 1u);
+            uint64_t bz = __ballot((estz >> cc) & 1u);
+            if (lane == 0) {
+                int w = (cc * P + t) >> 5;
+                ox[w] = static_cast<uint32_t>(bx);
+                ox[w + 1] = static_cast<uint32_t>(bx >> 32);
+                oz[w] = static_cast<uint32_t>(bz);
+                oz[w + 1] = static_cast<uint32_t>(bz >> 32);
+            }
+        }
+        if (t == 0) {
+            tp.iters[trial] = iterations;
+            tp.syn_ok[trial] = static_cast<uint8_t>(matched);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+bool qc_f32_supported(const QcParams &qc, int device, char *why, size_t len) {
+    auto fail = [&](const char *msg) {
+        if (why && len) snprintf(why, len, "%s", msg);
+        return false;
+    };
+    if (qc.P != 768 || qc.J != 3 || qc.L != 12)
+        return fail("LDS decoder is instantiated for P=768, J=3, L=12");
+    size_t lds = static_cast<size_t>(qc.J + 1) * qc.L * qc.P * sizeof(float);
+    int max_lds = 0;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+        hipSuccess)
+        return fail("cannot query LDS size");
+    if (lds + 64 > static_cast<size_t>(max_lds)) return fail("exchange buffer exceeds LDS");
+    return true;
+}
+
+void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_tables,
+                          TrialPlanes tp, int count, Prior prior, int max_iter, int check_warmup,
+                          int check_interval, int slots, int *counter, hipStream_t st) {
+    if (count <= 0) return;
+    PriorF pf;
+    const double *p = prior.p;
+    for (int i = 0; i < 4; ++i) pf.lp[i] = static_cast<float>(std::log2(p[i]));
+    const double fx[4] = {p[0], p[2], p[1], p[3]};
+    const double fz[4] = {p[0], p[1], p[2], p[3]};
+    for (int i = 0; i < 4; ++i) {
+        pf.fx[i] = static_cast<float>(fx[i]);
+        pf.fz[i] = static_cast<float>(fz[i]);
+    }
+    QcDev dev;
+    dev.P = qc.P;
+    dev.ainv = d_tables;                  // [2][J][L]
+    dev.b = d_tables + 2 * qc.J * qc.L;   // [2][J][L]
+    size_t lds = static_cast<size_t>(qc.J + 1) * qc.L * qc.P * sizeof(float);
+    int grid = slots < count ? slots : count;
+    int interval = check_interval > 0 ? check_interval : 1;
+    int warmup = check_warmup > 0 ? check_warmup : 0;
+    auto kern = decode_qc_f32_kernel<768, 3, 12>;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(768), lds, st, g, dev, tp, count, pf, max_iter,
+                       warmup, interval, counter);
+}
+
+}  // namespace jbp

@@ -1,0 +1,309 @@
+// FP64 parity decoder: the reference's joint BP (jointbp_ets.cpp:1943-2535,
+// 2733-2735, 3078) in the reference's exact operation order, so that the
+// per-trial estimate and iteration count are identical to its CPU decoder.
+// This translation unit is compiled with -ffp-contract=off (no FMA), and
+// normalisation divides by the left-to-right sum as normalize_msg does
+// (jointbp_ets.cpp:650-657).
+//
+// Layout: one workgroup decodes one trial at a time and pulls the next trial
+// index from a device counter when it finishes (persistent, refill on
+// completion).  Its messages live in a private slice of a global scratch
+// buffer:
+//   v2c[side][E][4]  variable-to-check, four Pauli components
+//   c2v[side][E][2]  check-to-variable; the reference's X-side message is
+//                    {a, b, a, b} and Z-side {a, a, b, b}, so two values
+//                    carry it without loss
+//   est[n]           hard decision, one byte per qubit
+// Checks and variables are walked over the generic check-major CSR, so the
+// kernel serves any code, including ones without permutation structure.
+#include "jbp_common.h"
+
+namespace jbp {
+namespace {
+
+struct M4 {
+    double v[4];
+};
+
+__device__ __forceinline__ M4 mul4(const M4 &a, const M4 &b) {
+    return M4{{a.v[0] * b.v[0], a.v[1] * b.v[1], a.v[2] * b.v[2], a.v[3] * b.v[3]}};
+}
+
+__device__ __forceinline__ void normalize4(M4 &m) {
+    double sum = m.v[0] + m.v[1] + m.v[2] + m.v[3];
+    if (sum <= 0.0) {
+        m.v[0] = m.v[1] = m.v[2] = m.v[3] = 0.25;
+        return;
+    }
+    m.v[0] /= sum;
+    m.v[1] /= sum;
+    m.v[2] /= sum;
+    m.v[3] /= sum;
+}
+
+// scaled_mix (jointbp_ets.cpp:1903-1911)
+__device__ __forceinline__ M4 mix4(const M4 &nw, const M4 &old, double d) {
+    if (d <= 0.0) return nw;
+    M4 out;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out.v[i] = (1.0 - d) * nw.v[i] + d * old.v[i];
+    normalize4(out);
+    return out;
+}
+
+struct SlotPtrs {
+    double *v2c[2];
+    double *c2v[2];
+    uint8_t *est;
+};
+
+__device__ __forceinline__ SlotPtrs slot_ptrs(const DevGraph &g, double *scratch, size_t stride) {
+    double *base = scratch + static_cast<size_t>(blockIdx.x) * stride;
+    SlotPtrs s;
+    s.v2c[0] = base;
+    s.v2c[1] = s.v2c[0] + 4 * static_cast<size_t>(g.side[0].E);
+    s.c2v[0] = s.v2c[1] + 4 * static_cast<size_t>(g.side[1].E);
+    s.c2v[1] = s.c2v[0] + 2 * static_cast<size_t>(g.side[0].E);
+    s.est = reinterpret_cast<uint8_t *>(s.c2v[1] + 2 * static_cast<size_t>(g.side[1].E));
+    return s;
+}
+
+// Check update of one check (jointbp_ets.cpp:2373-2449).
+template <int MAXC>
+__device__ void check_update(const DevSide &sd, int side, int c, int syn, const double *v2c,
+                             double *c2v) {
+    const int off = sd.chk_off[c];
+    const int deg = sd.chk_off[c + 1] - off;
+    double q0[MAXC], q1[MAXC], pe[MAXC + 1], po[MAXC + 1];
+    pe[0] = 1.0;
+    po[0] = 0.0;
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+        if (i < deg) {
+            const double *m = v2c + 4 * static_cast<size_t>(off + i);
+            q0[i] = side == 0 ? m[0] + m[2] : m[0] + m[1];
+            q1[i] = side == 0 ? m[1] + m[3] : m[2] + m[3];
+            pe[i + 1] = pe[i] * q0[i] + po[i] * q1[i];
+            po[i + 1] = pe[i] * q1[i] + po[i] * q0[i];
+        }
+    }
+    double se = 1.0, so = 0.0;
+#pragma unroll
+    for (int i = MAXC - 1; i >= 0; --i) {
+        if (i < deg) {
+            double p_even = pe[i] * se + po[i] * so;
+            double p_odd = pe[i] * so + po[i] * se;
+            double v0 = syn == 0 ? p_even : p_odd;
+            double v1 = syn == 0 ? p_odd : p_even;
+            double sum = side == 0 ? ((v0 + v1) + v0) + v1 : ((v0 + v0) + v1) + v1;
+            double *o = c2v + 2 * static_cast<size_t>(off + i);
+            if (sum <= 0.0) {
+                o[0] = 0.25;
+                o[1] = 0.25;
+            } else {
+                o[0] = v0 / sum;
+                o[1] = v1 / sum;
+            }
+            double ne = se * q0[i] + so * q1[i];
+            double no = se * q1[i] + so * q0[i];
+            se = ne;
+            so = no;
+        }
+    }
+}
+
+__device__ __forceinline__ M4 c2v_msg(const double *c2v, int e, int side) {
+    double a = c2v[2 * static_cast<size_t>(e)];
+    double b = c2v[2 * static_cast<size_t>(e) + 1];
+    return side == 0 ? M4{{a, b, a, b}} : M4{{a, a, b, b}};
+}
+
+// Joint variable update + hard decision (jointbp_ets.cpp:2451-2519).  The
+// suffix products are formed on the fly while walking the edges backwards;
+// each outgoing message uses the same operands as the reference's
+// pref[i] (*) suf[i+1].
+template <int MAXV>
+__device__ void var_update(const DevGraph &g, int v, const M4 &prior, double damping,
+                           const SlotPtrs &sp) {
+    const M4 ident{{1.0, 1.0, 1.0, 1.0}};
+    int b[2], deg[2];
+    M4 pref[2][MAXV + 1];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        b[s] = g.side[s].var_off[v];
+        deg[s] = g.side[s].var_off[v + 1] - b[s];
+        pref[s][0] = ident;
+#pragma unroll
+        for (int i = 0; i < MAXV; ++i)
+            if (i < deg[s])
+                pref[s][i + 1] = mul4(pref[s][i], c2v_msg(sp.c2v[s], g.side[s].var_edge[b[s] + i], s));
+    }
+    M4 allx = ident, allz = ident;
+#pragma unroll
+    for (int i = 0; i <= MAXV; ++i) {
+        if (i == deg[0]) allx = pref[0][i];
+        if (i == deg[1]) allz = pref[1][i];
+    }
+    M4 total = mul4(prior, mul4(allx, allz));
+    normalize4(total);
+    int best = 0;
+#pragma unroll
+    for (int k = 1; k < 4; ++k)
+        if (total.v[k] > total.v[best]) best = k;
+    sp.est[v] = static_cast<uint8_t>(best);
+
+    // X edges: ((prior (*) x_excl) (*) z_all)
+    M4 suf = ident;
+#pragma unroll
+    for (int i = MAXV - 1; i >= 0; --i) {
+        if (i < deg[0]) {
+            int e = g.side[0].var_edge[b[0] + i];
+            M4 out = mul4(mul4(prior, mul4(pref[0][i], suf)), allz);
+            normalize4(out);
+            double *slot = sp.v2c[0] + 4 * static_cast<size_t>(e);
+            if (damping > 0.0) {
+                M4 old{{slot[0], slot[1], slot[2], slot[3]}};
+                out = mix4(out, old, damping);
+            }
+            slot[0] = out.v[0]; slot[1] = out.v[1]; slot[2] = out.v[2]; slot[3] = out.v[3];
+            suf = mul4(suf, c2v_msg(sp.c2v[0], e, 0));
+        }
+    }
+    // Z edges: ((prior (*) x_all) (*) z_excl)
+    suf = ident;
+    const M4 px = mul4(prior, allx);
+#pragma unroll
+    for (int i = MAXV - 1; i >= 0; --i) {
+        if (i < deg[1]) {
+            int e = g.side[1].var_edge[b[1] + i];
+            M4 out = mul4(px, mul4(pref[1][i], suf));
+            normalize4(out);
+            double *slot = sp.v2c[1] + 4 * static_cast<size_t>(e);
+            if (damping > 0.0) {
+                M4 old{{slot[0], slot[1], slot[2], slot[3]}};
+                out = mix4(out, old, damping);
+            }
+            slot[0] = out.v[0]; slot[1] = out.v[1]; slot[2] = out.v[2]; slot[3] = out.v[3];
+            suf = mul4(suf, c2v_msg(sp.c2v[1], e, 1));
+        }
+    }
+}
+
+template <int MAXC, int MAXV>
+__global__ void decode_f64_kernel(DevGraph g, TrialPlanes tp, int count, Prior pr, int max_iter,
+                                  double damping, double *scratch, size_t stride, int *counter) {
+    __shared__ int s_trial;
+    const int tid = threadIdx.x;
+    const int bs = blockDim.x;
+    const SlotPtrs sp = slot_ptrs(g, scratch, stride);
+    const M4 prior{{pr.p[0], pr.p[1], pr.p[2], pr.p[3]}};
+    const int mx = g.side[0].m, mz = g.side[1].m;
+    for (;;) {
+        if (tid == 0) s_trial = atomicAdd(counter, 1);
+        __syncthreads();
+        const int trial = s_trial;
+        __syncthreads();
+        if (trial >= count) return;
+        const uint32_t *tsx = tp.syn_x + static_cast<size_t>(trial) * g.wm[0];
+        const uint32_t *tsz = tp.syn_z + static_cast<size_t>(trial) * g.wm[1];
+
+        // init (jointbp_ets.cpp:1970-1982): v2c = prior, c2v = uniform
+        for (int s = 0; s < 2; ++s) {
+            const int E = g.side[s].E;
+            for (int e = tid; e < E; e += bs) {
+                double *m = sp.v2c[s] + 4 * static_cast<size_t>(e);
+                m[0] = prior.v[0]; m[1] = prior.v[1]; m[2] = prior.v[2]; m[3] = prior.v[3];
+                sp.c2v[s][2 * static_cast<size_t>(e)] = 0.25;
+                sp.c2v[s][2 * static_cast<size_t>(e) + 1] = 0.25;
+            }
+        }
+        __syncthreads();
+
+        int iterations = max_iter;
+        int matched = 0;
+        for (int iter = 0; iter < max_iter; ++iter) {
+            for (int c = tid; c < mx + mz; c += bs) {
+                if (c < mx) {
+                    int syn = (tsx[c >> 5] >> (c & 31)) & 1;
+                    check_update<MAXC>(g.side[0], 0, c, syn, sp.v2c[0], sp.c2v[0]);
+                } else {
+                    int cz = c - mx;
+                    int syn = (tsz[cz >> 5] >> (cz & 31)) & 1;
+                    check_update<MAXC>(g.side[1], 1, cz, syn, sp.v2c[1], sp.c2v[1]);
+                }
+            }
+            __syncthreads();
+            for (int v = tid; v < g.n; v += bs) var_update<MAXV>(g, v, prior, damping, sp);
+            __syncthreads();
+            // both syndromes of est (jointbp_ets.cpp:2521-2525)
+            int bad = 0;
+            for (int c = tid; c < mx + mz && !bad; c += bs) {
+                int s = c < mx ? 0 : 1;
+                int cc = c < mx ? c : c - mx;
+                const DevSide &sd = g.side[s];
+                int par = 0;
+                for (int e = sd.chk_off[cc]; e < sd.chk_off[cc + 1]; ++e) {
+                    int st = sp.est[sd.edge_var[e]];
+                    par ^= s == 0 ? (st & 1) : (st >> 1);
+                }
+                const uint32_t *t = s == 0 ? tsx : tsz;
+                bad |= par != static_cast<int>((t[cc >> 5] >> (cc & 31)) & 1u);
+            }
+            if (!__syncthreads_or(bad)) {
+                iterations = iter + 1;
+                matched = 1;
+                break;
+            }
+        }
+        // est bit planes, one ballot per 64 qubits
+        const int lane = tid & 63;
+        const int wave = tid >> 6;
+        const int nwaves = (bs + 63) >> 6;
+        uint32_t *ox = tp.est_x + static_cast<size_t>(trial) * g.wn;
+        uint32_t *oz = tp.est_z + static_cast<size_t>(trial) * g.wn;
+        for (int v0 = wave * 64; v0 < g.n; v0 += nwaves * 64) {
+            int v = v0 + lane;
+            int st = v < g.n ? sp.est[v] : 0;
+            uint64_t bx = __ballot(st & 1);
+            uint64_t bz = __ballot(st >> 1);
+            if (lane == 0) {
+                ox[v0 >> 5] = static_cast<uint32_t>(bx);
+                oz[v0 >> 5] = static_cast<uint32_t>(bz);
+                if ((v0 >> 5) + 1 < g.wn) {
+                    ox[(v0 >> 5) + 1] = static_cast<uint32_t>(bx >> 32);
+                    oz[(v0 >> 5) + 1] = static_cast<uint32_t>(bz >> 32);
+                }
+            }
+        }
+        if (tid == 0) {
+            tp.iters[trial] = iterations;
+            tp.syn_ok[trial] = static_cast<uint8_t>(matched);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+size_t decode_f64_scratch_bytes(const DevGraph &g, int slots) {
+    size_t per = (6 * static_cast<size_t>(g.side[0].E + g.side[1].E)) * sizeof(double) +
+                 ((static_cast<size_t>(g.n) + 7) / 8) * sizeof(double);
+    return per * static_cast<size_t>(slots);
+}
+
+void launch_decode_f64(const DevGraph &g, TrialPlanes tp, int count, Prior prior, int max_iter,
+                       double damping, int slots, int block, double *scratch, int *counter,
+                       hipStream_t st) {
+    if (count <= 0) return;
+    size_t stride = decode_f64_scratch_bytes(g, 1) / sizeof(double);
+    int grid = slots < count ? slots : count;
+    if (g.max_cdeg <= 16 && g.max_vdeg <= 4) {
+        hipLaunchKernelGGL((decode_f64_kernel<16, 4>), dim3(grid), dim3(block), 0, st, g, tp,
+                           count, prior, max_iter, damping, scratch, stride, counter);
+    } else {
+        hipLaunchKernelGGL((decode_f64_kernel<32, 16>), dim3(grid), dim3(block), 0, st, g, tp,
+                           count, prior, max_iter, damping, scratch, stride, counter);
+    }
+}
+
+}  // namespace jbp

@@ -1,0 +1,300 @@
+// Trial sampling, syndromes and the stabilizer-residual test on the device.
+//
+// Sampling is bit-exact with the reference's host loop
+// (jointbp_ets.cpp:5932-5946): per-trial seed splitmix64(base ^ idx*phi)
+// (jointbp_ets.cpp:4401-4411), one std::mt19937_64 stream per trial, one
+// uniform double per qubit in qubit order through libstdc++'s
+// generate_canonical<double, 53> (one 64-bit draw d -> double(d) * 2^-64,
+// values that round to 1.0 clamped to the largest double below 1), then the
+// depolarizing thresholds in double precision.
+//
+// One wavefront per trial: the 312-word Mersenne-Twister state lives in
+// LDS, the twist runs in three dependency phases across the 64 lanes, and
+// each lane tempers and classifies five of every 312 outputs.  Error and
+// syndrome bits are assembled with wave ballots into bit planes.
+#include "jbp_common.h"
+
+namespace jbp {
+namespace {
+
+constexpr int kMtN = 312;
+constexpr int kMtM = 156;
+constexpr uint64_t kMtA = 0xb5026f5aa96619e9ull;
+constexpr uint64_t kUpper = 0xffffffff80000000ull;  // upper 33 bits (r = 31)
+constexpr uint64_t kLower = 0x000000007fffffffull;
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ uint64_t twist_word(uint64_t cur, uint64_t next, uint64_t far) {
+    uint64_t y = (cur & kUpper) | (next & kLower);
+    return far ^ (y >> 1) ^ ((y & 1ull) ? kMtA : 0ull);
+}
+
+__device__ __forceinline__ uint64_t temper(uint64_t z) {
+    z ^= (z >> 29) & 0x5555555555555555ull;
+    z ^= (z << 17) & 0x71d67fffeda60000ull;
+    z ^= (z << 37) & 0xfff7eee000000000ull;
+    z ^= (z >> 43);
+    return z;
+}
+
+// OR a 64-bit ballot covering bits q0 .. q0+63 into an LDS bit plane.
+__device__ __forceinline__ void or_ballot(uint32_t *plane, int words, int q0, uint64_t m) {
+    int w = q0 >> 5;
+    uint32_t sh = static_cast<uint32_t>(q0 & 31);
+    plane[w] |= static_cast<uint32_t>(m << sh);
+    if (w + 1 < words) plane[w + 1] |= static_cast<uint32_t>(m >> (32 - sh));
+    if (sh && w + 2 < words) plane[w + 2] |= static_cast<uint32_t>(m >> (64 - sh));
+}
+
+// grid = count trials, block = 64 (one wavefront), dynamic LDS = 2*wn words.
+__global__ void __launch_bounds__(64) sample_kernel(DevGraph g, TrialPlanes tp, uint64_t base_seed,
+                                                    int64_t first_trial, double p) {
+    __shared__ uint64_t mt[kMtN];
+    extern __shared__ uint32_t planes[];  // err_x | err_z
+    const int lane = threadIdx.x;
+    const int trial = blockIdx.x;
+    const int n = g.n, wn = g.wn;
+    uint32_t *ex = planes;
+    uint32_t *ez = planes + wn;
+    for (int w = lane; w < 2 * wn; w += 64) planes[w] = 0u;
+
+    if (lane == 0) {
+        uint64_t idx = static_cast<uint64_t>(first_trial + trial);
+        uint64_t x = splitmix64(base_seed ^ (idx * 0x9e3779b97f4a7c15ull));
+        mt[0] = x;
+        for (int i = 1; i < kMtN; ++i) {
+            x = 6364136223846793005ull * (x ^ (x >> 62)) + static_cast<uint64_t>(i);
+            mt[i] = x;
+        }
+    }
+    __syncthreads();
+
+    const double omp = 1.0 - p;
+    for (int base = 0; base < n; base += kMtN) {
+        // Twist phase A: k in [0, 156) reads old k, k+1, k+156.
+        uint64_t nv[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            int k = lane + 64 * j;
+            if (k < kMtM) nv[j] = twist_word(mt[k], mt[k + 1], mt[k + kMtM]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            int k = lane + 64 * j;
+            if (k < kMtM) mt[k] = nv[j];
+        }
+        __syncthreads();
+        // Phase B: k in [156, 311) reads old k, k+1 and new k-156.
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            int k = kMtM + lane + 64 * j;
+            if (k < kMtN - 1) nv[j] = twist_word(mt[k], mt[k + 1], mt[k - kMtM]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            int k = kMtM + lane + 64 * j;
+            if (k < kMtN - 1) mt[k] = nv[j];
+        }
+        __syncthreads();
+        // Phase C: the last word reads new 0 and new 155.
+        if (lane == 0) mt[kMtN - 1] = twist_word(mt[kMtN - 1], mt[0], mt[kMtM - 1]);
+        __syncthreads();
+
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            int k = lane + 64 * j;
+            int q = base + k;
+            bool live = (k < kMtN) && (q < n);
+            int s = 0;
+            if (live) {
+                uint64_t z = temper(mt[k]);
+                double u = static_cast<double>(z) * 0x1p-64;
+                if (u >= 1.0) u = 0x1.fffffffffffffp-1;
+                if (u < omp) {
+                    s = 0;
+                } else {
+                    double v = (u - omp) / p;
+                    s = (v < 1.0 / 3.0) ? 1 : (v < 2.0 / 3.0) ? 2 : 3;
+                }
+            }
+            uint64_t bx = __ballot(live && (s & 1));
+            uint64_t bz = __ballot(live && (s >> 1));
+            if (lane == 0 && base + 64 * j < n) {
+                or_ballot(ex, wn, base + 64 * j, bx);
+                or_ballot(ez, wn, base + 64 * j, bz);
+            }
+        }
+        __syncthreads();
+    }
+
+    // Syndromes (jointbp_ets.cpp:1834-1849): parity of the neighbours' bits.
+    for (int s = 0; s < 2; ++s) {
+        const DevSide &sd = g.side[s];
+        const uint32_t *bits = s == 0 ? ex : ez;
+        uint32_t *out = (s == 0 ? tp.syn_x : tp.syn_z) + static_cast<size_t>(trial) * g.wm[s];
+        for (int c0 = 0; c0 < sd.m; c0 += 64) {
+            int c = c0 + lane;
+            int par = 0;
+            if (c < sd.m) {
+                for (int e = sd.chk_off[c]; e < sd.chk_off[c + 1]; ++e) {
+                    int v = sd.edge_var[e];
+                    par ^= (bits[v >> 5] >> (v & 31)) & 1;
+                }
+            }
+            uint64_t m = __ballot(par != 0);
+            if (lane == 0) {
+                out[c0 >> 5] = static_cast<uint32_t>(m);
+                if ((c0 >> 5) + 1 < g.wm[s]) out[(c0 >> 5) + 1] = static_cast<uint32_t>(m >> 32);
+            }
+        }
+    }
+    uint32_t *gx = tp.err_x + static_cast<size_t>(trial) * wn;
+    uint32_t *gz = tp.err_z + static_cast<size_t>(trial) * wn;
+    for (int w = lane; w < wn; w += 64) {
+        gx[w] = ex[w];
+        gz[w] = ez[w];
+    }
+}
+
+// Stabilizer-residual test (jointbp_ets.cpp:887-920, 6199-6207): the X part
+// of est xor err must lie in rowspace(Hz) and the Z part in rowspace(Hx).
+// With a reduced row-echelon basis, v is in the span iff v equals the xor of
+// the basis rows whose pivot bit is set in v.  One block per trial.
+constexpr int kResBlock = 256;
+
+__device__ bool in_span(const DevBasis &b, const uint32_t *d, uint32_t *acc, int wn) {
+    const int tid = threadIdx.x;
+    for (int w = tid; w < wn; w += kResBlock) acc[w] = 0u;
+    __syncthreads();
+    __shared__ int sel[kResBlock];
+    for (int i0 = 0; i0 < b.rank; i0 += kResBlock) {
+        int i = i0 + tid;
+        int hit = 0;
+        if (i < b.rank) {
+            int col = b.pivot[i];
+            hit = (d[col >> 5] >> (col & 31)) & 1;
+        }
+        sel[tid] = hit;
+        __syncthreads();
+        int lim = min(kResBlock, b.rank - i0);
+        for (int k = 0; k < lim; ++k) {
+            if (!sel[k]) continue;
+            const uint32_t *row = b.rows + static_cast<size_t>(i0 + k) * b.words;
+            for (int w = tid; w < wn; w += kResBlock) acc[w] ^= row[w];
+        }
+        __syncthreads();
+    }
+    int bad = 0;
+    for (int w = tid; w < wn; w += kResBlock) bad |= (acc[w] != d[w]);
+    return !__syncthreads_or(bad);
+}
+
+__global__ void __launch_bounds__(kResBlock) residual_kernel(DevGraph g, DevBasis bx, DevBasis bz,
+                                                            TrialPlanes tp) {
+    extern __shared__ uint32_t sm[];
+    const int wn = g.wn;
+    uint32_t *dx = sm;
+    uint32_t *dz = sm + wn;
+    uint32_t *acc = sm + 2 * wn;
+    const int trial = blockIdx.x;
+    const int tid = threadIdx.x;
+    const size_t off = static_cast<size_t>(trial) * wn;
+    int nz_x = 0, nz_z = 0;
+    for (int w = tid; w < wn; w += kResBlock) {
+        dx[w] = tp.err_x[off + w] ^ tp.est_x[off + w];
+        dz[w] = tp.err_z[off + w] ^ tp.est_z[off + w];
+        nz_x |= dx[w] != 0u;
+        nz_z |= dz[w] != 0u;
+    }
+    int any_x = __syncthreads_or(nz_x);
+    int any_z = __syncthreads_or(nz_z);
+    bool ok = tp.syn_ok[trial] != 0;
+    bool eq = true;
+    if (ok && any_x) eq = in_span(bz, dx, acc, wn);
+    if (ok && eq && any_z) eq = in_span(bx, dz, acc, wn);
+    if (tid == 0) {
+        tp.exact[trial] = (!any_x && !any_z) ? 1 : 0;
+        tp.success[trial] = (ok && eq) ? 1 : 0;
+    }
+}
+
+__global__ void fill_i32_kernel(int *p, int v, int count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = v;
+}
+
+// out[t][q] = xbit | zbit << 1  (0 = I, 1 = X, 2 = Z, 3 = Y)
+__global__ void unpack_states_kernel(const uint32_t *xb, const uint32_t *zb, int words, int n,
+                                     int count, int *out) {
+    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= static_cast<int64_t>(n) * count) return;
+    int t = static_cast<int>(i / n);
+    int q = static_cast<int>(i % n);
+    size_t w = static_cast<size_t>(t) * words + (q >> 5);
+    int x = (xb[w] >> (q & 31)) & 1;
+    int z = (zb[w] >> (q & 31)) & 1;
+    out[i] = x | (z << 1);
+}
+
+// Bit planes from int vectors (nonzero = 1), one word per thread.
+__global__ void pack_bits_kernel(const int *vals, int len, int words, int count, uint32_t *out) {
+    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= static_cast<int64_t>(words) * count) return;
+    int t = static_cast<int>(i / words);
+    int w = static_cast<int>(i % words);
+    uint32_t acc = 0;
+    for (int b = 0; b < 32; ++b) {
+        int q = w * 32 + b;
+        if (q < len && vals[static_cast<size_t>(t) * len + q] != 0) acc |= 1u << b;
+    }
+    out[i] = acc;
+}
+
+}  // namespace
+
+void launch_sample(const DevGraph &g, TrialPlanes tp, uint64_t base_seed, int64_t first_trial,
+                   int count, double p, hipStream_t st) {
+    if (count <= 0) return;
+    size_t lds = static_cast<size_t>(2) * g.wn * sizeof(uint32_t);
+    hipLaunchKernelGGL(sample_kernel, dim3(count), dim3(64), lds, st, g, tp, base_seed,
+                       first_trial, p);
+}
+
+void launch_residual(const DevGraph &g, DevBasis bx, DevBasis bz, TrialPlanes tp, int count,
+                     hipStream_t st) {
+    if (count <= 0) return;
+    size_t lds = static_cast<size_t>(3) * g.wn * sizeof(uint32_t);
+    hipLaunchKernelGGL(residual_kernel, dim3(count), dim3(kResBlock), lds, st, g, bx, bz, tp);
+}
+
+void launch_fill_i32(int *ptr, int value, int count, hipStream_t st) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((count + 255) / 256), dim3(256), 0, st, ptr, value,
+                       count);
+}
+
+void launch_unpack_states(const uint32_t *xbits, const uint32_t *zbits, int words, int n,
+                          int count, int *out, hipStream_t st) {
+    int64_t total = static_cast<int64_t>(n) * count;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(unpack_states_kernel, dim3(static_cast<unsigned>((total + 255) / 256)),
+                       dim3(256), 0, st, xbits, zbits, words, n, count, out);
+}
+
+void launch_pack_bits(const int *vals, int len, int words, int count, uint32_t *out,
+                      hipStream_t st) {
+    int64_t total = static_cast<int64_t>(words) * count;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(pack_bits_kernel, dim3(static_cast<unsigned>((total + 255) / 256)),
+                       dim3(256), 0, st, vals, len, words, count, out);
+}
+
+}  // namespace jbp

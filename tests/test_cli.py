@@ -1,0 +1,84 @@
+"""The jointbp_ets command line: the reference's flags, summary lines and
+exit codes (jointbp_ets.cpp:4822-6470), driven as a subprocess."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import PARAMS, ROOT, fnv1a, golden_json
+
+CLI = os.path.join(ROOT, "joint_bp_plus_pp_amd", "jointbp_ets")
+
+
+def run(args, **kw):
+    if not os.path.exists(CLI):
+        pytest.skip("jointbp_ets not built")
+    return subprocess.run([CLI, "--params", PARAMS] + args, capture_output=True, text=True,
+                          timeout=300, env=dict(os.environ, NO_COLOR="1"), **kw)
+
+
+def summary_fields(text):
+    line = [ln for ln in text.splitlines() if ln.startswith("trials=")][-1]
+    return dict(re.findall(r"(\w+)=([^\s]+)", line))
+
+
+def test_code_summary_lines_match_reference():
+    out = run(["--simulate", "--trials", "1", "--seed", "1"])  # stops before decoding (PP on)
+    ref = golden_json("aggregates.json")["cli_simulate"]["lines"]
+    for line in ref[:4]:
+        assert line in out.stdout, line
+    assert out.returncode == 1 and "--no-pp" in out.stderr
+
+
+def test_argument_errors():
+    assert run(["--simulate", "--no-pp", "--err", "x.txt"]).returncode == 1
+    assert run(["--no-pp", "--trials", "3"]).returncode == 1
+    assert run(["--simulate", "--no-pp", "--trial-index", "4", "--trials", "2"]).returncode == 1
+    assert run(["--bogus"]).returncode == 1
+    bad = subprocess.run([CLI, "--params", "/nonexistent"], capture_output=True, text=True)
+    assert bad.returncode == 1 and "Failed to read params" in bad.stderr
+
+
+@pytest.mark.gpu
+def test_simulate_fp64_matches_reference_run_summary():
+    ref = golden_json("aggregates.json")["cli_simulate"]
+    want = summary_fields("\n".join(ref["lines"]))
+    out = run(["--simulate", "--p", "0.05", "--trials", "200", "--max-iter", "50", "--no-pp",
+               "--seed", "106", "--report-every", "0", "--cuda", "--precision", "64"])
+    assert out.returncode == 0, out.stderr
+    got = summary_fields(out.stdout)
+    for key in ("trials", "failures", "bp_fail", "stab_success", "fer", "ci95", "iters",
+                "avg_iter", "exact_rate"):
+        assert got[key] == want[key], key
+
+
+@pytest.mark.gpu
+def test_err_decode_matches_reference(tmp_path):
+    from _oracle import Oracle
+    ref = golden_json("aggregates.json")["cli_err_trial7"]
+    err = Oracle().code_from_params(PARAMS).sample(106, 7, 0.05)
+    assert fnv1a(err) == ref["err_hash"]
+    ep, es = tmp_path / "err.txt", tmp_path / "est.txt"
+    np.savetxt(ep, err, fmt="%d")
+    out = run(["--err", str(ep), "--p", "0.05", "--max-iter", "50", "--no-pp", "--cuda",
+               "--precision", "64", "--est", str(es)])
+    assert out.returncode == 0, out.stderr
+    for line in ref["lines"][4:]:
+        assert line in out.stdout, line
+    assert fnv1a(np.loadtxt(es, dtype=np.int64)) == ref["est_hash"]
+
+
+@pytest.mark.gpu
+def test_min_fails_and_hist(tmp_path):
+    h = tmp_path / "hist.txt"
+    out = run(["--simulate", "--p", "0.05", "--trials", "256", "--max-iter", "50", "--no-pp",
+               "--seed", "106", "--report-every", "0", "--min-fails", "2", "--precision", "64",
+               "--iter-hist-out", str(h)])
+    assert out.returncode == 0, out.stderr
+    got = summary_fields(out.stdout)
+    assert got["failures"] == "2"
+    lines = h.read_text().splitlines()
+    assert lines[0].startswith("# costs") and lines[1] == "# iter fraction"
+    assert abs(sum(float(x.split()[1]) for x in lines[2:]) - 1.0) < 1e-6
