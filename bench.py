@@ -95,11 +95,26 @@ def cpu_baseline(args, first_trial):
                                        args.max_iter, secs, iters / count, count - sum(su))}
 
 
+def copy_calibration(mib):
+    """One hipMemcpy device-to-device of `mib` MiB (the runtime's copy kernel)."""
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("copy calibration: %d MiB device-to-device" % mib, file=sys.stderr)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--trials", type=int, default=8192, help="trials per step")
     ap.add_argument("--p", type=float, default=0.04)
     ap.add_argument("--max-iter", type=int, default=1000)
@@ -107,6 +122,9 @@ def main():
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--cpu-trials", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a plain copy "
+                         "kernel of known size, to scale profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the decoder shards trials, and this build "
@@ -115,6 +133,8 @@ def main():
 
     import joint_bp_plus_pp_amd as jbp
 
+    if args.copy_calibration_mb > 0:
+        copy_calibration(args.copy_calibration_mb)
     code = jbp.Code.from_params(PARAMS)
     dec = jbp.Decoder(code, 0)
     info = dec.info()

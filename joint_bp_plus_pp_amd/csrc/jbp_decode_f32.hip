@@ -220,6 +220,9 @@ __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, 
 #pragma unroll
                 for (int r = 0; r < J; ++r) B[slot(0, r, cc)] = with_lsb(gx - lam[0][r][cc], bx);
             }
+            // One register per mask: without this the per-column bits are
+            // forwarded to their later uses individually and spill.
+            asm volatile("" : "+v"(estx), "+v"(estz));
             __syncthreads();
             // X check pass: rows (r, t).
             uint32_t bad = 0;

@@ -12,6 +12,14 @@
 // LDS, the twist runs in three dependency phases across the 64 lanes, and
 // each lane tempers and classifies five of every 312 outputs.  Error and
 // syndrome bits are assembled with wave ballots into bit planes.
+//
+// Classification works on the raw 64-bit draw d.  The map d -> u (round to
+// double, scale by 2^-64, clamp below 1) is non-decreasing, and so are
+// u -> (u - (1-p)) / p under IEEE subtraction and division by p > 0; each of
+// the reference's three comparisons is therefore "d below a threshold".  The
+// host finds the three thresholds by bisection over d with the same double
+// arithmetic (first_draw_above below), so the device compares integers and the
+// result is identical to evaluating the doubles draw by draw.
 #include "jbp_common.h"
 
 namespace jbp {
@@ -52,9 +60,15 @@ __device__ __forceinline__ void or_ballot(uint32_t *plane, int words, int q0, ui
     if (sh && w + 2 < words) plane[w + 2] |= static_cast<uint32_t>(m >> (64 - sh));
 }
 
+// State of draw d: I when d < t[0], X when d < t[1], Z when d < t[2], else Y.
+struct DrawThresholds {
+    uint64_t t[3];
+    uint32_t all_below;  // bit k: every draw is below threshold k
+};
+
 // grid = count trials, block = 64 (one wavefront), dynamic LDS = 2*wn words.
 __global__ void __launch_bounds__(64) sample_kernel(DevGraph g, TrialPlanes tp, uint64_t base_seed,
-                                                    int64_t first_trial, double p) {
+                                                    int64_t first_trial, DrawThresholds th) {
     __shared__ uint64_t mt[kMtN];
     extern __shared__ uint32_t planes[];  // err_x | err_z
     const int lane = threadIdx.x;
@@ -75,7 +89,6 @@ __global__ void __launch_bounds__(64) sample_kernel(DevGraph g, TrialPlanes tp, 
     }
     __syncthreads();
 
-    const double omp = 1.0 - p;
     for (int base = 0; base < n; base += kMtN) {
         // Twist phase A: k in [0, 156) reads old k, k+1, k+156.
         uint64_t nv[3];
@@ -115,15 +128,11 @@ __global__ void __launch_bounds__(64) sample_kernel(DevGraph g, TrialPlanes tp, 
             bool live = (k < kMtN) && (q < n);
             int s = 0;
             if (live) {
-                uint64_t z = temper(mt[k]);
-                double u = static_cast<double>(z) * 0x1p-64;
-                if (u >= 1.0) u = 0x1.fffffffffffffp-1;
-                if (u < omp) {
-                    s = 0;
-                } else {
-                    double v = (u - omp) / p;
-                    s = (v < 1.0 / 3.0) ? 1 : (v < 2.0 / 3.0) ? 2 : 3;
-                }
+                const uint64_t z = temper(mt[k]);
+                const bool b0 = (th.all_below & 1u) || z < th.t[0];
+                const bool b1 = (th.all_below & 2u) || z < th.t[1];
+                const bool b2 = (th.all_below & 4u) || z < th.t[2];
+                s = b0 ? 0 : (b1 ? 1 : (b2 ? 2 : 3));
             }
             uint64_t bx = __ballot(live && (s & 1));
             uint64_t bz = __ballot(live && (s >> 1));
@@ -260,12 +269,55 @@ __global__ void pack_bits_kernel(const int *vals, int len, int words, int count,
 
 }  // namespace
 
+namespace {
+
+// libstdc++ generate_canonical<double, 53> of one mt19937_64 draw.
+double canonical(uint64_t d) {
+    double u = static_cast<double>(d) * 0x1p-64;
+    return u >= 1.0 ? 0x1.fffffffffffffp-1 : u;
+}
+
+// The reference's classification of a uniform u (jointbp_ets.cpp:5936-5945),
+// as the index of the first test that fails: 0 = I, 1 = X, 2 = Z, 3 = Y.
+int classify(double u, double p) {
+    volatile double omp = 1.0 - p;  // keep the host's double rounding explicit
+    if (u < omp) return 0;
+    volatile double v = (u - omp) / p;
+    if (v < 1.0 / 3.0) return 1;
+    if (v < 2.0 / 3.0) return 2;
+    return 3;
+}
+
+// Smallest draw whose state is above k (i.e. >= k+1); classify(canonical(d))
+// is non-decreasing in d.  Returns false when no draw gets there.
+bool first_draw_above(int k, double p, uint64_t &out) {
+    if (classify(canonical(~0ull), p) <= k) return false;
+    uint64_t lo = 0, hi = ~0ull;  // answer in [lo, hi]
+    while (lo < hi) {
+        uint64_t mid = lo + (hi - lo) / 2;
+        if (classify(canonical(mid), p) > k) hi = mid;
+        else lo = mid + 1;
+    }
+    out = lo;
+    return true;
+}
+
+}  // namespace
+
 void launch_sample(const DevGraph &g, TrialPlanes tp, uint64_t base_seed, int64_t first_trial,
                    int count, double p, hipStream_t st) {
     if (count <= 0) return;
+    DrawThresholds th;
+    th.all_below = 0;
+    for (int k = 0; k < 3; ++k) {
+        if (!first_draw_above(k, p, th.t[k])) {
+            th.t[k] = 0;
+            th.all_below |= 1u << k;
+        }
+    }
     size_t lds = static_cast<size_t>(2) * g.wn * sizeof(uint32_t);
     hipLaunchKernelGGL(sample_kernel, dim3(count), dim3(64), lds, st, g, tp, base_seed,
-                       first_trial, p);
+                       first_trial, th);
 }
 
 void launch_residual(const DevGraph &g, DevBasis bx, DevBasis bz, TrialPlanes tp, int count,

@@ -40,7 +40,8 @@ def test_device_info(main_decoder):
     assert info["basis"] == 1
 
 
-@pytest.mark.parametrize("p,base", [(0.05, 106), (0.04, 106), (0.3, 7)])
+@pytest.mark.parametrize("p,base", [(0.05, 106), (0.04, 106), (0.3, 7), (0.001, 5), (0.999, 5),
+                                    (1.0, 9), (0.0, 9)])
 def test_sampler_bit_exact(main_decoder, main_code_oracle, p, base):
     print("sampler", p, base, flush=True)
     err, sx, sz = main_decoder.sample(base, 1, 48, p)
