@@ -76,7 +76,8 @@ typedef struct jbp_decode_result {
  * CSR decoders.  On failure *out is NULL. */
 int jbp_create(const jbp_graph_desc *graph, int device_id, jbp_ctx **out, char *err, size_t errlen);
 
-/* cuda_bp_destroy (jointbp_cuda.h:49, jointbp_cuda.cu:644-660); NULL is a no-op. */
+/* cuda_bp_destroy (jointbp_cuda.h:49, jointbp_cuda.cu:644-660); NULL is a no-op.
+ * Learned models created on the context must be destroyed first. */
 void jbp_destroy(jbp_ctx *ctx);
 
 /* cuda_bp_decode (jointbp_cuda.h:50-63, jointbp_cuda.cu:681-1104): decode one
@@ -164,6 +165,54 @@ double jbp_last_kernel_ms(jbp_ctx *ctx, int which, int64_t *launches);
  * out[1] = permutation size P, out[2] = J, out[3] = L, out[4] = device id,
  * out[5] = compute units, out[6] = tuned threads per block. */
 int jbp_info(jbp_ctx *ctx, int64_t out[8]);
+
+/* ---------------------------------------------------------------------------
+ * Learned (neural-weighted) BP.  No counterpart in the reference program; it
+ * extends the reference's joint BP with a weight per (side, permutation
+ * block, iteration) on every variable-to-check and every check-to-variable
+ * message and a 2-16-1 MLP that damps the check-to-variable messages from
+ * (iteration / T, unsatisfied-check fraction).  Parameter vector (doubles):
+ *   wc[2][nb][T] | wv[2][nb][T] | mlp[65] (W1[16][2], b1[16], W2[16], b2),
+ * nb = J*L blocks per side; the block of the edge at position cc of check
+ * r*P + x is r*L + cc.  All weights 1 and damping off is plain BP.
+ * Needs a permutation-block code (jbp_info out[0] == 1).
+ * ------------------------------------------------------------------------- */
+typedef struct jbp_learned jbp_learned;
+
+/* T learned iterations (1..64); parameters start at weights 1 and an MLP
+ * whose damping is close to 0. */
+int jbp_learned_create(jbp_ctx *ctx, int T, jbp_learned **out, char *err, size_t errlen);
+/* Destroy a model before the context it was created on. */
+void jbp_learned_destroy(jbp_learned *model);
+int jbp_learned_param_count(const jbp_learned *model);
+/* use_mlp = 0 switches damping off (d = 0). */
+int jbp_learned_set_params(jbp_learned *model, const double *params, int use_mlp, char *err,
+                           size_t errlen);
+int jbp_learned_get_params(const jbp_learned *model, double *params);
+
+/* Loss (mean binary cross-entropy of the per-qubit X/Z marginals against
+ * the error, averaged over iterations 1..T) and its gradient for explicit
+ * errors (batch x nvars states 0..3, one p per trial), computed on the
+ * device by the forward and adjoint kernels in FP32 or FP64.  conv_out
+ * (may be NULL): first iteration whose estimate satisfies both syndromes
+ * (0 = none).  grad (may be NULL): jbp_learned_param_count values. */
+int jbp_learned_loss_grad(jbp_learned *model, int precision, int batch, const int *err_states,
+                          const double *p, double *loss, double *grad, int *conv_out, char *err,
+                          size_t errlen);
+
+/* One training step on device-sampled trials first..first+batch-1 (trial
+ * seeds as in jbp_simulate) with p uniform in [p_lo, p_hi] per trial:
+ * FP32 forward + adjoint, then an Adam update (betas 0.9 / 0.999). */
+int jbp_learned_train_step(jbp_learned *model, uint64_t base_seed, int64_t first_trial, int batch,
+                           double p_lo, double p_hi, double lr, double *loss, char *err,
+                           size_t errlen);
+
+/* jbp_simulate with the learned decoder for the first T iterations; a trial
+ * whose estimate has not satisfied both syndromes by then is decoded again
+ * by plain FP32 BP from the channel prior for up to cfg->max_iter
+ * iterations (its iteration count is T plus those).  FP32 only. */
+int jbp_simulate_learned(jbp_ctx *ctx, const jbp_learned *model, const jbp_sim_cfg *cfg,
+                         jbp_trial_out *per_trial, jbp_run_stats *stats, char *err, size_t errlen);
 
 #ifdef __cplusplus
 }

@@ -12,10 +12,12 @@ bindings raises.
 import ctypes as C
 import os
 import re
+import weakref
 
 import numpy as np
 
-__all__ = ["Code", "Decoder", "JbpError", "lib", "LIB_PATH", "FP32", "FP64", "prior_from_p"]
+__all__ = ["Code", "Decoder", "LearnedModel", "JbpError", "lib", "LIB_PATH", "FP32", "FP64",
+           "prior_from_p"]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libjbp.so")
@@ -88,6 +90,14 @@ def lib():
         "jbp_in_row_space": (i32, [vp, i32, i32, vp, vp, cp, sz]),
         "jbp_last_kernel_ms": (dbl, [vp, i32, C.POINTER(i64)]),
         "jbp_info": (i32, [vp, vp]),
+        "jbp_learned_create": (i32, [vp, i32, C.POINTER(vp), cp, sz]),
+        "jbp_learned_destroy": (None, [vp]),
+        "jbp_learned_param_count": (i32, [vp]),
+        "jbp_learned_set_params": (i32, [vp, vp, i32, cp, sz]),
+        "jbp_learned_get_params": (i32, [vp, vp]),
+        "jbp_learned_loss_grad": (i32, [vp, i32, i32, vp, vp, vp, vp, vp, cp, sz]),
+        "jbp_learned_train_step": (i32, [vp, u64, i64, i32, dbl, dbl, dbl, vp, cp, sz]),
+        "jbp_simulate_learned": (i32, [vp, vp, C.POINTER(SimCfg), C.POINTER(TrialOut), C.POINTER(RunStats), cp, sz]),
         "jbp_code_from_params": (i32, [cp, C.POINTER(vp), cp, sz]),
         "jbp_code_from_checks": (i32, [i32, i32, i32, vp, i32, i32, vp, C.POINTER(vp), cp, sz]),
         "jbp_code_hgp": (i32, [i32, i32, i32, i32, u64, C.POINTER(vp), cp, sz]),
@@ -225,8 +235,12 @@ class Decoder:
         h, e = C.c_void_p(), _errbuf()
         _check(lib().jbp_create(C.byref(self._graph), device, C.byref(h), e, 512), e)
         self._h = h
+        self._models = weakref.WeakSet()  # learned models bound to this context
 
     def destroy(self):
+        """Free the context; learned models bound to it are freed first."""
+        for m in list(getattr(self, "_models", ())):
+            m.destroy()
         if self._h and self._h.value:
             lib().jbp_destroy(self._h)
             self._h = C.c_void_p()
@@ -300,9 +314,20 @@ class Decoder:
         _check(lib().jbp_in_row_space(self._h, side, bits.shape[0], _ptr(bits), _ptr(out), e, 512), e)
         return out.astype(bool)
 
+    def simulate_learned(self, model, base_seed, n_trials, p, max_iter=1000, first_trial=1,
+                         check_warmup=0, check_interval=1, min_fails=0, per_trial=True,
+                         want_err=False, want_est=False):
+        """jbp_simulate with the learned decoder for model.T iterations, then
+        plain FP32 BP from the prior on trials that did not converge."""
+        return self.simulate(base_seed, n_trials, p, max_iter=max_iter, first_trial=first_trial,
+                             precision=FP32, check_warmup=check_warmup,
+                             check_interval=check_interval, min_fails=min_fails,
+                             per_trial=per_trial, want_err=want_err, want_est=want_est,
+                             _model=model)
+
     def simulate(self, base_seed, n_trials, p, max_iter=50, first_trial=1, damping=0.0,
                  precision=FP32, check_warmup=0, check_interval=1, min_fails=0, slots=0, block=0,
-                 per_trial=True, want_err=False, want_est=False):
+                 per_trial=True, want_err=False, want_est=False, _model=None):
         """The --simulate --no-pp inner loop on the device.  Returns (stats,
         per-trial dict or None)."""
         cfg = SimCfg(base_seed, first_trial, n_trials, p, max_iter, damping, precision, check_warmup,
@@ -322,13 +347,81 @@ class Decoder:
             for k, a in arrays.items():
                 setattr(out, k, a.ctypes.data_as(C.POINTER(C.c_int if a.dtype == np.int32 else C.c_uint8)))
         st, e = RunStats(), _errbuf()
-        _check(lib().jbp_simulate(self._h, C.byref(cfg), C.byref(out) if out else None, C.byref(st),
-                                  e, 512), e)
+        if _model is None:
+            rc = lib().jbp_simulate(self._h, C.byref(cfg), C.byref(out) if out else None,
+                                    C.byref(st), e, 512)
+        else:
+            rc = lib().jbp_simulate_learned(self._h, _model._h, C.byref(cfg),
+                                            C.byref(out) if out else None, C.byref(st), e, 512)
+        _check(rc, e)
         stats = {k: getattr(st, k) for k, _ in RunStats._fields_}
         if per_trial:
             d = int(stats["done"])
             arrays = {k: v[:d] for k, v in arrays.items()}
         return stats, (arrays if per_trial else None)
+
+
+class LearnedModel:
+    """Learned-BP parameters on a decoder's device (include/jbp.h,
+    jbp_learned_*).  Layout: wc[2][nb][T] | wv[2][nb][T] | mlp[65]."""
+
+    MLP_PARAMS = 65
+
+    def __init__(self, decoder, T=25):
+        self.decoder = decoder
+        self.T = T
+        h, e = C.c_void_p(), _errbuf()
+        _check(lib().jbp_learned_create(decoder._h, T, C.byref(h), e, 512), e)
+        self._h = h
+        decoder._models.add(self)
+        self.count = lib().jbp_learned_param_count(h)
+        self.nb = (self.count - self.MLP_PARAMS) // (4 * T)
+
+    def destroy(self):
+        if self._h and self._h.value:
+            lib().jbp_learned_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def get_params(self):
+        out = np.zeros(self.count, np.float64)
+        lib().jbp_learned_get_params(self._h, _ptr(out))
+        return out
+
+    def set_params(self, params, use_mlp=True):
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        assert params.size == self.count
+        e = _errbuf()
+        _check(lib().jbp_learned_set_params(self._h, _ptr(params), int(use_mlp), e, 512), e)
+
+    def split(self, params):
+        k = 2 * self.nb * self.T
+        return (params[:k].reshape(2, self.nb, self.T), params[k:2 * k].reshape(2, self.nb, self.T),
+                params[2 * k:])
+
+    def loss_grad(self, err, p, precision=FP64, grad=True):
+        err = np.ascontiguousarray(np.atleast_2d(err), dtype=np.int32)
+        batch = err.shape[0]
+        p = np.ascontiguousarray(np.broadcast_to(p, (batch,)), dtype=np.float64)
+        loss = C.c_double(0.0)
+        g = np.zeros(self.count, np.float64) if grad else None
+        conv = np.zeros(batch, np.int32)
+        e = _errbuf()
+        _check(lib().jbp_learned_loss_grad(self._h, precision, batch, _ptr(err), _ptr(p),
+                                           C.byref(loss), _ptr(g), _ptr(conv), e, 512), e)
+        return loss.value, g, conv
+
+    def train_step(self, base_seed, first_trial, batch, p_lo=0.03, p_hi=0.05, lr=1e-2):
+        loss = C.c_double(0.0)
+        e = _errbuf()
+        _check(lib().jbp_learned_train_step(self._h, base_seed, first_trial, batch, p_lo, p_hi, lr,
+                                            C.byref(loss), e, 512), e)
+        return loss.value
 
 
 def pack_bits(a):

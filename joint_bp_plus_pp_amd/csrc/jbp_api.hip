@@ -16,16 +16,11 @@
 #include <string>
 #include <vector>
 
-#include "../../include/jbp.h"
-#include "jbp_common.h"
+#include "jbp_ctx.h"
 
 using namespace jbp;
 
-namespace {
-
-// Largest dense basis (bytes per side) built for the device residual test.
-constexpr size_t kMaxBasisBytes = size_t(256) << 20;
-constexpr int kDefaultChunk = 65536;
+namespace jbp {
 
 void set_err(char *err, size_t len, const char *fmt, ...) {
     if (!err || !len) return;
@@ -35,60 +30,15 @@ void set_err(char *err, size_t len, const char *fmt, ...) {
     va_end(ap);
 }
 
-// Host-side fault injection for CPU-only tests of the caller's fallback:
-// JBP_FAULT_INJECT=create|decode|simulate makes that entry point fail.
 bool fault(const char *what) {
     const char *v = std::getenv("JBP_FAULT_INJECT");
     return v && std::strstr(v, what) != nullptr;
 }
 
-struct HostSide {
-    int m = 0, E = 0;
-    std::vector<int> chk_off, edge_var, var_off, var_edge;
-};
-
-}  // namespace
-
-struct jbp_ctx {
-    int device = 0;
-    int cus = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int n = 0;
-    HostSide hs[2];
-    DevGraph dg;
-    std::vector<void *> allocs;  // graph arrays, basis, tables
-    bool qc = false;
-    QcParams qcp;
-    int *d_qc_tables = nullptr;
-    bool have_basis = false;
-    DevBasis basis[2];
-    // chunk buffers
-    int chunk_cap = 0;
-    TrialPlanes tp;
-    std::vector<void *> chunk_allocs;
-    double *f64_scratch = nullptr;
-    int f64_slots = 0;
-    int f64_block = 256;
-    int *d_counter = nullptr;
-    int *d_states = nullptr;  // unpack buffer
-    size_t d_states_cap = 0;
-    int *d_ints = nullptr;    // pack input buffer
-    size_t d_ints_cap = 0;
-    double last_ms[3] = {0, 0, 0};
-    int64_t launches[3] = {0, 0, 0};
-};
+}  // namespace jbp
 
 namespace {
 
-#define HIPCHK(call)                                                                    \
-    do {                                                                                \
-        hipError_t e_ = (call);                                                         \
-        if (e_ != hipSuccess) {                                                         \
-            set_err(err, errlen, "%s failed: %s", #call, hipGetErrorString(e_));        \
-            return JBP_ERR_DEVICE;                                                      \
-        }                                                                               \
-    } while (0)
 
 template <typename T>
 int dev_upload(jbp_ctx *ctx, const std::vector<T> &h, const T **out, char *err, size_t errlen) {
@@ -234,6 +184,10 @@ void build_rref(const HostSide &h, int n, std::vector<uint32_t> &rows32, std::ve
                 static_cast<uint32_t>(rows[r][k >> 1] >> (32 * (k & 1)));
 }
 
+}  // namespace
+
+namespace jbp {
+
 void free_chunk(jbp_ctx *ctx) {
     for (void *p : ctx->chunk_allocs) (void)hipFree(p);
     ctx->chunk_allocs.clear();
@@ -313,7 +267,8 @@ Prior prior_from_p(double p) {
 
 // Decode `count` trials whose syndrome planes are already in ctx->tp.
 int run_decode(jbp_ctx *ctx, int count, const Prior &prior, int max_iter, int warmup, int interval,
-               double damping, int precision, char *err, size_t errlen) {
+               double damping, int precision, char *err, size_t errlen,
+               const LearnedInfer *learned) {
     if (precision != JBP_FP32 && precision != JBP_FP64) {
         set_err(err, errlen, "precision must be 32 or 64");
         return JBP_ERR_ARG;
@@ -331,7 +286,8 @@ int run_decode(jbp_ctx *ctx, int count, const Prior &prior, int max_iter, int wa
             return JBP_ERR_UNSUPPORTED;
         }
         launch_decode_qc_f32(ctx->dg, ctx->qcp, ctx->d_qc_tables, ctx->tp, count, prior, max_iter,
-                             warmup, interval, std::max(1, ctx->cus), ctx->d_counter, ctx->stream);
+                             warmup, interval, std::max(1, ctx->cus), ctx->d_counter, ctx->stream,
+                             learned);
     } else {
         int rc = ensure_f64(ctx, err, errlen);
         if (rc) return rc;
@@ -348,7 +304,118 @@ int run_decode(jbp_ctx *ctx, int count, const Prior &prior, int max_iter, int wa
     return JBP_OK;
 }
 
-}  // namespace
+int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_run_stats *stats,
+                  const LearnedInfer *learned, char *err, size_t errlen) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (!ctx || !cfg || cfg->n_trials < 0 || cfg->max_iter < 0) {
+        set_err(err, errlen, "bad arguments");
+        return JBP_ERR_ARG;
+    }
+    if (fault("simulate")) {
+        set_err(err, errlen, "fault injected (JBP_FAULT_INJECT=simulate)");
+        return JBP_ERR_DEVICE;
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    jbp_run_stats st;
+    std::memset(&st, 0, sizeof(st));
+    const DevGraph &g = ctx->dg;
+    const Prior prior = prior_from_p(cfg->p);
+    if (cfg->block > 0) ctx->f64_block = cfg->block;
+    if (cfg->slots > 0 && cfg->precision == JBP_FP64 && cfg->slots != ctx->f64_slots) {
+        if (ctx->f64_scratch) (void)hipFree(ctx->f64_scratch);
+        ctx->f64_scratch = nullptr;
+        ctx->f64_slots = cfg->slots;
+    }
+    const int64_t total = cfg->n_trials;
+    const int chunk = static_cast<int>(std::min<int64_t>(std::max<int64_t>(total, 1), kDefaultChunk));
+    int rc = ensure_chunk(ctx, chunk, err, errlen);
+    if (rc) return rc;
+    std::vector<int> iters(chunk);
+    std::vector<uint8_t> ok(chunk), succ(chunk), exact(chunk);
+    double dev_ms = 0.0;
+    bool stop = false;
+    for (int64_t done = 0; done < total && !stop;) {
+        const int cnt = static_cast<int>(std::min<int64_t>(chunk, total - done));
+        const int64_t first = cfg->first_trial + done;
+        HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+        launch_sample(g, ctx->tp, cfg->base_seed, first, cnt, cfg->p, nullptr, ctx->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        HIPCHK(hipEventSynchronize(ctx->ev1));
+        float sms = 0.f;
+        HIPCHK(hipEventElapsedTime(&sms, ctx->ev0, ctx->ev1));
+        ctx->last_ms[0] = sms;
+        ctx->launches[0]++;
+        rc = run_decode(ctx, cnt, prior, cfg->max_iter, cfg->check_warmup, cfg->check_interval,
+                        cfg->damping, cfg->precision, err, errlen, learned);
+        if (rc) return rc;
+        dev_ms += ctx->last_ms[1];
+        if (ctx->have_basis) {
+            HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+            launch_residual(g, ctx->basis[0], ctx->basis[1], ctx->tp, cnt, ctx->stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        }
+        HIPCHK(hipMemcpyAsync(iters.data(), ctx->tp.iters, cnt * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ok.data(), ctx->tp.syn_ok, cnt, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(succ.data(), ctx->tp.success, cnt, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(exact.data(), ctx->tp.exact, cnt, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (ctx->have_basis) {
+            float rms = 0.f;
+            HIPCHK(hipEventElapsedTime(&rms, ctx->ev0, ctx->ev1));
+            ctx->last_ms[2] = rms;
+            ctx->launches[2]++;
+        } else {
+            // Without a dense basis only exact matches are counted as successes.
+            for (int i = 0; i < cnt; ++i) succ[i] = 0;
+        }
+        if (pt && (pt->err || pt->est)) {
+            if ((rc = ensure_states(ctx, static_cast<size_t>(cnt) * g.n, err, errlen))) return rc;
+            for (int which = 0; which < 2; ++which) {
+                int *dst = which == 0 ? pt->err : pt->est;
+                if (!dst) continue;
+                const uint32_t *xb = which == 0 ? ctx->tp.err_x : ctx->tp.est_x;
+                const uint32_t *zb = which == 0 ? ctx->tp.err_z : ctx->tp.est_z;
+                launch_unpack_states(xb, zb, g.wn, g.n, cnt, ctx->d_states, ctx->stream);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpy(dst + static_cast<size_t>(done) * g.n, ctx->d_states,
+                                 static_cast<size_t>(cnt) * g.n * 4, hipMemcpyDeviceToHost));
+            }
+        }
+        int take = cnt;
+        for (int i = 0; i < cnt; ++i) {
+            if (!ctx->have_basis) succ[i] = (ok[i] && exact[i]) ? 1 : 0;
+            st.total_iters += iters[i];
+            st.failures += succ[i] ? 0 : 1;
+            st.bp_failures += ok[i] ? 0 : 1;
+            st.exact += exact[i] ? 1 : 0;
+            if (cfg->min_fails > 0 && st.failures >= cfg->min_fails) {
+                take = i + 1;
+                stop = true;
+                break;
+            }
+        }
+        if (pt) {
+            for (int i = 0; i < take; ++i) {
+                if (pt->iterations) pt->iterations[done + i] = iters[i];
+                if (pt->syn_match) pt->syn_match[done + i] = ok[i];
+                if (pt->success) pt->success[done + i] = succ[i];
+                if (pt->exact) pt->exact[done + i] = exact[i];
+            }
+        }
+        done += take;
+        st.done = done;
+    }
+    st.device_ms = dev_ms;
+    st.elapsed_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = st;
+    return JBP_OK;
+}
+
+
+}  // namespace jbp
 
 extern "C" {
 
@@ -510,7 +577,7 @@ int jbp_sample(jbp_ctx *ctx, uint64_t base_seed, int64_t first_trial, int count,
     if (rc) return rc;
     const DevGraph &g = ctx->dg;
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    launch_sample(g, ctx->tp, base_seed, first_trial, count, p, ctx->stream);
+    launch_sample(g, ctx->tp, base_seed, first_trial, count, p, nullptr, ctx->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     if (err_out) {
@@ -683,112 +750,7 @@ int jbp_in_row_space(jbp_ctx *ctx, int side, int count, const int *bits, uint8_t
 
 int jbp_simulate(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_run_stats *stats,
                  char *err, size_t errlen) {
-    auto t0 = std::chrono::steady_clock::now();
-    if (!ctx || !cfg || cfg->n_trials < 0 || cfg->max_iter < 0) {
-        set_err(err, errlen, "bad arguments");
-        return JBP_ERR_ARG;
-    }
-    if (fault("simulate")) {
-        set_err(err, errlen, "fault injected (JBP_FAULT_INJECT=simulate)");
-        return JBP_ERR_DEVICE;
-    }
-    HIPCHK(hipSetDevice(ctx->device));
-    jbp_run_stats st;
-    std::memset(&st, 0, sizeof(st));
-    const DevGraph &g = ctx->dg;
-    const Prior prior = prior_from_p(cfg->p);
-    if (cfg->block > 0) ctx->f64_block = cfg->block;
-    if (cfg->slots > 0 && cfg->precision == JBP_FP64 && cfg->slots != ctx->f64_slots) {
-        if (ctx->f64_scratch) (void)hipFree(ctx->f64_scratch);
-        ctx->f64_scratch = nullptr;
-        ctx->f64_slots = cfg->slots;
-    }
-    const int64_t total = cfg->n_trials;
-    const int chunk = static_cast<int>(std::min<int64_t>(std::max<int64_t>(total, 1), kDefaultChunk));
-    int rc = ensure_chunk(ctx, chunk, err, errlen);
-    if (rc) return rc;
-    std::vector<int> iters(chunk);
-    std::vector<uint8_t> ok(chunk), succ(chunk), exact(chunk);
-    double dev_ms = 0.0;
-    bool stop = false;
-    for (int64_t done = 0; done < total && !stop;) {
-        const int cnt = static_cast<int>(std::min<int64_t>(chunk, total - done));
-        const int64_t first = cfg->first_trial + done;
-        HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-        launch_sample(g, ctx->tp, cfg->base_seed, first, cnt, cfg->p, ctx->stream);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-        HIPCHK(hipEventSynchronize(ctx->ev1));
-        float sms = 0.f;
-        HIPCHK(hipEventElapsedTime(&sms, ctx->ev0, ctx->ev1));
-        ctx->last_ms[0] = sms;
-        ctx->launches[0]++;
-        rc = run_decode(ctx, cnt, prior, cfg->max_iter, cfg->check_warmup, cfg->check_interval,
-                        cfg->damping, cfg->precision, err, errlen);
-        if (rc) return rc;
-        dev_ms += ctx->last_ms[1];
-        if (ctx->have_basis) {
-            HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-            launch_residual(g, ctx->basis[0], ctx->basis[1], ctx->tp, cnt, ctx->stream);
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-        }
-        HIPCHK(hipMemcpyAsync(iters.data(), ctx->tp.iters, cnt * 4, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipMemcpyAsync(ok.data(), ctx->tp.syn_ok, cnt, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipMemcpyAsync(succ.data(), ctx->tp.success, cnt, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipMemcpyAsync(exact.data(), ctx->tp.exact, cnt, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        if (ctx->have_basis) {
-            float rms = 0.f;
-            HIPCHK(hipEventElapsedTime(&rms, ctx->ev0, ctx->ev1));
-            ctx->last_ms[2] = rms;
-            ctx->launches[2]++;
-        } else {
-            // Without a dense basis only exact matches are counted as successes.
-            for (int i = 0; i < cnt; ++i) succ[i] = 0;
-        }
-        if (pt && (pt->err || pt->est)) {
-            if ((rc = ensure_states(ctx, static_cast<size_t>(cnt) * g.n, err, errlen))) return rc;
-            for (int which = 0; which < 2; ++which) {
-                int *dst = which == 0 ? pt->err : pt->est;
-                if (!dst) continue;
-                const uint32_t *xb = which == 0 ? ctx->tp.err_x : ctx->tp.est_x;
-                const uint32_t *zb = which == 0 ? ctx->tp.err_z : ctx->tp.est_z;
-                launch_unpack_states(xb, zb, g.wn, g.n, cnt, ctx->d_states, ctx->stream);
-                HIPCHK(hipGetLastError());
-                HIPCHK(hipMemcpy(dst + static_cast<size_t>(done) * g.n, ctx->d_states,
-                                 static_cast<size_t>(cnt) * g.n * 4, hipMemcpyDeviceToHost));
-            }
-        }
-        int take = cnt;
-        for (int i = 0; i < cnt; ++i) {
-            if (!ctx->have_basis) succ[i] = (ok[i] && exact[i]) ? 1 : 0;
-            st.total_iters += iters[i];
-            st.failures += succ[i] ? 0 : 1;
-            st.bp_failures += ok[i] ? 0 : 1;
-            st.exact += exact[i] ? 1 : 0;
-            if (cfg->min_fails > 0 && st.failures >= cfg->min_fails) {
-                take = i + 1;
-                stop = true;
-                break;
-            }
-        }
-        if (pt) {
-            for (int i = 0; i < take; ++i) {
-                if (pt->iterations) pt->iterations[done + i] = iters[i];
-                if (pt->syn_match) pt->syn_match[done + i] = ok[i];
-                if (pt->success) pt->success[done + i] = succ[i];
-                if (pt->exact) pt->exact[done + i] = exact[i];
-            }
-        }
-        done += take;
-        st.done = done;
-    }
-    st.device_ms = dev_ms;
-    st.elapsed_ms =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    if (stats) *stats = st;
-    return JBP_OK;
+    return jbp::simulate_impl(ctx, cfg, pt, stats, nullptr, err, errlen);
 }
 
 }  // extern "C"

@@ -66,9 +66,20 @@ struct Prior {
     double p[4];
 };
 
-// Launchers (defined in the .hip files).
+// Sampler classification of one raw 64-bit draw d: state I when d < t[0],
+// X when d < t[1], Z when d < t[2], else Y (all_below bit k: every draw is
+// below threshold k).  Computed on the host for a given p.
+struct DrawThresholds {
+    uint64_t t[3];
+    uint32_t all_below;
+};
+DrawThresholds draw_thresholds(double p);
+
+// Launchers (defined in the .hip files).  d_per_trial (device, may be null)
+// overrides p with one threshold set per trial.
 void launch_sample(const DevGraph &g, TrialPlanes tp, uint64_t base_seed, int64_t first_trial,
-                   int count, double p, hipStream_t st);
+                   int count, double p, const DrawThresholds *d_per_trial, hipStream_t st);
+void launch_syndromes(const DevGraph &g, TrialPlanes tp, int count, hipStream_t st);
 void launch_residual(const DevGraph &g, DevBasis bx, DevBasis bz, TrialPlanes tp, int count,
                      hipStream_t st);
 void launch_decode_f64(const DevGraph &g, TrialPlanes tp, int count, Prior prior, int max_iter,
@@ -76,9 +87,13 @@ void launch_decode_f64(const DevGraph &g, TrialPlanes tp, int count, Prior prior
                        hipStream_t st);
 size_t decode_f64_scratch_bytes(const DevGraph &g, int slots);
 // d_tables: device ints [2][J][L] ainv followed by [2][J][L] b.
+struct LearnedInfer;
+// learned == nullptr: plain BP.  Otherwise learned->T learned iterations,
+// then plain BP restarted on trials that did not converge.
 void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_tables,
                           TrialPlanes tp, int count, Prior prior, int max_iter, int check_warmup,
-                          int check_interval, int slots, int *counter, hipStream_t st);
+                          int check_interval, int slots, int *counter, hipStream_t st,
+                          const LearnedInfer *learned = nullptr);
 bool qc_f32_supported(const QcParams &qc, int device, char *why, size_t len);
 void launch_fill_i32(int *ptr, int value, int count, hipStream_t st);
 void launch_unpack_states(const uint32_t *xbits, const uint32_t *zbits, int words, int n,

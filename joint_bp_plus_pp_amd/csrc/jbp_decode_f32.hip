@@ -40,6 +40,7 @@
 // inputs.  The test of iteration k's estimate therefore completes during the
 // check pass of iteration k+1; a trial that converged at k keeps est_k.
 #include "jbp_common.h"
+#include "jbp_learned.h"
 
 namespace jbp {
 namespace {
@@ -117,15 +118,223 @@ __device__ __forceinline__ uint32_t check_row(float *row, uint32_t syn) {
     return par;
 }
 
-template <int P, int J, int L>
+// Damping of the learned decoder, d = sigmoid(W2 . tanh(W1 (x0, x1) + b1) + b2),
+// evaluated by the first wavefront: lane i < 16 forms hidden unit i, and an
+// xor-shuffle tree sums the 16 products.  Every lane of wave 0 returns d.
+__device__ __forceinline__ float mlp_damping_wave(const float *mlp, float x0, float x1, int lane) {
+    constexpr float kLog2e = 1.4426950408889634f;
+    float v = 0.0f;
+    if (lane < kMlpHidden) {
+        // tanh(a) = 1 - 2 / (e^{2a} + 1), with a clamped where tanh is +-1 in FP32
+        float a = fminf(fmaxf(mlp[2 * lane] * x0 + mlp[2 * lane + 1] * x1 + mlp[2 * kMlpHidden + lane],
+                              -15.0f), 15.0f);
+        float h = 1.0f - 2.0f * __builtin_amdgcn_rcpf(fast_exp2(2.0f * kLog2e * a) + 1.0f);
+        v = mlp[3 * kMlpHidden + lane] * h;
+    }
+#pragma unroll
+    for (int m = kMlpHidden / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+    const float z = fminf(fmaxf(v + mlp[4 * kMlpHidden], -30.0f), 30.0f);
+    return __builtin_amdgcn_rcpf(1.0f + fast_exp2(-kLog2e * z));
+}
+
+// One decoding phase of a trial on the workgroup: flooding iterations from
+// check messages 0 until both syndromes match or kmax iterations have run.
+// WEIGHTED applies the learned weights and damping of iteration k+1 (the
+// training kernels' decoder, in base 2).  Returns the iteration index at
+// which it stopped; `matched` says whether that estimate satisfied both
+// syndromes; estx / estz hold its hard decisions.  DAMPED (with WEIGHTED)
+// adds the MLP damping, which keeps the previous check messages live across
+// each check pass; without it the weighted phase has the register footprint
+// of the plain one.
+template <int P, int J, int L, bool WEIGHTED, bool DAMPED>
+__device__ __forceinline__ int qc_phase(float *B, float *GZ, int *s_cnt, float *s_d, float *s_w,
+                                        uint32_t (&xw)[2][J][(L + 2) / 3], float (&lam)[2][J][L],
+                                        uint32_t synx, uint32_t synz, const PriorF &pr, int kmax,
+                                        int warmup, int interval, const LearnedInfer &li, float u0,
+                                        uint32_t &estx, uint32_t &estz, int &matched) {
+    constexpr int W = (L + 2) / 3;
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    // Make the packed indices opaque to the optimiser at chosen points, so
+    // that unpacked addresses are recomputed instead of being kept live
+    // (and spilled) across the check passes or the iteration loop.
+    auto launder = [&]() {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int r = 0; r < J; ++r)
+#pragma unroll
+                for (int w = 0; w < W; ++w) asm volatile("" : "+v"(xw[s][r][w]));
+    };
+    auto slot = [&](int s, int r, int cc) -> int {
+        return (r * L + cc) * P + static_cast<int>((xw[s][r][cc / 3] >> (10 * (cc % 3))) & 1023u);
+    };
+    // Iteration-major weights (for iteration k: wc X | wc Z | wv X | wv Z, J*L
+    // each) are staged in an LDS double buffer s_w[2][4*J*L]: iteration k reads
+    // half k&1 while the next iteration's weights are loaded into the other.
+    constexpr int NW = 4 * J * L;
+    auto stage = [&](int kk) {
+        if (t < NW && kk < kmax) s_w[(kk & 1) * NW + t] = li.params[kk * NW + t];
+    };
+    if (WEIGHTED) stage(0);
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int r = 0; r < J; ++r)
+#pragma unroll
+            for (int cc = 0; cc < L; ++cc) lam[s][r][cc] = 0.0f;
+    float ukm1 = u0;  // unsatisfied fraction of est_{k-1} (est_0 for k <= 1)
+    int k = 0;
+    __syncthreads();
+    for (;;) {
+        launder();
+        const bool live = !WEIGHTED || k < kmax;  // weights exist for iteration k+1
+        const float *wk = s_w + (k & 1) * NW;
+        if (WEIGHTED) stage(k + 1);
+        // Variable pass k: estimate, extrinsic X messages (+ est bit) to B.
+        estx = 0;
+        estz = 0;
+#pragma unroll
+        for (int cc = 0; cc < L; ++cc) {
+            float lx = 0.0f, lz = 0.0f;
+#pragma unroll
+            for (int r = 0; r < J; ++r) {
+                lx += lam[0][r][cc];
+                lz += lam[1][r][cc];
+            }
+            float l0 = pr.lp[0] + lx + lz, l1 = pr.lp[1] + lz, l2 = pr.lp[2] + lx, l3 = pr.lp[3];
+            int best = 0;
+            float bv = l0;
+            if (l1 > bv) { best = 1; bv = l1; }
+            if (l2 > bv) { best = 2; bv = l2; }
+            if (l3 > bv) { best = 3; bv = l3; }
+            estx |= static_cast<uint32_t>(best & 1) << cc;
+            estz |= static_cast<uint32_t>(best >> 1) << cc;
+            float gx = lx + extrinsic_term(lz, pr.fx);
+            GZ[cc * P + t] = lz + extrinsic_term(lx, pr.fz);
+            uint32_t bx = static_cast<uint32_t>(best & 1);
+#pragma unroll
+            for (int r = 0; r < J; ++r) {
+                float mu = gx - lam[0][r][cc];
+                if (WEIGHTED && live) mu *= wk[2 * J * L + r * L + cc];
+                B[slot(0, r, cc)] = with_lsb(mu, bx);
+            }
+        }
+        if (WEIGHTED && t < 64) {
+            float d = 0.0f;
+            if (DAMPED && live)
+                d = mlp_damping_wave(li.params + 4 * J * L * li.T,
+                                     static_cast<float>(k + 1) / static_cast<float>(li.T),
+                                     ukm1, lane);
+            if (t == 0) {
+                *s_cnt = 0;
+                *s_d = d;
+            }
+        }
+        // One register per mask: without this the per-column bits are
+        // forwarded to their later uses individually and spill.
+        asm volatile("" : "+v"(estx), "+v"(estz));
+        __syncthreads();
+        // X check pass: rows (r, t).
+        uint32_t bad = 0;
+#pragma unroll 1
+        for (int r = 0; r < J; ++r) {
+            uint32_t sb = (synx >> r) & 1u;
+            bad |= (check_row<L, P>(B + r * L * P + t, sb) ^ sb) << r;
+        }
+        if (WEIGHTED) {
+            uint32_t c = 0;
+#pragma unroll
+            for (int r = 0; r < J; ++r) c += __popcll(__ballot((bad >> r) & 1u));
+            if (lane == 0 && c) atomicAdd(s_cnt, static_cast<int>(c));
+        }
+        const int badx = __syncthreads_or(static_cast<int>(bad));
+        launder();
+        // learned: lambda = (1 - d) * wc * CN + d * lambda(t-1); CN was clamped
+        // in check_row, before the weight, as in training
+        const float dmp = DAMPED ? *s_d : 0.0f;
+#pragma unroll
+        for (int r = 0; r < J; ++r)
+#pragma unroll
+            for (int cc = 0; cc < L; ++cc) {
+                float v = B[slot(0, r, cc)];
+                if (WEIGHTED && live) v *= wk[r * L + cc];
+                lam[0][r][cc] = DAMPED ? (1.0f - dmp) * v + dmp * lam[0][r][cc] : v;
+            }
+            // damping keeps the old messages live: bound the loads in flight
+            // to one row so they do not spill
+            if (DAMPED) __builtin_amdgcn_sched_barrier(0);
+        __syncthreads();
+#pragma unroll
+        for (int cc = 0; cc < L; ++cc) {
+            uint32_t bz = (estz >> cc) & 1u;
+            const float gz = GZ[cc * P + t];
+#pragma unroll
+            for (int r = 0; r < J; ++r) {
+                float mu = gz - lam[1][r][cc];
+                if (WEIGHTED && live) mu *= wk[3 * J * L + r * L + cc];
+                B[slot(1, r, cc)] = with_lsb(mu, bz);
+            }
+        }
+        __syncthreads();
+        bad = 0;
+#pragma unroll 1
+        for (int r = 0; r < J; ++r) {
+            uint32_t sb = (synz >> r) & 1u;
+            bad |= (check_row<L, P>(B + r * L * P + t, sb) ^ sb) << r;
+        }
+        if (WEIGHTED) {
+            uint32_t c = 0;
+#pragma unroll
+            for (int r = 0; r < J; ++r) c += __popcll(__ballot((bad >> r) & 1u));
+            if (lane == 0 && c) atomicAdd(s_cnt, static_cast<int>(c));
+        }
+        const int badz = __syncthreads_or(static_cast<int>(bad));
+        if (k >= 1) {
+            bool test = WEIGHTED || (k == kmax) || (k > warmup && ((k - warmup) % interval) == 0);
+            if (test && !badx && !badz) {
+                matched = 1;
+                return k;
+            }
+        }
+        if (k >= kmax) {
+            matched = 0;
+            return k;
+        }
+        if (WEIGHTED) ukm1 = static_cast<float>(*s_cnt) / static_cast<float>(2 * J * P);
+        launder();
+#pragma unroll
+        for (int r = 0; r < J; ++r)
+#pragma unroll
+            for (int cc = 0; cc < L; ++cc) {
+                float v = B[slot(1, r, cc)];
+                if (WEIGHTED && live) v *= wk[J * L + r * L + cc];
+                lam[1][r][cc] = DAMPED ? (1.0f - dmp) * v + dmp * lam[1][r][cc] : v;
+            }
+            // damping keeps the old messages live: bound the loads in flight
+            // to one row so they do not spill
+            if (DAMPED) __builtin_amdgcn_sched_barrier(0);
+        ++k;
+        __syncthreads();
+    }
+}
+
+// LEARNED: the first li.T iterations use the learned weights and damping; a
+// trial that has not converged by then restarts plain BP from the channel
+// prior for up to max_iter iterations, and its iteration count is li.T plus
+// the plain iterations.
+template <int P, int J, int L, bool LEARNED, bool DAMPED>
 __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, TrialPlanes tp,
                                                          int count, PriorF pr, int max_iter,
-                                                         int warmup, int interval, int *counter) {
+                                                         int warmup, int interval, int *counter,
+                                                         LearnedInfer li) {
     static_assert(P % 64 == 0 && P <= 1024, "one thread per block row, whole wavefronts");
-    static_assert(P <= 1024, "10-bit row indices");
     extern __shared__ float B[];  // [J][L][P] exchange buffer, then GZ[L][P]
     float *GZ = B + J * L * P;   // Z-side extrinsic base per variable, parked between passes
     __shared__ int s_trial;
+    __shared__ int s_cnt;        // learned: unsatisfied checks of the estimate under test
+    __shared__ float s_d;        // learned: damping of the current iteration
+    __shared__ float s_w[LEARNED ? 2 * 4 * J * L : 1];  // learned: weights, double buffer
     const int t = threadIdx.x;
     const int lane = t & 63;
     constexpr int W = (L + 2) / 3;  // three 10-bit row indices per register
@@ -151,20 +360,6 @@ __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, 
                 uint32_t x = static_cast<uint32_t>((qc.ainv[idx] * d) % P);
                 xw[s][r][cc / 3] |= x << (10 * (cc % 3));
             }
-    // Make the packed indices opaque to the optimiser at chosen points, so
-    // that unpacked addresses are recomputed instead of being kept live
-    // (and spilled) across the check passes or the iteration loop.
-    auto launder = [&]() {
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int r = 0; r < J; ++r)
-#pragma unroll
-                for (int w = 0; w < W; ++w) asm volatile("" : "+v"(xw[s][r][w]));
-    };
-    auto slot = [&](int s, int r, int cc) -> int {
-        return (r * L + cc) * P + static_cast<int>((xw[s][r][cc / 3] >> (10 * (cc % 3))) & 1023u);
-    };
 
     for (;;) {
         if (t == 0) s_trial = atomicAdd(counter, 1);
@@ -180,98 +375,33 @@ __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, 
             synx |= ((tp.syn_x[static_cast<size_t>(trial) * g.wm[0] + (c >> 5)] >> (c & 31)) & 1u) << r;
             synz |= ((tp.syn_z[static_cast<size_t>(trial) * g.wm[1] + (c >> 5)] >> (c & 31)) & 1u) << r;
         }
-
         float lam[2][J][L];
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int r = 0; r < J; ++r)
-#pragma unroll
-                for (int cc = 0; cc < L; ++cc) lam[s][r][cc] = 0.0f;
-
-        int k = 0;
-        int iterations = max_iter;
-        int matched = 0;
         uint32_t estx = 0, estz = 0;  // bit cc: hard decision of variable (cc, t)
-        for (;;) {
-            launder();
-            // Variable pass k: estimate, extrinsic X messages (+ est bit) to B.
-            estx = 0;
-            estz = 0;
-#pragma unroll
-            for (int cc = 0; cc < L; ++cc) {
-                float lx = 0.0f, lz = 0.0f;
-#pragma unroll
-                for (int r = 0; r < J; ++r) {
-                    lx += lam[0][r][cc];
-                    lz += lam[1][r][cc];
-                }
-                float l0 = pr.lp[0] + lx + lz, l1 = pr.lp[1] + lz, l2 = pr.lp[2] + lx, l3 = pr.lp[3];
-                int best = 0;
-                float bv = l0;
-                if (l1 > bv) { best = 1; bv = l1; }
-                if (l2 > bv) { best = 2; bv = l2; }
-                if (l3 > bv) { best = 3; bv = l3; }
-                estx |= static_cast<uint32_t>(best & 1) << cc;
-                estz |= static_cast<uint32_t>(best >> 1) << cc;
-                float gx = lx + extrinsic_term(lz, pr.fx);
-                GZ[cc * P + t] = lz + extrinsic_term(lx, pr.fz);
-                uint32_t bx = static_cast<uint32_t>(best & 1);
-#pragma unroll
-                for (int r = 0; r < J; ++r) B[slot(0, r, cc)] = with_lsb(gx - lam[0][r][cc], bx);
-            }
-            // One register per mask: without this the per-column bits are
-            // forwarded to their later uses individually and spill.
-            asm volatile("" : "+v"(estx), "+v"(estz));
+        int matched = 0, iterations = 0;
+        if (LEARNED) {
+            // unsatisfied fraction of est_0 (all identity): the syndrome weight
+            if (t == 0) s_cnt = 0;
             __syncthreads();
-            // X check pass: rows (r, t).
-            uint32_t bad = 0;
-#pragma unroll 1
-            for (int r = 0; r < J; ++r) {
-                uint32_t sb = (synx >> r) & 1u;
-                bad |= check_row<L, P>(B + r * L * P + t, sb) ^ sb;
-            }
-            const int badx = __syncthreads_or(static_cast<int>(bad));
-            launder();
-#pragma unroll
-            for (int r = 0; r < J; ++r)
-#pragma unroll
-                for (int cc = 0; cc < L; ++cc) lam[0][r][cc] = B[slot(0, r, cc)];
+            atomicAdd(&s_cnt, __popc(synx) + __popc(synz));
             __syncthreads();
-#pragma unroll
-            for (int cc = 0; cc < L; ++cc) {
-                uint32_t bz = (estz >> cc) & 1u;
-                const float gz = GZ[cc * P + t];
-#pragma unroll
-                for (int r = 0; r < J; ++r) B[slot(1, r, cc)] = with_lsb(gz - lam[1][r][cc], bz);
+            const float u0 = static_cast<float>(s_cnt) / static_cast<float>(2 * J * P);
+            int k = qc_phase<P, J, L, true, DAMPED>(B, GZ, &s_cnt, &s_d, s_w, xw, lam, synx, synz, pr, li.T, 0,
+                                            1, li, u0, estx, estz, matched);
+            if (matched) {
+                iterations = k;
+            } else {
+                k = qc_phase<P, J, L, false, false>(B, GZ, &s_cnt, &s_d, s_w, xw, lam, synx, synz, pr,
+                                             max_iter, warmup, interval, li, 0.0f, estx, estz,
+                                             matched);
+                iterations = li.T + k;
             }
-            __syncthreads();
-            bad = 0;
-#pragma unroll 1
-            for (int r = 0; r < J; ++r) {
-                uint32_t sb = (synz >> r) & 1u;
-                bad |= check_row<L, P>(B + r * L * P + t, sb) ^ sb;
-            }
-            const int badz = __syncthreads_or(static_cast<int>(bad));
-            if (k >= 1) {
-                bool test = (k == max_iter) || (k > warmup && ((k - warmup) % interval) == 0);
-                if (test && !badx && !badz) {
-                    iterations = k;
-                    matched = 1;
-                    break;
-                }
-            }
-            if (k >= max_iter) break;
-            launder();
-#pragma unroll
-            for (int r = 0; r < J; ++r)
-#pragma unroll
-                for (int cc = 0; cc < L; ++cc) lam[1][r][cc] = B[slot(1, r, cc)];
-            ++k;
-            __syncthreads();
+        } else {
+            iterations = qc_phase<P, J, L, false, false>(B, GZ, &s_cnt, &s_d, s_w, xw, lam, synx, synz, pr,
+                                                  max_iter, warmup, interval, li, 0.0f, estx,
+                                                  estz, matched);
         }
 
-        // est_k bit planes: variable (cc, t) is bit cc*P + t; P % 64 == 0
+        // est bit planes: variable (cc, t) is bit cc*P + t; P % 64 == 0
         // makes each wavefront's 64 variables one aligned 64-bit span.
         uint32_t *ox = tp.est_x + static_cast<size_t>(trial) * g.wn;
         uint32_t *oz = tp.est_z + static_cast<size_t>(trial) * g.wn;
@@ -315,7 +445,8 @@ bool qc_f32_supported(const QcParams &qc, int device, char *why, size_t len) {
 
 void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_tables,
                           TrialPlanes tp, int count, Prior prior, int max_iter, int check_warmup,
-                          int check_interval, int slots, int *counter, hipStream_t st) {
+                          int check_interval, int slots, int *counter, hipStream_t st,
+                          const LearnedInfer *learned) {
     if (count <= 0) return;
     PriorF pf;
     const double *p = prior.p;
@@ -334,11 +465,20 @@ void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_ta
     int grid = slots < count ? slots : count;
     int interval = check_interval > 0 ? check_interval : 1;
     int warmup = check_warmup > 0 ? check_warmup : 0;
-    auto kern = decode_qc_f32_kernel<768, 3, 12>;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(768), lds, st, g, dev, tp, count, pf, max_iter,
-                       warmup, interval, counter);
+    LearnedInfer li;
+    if (learned) li = *learned;
+    auto launch = [&](auto kern) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(768), lds, st, g, dev, tp, count, pf, max_iter,
+                           warmup, interval, counter, li);
+    };
+    if (learned && learned->T > 0) {
+        if (learned->use_mlp) launch(decode_qc_f32_kernel<768, 3, 12, true, true>);
+        else launch(decode_qc_f32_kernel<768, 3, 12, true, false>);
+    } else {
+        launch(decode_qc_f32_kernel<768, 3, 12, false, false>);
+    }
 }
 
 }  // namespace jbp

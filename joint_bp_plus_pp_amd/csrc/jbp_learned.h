@@ -1,0 +1,53 @@
+// Learned-BP model description shared by the training kernels
+// (jbp_learned.hip), the inference decoder (jbp_decode_f32.hip) and the host
+// orchestration (jbp_api.hip).
+//
+// Parameter vector layout (one array of 4*nb*T + 65 values):
+//   wc[2][nb][T]   check-to-variable weights per side, block, iteration
+//   wv[2][nb][T]   variable-to-check weights
+//   mlp[65]        damping MLP: W1[16][2] | b1[16] | W2[16] | b2
+// For a permutation-block code the block of the edge at position cc of
+// check r*P + x is r*L + cc (nb = J*L).
+#pragma once
+
+#include "jbp_common.h"
+
+namespace jbp {
+
+constexpr int kMlpHidden = 16;
+constexpr int kMlpParams = 4 * kMlpHidden + 1;  // 65
+constexpr int kMaxLearnedT = 64;
+constexpr int kMaxLearnedBlocks = 64;
+
+struct LearnedDevParams {
+    int nb = 0, T = 0, use_mlp = 1;
+    const int *blk0 = nullptr;  // block of each X edge (device)
+    const int *blk1 = nullptr;  // block of each Z edge (device)
+};
+
+inline int learned_param_count(int nb, int T) { return 4 * nb * T + kMlpParams; }
+
+size_t learned_slot_elems(const DevGraph &g);
+
+template <typename R>
+void launch_learned_train(const DevGraph &g, const LearnedDevParams &lp, const R *params,
+                          TrialPlanes tp, int batch, R *act, R *scratch, int slots, R *tape_d,
+                          R *tape_u, const double *prior, R *loss, R *grad, int *conv,
+                          int *counters, bool backward, hipStream_t st);
+
+void launch_adam(double *p, const double *grad, double *m, double *v, float *pf, int count,
+                 double lr, int step, hipStream_t st);
+void launch_to_double(const float *src, double *dst, int count, hipStream_t st);
+
+// Learned inference on the LDS-resident decoder: T learned iterations with
+// the float parameter vector, then plain BP restarted from the channel
+// prior for up to max_iter iterations on trials that did not converge.
+struct LearnedInfer {
+    // device, learned_param_count(nb, T) floats, iteration-major: for each
+    // iteration t, wc X[nb] | wc Z[nb] | wv X[nb] | wv Z[nb]; then mlp[65]
+    const float *params = nullptr;
+    int T = 0;
+    int use_mlp = 1;
+};
+
+}  // namespace jbp

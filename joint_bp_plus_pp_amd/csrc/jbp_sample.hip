@@ -60,20 +60,50 @@ __device__ __forceinline__ void or_ballot(uint32_t *plane, int words, int q0, ui
     if (sh && w + 2 < words) plane[w + 2] |= static_cast<uint32_t>(m >> (64 - sh));
 }
 
-// State of draw d: I when d < t[0], X when d < t[1], Z when d < t[2], else Y.
-struct DrawThresholds {
-    uint64_t t[3];
-    uint32_t all_below;  // bit k: every draw is below threshold k
-};
+// Syndromes (jointbp_ets.cpp:1834-1849) of one trial from its error bit
+// planes, written as bit planes; one wavefront, 64 checks per ballot.
+__device__ void write_syndromes(const DevGraph &g, TrialPlanes &tp, int trial, const uint32_t *ex,
+                                const uint32_t *ez) {
+    const int lane = threadIdx.x & 63;
+    for (int s = 0; s < 2; ++s) {
+        const DevSide &sd = g.side[s];
+        const uint32_t *bits = s == 0 ? ex : ez;
+        uint32_t *out = (s == 0 ? tp.syn_x : tp.syn_z) + static_cast<size_t>(trial) * g.wm[s];
+        for (int c0 = 0; c0 < sd.m; c0 += 64) {
+            int c = c0 + lane;
+            int par = 0;
+            if (c < sd.m) {
+                for (int e = sd.chk_off[c]; e < sd.chk_off[c + 1]; ++e) {
+                    int v = sd.edge_var[e];
+                    par ^= (bits[v >> 5] >> (v & 31)) & 1;
+                }
+            }
+            uint64_t m = __ballot(par != 0);
+            if (lane == 0) {
+                out[c0 >> 5] = static_cast<uint32_t>(m);
+                if ((c0 >> 5) + 1 < g.wm[s]) out[(c0 >> 5) + 1] = static_cast<uint32_t>(m >> 32);
+            }
+        }
+    }
+}
+
+// Syndromes of caller-supplied error planes; grid = count, block = 64.
+__global__ void __launch_bounds__(64) syndrome_kernel(DevGraph g, TrialPlanes tp) {
+    const int trial = blockIdx.x;
+    write_syndromes(g, tp, trial, tp.err_x + static_cast<size_t>(trial) * g.wn,
+                    tp.err_z + static_cast<size_t>(trial) * g.wn);
+}
 
 // grid = count trials, block = 64 (one wavefront), dynamic LDS = 2*wn words.
 __global__ void __launch_bounds__(64) sample_kernel(DevGraph g, TrialPlanes tp, uint64_t base_seed,
-                                                    int64_t first_trial, DrawThresholds th) {
+                                                    int64_t first_trial, DrawThresholds th_all,
+                                                    const DrawThresholds *th_trial) {
     __shared__ uint64_t mt[kMtN];
     extern __shared__ uint32_t planes[];  // err_x | err_z
     const int lane = threadIdx.x;
     const int trial = blockIdx.x;
     const int n = g.n, wn = g.wn;
+    const DrawThresholds th = th_trial ? th_trial[trial] : th_all;
     uint32_t *ex = planes;
     uint32_t *ez = planes + wn;
     for (int w = lane; w < 2 * wn; w += 64) planes[w] = 0u;
@@ -144,27 +174,7 @@ __global__ void __launch_bounds__(64) sample_kernel(DevGraph g, TrialPlanes tp, 
         __syncthreads();
     }
 
-    // Syndromes (jointbp_ets.cpp:1834-1849): parity of the neighbours' bits.
-    for (int s = 0; s < 2; ++s) {
-        const DevSide &sd = g.side[s];
-        const uint32_t *bits = s == 0 ? ex : ez;
-        uint32_t *out = (s == 0 ? tp.syn_x : tp.syn_z) + static_cast<size_t>(trial) * g.wm[s];
-        for (int c0 = 0; c0 < sd.m; c0 += 64) {
-            int c = c0 + lane;
-            int par = 0;
-            if (c < sd.m) {
-                for (int e = sd.chk_off[c]; e < sd.chk_off[c + 1]; ++e) {
-                    int v = sd.edge_var[e];
-                    par ^= (bits[v >> 5] >> (v & 31)) & 1;
-                }
-            }
-            uint64_t m = __ballot(par != 0);
-            if (lane == 0) {
-                out[c0 >> 5] = static_cast<uint32_t>(m);
-                if ((c0 >> 5) + 1 < g.wm[s]) out[(c0 >> 5) + 1] = static_cast<uint32_t>(m >> 32);
-            }
-        }
-    }
+    write_syndromes(g, tp, trial, ex, ez);
     uint32_t *gx = tp.err_x + static_cast<size_t>(trial) * wn;
     uint32_t *gz = tp.err_z + static_cast<size_t>(trial) * wn;
     for (int w = lane; w < wn; w += 64) {
@@ -304,9 +314,7 @@ bool first_draw_above(int k, double p, uint64_t &out) {
 
 }  // namespace
 
-void launch_sample(const DevGraph &g, TrialPlanes tp, uint64_t base_seed, int64_t first_trial,
-                   int count, double p, hipStream_t st) {
-    if (count <= 0) return;
+DrawThresholds draw_thresholds(double p) {
     DrawThresholds th;
     th.all_below = 0;
     for (int k = 0; k < 3; ++k) {
@@ -315,9 +323,21 @@ void launch_sample(const DevGraph &g, TrialPlanes tp, uint64_t base_seed, int64_
             th.all_below |= 1u << k;
         }
     }
+    return th;
+}
+
+void launch_sample(const DevGraph &g, TrialPlanes tp, uint64_t base_seed, int64_t first_trial,
+                   int count, double p, const DrawThresholds *d_per_trial, hipStream_t st) {
+    if (count <= 0) return;
+    DrawThresholds th = draw_thresholds(p);
     size_t lds = static_cast<size_t>(2) * g.wn * sizeof(uint32_t);
     hipLaunchKernelGGL(sample_kernel, dim3(count), dim3(64), lds, st, g, tp, base_seed,
-                       first_trial, th);
+                       first_trial, th, d_per_trial);
+}
+
+void launch_syndromes(const DevGraph &g, TrialPlanes tp, int count, hipStream_t st) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(syndrome_kernel, dim3(count), dim3(64), 0, st, g, tp);
 }
 
 void launch_residual(const DevGraph &g, DevBasis bx, DevBasis bz, TrialPlanes tp, int count,

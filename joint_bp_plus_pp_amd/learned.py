@@ -1,0 +1,65 @@
+"""Training driver for the learned (neural-weighted) joint BP decoder.
+
+The forward and adjoint passes, the Adam update and the sampling of the
+training syndromes all run on the device (libjbp: jbp_learned_train_step);
+this module only sequences the steps, evaluates held-out batches, and stores
+parameters as .npz files (plain arrays, loaded with allow_pickle=False).
+
+No counterpart in the reference program: its only learned-ish knob is the
+fixed --damping (jointbp_ets.cpp:1903-1911).
+"""
+import os
+import time
+
+import numpy as np
+
+from . import FP32, LearnedModel
+
+
+def train(decoder, T=25, steps=200, batch=4096, lr=1e-2, p_lo=0.03, p_hi=0.05, seed=2024,
+          first_trial=1, log_every=20, lr_decay=0.0, log=None):
+    """Train a LearnedModel on device-sampled trials (p uniform in
+    [p_lo, p_hi] per trial, consecutive trial indices per step).  Returns
+    (model, history) with one (step, loss, seconds) entry per logged step."""
+    model = LearnedModel(decoder, T)
+    history = []
+    t0 = time.time()
+    nxt = first_trial
+    for step in range(1, steps + 1):
+        rate = lr / (1.0 + lr_decay * (step - 1))
+        loss = model.train_step(seed, nxt, batch, p_lo, p_hi, lr=rate)
+        nxt += batch
+        if step % log_every == 0 or step == 1 or step == steps:
+            history.append((step, loss, time.time() - t0))
+            if log:
+                log("train step %d loss %.6f lr %.2e (%.1f s)" % (step, loss, rate, time.time() - t0))
+    return model, history
+
+
+def save(model, path, meta=None):
+    d = {"params": model.get_params(), "T": np.array(model.T), "nb": np.array(model.nb)}
+    for k, v in (meta or {}).items():
+        d["meta_" + k] = np.asarray(v)
+    np.savez(path, **d)
+
+
+def load(decoder, path, use_mlp=True):
+    with np.load(path, allow_pickle=False) as z:
+        T = int(z["T"])
+        model = LearnedModel(decoder, T)
+        params = z["params"]
+        if params.size != model.count:
+            raise ValueError("%s holds %d parameters, the code needs %d" % (path, params.size,
+                                                                            model.count))
+        model.set_params(params, use_mlp=use_mlp)
+    return model
+
+
+def evaluate(decoder, model, base_seed, n_trials, p, max_iter=1000, first_trial=1):
+    """Learned decoder with plain fallback against plain BP on the same
+    trials: (learned stats, plain stats, per-trial dicts)."""
+    sl, rl = decoder.simulate_learned(model, base_seed, n_trials, p, max_iter=max_iter,
+                                      first_trial=first_trial)
+    sp, rp = decoder.simulate(base_seed, n_trials, p, max_iter=max_iter, first_trial=first_trial,
+                              precision=FP32)
+    return sl, sp, rl, rp

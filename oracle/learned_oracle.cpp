@@ -1,1 +1,420 @@
-// Learned-weight BP oracle (filled in with the learned decoder).
+// ORACLE — test infrastructure, not product code (see oracle.cpp's header).
+//
+// FP64 reference for the learned (neural-weighted) joint BP.  This decoder
+// has no counterpart in the reference program; it extends the reference's
+// joint BP (jointbp_ets.cpp:2373-2519) with
+//   * a weight per (side, permutation block, iteration) on every
+//     variable-to-check message:   mu = wv[g][blk][t] * (S_g - lambda + F_g(S_other))
+//   * a weight per (side, block, iteration) on every check-to-variable
+//     message:                     lambda~ = wc[g][blk][t] * CN(mu)
+//   * damping of the check-to-variable messages set by a 2-16-1 MLP from
+//     (t / T, unsatisfied-check fraction):
+//                                  lambda_t = (1 - d_t) lambda~ + d_t lambda_{t-1}
+// in natural-log likelihood ratios.  With all weights 1 and damping 0 it is
+// the reference's joint BP in the log domain (pinned by
+// tests/test_learned_oracle.py against the plain oracle's hard decisions).
+//
+// Loss: mean over iterations 1..T, trials and the 2n per-qubit X/Z bits of
+// the binary cross-entropy of the marginal bit probabilities against the
+// sampled error.  The gradient is the hand-derived adjoint below; it is
+// pinned against central finite differences of orl_loss.
+//
+// Check-node derivative: written as the O(deg^3) product over all other
+// inputs, independent of the O(deg) recursion the device adjoint uses.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr int kHidden = 16;
+constexpr int kMlpParams = kHidden * 2 + kHidden + kHidden + 1;  // 65
+constexpr double kLlrMax = 64.0 * 0.69314718055994530942;     // 64 bits, in nats
+
+struct Graph {
+    int n = 0;
+    int m[2] = {0, 0};
+    int nb = 0;  // weight blocks per side
+    std::vector<int> chk_off[2], edge_var[2], var_off[2], var_edge[2], blk[2];
+};
+
+inline double sigmoid(double x) {
+    return x >= 0 ? 1.0 / (1.0 + std::exp(-x)) : std::exp(x) / (1.0 + std::exp(x));
+}
+inline double softplus(double x) { return x > 0 ? x + std::log1p(std::exp(-x)) : std::log1p(std::exp(x)); }
+
+// F(L) = ln((a e^L + b) / (c e^L + d)) and its derivative.
+inline double fterm(double L, const double k[4]) {
+    if (L >= 0) return std::log(k[0] + k[1] * std::exp(-L)) - std::log(k[2] + k[3] * std::exp(-L));
+    return std::log(k[0] * std::exp(L) + k[1]) - std::log(k[2] * std::exp(L) + k[3]);
+}
+inline double fterm_d(double L, const double k[4]) {
+    return sigmoid(L + std::log(k[0] / k[1])) - sigmoid(L + std::log(k[2] / k[3]));
+}
+
+struct Params {
+    const double *wc;   // [2][nb][T]
+    const double *wv;   // [2][nb][T]
+    const double *mlp;  // W1[16][2] | b1[16] | W2[16] | b2
+    bool use_mlp;
+};
+
+double mlp_forward(const double *mlp, double x0, double x1, double *h) {
+    double z = mlp[kHidden * 2 + kHidden + kHidden];
+    for (int i = 0; i < kHidden; ++i) {
+        h[i] = std::tanh(mlp[2 * i] * x0 + mlp[2 * i + 1] * x1 + mlp[2 * kHidden + i]);
+        z += mlp[3 * kHidden + i] * h[i];
+    }
+    return sigmoid(z);
+}
+
+struct TrialTape {
+    std::vector<std::vector<double>> lam[2];   // [t][e], t = 0..T
+    std::vector<double> d, x0, x1;             // per iteration 1..T (index t-1)
+    std::vector<int> est;                      // last estimate
+    int converged_at = 0;
+};
+
+// Check update in the pair form used by the device kernels (natural log):
+// raw lambda_e = sign * ln(PE/PO); returns the clamped value.
+void check_node(const double *mu, int deg, int syn, double *raw, double *out) {
+    std::vector<double> e(deg), pe(deg + 1), po(deg + 1);
+    unsigned negall = syn & 1u;
+    std::vector<unsigned> neg(deg);
+    for (int i = 0; i < deg; ++i) {
+        neg[i] = mu[i] < 0 ? 1u : 0u;
+        negall ^= neg[i];
+        e[i] = std::exp(-std::fabs(mu[i]));
+    }
+    pe[0] = 1.0;
+    po[0] = 0.0;
+    for (int i = 0; i < deg; ++i) {
+        pe[i + 1] = pe[i] + po[i] * e[i];
+        po[i + 1] = pe[i] * e[i] + po[i];
+    }
+    double se = 1.0, so = 0.0;
+    for (int i = deg - 1; i >= 0; --i) {
+        double PE = pe[i] * se + po[i] * so;
+        double PO = pe[i] * so + po[i] * se;
+        double mag = std::log(PE) - std::log(PO);
+        double sg = (negall ^ neg[i]) ? -1.0 : 1.0;
+        raw[i] = sg * mag;
+        out[i] = sg * std::min(mag, kLlrMax);
+        double ne = se + so * e[i], no = se * e[i] + so;
+        se = ne;
+        so = no;
+    }
+}
+
+// One trial forward; fills the tape; returns the trial's loss sum (not yet
+// averaged).  err: states 0..3; prior: 4-ary normalized.
+double trial_forward(const Graph &g, const Params &P, int T, const int *err, const double *prior,
+                     TrialTape &tape) {
+    const double fx[4] = {prior[0], prior[2], prior[1], prior[3]};
+    const double fz[4] = {prior[0], prior[1], prior[2], prior[3]};
+    const double lp[4] = {std::log(prior[0]), std::log(prior[1]), std::log(prior[2]), std::log(prior[3])};
+    std::vector<int> syn[2];
+    int synw = 0;
+    for (int s = 0; s < 2; ++s) {
+        syn[s].assign(g.m[s], 0);
+        for (int c = 0; c < g.m[s]; ++c) {
+            int par = 0;
+            for (int e = g.chk_off[s][c]; e < g.chk_off[s][c + 1]; ++e) {
+                int st = err[g.edge_var[s][e]];
+                par ^= s == 0 ? (st & 1) : (st >> 1);
+            }
+            syn[s][c] = par;
+            synw += par;
+        }
+    }
+    const double mtot = g.m[0] + g.m[1];
+    for (int s = 0; s < 2; ++s) tape.lam[s].assign(T + 1, std::vector<double>(g.edge_var[s].size(), 0.0));
+    tape.d.assign(T, 0.0);
+    tape.x0.assign(T, 0.0);
+    tape.x1.assign(T, 0.0);
+    std::vector<double> ubar(T + 1, 0.0);  // unsat fraction of est_t; est_0 = all I
+    ubar[0] = synw / mtot;
+    std::vector<double> S[2], mu[2], raw, out;
+    std::vector<int> est(g.n, 0);
+    double loss = 0.0;
+    tape.converged_at = 0;
+    for (int t = 1; t <= T; ++t) {
+        const std::vector<double> *lp_prev[2] = {&tape.lam[0][t - 1], &tape.lam[1][t - 1]};
+        for (int s = 0; s < 2; ++s) {
+            S[s].assign(g.n, 0.0);
+            for (int v = 0; v < g.n; ++v)
+                for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) S[s][v] += (*lp_prev[s])[g.var_edge[s][k]];
+        }
+        for (int s = 0; s < 2; ++s) {
+            mu[s].assign(g.edge_var[s].size(), 0.0);
+            for (int v = 0; v < g.n; ++v) {
+                double f = s == 0 ? fterm(S[1][v], fx) : fterm(S[0][v], fz);
+                for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) {
+                    int e = g.var_edge[s][k];
+                    double a = S[s][v] - (*lp_prev[s])[e] + f;
+                    mu[s][e] = P.wv[(s * g.nb + g.blk[s][e]) * T + (t - 1)] * a;
+                }
+            }
+        }
+        const double uin = ubar[t >= 2 ? t - 2 : 0];
+        double h[kHidden];
+        const double x0 = static_cast<double>(t) / T;
+        double d = P.use_mlp ? mlp_forward(P.mlp, x0, uin, h) : 0.0;
+        tape.d[t - 1] = d;
+        tape.x0[t - 1] = x0;
+        tape.x1[t - 1] = uin;
+        for (int s = 0; s < 2; ++s) {
+            for (int c = 0; c < g.m[s]; ++c) {
+                int off = g.chk_off[s][c], deg = g.chk_off[s][c + 1] - off;
+                raw.assign(deg, 0.0);
+                out.assign(deg, 0.0);
+                check_node(&mu[s][off], deg, syn[s][c], raw.data(), out.data());
+                for (int i = 0; i < deg; ++i) {
+                    int e = off + i;
+                    double w = P.wc[(s * g.nb + g.blk[s][e]) * T + (t - 1)];
+                    tape.lam[s][t][e] = (1.0 - d) * w * out[i] + d * tape.lam[s][t - 1][e];
+                }
+            }
+        }
+        // marginals, loss, estimate
+        for (int s = 0; s < 2; ++s) {
+            S[s].assign(g.n, 0.0);
+            for (int v = 0; v < g.n; ++v)
+                for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) S[s][v] += tape.lam[s][t][g.var_edge[s][k]];
+        }
+        for (int v = 0; v < g.n; ++v) {
+            double lx = S[0][v] + fterm(S[1][v], fx);
+            double lz = S[1][v] + fterm(S[0][v], fz);
+            int yx = err[v] & 1, yz = err[v] >> 1;
+            loss += softplus((2 * yx - 1) * lx) + softplus((2 * yz - 1) * lz);
+            double l0 = lp[0] + S[0][v] + S[1][v], l1 = lp[1] + S[1][v], l2 = lp[2] + S[0][v], l3 = lp[3];
+            int best = 0;
+            double bv = l0;
+            if (l1 > bv) { best = 1; bv = l1; }
+            if (l2 > bv) { best = 2; bv = l2; }
+            if (l3 > bv) { best = 3; }
+            est[v] = best;
+        }
+        int unsat = 0;
+        for (int s = 0; s < 2; ++s)
+            for (int c = 0; c < g.m[s]; ++c) {
+                int par = 0;
+                for (int e = g.chk_off[s][c]; e < g.chk_off[s][c + 1]; ++e) {
+                    int st = est[g.edge_var[s][e]];
+                    par ^= s == 0 ? (st & 1) : (st >> 1);
+                }
+                unsat += par != syn[s][c];
+            }
+        ubar[t] = unsat / mtot;
+        if (unsat == 0 && tape.converged_at == 0) tape.converged_at = t;
+    }
+    tape.est = est;
+    return loss;
+}
+
+// Adjoint of trial_forward.  scale multiplies the trial's loss sum.
+void trial_backward(const Graph &g, const Params &P, int T, const int *err, const double *prior,
+                    const TrialTape &tape, double scale, double *gwc, double *gwv, double *gmlp) {
+    const double fx[4] = {prior[0], prior[2], prior[1], prior[3]};
+    const double fz[4] = {prior[0], prior[1], prior[2], prior[3]};
+    std::vector<int> syn[2];
+    for (int s = 0; s < 2; ++s) {
+        syn[s].assign(g.m[s], 0);
+        for (int c = 0; c < g.m[s]; ++c) {
+            int par = 0;
+            for (int e = g.chk_off[s][c]; e < g.chk_off[s][c + 1]; ++e) {
+                int st = err[g.edge_var[s][e]];
+                par ^= s == 0 ? (st & 1) : (st >> 1);
+            }
+            syn[s][c] = par;
+        }
+    }
+    std::vector<double> G[2], Gp[2], S[2], Sp[2], mu[2], a[2], raw[2], out[2], Gmu[2];
+    for (int s = 0; s < 2; ++s) G[s].assign(g.edge_var[s].size(), 0.0);
+    for (int t = T; t >= 1; --t) {
+        const auto &lam_t = tape.lam;
+        // loss term at t
+        for (int s = 0; s < 2; ++s) {
+            S[s].assign(g.n, 0.0);
+            for (int v = 0; v < g.n; ++v)
+                for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) S[s][v] += lam_t[s][t][g.var_edge[s][k]];
+        }
+        for (int v = 0; v < g.n; ++v) {
+            double lx = S[0][v] + fterm(S[1][v], fx);
+            double lz = S[1][v] + fterm(S[0][v], fz);
+            double kx = 2 * (err[v] & 1) - 1, kz = 2 * (err[v] >> 1) - 1;
+            double glx = scale * kx * sigmoid(kx * lx);
+            double glz = scale * kz * sigmoid(kz * lz);
+            double gsx = glx + glz * fterm_d(S[0][v], fz);
+            double gsz = glz + glx * fterm_d(S[1][v], fx);
+            for (int k = g.var_off[0][v]; k < g.var_off[0][v + 1]; ++k) G[0][g.var_edge[0][k]] += gsx;
+            for (int k = g.var_off[1][v]; k < g.var_off[1][v + 1]; ++k) G[1][g.var_edge[1][k]] += gsz;
+        }
+        // recompute the var pass and check pass of iteration t
+        for (int s = 0; s < 2; ++s) {
+            Sp[s].assign(g.n, 0.0);
+            for (int v = 0; v < g.n; ++v)
+                for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) Sp[s][v] += lam_t[s][t - 1][g.var_edge[s][k]];
+        }
+        for (int s = 0; s < 2; ++s) {
+            mu[s].assign(g.edge_var[s].size(), 0.0);
+            a[s].assign(g.edge_var[s].size(), 0.0);
+            for (int v = 0; v < g.n; ++v) {
+                double f = s == 0 ? fterm(Sp[1][v], fx) : fterm(Sp[0][v], fz);
+                for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) {
+                    int e = g.var_edge[s][k];
+                    a[s][e] = Sp[s][v] - lam_t[s][t - 1][e] + f;
+                    mu[s][e] = P.wv[(s * g.nb + g.blk[s][e]) * T + (t - 1)] * a[s][e];
+                }
+            }
+            raw[s].assign(g.edge_var[s].size(), 0.0);
+            out[s].assign(g.edge_var[s].size(), 0.0);
+            for (int c = 0; c < g.m[s]; ++c) {
+                int off = g.chk_off[s][c], deg = g.chk_off[s][c + 1] - off;
+                check_node(&mu[s][off], deg, syn[s][c], &raw[s][off], &out[s][off]);
+            }
+        }
+        const double d = tape.d[t - 1];
+        // damping and check weights
+        double gd = 0.0;
+        for (int s = 0; s < 2; ++s) {
+            Gp[s].assign(g.edge_var[s].size(), 0.0);
+            for (size_t e = 0; e < g.edge_var[s].size(); ++e) {
+                size_t wi = (s * g.nb + g.blk[s][e]) * T + (t - 1);
+                double w = P.wc[wi];
+                gd += G[s][e] * (lam_t[s][t - 1][e] - w * out[s][e]);
+                Gp[s][e] = d * G[s][e];
+                double gtil = (1.0 - d) * G[s][e];
+                gwc[wi] += gtil * out[s][e];
+                G[s][e] = std::fabs(raw[s][e]) > kLlrMax ? 0.0 : w * gtil;  // now d loss / d lambda-hat
+            }
+        }
+        if (P.use_mlp) {
+            double h[kHidden];
+            mlp_forward(P.mlp, tape.x0[t - 1], tape.x1[t - 1], h);
+            double gz = gd * d * (1.0 - d);
+            gmlp[2 * kHidden + kHidden + kHidden] += gz;
+            for (int i = 0; i < kHidden; ++i) {
+                gmlp[3 * kHidden + i] += gz * h[i];
+                double ga = gz * P.mlp[3 * kHidden + i] * (1.0 - h[i] * h[i]);
+                gmlp[2 * i] += ga * tape.x0[t - 1];
+                gmlp[2 * i + 1] += ga * tape.x1[t - 1];
+                gmlp[2 * kHidden + i] += ga;
+            }
+        }
+        // check-node adjoint: d lambda_e / d mu_j = sign_c * P_ej sech^2(mu_j/2) / sech^2(raw_e/2)
+        for (int s = 0; s < 2; ++s) {
+            Gmu[s].assign(g.edge_var[s].size(), 0.0);
+            for (int c = 0; c < g.m[s]; ++c) {
+                int off = g.chk_off[s][c], deg = g.chk_off[s][c + 1] - off;
+                double sgn = syn[s][c] ? -1.0 : 1.0;
+                for (int e = 0; e < deg; ++e) {
+                    double ge = G[s][off + e];
+                    if (ge == 0.0) continue;
+                    double ch = std::cosh(raw[s][off + e] / 2.0);
+                    for (int j = 0; j < deg; ++j) {
+                        if (j == e) continue;
+                        double prod = 1.0;
+                        for (int k = 0; k < deg; ++k)
+                            if (k != e && k != j) prod *= std::tanh(mu[s][off + k] / 2.0);
+                        double cj = std::cosh(mu[s][off + j] / 2.0);
+                        Gmu[s][off + j] += ge * sgn * prod * (ch * ch) / (cj * cj);
+                    }
+                }
+            }
+        }
+        // variable-pass adjoint
+        for (int s = 0; s < 2; ++s) {
+            const int o = 1 - s;
+            for (int v = 0; v < g.n; ++v) {
+                double ga_sum = 0.0;
+                for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) {
+                    int e = g.var_edge[s][k];
+                    size_t wi = (s * g.nb + g.blk[s][e]) * T + (t - 1);
+                    gwv[wi] += Gmu[s][e] * a[s][e];
+                    double ga = P.wv[wi] * Gmu[s][e];
+                    ga_sum += ga;
+                    Gp[s][e] -= ga;
+                }
+                for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) Gp[s][g.var_edge[s][k]] += ga_sum;
+                // F_s(S_o) term: d a / d S_o = F_s'(S_o)
+                double fd = s == 0 ? fterm_d(Sp[1][v], fx) : fterm_d(Sp[0][v], fz);
+                for (int k = g.var_off[o][v]; k < g.var_off[o][v + 1]; ++k) Gp[o][g.var_edge[o][k]] += ga_sum * fd;
+            }
+        }
+        G[0].swap(Gp[0]);
+        G[1].swap(Gp[1]);
+    }
+}
+
+Graph make_graph(int n, int mx, int mz, const int *xoff, const int *xvar, const int *zoff,
+                 const int *zvar, const int *xblk, const int *zblk, int nb) {
+    Graph g;
+    g.n = n;
+    g.m[0] = mx;
+    g.m[1] = mz;
+    g.nb = nb;
+    const int *off[2] = {xoff, zoff}, *var[2] = {xvar, zvar}, *blk[2] = {xblk, zblk};
+    for (int s = 0; s < 2; ++s) {
+        int E = off[s][g.m[s]];
+        g.chk_off[s].assign(off[s], off[s] + g.m[s] + 1);
+        g.edge_var[s].assign(var[s], var[s] + E);
+        g.blk[s].assign(blk[s], blk[s] + E);
+        std::vector<int> cnt(n + 1, 0);
+        for (int v : g.edge_var[s]) cnt[v + 1]++;
+        for (int v = 0; v < n; ++v) cnt[v + 1] += cnt[v];
+        g.var_off[s] = cnt;
+        g.var_edge[s].assign(E, 0);
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (int e = 0; e < E; ++e) g.var_edge[s][fill[g.edge_var[s][e]]++] = e;
+    }
+    return g;
+}
+
+void prior_of(double p, double *out) {
+    double m[4] = {1.0 - p, p / 3.0, p / 3.0, p / 3.0};
+    double s = m[0] + m[1] + m[2] + m[3];
+    for (int i = 0; i < 4; ++i) out[i] = m[i] / s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orl_mlp_params() { return kMlpParams; }
+
+// Mean loss over `batch` trials (errors batch x n, one p per trial) and,
+// when grad_* are non-null, its gradient.  Per trial, conv_out (may be
+// null) receives the first iteration whose estimate satisfies both
+// syndromes (0 = none) and est_out (may be null) the final estimate.
+double orl_loss(int n, int mx, int mz, const int *xoff, const int *xvar, const int *zoff,
+                const int *zvar, const int *xblk, const int *zblk, int nb, int T, const double *wc,
+                const double *wv, const double *mlp, int use_mlp, int batch, const int *err,
+                const double *p, double *grad_wc, double *grad_wv, double *grad_mlp, int *conv_out,
+                int *est_out) {
+    Graph g = make_graph(n, mx, mz, xoff, xvar, zoff, zvar, xblk, zblk, nb);
+    Params P{wc, wv, mlp, use_mlp != 0};
+    const double scale = 1.0 / (static_cast<double>(batch) * T * 2.0 * n);
+    if (grad_wc) std::memset(grad_wc, 0, sizeof(double) * 2 * nb * T);
+    if (grad_wv) std::memset(grad_wv, 0, sizeof(double) * 2 * nb * T);
+    if (grad_mlp) std::memset(grad_mlp, 0, sizeof(double) * kMlpParams);
+    std::vector<double> dummy_wc(2 * nb * T), dummy_wv(2 * nb * T), dummy_mlp(kMlpParams);
+    double total = 0.0;
+    for (int b = 0; b < batch; ++b) {
+        double prior[4];
+        prior_of(p[b], prior);
+        TrialTape tape;
+        total += trial_forward(g, P, T, err + static_cast<size_t>(b) * n, prior, tape);
+        if (conv_out) conv_out[b] = tape.converged_at;
+        if (est_out) std::copy(tape.est.begin(), tape.est.end(), est_out + static_cast<size_t>(b) * n);
+        if (grad_wc || grad_wv || grad_mlp)
+            trial_backward(g, P, T, err + static_cast<size_t>(b) * n, prior, tape, scale,
+                           grad_wc ? grad_wc : dummy_wc.data(), grad_wv ? grad_wv : dummy_wv.data(),
+                           grad_mlp ? grad_mlp : dummy_mlp.data());
+    }
+    return total * scale;
+}
+
+}  // extern "C"

@@ -135,3 +135,42 @@ class Oracle:
         out = np.zeros(4, np.float64)
         self.lib.orc_prior(p, _p(out))
         return out
+
+
+class LearnedOracle:
+    """FP64 learned-BP forward / loss / adjoint (oracle/learned_oracle.cpp)."""
+
+    def __init__(self, code, P=768, L=12, path=ORACLE_SO):
+        L_ = C.CDLL(path)
+        vp = C.c_void_p
+        L_.orl_loss.restype = C.c_double
+        L_.orl_loss.argtypes = [C.c_int] * 3 + [vp] * 6 + [C.c_int, C.c_int, vp, vp, vp, C.c_int,
+                                                           C.c_int, vp, vp, vp, vp, vp, vp, vp]
+        self.lib = L_
+        self.code = code
+        self.xoff, self.xvar = code.csr(0)
+        self.zoff, self.zvar = code.csr(1)
+        self.nb = (code.mx // P) * L
+        self.blk = []
+        for off in (self.xoff, self.zoff):
+            b = np.zeros(off[-1], np.int32)
+            for c in range(len(off) - 1):
+                r = c // P
+                b[off[c]:off[c + 1]] = r * L + np.arange(off[c + 1] - off[c])
+            self.blk.append(b)
+
+    def loss(self, T, wc, wv, mlp, use_mlp, err, p, grad=False):
+        err = np.ascontiguousarray(err, dtype=np.int32)
+        p = np.ascontiguousarray(np.broadcast_to(p, (err.shape[0],)), dtype=np.float64)
+        wc = np.ascontiguousarray(wc, dtype=np.float64)
+        wv = np.ascontiguousarray(wv, dtype=np.float64)
+        mlp = np.ascontiguousarray(mlp, dtype=np.float64)
+        g = [np.zeros_like(wc), np.zeros_like(wv), np.zeros_like(mlp)] if grad else [None] * 3
+        conv = np.zeros(err.shape[0], np.int32)
+        est = np.zeros_like(err)
+        c = self.code
+        val = self.lib.orl_loss(c.n, c.mx, c.mz, _p(self.xoff), _p(self.xvar), _p(self.zoff),
+                                _p(self.zvar), _p(self.blk[0]), _p(self.blk[1]), self.nb, T, _p(wc),
+                                _p(wv), _p(mlp), int(use_mlp), err.shape[0], _p(err), _p(p),
+                                _p(g[0]), _p(g[1]), _p(g[2]), _p(conv), _p(est))
+        return (val, g, conv, est) if grad else (val, conv, est)
