@@ -118,7 +118,21 @@ typedef struct jbp_sim_cfg {
     int block;           /* threads per block for the generic decoders; 0 = tuned */
     int64_t min_fails;   /* >0: stop at the first trial index where the
                             failure count reaches this (reference 6409-6411) */
+    const double *p_list; /* n_p > 0: trial index idx uses p_list[idx % n_p]
+                             instead of p (interleaved mixed-noise runs)     */
+    int n_p;              /* 0 = single p; at most JBP_MAX_P_LIST            */
+    int engine;           /* FP32 decoder: JBP_ENGINE_AUTO, _LDS or _HBM     */
 } jbp_sim_cfg;
+
+#define JBP_MAX_P_LIST 4096
+
+/* FP32 decoders.  LDS: one trial per compute unit with all messages on chip,
+ * for the 768 x (3 x 12) permutation-block codes.  HBM: any code; 64 trials
+ * interleaved per wavefront, messages in device memory (slots = trials
+ * resident, a multiple of 64).  AUTO picks LDS when it applies. */
+#define JBP_ENGINE_AUTO 0
+#define JBP_ENGINE_LDS 1
+#define JBP_ENGINE_HBM 2
 
 /* Per-trial outcome (structure of arrays; any pointer may be NULL). */
 typedef struct jbp_trial_out {
@@ -149,6 +163,18 @@ int jbp_simulate(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *per_trial,
  * and their syndromes (count x mX, count x mZ ints; either may be NULL). */
 int jbp_sample(jbp_ctx *ctx, uint64_t base_seed, int64_t first_trial, int count, double p,
                int *err_out, int *sx_out, int *sz_out, char *err, size_t errlen);
+
+/* Success test by logical operators instead of the row basis built at
+ * create (which is skipped for codes too large to eliminate densely on the
+ * host): side 0 rows span ker(Hz) modulo rowspace(Hx) and are checked
+ * against the X part of est ^ err, side 1 rows span ker(Hx) modulo
+ * rowspace(Hz) against the Z part; a syndrome-matching estimate succeeds
+ * iff every row has even overlap.  Row i = qubits[row_off[i] ..
+ * row_off[i+1]).  Used once both sides are set; side = -1 clears both.
+ * Replaces the in_row_space test of jointbp_ets.cpp:904-920 (membership in
+ * a row space is orthogonality to its dual, so both tests agree). */
+int jbp_set_logicals(jbp_ctx *ctx, int side, int64_t n_rows, const int64_t *row_off,
+                     const int *qubits, char *err, size_t errlen);
 
 /* Row-space membership of count bit vectors (count x nvars ints, nonzero = 1)
  * in the row space of Hx (side 0) or Hz (side 1): out[i] = 1 when vector i

@@ -43,6 +43,16 @@ int jbp_code_from_checks(int nvars, int mx, int xdeg, const int *xflat, int mz, 
 int jbp_code_hgp(int m, int n, int col_w, int row_w, uint64_t seed, jbp_code **out, char *err,
                  size_t errlen);
 
+/* Logical-operator representatives of a hypergraph-product code, for the
+ * success test without a row basis: side 0 spans ker(Hz) modulo
+ * rowspace(Hx) (a residual X part r with Hx r = 0 is in rowspace(Hz) iff it
+ * has even overlap with every one), side 1 spans ker(Hx) modulo
+ * rowspace(Hz).  Sparse rows: row i is qubits[row_off[i] .. row_off[i+1]).
+ * Call with row_off = qubits = NULL for the sizes.  JBP_ERR_UNSUPPORTED for
+ * codes not built by jbp_code_hgp. */
+int jbp_code_logicals(const jbp_code *code, int side, int64_t *n_rows, int64_t *n_entries,
+                      int64_t *row_off, int *qubits);
+
 void jbp_code_free(jbp_code *code);
 
 /* out: n, mX, mZ, x_edges, z_edges, P, J, L (P/J/L are 0 for codes not read

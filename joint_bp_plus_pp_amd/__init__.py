@@ -26,6 +26,8 @@ INCLUDE_DIR = os.path.join(os.path.dirname(PKG_DIR), "include")
 FP32 = 32
 FP64 = 64
 STATUS = {0: "ok", 1: "invalid argument", 2: "device error", 3: "unsupported", 4: "out of memory"}
+STATUS_UNSUPPORTED = 3
+ENGINE_AUTO, ENGINE_LDS, ENGINE_HBM = 0, 1, 2
 
 
 class JbpError(RuntimeError):
@@ -52,7 +54,8 @@ class SimCfg(C.Structure):
     _fields_ = [("base_seed", C.c_uint64), ("first_trial", C.c_int64), ("n_trials", C.c_int64),
                 ("p", C.c_double), ("max_iter", C.c_int), ("damping", C.c_double),
                 ("precision", C.c_int), ("check_warmup", C.c_int), ("check_interval", C.c_int),
-                ("slots", C.c_int), ("block", C.c_int), ("min_fails", C.c_int64)]
+                ("slots", C.c_int), ("block", C.c_int), ("min_fails", C.c_int64),
+                ("p_list", C.POINTER(C.c_double)), ("n_p", C.c_int), ("engine", C.c_int)]
 
 
 class TrialOut(C.Structure):
@@ -107,6 +110,8 @@ def lib():
         "jbp_code_params": (i32, [vp, vp]),
         "jbp_code_checks": (i32, [vp, i32, vp]),
         "jbp_code_diagnostics": (i32, [vp, i32, vp]),
+        "jbp_code_logicals": (i32, [vp, i32, C.POINTER(i64), C.POINTER(i64), vp, vp]),
+        "jbp_set_logicals": (i32, [vp, i32, i64, vp, vp, cp, sz]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -206,6 +211,22 @@ class Code:
         lib().jbp_code_params(self._h, _ptr(out))
         return out.reshape(4, -1)
 
+    def logicals(self, side):
+        """Logical-operator representatives (row_off int64, qubits int32) of
+        a hypergraph-product code, or None for other codes: side 0 spans
+        ker(Hz) / rowspace(Hx), side 1 spans ker(Hx) / rowspace(Hz)."""
+        rows, ent = C.c_int64(), C.c_int64()
+        rc = lib().jbp_code_logicals(self._h, side, C.byref(rows), C.byref(ent), None, None)
+        if rc == STATUS_UNSUPPORTED:
+            return None
+        msg = C.create_string_buffer(b"jbp_code_logicals failed")
+        _check(rc, msg)
+        off = np.zeros(rows.value + 1, np.int64)
+        q = np.zeros(max(ent.value, 1), np.int32)
+        _check(lib().jbp_code_logicals(self._h, side, C.byref(rows), C.byref(ent), _ptr(off), _ptr(q)),
+               msg)
+        return off, q[:ent.value]
+
     def diagnostics(self, girth=True):
         out = np.zeros(15, np.int64)
         lib().jbp_code_diagnostics(self._h, 1 if girth else 0, _ptr(out))
@@ -236,6 +257,23 @@ class Decoder:
         _check(lib().jbp_create(C.byref(self._graph), device, C.byref(h), e, 512), e)
         self._h = h
         self._models = weakref.WeakSet()  # learned models bound to this context
+        # hypergraph-product codes carry their logical operators: the
+        # success test uses them (the dense row basis is too large to build)
+        lx, lz = code.logicals(0), code.logicals(1)
+        if lx is not None and lz is not None:
+            self.set_logicals(lx, lz)
+
+    def set_logicals(self, lx, lz):
+        """Judge success by logical operators (row_off, qubits) per side
+        (jbp_set_logicals); None for both restores the row-basis test."""
+        e = _errbuf()
+        if lx is None and lz is None:
+            _check(lib().jbp_set_logicals(self._h, -1, 0, None, None, e, 512), e)
+            return
+        for side, (off, q) in enumerate((lx, lz)):
+            off = np.ascontiguousarray(off, np.int64)
+            q = np.ascontiguousarray(q, np.int32)
+            _check(lib().jbp_set_logicals(self._h, side, off.size - 1, _ptr(off), _ptr(q), e, 512), e)
 
     def destroy(self):
         """Free the context; learned models bound to it are freed first."""
@@ -327,11 +365,21 @@ class Decoder:
 
     def simulate(self, base_seed, n_trials, p, max_iter=50, first_trial=1, damping=0.0,
                  precision=FP32, check_warmup=0, check_interval=1, min_fails=0, slots=0, block=0,
-                 per_trial=True, want_err=False, want_est=False, _model=None):
+                 per_trial=True, want_err=False, want_est=False, engine=0, _model=None):
         """The --simulate --no-pp inner loop on the device.  Returns (stats,
-        per-trial dict or None)."""
+        per-trial dict or None).  p may be a sequence: trial index idx then
+        uses p[idx % len(p)] (interleaved mixed-noise run).  engine selects
+        the FP32 decoder (ENGINE_AUTO, ENGINE_LDS, ENGINE_HBM); slots is the
+        number of resident trials of the HBM engine (FP64: of the parity
+        decoder)."""
+        plist = None
+        if np.ndim(p) > 0:
+            plist = np.ascontiguousarray(p, dtype=np.float64)
+            p = float(plist[0]) if plist.size else 0.0
         cfg = SimCfg(base_seed, first_trial, n_trials, p, max_iter, damping, precision, check_warmup,
-                     check_interval, slots, block, min_fails)
+                     check_interval, slots, block, min_fails,
+                     plist.ctypes.data_as(C.POINTER(C.c_double)) if plist is not None else None,
+                     int(plist.size) if plist is not None else 0, engine)
         out = None
         arrays = {}
         if per_trial:

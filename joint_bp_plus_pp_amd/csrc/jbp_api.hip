@@ -265,33 +265,181 @@ Prior prior_from_p(double p) {
     return pr;
 }
 
+void free_noise(jbp_ctx *ctx) {
+    if (ctx->d_th) (void)hipFree(ctx->d_th);
+    if (ctx->d_prior) (void)hipFree(ctx->d_prior);
+    if (ctx->d_prior32) (void)hipFree(ctx->d_prior32);
+    ctx->d_th = nullptr;
+    ctx->d_prior = nullptr;
+    ctx->d_prior32 = nullptr;
+    ctx->noise_cap = 0;
+    ctx->noise = NoiseTab();
+    ctx->noise_key.clear();
+}
+
+int set_noise(jbp_ctx *ctx, const Prior *priors, const double *p, int n, char *err, size_t errlen) {
+    if (n <= 0) {
+        set_err(err, errlen, "empty channel table");
+        return JBP_ERR_ARG;
+    }
+    std::vector<double> key;
+    key.reserve(5 * static_cast<size_t>(n) + 1);
+    key.push_back(p ? 1.0 : 0.0);
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < 4; ++k) key.push_back(priors[i].p[k]);
+        key.push_back(p ? p[i] : 0.0);
+    }
+    if (key == ctx->noise_key) {
+        ctx->noise.phase = 0;
+        return JBP_OK;
+    }
+    if (n > ctx->noise_cap) {
+        free_noise(ctx);
+        HIPCHK(hipMalloc(&ctx->d_th, n * sizeof(DrawThresholds)));
+        HIPCHK(hipMalloc(&ctx->d_prior, n * 4 * sizeof(double)));
+        HIPCHK(hipMalloc(&ctx->d_prior32, n * kPrior32 * sizeof(float)));
+        ctx->noise_cap = n;
+    }
+    std::vector<DrawThresholds> th(n);
+    std::vector<double> pr(4 * static_cast<size_t>(n));
+    std::vector<float> p32(kPrior32 * static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) {
+        if (p) th[i] = draw_thresholds(p[i]);
+        else std::memset(&th[i], 0, sizeof(DrawThresholds));
+        for (int k = 0; k < 4; ++k) pr[4 * i + k] = priors[i].p[k];
+        prior32_from(priors[i], &p32[kPrior32 * i]);
+    }
+    // The previous call's kernels have finished (every entry point ends with
+    // a synchronization), so the tables can be overwritten.
+    HIPCHK(hipMemcpy(ctx->d_th, th.data(), n * sizeof(DrawThresholds), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_prior, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_prior32, p32.data(), p32.size() * sizeof(float), hipMemcpyHostToDevice));
+    ctx->noise.th = ctx->d_th;
+    ctx->noise.prior = ctx->d_prior;
+    ctx->noise.prior32 = ctx->d_prior32;
+    ctx->noise.n = n;
+    ctx->noise.phase = 0;
+    ctx->noise_key.swap(key);
+    return JBP_OK;
+}
+
+void free_logicals(jbp_ctx *ctx) {
+    for (int k = 0; k < 2; ++k) {
+        for (void *&p : ctx->lg_mem[k]) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+        }
+        ctx->lg[k] = DevLogicals();
+    }
+    ctx->have_logicals = false;
+}
+
+void free_gen(jbp_ctx *ctx) {
+    if (ctx->gen_mem) (void)hipFree(ctx->gen_mem);
+    ctx->gen_mem = nullptr;
+    ctx->gen_cap = 0;
+    ctx->gen = GenState();
+    if (ctx->h_ctrl) (void)hipHostFree(ctx->h_ctrl);
+    ctx->h_ctrl = nullptr;
+    for (auto &e : ctx->gen_ev)
+        if (e) (void)hipEventDestroy(e);
+    ctx->gen_ev[0] = ctx->gen_ev[1] = nullptr;
+}
+
+namespace {
+
+// Resident trials of the generic decoder: enough wavefronts to fill the
+// device several times over in the check pass, capped by memory.
+int default_gen_slots(const jbp_ctx *ctx) {
+    if (ctx->gen_slots > 0) return ctx->gen_slots;
+    const DevGraph &g = ctx->dg;
+    const int64_t waves_per_group = (g.side[0].m + g.side[1].m + 3) / 4;
+    int64_t groups = (64LL * std::max(1, ctx->cus) * 8 + waves_per_group - 1) / waves_per_group;
+    groups = std::max<int64_t>(groups, 8);
+    while (groups > 1 && gen_carve_bytes(g, static_cast<int>(groups)) > (size_t(8) << 30)) groups /= 2;
+    return static_cast<int>(std::min<int64_t>(groups, 256) * 64);
+}
+
+int run_gen(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmup, int interval,
+            int slots, char *err, size_t errlen) {
+    const DevGraph &g = ctx->dg;
+    if (!gen_supported(g)) {
+        set_err(err, errlen, "generic FP32 decoder supports check degree <= 32");
+        return JBP_ERR_UNSUPPORTED;
+    }
+    int S = slots > 0 ? slots : default_gen_slots(ctx);
+    S = std::min(S, ((count + 63) / 64) * 64);
+    const int groups = std::max(1, (S + 63) / 64);
+    const size_t bytes = gen_carve_bytes(g, groups);
+    if (bytes > ctx->gen_cap) {
+        if (ctx->gen_mem) (void)hipFree(ctx->gen_mem);
+        ctx->gen_mem = nullptr;
+        ctx->gen_cap = 0;
+        HIPCHK(hipMalloc(&ctx->gen_mem, bytes));
+        ctx->gen_cap = bytes;
+    }
+    if (!ctx->h_ctrl) HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ctx->h_ctrl), 2 * sizeof(int)));
+    for (auto &e : ctx->gen_ev)
+        if (!e) HIPCHK(hipEventCreate(&e));
+    gen_carve(g, groups, ctx->gen_mem, ctx->gen);
+    int64_t steps = 0;
+    const int rc = gen_decode(g, ctx->gen, ctx->tp, count, nt, max_iter, warmup, interval,
+                              ctx->h_ctrl, ctx->gen_ev, ctx->stream, &steps);
+    ctx->gen_steps = steps;
+    if (rc == -1) {
+        set_err(err, errlen, "generic decoder exceeded its step bound");
+        return JBP_ERR_DEVICE;
+    }
+    if (rc != 0) {
+        set_err(err, errlen, "generic decoder: %s", hipGetErrorString(static_cast<hipError_t>(rc)));
+        return JBP_ERR_DEVICE;
+    }
+    return JBP_OK;
+}
+
+}  // namespace
+
 // Decode `count` trials whose syndrome planes are already in ctx->tp.
-int run_decode(jbp_ctx *ctx, int count, const Prior &prior, int max_iter, int warmup, int interval,
+int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmup, int interval,
                double damping, int precision, char *err, size_t errlen,
-               const LearnedInfer *learned) {
+               const LearnedInfer *learned, int engine, int slots) {
     if (precision != JBP_FP32 && precision != JBP_FP64) {
         set_err(err, errlen, "precision must be 32 or 64");
+        return JBP_ERR_ARG;
+    }
+    if (engine < JBP_ENGINE_AUTO || engine > JBP_ENGINE_HBM) {
+        set_err(err, errlen, "unknown engine %d", engine);
         return JBP_ERR_ARG;
     }
     launch_fill_i32(ctx->d_counter, 0, 1, ctx->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     if (precision == JBP_FP32) {
-        if (!ctx->qc) {
-            set_err(err, errlen, "FP32 decoding needs a permutation-block code in this build");
-            return JBP_ERR_UNSUPPORTED;
-        }
         if (damping > 0.0) {
             set_err(err, errlen, "FP32 decoder runs damping 0 only; use precision 64");
             return JBP_ERR_UNSUPPORTED;
         }
-        launch_decode_qc_f32(ctx->dg, ctx->qcp, ctx->d_qc_tables, ctx->tp, count, prior, max_iter,
-                             warmup, interval, std::max(1, ctx->cus), ctx->d_counter, ctx->stream,
-                             learned);
+        const bool lds = engine == JBP_ENGINE_LDS || (engine == JBP_ENGINE_AUTO && (ctx->qc || learned));
+        if (lds && !ctx->qc) {
+            set_err(err, errlen, "the LDS-resident FP32 decoder needs a 768 x (3 x 12) permutation-block code");
+            return JBP_ERR_UNSUPPORTED;
+        }
+        if (!lds && learned) {
+            set_err(err, errlen, "learned decoding runs on the LDS-resident decoder only");
+            return JBP_ERR_UNSUPPORTED;
+        }
+        if (lds) {
+            launch_decode_qc_f32(ctx->dg, ctx->qcp, ctx->d_qc_tables, ctx->tp, count, nt, max_iter,
+                                 warmup, interval, std::max(1, ctx->cus), ctx->d_counter, ctx->stream,
+                                 learned);
+        } else {
+            int rc = run_gen(ctx, count, nt, max_iter, warmup, interval, slots, err, errlen);
+            if (rc) return rc;
+        }
     } else {
         int rc = ensure_f64(ctx, err, errlen);
         if (rc) return rc;
-        launch_decode_f64(ctx->dg, ctx->tp, count, prior, max_iter, damping, ctx->f64_slots,
+        launch_decode_f64(ctx->dg, ctx->tp, count, nt, max_iter, damping, ctx->f64_slots,
                           ctx->f64_block, ctx->f64_scratch, ctx->d_counter, ctx->stream);
     }
     HIPCHK(hipGetLastError());
@@ -319,7 +467,23 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
     jbp_run_stats st;
     std::memset(&st, 0, sizeof(st));
     const DevGraph &g = ctx->dg;
-    const Prior prior = prior_from_p(cfg->p);
+    if (cfg->n_p < 0 || (cfg->n_p > 0 && !cfg->p_list) || cfg->n_p > kMaxPList) {
+        set_err(err, errlen, "p_list needs 1..%d entries", kMaxPList);
+        return JBP_ERR_ARG;
+    }
+    const int np = cfg->n_p > 0 ? cfg->n_p : 1;
+    const double *plist = cfg->n_p > 0 ? cfg->p_list : &cfg->p;
+    for (int i = 0; i < np; ++i)
+        if (!(plist[i] >= 0.0 && plist[i] <= 1.0)) {
+            set_err(err, errlen, "p = %g is outside [0, 1]", plist[i]);
+            return JBP_ERR_ARG;
+        }
+    {
+        std::vector<Prior> pri(np);
+        for (int i = 0; i < np; ++i) pri[i] = prior_from_p(plist[i]);
+        int rc0 = set_noise(ctx, pri.data(), plist, np, err, errlen);
+        if (rc0) return rc0;
+    }
     if (cfg->block > 0) ctx->f64_block = cfg->block;
     if (cfg->slots > 0 && cfg->precision == JBP_FP64 && cfg->slots != ctx->f64_slots) {
         if (ctx->f64_scratch) (void)hipFree(ctx->f64_scratch);
@@ -337,8 +501,11 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
     for (int64_t done = 0; done < total && !stop;) {
         const int cnt = static_cast<int>(std::min<int64_t>(chunk, total - done));
         const int64_t first = cfg->first_trial + done;
+        // trial index idx uses channel entry idx mod n_p
+        NoiseTab nt = ctx->noise;
+        nt.phase = static_cast<int>(((first % np) + np) % np);
         HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-        launch_sample(g, ctx->tp, cfg->base_seed, first, cnt, cfg->p, nullptr, ctx->stream);
+        launch_sample(g, ctx->tp, cfg->base_seed, first, cnt, nt, ctx->stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
         HIPCHK(hipEventSynchronize(ctx->ev1));
@@ -346,29 +513,38 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
         HIPCHK(hipEventElapsedTime(&sms, ctx->ev0, ctx->ev1));
         ctx->last_ms[0] = sms;
         ctx->launches[0]++;
-        rc = run_decode(ctx, cnt, prior, cfg->max_iter, cfg->check_warmup, cfg->check_interval,
-                        cfg->damping, cfg->precision, err, errlen, learned);
+        rc = run_decode(ctx, cnt, nt, cfg->max_iter, cfg->check_warmup, cfg->check_interval,
+                        cfg->damping, cfg->precision, err, errlen, learned, cfg->engine,
+                        cfg->precision == JBP_FP32 ? cfg->slots : 0);
         if (rc) return rc;
         dev_ms += ctx->last_ms[1];
-        if (ctx->have_basis) {
+        if (ctx->have_logicals) {
+            HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+            launch_logical(g, ctx->lg[0], ctx->lg[1], ctx->tp, cnt, ctx->stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        } else if (ctx->have_basis) {
             HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
             launch_residual(g, ctx->basis[0], ctx->basis[1], ctx->tp, cnt, ctx->stream);
             HIPCHK(hipGetLastError());
             HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        } else {
+            // no test of the residual: the kernel with no operators still
+            // sets the exact-match flags, and success becomes exact below
+            launch_logical(g, DevLogicals(), DevLogicals(), ctx->tp, cnt, ctx->stream);
+            HIPCHK(hipGetLastError());
         }
         HIPCHK(hipMemcpyAsync(iters.data(), ctx->tp.iters, cnt * 4, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipMemcpyAsync(ok.data(), ctx->tp.syn_ok, cnt, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipMemcpyAsync(succ.data(), ctx->tp.success, cnt, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipMemcpyAsync(exact.data(), ctx->tp.exact, cnt, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        if (ctx->have_basis) {
+        const bool judged = ctx->have_logicals || ctx->have_basis;
+        if (judged) {
             float rms = 0.f;
             HIPCHK(hipEventElapsedTime(&rms, ctx->ev0, ctx->ev1));
             ctx->last_ms[2] = rms;
             ctx->launches[2]++;
-        } else {
-            // Without a dense basis only exact matches are counted as successes.
-            for (int i = 0; i < cnt; ++i) succ[i] = 0;
         }
         if (pt && (pt->err || pt->est)) {
             if ((rc = ensure_states(ctx, static_cast<size_t>(cnt) * g.n, err, errlen))) return rc;
@@ -385,7 +561,7 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
         }
         int take = cnt;
         for (int i = 0; i < cnt; ++i) {
-            if (!ctx->have_basis) succ[i] = (ok[i] && exact[i]) ? 1 : 0;
+            if (!judged) succ[i] = (ok[i] && exact[i]) ? 1 : 0;
             st.total_iters += iters[i];
             st.failures += succ[i] ? 0 : 1;
             st.bp_failures += ok[i] ? 0 : 1;
@@ -507,8 +683,12 @@ int jbp_create(const jbp_graph_desc *g, int device_id, jbp_ctx **out, char *err,
         ctx->d_qc_tables = const_cast<int *>(dt);
     }
     // Residual-test bases: row spaces of Hx (side 0) and Hz (side 1).
-    size_t basis_bytes = static_cast<size_t>(std::max(ctx->hs[0].m, ctx->hs[1].m)) * dg.wn * 4;
-    if (basis_bytes <= kMaxBasisBytes) {
+    // Skipped when the dense elimination would be too slow on the host (the
+    // caller can then install logical operators with jbp_set_logicals).
+    const int mmax = std::max(ctx->hs[0].m, ctx->hs[1].m);
+    size_t basis_bytes = static_cast<size_t>(mmax) * dg.wn * 4;
+    const double elim_work = static_cast<double>(mmax) * mmax * ((ctx->n + 63) / 64);
+    if (basis_bytes <= kMaxBasisBytes && elim_work <= kMaxBasisWork) {
         for (int s = 0; s < 2; ++s) {
             std::vector<uint32_t> rows;
             std::vector<int> piv;
@@ -535,6 +715,9 @@ void jbp_destroy(jbp_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_chunk(ctx);
+    free_noise(ctx);
+    free_gen(ctx);
+    free_logicals(ctx);
     for (void *p : ctx->allocs) (void)hipFree(p);
     if (ctx->f64_scratch) (void)hipFree(ctx->f64_scratch);
     if (ctx->d_counter) (void)hipFree(ctx->d_counter);
@@ -555,7 +738,7 @@ int jbp_info(jbp_ctx *ctx, int64_t out[8]) {
     out[4] = ctx->device;
     out[5] = ctx->cus;
     out[6] = ctx->f64_block;
-    out[7] = ctx->have_basis ? 1 : 0;
+    out[7] = ctx->have_logicals ? 2 : ctx->have_basis ? 1 : 0;
     return JBP_OK;
 }
 
@@ -576,8 +759,12 @@ int jbp_sample(jbp_ctx *ctx, uint64_t base_seed, int64_t first_trial, int count,
     int rc = ensure_chunk(ctx, count, err, errlen);
     if (rc) return rc;
     const DevGraph &g = ctx->dg;
+    {
+        const Prior pri = prior_from_p(p);
+        if ((rc = set_noise(ctx, &pri, &p, 1, err, errlen))) return rc;
+    }
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    launch_sample(g, ctx->tp, base_seed, first_trial, count, p, nullptr, ctx->stream);
+    launch_sample(g, ctx->tp, base_seed, first_trial, count, ctx->noise, ctx->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     if (err_out) {
@@ -629,7 +816,11 @@ int jbp_decode_batch(jbp_ctx *ctx, int count, const uint32_t *sx_bits, const uin
                           hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->tp.syn_z, sz_bits, static_cast<size_t>(count) * g.wm[1] * 4,
                           hipMemcpyHostToDevice, ctx->stream));
-    rc = run_decode(ctx, count, make_prior(prior), max_iter, 0, 1, damping, precision, err, errlen);
+    {
+        const Prior pri = make_prior(prior);
+        if ((rc = set_noise(ctx, &pri, nullptr, 1, err, errlen))) return rc;
+    }
+    rc = run_decode(ctx, count, ctx->noise, max_iter, 0, 1, damping, precision, err, errlen);
     if (rc) return rc;
     if (iterations_out)
         HIPCHK(hipMemcpy(iterations_out, ctx->tp.iters, static_cast<size_t>(count) * 4,
@@ -684,7 +875,11 @@ int jbp_decode(jbp_ctx *ctx, const int *sx, const int *sz, const double prior[4]
     auto m1 = std::chrono::steady_clock::now();
     int warmup = precision == JBP_FP32 ? std::max(0, check_warmup) : 0;
     int interval = precision == JBP_FP32 ? std::max(1, check_interval) : 1;
-    rc = run_decode(ctx, 1, make_prior(prior), max_iter, warmup, interval, damping, precision, err,
+    {
+        const Prior pri = make_prior(prior);
+        if ((rc = set_noise(ctx, &pri, nullptr, 1, err, errlen))) return rc;
+    }
+    rc = run_decode(ctx, 1, ctx->noise, max_iter, warmup, interval, damping, precision, err,
                     errlen);
     if (rc) return rc;
     out->kernel_ms = ctx->last_ms[1];
@@ -751,6 +946,55 @@ int jbp_in_row_space(jbp_ctx *ctx, int side, int count, const int *bits, uint8_t
 int jbp_simulate(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_run_stats *stats,
                  char *err, size_t errlen) {
     return jbp::simulate_impl(ctx, cfg, pt, stats, nullptr, err, errlen);
+}
+
+int jbp_set_logicals(jbp_ctx *ctx, int side, int64_t n_rows, const int64_t *row_off,
+                     const int *qubits, char *err, size_t errlen) {
+    if (!ctx || side < -1 || side > 1 || n_rows < 0 || (n_rows > 0 && (!row_off || !qubits))) {
+        set_err(err, errlen, "bad arguments");
+        return JBP_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    if (side == -1) {
+        free_logicals(ctx);
+        return JBP_OK;
+    }
+    const int64_t total = n_rows > 0 ? row_off[n_rows] : 0;
+    if (n_rows > 0 && row_off[0] != 0) {
+        set_err(err, errlen, "row_off[0] must be 0");
+        return JBP_ERR_ARG;
+    }
+    for (int64_t r = 0; r < n_rows; ++r)
+        if (row_off[r + 1] < row_off[r]) {
+            set_err(err, errlen, "row offsets decrease at row %lld", static_cast<long long>(r));
+            return JBP_ERR_ARG;
+        }
+    for (int64_t e = 0; e < total; ++e)
+        if (qubits[e] < 0 || qubits[e] >= ctx->n) {
+            set_err(err, errlen, "qubit %d out of range", qubits[e]);
+            return JBP_ERR_ARG;
+        }
+    for (void *&p : ctx->lg_mem[side]) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
+    ctx->lg[side] = DevLogicals();
+    ctx->lg_set[side] = false;
+    ctx->have_logicals = false;
+    HIPCHK(hipMalloc(&ctx->lg_mem[side][0], (n_rows + 1) * sizeof(int64_t)));
+    HIPCHK(hipMalloc(&ctx->lg_mem[side][1], std::max<int64_t>(total, 1) * sizeof(int)));
+    std::vector<int64_t> off(row_off ? row_off : nullptr, row_off ? row_off + n_rows + 1 : nullptr);
+    if (off.empty()) off.push_back(0);
+    HIPCHK(hipMemcpy(ctx->lg_mem[side][0], off.data(), off.size() * sizeof(int64_t),
+                     hipMemcpyHostToDevice));
+    if (total > 0)
+        HIPCHK(hipMemcpy(ctx->lg_mem[side][1], qubits, total * sizeof(int), hipMemcpyHostToDevice));
+    ctx->lg[side].rows = n_rows;
+    ctx->lg[side].off = static_cast<const int64_t *>(ctx->lg_mem[side][0]);
+    ctx->lg[side].qubit = static_cast<const int *>(ctx->lg_mem[side][1]);
+    ctx->lg_set[side] = true;
+    ctx->have_logicals = ctx->lg_set[0] && ctx->lg_set[1];
+    return JBP_OK;
 }
 
 }  // extern "C"

@@ -49,6 +49,9 @@ struct jbp_code {
     // CSR in the reference's build_cuda_graph order
     std::vector<int> chk_off[2], chk_edge[2], edge_var[2], edge_chk[2], edge_pos[2];
     std::vector<int> var_off[2], var_edge[2];
+    // hypergraph-product codes: the classical matrix (rows of column indices)
+    int hgp_m = 0, hgp_n = 0;
+    std::vector<std::vector<int>> hgp_H;
 };
 
 namespace {
@@ -116,6 +119,57 @@ long long gf2_rank(const jbp_code &c, int s) {
         }
     }
     return static_cast<long long>(basis.size());
+}
+
+// Reduced row echelon form over GF(2) of a dense matrix with `cols` columns
+// (rows as 64-bit words); returns the pivot column of each nonzero row and
+// leaves the reduced rows in `rows` (first rank entries).
+std::vector<int> gf2_rref(std::vector<std::vector<uint64_t>> &rows, int cols) {
+    std::vector<int> piv;
+    int rank = 0;
+    for (int col = 0; col < cols && rank < static_cast<int>(rows.size()); ++col) {
+        const int w = col >> 6;
+        const uint64_t bit = 1ull << (col & 63);
+        int sel = -1;
+        for (int r = rank; r < static_cast<int>(rows.size()); ++r)
+            if (rows[r][w] & bit) { sel = r; break; }
+        if (sel < 0) continue;
+        std::swap(rows[rank], rows[sel]);
+        for (int r = 0; r < static_cast<int>(rows.size()); ++r)
+            if (r != rank && (rows[r][w] & bit))
+                for (size_t k = 0; k < rows[r].size(); ++k) rows[r][k] ^= rows[rank][k];
+        piv.push_back(col);
+        ++rank;
+    }
+    rows.resize(rank);
+    return piv;
+}
+
+// Null-space basis of a matrix (rows of column indices, `cols` columns) and
+// the columns that are not pivots of its reduced row echelon form: unit
+// vectors at those columns complete the row space to the whole space.
+void gf2_kernel(const std::vector<std::vector<int>> &M, int cols,
+                std::vector<std::vector<int>> &ker_support, std::vector<int> &free_cols) {
+    const int W = (cols + 63) / 64;
+    std::vector<std::vector<uint64_t>> rows(M.size(), std::vector<uint64_t>(W, 0));
+    for (size_t r = 0; r < M.size(); ++r)
+        for (int c : M[r]) rows[r][c >> 6] ^= 1ull << (c & 63);
+    const std::vector<int> piv = gf2_rref(rows, cols);
+    std::vector<char> is_piv(cols, 0);
+    for (int c : piv) is_piv[c] = 1;
+    free_cols.clear();
+    ker_support.clear();
+    for (int f = 0; f < cols; ++f) {
+        if (is_piv[f]) continue;
+        free_cols.push_back(f);
+        // x[f] = 1, x[pivot_i] = rref[i][f], other free columns 0
+        std::vector<int> sup;
+        for (size_t i = 0; i < piv.size(); ++i)
+            if ((rows[i][f >> 6] >> (f & 63)) & 1ull) sup.push_back(piv[i]);
+        sup.push_back(f);
+        std::sort(sup.begin(), sup.end());
+        ker_support.push_back(std::move(sup));
+    }
 }
 
 // Shortest cycle of the Tanner graph by breadth-first search from every
@@ -336,8 +390,51 @@ int jbp_code_hgp(int m, int n, int col_w, int row_w, uint64_t seed, jbp_code **o
             for (int j : H[b]) zf.push_back(i * n + j);
             for (int a : Ht[i]) zf.push_back(n * n + a * m + b);
         }
-    return jbp_code_from_checks(nq, m * n, degx, xf.data(), n * m, degz, zf.data(), out, err,
-                                errlen);
+    int rc = jbp_code_from_checks(nq, m * n, degx, xf.data(), n * m, degz, zf.data(), out, err,
+                                  errlen);
+    if (rc == JBP_OK) {
+        (*out)->hgp_m = m;
+        (*out)->hgp_n = n;
+        (*out)->hgp_H = std::move(H);
+    }
+    return rc;
+}
+
+int jbp_code_logicals(const jbp_code *c, int side, int64_t *n_rows, int64_t *n_entries,
+                      int64_t *row_off, int *qubits) {
+    if (!c || (side != 0 && side != 1) || !n_rows || !n_entries) return JBP_ERR_ARG;
+    if (c->hgp_H.empty()) return JBP_ERR_UNSUPPORTED;
+    const int m = c->hgp_m, n = c->hgp_n;
+    const long long nn = static_cast<long long>(n) * n;
+    std::vector<std::vector<int>> Ht(n);
+    for (int r = 0; r < m; ++r)
+        for (int col : c->hgp_H[r]) Ht[col].push_back(r);
+    std::vector<std::vector<int>> kerH, kerHt;  // supports over [n] and [m]
+    std::vector<int> freeH, freeHt;             // complements of row(H), col(H)
+    gf2_kernel(c->hgp_H, n, kerH, freeH);
+    gf2_kernel(Ht, m, kerHt, freeHt);
+    // Qubit (i, j) of the n x n block is i*n + j; (a, b) of the m x m block
+    // is n*n + a*m + b.  Side 0 (parities with the X part of a residual):
+    // e_p (x) z and y (x) e_q; side 1: z (x) e_p and e_q (x) y.
+    int64_t rows = 0, ent = 0;
+    auto emit = [&](const std::vector<int> &sup, auto qubit) {
+        if (row_off) row_off[rows] = ent;
+        for (int t : sup) {
+            if (qubits) qubits[ent] = static_cast<int>(qubit(t));
+            ++ent;
+        }
+        ++rows;
+    };
+    for (int p : freeH)
+        for (const auto &z : kerH)
+            emit(z, [&](int t) { return side == 0 ? 1LL * p * n + t : 1LL * t * n + p; });
+    for (int q : freeHt)
+        for (const auto &y : kerHt)
+            emit(y, [&](int t) { return side == 0 ? nn + 1LL * t * m + q : nn + 1LL * q * m + t; });
+    if (row_off) row_off[rows] = ent;
+    *n_rows = rows;
+    *n_entries = ent;
+    return JBP_OK;
 }
 
 void jbp_code_free(jbp_code *code) { delete code; }

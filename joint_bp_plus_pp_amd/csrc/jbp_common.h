@@ -47,6 +47,15 @@ struct DevBasis {
     const int *pivot = nullptr;        // rank: pivot column of each row
 };
 
+// Logical-operator representatives of one side as sparse rows: a residual
+// part r with zero syndrome is a stabilizer iff every row has even overlap
+// with r.
+struct DevLogicals {
+    int64_t rows = 0;
+    const int64_t *off = nullptr;  // rows + 1
+    const int *qubit = nullptr;    // off[rows]
+};
+
 // Per-trial bit planes resident in HBM (one row per trial in a chunk).
 struct TrialPlanes {
     uint32_t *err_x = nullptr;   // chunk x wn
@@ -75,14 +84,36 @@ struct DrawThresholds {
 };
 DrawThresholds draw_thresholds(double p);
 
-// Launchers (defined in the .hip files).  d_per_trial (device, may be null)
-// overrides p with one threshold set per trial.
+// FP32 decoder channel constants of one prior: log2 prior (I, X, Z, Y), then
+// (pi_I, pi_Z, pi_X, pi_Y) for the X-side extrinsic term, then
+// (pi_I, pi_X, pi_Z, pi_Y) for the Z side.
+constexpr int kPrior32 = 12;
+void prior32_from(const Prior &pr, float out[kPrior32]);
+
+// Channel of each trial of a chunk: slot i uses entry (phase + i) % n of the
+// device tables.  One entry for a single p; entry idx % n for an interleaved
+// list of p values (phase = first trial index % n); one per slot for a
+// training batch (n = batch, phase 0).
+struct NoiseTab {
+    const DrawThresholds *th = nullptr;  // sampler thresholds [n]
+    const double *prior = nullptr;       // normalized prior [n][4]
+    const float *prior32 = nullptr;      // [n][kPrior32]
+    int n = 1;
+    int phase = 0;
+    __host__ __device__ int entry(int slot) const { return (phase + slot) % n; }
+};
+
+// Launchers (defined in the .hip files).
 void launch_sample(const DevGraph &g, TrialPlanes tp, uint64_t base_seed, int64_t first_trial,
-                   int count, double p, const DrawThresholds *d_per_trial, hipStream_t st);
+                   int count, const NoiseTab &nt, hipStream_t st);
 void launch_syndromes(const DevGraph &g, TrialPlanes tp, int count, hipStream_t st);
 void launch_residual(const DevGraph &g, DevBasis bx, DevBasis bz, TrialPlanes tp, int count,
                      hipStream_t st);
-void launch_decode_f64(const DevGraph &g, TrialPlanes tp, int count, Prior prior, int max_iter,
+// Success test with logical operators: lx against the X part of est ^ err,
+// lz against the Z part.
+void launch_logical(const DevGraph &g, DevLogicals lx, DevLogicals lz, TrialPlanes tp, int count,
+                    hipStream_t st);
+void launch_decode_f64(const DevGraph &g, TrialPlanes tp, int count, const NoiseTab &nt, int max_iter,
                        double damping, int slots, int block, double *scratch, int *counter,
                        hipStream_t st);
 size_t decode_f64_scratch_bytes(const DevGraph &g, int slots);
@@ -91,7 +122,7 @@ struct LearnedInfer;
 // learned == nullptr: plain BP.  Otherwise learned->T learned iterations,
 // then plain BP restarted on trials that did not converge.
 void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_tables,
-                          TrialPlanes tp, int count, Prior prior, int max_iter, int check_warmup,
+                          TrialPlanes tp, int count, const NoiseTab &nt, int max_iter, int check_warmup,
                           int check_interval, int slots, int *counter, hipStream_t st,
                           const LearnedInfer *learned = nullptr);
 bool qc_f32_supported(const QcParams &qc, int device, char *why, size_t len);

@@ -10,13 +10,17 @@
 
 #include "../../include/jbp.h"
 #include "jbp_common.h"
+#include "jbp_gen.h"
 #include "jbp_learned.h"
 
 namespace jbp {
 
 // Largest dense basis (bytes per side) built for the device residual test.
 constexpr size_t kMaxBasisBytes = size_t(256) << 20;
+// Largest host elimination (rows x rows x 64-bit words) done at create.
+constexpr double kMaxBasisWork = 2e10;
 constexpr int kDefaultChunk = 65536;
+constexpr int kMaxPList = JBP_MAX_P_LIST;
 
 void set_err(char *err, size_t len, const char *fmt, ...);
 
@@ -45,6 +49,12 @@ struct jbp_ctx {
     int *d_qc_tables = nullptr;
     bool have_basis = false;
     jbp::DevBasis basis[2];
+    // logical operators (jbp_set_logicals); used instead of the basis when
+    // both sides are set
+    bool have_logicals = false;
+    bool lg_set[2] = {false, false};
+    jbp::DevLogicals lg[2];
+    void *lg_mem[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     // chunk buffers
     int chunk_cap = 0;
     jbp::TrialPlanes tp;
@@ -57,6 +67,21 @@ struct jbp_ctx {
     size_t d_states_cap = 0;
     int *d_ints = nullptr;    // pack input buffer
     size_t d_ints_cap = 0;
+    // channel tables (set_noise)
+    jbp::NoiseTab noise;
+    int noise_cap = 0;
+    jbp::DrawThresholds *d_th = nullptr;
+    double *d_prior = nullptr;
+    float *d_prior32 = nullptr;
+    std::vector<double> noise_key;  // what the tables hold (skip re-uploads)
+    // generic (HBM) FP32 decoder
+    jbp::GenState gen;
+    void *gen_mem = nullptr;
+    size_t gen_cap = 0;
+    int gen_slots = 0;        // default resident trials (0: not chosen yet)
+    int *h_ctrl = nullptr;    // pinned [2]
+    hipEvent_t gen_ev[2] = {nullptr, nullptr};
+    int64_t gen_steps = 0;    // engine steps of the last generic decode
     double last_ms[3] = {0, 0, 0};
     int64_t launches[3] = {0, 0, 0};
 };
@@ -78,11 +103,19 @@ int ensure_states(jbp_ctx *ctx, size_t elems, char *err, size_t errlen);
 int ensure_ints(jbp_ctx *ctx, size_t elems, char *err, size_t errlen);
 Prior make_prior(const double p4[4]);
 Prior prior_from_p(double p);
-// Decode `count` trials whose syndrome planes are already in ctx->tp; records
-// the decoder launch time (HIP events) in ctx->last_ms[1].
-int run_decode(jbp_ctx *ctx, int count, const Prior &prior, int max_iter, int warmup, int interval,
+// Upload n channel entries into ctx->noise (phase 0): priors[i], and the
+// sampler thresholds of p[i] when p is not NULL.
+int set_noise(jbp_ctx *ctx, const Prior *priors, const double *p, int n, char *err, size_t errlen);
+void free_noise(jbp_ctx *ctx);
+// Decode `count` trials whose syndrome planes are already in ctx->tp, slot i
+// with channel entry nt.entry(i); records the decoder launch time (HIP
+// events) in ctx->last_ms[1].
+int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmup, int interval,
                double damping, int precision, char *err, size_t errlen,
-               const LearnedInfer *learned = nullptr);
+               const LearnedInfer *learned = nullptr, int engine = JBP_ENGINE_AUTO,
+               int slots = 0);
+void free_gen(jbp_ctx *ctx);
+void free_logicals(jbp_ctx *ctx);
 
 int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_run_stats *stats,
                   const LearnedInfer *learned, char *err, size_t errlen);

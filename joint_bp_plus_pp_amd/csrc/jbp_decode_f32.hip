@@ -39,42 +39,18 @@
 // check owner recovers the parity of est over its row while reading its
 // inputs.  The test of iteration k's estimate therefore completes during the
 // check pass of iteration k+1; a trial that converged at k keeps est_k.
+#include "jbp_bp32.h"
 #include "jbp_common.h"
 #include "jbp_learned.h"
 
 namespace jbp {
 namespace {
 
-constexpr float kLlrMax = 64.0f;  // clamp of |lambda| in bits
-
 struct QcDev {
     const int *ainv;  // [2][J][L]
     const int *b;     // [2][J][L]
     int P;
 };
-
-struct PriorF {
-    float lp[4];            // log2 prior (I, X, Z, Y)
-    float fx[4];            // (pi_I, pi_Z, pi_X, pi_Y) for the X-side extrinsic
-    float fz[4];            // (pi_I, pi_X, pi_Z, pi_Y) for the Z-side extrinsic
-};
-
-__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
-__device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
-
-// log2((a 2^L + b) / (c 2^L + d)) without overflow for any L.
-__device__ __forceinline__ float extrinsic_term(float L, const float *k) {
-    float t = fast_exp2(-fabsf(L));
-    float num, den;
-    if (L >= 0.0f) {
-        num = k[0] + k[1] * t;
-        den = k[2] + k[3] * t;
-    } else {
-        num = k[0] * t + k[1];
-        den = k[2] * t + k[3];
-    }
-    return fast_log2(num) - fast_log2(den);
-}
 
 __device__ __forceinline__ float with_lsb(float v, uint32_t bit) {
     return __uint_as_float((__float_as_uint(v) & ~1u) | bit);
@@ -202,12 +178,7 @@ __device__ __forceinline__ int qc_phase(float *B, float *GZ, int *s_cnt, float *
                 lx += lam[0][r][cc];
                 lz += lam[1][r][cc];
             }
-            float l0 = pr.lp[0] + lx + lz, l1 = pr.lp[1] + lz, l2 = pr.lp[2] + lx, l3 = pr.lp[3];
-            int best = 0;
-            float bv = l0;
-            if (l1 > bv) { best = 1; bv = l1; }
-            if (l2 > bv) { best = 2; bv = l2; }
-            if (l3 > bv) { best = 3; bv = l3; }
+            const int best = hard_decision(lx, lz, pr);
             estx |= static_cast<uint32_t>(best & 1) << cc;
             estz |= static_cast<uint32_t>(best >> 1) << cc;
             float gx = lx + extrinsic_term(lz, pr.fx);
@@ -325,7 +296,7 @@ __device__ __forceinline__ int qc_phase(float *B, float *GZ, int *s_cnt, float *
 // the plain iterations.
 template <int P, int J, int L, bool LEARNED, bool DAMPED>
 __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, TrialPlanes tp,
-                                                         int count, PriorF pr, int max_iter,
+                                                         int count, NoiseTab nt, int max_iter,
                                                          int warmup, int interval, int *counter,
                                                          LearnedInfer li) {
     static_assert(P % 64 == 0 && P <= 1024, "one thread per block row, whole wavefronts");
@@ -367,6 +338,21 @@ __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, 
         const int trial = s_trial;
         __syncthreads();
         if (trial >= count) return;
+        // this trial's channel constants, uniform across the workgroup
+        PriorF pr;
+        {
+            const int ent = __builtin_amdgcn_readfirstlane(nt.entry(trial));
+            const float *q = nt.prior32 + kPrior32 * ent;
+            auto uni = [](float x) {
+                return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+            };
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                pr.lp[i] = uni(q[i]);
+                pr.fx[i] = uni(q[4 + i]);
+                pr.fz[i] = uni(q[8 + i]);
+            }
+        }
 
         uint32_t synx = 0, synz = 0;  // bit r: syndrome of check (r, t)
 #pragma unroll
@@ -427,6 +413,16 @@ __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, 
 
 }  // namespace
 
+void prior32_from(const Prior &pr, float out[kPrior32]) {
+    const double *p = pr.p;
+    const int fx[4] = {0, 2, 1, 3}, fz[4] = {0, 1, 2, 3};
+    for (int i = 0; i < 4; ++i) {
+        out[i] = static_cast<float>(std::log2(p[i]));
+        out[4 + i] = static_cast<float>(p[fx[i]]);
+        out[8 + i] = static_cast<float>(p[fz[i]]);
+    }
+}
+
 bool qc_f32_supported(const QcParams &qc, int device, char *why, size_t len) {
     auto fail = [&](const char *msg) {
         if (why && len) snprintf(why, len, "%s", msg);
@@ -444,19 +440,10 @@ bool qc_f32_supported(const QcParams &qc, int device, char *why, size_t len) {
 }
 
 void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_tables,
-                          TrialPlanes tp, int count, Prior prior, int max_iter, int check_warmup,
+                          TrialPlanes tp, int count, const NoiseTab &nt, int max_iter, int check_warmup,
                           int check_interval, int slots, int *counter, hipStream_t st,
                           const LearnedInfer *learned) {
     if (count <= 0) return;
-    PriorF pf;
-    const double *p = prior.p;
-    for (int i = 0; i < 4; ++i) pf.lp[i] = static_cast<float>(std::log2(p[i]));
-    const double fx[4] = {p[0], p[2], p[1], p[3]};
-    const double fz[4] = {p[0], p[1], p[2], p[3]};
-    for (int i = 0; i < 4; ++i) {
-        pf.fx[i] = static_cast<float>(fx[i]);
-        pf.fz[i] = static_cast<float>(fz[i]);
-    }
     QcDev dev;
     dev.P = qc.P;
     dev.ainv = d_tables;                  // [2][J][L]
@@ -470,7 +457,7 @@ void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_ta
     auto launch = [&](auto kern) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(768), lds, st, g, dev, tp, count, pf, max_iter,
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(768), lds, st, g, dev, tp, count, nt, max_iter,
                            warmup, interval, counter, li);
     };
     if (learned && learned->T > 0) {

@@ -190,13 +190,12 @@ __device__ void var_update(const DevGraph &g, int v, const M4 &prior, double dam
 }
 
 template <int MAXC, int MAXV>
-__global__ void decode_f64_kernel(DevGraph g, TrialPlanes tp, int count, Prior pr, int max_iter,
+__global__ void decode_f64_kernel(DevGraph g, TrialPlanes tp, int count, NoiseTab nt, int max_iter,
                                   double damping, double *scratch, size_t stride, int *counter) {
     __shared__ int s_trial;
     const int tid = threadIdx.x;
     const int bs = blockDim.x;
     const SlotPtrs sp = slot_ptrs(g, scratch, stride);
-    const M4 prior{{pr.p[0], pr.p[1], pr.p[2], pr.p[3]}};
     const int mx = g.side[0].m, mz = g.side[1].m;
     for (;;) {
         if (tid == 0) s_trial = atomicAdd(counter, 1);
@@ -204,6 +203,8 @@ __global__ void decode_f64_kernel(DevGraph g, TrialPlanes tp, int count, Prior p
         const int trial = s_trial;
         __syncthreads();
         if (trial >= count) return;
+        const double *pq = nt.prior + 4 * nt.entry(trial);
+        const M4 prior{{pq[0], pq[1], pq[2], pq[3]}};
         const uint32_t *tsx = tp.syn_x + static_cast<size_t>(trial) * g.wm[0];
         const uint32_t *tsz = tp.syn_z + static_cast<size_t>(trial) * g.wm[1];
 
@@ -291,7 +292,7 @@ size_t decode_f64_scratch_bytes(const DevGraph &g, int slots) {
     return per * static_cast<size_t>(slots);
 }
 
-void launch_decode_f64(const DevGraph &g, TrialPlanes tp, int count, Prior prior, int max_iter,
+void launch_decode_f64(const DevGraph &g, TrialPlanes tp, int count, const NoiseTab &nt, int max_iter,
                        double damping, int slots, int block, double *scratch, int *counter,
                        hipStream_t st) {
     if (count <= 0) return;
@@ -299,10 +300,10 @@ void launch_decode_f64(const DevGraph &g, TrialPlanes tp, int count, Prior prior
     int grid = slots < count ? slots : count;
     if (g.max_cdeg <= 16 && g.max_vdeg <= 4) {
         hipLaunchKernelGGL((decode_f64_kernel<16, 4>), dim3(grid), dim3(block), 0, st, g, tp,
-                           count, prior, max_iter, damping, scratch, stride, counter);
+                           count, nt, max_iter, damping, scratch, stride, counter);
     } else {
         hipLaunchKernelGGL((decode_f64_kernel<32, 16>), dim3(grid), dim3(block), 0, st, g, tp,
-                           count, prior, max_iter, damping, scratch, stride, counter);
+                           count, nt, max_iter, damping, scratch, stride, counter);
     }
 }
 

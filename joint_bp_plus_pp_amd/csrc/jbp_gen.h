@@ -1,0 +1,38 @@
+// Generic FP32 decoder state (jbp_decode_gen.hip): S = 64*groups resident
+// trial slots, every array stored [group][item][64 lanes] or as one 64-bit
+// word per (group, item).
+#pragma once
+
+#include "jbp_common.h"
+
+namespace jbp {
+
+struct GenState {
+    int groups = 0;
+    float *lam[2] = {nullptr, nullptr};      // [groups][E_side][64] check messages
+    float *gb[2] = {nullptr, nullptr};       // [groups][n][64] G_x, G_z per variable
+    uint64_t *estw[2] = {nullptr, nullptr};  // [groups][n] hard-decision bits
+    uint64_t *synw[2] = {nullptr, nullptr};  // [groups][m_side] syndrome bits
+    uint64_t *bad = nullptr;     // [groups] slots with an unsatisfied check this step
+    uint64_t *active = nullptr;  // [groups] slots holding a trial (bad..refill contiguous)
+    uint64_t *fresh = nullptr;   // [groups] slots whose messages read as 0 this step
+    uint64_t *refill = nullptr;  // [groups] slots given a trial by the last book pass
+    int *slot_trial = nullptr;   // [S] chunk index of the slot's trial
+    int *slot_iter = nullptr;    // [S] iteration index of the slot's trial
+    float *prior32 = nullptr;    // [groups][kPrior32][64]
+    int2 *fin = nullptr;         // [S] (trial, slot) stopped this step
+    int *ctrl = nullptr;         // [0] next trial, [1] trials finished, [2] fin entries
+};
+
+bool gen_supported(const DevGraph &g);
+size_t gen_carve_bytes(const DevGraph &g, int groups);
+void gen_carve(const DevGraph &g, int groups, void *base, GenState &s);
+
+// Decode trials 0..count-1 of tp (syndrome planes in place) to est planes,
+// iterations and syn_ok.  h_ctrl: pinned host ints [2]; evs: two events.
+// Returns 0 or a hipError_t value, -1 when the step bound is exceeded.
+int gen_decode(const DevGraph &g, const GenState &s, TrialPlanes tp, int count, const NoiseTab &nt,
+               int max_iter, int warmup, int interval, int *h_ctrl, hipEvent_t *evs,
+               hipStream_t st, int64_t *steps_out);
+
+}  // namespace jbp

@@ -310,7 +310,10 @@ int jbp_learned_train_step(jbp_learned *m, uint64_t base_seed, int64_t first_tri
     }
     HIPCHK(hipMemcpy(m->d_th, th.data(), th.size() * sizeof(DrawThresholds), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(m->d_prior, pr.data(), pr.size() * sizeof(double), hipMemcpyHostToDevice));
-    launch_sample(ctx->dg, ctx->tp, base_seed, first_trial, batch, 0.0, m->d_th, ctx->stream);
+    NoiseTab nt;
+    nt.th = m->d_th;
+    nt.n = batch;
+    launch_sample(ctx->dg, ctx->tp, base_seed, first_trial, batch, nt, ctx->stream);
     HIPCHK(hipGetLastError());
     if ((rc = run_train<float>(m, batch, true, err, errlen))) return rc;
     launch_to_double(static_cast<const float *>(m->d_grad), m->d_grad64, m->count, ctx->stream);

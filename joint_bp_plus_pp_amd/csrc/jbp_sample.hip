@@ -96,14 +96,13 @@ __global__ void __launch_bounds__(64) syndrome_kernel(DevGraph g, TrialPlanes tp
 
 // grid = count trials, block = 64 (one wavefront), dynamic LDS = 2*wn words.
 __global__ void __launch_bounds__(64) sample_kernel(DevGraph g, TrialPlanes tp, uint64_t base_seed,
-                                                    int64_t first_trial, DrawThresholds th_all,
-                                                    const DrawThresholds *th_trial) {
+                                                    int64_t first_trial, NoiseTab nt) {
     __shared__ uint64_t mt[kMtN];
     extern __shared__ uint32_t planes[];  // err_x | err_z
     const int lane = threadIdx.x;
     const int trial = blockIdx.x;
     const int n = g.n, wn = g.wn;
-    const DrawThresholds th = th_trial ? th_trial[trial] : th_all;
+    const DrawThresholds th = nt.th[nt.entry(trial)];
     uint32_t *ex = planes;
     uint32_t *ez = planes + wn;
     for (int w = lane; w < 2 * wn; w += 64) planes[w] = 0u;
@@ -245,6 +244,51 @@ __global__ void __launch_bounds__(kResBlock) residual_kernel(DevGraph g, DevBasi
     }
 }
 
+// One trial per block: the residual planes in LDS, then one row of logical
+// operators per thread iteration.
+__global__ void __launch_bounds__(kResBlock) logical_kernel(DevGraph g, DevLogicals lx,
+                                                            DevLogicals lz, TrialPlanes tp) {
+    extern __shared__ uint32_t sm[];
+    const int wn = g.wn;
+    uint32_t *dx = sm;
+    uint32_t *dz = sm + wn;
+    const int trial = blockIdx.x;
+    const int tid = threadIdx.x;
+    const size_t off = static_cast<size_t>(trial) * wn;
+    int nz = 0;
+    for (int w = tid; w < wn; w += kResBlock) {
+        dx[w] = tp.err_x[off + w] ^ tp.est_x[off + w];
+        dz[w] = tp.err_z[off + w] ^ tp.est_z[off + w];
+        nz |= (dx[w] | dz[w]) != 0u;
+    }
+    const int any = __syncthreads_or(nz);
+    const bool ok = tp.syn_ok[trial] != 0;
+    int odd = 0;
+    if (ok && any) {
+        for (int64_t r = tid; r < lx.rows && !odd; r += kResBlock) {
+            uint32_t par = 0;
+            for (int64_t e = lx.off[r]; e < lx.off[r + 1]; ++e) {
+                const int q = lx.qubit[e];
+                par ^= dx[q >> 5] >> (q & 31);
+            }
+            odd |= par & 1u;
+        }
+        for (int64_t r = tid; r < lz.rows && !odd; r += kResBlock) {
+            uint32_t par = 0;
+            for (int64_t e = lz.off[r]; e < lz.off[r + 1]; ++e) {
+                const int q = lz.qubit[e];
+                par ^= dz[q >> 5] >> (q & 31);
+            }
+            odd |= par & 1u;
+        }
+    }
+    const int bad = __syncthreads_or(odd);
+    if (tid == 0) {
+        tp.exact[trial] = any ? 0 : 1;
+        tp.success[trial] = (ok && !bad) ? 1 : 0;
+    }
+}
+
 __global__ void fill_i32_kernel(int *p, int v, int count) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = v;
@@ -327,12 +371,11 @@ DrawThresholds draw_thresholds(double p) {
 }
 
 void launch_sample(const DevGraph &g, TrialPlanes tp, uint64_t base_seed, int64_t first_trial,
-                   int count, double p, const DrawThresholds *d_per_trial, hipStream_t st) {
+                   int count, const NoiseTab &nt, hipStream_t st) {
     if (count <= 0) return;
-    DrawThresholds th = draw_thresholds(p);
     size_t lds = static_cast<size_t>(2) * g.wn * sizeof(uint32_t);
     hipLaunchKernelGGL(sample_kernel, dim3(count), dim3(64), lds, st, g, tp, base_seed,
-                       first_trial, th, d_per_trial);
+                       first_trial, nt);
 }
 
 void launch_syndromes(const DevGraph &g, TrialPlanes tp, int count, hipStream_t st) {
@@ -345,6 +388,13 @@ void launch_residual(const DevGraph &g, DevBasis bx, DevBasis bz, TrialPlanes tp
     if (count <= 0) return;
     size_t lds = static_cast<size_t>(3) * g.wn * sizeof(uint32_t);
     hipLaunchKernelGGL(residual_kernel, dim3(count), dim3(kResBlock), lds, st, g, bx, bz, tp);
+}
+
+void launch_logical(const DevGraph &g, DevLogicals lx, DevLogicals lz, TrialPlanes tp, int count,
+                    hipStream_t st) {
+    if (count <= 0) return;
+    size_t lds = static_cast<size_t>(2) * g.wn * sizeof(uint32_t);
+    hipLaunchKernelGGL(logical_kernel, dim3(count), dim3(kResBlock), lds, st, g, lx, lz, tp);
 }
 
 void launch_fill_i32(int *ptr, int value, int count, hipStream_t st) {

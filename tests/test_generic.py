@@ -1,0 +1,142 @@
+"""GPU: interleaved mixed-noise runs, the generic HBM-resident FP32 decoder,
+and hypergraph-product codes (success judged by logical operators)."""
+import numpy as np
+import pytest
+
+import joint_bp_plus_pp_amd as jbp
+
+pytestmark = pytest.mark.gpu
+
+PS = [0.01, 0.025, 0.04, 0.05]
+
+
+@pytest.fixture(scope="module")
+def small_hgp():
+    code = jbp.Code.hgp(15, 20, 3, 4, seed=2)
+    dec = jbp.Decoder(code, 0)
+    yield code, dec
+    dec.destroy()
+
+
+@pytest.fixture(scope="module")
+def big_hgp():
+    code = jbp.Code.hgp(150, 200, 3, 4, seed=1)
+    dec = jbp.Decoder(code, 0)
+    yield code, dec
+    dec.destroy()
+
+
+def per_trial_equal(a, b, keys=("iterations", "syn_match", "success", "exact")):
+    return all(np.array_equal(a[k], b[k]) for k in keys)
+
+
+@pytest.mark.parametrize("precision,engine", [(jbp.FP64, 0), (jbp.FP32, jbp.ENGINE_LDS),
+                                              (jbp.FP32, jbp.ENGINE_HBM)])
+def test_mixed_p_matches_single_p_runs(main_decoder, precision, engine):
+    # trial index idx uses PS[idx % 4]: each trial must equal the same index
+    # of a single-p run at its p (same seed, same draws, same decode)
+    first, n, it = 5, 512 if precision == jbp.FP64 else 2048, 60
+    print("mixed p", precision, engine, flush=True)
+    _, mixed = main_decoder.simulate(106, n, PS, max_iter=it, first_trial=first,
+                                     precision=precision, engine=engine, want_est=True)
+    idx = np.arange(first, first + n)
+    for j, p in enumerate(PS):
+        _, single = main_decoder.simulate(106, n, p, max_iter=it, first_trial=first,
+                                          precision=precision, engine=engine, want_est=True)
+        sel = idx % 4 == j
+        for k in ("iterations", "syn_match", "success", "exact", "est"):
+            assert np.array_equal(mixed[k][sel], single[k][sel]), (p, k)
+
+
+def test_mixed_p_fp64_matches_oracle(main_decoder, main_code_oracle):
+    n = 96
+    _, r = main_decoder.simulate(106, n, PS, max_iter=50, first_trial=1, precision=jbp.FP64)
+    for t in range(n):
+        idx = 1 + t
+        p = PS[idx % 4]
+        err = main_code_oracle.sample(106, idx, p)
+        sx, sz = main_code_oracle.syndrome(err)
+        est, it, ok = main_code_oracle.decode(sx, sz, jbp.prior_from_p(p), 50)
+        assert r["iterations"][t] == it and bool(r["syn_match"][t]) == ok
+        assert bool(r["success"][t]) == (ok and main_code_oracle.stab_equiv(err, est))
+
+
+def test_hbm_engine_is_slot_invariant(main_decoder):
+    # a trial's decode does not depend on the slot it ran in or on S
+    print("slot invariance", flush=True)
+    runs = [main_decoder.simulate(106, 3000, 0.05, max_iter=80, precision=jbp.FP32,
+                                  engine=jbp.ENGINE_HBM, slots=s, want_est=True)[1]
+            for s in (64, 640, 2048)]
+    for r in runs[1:]:
+        assert per_trial_equal(runs[0], r) and np.array_equal(runs[0]["est"], r["est"])
+
+
+def test_hbm_engine_agrees_with_lds_engine(main_decoder):
+    # two FP32 implementations of the same arithmetic in different operation
+    # order: per-trial outcomes agree except for rare trials near a decision
+    n = 8192
+    print("hbm vs lds", flush=True)
+    sh, rh = main_decoder.simulate(106, n, 0.05, max_iter=100, precision=jbp.FP32,
+                                   engine=jbp.ENGINE_HBM)
+    sl, rl = main_decoder.simulate(106, n, 0.05, max_iter=100, precision=jbp.FP32,
+                                   engine=jbp.ENGINE_LDS)
+    assert np.mean(rh["success"] == rl["success"]) > 0.995
+    assert np.mean(rh["iterations"] == rl["iterations"]) > 0.98
+    assert abs(sh["failures"] - sl["failures"]) <= 3 * np.sqrt(max(sl["failures"], 1)) + 3
+    print("failures hbm %d lds %d" % (sh["failures"], sl["failures"]), flush=True)
+
+
+def test_small_hgp_fp64_matches_oracle(small_hgp, oracle):
+    # FP64 decoder on a hypergraph-product code, success judged by logical
+    # operators on the device, against the oracle (its own row-space test)
+    code, dec = small_hgp
+    assert dec.info()["basis"] == 2  # logical operators installed
+    oc = oracle.code_from_checks(code.n, code.checks(0), code.checks(1))
+    n, p, it = 400, 0.06, 40
+    _, r = dec.simulate(11, n, p, max_iter=it, precision=jbp.FP64, want_est=True)
+    o = oc.simulate(11, 1, n, p, it, threads=8)
+    assert np.array_equal(r["iterations"], o["iterations"])
+    assert np.array_equal(r["syn_match"], o["syn_ok"])
+    assert np.array_equal(r["success"], o["success"])
+    assert np.array_equal(r["exact"], o["exact"])
+    assert 0 < int(np.sum(r["syn_match"] & (1 - r["exact"])))  # residual tests exercised
+
+
+def test_small_hgp_logicals_equal_row_basis(small_hgp):
+    code, dec = small_hgp
+    _, a = dec.simulate(12, 600, 0.07, max_iter=30, precision=jbp.FP64)
+    dec.set_logicals(None, None)
+    assert dec.info()["basis"] == 1
+    _, b = dec.simulate(12, 600, 0.07, max_iter=30, precision=jbp.FP64)
+    dec.set_logicals(code.logicals(0), code.logicals(1))
+    assert per_trial_equal(a, b)
+    assert np.any(a["syn_match"] & (1 - a["success"]))  # some logical failures seen
+
+
+def test_small_hgp_fp32_hbm_vs_fp64(small_hgp):
+    _, dec = small_hgp
+    n = 4096
+    s32, r32 = dec.simulate(13, n, 0.05, max_iter=60, precision=jbp.FP32)
+    s64, r64 = dec.simulate(13, n, 0.05, max_iter=60, precision=jbp.FP64)
+    assert np.mean(r32["success"] == r64["success"]) > 0.99
+    assert abs(s32["failures"] - s64["failures"]) <= 3 * np.sqrt(max(s64["failures"], 1)) + 3
+
+
+def test_big_hgp_hbm_engine(big_hgp):
+    # the 62,500-qubit product code at p = 0.03, max-iter 200
+    code, dec = big_hgp
+    assert dec.info()["basis"] == 2
+    n = 1024
+    print("big hgp fp32", flush=True)
+    s32, r32 = dec.simulate(21, n, 0.03, max_iter=200, precision=jbp.FP32)
+    assert s32["done"] == n
+    assert np.all((r32["iterations"] >= 0) & (r32["iterations"] <= 200))
+    assert np.all(r32["success"] <= r32["syn_match"])
+    assert np.all(r32["iterations"][r32["syn_match"] == 0] == 200)
+    print("big hgp fp64", flush=True)
+    m = 96
+    _, r64 = dec.simulate(21, m, 0.03, max_iter=200, precision=jbp.FP64)
+    agree = np.mean(r64["success"] == r32["success"][:m])
+    print("big hgp: FER fp32 %.4f, agreement with fp64 on %d trials %.3f"
+          % (s32["failures"] / n, m, agree), flush=True)
+    assert agree > 0.95
