@@ -184,7 +184,9 @@ int jbp_in_row_space(jbp_ctx *ctx, int side, int count, const int *bits, uint8_t
 
 /* Device time of the most recent launch of each kernel family, for
  * profiling: which = 0 sample, 1 decode, 2 residual check.  Returns ms and
- * the launch count since the context was created. */
+ * the launch count since the context was created.  which = 3: the decode
+ * time again, with the number of engine steps (one flooding iteration of
+ * every resident trial) of the last generic-decoder call in *launches. */
 double jbp_last_kernel_ms(jbp_ctx *ctx, int which, int64_t *launches);
 
 /* Context facts: out[0] = 1 when the LDS-resident FP32 decoder applies,

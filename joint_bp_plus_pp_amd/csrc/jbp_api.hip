@@ -668,6 +668,18 @@ int jbp_create(const jbp_graph_desc *g, int device_id, jbp_ctx **out, char *err,
         set_err(err, errlen, "degrees above 32 (check) / 16 (variable) are not supported");
         return fail(JBP_ERR_UNSUPPORTED);
     }
+    dg.min_cdeg = dg.max_cdeg;
+    dg.vpad_w = dg.max_vdeg <= 4 ? 4 : dg.max_vdeg <= 8 ? 8 : 16;
+    for (int s = 0; s < 2; ++s) {
+        const HostSide &h = ctx->hs[s];
+        for (int c = 0; c < h.m; ++c) dg.min_cdeg = std::min(dg.min_cdeg, h.chk_off[c + 1] - h.chk_off[c]);
+        const int W = dg.vpad_w;
+        std::vector<int> pad(static_cast<size_t>(ctx->n) * W, h.E);  // E: the zero message
+        for (int v = 0; v < ctx->n; ++v)
+            for (int i = h.var_off[v]; i < h.var_off[v + 1]; ++i)
+                pad[static_cast<size_t>(v) * W + (i - h.var_off[v])] = h.var_edge[i];
+        if ((rc = dev_upload(ctx, pad, &dg.vpad[s], err, errlen))) return fail(rc);
+    }
     ctx->qc = detect_qc(ctx->hs, ctx->n, ctx->qcp) &&
               qc_f32_supported(ctx->qcp, device_id, nullptr, 0);
     if (ctx->qc) {
@@ -743,7 +755,11 @@ int jbp_info(jbp_ctx *ctx, int64_t out[8]) {
 }
 
 double jbp_last_kernel_ms(jbp_ctx *ctx, int which, int64_t *launches) {
-    if (!ctx || which < 0 || which > 2) return -1.0;
+    if (!ctx || which < 0 || which > 3) return -1.0;
+    if (which == 3) {  // generic decoder: engine steps of its last decode call
+        if (launches) *launches = ctx->gen_steps;
+        return ctx->last_ms[1];
+    }
     if (launches) *launches = ctx->launches[which];
     return ctx->last_ms[which];
 }

@@ -23,7 +23,13 @@ struct DevGraph {
     int wn = 0;                // 32-bit words of an n-bit vector
     int wm[2] = {0, 0};        // 32-bit words of an m-bit vector per side
     int max_cdeg = 0, max_vdeg = 0;
+    int min_cdeg = 0;
     DevSide side[2];
+    // variable -> edge lists padded to vpad_w entries (4, 8 or 16) with the
+    // side's edge count E, which the generic FP32 decoder stores as an
+    // always-zero message: [n][vpad_w] per side
+    int vpad_w = 0;
+    const int *vpad[2] = {nullptr, nullptr};
 };
 
 // Quasi-cyclic structure: check (r*P + x) of side g meets variable

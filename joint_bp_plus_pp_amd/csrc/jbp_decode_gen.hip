@@ -59,43 +59,75 @@ __device__ __forceinline__ PriorF slot_prior(const GenState &s, int group, int l
     return pr;
 }
 
-// grid (ceil(n / (4*VPW)), groups), block 256: wave w handles VPW variables.
-template <int VPW>
+// grid (ceil(n / (4*VPW)), groups), block 256: wave w handles VPW
+// variables; their edge lists come from the padded table (MAXV entries per
+// variable and side, -1 = none), so every gather of a variable is issued at
+// once.
+template <int VPW, int MAXV>
 __global__ void __launch_bounds__(256) gen_var_kernel(DevGraph g, GenState s) {
     const int group = blockIdx.y;
     const uint64_t act = uni64(s.active[group]);
     if (!act) return;
     const int lane = threadIdx.x & 63;
     const bool fresh = (uni64(s.fresh[group]) >> lane) & 1u;
+    const uint32_t keep = fresh ? 0u : ~0u;  // a fresh slot reads its old messages as 0
     const PriorF pr = slot_prior(s, group, lane);
     const int v0 = (blockIdx.x * 4 + wave_id()) * VPW;
     const int v1 = min(v0 + VPW, g.n);
     const size_t E0 = g.side[0].E, E1 = g.side[1].E;
-    const float *lam0 = s.lam[0] + static_cast<size_t>(group) * E0 * 64 + lane;
-    const float *lam1 = s.lam[1] + static_cast<size_t>(group) * E1 * 64 + lane;
+    const float *lam0 = s.lam[0] + static_cast<size_t>(group) * (E0 + 1) * 64 + lane;
+    const float *lam1 = s.lam[1] + static_cast<size_t>(group) * (E1 + 1) * 64 + lane;
     float *gb0 = s.gb[0] + static_cast<size_t>(group) * g.n * 64 + lane;
     float *gb1 = s.gb[1] + static_cast<size_t>(group) * g.n * 64 + lane;
     uint64_t *ew0 = s.estw[0] + static_cast<size_t>(group) * g.n;
     uint64_t *ew1 = s.estw[1] + static_cast<size_t>(group) * g.n;
-    for (int v = v0; v < v1; ++v) {
-        float lx = 0.0f, lz = 0.0f;
-        const int a0 = g.side[0].var_off[v], b0 = g.side[0].var_off[v + 1];
-        for (int i = a0; i < b0; ++i) lx += lam0[static_cast<size_t>(g.side[0].var_edge[i]) * 64];
-        const int a1 = g.side[1].var_off[v], b1 = g.side[1].var_off[v + 1];
-        for (int i = a1; i < b1; ++i) lz += lam1[static_cast<size_t>(g.side[1].var_edge[i]) * 64];
-        if (fresh) {
-            lx = 0.0f;
-            lz = 0.0f;
+    // Batches of VB variables: all gathers of a batch before its stores (the
+    // stores to gb cannot be proven disjoint from the next loads of lam).
+    constexpr int VB = 4;
+    for (int vb = v0; vb < v1; vb += VB) {
+        float lx[VB], lz[VB];
+#pragma unroll
+        for (int j = 0; j < VB; ++j) {
+            const int v = min(vb + j, v1 - 1);  // a short last batch repeats its final variable
+            const int *p0 = g.vpad[0] + static_cast<size_t>(v) * MAXV;
+            const int *p1 = g.vpad[1] + static_cast<size_t>(v) * MAXV;
+            // padding entries point at edge E, whose message is always 0
+            int i0[MAXV], i1[MAXV];
+#pragma unroll
+            for (int k = 0; k < MAXV; ++k) {
+                i0[k] = p0[k];
+                i1[k] = p1[k];
+            }
+            float x[MAXV], z[MAXV];
+#pragma unroll
+            for (int k = 0; k < MAXV; ++k) {
+                x[k] = lam0[static_cast<size_t>(i0[k]) * 64];
+                z[k] = lam1[static_cast<size_t>(i1[k]) * 64];
+            }
+            float sx = 0.0f, sz = 0.0f;
+#pragma unroll
+            for (int k = 0; k < MAXV; ++k) {
+                sx += x[k];
+                sz += z[k];
+            }
+            lx[j] = __uint_as_float(__float_as_uint(sx) & keep);
+            lz[j] = __uint_as_float(__float_as_uint(sz) & keep);
         }
-        const int best = hard_decision(lx, lz, pr);
-        const uint64_t bx = __ballot(best & 1);
-        const uint64_t bz = __ballot(best >> 1);
-        if (lane == 0) {
-            ew0[v] = bx;
-            ew1[v] = bz;
+#pragma unroll
+        for (int j = 0; j < VB; ++j) {
+            const int v = vb + j;
+            if (v < v1) {
+                const int best = hard_decision(lx[j], lz[j], pr);
+                const uint64_t bx = __ballot(best & 1);
+                const uint64_t bz = __ballot(best >> 1);
+                if (lane == 0) {
+                    ew0[v] = bx;
+                    ew1[v] = bz;
+                }
+                gb0[static_cast<size_t>(v) * 64] = lx[j] + extrinsic_term(lz[j], pr.fx);
+                gb1[static_cast<size_t>(v) * 64] = lz[j] + extrinsic_term(lx[j], pr.fz);
+            }
         }
-        gb0[static_cast<size_t>(v) * 64] = lx + extrinsic_term(lz, pr.fx);
-        gb1[static_cast<size_t>(v) * 64] = lz + extrinsic_term(lx, pr.fz);
     }
 }
 
@@ -138,7 +170,9 @@ __device__ __forceinline__ void check_update(float (&mu)[MAXC], int deg, uint32_
 
 // grid (ceil(chunks / 4), groups), block 256; chunk q < nch0 covers X checks
 // [q*CPW, (q+1)*CPW), the rest Z checks.
-template <int MAXC, int CPW>
+// DEG > 0: every check has exactly DEG edges (no per-edge guards, so all
+// loads of a check are issued together); DEG = 0: up to MAXC edges.
+template <int MAXC, int CPW, int DEG>
 __global__ void __launch_bounds__(256) gen_check_kernel(DevGraph g, GenState s, int nch0, int nch) {
     const int group = blockIdx.y;
     const uint64_t act = uni64(s.active[group]);
@@ -147,39 +181,84 @@ __global__ void __launch_bounds__(256) gen_check_kernel(DevGraph g, GenState s, 
     if (q >= nch) return;
     const int lane = threadIdx.x & 63;
     const bool fresh = (uni64(s.fresh[group]) >> lane) & 1u;
+    const uint32_t keep = fresh ? 0u : ~0u;  // a fresh slot reads its old messages as 0
     const int side = q < nch0 ? 0 : 1;
     const DevSide sd = side == 0 ? g.side[0] : g.side[1];
     const int c0 = (side == 0 ? q : q - nch0) * CPW;
     const int c1 = min(c0 + CPW, sd.m);
-    float *lam = (side == 0 ? s.lam[0] : s.lam[1]) + static_cast<size_t>(group) * sd.E * 64 + lane;
+    float *lam = (side == 0 ? s.lam[0] : s.lam[1]) + static_cast<size_t>(group) * (sd.E + 1) * 64 + lane;
     const float *gb = (side == 0 ? s.gb[0] : s.gb[1]) + static_cast<size_t>(group) * g.n * 64 + lane;
     const uint64_t *ew = (side == 0 ? s.estw[0] : s.estw[1]) + static_cast<size_t>(group) * g.n;
     const uint64_t *sw = (side == 0 ? s.synw[0] : s.synw[1]) + static_cast<size_t>(group) * sd.m;
     uint64_t bad = 0;
-    for (int c = c0; c < c1; ++c) {
-        const int e0 = sd.chk_off[c];
-        const int deg = sd.chk_off[c + 1] - e0;
-        float mu[MAXC];
-        uint64_t par = 0;
+    if constexpr (DEG > 0) {
+        // Load phase for all CPW checks, then update and store: stores of one
+        // check cannot be proven disjoint from the next check's loads, so
+        // interleaving them would serialize the gathers.
+        float mu[CPW][DEG];
+        uint64_t par[CPW];
+        int e0s[CPW];
+        int ev[CPW][DEG];
 #pragma unroll
-        for (int i = 0; i < MAXC; ++i) {
-            if (i < deg) {
-                const int v = sd.edge_var[e0 + i];
-                par ^= ew[v];
-                const float old = fresh ? 0.0f : lam[static_cast<size_t>(e0 + i) * 64];
-                mu[i] = gb[static_cast<size_t>(v) * 64] - old;
+        for (int j = 0; j < CPW; ++j)
+            e0s[j] = sd.chk_off[min(c0 + j, sd.m - 1)];  // a short last chunk repeats its final check (not stored)
+#pragma unroll
+        for (int j = 0; j < CPW; ++j)
+#pragma unroll
+            for (int i = 0; i < DEG; ++i) ev[j][i] = sd.edge_var[e0s[j] + i];
+#pragma unroll
+        for (int j = 0; j < CPW; ++j) {
+            uint64_t pj = 0;
+#pragma unroll
+            for (int i = 0; i < DEG; ++i) {
+                const int v = ev[j][i];
+                pj ^= ew[v];
+                const float old = __uint_as_float(
+                    __float_as_uint(lam[static_cast<size_t>(e0s[j] + i) * 64]) & keep);
+                mu[j][i] = gb[static_cast<size_t>(v) * 64] - old;
+            }
+            par[j] = pj;
+        }
+#pragma unroll
+        for (int j = 0; j < CPW; ++j) {
+            if (c0 + j < c1) {
+                const uint64_t syn = sw[c0 + j];
+                bad |= par[j] ^ syn;
+                check_update<DEG>(mu[j], DEG, static_cast<uint32_t>((syn >> lane) & 1u));
+#pragma unroll
+                for (int i = 0; i < DEG; ++i) lam[static_cast<size_t>(e0s[j] + i) * 64] = mu[j][i];
             }
         }
-        const uint64_t syn = sw[c];
-        bad |= par ^ syn;
-        check_update<MAXC>(mu, deg, static_cast<uint32_t>((syn >> lane) & 1u));
+    } else {
+        for (int c = c0; c < c1; ++c) {
+            const int e0 = sd.chk_off[c];
+            const int deg = sd.chk_off[c + 1] - e0;
+            float mu[MAXC];
+            uint64_t par = 0;
 #pragma unroll
-        for (int i = 0; i < MAXC; ++i)
-            if (i < deg) lam[static_cast<size_t>(e0 + i) * 64] = mu[i];
+            for (int i = 0; i < MAXC; ++i) {
+                if (i < deg) {
+                    const int v = sd.edge_var[e0 + i];
+                    par ^= ew[v];
+                    const float old = __uint_as_float(
+                        __float_as_uint(lam[static_cast<size_t>(e0 + i) * 64]) & keep);
+                    mu[i] = gb[static_cast<size_t>(v) * 64] - old;
+                }
+            }
+            const uint64_t syn = sw[c];
+            bad |= par ^ syn;
+            check_update<MAXC>(mu, deg, static_cast<uint32_t>((syn >> lane) & 1u));
+#pragma unroll
+            for (int i = 0; i < MAXC; ++i)
+                if (i < deg) lam[static_cast<size_t>(e0 + i) * 64] = mu[i];
+        }
     }
+    // one of kBadWays words per group, chosen by block, so the wavefronts of
+    // a group do not all update one address; the book pass ORs them
     bad = uni64(bad);
-    if (bad && lane == 0) atomicOr(reinterpret_cast<unsigned long long *>(s.bad + group),
-                                   static_cast<unsigned long long>(bad));
+    if (bad && lane == 0)
+        atomicOr(reinterpret_cast<unsigned long long *>(s.bad + group * kBadWays + blockIdx.x % kBadWays),
+                 static_cast<unsigned long long>(bad));
 }
 
 // grid = groups, block 64 (lane = slot).  Decides which slots stop, records
@@ -191,7 +270,15 @@ __global__ void __launch_bounds__(64) gen_book_kernel(GenState s, TrialPlanes tp
     const int lane = threadIdx.x;
     const int slot = group * 64 + lane;
     const uint64_t act = s.active[group];
-    const uint64_t badm = s.bad[group];
+    // OR of the group's kBadWays words (lane i reads word i), then clear them
+    uint64_t badm = s.bad[group * kBadWays + lane];
+    s.bad[group * kBadWays + lane] = 0;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const uint32_t lo = __shfl_xor(static_cast<uint32_t>(badm), m);
+        const uint32_t hi = __shfl_xor(static_cast<uint32_t>(badm >> 32), m);
+        badm |= (static_cast<uint64_t>(hi) << 32) | lo;
+    }
     const bool mine = (act >> lane) & 1u;
     bool fin = false;
     int matched = 0;
@@ -250,29 +337,31 @@ __global__ void __launch_bounds__(64) gen_book_kernel(GenState s, TrialPlanes tp
         s.active[group] = newact;
         s.fresh[group] = refill;
         s.refill[group] = refill;
-        s.bad[group] = 0;
     }
 }
 
-// grid = any, block 64: one stopped trial per block iteration; its hard
-// decisions (bit `slot % 64` of every estimate word) become plane words.
-__global__ void __launch_bounds__(64) gen_emit_kernel(DevGraph g, GenState s, TrialPlanes tp) {
+// grid (ceil(ceil(n/64) / 4), kEmitRows), block 256: wave w of block x
+// turns variables [64 (4x + w), +64) of stopped trials y, y + kEmitRows, ...
+// into plane words (bit `slot % 64` of each estimate word, one ballot).
+constexpr int kEmitRows = 32;
+__global__ void __launch_bounds__(256) gen_emit_kernel(DevGraph g, GenState s, TrialPlanes tp) {
     const int nfin = s.ctrl[2];
-    const int lane = threadIdx.x;
-    for (int i = blockIdx.x; i < nfin; i += gridDim.x) {
+    if (static_cast<int>(blockIdx.y) >= nfin) return;
+    const int lane = threadIdx.x & 63;
+    const int v0 = (blockIdx.x * 4 + wave_id()) * 64;
+    if (v0 >= g.n) return;
+    const int v = v0 + lane;
+    const int wd = (v0 >> 5) + lane;
+    for (int i = blockIdx.y; i < nfin; i += kEmitRows) {
         const int2 f = s.fin[i];
         const int group = f.y >> 6, sl = f.y & 63;
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            const uint64_t *ew = s.estw[side] + static_cast<size_t>(group) * g.n;
-            uint32_t *out = (side == 0 ? tp.est_x : tp.est_z) + static_cast<size_t>(f.x) * g.wn;
-            for (int v0 = 0; v0 < g.n; v0 += 64) {
-                const int v = v0 + lane;
-                const uint64_t w = v < g.n ? ew[v] : 0ull;
-                const uint64_t bits = __ballot((w >> sl) & 1ull);
-                const int wd = (v0 >> 5) + lane;
-                if (lane < 2 && wd < g.wn) out[wd] = static_cast<uint32_t>(bits >> (32 * lane));
-            }
+        const uint64_t wx = v < g.n ? s.estw[0][static_cast<size_t>(group) * g.n + v] : 0ull;
+        const uint64_t wz = v < g.n ? s.estw[1][static_cast<size_t>(group) * g.n + v] : 0ull;
+        const uint64_t bx = __ballot((wx >> sl) & 1ull);
+        const uint64_t bz = __ballot((wz >> sl) & 1ull);
+        if (lane < 2 && wd < g.wn) {
+            tp.est_x[static_cast<size_t>(f.x) * g.wn + wd] = static_cast<uint32_t>(bx >> (32 * lane));
+            tp.est_z[static_cast<size_t>(f.x) * g.wn + wd] = static_cast<uint32_t>(bz >> (32 * lane));
         }
     }
 }
@@ -310,13 +399,47 @@ __global__ void __launch_bounds__(64) gen_load_kernel(DevGraph g, GenState s, Tr
     }
 }
 
-template <int MAXC>
+// grid = groups, block 64: the extra edge E of each group and side, which
+// padded variable entries read, holds message 0.
+__global__ void __launch_bounds__(64) gen_zero_kernel(DevGraph g, GenState s) {
+    const int group = blockIdx.x;
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const size_t E = g.side[k].E;
+        s.lam[k][(static_cast<size_t>(group) * (E + 1) + E) * 64 + lane] = 0.0f;
+    }
+}
+
+template <int MAXC, int DEG>
 void launch_check(const DevGraph &g, const GenState &s, hipStream_t st) {
-    constexpr int CPW = 4;
+    constexpr int CPW = (DEG == 0 || DEG <= 8) ? 4 : 2;  // registers: CPW*DEG messages in flight
     const int nch0 = (g.side[0].m + CPW - 1) / CPW;
     const int nch = nch0 + (g.side[1].m + CPW - 1) / CPW;
-    hipLaunchKernelGGL((gen_check_kernel<MAXC, CPW>), dim3((nch + 3) / 4, s.groups), dim3(256), 0,
-                       st, g, s, nch0, nch);
+    hipLaunchKernelGGL((gen_check_kernel<MAXC, CPW, DEG>), dim3((nch + 3) / 4, s.groups), dim3(256),
+                       0, st, g, s, nch0, nch);
+}
+
+void launch_check_any(const DevGraph &g, const GenState &s, hipStream_t st) {
+    if (g.min_cdeg == g.max_cdeg) {
+        switch (g.max_cdeg) {
+            case 6: return launch_check<6, 6>(g, s, st);
+            case 7: return launch_check<7, 7>(g, s, st);
+            case 8: return launch_check<8, 8>(g, s, st);
+            case 12: return launch_check<12, 12>(g, s, st);
+            default: break;
+        }
+    }
+    if (g.max_cdeg <= 8) launch_check<8, 0>(g, s, st);
+    else if (g.max_cdeg <= 16) launch_check<16, 0>(g, s, st);
+    else launch_check<32, 0>(g, s, st);
+}
+
+void launch_var(const DevGraph &g, const GenState &s, hipStream_t st) {
+    const dim3 grid((g.n + 4 * 16 - 1) / (4 * 16), s.groups);
+    if (g.vpad_w == 4) hipLaunchKernelGGL((gen_var_kernel<16, 4>), grid, dim3(256), 0, st, g, s);
+    else if (g.vpad_w == 8) hipLaunchKernelGGL((gen_var_kernel<16, 8>), grid, dim3(256), 0, st, g, s);
+    else hipLaunchKernelGGL((gen_var_kernel<16, 16>), grid, dim3(256), 0, st, g, s);
 }
 
 }  // namespace
@@ -324,11 +447,11 @@ void launch_check(const DevGraph &g, const GenState &s, hipStream_t st) {
 size_t gen_state_bytes(const DevGraph &g, int groups) {
     const size_t G = static_cast<size_t>(groups), S = 64 * G;
     size_t b = 0;
-    b += S * (static_cast<size_t>(g.side[0].E) + g.side[1].E) * sizeof(float);  // lam
+    b += S * (static_cast<size_t>(g.side[0].E) + g.side[1].E + 2) * sizeof(float);  // lam (+ zero edge)
     b += 2 * S * static_cast<size_t>(g.n) * sizeof(float);                      // gb
     b += 2 * G * static_cast<size_t>(g.n) * 8;                                   // estw
     b += G * (static_cast<size_t>(g.side[0].m) + g.side[1].m) * 8;               // synw
-    b += 4 * G * 8 + 2 * S * 4 + S * kPrior32 * 4 + S * 8 + 64;
+    b += (kBadWays + 3) * G * 8 + 2 * S * 4 + S * kPrior32 * 4 + S * 8 + 64;
     return b;
 }
 
@@ -343,15 +466,15 @@ void gen_carve(const DevGraph &g, int groups, void *base, GenState &s) {
         return q;
     };
     s.groups = groups;
-    for (int k = 0; k < 2; ++k) s.lam[k] = reinterpret_cast<float *>(take(S * g.side[k].E * 4));
+    for (int k = 0; k < 2; ++k) s.lam[k] = reinterpret_cast<float *>(take(S * (g.side[k].E + 1) * 4));
     for (int k = 0; k < 2; ++k) s.gb[k] = reinterpret_cast<float *>(take(S * g.n * 4));
     for (int k = 0; k < 2; ++k) s.estw[k] = reinterpret_cast<uint64_t *>(take(G * g.n * 8));
     for (int k = 0; k < 2; ++k) s.synw[k] = reinterpret_cast<uint64_t *>(take(G * g.side[k].m * 8));
     // bad | active | fresh | refill, contiguous (cleared by one fill)
-    s.bad = reinterpret_cast<uint64_t *>(take(4 * G * 8));
-    s.active = s.bad + G;
-    s.fresh = s.bad + 2 * G;
-    s.refill = s.bad + 3 * G;
+    s.bad = reinterpret_cast<uint64_t *>(take((kBadWays + 3) * G * 8));
+    s.active = s.bad + kBadWays * G;
+    s.fresh = s.active + G;
+    s.refill = s.fresh + G;
     s.slot_trial = reinterpret_cast<int *>(take(S * 4));
     s.slot_iter = reinterpret_cast<int *>(take(S * 4));
     s.prior32 = reinterpret_cast<float *>(take(S * kPrior32 * 4));
@@ -372,12 +495,12 @@ int gen_decode(const DevGraph &g, const GenState &s, TrialPlanes tp, int count, 
     const int G = s.groups;
     // reset: every slot empty, counters zero (a fill kernel, not a memset,
     // so the sequence can be captured in a graph)
-    launch_fill_i32(reinterpret_cast<int *>(s.bad), 0, 4 * 2 * G, st);  // bad, active, fresh, refill
+    launch_fill_i32(reinterpret_cast<int *>(s.bad), 0, 2 * (kBadWays + 3) * G, st);  // bad, active, fresh, refill
     launch_fill_i32(s.ctrl, 0, 16, st);
+    hipLaunchKernelGGL(gen_zero_kernel, dim3(G), dim3(64), 0, st, g, s);
     const int wmax = std::max(g.wm[0], g.wm[1]);
     const dim3 load_grid((wmax + kLoadWords - 1) / kLoadWords, G);
-    const dim3 var_grid((g.n + 4 * 16 - 1) / (4 * 16), G);
-    const int emit_grid = std::min(64 * G, 2048);
+    const dim3 emit_grid(((g.n + 63) / 64 + 3) / 4, kEmitRows);
     interval = std::max(1, interval);
     warmup = std::max(0, warmup);
     hipLaunchKernelGGL(gen_book_kernel, dim3(G), dim3(64), 0, st, s, tp, count, nt, max_iter,
@@ -391,13 +514,11 @@ int gen_decode(const DevGraph &g, const GenState &s, TrialPlanes tp, int count, 
     int pend = -1;  // event slot whose copy is pending
     for (;;) {
         for (int b = 0; b < kBatch; ++b) {
-            hipLaunchKernelGGL((gen_var_kernel<16>), var_grid, dim3(256), 0, st, g, s);
-            if (g.max_cdeg <= 8) launch_check<8>(g, s, st);
-            else if (g.max_cdeg <= 16) launch_check<16>(g, s, st);
-            else launch_check<32>(g, s, st);
+            launch_var(g, s, st);
+            launch_check_any(g, s, st);
             hipLaunchKernelGGL(gen_book_kernel, dim3(G), dim3(64), 0, st, s, tp, count, nt,
                                max_iter, warmup, interval);
-            hipLaunchKernelGGL(gen_emit_kernel, dim3(emit_grid), dim3(64), 0, st, g, s, tp);
+            hipLaunchKernelGGL(gen_emit_kernel, emit_grid, dim3(256), 0, st, g, s, tp);
             hipLaunchKernelGGL(gen_load_kernel, load_grid, dim3(64), 0, st, g, s, tp);
         }
         steps += kBatch;

@@ -7,13 +7,17 @@
 
 namespace jbp {
 
+// Words per group of the unsatisfied-slot mask (spread to limit contention
+// of the check pass's atomics; one per lane of the book pass).
+constexpr int kBadWays = 64;
+
 struct GenState {
     int groups = 0;
-    float *lam[2] = {nullptr, nullptr};      // [groups][E_side][64] check messages
+    float *lam[2] = {nullptr, nullptr};      // [groups][E_side + 1][64] check messages (edge E: 0)
     float *gb[2] = {nullptr, nullptr};       // [groups][n][64] G_x, G_z per variable
     uint64_t *estw[2] = {nullptr, nullptr};  // [groups][n] hard-decision bits
     uint64_t *synw[2] = {nullptr, nullptr};  // [groups][m_side] syndrome bits
-    uint64_t *bad = nullptr;     // [groups] slots with an unsatisfied check this step
+    uint64_t *bad = nullptr;     // [groups][kBadWays] slots with an unsatisfied check (OR of the ways)
     uint64_t *active = nullptr;  // [groups] slots holding a trial (bad..refill contiguous)
     uint64_t *fresh = nullptr;   // [groups] slots whose messages read as 0 this step
     uint64_t *refill = nullptr;  // [groups] slots given a trial by the last book pass
