@@ -34,6 +34,8 @@ EDGES_PER_ITER = 2 * 27648
 BYTES_PER_ITER = 16 * EDGES_PER_ITER  # 884,736
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "decode_qc_f32_p0.04_traffic.json")
+LEARNED_FILE = os.path.join(ROOT, "joint_bp_plus_pp_amd", "codes", "learned_T25_P768_J3_L12.npz")
+MIXED_PS = (0.01, 0.025, 0.04, 0.05)
 
 
 def wilson(n, k, z=1.96):
@@ -95,6 +97,139 @@ def cpu_baseline(args, first_trial):
                                        args.max_iter, secs, iters / count, count - sum(su))}
 
 
+def learned_vs_plain(jbp, dec, args, first, plain):
+    """The learned decoder (shipped weights, T learned iterations then plain
+    BP restarted from the prior) on the first timed trial indices, beside
+    plain BP on the same indices."""
+    if not os.path.exists(LEARNED_FILE):
+        return None
+    from joint_bp_plus_pp_amd import learned as jl
+    model = jl.load(dec, LEARNED_FILE)
+    nxt, done, fails, iters, dev = first, 0, 0, 0, 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.learned_steps):
+        st, _ = dec.simulate_learned(model, args.seed, args.trials, args.p, max_iter=args.max_iter,
+                                     first_trial=nxt, per_trial=False)
+        nxt += args.trials
+        done += st["done"]
+        fails += st["failures"]
+        iters += st["total_iters"]
+        dev += st["device_ms"]
+    el = time.perf_counter() - t0
+    model.destroy()
+    lo, hi = wilson(done, fails)
+    p_done, p_fails, p_el = plain
+    plo, phi = wilson(p_done, p_fails)
+    return {"weights": os.path.relpath(LEARNED_FILE, ROOT), "T": model.T, "trials": done,
+            "trials_per_s": round(done / el, 2), "fer": fails / done, "fer_ci95": [lo, hi],
+            "avg_iter": round(iters / done, 4), "decode_ms_per_launch": round(dev / args.learned_steps, 4),
+            "plain_same_trials": {"trials": p_done, "trials_per_s": round(p_done / p_el, 2),
+                                  "fer": p_fails / p_done, "fer_ci95": [plo, phi]},
+            "speedup_vs_plain": round((done / el) / (p_done / p_el), 4)}
+
+
+def mixed_workload(jbp, qc_dec, args):
+    """Interleaved p over the included code (trial idx uses p[idx % 4]) and
+    the seeded hypergraph-product code HGP(150 x 200, (3,4)-regular),
+    n = 62,500, on the HBM-resident decoder (slots autotuned first)."""
+    out = {}
+    n_qc = 4 * args.mixed_trials
+    t0 = time.perf_counter()
+    st, r = qc_dec.simulate(args.seed, n_qc, list(MIXED_PS), max_iter=args.mixed_max_iter,
+                            first_trial=1, precision=32)
+    el = time.perf_counter() - t0
+    import numpy as np
+    idx = np.arange(1, n_qc + 1)
+    per_p = {}
+    for j, p in enumerate(MIXED_PS):
+        sel = idx % 4 == j
+        k = int(np.sum(1 - r["success"][sel]))
+        per_p["%g" % p] = {"trials": int(sel.sum()), "failures": k,
+                           "avg_iter": round(float(r["iterations"][sel].mean()), 4)}
+    out["qc_interleaved"] = {"trials": n_qc, "max_iter": args.mixed_max_iter,
+                             "trials_per_s": round(n_qc / el, 2),
+                             "decode_ms": round(st["device_ms"], 4), "per_p": per_p}
+    code = jbp.Code.hgp(150, 200, 3, 4, seed=1)
+    hdec = jbp.Decoder(code, 0)
+    n_h = args.hgp_trials
+    slots, _, cand = hdec.tune(args.seed + 1, n_h, args.hgp_p, max_iter=args.mixed_max_iter,
+                               precision=32)
+    t0 = time.perf_counter()
+    st, _ = hdec.simulate(args.seed, n_h, args.hgp_p, max_iter=args.mixed_max_iter, first_trial=1,
+                          precision=32, per_trial=False)
+    el = time.perf_counter() - t0
+    bytes_iter = 16 * (code.x_edges + code.z_edges)
+    ach = st["total_iters"] * bytes_iter / (st["device_ms"] / 1e3) / 1e9
+    lo, hi = wilson(st["done"], st["failures"])
+    out["hgp"] = {"code": "HGP(150x200,(3,4)-regular, seed 1): n=%d, %d+%d checks, %d edges/side"
+                          % (code.n, code.mx, code.mz, code.x_edges),
+                  "p": args.hgp_p, "trials": st["done"], "max_iter": args.mixed_max_iter,
+                  "trials_per_s": round(st["done"] / el, 2),
+                  "trial_iterations_per_s": round(st["total_iters"] / el, 1),
+                  "avg_iter": round(st["total_iters"] / st["done"], 4),
+                  "fer": st["failures"] / st["done"], "fer_ci95": [lo, hi],
+                  "bp_failures": st["bp_failures"], "tuned_slots": slots,
+                  "tune_ms": [round(x, 3) for x in cand],
+                  "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                               "bytes_per_trial_iteration": bytes_iter,
+                               "kernels": "gen_var_kernel + gen_check_kernel (+ book/emit/load)"}}
+    hdec.destroy()
+    code.close()
+    return out
+
+
+def bench_hgp(jbp, args):
+    """Steps of --trials trials of the hypergraph-product code at --hgp-p,
+    max-iter --mixed-max-iter, on the generic decoder after one untimed
+    tuning pass (resident slots)."""
+    code = jbp.Code.hgp(150, 200, 3, 4, seed=1)
+    dec = jbp.Decoder(code, 0)
+    trials = args.hgp_trials
+    slots, _, cand = dec.tune(args.seed + 1, trials, args.hgp_p, max_iter=args.mixed_max_iter,
+                              precision=32)
+    nxt = 1
+    for _ in range(args.warmup):
+        dec.simulate(args.seed, trials, args.hgp_p, max_iter=args.mixed_max_iter, first_trial=nxt,
+                     precision=32, per_trial=False)
+        nxt += trials
+    done = fails = iters = 0
+    dev = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st, _ = dec.simulate(args.seed, trials, args.hgp_p, max_iter=args.mixed_max_iter,
+                             first_trial=nxt, precision=32, per_trial=False)
+        nxt += trials
+        done += st["done"]
+        fails += st["failures"]
+        iters += st["total_iters"]
+        dev += st["device_ms"]
+    el = time.perf_counter() - t0
+    bytes_iter = 16 * (code.x_edges + code.z_edges)
+    ach = iters * bytes_iter / (dev / 1e3) / 1e9
+    lo, hi = wilson(done, fails)
+    line = {
+        "metric": "decoded trials/s, HGP(150x200) code, p=%g" % args.hgp_p,
+        "value": round(done / el, 2), "unit": "trials/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "plain joint BP, HGP(150x200,(3,4)-regular, seed 1), n=%d, p=%g, "
+                               "%d trials per step, max-iter %d, generic HBM-resident decoder, "
+                               "%d resident slots (tuned)" % (code.n, args.hgp_p, trials,
+                                                              args.mixed_max_iter, slots),
+                   "avg_iter": round(iters / done, 4), "fer": fails / done, "fer_ci95": [lo, hi],
+                   "trial_iterations_per_s": round(iters / el, 1), "tune_ms": cand},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "gen_check_kernel + gen_var_kernel (one engine step each)",
+                     "bytes_per_trial_iteration": bytes_iter},
+        "cpu_baseline": None,
+    }
+    dec.destroy()
+    print(json.dumps(line), flush=True)
+
+
 def copy_calibration(mib):
     """One hipMemcpy device-to-device of `mib` MiB (the runtime's copy kernel)."""
     hip = C.CDLL("libamdhip64.so")
@@ -113,6 +248,9 @@ def copy_calibration(mib):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=("qc", "hgp"), default="qc",
+                    help="qc: the included code (default, the headline); hgp: the seeded "
+                         "hypergraph-product code HGP(150x200) on the HBM-resident decoder")
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--trials", type=int, default=8192, help="trials per step")
@@ -122,6 +260,13 @@ def main():
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--cpu-trials", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--learned-steps", type=int, default=20,
+                    help="steps of the learned decoder on the first timed trial indices (0: skip)")
+    ap.add_argument("--mixed-trials", type=int, default=4096, help="trials per p of the mixed run")
+    ap.add_argument("--mixed-max-iter", type=int, default=200)
+    ap.add_argument("--hgp-trials", type=int, default=2048)
+    ap.add_argument("--hgp-p", type=float, default=0.03)
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a plain copy "
                          "kernel of known size, to scale profiler byte counters)")
@@ -135,6 +280,8 @@ def main():
 
     if args.copy_calibration_mb > 0:
         copy_calibration(args.copy_calibration_mb)
+    if args.workload == "hgp":
+        return bench_hgp(jbp, args)
     code = jbp.Code.from_params(PARAMS)
     dec = jbp.Decoder(code, 0)
     info = dec.info()
@@ -147,8 +294,10 @@ def main():
     done = fails = iters = 0
     dev_ms = []
     step_iters = []
+    head = [0, 0, 0.0]  # plain trials, failures, seconds over the first learned_steps steps
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for step in range(args.steps):
+        ts = time.perf_counter()
         st, _ = dec.simulate(args.seed, args.trials, args.p, max_iter=args.max_iter, first_trial=nxt,
                              precision=args.precision, per_trial=False)
         nxt += args.trials
@@ -157,6 +306,10 @@ def main():
         iters += st["total_iters"]
         dev_ms.append(st["device_ms"])
         step_iters.append(st["total_iters"])
+        if step < args.learned_steps:
+            head[0] += st["done"]
+            head[1] += st["failures"]
+            head[2] += time.perf_counter() - ts
     elapsed = time.perf_counter() - t0
     value = done / elapsed
     # decoder kernel: one launch per step (trials per step <= library chunk)
@@ -172,6 +325,15 @@ def main():
           "%.3f ms/launch; %.3g trial-iter/s; info %s" % (done, elapsed, iters / done, fails / done,
                                                         lo, hi, sum(dev_ms) / len(dev_ms),
                                                         iters / elapsed, info), file=sys.stderr)
+    learned = None
+    if args.learned_steps > 0 and args.precision == 32:
+        learned = learned_vs_plain(jbp, dec, args, first_timed, head)
+        if learned:
+            print("bench: learned %s" % json.dumps(learned), file=sys.stderr)
+    mixed = None
+    if not args.no_mixed and args.precision == 32:
+        mixed = mixed_workload(jbp, dec, args)
+        print("bench: mixed %s" % json.dumps(mixed), file=sys.stderr)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args, first_timed)
@@ -201,6 +363,8 @@ def main():
             "fer_ci95": [lo, hi],
             "trial_iterations_per_s": round(iters / elapsed, 1),
             "edge_updates_per_s": round(iters * EDGES_PER_ITER / elapsed, 1),
+            "learned": learned,
+            "mixed": mixed,
         },
         "roofline": {
             "bound": "hbm",

@@ -164,6 +164,23 @@ int jbp_simulate(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *per_trial,
 int jbp_sample(jbp_ctx *ctx, uint64_t base_seed, int64_t first_trial, int count, double p,
                int *err_out, int *sx_out, int *sz_out, char *err, size_t errlen);
 
+/* Autotuner.  Runs cfg once per candidate launch configuration of the
+ * decoder cfg selects and keeps the fastest (device time) for later calls on
+ * this context that leave slots / block at 0:
+ *   FP32 HBM engine: resident trials S = 64, 128, ... (up to n_trials and an
+ *   8 GB state cap; the best S depends on the workload: small runs with
+ *   long-running failures favour small S);
+ *   FP64: threads per block {64, 128, 256, 512} x resident trials {1, 2, 4}
+ *   per compute unit.
+ * The LDS engine has no free parameter (one 768-thread workgroup per compute
+ * unit is fixed by the code); it returns best = {0, 0}, *n_cand = 0.
+ * best[0] = slots, best[1] = block; cand_ms (optional, *n_cand entries on
+ * input) receives each candidate's device ms, *n_cand their number.  No
+ * counterpart in the reference (its kernels use fixed 256-thread blocks,
+ * jointbp_cuda.cu:681-1104). */
+int jbp_tune(jbp_ctx *ctx, const jbp_sim_cfg *cfg, int best[2], double *cand_ms, int *n_cand,
+             char *err, size_t errlen);
+
 /* Success test by logical operators instead of the row basis built at
  * create (which is skipped for codes too large to eliminate densely on the
  * host): side 0 rows span ker(Hz) modulo rowspace(Hx) and are checked

@@ -112,6 +112,7 @@ def lib():
         "jbp_code_diagnostics": (i32, [vp, i32, vp]),
         "jbp_code_logicals": (i32, [vp, i32, C.POINTER(i64), C.POINTER(i64), vp, vp]),
         "jbp_set_logicals": (i32, [vp, i32, i64, vp, vp, cp, sz]),
+        "jbp_tune": (i32, [vp, C.POINTER(SimCfg), vp, vp, C.POINTER(i32), cp, sz]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -351,6 +352,25 @@ class Decoder:
         e = _errbuf()
         _check(lib().jbp_in_row_space(self._h, side, bits.shape[0], _ptr(bits), _ptr(out), e, 512), e)
         return out.astype(bool)
+
+    def tune(self, base_seed, n_trials, p, max_iter=50, precision=FP32, engine=0, first_trial=1,
+             check_warmup=0, check_interval=1):
+        """jbp_tune on this workload: returns (slots, block, {(slots, block): ms})
+        and keeps the fastest for later calls that pass slots = block = 0."""
+        plist = None
+        if np.ndim(p) > 0:
+            plist = np.ascontiguousarray(p, dtype=np.float64)
+            p = float(plist[0])
+        cfg = SimCfg(base_seed, first_trial, n_trials, p, max_iter, 0.0, precision, check_warmup,
+                     check_interval, 0, 0, 0,
+                     plist.ctypes.data_as(C.POINTER(C.c_double)) if plist is not None else None,
+                     int(plist.size) if plist is not None else 0, engine)
+        best = np.zeros(2, np.int32)
+        ms = np.zeros(64, np.float64)
+        n = C.c_int(64)
+        e = _errbuf()
+        _check(lib().jbp_tune(self._h, C.byref(cfg), _ptr(best), _ptr(ms), C.byref(n), e, 512), e)
+        return int(best[0]), int(best[1]), ms[:n.value].tolist()
 
     def simulate_learned(self, model, base_seed, n_trials, p, max_iter=1000, first_trial=1,
                          check_warmup=0, check_interval=1, min_fails=0, per_trial=True,

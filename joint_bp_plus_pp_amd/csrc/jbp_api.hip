@@ -593,6 +593,101 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
 
 }  // namespace jbp
 
+namespace jbp {
+namespace {
+
+// One trial from syndrome planes bx / bz through a graph captured on first
+// use: H2D of the syndrome words from pinned staging, the counter reset (a
+// fill kernel), the decoder launch, and D2H of iterations, match flag and
+// estimate planes.  Re-captured when the decode parameters change; the
+// prior lives in the context's channel table, which the graph reads.
+int decode_graph(jbp_ctx *ctx, const std::vector<uint32_t> &bx, const std::vector<uint32_t> &bz,
+                 int max_iter, int warmup, int interval, double damping, int precision,
+                 jbp_decode_result *out, int *est_out,
+                 std::chrono::steady_clock::time_point t0, char *err, size_t errlen) {
+    const DevGraph &g = ctx->dg;
+    const size_t in_words = static_cast<size_t>(g.wm[0]) + g.wm[1];
+    const size_t out_words = 2 + 2 * static_cast<size_t>(g.wn);
+    if (!ctx->h_stage)
+        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ctx->h_stage), (in_words + out_words) * 4));
+    uint32_t *hin = ctx->h_stage;
+    uint32_t *hout = ctx->h_stage + in_words;
+    if (precision == JBP_FP64) {
+        int rc = ensure_f64(ctx, err, errlen);
+        if (rc) return rc;
+    }
+    uint64_t dbits = 0;
+    std::memcpy(&dbits, &damping, sizeof(dbits));
+    const TrialPlanes &tp = ctx->tp;
+    const std::vector<uintptr_t> key = {
+        static_cast<uintptr_t>(precision), static_cast<uintptr_t>(max_iter),
+        static_cast<uintptr_t>(warmup), static_cast<uintptr_t>(interval),
+        static_cast<uintptr_t>(dbits), reinterpret_cast<uintptr_t>(tp.syn_x),
+        reinterpret_cast<uintptr_t>(tp.syn_z), reinterpret_cast<uintptr_t>(tp.iters),
+        reinterpret_cast<uintptr_t>(tp.syn_ok), reinterpret_cast<uintptr_t>(tp.est_x),
+        reinterpret_cast<uintptr_t>(tp.est_z), reinterpret_cast<uintptr_t>(ctx->noise.prior),
+        reinterpret_cast<uintptr_t>(ctx->noise.prior32), reinterpret_cast<uintptr_t>(ctx->f64_scratch),
+        static_cast<uintptr_t>(ctx->f64_slots), static_cast<uintptr_t>(ctx->f64_block)};
+    const bool same = ctx->g1_exec && key == ctx->g1_key;
+    if (!same) {
+        if (ctx->g1_exec) (void)hipGraphExecDestroy(ctx->g1_exec);
+        ctx->g1_exec = nullptr;
+        hipStream_t st = ctx->stream;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        (void)hipMemcpyAsync(ctx->tp.syn_x, hin, g.wm[0] * 4, hipMemcpyHostToDevice, st);
+        (void)hipMemcpyAsync(ctx->tp.syn_z, hin + g.wm[0], g.wm[1] * 4, hipMemcpyHostToDevice, st);
+        launch_fill_i32(ctx->d_counter, 0, 1, st);
+        if (precision == JBP_FP32)
+            launch_decode_qc_f32(g, ctx->qcp, ctx->d_qc_tables, ctx->tp, 1, ctx->noise, max_iter,
+                                 warmup, interval, std::max(1, ctx->cus), ctx->d_counter, st, nullptr);
+        else
+            launch_decode_f64(g, ctx->tp, 1, ctx->noise, max_iter, damping, ctx->f64_slots,
+                              ctx->f64_block, ctx->f64_scratch, ctx->d_counter, st);
+        (void)hipMemcpyAsync(hout, ctx->tp.iters, 4, hipMemcpyDeviceToHost, st);
+        (void)hipMemcpyAsync(hout + 1, ctx->tp.syn_ok, 1, hipMemcpyDeviceToHost, st);
+        (void)hipMemcpyAsync(hout + 2, ctx->tp.est_x, g.wn * 4, hipMemcpyDeviceToHost, st);
+        (void)hipMemcpyAsync(hout + 2 + g.wn, ctx->tp.est_z, g.wn * 4, hipMemcpyDeviceToHost, st);
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamEndCapture(st, &graph));
+        hipError_t e = hipGraphInstantiate(&ctx->g1_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            ctx->g1_exec = nullptr;
+            set_err(err, errlen, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+            return JBP_ERR_DEVICE;
+        }
+        ctx->g1_key = key;
+    }
+    std::copy(bx.begin(), bx.end(), hin);
+    std::copy(bz.begin(), bz.end(), hin + g.wm[0]);
+    hout[1] = 0;
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(hipGraphLaunch(ctx->g1_exec, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_ms[1] = ms;
+    ctx->launches[1]++;
+    const uint32_t *ex = hout + 2;
+    const uint32_t *ez = hout + 2 + g.wn;
+    if (est_out)
+        for (int v = 0; v < g.n; ++v)
+            est_out[v] = static_cast<int>(((ex[v >> 5] >> (v & 31)) & 1u) |
+                                          (((ez[v >> 5] >> (v & 31)) & 1u) << 1));
+    out->iterations = static_cast<int>(hout[0]);
+    out->syndrome_match = static_cast<int>(hout[1] & 0xffu);
+    out->check_count = out->iterations;
+    out->kernel_ms = ms;  // the whole graph: copies and decode
+    out->memcpy_ms = 0.0;
+    out->host_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return JBP_OK;
+}
+
+}  // namespace
+}  // namespace jbp
+
 extern "C" {
 
 int jbp_create(const jbp_graph_desc *g, int device_id, jbp_ctx **out, char *err, size_t errlen) {
@@ -726,6 +821,8 @@ void jbp_destroy(jbp_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->g1_exec) (void)hipGraphExecDestroy(ctx->g1_exec);
+    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     free_chunk(ctx);
     free_noise(ctx);
     free_gen(ctx);
@@ -861,7 +958,6 @@ int jbp_decode(jbp_ctx *ctx, const int *sx, const int *sz, const double prior[4]
                int freeze_syn, double damping, int precision, jbp_decode_result *out,
                int *est_out, char *err, size_t errlen) {
     (void)measure_costs;
-    (void)use_graph;
     auto t0 = std::chrono::steady_clock::now();
     if (!ctx || !sx || !sz || !prior || !out || max_iter < 0) {
         set_err(err, errlen, "bad arguments");
@@ -885,16 +981,25 @@ int jbp_decode(jbp_ctx *ctx, const int *sx, const int *sz, const double prior[4]
         if (sx[c]) bx[c >> 5] |= 1u << (c & 31);
     for (int c = 0; c < ctx->hs[1].m; ++c)
         if (sz[c]) bz[c >> 5] |= 1u << (c & 31);
-    auto m0 = std::chrono::steady_clock::now();
-    HIPCHK(hipMemcpy(ctx->tp.syn_x, bx.data(), bx.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->tp.syn_z, bz.data(), bz.size() * 4, hipMemcpyHostToDevice));
-    auto m1 = std::chrono::steady_clock::now();
     int warmup = precision == JBP_FP32 ? std::max(0, check_warmup) : 0;
     int interval = precision == JBP_FP32 ? std::max(1, check_interval) : 1;
     {
         const Prior pri = make_prior(prior);
         if ((rc = set_noise(ctx, &pri, nullptr, 1, err, errlen))) return rc;
     }
+    // Graph replay applies to the single-launch decoders (the LDS-resident
+    // FP32 decoder and the FP64 decoder); the generic FP32 decoder's host
+    // loop cannot be captured, so it runs eagerly.
+    if (use_graph && (precision == JBP_FP64 || (ctx->qc && damping <= 0.0)) &&
+        (precision == JBP_FP32 || precision == JBP_FP64)) {
+        rc = decode_graph(ctx, bx, bz, max_iter, warmup, interval, damping, precision, out, est_out,
+                          t0, err, errlen);
+        return rc;
+    }
+    auto m0 = std::chrono::steady_clock::now();
+    HIPCHK(hipMemcpy(ctx->tp.syn_x, bx.data(), bx.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->tp.syn_z, bz.data(), bz.size() * 4, hipMemcpyHostToDevice));
+    auto m1 = std::chrono::steady_clock::now();
     rc = run_decode(ctx, 1, ctx->noise, max_iter, warmup, interval, damping, precision, err,
                     errlen);
     if (rc) return rc;
@@ -1010,6 +1115,68 @@ int jbp_set_logicals(jbp_ctx *ctx, int side, int64_t n_rows, const int64_t *row_
     ctx->lg[side].qubit = static_cast<const int *>(ctx->lg_mem[side][1]);
     ctx->lg_set[side] = true;
     ctx->have_logicals = ctx->lg_set[0] && ctx->lg_set[1];
+    return JBP_OK;
+}
+
+int jbp_tune(jbp_ctx *ctx, const jbp_sim_cfg *cfg, int best[2], double *cand_ms, int *n_cand,
+             char *err, size_t errlen) {
+    if (!ctx || !cfg || !best) {
+        set_err(err, errlen, "bad arguments");
+        return JBP_ERR_ARG;
+    }
+    const bool fp32 = cfg->precision == JBP_FP32;
+    const bool hbm = fp32 && (cfg->engine == JBP_ENGINE_HBM ||
+                              (cfg->engine == JBP_ENGINE_AUTO && !ctx->qc));
+    std::vector<std::pair<int, int>> cands;  // (slots, block)
+    if (hbm) {
+        const int cap = std::max(64, static_cast<int>(((cfg->n_trials + 63) / 64) * 64));
+        for (int S = 64; S <= std::min(cap, 64 * 512); S *= 2)
+            if (gen_carve_bytes(ctx->dg, S / 64) <= (size_t(8) << 30)) cands.push_back({S, 0});
+    } else if (!fp32) {
+        const int cus = std::max(1, ctx->cus);
+        for (int mult : {1, 2, 4})
+            if (decode_f64_scratch_bytes(ctx->dg, mult * cus) <= (size_t(8) << 30))
+                for (int blk : {64, 128, 256, 512}) cands.push_back({mult * cus, blk});
+    }
+    if (cands.empty()) {  // the LDS decoder: one 768-thread workgroup per compute unit
+        best[0] = 0;
+        best[1] = 0;
+        if (n_cand) *n_cand = 0;
+        return JBP_OK;
+    }
+    if (n_cand && *n_cand < static_cast<int>(cands.size())) {
+        set_err(err, errlen, "cand_ms needs %zu entries", cands.size());
+        return JBP_ERR_ARG;
+    }
+    double best_ms = 0.0;
+    int bi = -1;
+    for (size_t i = 0; i < cands.size(); ++i) {
+        jbp_sim_cfg c = *cfg;
+        c.slots = cands[i].first;
+        c.block = cands[i].second;
+        c.min_fails = 0;
+        jbp_run_stats st;
+        int rc = jbp::simulate_impl(ctx, &c, nullptr, &st, nullptr, err, errlen);
+        if (rc) return rc;
+        if (cand_ms) cand_ms[i] = st.device_ms;
+        if (bi < 0 || st.device_ms < best_ms) {
+            best_ms = st.device_ms;
+            bi = static_cast<int>(i);
+        }
+    }
+    if (n_cand) *n_cand = static_cast<int>(cands.size());
+    best[0] = cands[bi].first;
+    best[1] = cands[bi].second;
+    if (hbm) {
+        ctx->gen_slots = best[0];
+    } else {
+        if (ctx->f64_slots != best[0] && ctx->f64_scratch) {
+            (void)hipFree(ctx->f64_scratch);
+            ctx->f64_scratch = nullptr;
+        }
+        ctx->f64_slots = best[0];
+        ctx->f64_block = best[1];
+    }
     return JBP_OK;
 }
 

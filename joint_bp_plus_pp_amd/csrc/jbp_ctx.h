@@ -82,6 +82,12 @@ struct jbp_ctx {
     int *h_ctrl = nullptr;    // pinned [2]
     hipEvent_t gen_ev[2] = {nullptr, nullptr};
     int64_t gen_steps = 0;    // engine steps of the last generic decode
+    // single-trial decode replayed from a captured graph (use_graph)
+    hipGraphExec_t g1_exec = nullptr;
+    // what the captured graph baked in: parameters and every device pointer
+    // it uses (buffers that a later call may reallocate)
+    std::vector<uintptr_t> g1_key;
+    uint32_t *h_stage = nullptr;   // pinned: syndrome words in, outcome out
     double last_ms[3] = {0, 0, 0};
     int64_t launches[3] = {0, 0, 0};
 };

@@ -20,8 +20,7 @@
 //   book    per slot: stop (converged / max_iter), or count the iteration;
 //           stopped and empty slots take the next trial from a counter
 //   emit    hard decisions of the stopped trials to their bit planes
-//   load    syndrome words of the refilled slots (a 64x32 bit transpose by
-//           wave ballots)
+//   load    syndrome words of the refilled slots
 // A refilled slot is marked fresh: the var and check kernels read its old
 // messages as 0 for one step, so no per-slot clearing pass is needed.
 // Trials are independent of the slot they run in, so results do not depend
@@ -366,36 +365,29 @@ __global__ void __launch_bounds__(256) gen_emit_kernel(DevGraph g, GenState s, T
     }
 }
 
-// grid (ceil(maxw / WPB), groups), block 64: syndrome words of the slots
-// refilled by the book pass.  Lane j loads word w of its trial's plane; 32
-// ballots turn the 64 x 32 bit block into the 32 check words.
-constexpr int kLoadWords = 8;
-__global__ void __launch_bounds__(64) gen_load_kernel(DevGraph g, GenState s, TrialPlanes tp) {
+// grid (ceil(max(mX, mZ) / 256), groups), block 256: syndrome words of the
+// slots refilled by the book pass.  Thread c rebuilds word c of each side
+// from the refilled slots' planes (a loop over the set bits of the refill
+// mask; each plane word is read by 32 neighbouring threads at once).
+__global__ void __launch_bounds__(256) gen_load_kernel(DevGraph g, GenState s, TrialPlanes tp) {
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) s.ctrl[2] = 0;  // emit list consumed
     const int group = blockIdx.y;
     const uint64_t rf = uni64(s.refill[group]);
     if (!rf) return;
-    const int lane = threadIdx.x;
-    const bool mine = (rf >> lane) & 1u;
-    const int trial = mine ? s.slot_trial[group * 64 + lane] : 0;
+    const int c = blockIdx.x * 256 + threadIdx.x;
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
         const int m = g.side[side].m, wm = g.wm[side];
-        const uint32_t *plane = (side == 0 ? tp.syn_x : tp.syn_z) + static_cast<size_t>(trial) * wm;
-        uint64_t *sw = s.synw[side] + static_cast<size_t>(group) * m;
-        const int w0 = blockIdx.x * kLoadWords;
-        const int w1 = min(w0 + kLoadWords, wm);
-        for (int w = w0; w < w1; ++w) {
-            const uint32_t word = mine ? plane[w] : 0u;
-            uint64_t col = 0;
-#pragma unroll 8
-            for (int b = 0; b < 32; ++b) {
-                const uint64_t bal = __ballot((word >> b) & 1u);
-                if (lane == b) col = bal;
-            }
-            const int c = 32 * w + lane;
-            if (lane < 32 && c < m) sw[c] = (sw[c] & ~rf) | (col & rf);
+        if (c >= m) continue;
+        const uint32_t *planes = side == 0 ? tp.syn_x : tp.syn_z;
+        uint64_t acc = 0;
+        for (uint64_t r = rf; r; r &= r - 1) {
+            const int j = __builtin_ctzll(r);
+            const int trial = uni(s.slot_trial[group * 64 + j]);
+            acc |= static_cast<uint64_t>((planes[static_cast<size_t>(trial) * wm + (c >> 5)] >> (c & 31)) & 1u) << j;
         }
+        uint64_t *sw = s.synw[side] + static_cast<size_t>(group) * m;
+        sw[c] = (sw[c] & ~rf) | acc;
     }
 }
 
@@ -498,14 +490,14 @@ int gen_decode(const DevGraph &g, const GenState &s, TrialPlanes tp, int count, 
     launch_fill_i32(reinterpret_cast<int *>(s.bad), 0, 2 * (kBadWays + 3) * G, st);  // bad, active, fresh, refill
     launch_fill_i32(s.ctrl, 0, 16, st);
     hipLaunchKernelGGL(gen_zero_kernel, dim3(G), dim3(64), 0, st, g, s);
-    const int wmax = std::max(g.wm[0], g.wm[1]);
-    const dim3 load_grid((wmax + kLoadWords - 1) / kLoadWords, G);
+    const int mmax = std::max(g.side[0].m, g.side[1].m);
+    const dim3 load_grid((mmax + 255) / 256, G);
     const dim3 emit_grid(((g.n + 63) / 64 + 3) / 4, kEmitRows);
     interval = std::max(1, interval);
     warmup = std::max(0, warmup);
     hipLaunchKernelGGL(gen_book_kernel, dim3(G), dim3(64), 0, st, s, tp, count, nt, max_iter,
                        warmup, interval);
-    hipLaunchKernelGGL(gen_load_kernel, load_grid, dim3(64), 0, st, g, s, tp);
+    hipLaunchKernelGGL(gen_load_kernel, load_grid, dim3(256), 0, st, g, s, tp);
     // Each trial occupies its slot for at most max_iter + 1 steps.
     const int64_t waves = (count + 64LL * G - 1) / (64LL * G);
     constexpr int kBatch = 4;
@@ -519,7 +511,7 @@ int gen_decode(const DevGraph &g, const GenState &s, TrialPlanes tp, int count, 
             hipLaunchKernelGGL(gen_book_kernel, dim3(G), dim3(64), 0, st, s, tp, count, nt,
                                max_iter, warmup, interval);
             hipLaunchKernelGGL(gen_emit_kernel, emit_grid, dim3(256), 0, st, g, s, tp);
-            hipLaunchKernelGGL(gen_load_kernel, load_grid, dim3(64), 0, st, g, s, tp);
+            hipLaunchKernelGGL(gen_load_kernel, load_grid, dim3(256), 0, st, g, s, tp);
         }
         steps += kBatch;
         // finished count after this batch, read back one batch later

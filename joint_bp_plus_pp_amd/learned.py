@@ -63,3 +63,41 @@ def evaluate(decoder, model, base_seed, n_trials, p, max_iter=1000, first_trial=
     sp, rp = decoder.simulate(base_seed, n_trials, p, max_iter=max_iter, first_trial=first_trial,
                               precision=FP32)
     return sl, sp, rl, rp
+
+
+def main(argv=None):
+    """Train on the device and write the parameters:
+    python -m joint_bp_plus_pp_amd.learned --params CODE.txt --out W.npz"""
+    import argparse
+    import sys
+
+    from . import Code, Decoder
+
+    ap = argparse.ArgumentParser(description=main.__doc__.split("\n")[0])
+    ap.add_argument("--params", required=True, help="params file of the code")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--T", type=int, default=25)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--lr", type=float, default=1e-2)
+    ap.add_argument("--lr-decay", type=float, default=0.01)
+    ap.add_argument("--p-lo", type=float, default=0.03)
+    ap.add_argument("--p-hi", type=float, default=0.05)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--device", type=int, default=0)
+    a = ap.parse_args(argv)
+    dec = Decoder(Code.from_params(a.params), a.device)
+    model, hist = train(dec, T=a.T, steps=a.steps, batch=a.batch, lr=a.lr, p_lo=a.p_lo,
+                        p_hi=a.p_hi, seed=a.seed, lr_decay=a.lr_decay,
+                        log=lambda m: print(m, flush=True))
+    save(model, a.out, meta={"steps": a.steps, "batch": a.batch, "lr": a.lr, "p_lo": a.p_lo,
+                             "p_hi": a.p_hi, "seed": a.seed, "loss": hist[-1][1]})
+    print("wrote %s (final loss %.6f, %.1f s)" % (a.out, hist[-1][1], hist[-1][2]), flush=True)
+    model.destroy()
+    dec.destroy()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
+

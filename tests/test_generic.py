@@ -140,3 +140,24 @@ def test_big_hgp_hbm_engine(big_hgp):
     print("big hgp: FER fp32 %.4f, agreement with fp64 on %d trials %.3f"
           % (s32["failures"] / n, m, agree), flush=True)
     assert agree > 0.95
+
+
+def test_tune_picks_a_candidate_and_keeps_results(small_hgp):
+    _, dec = small_hgp
+    print("tune fp32", flush=True)
+    slots, block, ms = dec.tune(31, 1024, 0.05, max_iter=60, precision=jbp.FP32)
+    assert slots % 64 == 0 and 64 <= slots <= 1024 and block == 0
+    assert len(ms) >= 2 and min(ms) > 0
+    _, a = dec.simulate(31, 1024, 0.05, max_iter=60, precision=jbp.FP32, want_est=True)
+    _, b = dec.simulate(31, 1024, 0.05, max_iter=60, precision=jbp.FP32, slots=64, want_est=True)
+    assert per_trial_equal(a, b) and np.array_equal(a["est"], b["est"])
+    print("tune fp64", flush=True)
+    slots, block, ms = dec.tune(32, 256, 0.05, max_iter=40, precision=jbp.FP64)
+    assert block in (64, 128, 256, 512) and slots > 0 and len(ms) >= 4
+    _, a = dec.simulate(32, 256, 0.05, max_iter=40, precision=jbp.FP64)
+    _, b = dec.simulate(32, 256, 0.05, max_iter=40, precision=jbp.FP64, block=64)
+    assert per_trial_equal(a, b)
+
+
+def test_tune_lds_engine_has_nothing_to_tune(main_decoder):
+    assert main_decoder.tune(1, 256, 0.05, max_iter=20, precision=jbp.FP32)[:2] == (0, 0)

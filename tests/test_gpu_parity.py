@@ -197,3 +197,25 @@ def test_fp32_check_interval_semantics(main_decoder):
     conv = r1["syn_match"].astype(bool) & r5["syn_match"].astype(bool)
     assert np.all(r5["iterations"][conv] >= r1["iterations"][conv])
     assert np.all(((r5["iterations"][conv] - 2) % 5 == 0) | (r5["iterations"][conv] == 50))
+
+
+@pytest.mark.parametrize("precision", [jbp.FP32, jbp.FP64])
+def test_single_decode_graph_replay(main_decoder, main_code_oracle, precision):
+    # use_graph replays a captured graph (syndrome copies, counter fill,
+    # decode, result copies).  Every replay must equal the eager decode of the
+    # same syndromes, with eager work and a re-capture (new max_iter) between
+    # replays; FP64 must also equal the oracle.
+    prior = jbp.prior_from_p(0.05)
+    print("graph replay", precision, flush=True)
+    for rep, t in enumerate((7, 11, 15, 7, 23, 11)):
+        e = main_code_oracle.sample(106, t, 0.05)
+        sx, sz = main_code_oracle.syndrome(e)
+        max_iter = 50 if rep < 4 else 40
+        g_est, g_it, g_ok, _ = main_decoder.decode(sx, sz, prior, max_iter=max_iter,
+                                                   precision=precision, use_graph=True)
+        main_decoder.simulate(5, 64, 0.03, max_iter=20, precision=jbp.FP32, per_trial=False)
+        est, it, ok, _ = main_decoder.decode(sx, sz, prior, max_iter=max_iter, precision=precision)
+        assert (g_it, g_ok) == (it, ok) and np.array_equal(g_est, est), (rep, t)
+        if precision == jbp.FP64:
+            w_est, w_it, w_ok = main_code_oracle.decode(sx, sz, prior, max_iter)
+            assert (g_it, g_ok) == (w_it, w_ok) and np.array_equal(g_est, w_est)
