@@ -452,6 +452,151 @@ int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int wa
     return JBP_OK;
 }
 
+void free_pipe(jbp_ctx *ctx) {
+    for (PipeBuf &b : ctx->pipe) {
+        if (b.st) (void)hipStreamSynchronize(b.st);
+        for (void *p : b.allocs) (void)hipFree(p);
+        b.allocs.clear();
+        if (b.h_iters) (void)hipHostFree(b.h_iters);
+        if (b.h_flags) (void)hipHostFree(b.h_flags);
+        for (auto &e : b.ev)
+            if (e) (void)hipEventDestroy(e);
+        if (b.st) (void)hipStreamDestroy(b.st);
+        b = PipeBuf();
+    }
+}
+
+namespace {
+
+int ensure_pipe(jbp_ctx *ctx, int cap, char *err, size_t errlen) {
+    if (ctx->pipe[0].cap >= cap) return JBP_OK;
+    free_pipe(ctx);
+    const DevGraph &g = ctx->dg;
+    const size_t c = static_cast<size_t>(cap);
+    for (PipeBuf &b : ctx->pipe) {
+        auto alloc = [&](size_t bytes, void **p) -> int {
+            HIPCHK(hipMalloc(p, std::max<size_t>(bytes, 4)));
+            b.allocs.push_back(*p);
+            return JBP_OK;
+        };
+        int rc = JBP_OK;
+        void *p = nullptr;
+        if ((rc = alloc(c * g.wn * 4, &p))) return rc; b.tp.err_x = static_cast<uint32_t *>(p);
+        if ((rc = alloc(c * g.wn * 4, &p))) return rc; b.tp.err_z = static_cast<uint32_t *>(p);
+        if ((rc = alloc(c * g.wn * 4, &p))) return rc; b.tp.est_x = static_cast<uint32_t *>(p);
+        if ((rc = alloc(c * g.wn * 4, &p))) return rc; b.tp.est_z = static_cast<uint32_t *>(p);
+        if ((rc = alloc(c * g.wm[0] * 4, &p))) return rc; b.tp.syn_x = static_cast<uint32_t *>(p);
+        if ((rc = alloc(c * g.wm[1] * 4, &p))) return rc; b.tp.syn_z = static_cast<uint32_t *>(p);
+        if ((rc = alloc(c * 4, &p))) return rc; b.tp.iters = static_cast<int *>(p);
+        if ((rc = alloc(3 * c, &p))) return rc;
+        b.tp.syn_ok = static_cast<uint8_t *>(p);
+        b.tp.success = b.tp.syn_ok + c;
+        b.tp.exact = b.tp.syn_ok + 2 * c;
+        if ((rc = alloc(4, &p))) return rc; b.counter = static_cast<int *>(p);
+        HIPCHK(hipStreamCreate(&b.st));
+        for (auto &e : b.ev) HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&b.h_iters), c * 4));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&b.h_flags), 3 * c));
+        b.cap = cap;
+    }
+    return JBP_OK;
+}
+
+// Runs of the LDS-resident FP32 decoder longer than one launch: chunks of
+// kPipeChunk trials alternate between two streams and two buffer sets, so
+// chunk j+1 is sampled while chunk j decodes and its decoder's workgroups
+// take over compute units as chunk j's last trials finish.  Outcomes are
+// reduced on the host in chunk order (min_fails cuts at the same index as
+// the serial loop).
+int simulate_pipelined(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_run_stats &st, int np,
+                       const LearnedInfer *learned, jbp_trial_out *pt, char *err, size_t errlen) {
+    const DevGraph &g = ctx->dg;
+    const int64_t total = cfg->n_trials;
+    const int C = kPipeChunk;
+    int rc = ensure_pipe(ctx, C, err, errlen);
+    if (rc) return rc;
+    const int64_t nch = (total + C - 1) / C;
+    const bool judged = ctx->have_logicals || ctx->have_basis;
+    const int interval = std::max(1, cfg->check_interval);
+    const int warmup = std::max(0, cfg->check_warmup);
+    // the previous call's work on the context stream (e.g. table uploads) first
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    auto enqueue = [&](int64_t j) -> int {
+        PipeBuf &b = ctx->pipe[j & 1];
+        const int cnt = static_cast<int>(std::min<int64_t>(C, total - j * C));
+        const int64_t first = cfg->first_trial + j * C;
+        NoiseTab nt = ctx->noise;
+        nt.phase = static_cast<int>(((first % np) + np) % np);
+        HIPCHK(hipEventRecord(b.ev[0], b.st));
+        launch_sample(g, b.tp, cfg->base_seed, first, cnt, nt, b.st);
+        launch_fill_i32(b.counter, 0, 1, b.st);
+        HIPCHK(hipEventRecord(b.ev[1], b.st));
+        launch_decode_qc_f32(g, ctx->qcp, ctx->d_qc_tables, b.tp, cnt, nt, cfg->max_iter, warmup,
+                             interval, std::max(1, ctx->cus), b.counter, b.st, learned);
+        HIPCHK(hipEventRecord(b.ev[2], b.st));
+        if (ctx->have_logicals) launch_logical(g, ctx->lg[0], ctx->lg[1], b.tp, cnt, b.st);
+        else if (ctx->have_basis) launch_residual(g, ctx->basis[0], ctx->basis[1], b.tp, cnt, b.st);
+        else launch_logical(g, DevLogicals(), DevLogicals(), b.tp, cnt, b.st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(b.h_iters, b.tp.iters, cnt * 4, hipMemcpyDeviceToHost, b.st));
+        HIPCHK(hipMemcpyAsync(b.h_flags, b.tp.syn_ok, 3 * static_cast<size_t>(C), hipMemcpyDeviceToHost, b.st));
+        HIPCHK(hipEventRecord(b.ev[3], b.st));
+        return JBP_OK;
+    };
+    double dev_ms = 0.0;
+    bool stop = false;
+    int64_t done = 0;
+    for (int64_t j = 0; j < std::min<int64_t>(2, nch); ++j)
+        if ((rc = enqueue(j))) return rc;
+    for (int64_t j = 0; j < nch && !stop; ++j) {
+        PipeBuf &b = ctx->pipe[j & 1];
+        const int cnt = static_cast<int>(std::min<int64_t>(C, total - j * C));
+        HIPCHK(hipEventSynchronize(b.ev[3]));
+        float sms = 0.f, dms = 0.f, rms = 0.f;
+        HIPCHK(hipEventElapsedTime(&sms, b.ev[0], b.ev[1]));
+        HIPCHK(hipEventElapsedTime(&dms, b.ev[1], b.ev[2]));
+        HIPCHK(hipEventElapsedTime(&rms, b.ev[2], b.ev[3]));
+        ctx->last_ms[0] = sms;
+        ctx->last_ms[1] = dms;
+        ctx->last_ms[2] = rms;
+        ctx->launches[0]++;
+        ctx->launches[1]++;
+        ctx->launches[2]++;
+        dev_ms += dms;
+        const uint8_t *ok = b.h_flags, *succ = b.h_flags + C, *exact = b.h_flags + 2 * C;
+        int take = cnt;
+        for (int i = 0; i < cnt; ++i) {
+            const uint8_t s = judged ? succ[i] : static_cast<uint8_t>(ok[i] && exact[i]);
+            st.total_iters += b.h_iters[i];
+            st.failures += s ? 0 : 1;
+            st.bp_failures += ok[i] ? 0 : 1;
+            st.exact += exact[i] ? 1 : 0;
+            if (cfg->min_fails > 0 && st.failures >= cfg->min_fails) {
+                take = i + 1;
+                stop = true;
+                break;
+            }
+        }
+        if (pt) {
+            for (int i = 0; i < take; ++i) {
+                const uint8_t s = judged ? succ[i] : static_cast<uint8_t>(ok[i] && exact[i]);
+                if (pt->iterations) pt->iterations[done + i] = b.h_iters[i];
+                if (pt->syn_match) pt->syn_match[done + i] = ok[i];
+                if (pt->success) pt->success[done + i] = s;
+                if (pt->exact) pt->exact[done + i] = exact[i];
+            }
+        }
+        done += take;
+        if (!stop && j + 2 < nch && (rc = enqueue(j + 2))) return rc;
+    }
+    for (PipeBuf &b : ctx->pipe) HIPCHK(hipStreamSynchronize(b.st));
+    st.done = done;
+    st.device_ms = dev_ms;
+    return JBP_OK;
+}
+
+}  // namespace
+
 int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_run_stats *stats,
                   const LearnedInfer *learned, char *err, size_t errlen) {
     auto t0 = std::chrono::steady_clock::now();
@@ -491,6 +636,17 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
         ctx->f64_slots = cfg->slots;
     }
     const int64_t total = cfg->n_trials;
+    const bool lds = cfg->precision == JBP_FP32 && cfg->damping <= 0.0 && ctx->qc &&
+                     (cfg->engine == JBP_ENGINE_LDS ||
+                      (cfg->engine == JBP_ENGINE_AUTO) || learned);
+    if (lds && total > kPipeChunk && !(pt && (pt->err || pt->est))) {
+        int rc0 = simulate_pipelined(ctx, cfg, st, np, learned, pt, err, errlen);
+        if (rc0) return rc0;
+        st.elapsed_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (stats) *stats = st;
+        return JBP_OK;
+    }
     const int chunk = static_cast<int>(std::min<int64_t>(std::max<int64_t>(total, 1), kDefaultChunk));
     int rc = ensure_chunk(ctx, chunk, err, errlen);
     if (rc) return rc;
@@ -765,6 +921,16 @@ int jbp_create(const jbp_graph_desc *g, int device_id, jbp_ctx **out, char *err,
     }
     dg.min_cdeg = dg.max_cdeg;
     dg.vpad_w = dg.max_vdeg <= 4 ? 4 : dg.max_vdeg <= 8 ? 8 : 16;
+    dg.cpad_w = dg.max_cdeg <= 8 ? 8 : dg.max_cdeg <= 12 ? 12 : dg.max_cdeg <= 16 ? 16 : 32;
+    for (int s = 0; s < 2; ++s) {
+        const HostSide &h = ctx->hs[s];
+        const int W = dg.cpad_w;
+        std::vector<int> pad(std::max<size_t>(static_cast<size_t>(h.m) * W, 1), ctx->n);  // n: a zero bit
+        for (int c = 0; c < h.m; ++c)
+            for (int i = h.chk_off[c]; i < h.chk_off[c + 1]; ++i)
+                pad[static_cast<size_t>(c) * W + (i - h.chk_off[c])] = h.edge_var[i];
+        if ((rc = dev_upload(ctx, pad, &dg.cpad[s], err, errlen))) return fail(rc);
+    }
     for (int s = 0; s < 2; ++s) {
         const HostSide &h = ctx->hs[s];
         for (int c = 0; c < h.m; ++c) dg.min_cdeg = std::min(dg.min_cdeg, h.chk_off[c + 1] - h.chk_off[c]);
@@ -823,6 +989,7 @@ void jbp_destroy(jbp_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->g1_exec) (void)hipGraphExecDestroy(ctx->g1_exec);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+    free_pipe(ctx);
     free_chunk(ctx);
     free_noise(ctx);
     free_gen(ctx);

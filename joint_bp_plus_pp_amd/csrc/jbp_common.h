@@ -30,6 +30,11 @@ struct DevGraph {
     // always-zero message: [n][vpad_w] per side
     int vpad_w = 0;
     const int *vpad[2] = {nullptr, nullptr};
+    // check -> variable lists padded to cpad_w entries (8, 12, 16 or 32)
+    // with the variable index n, for the syndrome gathers: [m][cpad_w] per
+    // side (rows 16-byte aligned)
+    int cpad_w = 0;
+    const int *cpad[2] = {nullptr, nullptr};
 };
 
 // Quasi-cyclic structure: check (r*P + x) of side g meets variable

@@ -20,6 +20,22 @@ constexpr size_t kMaxBasisBytes = size_t(256) << 20;
 // Largest host elimination (rows x rows x 64-bit words) done at create.
 constexpr double kMaxBasisWork = 2e10;
 constexpr int kDefaultChunk = 65536;
+// Trials per persistent launch of the LDS-resident decoder when a run is
+// pipelined over two streams.
+constexpr int kPipeChunk = 8192;
+
+// One half of the two-stream pipeline: its own trial planes, decoder
+// counter, stream, events and pinned outcome arrays.
+struct PipeBuf {
+    int cap = 0;
+    TrialPlanes tp;
+    std::vector<void *> allocs;
+    int *counter = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // sample, decode start, decode end, done
+    int *h_iters = nullptr;   // pinned
+    uint8_t *h_flags = nullptr;  // pinned: ok | success | exact, cap each
+};
 constexpr int kMaxPList = JBP_MAX_P_LIST;
 
 void set_err(char *err, size_t len, const char *fmt, ...);
@@ -88,6 +104,7 @@ struct jbp_ctx {
     // it uses (buffers that a later call may reallocate)
     std::vector<uintptr_t> g1_key;
     uint32_t *h_stage = nullptr;   // pinned: syndrome words in, outcome out
+    jbp::PipeBuf pipe[2];
     double last_ms[3] = {0, 0, 0};
     int64_t launches[3] = {0, 0, 0};
 };
@@ -122,6 +139,7 @@ int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int wa
                int slots = 0);
 void free_gen(jbp_ctx *ctx);
 void free_logicals(jbp_ctx *ctx);
+void free_pipe(jbp_ctx *ctx);
 
 int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_run_stats *stats,
                   const LearnedInfer *learned, char *err, size_t errlen);

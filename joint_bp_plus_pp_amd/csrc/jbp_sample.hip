@@ -61,51 +61,69 @@ __device__ __forceinline__ void or_ballot(uint32_t *plane, int words, int q0, ui
 }
 
 // Syndromes (jointbp_ets.cpp:1834-1849) of one trial from its error bit
-// planes, written as bit planes; one wavefront, 64 checks per ballot.
+// planes in LDS (wn + 1 words each, the last word 0: padded entries of the
+// check table name variable n), written as bit planes; one wavefront, 64
+// checks per ballot.  Each lane reads its check's variables as CW/4 aligned
+// int4 loads, all in flight together.
+template <int CW>
 __device__ void write_syndromes(const DevGraph &g, TrialPlanes &tp, int trial, const uint32_t *ex,
                                 const uint32_t *ez) {
     const int lane = threadIdx.x & 63;
+#pragma unroll
     for (int s = 0; s < 2; ++s) {
-        const DevSide &sd = g.side[s];
+        const int m = g.side[s].m;
+        const int4 *rows = reinterpret_cast<const int4 *>(g.cpad[s]);
         const uint32_t *bits = s == 0 ? ex : ez;
         uint32_t *out = (s == 0 ? tp.syn_x : tp.syn_z) + static_cast<size_t>(trial) * g.wm[s];
-        for (int c0 = 0; c0 < sd.m; c0 += 64) {
-            int c = c0 + lane;
-            int par = 0;
-            if (c < sd.m) {
-                for (int e = sd.chk_off[c]; e < sd.chk_off[c + 1]; ++e) {
-                    int v = sd.edge_var[e];
-                    par ^= (bits[v >> 5] >> (v & 31)) & 1;
-                }
+        for (int c0 = 0; c0 < m; c0 += 64) {
+            const int c = c0 + lane;
+            uint32_t par = 0;
+            if (c < m) {
+                int4 v[CW / 4];
+#pragma unroll
+                for (int k = 0; k < CW / 4; ++k) v[k] = rows[static_cast<size_t>(c) * (CW / 4) + k];
+#pragma unroll
+                for (int k = 0; k < CW / 4; ++k)
+                    par ^= (bits[v[k].x >> 5] >> (v[k].x & 31)) ^ (bits[v[k].y >> 5] >> (v[k].y & 31)) ^
+                           (bits[v[k].z >> 5] >> (v[k].z & 31)) ^ (bits[v[k].w >> 5] >> (v[k].w & 31));
             }
-            uint64_t m = __ballot(par != 0);
-            if (lane == 0) {
-                out[c0 >> 5] = static_cast<uint32_t>(m);
-                if ((c0 >> 5) + 1 < g.wm[s]) out[(c0 >> 5) + 1] = static_cast<uint32_t>(m >> 32);
-            }
+            const uint64_t mk = __ballot(par & 1u);
+            if (lane < 2 && (c0 >> 5) + lane < g.wm[s])
+                out[(c0 >> 5) + lane] = static_cast<uint32_t>(mk >> (32 * lane));
         }
     }
 }
 
-// Syndromes of caller-supplied error planes; grid = count, block = 64.
+// Syndromes of caller-supplied error planes; grid = count, block = 64,
+// dynamic LDS = 2*(wn+1) words.
+template <int CW>
 __global__ void __launch_bounds__(64) syndrome_kernel(DevGraph g, TrialPlanes tp) {
+    extern __shared__ uint32_t planes[];
     const int trial = blockIdx.x;
-    write_syndromes(g, tp, trial, tp.err_x + static_cast<size_t>(trial) * g.wn,
-                    tp.err_z + static_cast<size_t>(trial) * g.wn);
+    const int wn = g.wn;
+    uint32_t *ex = planes;
+    uint32_t *ez = planes + wn + 1;
+    for (int w = threadIdx.x; w <= wn; w += 64) {
+        ex[w] = w < wn ? tp.err_x[static_cast<size_t>(trial) * wn + w] : 0u;
+        ez[w] = w < wn ? tp.err_z[static_cast<size_t>(trial) * wn + w] : 0u;
+    }
+    __syncthreads();
+    write_syndromes<CW>(g, tp, trial, ex, ez);
 }
 
 // grid = count trials, block = 64 (one wavefront), dynamic LDS = 2*wn words.
+template <int CW>
 __global__ void __launch_bounds__(64) sample_kernel(DevGraph g, TrialPlanes tp, uint64_t base_seed,
                                                     int64_t first_trial, NoiseTab nt) {
     __shared__ uint64_t mt[kMtN];
-    extern __shared__ uint32_t planes[];  // err_x | err_z
+    extern __shared__ uint32_t planes[];  // err_x | err_z, wn + 1 words each
     const int lane = threadIdx.x;
     const int trial = blockIdx.x;
     const int n = g.n, wn = g.wn;
     const DrawThresholds th = nt.th[nt.entry(trial)];
     uint32_t *ex = planes;
-    uint32_t *ez = planes + wn;
-    for (int w = lane; w < 2 * wn; w += 64) planes[w] = 0u;
+    uint32_t *ez = planes + wn + 1;
+    for (int w = lane; w < 2 * (wn + 1); w += 64) planes[w] = 0u;
 
     if (lane == 0) {
         uint64_t idx = static_cast<uint64_t>(first_trial + trial);
@@ -173,7 +191,7 @@ __global__ void __launch_bounds__(64) sample_kernel(DevGraph g, TrialPlanes tp, 
         __syncthreads();
     }
 
-    write_syndromes(g, tp, trial, ex, ez);
+    write_syndromes<CW>(g, tp, trial, ex, ez);
     uint32_t *gx = tp.err_x + static_cast<size_t>(trial) * wn;
     uint32_t *gz = tp.err_z + static_cast<size_t>(trial) * wn;
     for (int w = lane; w < wn; w += 64) {
@@ -373,14 +391,24 @@ DrawThresholds draw_thresholds(double p) {
 void launch_sample(const DevGraph &g, TrialPlanes tp, uint64_t base_seed, int64_t first_trial,
                    int count, const NoiseTab &nt, hipStream_t st) {
     if (count <= 0) return;
-    size_t lds = static_cast<size_t>(2) * g.wn * sizeof(uint32_t);
-    hipLaunchKernelGGL(sample_kernel, dim3(count), dim3(64), lds, st, g, tp, base_seed,
-                       first_trial, nt);
+    const size_t lds = static_cast<size_t>(2) * (g.wn + 1) * sizeof(uint32_t);
+    switch (g.cpad_w) {
+        case 8: hipLaunchKernelGGL((sample_kernel<8>), dim3(count), dim3(64), lds, st, g, tp, base_seed, first_trial, nt); break;
+        case 12: hipLaunchKernelGGL((sample_kernel<12>), dim3(count), dim3(64), lds, st, g, tp, base_seed, first_trial, nt); break;
+        case 16: hipLaunchKernelGGL((sample_kernel<16>), dim3(count), dim3(64), lds, st, g, tp, base_seed, first_trial, nt); break;
+        default: hipLaunchKernelGGL((sample_kernel<32>), dim3(count), dim3(64), lds, st, g, tp, base_seed, first_trial, nt); break;
+    }
 }
 
 void launch_syndromes(const DevGraph &g, TrialPlanes tp, int count, hipStream_t st) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(syndrome_kernel, dim3(count), dim3(64), 0, st, g, tp);
+    const size_t lds = static_cast<size_t>(2) * (g.wn + 1) * sizeof(uint32_t);
+    switch (g.cpad_w) {
+        case 8: hipLaunchKernelGGL((syndrome_kernel<8>), dim3(count), dim3(64), lds, st, g, tp); break;
+        case 12: hipLaunchKernelGGL((syndrome_kernel<12>), dim3(count), dim3(64), lds, st, g, tp); break;
+        case 16: hipLaunchKernelGGL((syndrome_kernel<16>), dim3(count), dim3(64), lds, st, g, tp); break;
+        default: hipLaunchKernelGGL((syndrome_kernel<32>), dim3(count), dim3(64), lds, st, g, tp); break;
+    }
 }
 
 void launch_residual(const DevGraph &g, DevBasis bx, DevBasis bz, TrialPlanes tp, int count,

@@ -219,3 +219,22 @@ def test_single_decode_graph_replay(main_decoder, main_code_oracle, precision):
         if precision == jbp.FP64:
             w_est, w_it, w_ok = main_code_oracle.decode(sx, sz, prior, max_iter)
             assert (g_it, g_ok) == (w_it, w_ok) and np.array_equal(g_est, w_est)
+
+
+def test_pipelined_run_equals_serial_run(main_decoder):
+    # Runs longer than one launch alternate two streams and buffer sets; the
+    # per-trial outcome must equal the serial path (taken when estimates are
+    # requested), and min_fails must cut at the same trial index.
+    n = 3 * 8192 + 77
+    print("pipelined vs serial", flush=True)
+    sa, a = main_decoder.simulate(106, n, 0.05, max_iter=100, precision=jbp.FP32, want_est=True)
+    sb, b = main_decoder.simulate(106, n, 0.05, max_iter=100, precision=jbp.FP32)
+    for k in ("iterations", "syn_match", "success", "exact"):
+        assert np.array_equal(a[k], b[k]), k
+    assert (sa["done"], sa["failures"], sa["total_iters"]) == (sb["done"], sb["failures"], sb["total_iters"])
+    limit = max(2, int(sa["failures"]) // 2)
+    sa, a = main_decoder.simulate(106, n, 0.05, max_iter=100, precision=jbp.FP32, want_est=True,
+                                  min_fails=limit)
+    sb, b = main_decoder.simulate(106, n, 0.05, max_iter=100, precision=jbp.FP32, min_fails=limit)
+    assert sa["done"] == sb["done"] < n and sa["failures"] == sb["failures"] == limit
+    assert np.array_equal(a["success"], b["success"])
