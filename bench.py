@@ -36,7 +36,8 @@ EDGES_PER_ITER = 2 * 27648
 BYTES_PER_ITER = 16 * EDGES_PER_ITER  # 884,736
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "decode_qc_f32_p0.04_traffic.json")
-LEARNED_FILE = os.path.join(ROOT, "joint_bp_plus_pp_amd", "codes", "learned_T25_P768_J3_L12.npz")
+LEARNED_FILES = [os.path.join(ROOT, "joint_bp_plus_pp_amd", "codes", f)
+                 for f in ("learned_T25_P768_J3_L12.npz", "learned_T25_nomlp_P768_J3_L12.npz")]
 MIXED_PS = (0.01, 0.025, 0.04, 0.05)
 
 
@@ -100,36 +101,39 @@ def cpu_baseline(args, first_trial):
 
 
 def learned_vs_plain(jbp, dec, args, first, plain):
-    """The learned decoder (shipped weights, T learned iterations then plain
-    BP restarted from the prior) on the first timed trial indices, beside
-    plain BP on the same indices."""
-    if not os.path.exists(LEARNED_FILE):
-        return None
+    """Each shipped learned model (T learned iterations, then plain BP
+    restarted from the prior) on the first timed trial indices, beside plain
+    BP on the same indices."""
     from joint_bp_plus_pp_amd import learned as jl
-    model = jl.load(dec, LEARNED_FILE)
-    nxt, done, fails, iters, dev = first, 0, 0, 0, 0.0
-    l0 = dec.last_kernel_ms(1)[1]
-    t0 = time.perf_counter()
-    for _ in range(args.learned_steps):
-        st, _ = dec.simulate_learned(model, args.seed, args.trials, args.p, max_iter=args.max_iter,
-                                     first_trial=nxt, per_trial=False)
-        nxt += args.trials
-        done += st["done"]
-        fails += st["failures"]
-        iters += st["total_iters"]
-        dev += st["device_ms"]
-    el = time.perf_counter() - t0
-    nl = max(1, dec.last_kernel_ms(1)[1] - l0)
-    model.destroy()
-    lo, hi = wilson(done, fails)
     p_done, p_fails, p_el = plain
     plo, phi = wilson(p_done, p_fails)
-    return {"weights": os.path.relpath(LEARNED_FILE, ROOT), "T": model.T, "trials": done,
-            "trials_per_s": round(done / el, 2), "fer": fails / done, "fer_ci95": [lo, hi],
-            "avg_iter": round(iters / done, 4), "decode_ms_per_launch": round(dev / nl, 4),
-            "plain_same_trials": {"trials": p_done, "trials_per_s": round(p_done / p_el, 2),
-                                  "fer": p_fails / p_done, "fer_ci95": [plo, phi]},
-            "speedup_vs_plain": round((done / el) / (p_done / p_el), 4)}
+    out = {"plain_same_trials": {"trials": p_done, "trials_per_s": round(p_done / p_el, 2),
+                                 "fer": p_fails / p_done, "fer_ci95": [plo, phi]}}
+    for path in LEARNED_FILES:
+        if not os.path.exists(path):
+            continue
+        model = jl.load(dec, path)
+        nxt, done, fails, iters, dev = first, 0, 0, 0, 0.0
+        l0 = dec.last_kernel_ms(1)[1]
+        t0 = time.perf_counter()
+        for _ in range(args.learned_steps):
+            st, _ = dec.simulate_learned(model, args.seed, args.trials, args.p,
+                                         max_iter=args.max_iter, first_trial=nxt, per_trial=False)
+            nxt += args.trials
+            done += st["done"]
+            fails += st["failures"]
+            iters += st["total_iters"]
+            dev += st["device_ms"]
+        el = time.perf_counter() - t0
+        nl = max(1, dec.last_kernel_ms(1)[1] - l0)
+        lo, hi = wilson(done, fails)
+        key = "mlp_damping" if model.use_mlp else "weights_only"
+        out[key] = {"weights": os.path.relpath(path, ROOT), "T": model.T, "trials": done,
+                    "trials_per_s": round(done / el, 2), "fer": fails / done, "fer_ci95": [lo, hi],
+                    "avg_iter": round(iters / done, 4), "decode_ms_per_launch": round(dev / nl, 4),
+                    "speedup_vs_plain": round((done / el) / (p_done / p_el), 4)}
+        model.destroy()
+    return out if len(out) > 1 else None
 
 
 def mixed_workload(jbp, qc_dec, args):

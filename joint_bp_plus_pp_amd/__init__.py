@@ -444,6 +444,7 @@ class LearnedModel:
         decoder._models.add(self)
         self.count = lib().jbp_learned_param_count(h)
         self.nb = (self.count - self.MLP_PARAMS) // (4 * T)
+        self.use_mlp = True
 
     def destroy(self):
         if self._h and self._h.value:
@@ -466,6 +467,7 @@ class LearnedModel:
         assert params.size == self.count
         e = _errbuf()
         _check(lib().jbp_learned_set_params(self._h, _ptr(params), int(use_mlp), e, 512), e)
+        self.use_mlp = bool(use_mlp)
 
     def split(self, params):
         k = 2 * self.nb * self.T

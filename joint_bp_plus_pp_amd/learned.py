@@ -17,11 +17,15 @@ from . import FP32, LearnedModel
 
 
 def train(decoder, T=25, steps=200, batch=4096, lr=1e-2, p_lo=0.03, p_hi=0.05, seed=2024,
-          first_trial=1, log_every=20, lr_decay=0.0, log=None):
+          first_trial=1, log_every=20, lr_decay=0.0, log=None, use_mlp=True):
     """Train a LearnedModel on device-sampled trials (p uniform in
     [p_lo, p_hi] per trial, consecutive trial indices per step).  Returns
-    (model, history) with one (step, loss, seconds) entry per logged step."""
+    (model, history) with one (step, loss, seconds) entry per logged step.
+    use_mlp=False trains the message weights alone (damping 0), which the
+    inference decoder runs without the damping state."""
     model = LearnedModel(decoder, T)
+    if not use_mlp:
+        model.set_params(model.get_params(), use_mlp=False)
     history = []
     t0 = time.time()
     nxt = first_trial
@@ -37,15 +41,19 @@ def train(decoder, T=25, steps=200, batch=4096, lr=1e-2, p_lo=0.03, p_hi=0.05, s
 
 
 def save(model, path, meta=None):
-    d = {"params": model.get_params(), "T": np.array(model.T), "nb": np.array(model.nb)}
+    d = {"params": model.get_params(), "T": np.array(model.T), "nb": np.array(model.nb),
+         "use_mlp": np.array(int(getattr(model, "use_mlp", True)))}
     for k, v in (meta or {}).items():
         d["meta_" + k] = np.asarray(v)
     np.savez(path, **d)
 
 
-def load(decoder, path, use_mlp=True):
+def load(decoder, path, use_mlp=None):
+    """use_mlp=None takes the setting stored with the parameters."""
     with np.load(path, allow_pickle=False) as z:
         T = int(z["T"])
+        if use_mlp is None:
+            use_mlp = bool(int(z["use_mlp"])) if "use_mlp" in z.files else True
         model = LearnedModel(decoder, T)
         params = z["params"]
         if params.size != model.count:
@@ -85,11 +93,12 @@ def main(argv=None):
     ap.add_argument("--p-hi", type=float, default=0.05)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--no-mlp", action="store_true", help="message weights only, no damping MLP")
     a = ap.parse_args(argv)
     dec = Decoder(Code.from_params(a.params), a.device)
     model, hist = train(dec, T=a.T, steps=a.steps, batch=a.batch, lr=a.lr, p_lo=a.p_lo,
                         p_hi=a.p_hi, seed=a.seed, lr_decay=a.lr_decay,
-                        log=lambda m: print(m, flush=True))
+                        log=lambda m: print(m, flush=True), use_mlp=not a.no_mlp)
     save(model, a.out, meta={"steps": a.steps, "batch": a.batch, "lr": a.lr, "p_lo": a.p_lo,
                              "p_hi": a.p_hi, "seed": a.seed, "loss": hist[-1][1]})
     print("wrote %s (final loss %.6f, %.1f s)" % (a.out, hist[-1][1], hist[-1][2]), flush=True)

@@ -161,3 +161,49 @@ def test_tune_picks_a_candidate_and_keeps_results(small_hgp):
 
 def test_tune_lds_engine_has_nothing_to_tune(main_decoder):
     assert main_decoder.tune(1, 256, 0.05, max_iter=20, precision=jbp.FP32)[:2] == (0, 0)
+
+
+def irregular_hgp_checks(m, n, seed):
+    """Hypergraph product of a random irregular m x n matrix (column weights
+    2..4, so check degrees and variable degrees vary), as check lists."""
+    rng = np.random.default_rng(seed)
+    H = np.zeros((m, n), np.uint8)
+    for j in range(n):
+        H[rng.choice(m, size=2 + j % 3, replace=False), j] = 1
+    rows = [np.nonzero(H[a])[0] for a in range(m)]
+    cols = [np.nonzero(H[:, j])[0] for j in range(n)]
+    xs, zs = [], []
+    for a in range(m):          # X check (a, j): (i, j) for i in row a; (a, b) for b in column j
+        for j in range(n):
+            xs.append([i * n + j for i in rows[a]] + [n * n + a * m + b for b in cols[j]])
+    for i in range(n):          # Z check (i, b): (i, j) for j in row b; (a, b) for a in column i
+        for b in range(m):
+            zs.append([i * n + j for j in rows[b]] + [n * n + a * m + b for a in cols[i]])
+    width = max(max(map(len, xs)), max(map(len, zs)))
+    pad = lambda L: np.array([c + [-1] * (width - len(c)) for c in L], np.int32)
+    return n * n + m * m, pad(xs), pad(zs)
+
+
+def test_irregular_code_hbm_engine_vs_fp64(oracle):
+    # variable check degrees (the generic check pass without a fixed degree)
+    # and variable degrees up to 8 per side
+    nq, xs, zs = irregular_hgp_checks(8, 12, seed=3)
+    code = jbp.Code.from_checks(nq, xs, zs)
+    dec = jbp.Decoder(code, 0)
+    try:
+        info = code.diagnostics(girth=False)
+        assert info["x_check_deg_min"] < info["x_check_deg_max"]
+        print("irregular code", nq, info["x_check_deg_min"], info["x_check_deg_max"], flush=True)
+        n = 3000
+        s32, r32 = dec.simulate(41, n, 0.04, max_iter=60, precision=jbp.FP32, engine=jbp.ENGINE_HBM,
+                                want_est=True)
+        s64, r64 = dec.simulate(41, n, 0.04, max_iter=60, precision=jbp.FP64, want_est=True)
+        assert np.mean(r32["success"] == r64["success"]) > 0.99
+        assert np.mean(np.all(r32["est"] == r64["est"], axis=1)) > 0.97
+        assert abs(s32["failures"] - s64["failures"]) <= 3 * np.sqrt(max(s64["failures"], 1)) + 3
+        oc = oracle.code_from_checks(nq, xs, zs)
+        o = oc.simulate(41, 1, 200, 0.04, 60, threads=8)
+        assert np.array_equal(r64["iterations"][:200], o["iterations"])
+        assert np.array_equal(r64["success"][:200], o["success"])
+    finally:
+        dec.destroy()
