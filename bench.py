@@ -3,14 +3,15 @@
 
 Workload (the batched GPU configuration): the included girth-8 (3,12)-regular
 [[9216,4612]] code, i.i.d. depolarizing noise at p = 0.04, 8192 seeded trials
-per persistent decoder launch (each compute unit pulls the next trial when
-its current one stops), max-iter 1000 with per-trial early termination, FP32
-messages.  A step is one library call over 65,536 trials: eight launches,
-alternating two streams so that the next launch's trials are sampled while
-the current one decodes.  Each launch samples its trials on the device
+per step = one persistent decoder launch (each compute unit pulls the next
+trial when its current one stops), max-iter 1000 with per-trial early
+termination, FP32 messages.  A step samples its trials on the device
 (seed = splitmix64(base ^ idx*phi), mt19937_64), computes syndromes,
 decodes, and runs the stabilizer-residual test; consecutive steps take
-consecutive trial indices.
+consecutive trial indices.  config.pipelined reports calls of 65,536 trials
+(eight launches alternating two streams, the next launch's trials sampled
+while the current one decodes); their launches overlap, so the roofline is
+taken from the one-launch steps.
 
 Prints one JSON line (last line of stdout).  `value` is decoded trials/s over
 the timed steps.  `roofline` is for the decoder kernel: algorithmic bytes
@@ -259,18 +260,20 @@ def main():
     ap.add_argument("--workload", choices=("qc", "hgp"), default="qc",
                     help="qc: the included code (default, the headline); hgp: the seeded "
                          "hypergraph-product code HGP(150x200) on the HBM-resident decoder")
-    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--trials", type=int, default=65536,
-                    help="trials per step (the library runs them as persistent launches of "
-                         "8192 trials, pipelined over two streams)")
+    ap.add_argument("--trials", type=int, default=8192,
+                    help="trials per step: one persistent decoder launch")
+    ap.add_argument("--pipelined-steps", type=int, default=8,
+                    help="also time this many calls of 65,536 trials (eight launches each, "
+                         "pipelined over two streams); 0 = skip")
     ap.add_argument("--p", type=float, default=0.04)
     ap.add_argument("--max-iter", type=int, default=1000)
     ap.add_argument("--seed", type=int, default=106)
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--cpu-trials", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--learned-steps", type=int, default=10,
+    ap.add_argument("--learned-steps", type=int, default=40,
                     help="steps of the learned decoder on the first timed trial indices (0: skip)")
     ap.add_argument("--mixed-trials", type=int, default=4096, help="trials per p of the mixed run")
     ap.add_argument("--mixed-max-iter", type=int, default=200)
@@ -336,6 +339,25 @@ def main():
           "%.3f ms/launch (%d launches); %.3g trial-iter/s; info %s"
           % (done, elapsed, iters / done, fails / done, lo, hi, sum(dev_ms) / launches, launches,
              iters / elapsed, info), file=sys.stderr)
+    pipelined = None
+    if args.pipelined_steps > 0 and args.precision == 32:
+        pn = 65536
+        pdone = pfails = piters = 0
+        tp0 = time.perf_counter()
+        for _ in range(args.pipelined_steps):
+            st, _ = dec.simulate(args.seed, pn, args.p, max_iter=args.max_iter, first_trial=nxt,
+                                 precision=args.precision, per_trial=False)
+            nxt += pn
+            pdone += st["done"]
+            pfails += st["failures"]
+            piters += st["total_iters"]
+        pel = time.perf_counter() - tp0
+        plo, phi = wilson(pdone, pfails)
+        pipelined = {"trials_per_call": pn, "calls": args.pipelined_steps,
+                     "trials_per_s": round(pdone / pel, 2),
+                     "trial_iterations_per_s": round(piters / pel, 1),
+                     "fer": pfails / pdone, "fer_ci95": [plo, phi]}
+        print("bench: pipelined %s" % json.dumps(pipelined), file=sys.stderr)
     learned = None
     if args.learned_steps > 0 and args.precision == 32:
         learned = learned_vs_plain(jbp, dec, args, first_timed, head)
@@ -362,9 +384,8 @@ def main():
         "dtype": "fp32" if args.precision == 32 else "f64",
         "data": "synthetic",
         "config": {
-            "workload": "plain joint BP, batched, included [[9216,4612]] code, p=%g, 8192 trials per "
-                        "persistent launch, %d trials per step (two-stream pipeline), max-iter %d, "
-                        "early termination, seed %d"
+            "workload": "plain joint BP, batched, included [[9216,4612]] code, p=%g, %d trials per "
+                        "step (one persistent launch), max-iter %d, early termination, seed %d"
                         % (args.p, args.trials, args.max_iter, args.seed),
             "decoder": "plain FP32 LDS-resident" if args.precision == 32 else "plain FP64 parity",
             "trials_per_step": args.trials,
@@ -375,6 +396,7 @@ def main():
             "fer_ci95": [lo, hi],
             "trial_iterations_per_s": round(iters / elapsed, 1),
             "edge_updates_per_s": round(iters * EDGES_PER_ITER / elapsed, 1),
+            "pipelined": pipelined,
             "learned": learned,
             "mixed": mixed,
         },

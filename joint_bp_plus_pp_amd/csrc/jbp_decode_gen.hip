@@ -20,7 +20,8 @@
 //   book    per slot: stop (converged / max_iter), or count the iteration;
 //           stopped and empty slots take the next trial from a counter
 //   emit    hard decisions of the stopped trials to their bit planes
-//   load    syndrome words of the refilled slots
+//   load    syndrome words of the refilled slots (emit and load share one
+//           launch)
 // A refilled slot is marked fresh: the var and check kernels read its old
 // messages as 0 for one step, so no per-slot clearing pass is needed.
 // Trials are independent of the slot they run in, so results do not depend
@@ -64,6 +65,9 @@ __device__ __forceinline__ PriorF slot_prior(const GenState &s, int group, int l
 // once.
 template <int VPW, int MAXV>
 __global__ void __launch_bounds__(256) gen_var_kernel(DevGraph g, GenState s) {
+    // the previous step's stopped-trial list has been emitted: clear it for
+    // this step's book pass
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) s.ctrl[2] = 0;
     const int group = blockIdx.y;
     const uint64_t act = uni64(s.active[group]);
     if (!act) return;
@@ -167,17 +171,95 @@ __device__ __forceinline__ void check_update(float (&mu)[MAXC], int deg, uint32_
     }
 }
 
-// grid (ceil(chunks / 4), groups), block 256; chunk q < nch0 covers X checks
-// [q*CPW, (q+1)*CPW), the rest Z checks.
-// DEG > 0: every check has exactly DEG edges (no per-edge guards, so all
-// loads of a check are issued together); DEG = 0: up to MAXC edges.
+// One wavefront per group (lane = slot): decides which slots stop, records
+// their outcome, and hands the next trials to stopped and empty slots.
+__device__ void book_group(const GenState &s, const TrialPlanes &tp, int count, const NoiseTab &nt,
+                           int max_iter, int warmup, int interval, int group, int lane) {
+    const int slot = group * 64 + lane;
+    const uint64_t act = s.active[group];
+    // OR of the group's kBadWays words (lane i reads word i; set by atomics,
+    // read at device scope), then clear them
+    uint64_t badm = __hip_atomic_load(reinterpret_cast<unsigned long long *>(s.bad + group * kBadWays + lane),
+                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s.bad[group * kBadWays + lane] = 0;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const uint32_t lo = __shfl_xor(static_cast<uint32_t>(badm), m);
+        const uint32_t hi = __shfl_xor(static_cast<uint32_t>(badm >> 32), m);
+        badm |= (static_cast<uint64_t>(hi) << 32) | lo;
+    }
+    const bool mine = (act >> lane) & 1u;
+    bool fin = false;
+    int matched = 0;
+    int k = 0;
+    if (mine) {
+        k = s.slot_iter[slot];
+        const bool bad = (badm >> lane) & 1u;
+        const bool test = k == max_iter || (k > warmup && ((k - warmup) % interval) == 0);
+        if (k >= 1 && test && !bad) {
+            fin = true;
+            matched = 1;
+        } else if (k >= max_iter) {
+            fin = true;
+        } else {
+            s.slot_iter[slot] = k + 1;
+        }
+    }
+    const uint64_t finm = __ballot(fin);
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    if (finm) {
+        int base = 0;
+        if (lane == 0) {
+            base = atomicAdd(&s.ctrl[2], __popcll(finm));
+            atomicAdd(&s.ctrl[1], __popcll(finm));
+        }
+        base = __shfl(base, 0);
+        if (fin) {
+            const int trial = s.slot_trial[slot];
+            tp.iters[trial] = k;
+            tp.syn_ok[trial] = static_cast<uint8_t>(matched);
+            s.fin[base + __popcll(finm & below)] = make_int2(trial, slot);
+        }
+    }
+    const bool free_slot = !mine || fin;
+    const uint64_t freem = __ballot(free_slot);
+    uint64_t newact = act & ~finm;
+    uint64_t refill = 0;
+    if (freem) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&s.ctrl[0], __popcll(freem));
+        base = __shfl(base, 0);
+        const int t = base + __popcll(freem & below);
+        const bool take = free_slot && t < count;
+        if (take) {
+            s.slot_trial[slot] = t;
+            s.slot_iter[slot] = 0;
+            const float *q = nt.prior32 + kPrior32 * nt.entry(t);
+            float *dst = s.prior32 + static_cast<size_t>(group) * kPrior32 * 64 + lane;
+#pragma unroll
+            for (int i = 0; i < kPrior32; ++i) dst[i * 64] = q[i];
+        }
+        refill = __ballot(take);
+        newact |= refill;
+    }
+    if (lane == 0) {
+        s.active[group] = newact;
+        s.fresh[group] = refill;
+        s.refill[group] = refill;
+    }
+}
+
+// grid = groups, block 64.
+__global__ void __launch_bounds__(64) gen_book_kernel(GenState s, TrialPlanes tp, int count,
+                                                      NoiseTab nt, int max_iter, int warmup,
+                                                      int interval) {
+    book_group(s, tp, count, nt, max_iter, warmup, interval, blockIdx.x, threadIdx.x);
+}
+
+// Check messages of chunk q (CPW checks of one side) of one group.
 template <int MAXC, int CPW, int DEG>
-__global__ void __launch_bounds__(256) gen_check_kernel(DevGraph g, GenState s, int nch0, int nch) {
-    const int group = blockIdx.y;
-    const uint64_t act = uni64(s.active[group]);
-    if (!act) return;
-    const int q = blockIdx.x * 4 + wave_id();
-    if (q >= nch) return;
+__device__ __forceinline__ void check_chunk(const DevGraph &g, const GenState &s, int group, int q,
+                                            int nch0) {
     const int lane = threadIdx.x & 63;
     const bool fresh = (uni64(s.fresh[group]) >> lane) & 1u;
     const uint32_t keep = fresh ? 0u : ~0u;  // a fresh slot reads its old messages as 0
@@ -260,98 +342,33 @@ __global__ void __launch_bounds__(256) gen_check_kernel(DevGraph g, GenState s, 
                  static_cast<unsigned long long>(bad));
 }
 
-// grid = groups, block 64 (lane = slot).  Decides which slots stop, records
-// their outcome, and hands the next trials to stopped and empty slots.
-__global__ void __launch_bounds__(64) gen_book_kernel(GenState s, TrialPlanes tp, int count,
-                                                      NoiseTab nt, int max_iter, int warmup,
-                                                      int interval) {
-    const int group = blockIdx.x;
-    const int lane = threadIdx.x;
-    const int slot = group * 64 + lane;
-    const uint64_t act = s.active[group];
-    // OR of the group's kBadWays words (lane i reads word i), then clear them
-    uint64_t badm = s.bad[group * kBadWays + lane];
-    s.bad[group * kBadWays + lane] = 0;
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        const uint32_t lo = __shfl_xor(static_cast<uint32_t>(badm), m);
-        const uint32_t hi = __shfl_xor(static_cast<uint32_t>(badm >> 32), m);
-        badm |= (static_cast<uint64_t>(hi) << 32) | lo;
-    }
-    const bool mine = (act >> lane) & 1u;
-    bool fin = false;
-    int matched = 0;
-    int k = 0;
-    if (mine) {
-        k = s.slot_iter[slot];
-        const bool bad = (badm >> lane) & 1u;
-        const bool test = k == max_iter || (k > warmup && ((k - warmup) % interval) == 0);
-        if (k >= 1 && test && !bad) {
-            fin = true;
-            matched = 1;
-        } else if (k >= max_iter) {
-            fin = true;
-        } else {
-            s.slot_iter[slot] = k + 1;
-        }
-    }
-    const uint64_t finm = __ballot(fin);
-    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    if (finm) {
-        int base = 0;
-        if (lane == 0) {
-            base = atomicAdd(&s.ctrl[2], __popcll(finm));
-            atomicAdd(&s.ctrl[1], __popcll(finm));
-        }
-        base = __shfl(base, 0);
-        if (fin) {
-            const int trial = s.slot_trial[slot];
-            tp.iters[trial] = k;
-            tp.syn_ok[trial] = static_cast<uint8_t>(matched);
-            s.fin[base + __popcll(finm & below)] = make_int2(trial, slot);
-        }
-    }
-    const bool free_slot = !mine || fin;
-    const uint64_t freem = __ballot(free_slot);
-    uint64_t newact = act & ~finm;
-    uint64_t refill = 0;
-    if (freem) {
-        int base = 0;
-        if (lane == 0) base = atomicAdd(&s.ctrl[0], __popcll(freem));
-        base = __shfl(base, 0);
-        const int t = base + __popcll(freem & below);
-        const bool take = free_slot && t < count;
-        if (take) {
-            s.slot_trial[slot] = t;
-            s.slot_iter[slot] = 0;
-            const float *q = nt.prior32 + kPrior32 * nt.entry(t);
-            float *dst = s.prior32 + static_cast<size_t>(group) * kPrior32 * 64 + lane;
-#pragma unroll
-            for (int i = 0; i < kPrior32; ++i) dst[i * 64] = q[i];
-        }
-        refill = __ballot(take);
-        newact |= refill;
-    }
-    if (lane == 0) {
-        s.active[group] = newact;
-        s.fresh[group] = refill;
-        s.refill[group] = refill;
-    }
+// grid (ceil(chunks / 4), groups), block 256; chunk q < nch0 covers X checks
+// [q*CPW, (q+1)*CPW), the rest Z checks.
+// DEG > 0: every check has exactly DEG edges (no per-edge guards, so all
+// loads of a check are issued together); DEG = 0: up to MAXC edges.
+template <int MAXC, int CPW, int DEG>
+__global__ void __launch_bounds__(256) gen_check_kernel(DevGraph g, GenState s, int nch0, int nch) {
+    const int group = blockIdx.y;
+    const uint64_t act = uni64(s.active[group]);
+    const int q = blockIdx.x * 4 + wave_id();
+    if (act && q < nch) check_chunk<MAXC, CPW, DEG>(g, s, group, q, nch0);
 }
 
-// grid (ceil(ceil(n/64) / 4), kEmitRows), block 256: wave w of block x
-// turns variables [64 (4x + w), +64) of stopped trials y, y + kEmitRows, ...
-// into plane words (bit `slot % 64` of each estimate word, one ballot).
+// Emit pass, block (x, y) of a (ceil(ceil(n/64) / 4), kEmitRows) grid of
+// 256 threads: wave w turns variables [64 (4x + w), +64) of stopped trials
+// y, y + kEmitRows, ... into plane words (bit `slot % 64` of each estimate
+// word, one ballot).
 constexpr int kEmitRows = 32;
-__global__ void __launch_bounds__(256) gen_emit_kernel(DevGraph g, GenState s, TrialPlanes tp) {
+__device__ __forceinline__ void emit_block(const DevGraph &g, const GenState &s, const TrialPlanes &tp,
+                                           int bx, int by) {
     const int nfin = s.ctrl[2];
-    if (static_cast<int>(blockIdx.y) >= nfin) return;
+    if (by >= nfin) return;
     const int lane = threadIdx.x & 63;
-    const int v0 = (blockIdx.x * 4 + wave_id()) * 64;
+    const int v0 = (bx * 4 + wave_id()) * 64;
     if (v0 >= g.n) return;
     const int v = v0 + lane;
     const int wd = (v0 >> 5) + lane;
-    for (int i = blockIdx.y; i < nfin; i += kEmitRows) {
+    for (int i = by; i < nfin; i += kEmitRows) {
         const int2 f = s.fin[i];
         const int group = f.y >> 6, sl = f.y & 63;
         const uint64_t wx = v < g.n ? s.estw[0][static_cast<size_t>(group) * g.n + v] : 0ull;
@@ -365,16 +382,15 @@ __global__ void __launch_bounds__(256) gen_emit_kernel(DevGraph g, GenState s, T
     }
 }
 
-// grid (ceil(max(mX, mZ) / 256), groups), block 256: syndrome words of the
-// slots refilled by the book pass.  Thread c rebuilds word c of each side
+// Load pass, block (x, group) of a (ceil(max(mX, mZ) / 256), groups) grid of
+// 256 threads: syndrome words of the slots refilled by the book pass.  Thread c rebuilds word c of each side
 // from the refilled slots' planes (a loop over the set bits of the refill
 // mask; each plane word is read by 32 neighbouring threads at once).
-__global__ void __launch_bounds__(256) gen_load_kernel(DevGraph g, GenState s, TrialPlanes tp) {
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) s.ctrl[2] = 0;  // emit list consumed
-    const int group = blockIdx.y;
+__device__ __forceinline__ void load_block(const DevGraph &g, const GenState &s, const TrialPlanes &tp,
+                                           int bx, int group) {
     const uint64_t rf = uni64(s.refill[group]);
     if (!rf) return;
-    const int c = blockIdx.x * 256 + threadIdx.x;
+    const int c = bx * 256 + threadIdx.x;
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
         const int m = g.side[side].m, wm = g.wm[side];
@@ -391,6 +407,17 @@ __global__ void __launch_bounds__(256) gen_load_kernel(DevGraph g, GenState s, T
     }
 }
 
+// Emit and load in one launch: blocks [0, ex*kEmitRows) run the emit pass
+// (block b as (b % ex, b / ex)), the rest the load pass (as (b' % lx, b' /
+// lx)).  They touch disjoint data (estimate words and output planes; syndrome
+// words).
+__global__ void __launch_bounds__(256) gen_post_kernel(DevGraph g, GenState s, TrialPlanes tp, int ex,
+                                                       int lx) {
+    const int b = blockIdx.x;
+    if (b < ex * kEmitRows) emit_block(g, s, tp, b % ex, b / ex);
+    else load_block(g, s, tp, (b - ex * kEmitRows) % lx, (b - ex * kEmitRows) / lx);
+}
+
 // grid = groups, block 64: the extra edge E of each group and side, which
 // padded variable entries read, holds message 0.
 __global__ void __launch_bounds__(64) gen_zero_kernel(DevGraph g, GenState s) {
@@ -403,28 +430,37 @@ __global__ void __launch_bounds__(64) gen_zero_kernel(DevGraph g, GenState s) {
     }
 }
 
+struct BookArgs {
+    TrialPlanes tp;
+    int count;
+    NoiseTab nt;
+    int max_iter, warmup, interval;
+};
+
 template <int MAXC, int DEG>
-void launch_check(const DevGraph &g, const GenState &s, hipStream_t st) {
+void launch_check(const DevGraph &g, const GenState &s, const BookArgs &b, hipStream_t st) {
     constexpr int CPW = (DEG == 0 || DEG <= 8) ? 4 : 2;  // registers: CPW*DEG messages in flight
     const int nch0 = (g.side[0].m + CPW - 1) / CPW;
     const int nch = nch0 + (g.side[1].m + CPW - 1) / CPW;
     hipLaunchKernelGGL((gen_check_kernel<MAXC, CPW, DEG>), dim3((nch + 3) / 4, s.groups), dim3(256),
                        0, st, g, s, nch0, nch);
+    hipLaunchKernelGGL(gen_book_kernel, dim3(s.groups), dim3(64), 0, st, s, b.tp, b.count, b.nt,
+                       b.max_iter, b.warmup, b.interval);
 }
 
-void launch_check_any(const DevGraph &g, const GenState &s, hipStream_t st) {
+void launch_check_any(const DevGraph &g, const GenState &s, const BookArgs &b, hipStream_t st) {
     if (g.min_cdeg == g.max_cdeg) {
         switch (g.max_cdeg) {
-            case 6: return launch_check<6, 6>(g, s, st);
-            case 7: return launch_check<7, 7>(g, s, st);
-            case 8: return launch_check<8, 8>(g, s, st);
-            case 12: return launch_check<12, 12>(g, s, st);
+            case 6: return launch_check<6, 6>(g, s, b, st);
+            case 7: return launch_check<7, 7>(g, s, b, st);
+            case 8: return launch_check<8, 8>(g, s, b, st);
+            case 12: return launch_check<12, 12>(g, s, b, st);
             default: break;
         }
     }
-    if (g.max_cdeg <= 8) launch_check<8, 0>(g, s, st);
-    else if (g.max_cdeg <= 16) launch_check<16, 0>(g, s, st);
-    else launch_check<32, 0>(g, s, st);
+    if (g.max_cdeg <= 8) launch_check<8, 0>(g, s, b, st);
+    else if (g.max_cdeg <= 16) launch_check<16, 0>(g, s, b, st);
+    else launch_check<32, 0>(g, s, b, st);
 }
 
 void launch_var(const DevGraph &g, const GenState &s, hipStream_t st) {
@@ -491,13 +527,15 @@ int gen_decode(const DevGraph &g, const GenState &s, TrialPlanes tp, int count, 
     launch_fill_i32(s.ctrl, 0, 16, st);
     hipLaunchKernelGGL(gen_zero_kernel, dim3(G), dim3(64), 0, st, g, s);
     const int mmax = std::max(g.side[0].m, g.side[1].m);
-    const dim3 load_grid((mmax + 255) / 256, G);
-    const dim3 emit_grid(((g.n + 63) / 64 + 3) / 4, kEmitRows);
+    const int lx = (mmax + 255) / 256;
+    const int ex = ((g.n + 63) / 64 + 3) / 4;
+    const dim3 post_grid(ex * kEmitRows + lx * G);
+    const BookArgs bargs{tp, count, nt, max_iter, std::max(0, warmup), std::max(1, interval)};
     interval = std::max(1, interval);
     warmup = std::max(0, warmup);
     hipLaunchKernelGGL(gen_book_kernel, dim3(G), dim3(64), 0, st, s, tp, count, nt, max_iter,
                        warmup, interval);
-    hipLaunchKernelGGL(gen_load_kernel, load_grid, dim3(256), 0, st, g, s, tp);
+    hipLaunchKernelGGL(gen_post_kernel, post_grid, dim3(256), 0, st, g, s, tp, ex, lx);
     // Each trial occupies its slot for at most max_iter + 1 steps.
     const int64_t waves = (count + 64LL * G - 1) / (64LL * G);
     constexpr int kBatch = 4;
@@ -507,11 +545,8 @@ int gen_decode(const DevGraph &g, const GenState &s, TrialPlanes tp, int count, 
     for (;;) {
         for (int b = 0; b < kBatch; ++b) {
             launch_var(g, s, st);
-            launch_check_any(g, s, st);
-            hipLaunchKernelGGL(gen_book_kernel, dim3(G), dim3(64), 0, st, s, tp, count, nt,
-                               max_iter, warmup, interval);
-            hipLaunchKernelGGL(gen_emit_kernel, emit_grid, dim3(256), 0, st, g, s, tp);
-            hipLaunchKernelGGL(gen_load_kernel, load_grid, dim3(256), 0, st, g, s, tp);
+            launch_check_any(g, s, bargs, st);  // check pass, then the book pass
+            hipLaunchKernelGGL(gen_post_kernel, post_grid, dim3(256), 0, st, g, s, tp, ex, lx);
         }
         steps += kBatch;
         // finished count after this batch, read back one batch later
