@@ -57,11 +57,14 @@ __device__ __forceinline__ float with_lsb(float v, uint32_t bit) {
 }
 
 // Check update of one row of L inputs (stride P floats apart), in place.
-// The forward pass parks e_i = 2^-|mu_i| in the row slot it read, so the
-// backward pass needs no register copy of it.
+// One XOR over the inputs' bit patterns gives both parities at once: its
+// sign bit is the parity of negative inputs and its least-significant bit
+// that of the estimate bits the variable owners stored there.  The forward
+// pass parks e_i = 2^-|mu_i| in the row slot it read, carrying mu_i's sign,
+// so the backward pass needs no register copy of either.
 template <int L, int P>
 __device__ __forceinline__ uint32_t check_row(float *row, uint32_t syn) {
-    uint32_t neg = 0, par = 0;
+    uint32_t acc = 0;
     float pe[L + 1], po[L + 1];
     pe[0] = 1.0f;
     po[0] = 0.0f;
@@ -69,29 +72,43 @@ __device__ __forceinline__ uint32_t check_row(float *row, uint32_t syn) {
     for (int i = 0; i < L; ++i) {
         float mu = row[i * P];
         uint32_t u = __float_as_uint(mu);
-        par ^= u & 1u;
-        neg |= (u >> 31) << i;
+        acc ^= u;
         float e = fast_exp2(-fabsf(mu));
-        row[i * P] = e;
-        pe[i + 1] = pe[i] + po[i] * e;
-        po[i + 1] = pe[i] * e + po[i];
+        row[i * P] = __uint_as_float((__float_as_uint(e) & 0x7fffffffu) | (u & 0x80000000u));
+        if (i == 0) {  // (1, 0) then (1, e): no arithmetic
+            pe[1] = 1.0f;
+            po[1] = e;
+        } else {
+            pe[i + 1] = pe[i] + po[i] * e;
+            po[i + 1] = pe[i] * e + po[i];
+        }
     }
-    const uint32_t negall = (__popc(neg) & 1u) ^ syn;
+    uint32_t sall = acc ^ (syn << 31);  // sign of the product of the others, times syn
+    asm volatile("" : "+v"(sall));      // one register, so each output is one xor and one insert
     float se = 1.0f, so = 0.0f;
 #pragma unroll
     for (int i = L - 1; i >= 0; --i) {
-        float e = row[i * P];
-        float PE = pe[i] * se + po[i] * so;
-        float PO = pe[i] * so + po[i] * se;
+        const float es = row[i * P];
+        const float e = fabsf(es);
+        // the suffix starts at (1, 0): the last input's products need no arithmetic
+        float PE = i == L - 1 ? pe[i] : pe[i] * se + po[i] * so;
+        float PO = i == L - 1 ? po[i] : pe[i] * so + po[i] * se;
+        // PE >= PO (both recursions keep even >= odd), so mag >= 0 and its
+        // sign bit is free for the output sign
         float mag = fminf(fast_log2(PE) - fast_log2(PO), kLlrMax);
-        uint32_t sg = negall ^ ((neg >> i) & 1u);
-        row[i * P] = sg ? -mag : mag;
-        float ne = se + so * e;
-        float no = se * e + so;
-        se = ne;
-        so = no;
+        row[i * P] = __uint_as_float(__float_as_uint(mag) |
+                                     ((sall ^ __float_as_uint(es)) & 0x80000000u));
+        if (i == L - 1) {
+            se = 1.0f;
+            so = e;
+        } else {
+            float ne = se + so * e;
+            float no = se * e + so;
+            se = ne;
+            so = no;
+        }
     }
-    return par;
+    return acc & 1u;
 }
 
 // Damping of the learned decoder, d = sigmoid(W2 . tanh(W1 (x0, x1) + b1) + b2),
