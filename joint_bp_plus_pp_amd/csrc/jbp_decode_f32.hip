@@ -46,6 +46,8 @@
 namespace jbp {
 namespace {
 
+typedef __attribute__((address_space(3))) float lds_float;
+
 struct QcDev {
     const int *ainv;  // [2][J][L]
     const int *b;     // [2][J][L]
@@ -141,26 +143,34 @@ __device__ __forceinline__ float mlp_damping_wave(const float *mlp, float x0, fl
 // of the plain one.
 template <int P, int J, int L, bool WEIGHTED, bool DAMPED>
 __device__ __forceinline__ int qc_phase(float *B, float *GZ, int *s_cnt, float *s_d, float *s_w,
-                                        uint32_t (&xw)[2][J][(L + 2) / 3], float (&lam)[2][J][L],
+                                        uint32_t (&aw)[2][J][(L + 1) / 2], float (&lam)[2][J][L],
                                         uint32_t synx, uint32_t synz, const PriorF &pr, int kmax,
                                         int warmup, int interval, const LearnedInfer &li, float u0,
                                         uint32_t &estx, uint32_t &estz, int &matched) {
-    constexpr int W = (L + 2) / 3;
+    constexpr int W = (L + 1) / 2;
+    constexpr int HB = J * L / 2;  // blocks in the lower half of B
     const int t = threadIdx.x;
     const int lane = t & 63;
-    // Make the packed indices opaque to the optimiser at chosen points, so
-    // that unpacked addresses are recomputed instead of being kept live
-    // (and spilled) across the check passes or the iteration loop.
+    // Make the packed addresses opaque to the optimiser at chosen points, so
+    // that unpacked values are recomputed instead of being kept live (and
+    // spilled) across the check passes or the iteration loop.
     auto launder = [&]() {
 #pragma unroll
         for (int s = 0; s < 2; ++s)
 #pragma unroll
             for (int r = 0; r < J; ++r)
 #pragma unroll
-                for (int w = 0; w < W; ++w) asm volatile("" : "+v"(xw[s][r][w]));
+                for (int w = 0; w < W; ++w) asm volatile("" : "+v"(aw[s][r][w]));
     };
-    auto slot = [&](int s, int r, int cc) -> int {
-        return (r * L + cc) * P + static_cast<int>((xw[s][r][cc / 3] >> (10 * (cc % 3))) & 1023u);
+    // Exchange slot of variable (cc, t) in block (r, cc) of side s: a 16-bit
+    // LDS address (B's base + x*4, plus the lower half's size for blocks >=
+    // HB) and the block's position within its half, a constant that the LDS
+    // access takes as its immediate offset.
+    auto slot = [&](int s, int r, int cc) -> lds_float & {
+        const uint32_t a16 = (cc & 1) ? (aw[s][r][cc >> 1] >> 16) : (aw[s][r][cc >> 1] & 0xffffu);
+        const int blk = r * L + cc;
+        return *reinterpret_cast<lds_float *>(
+            static_cast<uintptr_t>(a16 + static_cast<uint32_t>((blk % HB) * P * 4)));
     };
     // Iteration-major weights (for iteration k: wc X | wc Z | wv X | wv Z, J*L
     // each) are staged in an LDS double buffer s_w[2][4*J*L]: iteration k reads
@@ -205,7 +215,7 @@ __device__ __forceinline__ int qc_phase(float *B, float *GZ, int *s_cnt, float *
             for (int r = 0; r < J; ++r) {
                 float mu = gx - lam[0][r][cc];
                 if (WEIGHTED && live) mu *= wk[2 * J * L + r * L + cc];
-                B[slot(0, r, cc)] = with_lsb(mu, bx);
+                slot(0, r, cc) = with_lsb(mu, bx);
             }
         }
         if (WEIGHTED && t < 64) {
@@ -245,7 +255,7 @@ __device__ __forceinline__ int qc_phase(float *B, float *GZ, int *s_cnt, float *
         for (int r = 0; r < J; ++r)
 #pragma unroll
             for (int cc = 0; cc < L; ++cc) {
-                float v = B[slot(0, r, cc)];
+                float v = slot(0, r, cc);
                 if (WEIGHTED && live) v *= wk[r * L + cc];
                 lam[0][r][cc] = DAMPED ? (1.0f - dmp) * v + dmp * lam[0][r][cc] : v;
             }
@@ -261,7 +271,7 @@ __device__ __forceinline__ int qc_phase(float *B, float *GZ, int *s_cnt, float *
             for (int r = 0; r < J; ++r) {
                 float mu = gz - lam[1][r][cc];
                 if (WEIGHTED && live) mu *= wk[3 * J * L + r * L + cc];
-                B[slot(1, r, cc)] = with_lsb(mu, bz);
+                slot(1, r, cc) = with_lsb(mu, bz);
             }
         }
         __syncthreads();
@@ -295,7 +305,7 @@ __device__ __forceinline__ int qc_phase(float *B, float *GZ, int *s_cnt, float *
         for (int r = 0; r < J; ++r)
 #pragma unroll
             for (int cc = 0; cc < L; ++cc) {
-                float v = B[slot(1, r, cc)];
+                float v = slot(1, r, cc);
                 if (WEIGHTED && live) v *= wk[J * L + r * L + cc];
                 lam[1][r][cc] = DAMPED ? (1.0f - dmp) * v + dmp * lam[1][r][cc] : v;
             }
@@ -325,17 +335,22 @@ __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, 
     __shared__ float s_w[LEARNED ? 2 * 4 * J * L : 1];  // learned: weights, double buffer
     const int t = threadIdx.x;
     const int lane = t & 63;
-    constexpr int W = (L + 2) / 3;  // three 10-bit row indices per register
 
     // Row index x = ainv*(t - b) mod P of this thread's variable (cc, t) in
-    // block (side, r, cc); its exchange slot is B[(r*L + cc)*P + x].
-    uint32_t xw[2][J][W];
+    // block (side, r, cc); its exchange slot is B[(r*L + cc)*P + x], kept as
+    // a 16-bit LDS address (two per register, see qc_phase).
+    constexpr int HB = J * L / 2;
+    static_assert((P - 1) * 4 + HB * P * 4 + 4096 < 65536 && (HB - 1) * P * 4 < 65536,
+                  "exchange addresses must fit 16 bits and the LDS immediate offset");
+    const uint32_t bbase = static_cast<uint32_t>((uintptr_t)((lds_float *)B));  // LDS address of B
+    constexpr int W = (L + 1) / 2;
+    uint32_t aw[2][J][W];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
         for (int r = 0; r < J; ++r)
 #pragma unroll
-            for (int w = 0; w < W; ++w) xw[s][r][w] = 0;
+            for (int w = 0; w < W; ++w) aw[s][r][w] = 0;
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -346,7 +361,10 @@ __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, 
                 int d = (t - qc.b[idx]) % P;
                 if (d < 0) d += P;
                 uint32_t x = static_cast<uint32_t>((qc.ainv[idx] * d) % P);
-                xw[s][r][cc / 3] |= x << (10 * (cc % 3));
+                const int blk = r * L + cc;
+                const uint32_t a16 =
+                    bbase + x * 4u + (blk >= HB ? static_cast<uint32_t>(HB * P * 4) : 0u);
+                aw[s][r][cc >> 1] |= a16 << (16 * (cc & 1));
             }
 
     for (;;) {
@@ -388,18 +406,18 @@ __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, 
             atomicAdd(&s_cnt, __popc(synx) + __popc(synz));
             __syncthreads();
             const float u0 = static_cast<float>(s_cnt) / static_cast<float>(2 * J * P);
-            int k = qc_phase<P, J, L, true, DAMPED>(B, GZ, &s_cnt, &s_d, s_w, xw, lam, synx, synz, pr, li.T, 0,
+            int k = qc_phase<P, J, L, true, DAMPED>(B, GZ, &s_cnt, &s_d, s_w, aw, lam, synx, synz, pr, li.T, 0,
                                             1, li, u0, estx, estz, matched);
             if (matched) {
                 iterations = k;
             } else {
-                k = qc_phase<P, J, L, false, false>(B, GZ, &s_cnt, &s_d, s_w, xw, lam, synx, synz, pr,
+                k = qc_phase<P, J, L, false, false>(B, GZ, &s_cnt, &s_d, s_w, aw, lam, synx, synz, pr,
                                              max_iter, warmup, interval, li, 0.0f, estx, estz,
                                              matched);
                 iterations = li.T + k;
             }
         } else {
-            iterations = qc_phase<P, J, L, false, false>(B, GZ, &s_cnt, &s_d, s_w, xw, lam, synx, synz, pr,
+            iterations = qc_phase<P, J, L, false, false>(B, GZ, &s_cnt, &s_d, s_w, aw, lam, synx, synz, pr,
                                                   max_iter, warmup, interval, li, 0.0f, estx,
                                                   estz, matched);
         }
