@@ -82,3 +82,24 @@ def test_min_fails_and_hist(tmp_path):
     lines = h.read_text().splitlines()
     assert lines[0].startswith("# costs") and lines[1] == "# iter fraction"
     assert abs(sum(float(x.split()[1]) for x in lines[2:]) - 1.0) < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("graph", [False, True])
+def test_syndrome_file_decode_matches_reference(tmp_path, graph):
+    # --sx/--sz (one int per line, jointbp_ets.cpp:3081-3090) of the reference's
+    # trial 7: the FP64 estimate must equal the reference's estimate of that
+    # trial, with or without the captured graph (--cuda-graph)
+    from _oracle import Oracle
+    ref = golden_json("aggregates.json")["cli_err_trial7"]
+    oc = Oracle().code_from_params(PARAMS)
+    err = oc.sample(106, 7, 0.05)
+    sx, sz = oc.syndrome(err)
+    px, pz, es = tmp_path / "sx.txt", tmp_path / "sz.txt", tmp_path / "est.txt"
+    np.savetxt(px, sx, fmt="%d")
+    np.savetxt(pz, sz, fmt="%d")
+    args = ["--sx", str(px), "--sz", str(pz), "--p", "0.05", "--max-iter", "50", "--no-pp",
+            "--cuda", "--precision", "64", "--est", str(es)]
+    out = run(args + (["--cuda-graph"] if graph else []))
+    assert out.returncode == 0, out.stderr
+    assert fnv1a(np.loadtxt(es, dtype=np.int64)) == ref["est_hash"]
