@@ -122,6 +122,9 @@ typedef struct jbp_sim_cfg {
                              instead of p (interleaved mixed-noise runs)     */
     int n_p;              /* 0 = single p; at most JBP_MAX_P_LIST            */
     int engine;           /* FP32 decoder: JBP_ENGINE_AUTO, _LDS or _HBM     */
+    int schedule;         /* 0: decode trials heaviest syndrome first (shorter
+                             launch tails), 1: in index order; per-trial
+                             results are the same either way                 */
 } jbp_sim_cfg;
 
 #define JBP_MAX_P_LIST 4096

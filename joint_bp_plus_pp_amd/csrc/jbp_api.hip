@@ -193,6 +193,8 @@ void free_chunk(jbp_ctx *ctx) {
     ctx->chunk_allocs.clear();
     ctx->chunk_cap = 0;
     ctx->tp = TrialPlanes();
+    ctx->d_keys = nullptr;
+    ctx->d_order = nullptr;
 }
 
 int ensure_chunk(jbp_ctx *ctx, int count, char *err, size_t errlen) {
@@ -217,6 +219,8 @@ int ensure_chunk(jbp_ctx *ctx, int count, char *err, size_t errlen) {
     if ((rc = alloc(c, &p))) return rc; ctx->tp.syn_ok = static_cast<uint8_t *>(p);
     if ((rc = alloc(c, &p))) return rc; ctx->tp.success = static_cast<uint8_t *>(p);
     if ((rc = alloc(c, &p))) return rc; ctx->tp.exact = static_cast<uint8_t *>(p);
+    if ((rc = alloc(c * 4, &p))) return rc; ctx->d_keys = static_cast<int *>(p);
+    if ((rc = alloc(c * 4, &p))) return rc; ctx->d_order = static_cast<int *>(p);
     ctx->chunk_cap = count;
     return JBP_OK;
 }
@@ -493,6 +497,8 @@ int ensure_pipe(jbp_ctx *ctx, int cap, char *err, size_t errlen) {
         b.tp.success = b.tp.syn_ok + c;
         b.tp.exact = b.tp.syn_ok + 2 * c;
         if ((rc = alloc(4, &p))) return rc; b.counter = static_cast<int *>(p);
+        if ((rc = alloc(c * 4, &p))) return rc; b.keys = static_cast<int *>(p);
+        if ((rc = alloc(c * 4, &p))) return rc; b.order = static_cast<int *>(p);
         HIPCHK(hipStreamCreate(&b.st));
         for (auto &e : b.ev) HIPCHK(hipEventCreate(&e));
         HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&b.h_iters), c * 4));
@@ -529,9 +535,14 @@ int simulate_pipelined(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_run_stats &st, 
         nt.phase = static_cast<int>(((first % np) + np) % np);
         HIPCHK(hipEventRecord(b.ev[0], b.st));
         launch_sample(g, b.tp, cfg->base_seed, first, cnt, nt, b.st);
+        TrialPlanes tpo = b.tp;
+        if (cfg->schedule == 0) {
+            launch_order_by_weight(g, b.tp, cnt, b.keys, b.order, b.st);
+            tpo.order = b.order;
+        }
         launch_fill_i32(b.counter, 0, 1, b.st);
         HIPCHK(hipEventRecord(b.ev[1], b.st));
-        launch_decode_qc_f32(g, ctx->qcp, ctx->d_qc_tables, b.tp, cnt, nt, cfg->max_iter, warmup,
+        launch_decode_qc_f32(g, ctx->qcp, ctx->d_qc_tables, tpo, cnt, nt, cfg->max_iter, warmup,
                              interval, std::max(1, ctx->cus), b.counter, b.st, learned);
         HIPCHK(hipEventRecord(b.ev[2], b.st));
         if (ctx->have_logicals) launch_logical(g, ctx->lg[0], ctx->lg[1], b.tp, cnt, b.st);
@@ -669,9 +680,15 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
         HIPCHK(hipEventElapsedTime(&sms, ctx->ev0, ctx->ev1));
         ctx->last_ms[0] = sms;
         ctx->launches[0]++;
+        if (cfg->schedule == 0) {
+            launch_order_by_weight(g, ctx->tp, cnt, ctx->d_keys, ctx->d_order, ctx->stream);
+            HIPCHK(hipGetLastError());
+            ctx->tp.order = ctx->d_order;
+        }
         rc = run_decode(ctx, cnt, nt, cfg->max_iter, cfg->check_warmup, cfg->check_interval,
                         cfg->damping, cfg->precision, err, errlen, learned, cfg->engine,
                         cfg->precision == JBP_FP32 ? cfg->slots : 0);
+        ctx->tp.order = nullptr;
         if (rc) return rc;
         dev_ms += ctx->last_ms[1];
         if (ctx->have_logicals) {

@@ -79,6 +79,9 @@ struct TrialPlanes {
     uint8_t *syn_ok = nullptr;   // chunk
     uint8_t *success = nullptr;  // chunk
     uint8_t *exact = nullptr;    // chunk
+    // decode order: the k-th trial a decoder pulls is order[k] (heaviest
+    // syndrome first); null = index order.  Results do not depend on it.
+    const int *order = nullptr;  // chunk
 };
 
 // Channel constants derived from the prior over (I, X, Z, Y).
@@ -118,6 +121,11 @@ struct NoiseTab {
 void launch_sample(const DevGraph &g, TrialPlanes tp, uint64_t base_seed, int64_t first_trial,
                    int count, const NoiseTab &nt, hipStream_t st);
 void launch_syndromes(const DevGraph &g, TrialPlanes tp, int count, hipStream_t st);
+// order[0..count) = trial indices by decreasing syndrome weight (a counting
+// sort on a 12-bit key; keys: count ints of scratch).  Long decodes tend to
+// be the heavy ones, so handing those out first shortens a launch's tail.
+void launch_order_by_weight(const DevGraph &g, TrialPlanes tp, int count, int *keys, int *order,
+                            hipStream_t st);
 void launch_residual(const DevGraph &g, DevBasis bx, DevBasis bz, TrialPlanes tp, int count,
                      hipStream_t st);
 // Success test with logical operators: lx against the X part of est ^ err,

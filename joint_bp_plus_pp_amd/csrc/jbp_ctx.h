@@ -35,6 +35,8 @@ struct PipeBuf {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // sample, decode start, decode end, done
     int *h_iters = nullptr;   // pinned
     uint8_t *h_flags = nullptr;  // pinned: ok | success | exact, cap each
+    int *keys = nullptr;      // decode-order scratch
+    int *order = nullptr;
 };
 constexpr int kMaxPList = JBP_MAX_P_LIST;
 
@@ -79,6 +81,8 @@ struct jbp_ctx {
     int f64_slots = 0;
     int f64_block = 256;
     int *d_counter = nullptr;
+    int *d_keys = nullptr;    // decode order of the chunk (chunk_cap each)
+    int *d_order = nullptr;
     int *d_states = nullptr;  // unpack buffer
     size_t d_states_cap = 0;
     int *d_ints = nullptr;    // pack input buffer

@@ -368,7 +368,10 @@ __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, 
             }
 
     for (;;) {
-        if (t == 0) s_trial = atomicAdd(counter, 1);
+        if (t == 0) {
+            const int k = atomicAdd(counter, 1);
+            s_trial = (tp.order && k < count) ? tp.order[k] : k;
+        }
         __syncthreads();
         const int trial = s_trial;
         __syncthreads();

@@ -198,7 +198,10 @@ __global__ void decode_f64_kernel(DevGraph g, TrialPlanes tp, int count, NoiseTa
     const SlotPtrs sp = slot_ptrs(g, scratch, stride);
     const int mx = g.side[0].m, mz = g.side[1].m;
     for (;;) {
-        if (tid == 0) s_trial = atomicAdd(counter, 1);
+        if (tid == 0) {
+            const int k = atomicAdd(counter, 1);
+            s_trial = (tp.order && k < count) ? tp.order[k] : k;
+        }
         __syncthreads();
         const int trial = s_trial;
         __syncthreads();

@@ -229,9 +229,10 @@ __device__ void book_group(const GenState &s, const TrialPlanes &tp, int count, 
         int base = 0;
         if (lane == 0) base = atomicAdd(&s.ctrl[0], __popcll(freem));
         base = __shfl(base, 0);
-        const int t = base + __popcll(freem & below);
-        const bool take = free_slot && t < count;
+        const int k = base + __popcll(freem & below);
+        const bool take = free_slot && k < count;
         if (take) {
+            const int t = tp.order ? tp.order[k] : k;
             s.slot_trial[slot] = t;
             s.slot_iter[slot] = 0;
             const float *q = nt.prior32 + kPrior32 * nt.entry(t);

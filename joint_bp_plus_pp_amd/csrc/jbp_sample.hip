@@ -307,6 +307,53 @@ __global__ void __launch_bounds__(kResBlock) logical_kernel(DevGraph g, DevLogic
     }
 }
 
+// Sort key of each trial: its syndrome weight (both sides), scaled to 12 bits.
+__global__ void __launch_bounds__(64) weight_kernel(DevGraph g, TrialPlanes tp, int shift, int *keys) {
+    const int trial = blockIdx.x;
+    const int lane = threadIdx.x;
+    int w = 0;
+    for (int i = lane; i < g.wm[0]; i += 64) w += __popc(tp.syn_x[static_cast<size_t>(trial) * g.wm[0] + i]);
+    for (int i = lane; i < g.wm[1]; i += 64) w += __popc(tp.syn_z[static_cast<size_t>(trial) * g.wm[1] + i]);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) w += __shfl_xor(w, m);
+    if (lane == 0) keys[trial] = min(w >> shift, 4095);
+}
+
+// One block: counting sort of the keys, largest first.
+constexpr int kOrderBlock = 1024;
+__global__ void __launch_bounds__(kOrderBlock) order_kernel(const int *keys, int count, int *order) {
+    __shared__ int bins[4096];
+    __shared__ int part[kOrderBlock];
+    const int tid = threadIdx.x;
+    for (int b = tid; b < 4096; b += kOrderBlock) bins[b] = 0;
+    __syncthreads();
+    for (int t = tid; t < count; t += kOrderBlock) atomicAdd(&bins[keys[t]], 1);
+    __syncthreads();
+    // start of key k in descending order = number of trials with key > k;
+    // thread i owns keys 4095-4i .. 4092-4i
+    int loc[4];
+    int sum = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        loc[j] = sum;
+        sum += bins[4095 - (4 * tid + j)];
+    }
+    part[tid] = sum;
+    __syncthreads();
+    for (int d = 1; d < kOrderBlock; d <<= 1) {  // inclusive scan of the per-thread sums
+        const int v = tid >= d ? part[tid - d] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    const int before = part[tid] - sum;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bins[4095 - (4 * tid + j)] = before + loc[j];
+    __syncthreads();
+    for (int t = tid; t < count; t += kOrderBlock) order[atomicAdd(&bins[keys[t]], 1)] = t;
+}
+
 __global__ void fill_i32_kernel(int *p, int v, int count) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = v;
@@ -423,6 +470,15 @@ void launch_logical(const DevGraph &g, DevLogicals lx, DevLogicals lz, TrialPlan
     if (count <= 0) return;
     size_t lds = static_cast<size_t>(2) * g.wn * sizeof(uint32_t);
     hipLaunchKernelGGL(logical_kernel, dim3(count), dim3(kResBlock), lds, st, g, lx, lz, tp);
+}
+
+void launch_order_by_weight(const DevGraph &g, TrialPlanes tp, int count, int *keys, int *order,
+                            hipStream_t st) {
+    if (count <= 0) return;
+    int shift = 0;
+    while (((g.side[0].m + g.side[1].m) >> shift) > 4095) ++shift;
+    hipLaunchKernelGGL(weight_kernel, dim3(count), dim3(64), 0, st, g, tp, shift, keys);
+    hipLaunchKernelGGL(order_kernel, dim3(1), dim3(kOrderBlock), 0, st, keys, count, order);
 }
 
 void launch_fill_i32(int *ptr, int value, int count, hipStream_t st) {

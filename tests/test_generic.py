@@ -207,3 +207,18 @@ def test_irregular_code_hbm_engine_vs_fp64(oracle):
         assert np.array_equal(r64["success"][:200], o["success"])
     finally:
         dec.destroy()
+
+
+@pytest.mark.parametrize("precision,engine,n", [(jbp.FP32, jbp.ENGINE_LDS, 3 * 8192 + 5),
+                                                (jbp.FP32, jbp.ENGINE_LDS, 3000),
+                                                (jbp.FP32, jbp.ENGINE_HBM, 3000),
+                                                (jbp.FP64, 0, 600)])
+def test_decode_order_does_not_change_results(main_decoder, precision, engine, n):
+    # trials are handed out heaviest syndrome first by default; index order
+    # must give the same per-trial outcome (the first case runs pipelined)
+    print("order", precision, engine, n, flush=True)
+    _, a = main_decoder.simulate(106, n, 0.05, max_iter=80, precision=precision, engine=engine,
+                                 schedule=0)
+    _, b = main_decoder.simulate(106, n, 0.05, max_iter=80, precision=precision, engine=engine,
+                                 schedule=1)
+    assert per_trial_equal(a, b)
