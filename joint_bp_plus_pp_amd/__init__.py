@@ -56,7 +56,7 @@ class SimCfg(C.Structure):
                 ("precision", C.c_int), ("check_warmup", C.c_int), ("check_interval", C.c_int),
                 ("slots", C.c_int), ("block", C.c_int), ("min_fails", C.c_int64),
                 ("p_list", C.POINTER(C.c_double)), ("n_p", C.c_int), ("engine", C.c_int),
-                ("schedule", C.c_int)]
+                ("schedule", C.c_int), ("graph", C.c_int)]
 
 
 class TrialOut(C.Structure):
@@ -365,7 +365,7 @@ class Decoder:
         cfg = SimCfg(base_seed, first_trial, n_trials, p, max_iter, 0.0, precision, check_warmup,
                      check_interval, 0, 0, 0,
                      plist.ctypes.data_as(C.POINTER(C.c_double)) if plist is not None else None,
-                     int(plist.size) if plist is not None else 0, engine, 0)
+                     int(plist.size) if plist is not None else 0, engine, 0, 0)
         best = np.zeros(2, np.int32)
         ms = np.zeros(64, np.float64)
         n = C.c_int(64)
@@ -387,14 +387,15 @@ class Decoder:
     def simulate(self, base_seed, n_trials, p, max_iter=50, first_trial=1, damping=0.0,
                  precision=FP32, check_warmup=0, check_interval=1, min_fails=0, slots=0, block=0,
                  per_trial=True, want_err=False, want_est=False, engine=0, schedule=0,
-                 _model=None):
+                 graph=False, _model=None):
         """The --simulate --no-pp inner loop on the device.  Returns (stats,
         per-trial dict or None).  p may be a sequence: trial index idx then
         uses p[idx % len(p)] (interleaved mixed-noise run).  engine selects
         the FP32 decoder (ENGINE_AUTO, ENGINE_LDS, ENGINE_HBM); slots is the
         number of resident trials of the HBM engine (FP64: of the parity
         decoder).  schedule=1 decodes in index order instead of heaviest
-        syndrome first (results are the same)."""
+        syndrome first (results are the same); graph=True replays the HBM
+        engine's step batches from a captured graph."""
         plist = None
         if np.ndim(p) > 0:
             plist = np.ascontiguousarray(p, dtype=np.float64)
@@ -402,7 +403,7 @@ class Decoder:
         cfg = SimCfg(base_seed, first_trial, n_trials, p, max_iter, damping, precision, check_warmup,
                      check_interval, slots, block, min_fails,
                      plist.ctypes.data_as(C.POINTER(C.c_double)) if plist is not None else None,
-                     int(plist.size) if plist is not None else 0, engine, schedule)
+                     int(plist.size) if plist is not None else 0, engine, schedule, int(graph))
         out = None
         arrays = {}
         if per_trial:

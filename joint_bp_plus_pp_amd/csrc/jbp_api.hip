@@ -365,7 +365,7 @@ int default_gen_slots(const jbp_ctx *ctx) {
 }
 
 int run_gen(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmup, int interval,
-            int slots, char *err, size_t errlen) {
+            int slots, bool use_graph, char *err, size_t errlen) {
     const DevGraph &g = ctx->dg;
     if (!gen_supported(g)) {
         set_err(err, errlen, "generic FP32 decoder supports check degree <= 32");
@@ -388,7 +388,7 @@ int run_gen(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmu
     gen_carve(g, groups, ctx->gen_mem, ctx->gen);
     int64_t steps = 0;
     const int rc = gen_decode(g, ctx->gen, ctx->tp, count, nt, max_iter, warmup, interval,
-                              ctx->h_ctrl, ctx->gen_ev, ctx->stream, &steps);
+                              ctx->h_ctrl, ctx->gen_ev, ctx->stream, &steps, use_graph);
     ctx->gen_steps = steps;
     if (rc == -1) {
         set_err(err, errlen, "generic decoder exceeded its step bound");
@@ -406,7 +406,7 @@ int run_gen(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmu
 // Decode `count` trials whose syndrome planes are already in ctx->tp.
 int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmup, int interval,
                double damping, int precision, char *err, size_t errlen,
-               const LearnedInfer *learned, int engine, int slots) {
+               const LearnedInfer *learned, int engine, int slots, bool use_graph) {
     if (precision != JBP_FP32 && precision != JBP_FP64) {
         set_err(err, errlen, "precision must be 32 or 64");
         return JBP_ERR_ARG;
@@ -437,7 +437,7 @@ int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int wa
                                  warmup, interval, std::max(1, ctx->cus), ctx->d_counter, ctx->stream,
                                  learned);
         } else {
-            int rc = run_gen(ctx, count, nt, max_iter, warmup, interval, slots, err, errlen);
+            int rc = run_gen(ctx, count, nt, max_iter, warmup, interval, slots, use_graph, err, errlen);
             if (rc) return rc;
         }
     } else {
@@ -687,7 +687,7 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
         }
         rc = run_decode(ctx, cnt, nt, cfg->max_iter, cfg->check_warmup, cfg->check_interval,
                         cfg->damping, cfg->precision, err, errlen, learned, cfg->engine,
-                        cfg->precision == JBP_FP32 ? cfg->slots : 0);
+                        cfg->precision == JBP_FP32 ? cfg->slots : 0, cfg->graph != 0);
         ctx->tp.order = nullptr;
         if (rc) return rc;
         dev_ms += ctx->last_ms[1];

@@ -140,7 +140,7 @@ void free_noise(jbp_ctx *ctx);
 int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmup, int interval,
                double damping, int precision, char *err, size_t errlen,
                const LearnedInfer *learned = nullptr, int engine = JBP_ENGINE_AUTO,
-               int slots = 0);
+               int slots = 0, bool use_graph = false);
 void free_gen(jbp_ctx *ctx);
 void free_logicals(jbp_ctx *ctx);
 void free_pipe(jbp_ctx *ctx);

@@ -22,6 +22,8 @@
 //   emit    hard decisions of the stopped trials to their bit planes
 //   load    syndrome words of the refilled slots (emit and load share one
 //           launch)
+// Steps go out in batches of four (launched, or replayed from a captured
+// graph on request).
 // A refilled slot is marked fresh: the var and check kernels read its old
 // messages as 0 for one step, so no per-slot clearing pass is needed.
 // Trials are independent of the slot they run in, so results do not depend
@@ -518,7 +520,7 @@ size_t gen_carve_bytes(const DevGraph &g, int groups) {
 
 int gen_decode(const DevGraph &g, const GenState &s, TrialPlanes tp, int count, const NoiseTab &nt,
                int max_iter, int warmup, int interval, int *h_ctrl, hipEvent_t *evs,
-               hipStream_t st, int64_t *steps_out) {
+               hipStream_t st, int64_t *steps_out, bool use_graph) {
     *steps_out = 0;
     if (count <= 0) return 0;
     const int G = s.groups;
@@ -541,29 +543,65 @@ int gen_decode(const DevGraph &g, const GenState &s, TrialPlanes tp, int count, 
     const int64_t waves = (count + 64LL * G - 1) / (64LL * G);
     constexpr int kBatch = 4;
     const int64_t limit = (waves + 1) * (static_cast<int64_t>(max_iter) + 1) + 4 * kBatch;
-    int64_t steps = 0;
-    int pend = -1;  // event slot whose copy is pending
-    for (;;) {
+    // One batch of kBatch steps (kernels only).
+    auto enqueue_batch = [&]() -> hipError_t {
         for (int b = 0; b < kBatch; ++b) {
             launch_var(g, s, st);
             launch_check_any(g, s, bargs, st);  // check pass, then the book pass
             hipLaunchKernelGGL(gen_post_kernel, post_grid, dim3(256), 0, st, g, s, tp, ex, lx);
         }
+        return hipGetLastError();
+    };
+    // With use_graph the batch is replayed as a captured graph (kernel nodes
+    // only); a capture failure falls back to launches.
+    hipGraphExec_t exec = nullptr;
+    if (use_graph) {
+        hipGraph_t graph = nullptr;
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) == hipSuccess) {
+            const hipError_t ce = enqueue_batch();
+            const hipError_t ee = hipStreamEndCapture(st, &graph);
+            if (ce != hipSuccess || ee != hipSuccess || !graph ||
+                hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+                exec = nullptr;
+            if (graph) (void)hipGraphDestroy(graph);
+        }
+        (void)hipGetLastError();
+    }
+    auto cleanup = [&]() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    };
+    int64_t steps = 0;
+    int pend = -1;  // event slot whose copy is pending
+    for (;;) {
         steps += kBatch;
-        // finished count after this batch, read back one batch later
+        // the finished count after this batch, read back one batch later;
+        // slot alternates between two copies and events
         const int slot = static_cast<int>((steps / kBatch) & 1);
-        hipError_t e = hipMemcpyAsync(h_ctrl + slot, s.ctrl + 1, sizeof(int), hipMemcpyDeviceToHost, st);
+        hipError_t e = exec ? hipGraphLaunch(exec, st) : enqueue_batch();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h_ctrl + slot, s.ctrl + 1, sizeof(int), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipEventRecord(evs[slot], st);
-        if (e != hipSuccess) return static_cast<int>(e);
+        if (e != hipSuccess) {
+            cleanup();
+            return static_cast<int>(e);
+        }
         if (pend >= 0) {
             e = hipEventSynchronize(evs[pend]);
-            if (e != hipSuccess) return static_cast<int>(e);
+            if (e != hipSuccess) {
+                cleanup();
+                return static_cast<int>(e);
+            }
             if (h_ctrl[pend] >= count) break;
         }
         pend = slot;
-        if (steps > limit) return -1;
+        if (steps > limit) {
+            (void)hipStreamSynchronize(st);
+            cleanup();
+            return -1;
+        }
     }
     hipError_t e = hipStreamSynchronize(st);
+    cleanup();
     *steps_out = steps;
     return static_cast<int>(e);
 }

@@ -34,9 +34,10 @@ void gen_carve(const DevGraph &g, int groups, void *base, GenState &s);
 
 // Decode trials 0..count-1 of tp (syndrome planes in place) to est planes,
 // iterations and syn_ok.  h_ctrl: pinned host ints [2]; evs: two events.
+// Step batches are replayed from a captured graph when use_graph.
 // Returns 0 or a hipError_t value, -1 when the step bound is exceeded.
 int gen_decode(const DevGraph &g, const GenState &s, TrialPlanes tp, int count, const NoiseTab &nt,
                int max_iter, int warmup, int interval, int *h_ctrl, hipEvent_t *evs,
-               hipStream_t st, int64_t *steps_out);
+               hipStream_t st, int64_t *steps_out, bool use_graph);
 
 }  // namespace jbp

@@ -222,3 +222,19 @@ def test_decode_order_does_not_change_results(main_decoder, precision, engine, n
     _, b = main_decoder.simulate(106, n, 0.05, max_iter=80, precision=precision, engine=engine,
                                  schedule=1)
     assert per_trial_equal(a, b)
+
+
+def test_hbm_engine_graph_replay_equals_eager(main_decoder, small_hgp):
+    # With graph=True the engine replays a captured step batch; every such
+    # call replays it many times, with eager calls (and their
+    # synchronizations) in between, and must equal the launched steps per
+    # trial.
+    _, dec = small_hgp
+    for d, n, p in ((main_decoder, 3000, 0.05), (dec, 3000, 0.06)):
+        for rep in range(2):
+            print("graph vs eager", n, p, rep, flush=True)
+            _, g = d.simulate(51 + rep, n, p, max_iter=80, precision=jbp.FP32,
+                              engine=jbp.ENGINE_HBM, slots=128, want_est=True, graph=True)
+            _, e = d.simulate(51 + rep, n, p, max_iter=80, precision=jbp.FP32,
+                              engine=jbp.ENGINE_HBM, slots=128, want_est=True)
+            assert per_trial_equal(g, e) and np.array_equal(g["est"], e["est"])
