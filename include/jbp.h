@@ -103,7 +103,7 @@ int jbp_decode_batch(jbp_ctx *ctx, int count, const uint32_t *sx_bits, const uin
                      size_t errlen);
 
 /* Monte-Carlo configuration: the inner loop of the reference's --simulate
- * path (jointbp_ets.cpp:5931-6207) with PP off. */
+ * path (jointbp_ets.cpp:5931-6207), post-processing off or on. */
 typedef struct jbp_sim_cfg {
     uint64_t base_seed;  /* --seed                                            */
     int64_t first_trial; /* trial index of the first trial (reference: 1)     */
@@ -127,6 +127,10 @@ typedef struct jbp_sim_cfg {
                              results are the same either way                 */
     int graph;            /* 1: replay the HBM engine's steps from captured
                              graphs (0: launch them; measured 2% faster)     */
+    int pp;               /* 1: the reference's post-processing (ETS / flip /
+                             OSD, default on there; off = --no-pp) with the
+                             tables installed by jbp_set_pp_tables          */
+    int flip_hist;        /* --flip-hist (reference default 5)              */
 } jbp_sim_cfg;
 
 #define JBP_MAX_P_LIST 4096
@@ -147,13 +151,30 @@ typedef struct jbp_trial_out {
     uint8_t *exact;           /* est == err                                  */
     int *err;                 /* n_trials x nvars, sampled error (optional)  */
     int *est;                 /* n_trials x nvars, final estimate (optional) */
+    uint8_t *pp_flags;        /* n_trials, JBP_PPF_* bits (optional)         */
 } jbp_trial_out;
+
+/* Per-trial post-processing flags (jbp_trial_out.pp_flags). */
+#define JBP_PPF_BP_OK 1        /* BP alone matched the syndrome             */
+#define JBP_PPF_PP_SUCCESS 2   /* success owed to a repair                  */
+#define JBP_PPF_ETS_SAVED 4    /* ... by ETS                                */
+#define JBP_PPF_FLIP_SAVED 8   /* ... by a flip set                         */
+#define JBP_PPF_OSD_SAVED 16   /* ... by OSD                                */
+#define JBP_PPF_ETS_USED 32    /* an ETS repair was applied                 */
+#define JBP_PPF_ETS6 64        /* ... from the 6-variable table             */
+#define JBP_PPF_ETS12 128      /* ... from the 12-variable table            */
 
 /* Aggregates in the reference's Run Summary terms (jointbp_ets.cpp:6414-6449). */
 typedef struct jbp_run_stats {
     int64_t done, failures, bp_failures, exact, total_iters;
     double device_ms;  /* device time of the decode launches (HIP events) */
     double elapsed_ms; /* wall time of the whole call                      */
+    /* post-processing counters, as the Run Summary names them
+     * (jointbp_ets.cpp:6113-6228); zero with pp off */
+    int64_t pp_success, flip_pp_success, osd_pp_success, stab_success;
+    int64_t ets_used, ets_saved, ets6_used, ets6_saved, ets12_used, ets12_saved;
+    int64_t pp_trials; /* trials handed to the post-processing stage     */
+    double pp_ms;      /* wall time of that stage                         */
 } jbp_run_stats;
 
 /* Sample, decode and judge trials first_trial .. first_trial+n_trials-1 on
@@ -163,6 +184,47 @@ typedef struct jbp_run_stats {
  * for the sampled errors, syndromes and residual tests. */
 int jbp_simulate(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *per_trial,
                  jbp_run_stats *stats, char *err, size_t errlen);
+
+/* ---------------------------------------------------------------------------
+ * Post-processing (PP) of unconverged decodes: the reference's ETS lookup,
+ * flip-set solve and ordered-statistics repair (jointbp_ets.cpp:2736-3074 in
+ * the decode loop from iteration 100, 5993-6197 after it).  With cfg.pp = 1,
+ * jbp_simulate decodes as usual, then re-decodes the trials still
+ * unconverged after min(99, max_iter - 1) iterations with FP64 BP on the
+ * device one iteration per launch, running the reference's trigger logic and
+ * repairs on host threads between launches; per trial the outcome is the
+ * reference CPU decoder's.
+ * ------------------------------------------------------------------------- */
+typedef struct jbp_pp_tables jbp_pp_tables;
+
+/* Empty set of trapping-set tables (host memory, no device needed). */
+int jbp_pp_tables_create(jbp_pp_tables **out);
+void jbp_pp_tables_free(jbp_pp_tables *t);
+
+/* Adds one table group, read from the reference's ETS text files (the X
+ * side from path_x, the Z side from path_z, either may be NULL):
+ * "size"-variable trapping sets of the code described by graph, variables
+ * "Vc:i" = c*P + i, checks "Cr:i" = r*P + i (read_ets_file,
+ * jointbp_ets.cpp:3199-3258).  counts[s] = entries loaded for side s, 0 for
+ * an empty file, -1 for a missing one.  The group is kept when either side
+ * has entries; groups are tried in the order they were added (the
+ * reference's order is 6, 8, 8p4, 10, 12, ..., 22, jointbp_ets.cpp:
+ * 5309-5327). */
+int jbp_pp_tables_add(jbp_pp_tables *t, const jbp_graph_desc *graph, int64_t P, int size,
+                      const char *label, const char *path_x, const char *path_z, int counts[2],
+                      char *err, size_t errlen);
+
+/* Installs a copy of the tables on the context (NULL: none). */
+int jbp_set_pp_tables(jbp_ctx *ctx, const jbp_pp_tables *t, char *err, size_t errlen);
+
+/* One repair on one estimate, on the host (no device): method 0 = ETS over
+ * the groups of t, 1 = flip set `vars` (n_vars entries), 2 = OSD with the
+ * per-qubit |LLR| abs_llr; side 0 repairs the X part against the X-check
+ * syndrome, 1 the Z part.  est (nvars states 0..3) is updated in place when
+ * the repair applies; *applied = 1 then (for ETS, the group's size). */
+int jbp_pp_repair(const jbp_graph_desc *graph, const jbp_pp_tables *t, int method, int side,
+                  const int *syndrome, const double *abs_llr, const int *vars, int n_vars,
+                  int *est, int *applied, char *err, size_t errlen);
 
 /* Sample only: errors of trials first..first+count-1 (count x nvars states)
  * and their syndromes (count x mX, count x mZ ints; either may be NULL). */

@@ -16,8 +16,8 @@ import weakref
 
 import numpy as np
 
-__all__ = ["Code", "Decoder", "LearnedModel", "JbpError", "lib", "LIB_PATH", "FP32", "FP64",
-           "prior_from_p"]
+__all__ = ["Code", "Decoder", "LearnedModel", "PPTables", "JbpError", "lib", "LIB_PATH", "FP32",
+           "FP64", "prior_from_p", "pp_repair", "default_ets_dir"]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libjbp.so")
@@ -56,18 +56,35 @@ class SimCfg(C.Structure):
                 ("precision", C.c_int), ("check_warmup", C.c_int), ("check_interval", C.c_int),
                 ("slots", C.c_int), ("block", C.c_int), ("min_fails", C.c_int64),
                 ("p_list", C.POINTER(C.c_double)), ("n_p", C.c_int), ("engine", C.c_int),
-                ("schedule", C.c_int), ("graph", C.c_int)]
+                ("schedule", C.c_int), ("graph", C.c_int), ("pp", C.c_int), ("flip_hist", C.c_int)]
 
 
 class TrialOut(C.Structure):
     _fields_ = [("iterations", C.POINTER(C.c_int)), ("syn_match", C.POINTER(C.c_uint8)),
                 ("success", C.POINTER(C.c_uint8)), ("exact", C.POINTER(C.c_uint8)),
-                ("err", C.POINTER(C.c_int)), ("est", C.POINTER(C.c_int))]
+                ("err", C.POINTER(C.c_int)), ("est", C.POINTER(C.c_int)),
+                ("pp_flags", C.POINTER(C.c_uint8))]
 
 
 class RunStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("done", "failures", "bp_failures", "exact", "total_iters")] + [
-        ("device_ms", C.c_double), ("elapsed_ms", C.c_double)]
+        ("device_ms", C.c_double), ("elapsed_ms", C.c_double)] + [
+        (n, C.c_int64) for n in ("pp_success", "flip_pp_success", "osd_pp_success", "stab_success",
+                                 "ets_used", "ets_saved", "ets6_used", "ets6_saved", "ets12_used",
+                                 "ets12_saved", "pp_trials")] + [("pp_ms", C.c_double)]
+
+
+# per-trial post-processing flags (jbp_trial_out.pp_flags)
+PPF_BP_OK, PPF_PP_SUCCESS, PPF_ETS_SAVED, PPF_FLIP_SAVED = 1, 2, 4, 8
+PPF_OSD_SAVED, PPF_ETS_USED, PPF_ETS6, PPF_ETS12 = 16, 32, 64, 128
+# jbp_pp_repair methods
+PP_ETS, PP_FLIP, PP_OSD = 0, 1, 2
+# the reference's table groups in the order it loads and tries them
+# (size, label, file name), jointbp_ets.cpp:5229-5327
+ETS_GROUPS = [(6, "6", "ets_6_2.txt"), (8, "8", "ets_8_2.txt"), (8, "8p4", "ets_8_2_path4.txt"),
+              (10, "10", "ets_10_2.txt"), (12, "12", "ets_12_2.txt"), (14, "14", "ets_14_2.txt"),
+              (16, "16", "ets_16_2.txt"), (18, "18", "ets_18_2.txt"), (20, "20", "ets_20_2.txt"),
+              (22, "22", "ets_22_2.txt")]
 
 
 _LIB = None
@@ -114,6 +131,11 @@ def lib():
         "jbp_code_logicals": (i32, [vp, i32, C.POINTER(i64), C.POINTER(i64), vp, vp]),
         "jbp_set_logicals": (i32, [vp, i32, i64, vp, vp, cp, sz]),
         "jbp_tune": (i32, [vp, C.POINTER(SimCfg), vp, vp, C.POINTER(i32), cp, sz]),
+        "jbp_pp_tables_create": (i32, [C.POINTER(vp)]),
+        "jbp_pp_tables_free": (None, [vp]),
+        "jbp_pp_tables_add": (i32, [vp, C.POINTER(GraphDesc), i64, i32, cp, cp, cp, vp, cp, sz]),
+        "jbp_set_pp_tables": (i32, [vp, vp, cp, sz]),
+        "jbp_pp_repair": (i32, [C.POINTER(GraphDesc), vp, i32, i32, vp, vp, vp, i32, vp, vp, cp, sz]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -248,6 +270,74 @@ class Code:
                 np.ctypeslib.as_array(ev, shape=(E,)).copy())
 
 
+class PPTables:
+    """Trapping-set tables for post-processing (jbp_pp_tables_*): host
+    memory, no device needed."""
+
+    def __init__(self, code):
+        self.code = code
+        self._graph = code.graph()
+        h = C.c_void_p()
+        rc = lib().jbp_pp_tables_create(C.byref(h))
+        _check(rc, C.create_string_buffer(b"jbp_pp_tables_create failed"))
+        self._h = h
+        self.loaded = []  # (label, count_x, count_z)
+
+    def add(self, size, label, path_x, path_z):
+        """One group from the reference's text files; returns (count_x,
+        count_z): entries, 0 for an empty file, -1 for a missing one."""
+        counts = np.zeros(2, np.int32)
+        e = _errbuf()
+        enc = lambda x: None if x is None else os.fspath(x).encode()
+        _check(lib().jbp_pp_tables_add(self._h, C.byref(self._graph), self.code.P, size,
+                                       label.encode(), enc(path_x), enc(path_z), _ptr(counts),
+                                       e, 512), e)
+        self.loaded.append((label, int(counts[0]), int(counts[1])))
+        return int(counts[0]), int(counts[1])
+
+    @classmethod
+    def from_dir(cls, code, directory):
+        """The reference's default set: every group of ETS_GROUPS read from
+        directory/<file> for both sides."""
+        t = cls(code)
+        for size, label, name in ETS_GROUPS:
+            path = os.path.join(os.fspath(directory), name)
+            t.add(size, label, path, path)
+        return t
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().jbp_pp_tables_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_ets_dir(params_path):
+    """Where the reference looks for a params file's tables: the directory
+    named after the file without its extension (jointbp_ets.cpp:5031)."""
+    return os.path.splitext(os.fspath(params_path))[0]
+
+
+def pp_repair(code, method, side, syndrome, est, tables=None, abs_llr=None, flip_vars=None):
+    """One host-side repair (jbp_pp_repair): returns (est_after, applied)."""
+    g = code.graph()
+    syn = np.ascontiguousarray(syndrome, dtype=np.int32)
+    out = np.ascontiguousarray(est, dtype=np.int32).copy()
+    llr = None if abs_llr is None else np.ascontiguousarray(abs_llr, dtype=np.float64)
+    fv = None if flip_vars is None else np.ascontiguousarray(flip_vars, dtype=np.int32)
+    applied = C.c_int(0)
+    e = _errbuf()
+    _check(lib().jbp_pp_repair(C.byref(g), tables._h if tables is not None else None, method, side,
+                               _ptr(syn), _ptr(llr), _ptr(fv), 0 if fv is None else int(fv.size),
+                               _ptr(out), C.byref(applied), e, 512), e)
+    return out, applied.value
+
+
 class Decoder:
     """A decoder context on one device: cuda_bp_create / cuda_bp_decode /
     cuda_bp_destroy (jointbp_cuda.h:48-63) plus the batched calls."""
@@ -276,6 +366,11 @@ class Decoder:
             off = np.ascontiguousarray(off, np.int64)
             q = np.ascontiguousarray(q, np.int32)
             _check(lib().jbp_set_logicals(self._h, side, off.size - 1, _ptr(off), _ptr(q), e, 512), e)
+
+    def set_pp_tables(self, tables):
+        """Install trapping-set tables for simulate(pp=True) (None: none)."""
+        e = _errbuf()
+        _check(lib().jbp_set_pp_tables(self._h, tables._h if tables is not None else None, e, 512), e)
 
     def destroy(self):
         """Free the context; learned models bound to it are freed first."""
@@ -387,7 +482,7 @@ class Decoder:
     def simulate(self, base_seed, n_trials, p, max_iter=50, first_trial=1, damping=0.0,
                  precision=FP32, check_warmup=0, check_interval=1, min_fails=0, slots=0, block=0,
                  per_trial=True, want_err=False, want_est=False, engine=0, schedule=0,
-                 graph=False, _model=None):
+                 graph=False, pp=False, flip_hist=5, _model=None):
         """The --simulate --no-pp inner loop on the device.  Returns (stats,
         per-trial dict or None).  p may be a sequence: trial index idx then
         uses p[idx % len(p)] (interleaved mixed-noise run).  engine selects
@@ -395,7 +490,9 @@ class Decoder:
         number of resident trials of the HBM engine (FP64: of the parity
         decoder).  schedule=1 decodes in index order instead of heaviest
         syndrome first (results are the same); graph=True replays the HBM
-        engine's step batches from a captured graph."""
+        engine's step batches from a captured graph.  pp=True runs the
+        reference's post-processing with the tables of set_pp_tables
+        (flip_hist: --flip-hist); per-trial "pp_flags" then holds PPF_* bits."""
         plist = None
         if np.ndim(p) > 0:
             plist = np.ascontiguousarray(p, dtype=np.float64)
@@ -403,14 +500,16 @@ class Decoder:
         cfg = SimCfg(base_seed, first_trial, n_trials, p, max_iter, damping, precision, check_warmup,
                      check_interval, slots, block, min_fails,
                      plist.ctypes.data_as(C.POINTER(C.c_double)) if plist is not None else None,
-                     int(plist.size) if plist is not None else 0, engine, schedule, int(graph))
+                     int(plist.size) if plist is not None else 0, engine, schedule, int(graph),
+                     int(pp), flip_hist)
         out = None
         arrays = {}
         if per_trial:
             arrays = {"iterations": np.zeros(n_trials, np.int32),
                       "syn_match": np.zeros(n_trials, np.uint8),
                       "success": np.zeros(n_trials, np.uint8),
-                      "exact": np.zeros(n_trials, np.uint8)}
+                      "exact": np.zeros(n_trials, np.uint8),
+                      "pp_flags": np.zeros(n_trials, np.uint8)}
             if want_err:
                 arrays["err"] = np.zeros((n_trials, self.code.n), np.int32)
             if want_est:

@@ -582,6 +582,7 @@ int simulate_pipelined(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_run_stats &st, 
             st.failures += s ? 0 : 1;
             st.bp_failures += ok[i] ? 0 : 1;
             st.exact += exact[i] ? 1 : 0;
+            st.stab_success += (s && !exact[i] && judged) ? 1 : 0;
             if (cfg->min_fails > 0 && st.failures >= cfg->min_fails) {
                 take = i + 1;
                 stop = true;
@@ -591,6 +592,7 @@ int simulate_pipelined(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_run_stats &st, 
         if (pt) {
             for (int i = 0; i < take; ++i) {
                 const uint8_t s = judged ? succ[i] : static_cast<uint8_t>(ok[i] && exact[i]);
+                if (pt->pp_flags) pt->pp_flags[done + i] = ok[i] ? JBP_PPF_BP_OK : 0;
                 if (pt->iterations) pt->iterations[done + i] = b.h_iters[i];
                 if (pt->syn_match) pt->syn_match[done + i] = ok[i];
                 if (pt->success) pt->success[done + i] = s;
@@ -647,10 +649,21 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
         ctx->f64_slots = cfg->slots;
     }
     const int64_t total = cfg->n_trials;
+    const bool pp_on = cfg->pp != 0;
+    if (pp_on && learned) {
+        set_err(err, errlen, "post-processing is not combined with the learned decoder");
+        return JBP_ERR_UNSUPPORTED;
+    }
+    if (pp_on && cfg->flip_hist < 0) {
+        set_err(err, errlen, "flip_hist must be 0 or more");
+        return JBP_ERR_ARG;
+    }
+    // decode iterations before the first post-processing trigger can fire
+    const int pp_h = std::min(99, cfg->max_iter - 1);
     const bool lds = cfg->precision == JBP_FP32 && cfg->damping <= 0.0 && ctx->qc &&
                      (cfg->engine == JBP_ENGINE_LDS ||
                       (cfg->engine == JBP_ENGINE_AUTO) || learned);
-    if (lds && total > kPipeChunk && !(pt && (pt->err || pt->est))) {
+    if (lds && total > kPipeChunk && !pp_on && !(pt && (pt->err || pt->est))) {
         int rc0 = simulate_pipelined(ctx, cfg, st, np, learned, pt, err, errlen);
         if (rc0) return rc0;
         st.elapsed_ms =
@@ -685,12 +698,33 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
             HIPCHK(hipGetLastError());
             ctx->tp.order = ctx->d_order;
         }
-        rc = run_decode(ctx, cnt, nt, cfg->max_iter, cfg->check_warmup, cfg->check_interval,
-                        cfg->damping, cfg->precision, err, errlen, learned, cfg->engine,
-                        cfg->precision == JBP_FP32 ? cfg->slots : 0, cfg->graph != 0);
+        if (!pp_on || pp_h > 0) {
+            rc = run_decode(ctx, cnt, nt, pp_on ? pp_h : cfg->max_iter, cfg->check_warmup,
+                            cfg->check_interval, cfg->damping, cfg->precision, err, errlen, learned,
+                            cfg->engine, cfg->precision == JBP_FP32 ? cfg->slots : 0, cfg->graph != 0);
+            ctx->tp.order = nullptr;
+            if (rc) return rc;
+            dev_ms += ctx->last_ms[1];
+        }
         ctx->tp.order = nullptr;
-        if (rc) return rc;
-        dev_ms += ctx->last_ms[1];
+        // post-processing: the trials still unconverged after pp_h iterations
+        std::vector<int> pp_list;
+        std::vector<PPOutcome> ppo;
+        std::vector<int> pp_of;
+        if (pp_on) {
+            if (pp_h > 0) {
+                HIPCHK(hipMemcpyAsync(ok.data(), ctx->tp.syn_ok, cnt, hipMemcpyDeviceToHost, ctx->stream));
+                HIPCHK(hipStreamSynchronize(ctx->stream));
+            }
+            for (int i = 0; i < cnt; ++i)
+                if (pp_h <= 0 || !ok[i]) pp_list.push_back(i);
+            auto p0 = std::chrono::steady_clock::now();
+            rc = run_pp(ctx, nt, cfg->max_iter, cfg->damping, cfg->flip_hist, pp_list, ppo, err, errlen);
+            if (rc) return rc;
+            st.pp_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - p0).count();
+            pp_of.assign(cnt, -1);
+            for (size_t k = 0; k < pp_list.size(); ++k) pp_of[pp_list[k]] = static_cast<int>(k);
+        }
         if (ctx->have_logicals) {
             HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
             launch_logical(g, ctx->lg[0], ctx->lg[1], ctx->tp, cnt, ctx->stream);
@@ -733,24 +767,77 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
             }
         }
         int take = cnt;
+        std::vector<uint8_t> flags(cnt, 0);
         for (int i = 0; i < cnt; ++i) {
             if (!judged) succ[i] = (ok[i] && exact[i]) ? 1 : 0;
+            const PPOutcome *po = pp_on && pp_of[i] >= 0 ? &ppo[pp_of[i]] : nullptr;
+            const bool bp_ok = po ? po->bp_ok : ok[i] != 0;
+            const bool success = succ[i] != 0;
             st.total_iters += iters[i];
-            st.failures += succ[i] ? 0 : 1;
-            st.bp_failures += ok[i] ? 0 : 1;
+            st.failures += success ? 0 : 1;
+            st.bp_failures += bp_ok ? 0 : 1;
             st.exact += exact[i] ? 1 : 0;
+            st.stab_success += (success && !exact[i] && judged) ? 1 : 0;
+            uint8_t f = bp_ok ? JBP_PPF_BP_OK : 0;
+            if (po) {
+                // the reference's counting (jointbp_ets.cpp:6113-6228)
+                bool used_any = po->used_any || po->used6 || po->used12;
+                bool used6 = po->used6, used12 = po->used12;
+                if (po->pp_success && used_any) {
+                    st.ets_used++;
+                    st.ets6_used += used6 ? 1 : 0;
+                    st.ets12_used += used12 ? 1 : 0;
+                }
+                bool ets_ok = po->pp_ets, flip_ok = po->pp_flip, osd_ok = false;
+                if (po->post_run) {
+                    used_any = used_any || po->post.ets_applied;
+                    used6 = used6 || po->post.used6;
+                    used12 = used12 || po->post.used12;
+                    st.ets_used += po->post.ets_applied ? 1 : 0;
+                    st.ets6_used += used6 ? 1 : 0;
+                    st.ets12_used += used12 ? 1 : 0;
+                    ets_ok = ets_ok || po->post.ets_ok;
+                    flip_ok = flip_ok || po->post.flip_ok;
+                    osd_ok = po->post.osd_ok;
+                }
+                if (success && ets_ok) {
+                    st.ets_saved++;
+                    st.ets6_saved += used6 ? 1 : 0;
+                    st.ets12_saved += used12 ? 1 : 0;
+                    f |= JBP_PPF_ETS_SAVED;
+                }
+                if (success && flip_ok) {
+                    st.flip_pp_success++;
+                    f |= JBP_PPF_FLIP_SAVED;
+                }
+                const bool osd_success = osd_ok || (po->pp_success && !po->pp_ets && !po->pp_flip);
+                if (success && (po->pp_success || ets_ok || flip_ok || osd_ok)) {
+                    st.pp_success++;
+                    f |= JBP_PPF_PP_SUCCESS;
+                }
+                if (success && osd_success) {
+                    st.osd_pp_success++;
+                    f |= JBP_PPF_OSD_SAVED;
+                }
+                f |= (used_any ? JBP_PPF_ETS_USED : 0) | (used6 ? JBP_PPF_ETS6 : 0) |
+                     (used12 ? JBP_PPF_ETS12 : 0);
+            }
+            flags[i] = f;
             if (cfg->min_fails > 0 && st.failures >= cfg->min_fails) {
                 take = i + 1;
                 stop = true;
                 break;
             }
         }
+        st.pp_trials += static_cast<int64_t>(std::count_if(
+            pp_list.begin(), pp_list.end(), [&](int i) { return i < take; }));
         if (pt) {
             for (int i = 0; i < take; ++i) {
                 if (pt->iterations) pt->iterations[done + i] = iters[i];
                 if (pt->syn_match) pt->syn_match[done + i] = ok[i];
                 if (pt->success) pt->success[done + i] = succ[i];
                 if (pt->exact) pt->exact[done + i] = exact[i];
+                if (pt->pp_flags) pt->pp_flags[done + i] = flags[i];
             }
         }
         done += take;
@@ -899,6 +986,12 @@ int jbp_create(const jbp_graph_desc *g, int device_id, jbp_ctx **out, char *err,
         delete ctx;
         return rc;
     }
+    {
+        const int m[2] = {ctx->hs[0].m, ctx->hs[1].m};
+        const int *off[2] = {ctx->hs[0].chk_off.data(), ctx->hs[1].chk_off.data()};
+        const int *var[2] = {ctx->hs[0].edge_var.data(), ctx->hs[1].edge_var.data()};
+        ctx->pp_graph = pp::graph_from_csr(ctx->n, m, off, var);
+    }
     auto fail = [&](int code) {
         jbp_destroy(ctx);
         return code;
@@ -1007,6 +1100,7 @@ void jbp_destroy(jbp_ctx *ctx) {
     if (ctx->g1_exec) (void)hipGraphExecDestroy(ctx->g1_exec);
     if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     free_pipe(ctx);
+    free_pp(ctx);
     free_chunk(ctx);
     free_noise(ctx);
     free_gen(ctx);

@@ -12,6 +12,7 @@
 #include "jbp_common.h"
 #include "jbp_gen.h"
 #include "jbp_learned.h"
+#include "jbp_pp.h"
 
 namespace jbp {
 
@@ -39,6 +40,31 @@ struct PipeBuf {
     int *order = nullptr;
 };
 constexpr int kMaxPList = JBP_MAX_P_LIST;
+
+// Device buffers of the post-processing stage (jbp_pp_stage.hip).
+struct PPBuf {
+    int cap = 0;            // trial slots
+    int rec_cap = 0;        // iteration records per slot
+    size_t stride = 0;      // bytes of FP64 state per slot
+    char *state = nullptr;
+    int *list = nullptr;        // slots advanced by a launch
+    int *slot_trial = nullptr;  // chunk index of each slot's trial
+    int *rec = nullptr;         // [slot][iteration][4]
+    uint8_t *out = nullptr;     // [slot][2][n] estimate, previous estimate
+};
+
+// What the post-processing stage decided for one trial.
+struct PPOutcome {
+    bool bp_ok = false;  // BP itself matched the syndrome
+    bool ok = false;     // syndrome matched in the end (BP or a repair)
+    int iterations = 0;
+    // repair inside the decode loop
+    bool pp_success = false, pp_ets = false, pp_flip = false;
+    bool used_any = false, used6 = false, used12 = false;
+    // repair after the decode
+    bool post_run = false;
+    pp::PostOut post;
+};
 
 void set_err(char *err, size_t len, const char *fmt, ...);
 
@@ -109,6 +135,10 @@ struct jbp_ctx {
     std::vector<uintptr_t> g1_key;
     uint32_t *h_stage = nullptr;   // pinned: syndrome words in, outcome out
     jbp::PipeBuf pipe[2];
+    // post-processing: host graph, trapping-set tables, device buffers
+    jbp::pp::Graph pp_graph;
+    jbp::pp::Tables pp_tables;
+    jbp::PPBuf ppb;
     double last_ms[3] = {0, 0, 0};
     int64_t launches[3] = {0, 0, 0};
 };
@@ -144,6 +174,14 @@ int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int wa
 void free_gen(jbp_ctx *ctx);
 void free_logicals(jbp_ctx *ctx);
 void free_pipe(jbp_ctx *ctx);
+void free_pp(jbp_ctx *ctx);
+// Post-processing stage over the chunk trials `trials` (their syndrome
+// planes in ctx->tp), which the first decode left unconverged after
+// min(99, max_iter - 1) iterations: finishes their decode with the
+// reference's PP and writes estimate planes, iterations and syndrome flags
+// back into ctx->tp; outs[i] describes trials[i].
+int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int hist_window,
+           const std::vector<int> &trials, std::vector<PPOutcome> &outs, char *err, size_t errlen);
 
 int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_run_stats *stats,
                   const LearnedInfer *learned, char *err, size_t errlen);

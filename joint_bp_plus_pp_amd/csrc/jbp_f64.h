@@ -1,0 +1,172 @@
+// FP64 joint BP in the reference's exact operation order
+// (jointbp_ets.cpp:1943-2535): shared by the parity decoder and the
+// post-processing stepper.  Translation units including this header are
+// compiled with -ffp-contract=off (no FMA), and normalisation divides by the
+// left-to-right sum as normalize_msg does (jointbp_ets.cpp:650-657).
+#pragma once
+
+#include "jbp_common.h"
+
+namespace jbp {
+namespace f64 {
+
+struct M4 {
+    double v[4];
+};
+
+__device__ __forceinline__ M4 mul4(const M4 &a, const M4 &b) {
+    return M4{{a.v[0] * b.v[0], a.v[1] * b.v[1], a.v[2] * b.v[2], a.v[3] * b.v[3]}};
+}
+
+__device__ __forceinline__ void normalize4(M4 &m) {
+    double sum = m.v[0] + m.v[1] + m.v[2] + m.v[3];
+    if (sum <= 0.0) {
+        m.v[0] = m.v[1] = m.v[2] = m.v[3] = 0.25;
+        return;
+    }
+    m.v[0] /= sum;
+    m.v[1] /= sum;
+    m.v[2] /= sum;
+    m.v[3] /= sum;
+}
+
+// scaled_mix (jointbp_ets.cpp:1903-1911)
+__device__ __forceinline__ M4 mix4(const M4 &nw, const M4 &old, double d) {
+    if (d <= 0.0) return nw;
+    M4 out;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) out.v[i] = (1.0 - d) * nw.v[i] + d * old.v[i];
+    normalize4(out);
+    return out;
+}
+
+// Check update of one check (jointbp_ets.cpp:2373-2449).
+template <int MAXC>
+__device__ void check_update(const DevSide &sd, int side, int c, int syn, const double *v2c,
+                             double *c2v) {
+    const int off = sd.chk_off[c];
+    const int deg = sd.chk_off[c + 1] - off;
+    double q0[MAXC], q1[MAXC], pe[MAXC + 1], po[MAXC + 1];
+    pe[0] = 1.0;
+    po[0] = 0.0;
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+        if (i < deg) {
+            const double *m = v2c + 4 * static_cast<size_t>(off + i);
+            q0[i] = side == 0 ? m[0] + m[2] : m[0] + m[1];
+            q1[i] = side == 0 ? m[1] + m[3] : m[2] + m[3];
+            pe[i + 1] = pe[i] * q0[i] + po[i] * q1[i];
+            po[i + 1] = pe[i] * q1[i] + po[i] * q0[i];
+        }
+    }
+    double se = 1.0, so = 0.0;
+#pragma unroll
+    for (int i = MAXC - 1; i >= 0; --i) {
+        if (i < deg) {
+            double p_even = pe[i] * se + po[i] * so;
+            double p_odd = pe[i] * so + po[i] * se;
+            double v0 = syn == 0 ? p_even : p_odd;
+            double v1 = syn == 0 ? p_odd : p_even;
+            double sum = side == 0 ? ((v0 + v1) + v0) + v1 : ((v0 + v0) + v1) + v1;
+            double *o = c2v + 2 * static_cast<size_t>(off + i);
+            if (sum <= 0.0) {
+                o[0] = 0.25;
+                o[1] = 0.25;
+            } else {
+                o[0] = v0 / sum;
+                o[1] = v1 / sum;
+            }
+            double ne = se * q0[i] + so * q1[i];
+            double no = se * q1[i] + so * q0[i];
+            se = ne;
+            so = no;
+        }
+    }
+}
+
+__device__ __forceinline__ M4 c2v_msg(const double *c2v, int e, int side) {
+    double a = c2v[2 * static_cast<size_t>(e)];
+    double b = c2v[2 * static_cast<size_t>(e) + 1];
+    return side == 0 ? M4{{a, b, a, b}} : M4{{a, a, b, b}};
+}
+
+// Joint variable update + hard decision (jointbp_ets.cpp:2451-2519).  The
+// suffix products are formed on the fly while walking the edges backwards;
+// each outgoing message uses the same operands as the reference's
+// pref[i] (*) suf[i+1].
+//
+// Messages: v2c[s] (4 doubles per edge), c2v[s] (2 per edge).  Returns the
+// hard decision; tot (optional) receives the normalised marginal.
+template <int MAXV>
+__device__ int var_update(const DevGraph &g, int v, const M4 &prior, double damping,
+                          double *const *v2c, const double *const *c2v, double *tot) {
+    const M4 ident{{1.0, 1.0, 1.0, 1.0}};
+    int b[2], deg[2];
+    M4 pref[2][MAXV + 1];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        b[s] = g.side[s].var_off[v];
+        deg[s] = g.side[s].var_off[v + 1] - b[s];
+        pref[s][0] = ident;
+#pragma unroll
+        for (int i = 0; i < MAXV; ++i)
+            if (i < deg[s])
+                pref[s][i + 1] = mul4(pref[s][i], c2v_msg(c2v[s], g.side[s].var_edge[b[s] + i], s));
+    }
+    M4 allx = ident, allz = ident;
+#pragma unroll
+    for (int i = 0; i <= MAXV; ++i) {
+        if (i == deg[0]) allx = pref[0][i];
+        if (i == deg[1]) allz = pref[1][i];
+    }
+    M4 total = mul4(prior, mul4(allx, allz));
+    normalize4(total);
+    int best = 0;
+#pragma unroll
+    for (int k = 1; k < 4; ++k)
+        if (total.v[k] > total.v[best]) best = k;
+    if (tot) {
+        double *t = tot + 4 * static_cast<size_t>(v);
+        t[0] = total.v[0]; t[1] = total.v[1]; t[2] = total.v[2]; t[3] = total.v[3];
+    }
+
+    // X edges: ((prior (*) x_excl) (*) z_all)
+    M4 suf = ident;
+#pragma unroll
+    for (int i = MAXV - 1; i >= 0; --i) {
+        if (i < deg[0]) {
+            int e = g.side[0].var_edge[b[0] + i];
+            M4 out = mul4(mul4(prior, mul4(pref[0][i], suf)), allz);
+            normalize4(out);
+            double *slot = v2c[0] + 4 * static_cast<size_t>(e);
+            if (damping > 0.0) {
+                M4 old{{slot[0], slot[1], slot[2], slot[3]}};
+                out = mix4(out, old, damping);
+            }
+            slot[0] = out.v[0]; slot[1] = out.v[1]; slot[2] = out.v[2]; slot[3] = out.v[3];
+            suf = mul4(suf, c2v_msg(c2v[0], e, 0));
+        }
+    }
+    // Z edges: ((prior (*) x_all) (*) z_excl)
+    suf = ident;
+    const M4 px = mul4(prior, allx);
+#pragma unroll
+    for (int i = MAXV - 1; i >= 0; --i) {
+        if (i < deg[1]) {
+            int e = g.side[1].var_edge[b[1] + i];
+            M4 out = mul4(px, mul4(pref[1][i], suf));
+            normalize4(out);
+            double *slot = v2c[1] + 4 * static_cast<size_t>(e);
+            if (damping > 0.0) {
+                M4 old{{slot[0], slot[1], slot[2], slot[3]}};
+                out = mix4(out, old, damping);
+            }
+            slot[0] = out.v[0]; slot[1] = out.v[1]; slot[2] = out.v[2]; slot[3] = out.v[3];
+            suf = mul4(suf, c2v_msg(c2v[1], e, 1));
+        }
+    }
+    return best;
+}
+
+}  // namespace f64
+}  // namespace jbp

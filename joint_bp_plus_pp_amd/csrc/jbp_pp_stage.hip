@@ -1,0 +1,504 @@
+// Post-processing stage of a simulate chunk: belief propagation for the
+// trials that are still unconverged when the reference's PP triggers first
+// become possible, one iteration per launch on the device, with the PP
+// decisions and repairs (jbp_pp.cpp) on host threads between launches.
+//
+// Why the split works: the reference only considers PP from iteration 100
+// on (or at the last iteration, jointbp_ets.cpp:2563), so a trial that BP
+// finishes in H = min(99, max_iter - 1) iterations never reaches it.  Those
+// trials keep the ordinary decoders' results.  The rest are re-decoded here
+// from iteration 1 in FP64 with the reference's operation order, so their
+// estimates, iteration counts and marginals are the reference's own:
+//   phase A: iterations 1..H in one launch per batch, recording per
+//            iteration the unsatisfied-check counts and whether the estimate
+//            equals the previous / the one before (all the trigger logic
+//            reads before a trigger can fire);
+//   phase B: from iteration H+1, one launch per iteration; the host reads
+//            the records and both latest estimates, runs the trigger state
+//            machine and, where it fires, ETS / flip / OSD (OSD's |LLR| come
+//            from the marginals the kernel leaves in device memory, copied
+//            only for the trials that need them); trials that end at
+//            max_iter get the post-decode pass.
+// Final estimates, iteration counts and syndrome flags go back into the
+// chunk's planes, so the residual test and counters that follow treat these
+// trials like any other.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
+
+#include "jbp_ctx.h"
+#include "jbp_f64.h"
+
+namespace jbp {
+namespace {
+
+using namespace f64;
+
+// Per-slot state, in bytes from the slot's base: v2c[2] (4 doubles per
+// edge), c2v[2] (2 per edge), marginals (4 doubles per qubit), then the
+// estimate and the previous estimate (one byte per qubit each).
+struct StepSlot {
+    double *v2c[2];
+    double *c2v[2];
+    double *tot;
+    uint8_t *est;
+    uint8_t *prev;
+};
+
+__host__ __device__ inline size_t step_slot_bytes(int E0, int E1, int n) {
+    size_t b = (6 * static_cast<size_t>(E0 + E1) + 4 * static_cast<size_t>(n)) * sizeof(double) +
+               2 * static_cast<size_t>(n);
+    return (b + 255) & ~static_cast<size_t>(255);
+}
+
+__device__ inline StepSlot step_slot(const DevGraph &g, char *state, size_t stride, int slot) {
+    double *base = reinterpret_cast<double *>(state + static_cast<size_t>(slot) * stride);
+    StepSlot s;
+    s.v2c[0] = base;
+    s.v2c[1] = s.v2c[0] + 4 * static_cast<size_t>(g.side[0].E);
+    s.c2v[0] = s.v2c[1] + 4 * static_cast<size_t>(g.side[1].E);
+    s.c2v[1] = s.c2v[0] + 2 * static_cast<size_t>(g.side[0].E);
+    s.tot = s.c2v[1] + 2 * static_cast<size_t>(g.side[1].E);
+    s.est = reinterpret_cast<uint8_t *>(s.tot + 4 * static_cast<size_t>(g.n));
+    s.prev = s.est + g.n;
+    return s;
+}
+
+// Record of one iteration: {unsat X, unsat Z, flags, 0}; flags bit 0 = the
+// estimate equals the previous one, bit 1 = equals the one before that.
+constexpr int kRecInts = 4;
+
+// Block b advances the trial of slot list[b] by up to `steps` iterations
+// (stopping when both syndromes match).  init: start from the reference's
+// initial messages.  want_tot: keep the last iteration's marginals.  At the
+// end the block copies its estimate and previous estimate to out[b].
+template <int MAXC, int MAXV>
+__global__ void pp_step_kernel(DevGraph g, TrialPlanes tp, NoiseTab nt, const int *list,
+                               const int *slot_trial, int steps, int init, int want_tot,
+                               double damping, char *state, size_t stride, int *rec, uint8_t *out) {
+    __shared__ int s_cnt[2];
+    const int tid = threadIdx.x, bs = blockDim.x;
+    const int b = blockIdx.x;
+    const int slot = list[b];
+    const int trial = slot_trial[slot];
+    const StepSlot sp = step_slot(g, state, stride, slot);
+    const int mx = g.side[0].m, mz = g.side[1].m, n = g.n;
+    const double *pq = nt.prior + 4 * nt.entry(trial);
+    const M4 prior{{pq[0], pq[1], pq[2], pq[3]}};
+    const uint32_t *tsx = tp.syn_x + static_cast<size_t>(trial) * g.wm[0];
+    const uint32_t *tsz = tp.syn_z + static_cast<size_t>(trial) * g.wm[1];
+    if (init) {
+        for (int s = 0; s < 2; ++s)
+            for (int e = tid; e < g.side[s].E; e += bs) {
+                double *m = sp.v2c[s] + 4 * static_cast<size_t>(e);
+                m[0] = prior.v[0]; m[1] = prior.v[1]; m[2] = prior.v[2]; m[3] = prior.v[3];
+                sp.c2v[s][2 * static_cast<size_t>(e)] = 0.25;
+                sp.c2v[s][2 * static_cast<size_t>(e) + 1] = 0.25;
+            }
+        for (int v = tid; v < n; v += bs) sp.est[v] = sp.prev[v] = 0;
+        __syncthreads();
+    }
+    for (int j = 0; j < steps; ++j) {
+        if (tid == 0) s_cnt[0] = s_cnt[1] = 0;
+        for (int c = tid; c < mx + mz; c += bs) {
+            if (c < mx) {
+                check_update<MAXC>(g.side[0], 0, c, (tsx[c >> 5] >> (c & 31)) & 1, sp.v2c[0], sp.c2v[0]);
+            } else {
+                const int cz = c - mx;
+                check_update<MAXC>(g.side[1], 1, cz, (tsz[cz >> 5] >> (cz & 31)) & 1, sp.v2c[1],
+                                   sp.c2v[1]);
+            }
+        }
+        __syncthreads();
+        double *tot = (want_tot && j == steps - 1) ? sp.tot : nullptr;
+        int changed1 = 0, changed2 = 0;
+        for (int v = tid; v < n; v += bs) {
+            const int best = var_update<MAXV>(g, v, prior, damping, sp.v2c, sp.c2v, tot);
+            const int e1 = sp.est[v], e2 = sp.prev[v];
+            changed1 |= best != e1;
+            changed2 |= best != e2;
+            sp.prev[v] = static_cast<uint8_t>(e1);
+            sp.est[v] = static_cast<uint8_t>(best);
+        }
+        __syncthreads();
+        int ux = 0, uz = 0;
+        for (int c = tid; c < mx + mz; c += bs) {
+            const int s = c < mx ? 0 : 1;
+            const int cc = c < mx ? c : c - mx;
+            const DevSide &sd = g.side[s];
+            int par = 0;
+            for (int e = sd.chk_off[cc]; e < sd.chk_off[cc + 1]; ++e) {
+                const int st = sp.est[sd.edge_var[e]];
+                par ^= s == 0 ? (st & 1) : (st >> 1);
+            }
+            const uint32_t *t = s == 0 ? tsx : tsz;
+            if (par != static_cast<int>((t[cc >> 5] >> (cc & 31)) & 1u)) {
+                if (s == 0) ++ux;
+                else ++uz;
+            }
+        }
+        if (ux) atomicAdd(&s_cnt[0], ux);
+        if (uz) atomicAdd(&s_cnt[1], uz);
+        const int any1 = __syncthreads_or(changed1);
+        const int any2 = __syncthreads_or(changed2);
+        const int usx = s_cnt[0], usz = s_cnt[1];
+        if (tid == 0) {
+            int *r = rec + (static_cast<size_t>(b) * steps + j) * kRecInts;
+            r[0] = usx;
+            r[1] = usz;
+            r[2] = (any1 ? 0 : 1) | (any2 ? 0 : 2);
+            r[3] = 0;
+        }
+        __syncthreads();
+        if (usx + usz == 0) break;
+    }
+    uint8_t *o = out + static_cast<size_t>(b) * 2 * n;
+    for (int v = tid; v < n; v += bs) {
+        o[v] = sp.est[v];
+        o[n + v] = sp.prev[v];
+    }
+}
+
+void launch_step(const DevGraph &g, const TrialPlanes &tp, const NoiseTab &nt, const int *list,
+                 int count, const int *slot_trial, int steps, bool init, bool want_tot,
+                 double damping, char *state, size_t stride, int *rec, uint8_t *out,
+                 hipStream_t st) {
+    if (g.max_cdeg <= 16 && g.max_vdeg <= 4) {
+        hipLaunchKernelGGL((pp_step_kernel<16, 4>), dim3(count), dim3(256), 0, st, g, tp, nt, list,
+                           slot_trial, steps, init ? 1 : 0, want_tot ? 1 : 0, damping, state,
+                           stride, rec, out);
+    } else {
+        hipLaunchKernelGGL((pp_step_kernel<32, 16>), dim3(count), dim3(256), 0, st, g, tp, nt,
+                           list, slot_trial, steps, init ? 1 : 0, want_tot ? 1 : 0, damping, state,
+                           stride, rec, out);
+    }
+}
+
+// Host worker pool: run(count, fn) calls fn(0..count-1) across the workers
+// and the calling thread and returns when all calls are done.
+class Pool {
+public:
+    explicit Pool(int threads) {
+        for (int i = 1; i < threads; ++i) th_.emplace_back([this] { work(); });
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    void run(int count, const std::function<void(int)> &fn) {
+        if (count <= 0) return;
+        if (th_.empty() || count == 1) {
+            for (int i = 0; i < count; ++i) fn(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            fn_ = &fn;
+            count_ = count;
+            next_.store(0);
+            pending_ = static_cast<int>(th_.size());
+            ++round_;
+        }
+        cv_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [this] { return pending_ == 0; });
+        fn_ = nullptr;
+    }
+
+private:
+    void drain() {
+        for (int i; (i = next_.fetch_add(1)) < count_;) (*fn_)(i);
+    }
+    void work() {
+        long seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || round_ != seen; });
+                if (stop_) return;
+                seen = round_;
+            }
+            drain();
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)> *fn_ = nullptr;
+    int count_ = 0;
+    std::atomic<int> next_{0};
+    int pending_ = 0;
+    long round_ = 0;
+    bool stop_ = false;
+};
+
+int pool_threads() {
+    unsigned hc = std::thread::hardware_concurrency();
+    return static_cast<int>(std::max(1u, std::min(hc ? hc : 1u, 16u)));
+}
+
+// Trials per batch of the stage (each holds about 3 MB of FP64 state for
+// the included code, 22 MB for HGP(150x200)), and the state cap per batch.
+constexpr int kPPSlots = 1024;
+constexpr size_t kPPStateBytes = size_t(8) << 30;
+
+int ensure_pp_buffers(jbp_ctx *ctx, int slots, int rec_per_slot, char *err, size_t errlen) {
+    PPBuf &pb = ctx->ppb;
+    const DevGraph &g = ctx->dg;
+    const size_t stride = step_slot_bytes(g.side[0].E, g.side[1].E, g.n);
+    if (pb.cap >= slots && pb.rec_cap >= rec_per_slot && pb.stride == stride) return JBP_OK;
+    free_pp(ctx);
+    pb.cap = slots;
+    pb.rec_cap = std::max(rec_per_slot, 1);
+    pb.stride = stride;
+    HIPCHK(hipMalloc(&pb.state, stride * slots));
+    HIPCHK(hipMalloc(&pb.list, sizeof(int) * slots));
+    HIPCHK(hipMalloc(&pb.slot_trial, sizeof(int) * slots));
+    HIPCHK(hipMalloc(&pb.rec, sizeof(int) * kRecInts * static_cast<size_t>(slots) * pb.rec_cap));
+    HIPCHK(hipMalloc(&pb.out, 2 * static_cast<size_t>(g.n) * slots));
+    return JBP_OK;
+}
+
+void unpack_bits(const uint32_t *w, int count, uint8_t *out) {
+    for (int i = 0; i < count; ++i) out[i] = static_cast<uint8_t>((w[i >> 5] >> (i & 31)) & 1u);
+}
+
+}  // namespace
+
+void free_pp(jbp_ctx *ctx) {
+    PPBuf &pb = ctx->ppb;
+    for (void *p : {static_cast<void *>(pb.state), static_cast<void *>(pb.list),
+                    static_cast<void *>(pb.slot_trial), static_cast<void *>(pb.rec),
+                    static_cast<void *>(pb.out)})
+        if (p) (void)hipFree(p);
+    pb = PPBuf();
+}
+
+int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int hist_window,
+           const std::vector<int> &trials, std::vector<PPOutcome> &outs, char *err, size_t errlen) {
+    const DevGraph &g = ctx->dg;
+    const pp::Graph &hg = ctx->pp_graph;
+    const pp::Tables &tabs = ctx->pp_tables;
+    const int n = g.n, mx = g.side[0].m, mz = g.side[1].m;
+    const int H = std::min(99, max_iter - 1);  // iterations before any trigger can fire
+    pp::Params prm;
+    prm.max_iter = max_iter;
+    prm.hist_window = hist_window;
+    outs.assign(trials.size(), PPOutcome());
+    Pool pool(pool_threads());
+    hipStream_t st = ctx->stream;
+    const size_t stride = step_slot_bytes(g.side[0].E, g.side[1].E, n);
+    const size_t batch = std::max<size_t>(1, std::min<size_t>(kPPSlots, kPPStateBytes / stride));
+    for (size_t b0 = 0; b0 < trials.size(); b0 += batch) {
+        const int K = static_cast<int>(std::min<size_t>(batch, trials.size() - b0));
+        int rc = ensure_pp_buffers(ctx, K, std::max(H, 1), err, errlen);
+        if (rc) return rc;
+        PPBuf &pb = ctx->ppb;
+        // syndromes as bytes
+        std::vector<uint32_t> wx(static_cast<size_t>(K) * g.wm[0]), wz(static_cast<size_t>(K) * g.wm[1]);
+        for (int i = 0; i < K; ++i) {
+            const int t = trials[b0 + i];
+            HIPCHK(hipMemcpyAsync(&wx[static_cast<size_t>(i) * g.wm[0]],
+                                  ctx->tp.syn_x + static_cast<size_t>(t) * g.wm[0], 4 * g.wm[0],
+                                  hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(&wz[static_cast<size_t>(i) * g.wm[1]],
+                                  ctx->tp.syn_z + static_cast<size_t>(t) * g.wm[1], 4 * g.wm[1],
+                                  hipMemcpyDeviceToHost, st));
+        }
+        std::vector<int> slot_trial(trials.begin() + b0, trials.begin() + b0 + K);
+        HIPCHK(hipMemcpyAsync(pb.slot_trial, slot_trial.data(), sizeof(int) * K,
+                              hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        std::vector<uint8_t> sx(static_cast<size_t>(K) * mx), sz(static_cast<size_t>(K) * mz);
+        for (int i = 0; i < K; ++i) {
+            unpack_bits(&wx[static_cast<size_t>(i) * g.wm[0]], mx, &sx[static_cast<size_t>(i) * mx]);
+            unpack_bits(&wz[static_cast<size_t>(i) * g.wm[1]], mz, &sz[static_cast<size_t>(i) * mz]);
+        }
+        std::vector<pp::Trial> tr(K);
+        std::vector<std::vector<uint8_t>> est(K, std::vector<uint8_t>(n, 0));
+        std::vector<PPOutcome> o(K);
+        std::vector<int> active(K);
+        for (int i = 0; i < K; ++i) active[i] = i;
+        auto finish = [&](int i) {
+            o[i].bp_ok = tr[i].bp_ok;
+            o[i].ok = tr[i].ok;
+            o[i].iterations = tr[i].iterations;
+            o[i].pp_success = tr[i].pp_success;
+            o[i].pp_ets = tr[i].pp_ets;
+            o[i].pp_flip = tr[i].pp_flip;
+            o[i].used_any = tr[i].used_any;
+            o[i].used6 = tr[i].used6;
+            o[i].used12 = tr[i].used12;
+        };
+        if (max_iter <= 0) {
+            // no iteration at all: the post-decode pass on the zero estimate
+            pool.run(K, [&](int i) {
+                o[i].iterations = max_iter < 0 ? 0 : max_iter;
+                o[i].post_run = true;
+                o[i].post = pp::post_decode(hg, tabs, prm, tr[i], est[i], &sx[static_cast<size_t>(i) * mx],
+                                            &sz[static_cast<size_t>(i) * mz], nullptr, nullptr);
+                o[i].ok = o[i].post.ok;
+            });
+            active.clear();
+        }
+        std::vector<int> rec;
+        std::vector<uint8_t> out;
+        if (H > 0 && !active.empty()) {
+            HIPCHK(hipMemcpyAsync(pb.list, active.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
+            launch_step(g, ctx->tp, nt, pb.list, K, pb.slot_trial, H, true, false, damping, pb.state,
+                        pb.stride, pb.rec, pb.out, st);
+            HIPCHK(hipGetLastError());
+            rec.resize(static_cast<size_t>(K) * H * kRecInts);
+            out.resize(static_cast<size_t>(K) * 2 * n);
+            HIPCHK(hipMemcpyAsync(rec.data(), pb.rec, sizeof(int) * rec.size(), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(out.data(), pb.out, out.size(), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            pool.run(K, [&](int i) {
+                for (int t = 0; t < H; ++t) {
+                    const int *r = &rec[(static_cast<size_t>(i) * H + t) * kRecInts];
+                    pp::IterRec ir;
+                    ir.us[0] = r[0];
+                    ir.us[1] = r[1];
+                    ir.syn_all = r[0] == 0 && r[1] == 0;
+                    ir.same = r[2] & 1;
+                    ir.osc = (r[2] >> 1) & 1;
+                    if (pp::observe(hg, prm, tr[i], t, ir, nullptr, nullptr) == pp::Step::kConverged) {
+                        std::copy(&out[static_cast<size_t>(i) * 2 * n], &out[static_cast<size_t>(i) * 2 * n] + n,
+                                  est[i].begin());
+                        finish(i);
+                        break;
+                    }
+                }
+            });
+            active.erase(std::remove_if(active.begin(), active.end(), [&](int i) { return tr[i].done; }),
+                         active.end());
+        }
+        std::vector<pp::Attempt> att(K);
+        std::vector<uint8_t> need_llr(K), at_end(K);
+        std::vector<double> tot, llr;
+        for (int t = std::max(H, 0); t < max_iter && !active.empty(); ++t) {
+            const int A = static_cast<int>(active.size());
+            HIPCHK(hipMemcpyAsync(pb.list, active.data(), sizeof(int) * A, hipMemcpyHostToDevice, st));
+            launch_step(g, ctx->tp, nt, pb.list, A, pb.slot_trial, 1, t == 0, true, damping, pb.state,
+                        pb.stride, pb.rec, pb.out, st);
+            HIPCHK(hipGetLastError());
+            rec.resize(static_cast<size_t>(A) * kRecInts);
+            out.resize(static_cast<size_t>(A) * 2 * n);
+            HIPCHK(hipMemcpyAsync(rec.data(), pb.rec, sizeof(int) * rec.size(), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(out.data(), pb.out, out.size(), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            pool.run(A, [&](int a) {
+                const int i = active[a];
+                const int *r = &rec[static_cast<size_t>(a) * kRecInts];
+                const uint8_t *e_now = &out[static_cast<size_t>(a) * 2 * n];
+                const uint8_t *e_prev = e_now + n;
+                pp::IterRec ir;
+                ir.us[0] = r[0];
+                ir.us[1] = r[1];
+                ir.syn_all = r[0] == 0 && r[1] == 0;
+                ir.same = r[2] & 1;
+                ir.osc = (r[2] >> 1) & 1;
+                need_llr[i] = at_end[i] = 0;
+                att[i].want_osd = false;
+                std::copy(e_now, e_now + n, est[i].begin());
+                const pp::Step step = pp::observe(hg, prm, tr[i], t, ir, e_now, t > 0 ? e_prev : nullptr);
+                if (step == pp::Step::kConverged) {
+                    finish(i);
+                    return;
+                }
+                if (step == pp::Step::kAttempt) {
+                    pp::attempt_begin(hg, tabs, prm, tr[i], est[i], &sx[static_cast<size_t>(i) * mx],
+                                      &sz[static_cast<size_t>(i) * mz], att[i]);
+                    if (pp::attempt_end(tr[i], att[i], t, est[i])) {
+                        finish(i);
+                        return;
+                    }
+                    need_llr[i] = att[i].want_osd;
+                }
+                if (t + 1 == max_iter) at_end[i] = need_llr[i] = 1;
+            });
+            // marginals of the trials that need |LLR| (OSD now or after the decode)
+            std::vector<int> want;
+            for (int i : active)
+                if (!tr[i].done && need_llr[i]) want.push_back(i);
+            if (!want.empty()) {
+                tot.resize(static_cast<size_t>(want.size()) * 4 * n);
+                llr.resize(static_cast<size_t>(want.size()) * 2 * n);
+                const size_t tot_off = 6 * static_cast<size_t>(g.side[0].E + g.side[1].E) * sizeof(double);
+                for (size_t w = 0; w < want.size(); ++w)
+                    HIPCHK(hipMemcpyAsync(&tot[w * 4 * n], pb.state + static_cast<size_t>(want[w]) * pb.stride + tot_off,
+                                          sizeof(double) * 4 * n, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+            }
+            pool.run(static_cast<int>(want.size()), [&](int w) {
+                const int i = want[w];
+                double *lx = &llr[static_cast<size_t>(w) * 2 * n], *lz = lx + n;
+                pp::abs_llr_from_marginals(n, &tot[static_cast<size_t>(w) * 4 * n], lx, lz);
+                const uint8_t *tsx = &sx[static_cast<size_t>(i) * mx], *tsz = &sz[static_cast<size_t>(i) * mz];
+                if (att[i].want_osd) {
+                    pp::attempt_osd(hg, att[i], tsx, tsz, lx, lz);
+                    if (pp::attempt_end(tr[i], att[i], t, est[i])) {
+                        finish(i);
+                        return;
+                    }
+                }
+                if (at_end[i]) {
+                    tr[i].done = true;
+                    tr[i].iterations = max_iter;
+                    finish(i);
+                    o[i].post_run = true;
+                    o[i].post = pp::post_decode(hg, tabs, prm, tr[i], est[i], tsx, tsz, lx, lz);
+                    o[i].ok = o[i].post.ok;
+                }
+            });
+            active.erase(std::remove_if(active.begin(), active.end(), [&](int i) { return tr[i].done; }),
+                         active.end());
+        }
+        // write back: estimate planes, iterations, syndrome flag
+        std::vector<uint32_t> planes(static_cast<size_t>(K) * 2 * g.wn, 0u);
+        std::vector<int> it(K);
+        std::vector<uint8_t> okb(K);
+        for (int i = 0; i < K; ++i) {
+            uint32_t *px = &planes[static_cast<size_t>(i) * 2 * g.wn], *pz = px + g.wn;
+            for (int v = 0; v < n; ++v) {
+                px[v >> 5] |= static_cast<uint32_t>(est[i][v] & 1) << (v & 31);
+                pz[v >> 5] |= static_cast<uint32_t>((est[i][v] >> 1) & 1) << (v & 31);
+            }
+            it[i] = o[i].iterations;
+            okb[i] = o[i].ok ? 1 : 0;
+            outs[b0 + i] = o[i];
+        }
+        for (int i = 0; i < K; ++i) {
+            const size_t t = static_cast<size_t>(trials[b0 + i]);
+            const uint32_t *px = &planes[static_cast<size_t>(i) * 2 * g.wn];
+            HIPCHK(hipMemcpyAsync(ctx->tp.est_x + t * g.wn, px, 4 * g.wn, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(ctx->tp.est_z + t * g.wn, px + g.wn, 4 * g.wn, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(ctx->tp.iters + t, &it[i], sizeof(int), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(ctx->tp.syn_ok + t, &okb[i], 1, hipMemcpyHostToDevice, st));
+        }
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    return JBP_OK;
+}
+
+}  // namespace jbp
+
+extern "C" int jbp_set_pp_tables(jbp_ctx *ctx, const jbp_pp_tables *t, char *err, size_t errlen) {
+    if (!ctx) {
+        jbp::set_err(err, errlen, "bad arguments");
+        return JBP_ERR_ARG;
+    }
+    ctx->pp_tables = t ? t->t : jbp::pp::Tables();
+    return JBP_OK;
+}

@@ -12,11 +12,16 @@
 //   joint_bp_decode (PP off)                jointbp_ets.cpp:1943-3079
 //   fill_diff_bits / in_row_space           jointbp_ets.cpp:887-920, 6199-6207
 //   compute_girth / orthogonality           jointbp_ets.cpp:1018-1171
+//   read_ets_file                           jointbp_ets.cpp:3199-3258
+//   attempt_ets_fix_side_multi / attempt_flip_pp_side / attempt_osd_pp_side
+//                                           jointbp_ets.cpp:3644, 3726, 4078
+//   joint_bp_decode (PP on)                 jointbp_ets.cpp:1943-3079
 // The only code written here is the loop glue that main() has inline
-// (jointbp_ets.cpp:5931-5948, 6199-6207), restated line for line.
+// (jointbp_ets.cpp:5931-5948, 5993-6228), restated line for line.
 #include "jointbp_ets.cpp"
 
 #include <atomic>
+#include <memory>
 #include <thread>
 
 namespace {
@@ -30,6 +35,27 @@ struct RefState {
 };
 
 RefState g_ref;
+
+// Trapping-set tables loaded by the reference's reader, in load order.
+struct RefEtsGroup {
+    int size = 0;
+    std::string label;
+    std::unique_ptr<ETSIndex> x, z;
+};
+std::vector<RefEtsGroup> g_ets;
+
+std::vector<ETSIndexRef> ets_refs() {
+    std::vector<ETSIndexRef> refs;
+    for (const auto &g : g_ets) {
+        ETSIndexRef r;
+        r.size = g.size;
+        r.label = g.label;
+        r.x = g.x.get();
+        r.z = g.z.get();
+        if (r.x || r.z) refs.push_back(r);
+    }
+    return refs;
+}
 
 void finish_code(int nvars) {
     g_ref.nvars = nvars;
@@ -213,6 +239,204 @@ double ref_simulate(uint64_t base, long long first_idx, long long count, double 
         std::vector<std::thread> pool;
         for (int i = 0; i < threads; ++i) pool.emplace_back(worker);
         for (auto &th : pool) th.join();
+    }
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Loads one table group with read_ets_file (jointbp_ets.cpp:5238-5307 keeps a
+// side when its file reads and has entries).  counts: entries per side, 0
+// empty, -1 missing.
+int ref_ets_add(int size, const char *label, const char *path_x, const char *path_z, int P,
+                int *counts) {
+    RefEtsGroup grp;
+    grp.size = size;
+    grp.label = label;
+    const char *paths[2] = {path_x, path_z};
+    for (int s = 0; s < 2; ++s) {
+        counts[s] = -1;
+        if (!paths[s]) continue;
+        auto idx = std::make_unique<ETSIndex>();
+        if (!read_ets_file(paths[s], P, size, s == 0 ? 'X' : 'Z', *idx)) continue;
+        counts[s] = static_cast<int>(idx->entries.size());
+        if (idx->entries.empty()) continue;
+        (s == 0 ? grp.x : grp.z) = std::move(idx);
+    }
+    g_ets.push_back(std::move(grp));
+    return 0;
+}
+
+void ref_ets_clear() { g_ets.clear(); }
+
+// One repair with the reference's own routine: method 0 ETS (all loaded
+// groups), 1 flip set, 2 OSD.  Returns 1 when est was changed by a repair
+// that reported success (ETS: the group's size), else 0.
+int ref_pp_repair(int method, int side, const int *syndrome, const double *abs_llr,
+                  const int *vars, int n_vars, int *est_io) {
+    const bool xs = side == 0;
+    const auto &checks = xs ? g_ref.x_checks : g_ref.z_checks;
+    std::vector<int> target(syndrome, syndrome + checks.size());
+    std::vector<int> est(est_io, est_io + g_ref.nvars);
+    int applied = 0;
+    if (method == 0) {
+        ETSFixResult r = attempt_ets_fix_side_multi(xs ? "X" : "Z", ets_refs(), checks,
+                                                    xs ? g_ref.var_to_x : g_ref.var_to_z, target,
+                                                    xs, est, false);
+        applied = r.applied ? r.size : 0;
+    } else if (method == 1) {
+        std::vector<int> set(vars, vars + n_vars);
+        applied = attempt_flip_pp_side(xs ? "X" : "Z", set, checks, target, xs, est, false,
+                                       nullptr, "") ? 1 : 0;
+    } else {
+        std::vector<double> llr(abs_llr, abs_llr + g_ref.nvars);
+        applied = attempt_osd_pp_side(xs ? "X" : "Z", llr, checks, target, xs, est, false,
+                                      nullptr) ? 1 : 0;
+    }
+    std::copy(est.begin(), est.end(), est_io);
+    return applied;
+}
+
+// The simulate loop with PP on (jointbp_ets.cpp:5931-6228): the decode with
+// in-loop PP, then the post-decode ETS / flip / OSD passes and the
+// reference's counting.  Per trial: iterations, final syndrome match,
+// success, exact, flags (bit 0 BP match, 1 PP success, 2 ETS saved, 3 flip
+// saved, 4 OSD saved, 5 ETS used, 6 ETS6, 7 ETS12) and, when est_out is not
+// NULL, the final estimate.  counters[10]: pp_success, flip_pp_success,
+// osd_pp_success, stab_success, ets_used, ets_saved, ets6_used, ets6_saved,
+// ets12_used, ets12_saved.  Returns wall seconds.
+double ref_simulate_pp(uint64_t base, long long first_idx, long long count, double p, int max_iter,
+                       int flip_hist, int threads, int *iters, unsigned char *syn_ok,
+                       unsigned char *success, unsigned char *exact, unsigned char *flags,
+                       int *est_out, long long *counters) {
+    Msg prior = make_prior(p);
+    const std::vector<ETSIndexRef> refs = ets_refs();
+    PPEarlyContext pp_ctx{true, false, refs, nullptr};
+    std::atomic<long long> next{0};
+    std::vector<std::array<long long, 10>> part(std::max(threads, 1));
+    for (auto &a : part) a.fill(0);
+    auto worker = [&](int wid) {
+        auto &cnt = part[wid];
+        std::vector<int> err;
+        for (;;) {
+            long long t = next.fetch_add(1);
+            if (t >= count) break;
+            sample_trial(base, first_idx + t, p, err);
+            auto sx = compute_syndrome(g_ref.x_checks, err, true);
+            auto sz = compute_syndrome(g_ref.z_checks, err, false);
+            JointBPResult res = joint_bp_decode(
+                g_ref.x_checks, g_ref.z_checks, g_ref.var_to_x, g_ref.var_to_z, sx, sz, prior,
+                max_iter, flip_hist, false, 0.0, false, false, true, &pp_ctx, &err,
+                &g_ref.hx_basis, &g_ref.hz_basis, nullptr, nullptr);
+            bool bp_ok = res.bp_syndrome_match;
+            bool used6 = res.pp_used_ets6;
+            bool used12 = res.pp_used_ets12;
+            bool used_any = res.pp_used_ets_any || used6 || used12;
+            bool ok = res.syndrome_match;
+            bool ets_pp_ok = res.pp_success_ets;
+            bool flip_pp_ok = res.pp_success_flip;
+            bool osd_pp_ok = false;
+            if (res.pp_success && used_any) {
+                cnt[4]++;
+                if (used6) cnt[6]++;
+                if (used12) cnt[8]++;
+            }
+            if (!ok) {
+                ETSFixResult ets_x = attempt_ets_fix_side_multi("X", refs, g_ref.x_checks,
+                                                                g_ref.var_to_x, sx, true, res.est, false);
+                ETSFixResult ets_z = attempt_ets_fix_side_multi("Z", refs, g_ref.z_checks,
+                                                                g_ref.var_to_z, sz, false, res.est, false);
+                bool applied = ets_x.applied || ets_z.applied;
+                used_any = used_any || applied;
+                used6 = used6 || ets_x.size == 6 || ets_z.size == 6;
+                used12 = used12 || ets_x.size == 12 || ets_z.size == 12;
+                if (applied) cnt[4]++;
+                if (used6) cnt[6]++;
+                if (used12) cnt[8]++;
+                if (applied) {
+                    ok = compute_syndrome(g_ref.x_checks, res.est, true) == sx &&
+                         compute_syndrome(g_ref.z_checks, res.est, false) == sz;
+                    if (ok) ets_pp_ok = true;
+                }
+            }
+            if (!ok) {
+                bool fx = false, fz = false;
+                for (const auto &set : build_flip_sets(res.flip_x_history, res.flip_x_window, res.flip_x)) {
+                    if (!set.vars || set.vars->empty()) continue;
+                    if (attempt_flip_pp_side("X", *set.vars, g_ref.x_checks, sx, true, res.est, false, nullptr, set.tag)) {
+                        fx = true;
+                        break;
+                    }
+                }
+                for (const auto &set : build_flip_sets(res.flip_z_history, res.flip_z_window, res.flip_z)) {
+                    if (!set.vars || set.vars->empty()) continue;
+                    if (attempt_flip_pp_side("Z", *set.vars, g_ref.z_checks, sz, false, res.est, false, nullptr, set.tag)) {
+                        fz = true;
+                        break;
+                    }
+                }
+                if (fx || fz) {
+                    ok = compute_syndrome(g_ref.x_checks, res.est, true) == sx &&
+                         compute_syndrome(g_ref.z_checks, res.est, false) == sz;
+                    if (ok) flip_pp_ok = true;
+                }
+            }
+            if (!ok) {
+                auto shx = compute_syndrome(g_ref.x_checks, res.est, true);
+                auto shz = compute_syndrome(g_ref.z_checks, res.est, false);
+                int us = std::max(unsatisfied_count(shx, sx), unsatisfied_count(shz, sz));
+                if (us <= 20) {
+                    bool ox = attempt_osd_pp_side("X", res.abs_llr_x, g_ref.x_checks, sx, true, res.est, false, &err);
+                    bool oz = attempt_osd_pp_side("Z", res.abs_llr_z, g_ref.z_checks, sz, false, res.est, false, &err);
+                    if (ox || oz) {
+                        ok = compute_syndrome(g_ref.x_checks, res.est, true) == sx &&
+                             compute_syndrome(g_ref.z_checks, res.est, false) == sz;
+                        if (ok) osd_pp_ok = true;
+                    }
+                }
+            }
+            bool ex = res.est == err;
+            bool stab = ok && stab_equiv(err, res.est);
+            bool succ = ok && stab;
+            unsigned char f = bp_ok ? 1 : 0;
+            if (succ && !ex && stab) cnt[3]++;
+            if (succ && ets_pp_ok) {
+                cnt[5]++;
+                if (used6) cnt[7]++;
+                if (used12) cnt[9]++;
+                f |= 4;
+            }
+            if (succ && flip_pp_ok) {
+                cnt[1]++;
+                f |= 8;
+            }
+            bool osd_success = osd_pp_ok || (res.pp_success && !res.pp_success_ets && !res.pp_success_flip);
+            if (succ && (res.pp_success || ets_pp_ok || flip_pp_ok || osd_pp_ok)) {
+                cnt[0]++;
+                f |= 2;
+            }
+            if (succ && osd_success) {
+                cnt[2]++;
+                f |= 16;
+            }
+            f |= (used_any ? 32 : 0) | (used6 ? 64 : 0) | (used12 ? 128 : 0);
+            iters[t] = res.iterations;
+            syn_ok[t] = ok ? 1 : 0;
+            success[t] = succ ? 1 : 0;
+            exact[t] = ex ? 1 : 0;
+            flags[t] = f;
+            if (est_out) std::copy(res.est.begin(), res.est.end(), est_out + t * g_ref.nvars);
+        }
+    };
+    auto t0 = std::chrono::steady_clock::now();
+    if (threads <= 1) {
+        worker(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (int i = 0; i < threads; ++i) pool.emplace_back(worker, i);
+        for (auto &th : pool) th.join();
+    }
+    for (int k = 0; k < 10; ++k) {
+        counters[k] = 0;
+        for (const auto &a : part) counters[k] += a[k];
     }
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }

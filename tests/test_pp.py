@@ -1,0 +1,120 @@
+"""Post-processing on the device path: jbp_simulate with pp=1 against the
+reference's simulate loop with PP on (jointbp_ets.cpp:5931-6228, decoder
+1943-3079), per trial.  The fixtures come from the reference's own code
+(tests/golden/make_pp_golden.py)."""
+import hashlib
+
+import numpy as np
+import pytest
+
+from conftest import PARAMS, golden_npz
+
+jbp = pytest.importorskip("joint_bp_plus_pp_amd")
+
+WORKLOADS = ["p0.04_it8", "p0.03_it5", "p0.04_it8_hist0", "p0.02_it1", "p0.05_it115"]
+COUNTERS = ["pp_success", "flip_pp_success", "osd_pp_success", "stab_success", "ets_used",
+            "ets_saved", "ets6_used", "ets6_saved", "ets12_used", "ets12_saved"]
+
+
+def digest(est):
+    return hashlib.sha1(np.asarray(est, dtype=np.uint8).tobytes()).hexdigest().encode()
+
+
+@pytest.fixture(scope="module")
+def pp_decoder(main_decoder):
+    tables = jbp.PPTables.from_dir(main_decoder.code, jbp.default_ets_dir(PARAMS))
+    main_decoder.set_pp_tables(tables)
+    yield main_decoder
+    main_decoder.set_pp_tables(None)
+
+
+def run(dec, g, precision, **kw):
+    return dec.simulate(int(g["seed"]), len(g["iterations"]), float(g["p"]),
+                        max_iter=int(g["max_iter"]), first_trial=int(g["first_trial"]),
+                        precision=precision, pp=True, flip_hist=int(g["flip_hist"]), want_est=True,
+                        **kw)
+
+
+def check_equal(stats, per, g):
+    n_bad = int(np.count_nonzero(per["iterations"] != g["iterations"]))
+    assert n_bad == 0, "iterations differ on %d trials" % n_bad
+    np.testing.assert_array_equal(per["syn_match"], g["syn_match"])
+    np.testing.assert_array_equal(per["success"], g["success"])
+    np.testing.assert_array_equal(per["exact"], g["exact"])
+    np.testing.assert_array_equal(per["pp_flags"], g["flags"])
+    got = np.array([digest(e) for e in per["est"]])
+    np.testing.assert_array_equal(got, g["est_sha1"])
+    assert [int(stats[k]) for k in COUNTERS] == g["counters"].tolist()
+    assert int(stats["failures"]) == int(np.count_nonzero(g["success"] == 0))
+    assert int(stats["bp_failures"]) == int(np.count_nonzero((g["flags"] & 1) == 0))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", WORKLOADS)
+def test_pp_fp64_equals_reference_per_trial(pp_decoder, name):
+    g = golden_npz("pp_trials_%s.npz" % name)
+    print("pp fp64", name, flush=True)
+    stats, per = run(pp_decoder, g, jbp.FP64)
+    check_equal(stats, per, g)
+
+
+@pytest.mark.gpu
+def test_pp_fp64_hgp_equals_reference_per_trial():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    g = golden_npz("pp_trials_hgp15x20.npz")
+    m, n, cw, rw, seed = (int(v) for v in g["hgp"])
+    code = jbp.Code.hgp(m, n, cw, rw, seed)
+    with jbp.Decoder(code, 0) as dec:
+        print("pp fp64 hgp", m, n, flush=True)
+        stats, per = run(dec, g, jbp.FP64)
+        check_equal(stats, per, g)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["p0.04_it8", "p0.03_it5"])
+def test_pp_fp32_first_pass(pp_decoder, name):
+    """FP32 decodes the first min(99, max_iter - 1) iterations; the trials it
+    leaves unconverged go through the same FP64 stage.  Outcomes agree with
+    the reference except where FP32 and FP64 rounding end a decode at a
+    different iteration."""
+    g = golden_npz("pp_trials_%s.npz" % name)
+    print("pp fp32", name, flush=True)
+    stats, per = run(pp_decoder, g, jbp.FP32)
+    agree = np.mean((per["success"] == g["success"]) & (per["pp_flags"] == g["flags"]))
+    assert agree >= 0.97, agree
+    assert stats["pp_trials"] > 0
+    want = g["counters"][0]
+    assert abs(int(stats["pp_success"]) - int(want)) <= max(3, 0.05 * want)
+
+
+@pytest.mark.gpu
+def test_pp_on_leaves_early_convergers_alone(pp_decoder):
+    """A trial that converges before iteration 100 never reaches a PP
+    trigger: its outcome with pp on is the outcome with pp off."""
+    kw = dict(max_iter=300, precision=jbp.FP32, want_est=True)
+    print("pp on/off p0.045", flush=True)
+    s_off, off = pp_decoder.simulate(31, 4096, 0.045, **kw)
+    s_on, on = pp_decoder.simulate(31, 4096, 0.045, pp=True, **kw)
+    early = off["iterations"] < 100
+    assert early.mean() > 0.99
+    for k in ("iterations", "syn_match", "success", "exact"):
+        np.testing.assert_array_equal(on[k][early], off[k][early])
+    np.testing.assert_array_equal(on["est"][early], off["est"][early])
+    assert s_on["pp_trials"] == int(np.count_nonzero(~early))
+
+
+@pytest.mark.gpu
+def test_pp_min_fails_and_argument_errors(pp_decoder):
+    g = golden_npz("pp_trials_p0.04_it8.npz")
+    fails = np.cumsum(g["success"] == 0)
+    limit = 5
+    stop = int(np.argmax(fails >= limit)) + 1
+    stats, per = pp_decoder.simulate(int(g["seed"]), len(g["iterations"]), float(g["p"]),
+                                     max_iter=int(g["max_iter"]), precision=jbp.FP64, pp=True,
+                                     min_fails=limit)
+    assert int(stats["done"]) == stop
+    np.testing.assert_array_equal(per["success"], g["success"][:stop])
+    with pytest.raises(jbp.JbpError):
+        pp_decoder.simulate(1, 8, 0.04, max_iter=8, pp=True, flip_hist=-1)
