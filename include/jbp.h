@@ -226,6 +226,17 @@ int jbp_pp_repair(const jbp_graph_desc *graph, const jbp_pp_tables *t, int metho
                   const int *syndrome, const double *abs_llr, const int *vars, int n_vars,
                   int *est, int *applied, char *err, size_t errlen);
 
+/* jbp_decode_batch with post-processing (the single-decode modes of the
+ * reference with PP on, jointbp_ets.cpp:5487-5710): same inputs plus
+ * flip_hist; pp_out (may be NULL) receives per decode the JBP_PPF_* bits,
+ * where the "saved" bits mean that the repair produced the syndrome match
+ * (whether that is a success depends on the caller's residual test). */
+int jbp_decode_batch_pp(jbp_ctx *ctx, int count, const uint32_t *sx_bits,
+                        const uint32_t *sz_bits, const double prior[4], int max_iter,
+                        double damping, int precision, int flip_hist, int *iterations_out,
+                        uint8_t *syn_match_out, uint8_t *pp_out, int *est_out, char *err,
+                        size_t errlen);
+
 /* Sample only: errors of trials first..first+count-1 (count x nvars states)
  * and their syndromes (count x mX, count x mZ ints; either may be NULL). */
 int jbp_sample(jbp_ctx *ctx, uint64_t base_seed, int64_t first_trial, int count, double p,

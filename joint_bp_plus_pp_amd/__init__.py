@@ -136,6 +136,7 @@ def lib():
         "jbp_pp_tables_add": (i32, [vp, C.POINTER(GraphDesc), i64, i32, cp, cp, cp, vp, cp, sz]),
         "jbp_set_pp_tables": (i32, [vp, vp, cp, sz]),
         "jbp_pp_repair": (i32, [C.POINTER(GraphDesc), vp, i32, i32, vp, vp, vp, i32, vp, vp, cp, sz]),
+        "jbp_decode_batch_pp": (i32, [vp, i32, vp, vp, vp, i32, dbl, i32, i32, vp, vp, vp, vp, cp, sz]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -417,8 +418,12 @@ class Decoder:
                                 precision, C.byref(res), _ptr(est), e, 512), e)
         return est, res.iterations, bool(res.syndrome_match), res
 
-    def decode_batch(self, sx, sz, prior, max_iter=50, damping=0.0, precision=FP32, want_est=True):
-        """Batched decode of int syndrome arrays (count x m)."""
+    def decode_batch(self, sx, sz, prior, max_iter=50, damping=0.0, precision=FP32, want_est=True,
+                     pp=False, flip_hist=5):
+        """Batched decode of int syndrome arrays (count x m).  pp=True adds
+        the reference's post-processing (jbp_decode_batch_pp) and returns a
+        fourth array of PPF_* flags ("saved" = the repair produced the
+        syndrome match)."""
         sx = np.atleast_2d(np.asarray(sx))
         sz = np.atleast_2d(np.asarray(sz))
         count = sx.shape[0]
@@ -428,6 +433,12 @@ class Decoder:
         ok = np.zeros(count, np.uint8)
         est = np.zeros((count, self.code.n), np.int32) if want_est else None
         e = _errbuf()
+        if pp:
+            flags = np.zeros(count, np.uint8)
+            _check(lib().jbp_decode_batch_pp(self._h, count, _ptr(bx), _ptr(bz), _ptr(pr), max_iter,
+                                             damping, precision, flip_hist, _ptr(it), _ptr(ok),
+                                             _ptr(flags), _ptr(est), e, 512), e)
+            return it, ok, est, flags
         _check(lib().jbp_decode_batch(self._h, count, _ptr(bx), _ptr(bz), _ptr(pr), max_iter,
                                       damping, precision, _ptr(it), _ptr(ok), _ptr(est), e, 512), e)
         return it, ok, est

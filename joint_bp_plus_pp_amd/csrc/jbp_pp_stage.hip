@@ -502,3 +502,75 @@ extern "C" int jbp_set_pp_tables(jbp_ctx *ctx, const jbp_pp_tables *t, char *err
     ctx->pp_tables = t ? t->t : jbp::pp::Tables();
     return JBP_OK;
 }
+
+extern "C" int jbp_decode_batch_pp(jbp_ctx *ctx, int count, const uint32_t *sx_bits,
+                                   const uint32_t *sz_bits, const double prior[4], int max_iter,
+                                   double damping, int precision, int flip_hist,
+                                   int *iterations_out, uint8_t *syn_match_out, uint8_t *pp_out,
+                                   int *est_out, char *err, size_t errlen) {
+    using namespace jbp;
+    if (!ctx || count < 0 || !sx_bits || !sz_bits || !prior || max_iter < 0 || flip_hist < 0) {
+        set_err(err, errlen, "bad arguments");
+        return JBP_ERR_ARG;
+    }
+    if (count == 0) return JBP_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = ensure_chunk(ctx, count, err, errlen);
+    if (rc) return rc;
+    const DevGraph &g = ctx->dg;
+    HIPCHK(hipMemcpyAsync(ctx->tp.syn_x, sx_bits, static_cast<size_t>(count) * g.wm[0] * 4,
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->tp.syn_z, sz_bits, static_cast<size_t>(count) * g.wm[1] * 4,
+                          hipMemcpyHostToDevice, ctx->stream));
+    {
+        const Prior pri = make_prior(prior);
+        if ((rc = set_noise(ctx, &pri, nullptr, 1, err, errlen))) return rc;
+    }
+    const int h = std::min(99, max_iter - 1);
+    std::vector<uint8_t> ok(count, 0);
+    if (h > 0) {
+        rc = run_decode(ctx, count, ctx->noise, h, 0, 1, damping, precision, err, errlen);
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(ok.data(), ctx->tp.syn_ok, count, hipMemcpyDeviceToHost));
+    }
+    std::vector<int> list;
+    for (int i = 0; i < count; ++i)
+        if (h <= 0 || !ok[i]) list.push_back(i);
+    std::vector<PPOutcome> outs;
+    rc = run_pp(ctx, ctx->noise, max_iter, damping, flip_hist, list, outs, err, errlen);
+    if (rc) return rc;
+    if (pp_out) {
+        for (int i = 0; i < count; ++i) pp_out[i] = JBP_PPF_BP_OK;
+        for (size_t k = 0; k < list.size(); ++k) {
+            const PPOutcome &o = outs[k];
+            const bool ets = o.pp_ets || (o.post_run && o.post.ets_ok);
+            const bool flip = o.pp_flip || (o.post_run && o.post.flip_ok);
+            const bool osd = (o.post_run && o.post.osd_ok) || (o.pp_success && !o.pp_ets && !o.pp_flip);
+            const bool any = o.pp_success || ets || flip || osd;
+            const bool u6 = o.used6 || (o.post_run && o.post.used6);
+            const bool u12 = o.used12 || (o.post_run && o.post.used12);
+            const bool used = o.used_any || u6 || u12 || (o.post_run && o.post.ets_applied);
+            pp_out[list[k]] = static_cast<uint8_t>(
+                (o.bp_ok ? JBP_PPF_BP_OK : 0) | (any && o.ok ? JBP_PPF_PP_SUCCESS : 0) |
+                (ets && o.ok ? JBP_PPF_ETS_SAVED : 0) | (flip && o.ok ? JBP_PPF_FLIP_SAVED : 0) |
+                (osd && o.ok ? JBP_PPF_OSD_SAVED : 0) | (used ? JBP_PPF_ETS_USED : 0) |
+                (u6 ? JBP_PPF_ETS6 : 0) | (u12 ? JBP_PPF_ETS12 : 0));
+        }
+    }
+    if (iterations_out)
+        HIPCHK(hipMemcpy(iterations_out, ctx->tp.iters, static_cast<size_t>(count) * 4,
+                         hipMemcpyDeviceToHost));
+    if (syn_match_out)
+        HIPCHK(hipMemcpy(syn_match_out, ctx->tp.syn_ok, static_cast<size_t>(count),
+                         hipMemcpyDeviceToHost));
+    if (est_out) {
+        if ((rc = ensure_states(ctx, static_cast<size_t>(count) * g.n, err, errlen))) return rc;
+        launch_unpack_states(ctx->tp.est_x, ctx->tp.est_z, g.wn, g.n, count, ctx->d_states,
+                             ctx->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(est_out, ctx->d_states, static_cast<size_t>(count) * g.n * 4,
+                              hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    return JBP_OK;
+}

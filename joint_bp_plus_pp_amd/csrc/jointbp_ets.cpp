@@ -3,12 +3,17 @@
 // Keeps the reference program's flags, input formats and result lines
 // (jointbp_ets.cpp:4733-4999 usage and parsing, 5003-5211 code summary,
 // 5426-5860 single decodes, 5862-6449 --simulate loop and Run Summary) for
-// its --no-pp path, and runs every decode through libjbp (include/jbp.h):
-// trials are sampled, decoded and judged on the device in batches.
+// post-processing on (its default, ETS / flip / OSD) or off (--no-pp), and
+// runs every decode through libjbp (include/jbp.h): trials are sampled,
+// decoded and judged on the device in batches; with post-processing the
+// trials still unconverged after 99 iterations finish on the device one
+// iteration per launch, with the repairs on host threads.
 //
 // Differences, by design:
-//   * post-processing (ETS / flip / OSD) is not part of this build: without
-//     --no-pp the program stops with a message;
+//   * trapping-set tables are looked up where the reference looks (the
+//     params file's stem as a directory under the working directory, then
+//     the bare file name) and, failing both, in that directory next to the
+//     params file;
 //   * there is no CPU decoder: if the device cannot be used the program
 //     reports the error and exits (the reference falls back to its CPU
 //     decoder, jointbp_ets.cpp:5421-5423);
@@ -20,7 +25,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
+#include <sys/stat.h>
 #include <iomanip>
 #include <iostream>
 #include <sstream>
@@ -140,20 +147,50 @@ void usage(const char *prog) {
               << "  --p N (0.05)  --max-iter N (50)  --damping R (0.0, FP64 only)\n"
               << "  --precision 32|64     FP32 messages (default) or the FP64 decoder that\n"
               << "                        reproduces the reference CPU decoder trial for trial\n"
-              << "  --no-pp               required: post-processing is not part of this build\n"
+              << "  --no-pp               disable post-processing (ETS / flip / OSD)\n"
+              << "  --flip-hist N (5)     flip history window of post-processing\n"
+              << "  --ets6-x/-z FILE  --ets12-x/-z FILE (--ets-x/-z)  --ets8-path4 FILE\n"
               << "Randomness / trials:\n"
               << "  --seed N  --trial-index N  --trials N (1)  --min-fails N  --report-every N (100)\n"
               << "Output:\n"
               << "  --report-fail  --est FILE  --iter-hist-out FILE\n"
               << "Device:\n"
               << "  --cuda  --cuda-device N (0)  --cuda-graph  --cuda-check-warmup N  --cuda-check-interval N\n"
-              << "Accepted for compatibility and ignored: --flip-hist, --progress-tsv, --progress-every,\n"
-              << "  --save-fail-prefix, --ets*-x/-z, --ets8-path4\n";
+              << "Accepted for compatibility and ignored: --progress-tsv, --progress-every,\n"
+              << "  --save-fail-prefix\n";
 }
 
 struct Progress {
     long long done = 0, failures = 0, bp_fail = 0, stab = 0, iters = 0;
+    long long pp = 0, flip = 0, ets_used = 0, ets_saved = 0, ets6_used = 0, ets6_saved = 0;
+    long long ets12_used = 0, ets12_saved = 0;
 };
+
+bool file_exists(const std::string &path) {
+    struct stat st;
+    return !path.empty() && stat(path.c_str(), &st) == 0;
+}
+
+std::string strip_dir(const std::string &path) {
+    size_t slash = path.find_last_of("/\\");
+    return slash == std::string::npos ? path : path.substr(slash + 1);
+}
+
+std::string dir_of(const std::string &path) {
+    size_t slash = path.find_last_of("/\\");
+    return slash == std::string::npos ? std::string(".") : path.substr(0, slash);
+}
+
+// resolve_existing_path (jointbp_ets.cpp:4439-4446), then next to the
+// params file.
+std::string resolve_table(const std::string &path, const std::string &params_dir) {
+    if (file_exists(path)) return path;
+    const std::string base = strip_dir(path);
+    if (base != path && file_exists(base)) return base;
+    if (!path.empty() && path[0] != '/' && file_exists(params_dir + "/" + path))
+        return params_dir + "/" + path;
+    return "";
+}
 
 void report_progress(const Progress &p, double elapsed, long long k, double latency) {
     double fer = static_cast<double>(p.failures) / static_cast<double>(p.done);
@@ -165,7 +202,7 @@ void report_progress(const Progress &p, double elapsed, long long k, double late
               {"fps", fixed(fps, 2)}, {"qbps", fmt_qbps(fps * static_cast<double>(k))},
               {"avg_iter", fixed(static_cast<double>(p.iters) / p.done, 2)},
               {"iters", std::to_string(p.iters)}, {"stab_success", std::to_string(p.stab)},
-              {"pp_success", "0"}});
+              {"pp_success", std::to_string(p.pp)}});
 }
 
 }  // namespace
@@ -178,6 +215,11 @@ int main(int argc, char **argv) {
     std::string params_path, sx_path, sz_path, err_path, est_path, hist_path;
     double p_err = 0.05, damping = 0.0;
     int max_iter = 50, cuda_device = 0, warmup = 0, interval = 1, precision = JBP_FP32;
+    int flip_hist = 5;
+    std::string ets6_x = "ets_6_2.txt", ets6_z = "ets_6_2.txt", ets12_x = "ets_12_2.txt",
+                ets12_z = "ets_12_2.txt", ets8_path4 = "ets_8_2_path4.txt";
+    bool ets6_x_set = false, ets6_z_set = false, ets12_x_set = false, ets12_z_set = false,
+         ets8_path4_set = false;
     bool simulate = false, verbose = false, verbose_all = false, report_fail = false;
     bool report_ets = false, freeze_syn = false, enable_pp = true, use_cuda = false, use_graph = false;
     unsigned long long seed = 0;
@@ -199,16 +241,19 @@ int main(int argc, char **argv) {
             else if (a == "--est") { need(1); est_path = argv[++i]; }
             else if (a == "--p") { need(1); p_err = std::stod(argv[++i]); }
             else if (a == "--max-iter") { need(1); max_iter = std::stoi(argv[++i]); }
-            else if (a == "--flip-hist") { need(1); ++i; }
+            else if (a == "--flip-hist") { need(1); flip_hist = std::max(0, std::stoi(argv[++i])); }
+            else if (a == "--ets6-x") { need(1); ets6_x = argv[++i]; ets6_x_set = true; }
+            else if (a == "--ets6-z") { need(1); ets6_z = argv[++i]; ets6_z_set = true; }
+            else if (a == "--ets12-x" || a == "--ets-x") { need(1); ets12_x = argv[++i]; ets12_x_set = true; }
+            else if (a == "--ets12-z" || a == "--ets-z") { need(1); ets12_z = argv[++i]; ets12_z_set = true; }
+            else if (a == "--ets8-path4") { need(1); ets8_path4 = argv[++i]; ets8_path4_set = true; }
             else if (a == "--freeze-syn") { freeze_syn = true; }
             else if (a == "--damping") { need(1); damping = std::stod(argv[++i]); }
             else if (a == "--simulate") { simulate = true; }
             else if (a == "--verbose-all") { verbose = verbose_all = true; }
             else if (a == "--verbose") { need(1); verbose = std::stoi(argv[++i]) != 0; }
             else if (a == "--seed") { need(1); seed = std::stoull(argv[++i]); seed_set = true; }
-            else if (a == "--ets6-x" || a == "--ets6-z" || a == "--ets12-x" || a == "--ets12-z" ||
-                     a == "--ets-x" || a == "--ets-z" || a == "--ets8-path4" ||
-                     a == "--save-fail-prefix" || a == "--progress-tsv" || a == "--progress-every") {
+            else if (a == "--save-fail-prefix" || a == "--progress-tsv" || a == "--progress-every") {
                 need(1);
                 ++i;
             }
@@ -299,10 +344,64 @@ int main(int argc, char **argv) {
     kv_lines({{"hx_hz_orthogonal", tf(d[10] == 0)}, {"hx_hz_odd_pairs", std::to_string(d[10])},
               {"girth_x", girth(d[11])}, {"girth_z", girth(d[12])}});
 
-    if (enable_pp) {
-        std::cerr << "This build runs the --no-pp path only (post-processing is not built); "
-                     "pass --no-pp.\n";
-        return 1;
+    // trapping-set tables (jointbp_ets.cpp:5031-5053 paths, 5211-5400 load
+    // and the ETS Load table)
+    jbp_graph_desc gd;
+    jbp_code_graph(code, &gd);
+    jbp_pp_tables *tables = nullptr;
+    jbp_pp_tables_create(&tables);
+    {
+        std::string stem = strip_dir(params_path);
+        const size_t dot = stem.find_last_of('.');
+        if (dot != std::string::npos) stem = stem.substr(0, dot);
+        const std::string pdir = dir_of(params_path);
+        if (!ets6_x_set) ets6_x = stem + "/ets_6_2.txt";
+        if (!ets6_z_set) ets6_z = stem + "/ets_6_2.txt";
+        if (!ets12_x_set) ets12_x = stem + "/ets_12_2.txt";
+        if (!ets12_z_set) ets12_z = stem + "/ets_12_2.txt";
+        if (!ets8_path4_set) ets8_path4 = stem + "/ets_8_2_path4.txt";
+        struct Row {
+            std::string label;
+            int cnt[2];
+        };
+        std::vector<Row> rows;
+        auto add = [&](int size, const std::string &label, const std::string &px, const std::string &pz) {
+            const std::string rx = resolve_table(px, pdir), rz = resolve_table(pz, pdir);
+            Row r{label, {-1, -1}};
+            if (jbp_pp_tables_add(tables, &gd, dims[5], size, label.c_str(), rx.empty() ? nullptr : rx.c_str(),
+                                  rz.empty() ? nullptr : rz.c_str(), r.cnt, err, sizeof(err)) != JBP_OK)
+                r.cnt[0] = r.cnt[1] = -1;
+            rows.push_back(r);
+        };
+        for (int size : {6, 8, 10, 12, 14, 16, 18, 20, 22}) {
+            const std::string gen = stem + "/ets_" + std::to_string(size) + "_2.txt";
+            if (size == 6) add(6, "6", ets6_x, ets6_z);
+            else if (size == 12) add(12, "12", ets12_x, ets12_z);
+            else add(size, std::to_string(size), gen, gen);
+            if (size == 8) add(8, "8p4", ets8_path4, ets8_path4);
+        }
+        section("ETS Load");
+        auto val = [](int c) { return c < 0 ? std::string("-") : std::to_string(c); };
+        auto status = [](const Row &r) {
+            if (r.cnt[0] > 0 || r.cnt[1] > 0) return std::string("ok");
+            if (r.cnt[0] == 0 || r.cnt[1] == 0) return std::string("empty");
+            return std::string("missing");
+        };
+        size_t lw = 0, xw = 1, zw = 1, sw = 0;
+        for (const Row &r : rows) {
+            lw = std::max(lw, r.label.size() + 4);
+            xw = std::max(xw, val(r.cnt[0]).size());
+            zw = std::max(zw, val(r.cnt[1]).size());
+            sw = std::max(sw, status(r).size());
+        }
+        for (const Row &r : rows) {
+            std::ostringstream o;
+            o << std::left << std::setw(static_cast<int>(lw)) << ("ETS" + r.label + ":") << " X="
+              << std::right << std::setw(static_cast<int>(xw)) << val(r.cnt[0]) << " Z="
+              << std::setw(static_cast<int>(zw)) << val(r.cnt[1]) << " status=" << std::left
+              << std::setw(static_cast<int>(sw)) << status(r);
+            std::cout << o.str() << "\n";
+        }
     }
     if (verbose || verbose_all || report_ets) {
         std::cerr << "Per-iteration diagnostics (--verbose, --verbose-all, --report-ets) are not "
@@ -361,13 +460,22 @@ int main(int argc, char **argv) {
         }
     }
 
-    jbp_graph_desc gd;
-    jbp_code_graph(code, &gd);
+    if (enable_pp && freeze_syn) {
+        std::cerr << "--freeze-syn with post-processing runs on the reference's CPU decoder only; "
+                     "add --no-pp.\n";
+        return 1;
+    }
     jbp_ctx *ctx = nullptr;
     if (jbp_create(&gd, cuda_device, &ctx, err, sizeof(err)) != JBP_OK) {
         std::cerr << "CUDA init failed: " << err << " (no CPU decoder in this build)\n";
         return 1;
     }
+    if (enable_pp && jbp_set_pp_tables(ctx, tables, err, sizeof(err)) != JBP_OK) {
+        std::cerr << "post-processing tables: " << err << "\n";
+        jbp_destroy(ctx);
+        return 1;
+    }
+    jbp_pp_tables_free(tables);
     // prior = normalize({1-p, p/3, p/3, p/3})  (jointbp_ets.cpp:5484-5485)
     double prior[4] = {1.0 - p_err, p_err / 3.0, p_err / 3.0, p_err / 3.0};
     {
@@ -388,9 +496,24 @@ int main(int argc, char **argv) {
         }
         std::vector<int> est(n, 0);
         jbp_decode_result res;
+        uint8_t ppf = JBP_PPF_BP_OK;
         auto t0 = std::chrono::steady_clock::now();
-        if (jbp_decode(ctx, sx.data(), sz.data(), prior, max_iter, warmup, interval, 0, use_graph ? 1 : 0,
-                       freeze_syn ? 1 : 0, damping, precision, &res, est.data(), err, sizeof(err)) != JBP_OK) {
+        if (enable_pp) {
+            std::vector<uint32_t> bx((mx + 31) / 32 + 1, 0u), bz((mz + 31) / 32 + 1, 0u);
+            for (int c = 0; c < mx; ++c) bx[c >> 5] |= static_cast<uint32_t>(sx[c] != 0) << (c & 31);
+            for (int c = 0; c < mz; ++c) bz[c >> 5] |= static_cast<uint32_t>(sz[c] != 0) << (c & 31);
+            uint8_t match = 0;
+            std::memset(&res, 0, sizeof(res));
+            if (jbp_decode_batch_pp(ctx, 1, bx.data(), bz.data(), prior, max_iter, damping, precision,
+                                    flip_hist, &res.iterations, &match, &ppf, est.data(), err,
+                                    sizeof(err)) != JBP_OK) {
+                std::cerr << "CUDA decode failed: " << err << " (no CPU decoder in this build)\n";
+                jbp_destroy(ctx);
+                return 1;
+            }
+            res.syndrome_match = match;
+        } else if (jbp_decode(ctx, sx.data(), sz.data(), prior, max_iter, warmup, interval, 0, use_graph ? 1 : 0,
+                              freeze_syn ? 1 : 0, damping, precision, &res, est.data(), err, sizeof(err)) != JBP_OK) {
             std::cerr << "CUDA decode failed: " << err << " (no CPU decoder in this build)\n";
             jbp_destroy(ctx);
             return 1;
@@ -416,10 +539,14 @@ int main(int argc, char **argv) {
                 success = ok && stab;
             }
         }
+        // jointbp_ets.cpp:5740-5745, 5794-5801
+        const bool pp_success = success && (ppf & JBP_PPF_PP_SUCCESS);
         section("Decode Result");
         std::cout << "iterations=" << res.iterations << " syndrome_match=" << tf(ok) << "\n"
-                  << "decode_success=" << tf(success) << "\npp_enabled=false\npp_success=false\n"
-                  << "pp_success_ets=false\npp_success_flip=false\navg_latency=" << fmt_latency(latency) << "\n";
+                  << "decode_success=" << tf(success) << "\npp_enabled=" << tf(enable_pp) << "\n"
+                  << "pp_success=" << tf(pp_success) << "\npp_success_ets="
+                  << tf(success && (ppf & JBP_PPF_ETS_SAVED)) << "\npp_success_flip="
+                  << tf(success && (ppf & JBP_PPF_FLIP_SAVED)) << "\navg_latency=" << fmt_latency(latency) << "\n";
         if (!errv.empty())
             std::cout << "exact_match=" << tf(exact) << "\nstabilizer_equiv=" << tf(stab) << "\n";
         if (!est_path.empty()) {
@@ -458,7 +585,7 @@ int main(int argc, char **argv) {
     long long exact_total = 0;
     auto start = std::chrono::steady_clock::now();
     std::vector<int> it;
-    std::vector<uint8_t> okv, succ, exv;
+    std::vector<uint8_t> okv, succ, exv, ppv;
     while (pr.done < trials) {
         if (min_fails > 0 && pr.failures >= min_fails) break;
         const long long cnt = std::min(block, trials - pr.done);
@@ -466,9 +593,12 @@ int main(int argc, char **argv) {
         okv.assign(cnt, 0);
         succ.assign(cnt, 0);
         exv.assign(cnt, 0);
-        jbp_trial_out out = {it.data(), okv.data(), succ.data(), exv.data(), nullptr, nullptr};
+        ppv.assign(cnt, 0);
+        jbp_trial_out out = {it.data(), okv.data(), succ.data(), exv.data(), nullptr, nullptr, ppv.data()};
         jbp_sim_cfg cfg = {seed, first_idx + pr.done, cnt, p_err, max_iter, damping, precision,
                            warmup, interval, 0, 0, min_fails > 0 ? min_fails - pr.failures : 0};
+        cfg.pp = enable_pp ? 1 : 0;
+        cfg.flip_hist = flip_hist;
         jbp_run_stats st;
         if (jbp_simulate(ctx, &cfg, &out, &st, err, sizeof(err)) != JBP_OK) {
             std::cerr << "CUDA decode failed: " << err << " (no CPU decoder in this build)\n";
@@ -479,16 +609,31 @@ int main(int argc, char **argv) {
         for (long long i = 0; i < st.done; ++i) {
             const long long idx = first_idx + pr.done;
             const bool ok = okv[i] != 0, s = succ[i] != 0, ex = exv[i] != 0;
+            const uint8_t f = ppv[i];
+            const bool bp_ok = (f & JBP_PPF_BP_OK) != 0;
             pr.done++;
             pr.iters += it[i];
             pr.failures += s ? 0 : 1;
-            pr.bp_fail += ok ? 0 : 1;
+            pr.bp_fail += bp_ok ? 0 : 1;
             pr.stab += (s && !ex) ? 1 : 0;
+            // the Run Summary counters (jointbp_ets.cpp:6113-6228) from the
+            // per-trial flags
+            pr.pp += (f & JBP_PPF_PP_SUCCESS) ? 1 : 0;
+            pr.flip += (f & JBP_PPF_FLIP_SAVED) ? 1 : 0;
+            pr.ets_used += (f & JBP_PPF_ETS_USED) ? 1 : 0;
+            pr.ets6_used += (f & JBP_PPF_ETS6) ? 1 : 0;
+            pr.ets12_used += (f & JBP_PPF_ETS12) ? 1 : 0;
+            if (f & JBP_PPF_ETS_SAVED) {
+                pr.ets_saved++;
+                pr.ets6_saved += (f & JBP_PPF_ETS6) ? 1 : 0;
+                pr.ets12_saved += (f & JBP_PPF_ETS12) ? 1 : 0;
+            }
             exact_total += ex ? 1 : 0;
             if (it[i] >= 1 && it[i] <= max_iter) hist[it[i]]++;
             if (report_fail && !s) {
-                std::cout << "[fail] trial=" << idx << " iter=" << it[i] << " bp_ok=" << tf(ok)
-                          << " syndrome_match=" << tf(ok) << " ets6=false ets12=false trial_seed="
+                std::cout << "[fail] trial=" << idx << " iter=" << it[i] << " bp_ok=" << tf(bp_ok)
+                          << " syndrome_match=" << tf(ok) << " ets6=" << tf(f & JBP_PPF_ETS6)
+                          << " ets12=" << tf(f & JBP_PPF_ETS12) << " trial_seed="
                           << splitmix64(seed ^ (static_cast<uint64_t>(idx) * 0x9e3779b97f4a7c15ull)) << "\n";
             }
             if (report_every > 0 && pr.done % report_every == 0)
@@ -502,9 +647,10 @@ int main(int argc, char **argv) {
     bool has_ci = wilson(pr.done, pr.failures, 1.96, lo, hi);
     section("Run Summary");
     std::cout << "trials=" << pr.done << " failures=" << pr.failures << " bp_fail=" << pr.bp_fail
-              << " pp_success=0 flip_pp_success=0 stab_success=" << pr.stab
-              << " ets_used=0 ets_saved=0 ets6_used=0 ets6_saved=0 ets12_used=0 ets12_saved=0"
-              << " fer=" << fixed(fer, 8);
+              << " pp_success=" << pr.pp << " flip_pp_success=" << pr.flip << " stab_success=" << pr.stab
+              << " ets_used=" << pr.ets_used << " ets_saved=" << pr.ets_saved << " ets6_used=" << pr.ets6_used
+              << " ets6_saved=" << pr.ets6_saved << " ets12_used=" << pr.ets12_used
+              << " ets12_saved=" << pr.ets12_saved << " fer=" << fixed(fer, 8);
     if (has_ci) std::cout << " ci95=[" << fixed(lo, 8) << "," << fixed(hi, 8) << "]";
     std::cout << " iters=" << pr.iters << " avg_iter=" << fixed(pr.done ? static_cast<double>(pr.iters) / pr.done : 0.0, 2)
               << " avg_latency=" << fmt_latency(pr.done ? elapsed / pr.done : 0.0)

@@ -17,6 +17,8 @@ import ctypes as C
 import hashlib
 import json
 import os
+import subprocess
+import tempfile
 
 import numpy as np
 
@@ -27,6 +29,7 @@ STEM = "H_P768_J3_L12_dmax3_nc0-3_1-2_seed11579811919164041"
 PARAMS = os.path.join(REF_DIR, STEM + ".txt")
 ETS_DIR = os.path.join(REF_DIR, STEM)
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "libref_harness.so")
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "jointbp_ets")
 P = 768
 # (size, label, file) in the reference's load order (jointbp_ets.cpp:5309-5327)
 GROUPS = [(6, "6", "ets_6_2.txt"), (8, "8", "ets_8_2.txt"), (8, "8p4", "ets_8_2_path4.txt"),
@@ -249,6 +252,44 @@ def main():
                         p=p, max_iter=max_iter, seed=seed, flip_hist=5, first_trial=1, iterations=it,
                         syn_match=ok, success=su, exact=ex, flags=fl, est_sha1=digests, counters=ctr)
     print("hgp15x20", "%.1fs" % sec, "counters", ctr.tolist())
+
+    # the reference program itself, PP on: its ETS Load table, a Run Summary
+    # and a single --err decode that PP repairs.  It runs in a scratch
+    # directory holding links to the params file and its table directory
+    # (it looks for the tables under the working directory).
+    g = np.load(os.path.join(HERE, "pp_trials_p0.04_it8.npz"))
+    saved = np.nonzero((g["flags"] & 2) != 0)[0]
+    trial = int(saved[0]) + 1
+    h.ref_load_params(PARAMS.encode())
+    err = np.zeros(n, np.int32)
+    h.ref_sample(7, trial, 0.04, err.ctypes.data)
+    with tempfile.TemporaryDirectory() as tmp:
+        os.symlink(PARAMS, os.path.join(tmp, STEM + ".txt"))
+        os.symlink(ETS_DIR, os.path.join(tmp, STEM))
+        np.savetxt(os.path.join(tmp, "err.txt"), err, fmt="%d")
+        sim = subprocess.run([REF_BIN, "--params", STEM + ".txt", "--simulate", "--p", "0.04",
+                              "--trials", "256", "--max-iter", "8", "--seed", "7", "--report-every",
+                              "0"], cwd=tmp, capture_output=True, text=True, timeout=600)
+        one = subprocess.run([REF_BIN, "--params", STEM + ".txt", "--err", "err.txt", "--p", "0.04",
+                              "--max-iter", "8", "--est", "est.txt"], cwd=tmp, capture_output=True,
+                             text=True, timeout=600)
+        est = np.loadtxt(os.path.join(tmp, "est.txt"), dtype=np.int64)
+    lines = sim.stdout.splitlines()
+    start = lines.index(next(ln for ln in lines if "ETS Load" in ln))
+    ets_lines = [ln for ln in lines[start + 2:] if ln.startswith("ETS")]
+    summary = [ln for ln in lines if ln.startswith("trials=")][-1]
+    one_lines = one.stdout.splitlines()
+    k = one_lines.index(next(ln for ln in one_lines if "Decode Result" in ln))
+    result = [ln for ln in one_lines[k + 2:] if "=" in ln and not ln.startswith("avg_latency")]
+    with open(os.path.join(HERE, "pp_cli.json"), "w") as f:
+        json.dump({"ets_load": ets_lines, "simulate_args": ["--simulate", "--p", "0.04", "--trials", "256",
+                                                           "--max-iter", "8", "--seed", "7"],
+                   "run_summary": summary, "err_trial": trial, "err_seed": 7,
+                   "err_args": ["--p", "0.04", "--max-iter", "8"], "err_exit": one.returncode,
+                   "decode_result": result,
+                   "est_sha1": hashlib.sha1(est.astype(np.uint8).tobytes()).hexdigest()}, f, indent=1)
+    print("cli:", summary)
+    print("err trial", trial, result)
 
 
 if __name__ == "__main__":

@@ -25,11 +25,28 @@ def summary_fields(text):
 
 
 def test_code_summary_lines_match_reference():
-    out = run(["--simulate", "--trials", "1", "--seed", "1"])  # stops before decoding (PP on)
+    out = run(["--simulate", "--trials", "1", "--seed", "1", "--max-iter", "3"])
     ref = golden_json("aggregates.json")["cli_simulate"]["lines"]
     for line in ref[:4]:
         assert line in out.stdout, line
-    assert out.returncode == 1 and "--no-pp" in out.stderr
+
+
+def test_ets_load_table_matches_reference(tmp_path):
+    """The trapping-set tables are found and reported as the reference
+    reports them (jointbp_ets.cpp:5336-5387), also when the program runs
+    elsewhere (the tables next to the params file), and as missing when
+    there are none."""
+    want = golden_json("pp_cli.json")["ets_load"]
+    for cwd in (ROOT, str(tmp_path)):
+        out = run(["--simulate", "--trials", "1", "--max-iter", "3", "--no-pp"], cwd=cwd)
+        lines = out.stdout.splitlines()
+        assert [ln for ln in lines if ln.startswith("ETS")] == want
+    lonely = tmp_path / "lonely.txt"
+    lonely.write_text(open(PARAMS).read())
+    out = subprocess.run([CLI, "--params", str(lonely), "--simulate", "--trials", "1", "--no-pp"],
+                         capture_output=True, text=True, timeout=300)
+    rows = [ln for ln in out.stdout.splitlines() if ln.startswith("ETS")]
+    assert len(rows) == 10 and all(r.endswith("X=- Z=- status=missing") for r in rows)
 
 
 def test_argument_errors():
@@ -103,3 +120,32 @@ def test_syndrome_file_decode_matches_reference(tmp_path, graph):
     out = run(args + (["--cuda-graph"] if graph else []))
     assert out.returncode == 0, out.stderr
     assert fnv1a(np.loadtxt(es, dtype=np.int64)) == ref["est_hash"]
+
+
+@pytest.mark.gpu
+def test_simulate_with_pp_matches_reference_run_summary():
+    ref = golden_json("pp_cli.json")
+    want = summary_fields(ref["run_summary"])
+    out = run(ref["simulate_args"] + ["--report-every", "0", "--precision", "64"])
+    assert out.returncode == 0, out.stderr
+    got = summary_fields(out.stdout)
+    for key in ("trials", "failures", "bp_fail", "pp_success", "flip_pp_success", "stab_success",
+                "ets_used", "ets_saved", "ets6_used", "ets6_saved", "ets12_used", "ets12_saved",
+                "fer", "ci95", "iters", "avg_iter", "exact_rate"):
+        assert got[key] == want[key], key
+
+
+@pytest.mark.gpu
+def test_err_decode_with_pp_matches_reference(tmp_path):
+    import hashlib
+    from _oracle import Oracle
+    ref = golden_json("pp_cli.json")
+    err = Oracle().code_from_params(PARAMS).sample(ref["err_seed"], ref["err_trial"], 0.04)
+    ep, es = tmp_path / "err.txt", tmp_path / "est.txt"
+    np.savetxt(ep, err, fmt="%d")
+    out = run(["--err", str(ep)] + ref["err_args"] + ["--precision", "64", "--est", str(es)])
+    assert out.returncode == ref["err_exit"], out.stderr
+    for line in ref["decode_result"]:
+        assert line in out.stdout.splitlines(), line
+    est = np.loadtxt(es, dtype=np.int64)
+    assert hashlib.sha1(est.astype(np.uint8).tobytes()).hexdigest() == ref["est_sha1"]

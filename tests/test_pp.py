@@ -118,3 +118,22 @@ def test_pp_min_fails_and_argument_errors(pp_decoder):
     np.testing.assert_array_equal(per["success"], g["success"][:stop])
     with pytest.raises(jbp.JbpError):
         pp_decoder.simulate(1, 8, 0.04, max_iter=8, pp=True, flip_hist=-1)
+
+
+@pytest.mark.gpu
+def test_decode_batch_pp_equals_simulate(pp_decoder):
+    """The same syndromes through jbp_decode_batch_pp give the simulate
+    loop's per-trial outcome (iterations, match, estimate, repair flags)."""
+    g = golden_npz("pp_trials_p0.04_it8.npz")
+    count = 64
+    err, sx, sz = pp_decoder.sample(int(g["seed"]), 1, count, float(g["p"]))
+    print("decode_batch_pp", count, flush=True)
+    it, ok, est, flags = pp_decoder.decode_batch(sx, sz, jbp.prior_from_p(float(g["p"])),
+                                                 max_iter=int(g["max_iter"]), precision=jbp.FP64,
+                                                 pp=True)
+    np.testing.assert_array_equal(it, g["iterations"][:count])
+    np.testing.assert_array_equal(ok, g["syn_match"][:count])
+    np.testing.assert_array_equal(np.array([digest(e) for e in est]), g["est_sha1"][:count])
+    # where the decode succeeded the flags are the reference's
+    succ = g["success"][:count] == 1
+    np.testing.assert_array_equal(flags[succ], g["flags"][:count][succ])
