@@ -51,6 +51,9 @@ struct PPBuf {
     int *slot_trial = nullptr;  // chunk index of each slot's trial
     int *rec = nullptr;         // [slot][iteration][4]
     uint8_t *out = nullptr;     // [slot][2][n] estimate, previous estimate
+    int *idx = nullptr;         // slots whose marginals are gathered
+    void *stage = nullptr;      // gather / write-back staging
+    size_t stage_bytes = 0;
 };
 
 // What the post-processing stage decided for one trial.

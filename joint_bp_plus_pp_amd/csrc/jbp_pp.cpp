@@ -440,6 +440,19 @@ bool flip_fix(const Graph &g, int side, const std::vector<int> &vars_in, const u
     const int nv = static_cast<int>(vars.size());
     // more columns than rows cannot all be independent
     if (nv > m) return false;
+    // an unsatisfied check that no variable of the set touches makes the
+    // system unsolvable (the reference's empty row with a nonzero right-hand
+    // side, jointbp_ets.cpp:3822-3831); found without any elimination
+    {
+        std::vector<uint8_t> hit(m, 0);
+        for (int v : vars) {
+            int cnt = 0;
+            const int *rows = col_rows(g, side, v, cnt);
+            for (int k = 0; k < cnt; ++k) hit[rows[k]] = 1;
+        }
+        for (int c = 0; c < m; ++c)
+            if (diff[c] && !hit[c]) return false;
+    }
     SpanBasis basis(m, nv);
     basis.set_target(diff.data());
     for (int j = 0; j < nv; ++j) {
@@ -468,14 +481,22 @@ bool osd_fix(const Graph &g, int side, const double *abs_llr, const uint8_t *tar
         any |= diff[c] != 0;
     }
     if (!any) return false;
+    // |LLR| order with ties by index (the reference's stable sort); only
+    // the prefix the walk reaches is sorted, in growing chunks
     std::vector<int> order(n);
     for (int v = 0; v < n; ++v) order[v] = v;
-    std::sort(order.begin(), order.end(), [&](int a, int b) {
+    const auto before = [&](int a, int b) {
         return abs_llr[a] < abs_llr[b] || (abs_llr[a] == abs_llr[b] && a < b);
-    });
+    };
+    int sorted = 0;
     SpanBasis basis(m, std::min(n, m + 1));
     basis.set_target(diff.data());
     for (int j = 0; j < n; ++j) {
+        if (j == sorted) {
+            const int next = std::min(n, std::max(2 * sorted, sorted + 256));
+            std::partial_sort(order.begin() + sorted, order.begin() + next, order.end(), before);
+            sorted = next;
+        }
         int cnt = 0;
         const int *rows = col_rows(g, side, order[j], cnt);
         if (!basis.add(rows, cnt, j)) return false;

@@ -178,6 +178,44 @@ void launch_step(const DevGraph &g, const TrialPlanes &tp, const NoiseTab &nt, c
     }
 }
 
+// dst[k][0..len) = src[idx[k] * stride + off + 0..len), in 8-byte units
+// (the chunk syndromes of listed trials, the marginals of listed slots).
+__global__ void gather_kernel(const uint64_t *src, size_t stride, size_t off, int len,
+                              const int *idx, int count, uint64_t *dst) {
+    const int k = blockIdx.y;
+    if (k >= count) return;
+    const uint64_t *s = src + static_cast<size_t>(idx[k]) * stride + off;
+    uint64_t *d = dst + static_cast<size_t>(k) * len;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < len; i += gridDim.x * blockDim.x) d[i] = s[i];
+}
+
+// Writes the stage's outcome of listed trial k back into the chunk planes:
+// est words (X then Z, wn each), iterations and syndrome flag.
+__global__ void scatter_outcome_kernel(const uint32_t *planes, const int *iters, const uint8_t *ok,
+                                       const int *idx, int count, int wn, TrialPlanes tp) {
+    const int k = blockIdx.y;
+    if (k >= count) return;
+    const size_t t = static_cast<size_t>(idx[k]);
+    const uint32_t *px = planes + static_cast<size_t>(k) * 2 * wn;
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < wn; w += gridDim.x * blockDim.x) {
+        tp.est_x[t * wn + w] = px[w];
+        tp.est_z[t * wn + w] = px[wn + w];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        tp.iters[t] = iters[k];
+        tp.syn_ok[t] = ok[k];
+    }
+}
+
+void launch_gather(const void *src, size_t stride, size_t off, int len, const int *idx, int count,
+                   void *dst, hipStream_t st) {
+    if (count <= 0 || len <= 0) return;
+    const int bx = std::min(64, (len + 255) / 256);
+    hipLaunchKernelGGL(gather_kernel, dim3(bx, count), dim3(256), 0, st,
+                       static_cast<const uint64_t *>(src), stride, off, len, idx, count,
+                       static_cast<uint64_t *>(dst));
+}
+
 // Host worker pool: run(count, fn) calls fn(0..count-1) across the workers
 // and the calling thread and returns when all calls are done.
 class Pool {
@@ -267,6 +305,14 @@ int ensure_pp_buffers(jbp_ctx *ctx, int slots, int rec_per_slot, char *err, size
     HIPCHK(hipMalloc(&pb.slot_trial, sizeof(int) * slots));
     HIPCHK(hipMalloc(&pb.rec, sizeof(int) * kRecInts * static_cast<size_t>(slots) * pb.rec_cap));
     HIPCHK(hipMalloc(&pb.out, 2 * static_cast<size_t>(g.n) * slots));
+    HIPCHK(hipMalloc(&pb.idx, sizeof(int) * slots));
+    // staging: syndromes (as 8-byte units), marginals of every slot, or the
+    // write-back planes + iterations + flags, whichever is largest
+    const size_t syn = 8 * static_cast<size_t>(slots) * ((g.wm[0] + 1) / 2 + (g.wm[1] + 1) / 2);
+    const size_t tot = 32 * static_cast<size_t>(slots) * g.n;
+    const size_t back = static_cast<size_t>(slots) * (8 * static_cast<size_t>(g.wn) + 8);
+    pb.stage_bytes = std::max(syn, std::max(tot, back));
+    HIPCHK(hipMalloc(&pb.stage, pb.stage_bytes));
     return JBP_OK;
 }
 
@@ -280,7 +326,7 @@ void free_pp(jbp_ctx *ctx) {
     PPBuf &pb = ctx->ppb;
     for (void *p : {static_cast<void *>(pb.state), static_cast<void *>(pb.list),
                     static_cast<void *>(pb.slot_trial), static_cast<void *>(pb.rec),
-                    static_cast<void *>(pb.out)})
+                    static_cast<void *>(pb.out), static_cast<void *>(pb.idx), pb.stage})
         if (p) (void)hipFree(p);
     pb = PPBuf();
 }
@@ -305,25 +351,30 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
         int rc = ensure_pp_buffers(ctx, K, std::max(H, 1), err, errlen);
         if (rc) return rc;
         PPBuf &pb = ctx->ppb;
-        // syndromes as bytes
-        std::vector<uint32_t> wx(static_cast<size_t>(K) * g.wm[0]), wz(static_cast<size_t>(K) * g.wm[1]);
-        for (int i = 0; i < K; ++i) {
-            const int t = trials[b0 + i];
-            HIPCHK(hipMemcpyAsync(&wx[static_cast<size_t>(i) * g.wm[0]],
-                                  ctx->tp.syn_x + static_cast<size_t>(t) * g.wm[0], 4 * g.wm[0],
-                                  hipMemcpyDeviceToHost, st));
-            HIPCHK(hipMemcpyAsync(&wz[static_cast<size_t>(i) * g.wm[1]],
-                                  ctx->tp.syn_z + static_cast<size_t>(t) * g.wm[1], 4 * g.wm[1],
-                                  hipMemcpyDeviceToHost, st));
-        }
+        // syndromes of the batch, gathered on the device and unpacked to bytes
         std::vector<int> slot_trial(trials.begin() + b0, trials.begin() + b0 + K);
         HIPCHK(hipMemcpyAsync(pb.slot_trial, slot_trial.data(), sizeof(int) * K,
                               hipMemcpyHostToDevice, st));
-        HIPCHK(hipStreamSynchronize(st));
         std::vector<uint8_t> sx(static_cast<size_t>(K) * mx), sz(static_cast<size_t>(K) * mz);
-        for (int i = 0; i < K; ++i) {
-            unpack_bits(&wx[static_cast<size_t>(i) * g.wm[0]], mx, &sx[static_cast<size_t>(i) * mx]);
-            unpack_bits(&wz[static_cast<size_t>(i) * g.wm[1]], mz, &sz[static_cast<size_t>(i) * mz]);
+        for (int s = 0; s < 2; ++s) {
+            const int wm = g.wm[s];
+            std::vector<uint32_t> w(static_cast<size_t>(K) * wm);
+            const uint32_t *src = s == 0 ? ctx->tp.syn_x : ctx->tp.syn_z;
+            if (wm % 2 == 0) {
+                launch_gather(src, wm / 2, 0, wm / 2, pb.slot_trial, K, pb.stage, st);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(w.data(), pb.stage, 4 * w.size(), hipMemcpyDeviceToHost, st));
+            } else {
+                for (int i = 0; i < K; ++i)
+                    HIPCHK(hipMemcpyAsync(&w[static_cast<size_t>(i) * wm],
+                                          src + static_cast<size_t>(slot_trial[i]) * wm, 4 * wm,
+                                          hipMemcpyDeviceToHost, st));
+            }
+            HIPCHK(hipStreamSynchronize(st));
+            const int m = s == 0 ? mx : mz;
+            uint8_t *dst = s == 0 ? sx.data() : sz.data();
+            for (int i = 0; i < K; ++i)
+                unpack_bits(&w[static_cast<size_t>(i) * wm], m, dst + static_cast<size_t>(i) * m);
         }
         std::vector<pp::Trial> tr(K);
         std::vector<std::vector<uint8_t>> est(K, std::vector<uint8_t>(n, 0));
@@ -340,6 +391,26 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             o[i].used_any = tr[i].used_any;
             o[i].used6 = tr[i].used6;
             o[i].used12 = tr[i].used12;
+        };
+        std::vector<pp::Attempt> att(K);
+        std::vector<uint8_t> need_llr(K), at_end(K), attempted(K);
+        // Trial i ends unconverged at max_iter after an in-loop attempt at that
+        // same iteration failed.  The reference's post-decode pass then
+        // repeats exactly that attempt (ETS, the same flip sets, OSD with the
+        // same |LLR|) on the decoder's estimate, in place: it fails the same
+        // way and leaves the estimate the attempt built.  Take that result
+        // instead of computing it twice.
+        auto end_trial = [&](int i) {
+            tr[i].done = true;
+            tr[i].iterations = max_iter;
+            finish(i);
+            o[i].post_run = true;
+            o[i].post = pp::PostOut();
+            o[i].post.ets_applied = att[i].ets_used;
+            o[i].post.used6 = att[i].ets6;
+            o[i].post.used12 = att[i].ets12;
+            o[i].ok = false;
+            est[i].swap(att[i].trial);
         };
         if (max_iter <= 0) {
             // no iteration at all: the post-decode pass on the zero estimate
@@ -384,8 +455,6 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             active.erase(std::remove_if(active.begin(), active.end(), [&](int i) { return tr[i].done; }),
                          active.end());
         }
-        std::vector<pp::Attempt> att(K);
-        std::vector<uint8_t> need_llr(K), at_end(K);
         std::vector<double> tot, llr;
         for (int t = std::max(H, 0); t < max_iter && !active.empty(); ++t) {
             const int A = static_cast<int>(active.size());
@@ -409,7 +478,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                 ir.syn_all = r[0] == 0 && r[1] == 0;
                 ir.same = r[2] & 1;
                 ir.osc = (r[2] >> 1) & 1;
-                need_llr[i] = at_end[i] = 0;
+                need_llr[i] = at_end[i] = attempted[i] = 0;
                 att[i].want_osd = false;
                 std::copy(e_now, e_now + n, est[i].begin());
                 const pp::Step step = pp::observe(hg, prm, tr[i], t, ir, e_now, t > 0 ? e_prev : nullptr);
@@ -418,6 +487,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                     return;
                 }
                 if (step == pp::Step::kAttempt) {
+                    attempted[i] = 1;
                     pp::attempt_begin(hg, tabs, prm, tr[i], est[i], &sx[static_cast<size_t>(i) * mx],
                                       &sz[static_cast<size_t>(i) * mz], att[i]);
                     if (pp::attempt_end(tr[i], att[i], t, est[i])) {
@@ -426,19 +496,28 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                     }
                     need_llr[i] = att[i].want_osd;
                 }
-                if (t + 1 == max_iter) at_end[i] = need_llr[i] = 1;
+                if (t + 1 == max_iter) {
+                    at_end[i] = 1;
+                    // the post-decode pass needs |LLR| only when it has to run
+                    // itself (see below)
+                    need_llr[i] |= !attempted[i];
+                }
+                if (at_end[i] && !need_llr[i]) end_trial(i);
             });
             // marginals of the trials that need |LLR| (OSD now or after the decode)
             std::vector<int> want;
             for (int i : active)
                 if (!tr[i].done && need_llr[i]) want.push_back(i);
             if (!want.empty()) {
-                tot.resize(static_cast<size_t>(want.size()) * 4 * n);
-                llr.resize(static_cast<size_t>(want.size()) * 2 * n);
-                const size_t tot_off = 6 * static_cast<size_t>(g.side[0].E + g.side[1].E) * sizeof(double);
-                for (size_t w = 0; w < want.size(); ++w)
-                    HIPCHK(hipMemcpyAsync(&tot[w * 4 * n], pb.state + static_cast<size_t>(want[w]) * pb.stride + tot_off,
-                                          sizeof(double) * 4 * n, hipMemcpyDeviceToHost, st));
+                const int W = static_cast<int>(want.size());
+                tot.resize(static_cast<size_t>(W) * 4 * n);
+                llr.resize(static_cast<size_t>(W) * 2 * n);
+                const size_t tot_off = 6 * static_cast<size_t>(g.side[0].E + g.side[1].E);
+                HIPCHK(hipMemcpyAsync(pb.idx, want.data(), sizeof(int) * W, hipMemcpyHostToDevice, st));
+                launch_gather(pb.state, pb.stride / 8, tot_off, 4 * n, pb.idx, W, pb.stage, st);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(tot.data(), pb.stage, sizeof(double) * tot.size(),
+                                      hipMemcpyDeviceToHost, st));
                 HIPCHK(hipStreamSynchronize(st));
             }
             pool.run(static_cast<int>(want.size()), [&](int w) {
@@ -454,12 +533,16 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                     }
                 }
                 if (at_end[i]) {
-                    tr[i].done = true;
-                    tr[i].iterations = max_iter;
-                    finish(i);
-                    o[i].post_run = true;
-                    o[i].post = pp::post_decode(hg, tabs, prm, tr[i], est[i], tsx, tsz, lx, lz);
-                    o[i].ok = o[i].post.ok;
+                    if (attempted[i]) {
+                        end_trial(i);
+                    } else {
+                        tr[i].done = true;
+                        tr[i].iterations = max_iter;
+                        finish(i);
+                        o[i].post_run = true;
+                        o[i].post = pp::post_decode(hg, tabs, prm, tr[i], est[i], tsx, tsz, lx, lz);
+                        o[i].ok = o[i].post.ok;
+                    }
                 }
             });
             active.erase(std::remove_if(active.begin(), active.end(), [&](int i) { return tr[i].done; }),
@@ -479,15 +562,20 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             okb[i] = o[i].ok ? 1 : 0;
             outs[b0 + i] = o[i];
         }
-        for (int i = 0; i < K; ++i) {
-            const size_t t = static_cast<size_t>(trials[b0 + i]);
-            const uint32_t *px = &planes[static_cast<size_t>(i) * 2 * g.wn];
-            HIPCHK(hipMemcpyAsync(ctx->tp.est_x + t * g.wn, px, 4 * g.wn, hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(ctx->tp.est_z + t * g.wn, px + g.wn, 4 * g.wn, hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(ctx->tp.iters + t, &it[i], sizeof(int), hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(ctx->tp.syn_ok + t, &okb[i], 1, hipMemcpyHostToDevice, st));
+        {
+            char *stage = static_cast<char *>(pb.stage);
+            uint32_t *d_planes = reinterpret_cast<uint32_t *>(stage);
+            int *d_it = reinterpret_cast<int *>(stage + 4 * planes.size());
+            uint8_t *d_ok = reinterpret_cast<uint8_t *>(d_it + K);
+            HIPCHK(hipMemcpyAsync(d_planes, planes.data(), 4 * planes.size(), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(d_it, it.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(d_ok, okb.data(), K, hipMemcpyHostToDevice, st));
+            const int bx = std::min(16, (g.wn + 255) / 256);
+            hipLaunchKernelGGL(scatter_outcome_kernel, dim3(bx, K), dim3(256), 0, st, d_planes, d_it,
+                               d_ok, pb.slot_trial, K, g.wn, ctx->tp);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(st));
         }
-        HIPCHK(hipStreamSynchronize(st));
     }
     return JBP_OK;
 }
