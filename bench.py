@@ -101,6 +101,61 @@ def cpu_baseline(args, first_trial):
                                        args.max_iter, secs, iters / count, count - sum(su))}
 
 
+def pp_workload(jbp, dec, args, first):
+    """Post-processing on: the reference's default mode (ETS / flip / OSD),
+    on a workload where most trials need it (p=0.04, max-iter 8: BP alone
+    fails on about 70% of the trials).  The reference's own PP loop (built
+    from its source, oracle/_ref/) runs a bounded sample beside it."""
+    n, max_iter, p = args.pp_trials, args.pp_max_iter, args.p
+    dec.set_pp_tables(jbp.PPTables.from_dir(dec.code, jbp.default_ets_dir(PARAMS)))
+    dec.simulate(args.seed, 1024, p, max_iter=max_iter, first_trial=first, pp=True, per_trial=False)
+    done = fails = pp_trials = pp_ok = bp_fail = 0
+    pp_ms = 0.0
+    t0 = time.perf_counter()
+    for k in range(args.pp_calls):
+        st, _ = dec.simulate(args.seed, n, p, max_iter=max_iter, first_trial=first + k * n, pp=True,
+                             per_trial=False)
+        done += st["done"]
+        fails += st["failures"]
+        bp_fail += st["bp_failures"]
+        pp_trials += st["pp_trials"]
+        pp_ok += st["pp_success"]
+        pp_ms += st["pp_ms"]
+    el = time.perf_counter() - t0
+    dec.set_pp_tables(None)
+    out = {"workload": "pp on, p=%g, max-iter %d, %d trials per call, FP32 first pass + FP64 "
+                       "stage" % (p, max_iter, n),
+           "trials_per_s": round(done / el, 2), "trials": done, "bp_failures": bp_fail,
+           "pp_trials": pp_trials, "pp_success": pp_ok, "fer": fails / done,
+           "stage_share": round(pp_ms / 1e3 / el, 3)}
+    harness = os.path.join(ROOT, "oracle", "_ref", "libref_harness.so")
+    if os.path.exists(harness) and args.pp_cpu_trials > 0:
+        L = C.CDLL(harness)
+        L.ref_load_params.argtypes = [C.c_char_p]
+        L.ref_ets_clear()
+        L.ref_ets_add.argtypes = [C.c_int, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_void_p]
+        L.ref_simulate_pp.restype = C.c_double
+        L.ref_simulate_pp.argtypes = [C.c_uint64, C.c_longlong, C.c_longlong, C.c_double, C.c_int,
+                                      C.c_int, C.c_int] + [C.c_void_p] * 7
+        if L.ref_load_params(PARAMS.encode()) != 0:
+            raise RuntimeError("reference could not read the params file")
+        counts = (C.c_int * 2)()
+        for size, label, name in jbp.ETS_GROUPS:
+            path = os.path.join(jbp.default_ets_dir(PARAMS), name).encode()
+            L.ref_ets_add(size, label.encode(), path, path, dec.code.P, counts)
+        m = args.pp_cpu_trials
+        bufs = [(C.c_int * m)()] + [(C.c_uint8 * m)() for _ in range(4)]
+        ctr = (C.c_longlong * 10)()
+        threads = cpu_threads()
+        secs = L.ref_simulate_pp(args.seed, first, m, p, max_iter, 5, threads, *bufs, None, ctr)
+        L.ref_ets_clear()
+        out["cpu_reference"] = {"trials_per_s": round(m / secs, 2), "cores": threads,
+                                "sample": "%d trials (indices %d..%d), %.1f s wall" %
+                                          (m, first, first + m - 1, secs)}
+        out["speedup_vs_cpu_reference"] = round(out["trials_per_s"] / (m / secs), 1)
+    return out
+
+
 def learned_vs_plain(jbp, dec, args, first, plain):
     """Each shipped learned model (T learned iterations, then plain BP
     restarted from the prior) on the first timed trial indices, beside plain
@@ -280,6 +335,12 @@ def main():
     ap.add_argument("--hgp-trials", type=int, default=2048)
     ap.add_argument("--hgp-p", type=float, default=0.03)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--pp-trials", type=int, default=8192,
+                    help="trials per call of the post-processing workload (0: skip it)")
+    ap.add_argument("--pp-calls", type=int, default=3)
+    ap.add_argument("--pp-max-iter", type=int, default=8)
+    ap.add_argument("--pp-cpu-trials", type=int, default=128,
+                    help="trials of the reference's PP loop timed beside it (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a plain copy "
                          "kernel of known size, to scale profiler byte counters)")
@@ -367,6 +428,10 @@ def main():
     if not args.no_mixed and args.precision == 32:
         mixed = mixed_workload(jbp, dec, args)
         print("bench: mixed %s" % json.dumps(mixed), file=sys.stderr)
+    pp = None
+    if args.pp_trials > 0 and args.precision == 32:
+        pp = pp_workload(jbp, dec, args, nxt)
+        print("bench: pp %s" % json.dumps(pp), file=sys.stderr)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args, first_timed)
@@ -399,6 +464,7 @@ def main():
             "pipelined": pipelined,
             "learned": learned,
             "mixed": mixed,
+            "pp": pp,
         },
         "roofline": {
             "bound": "hbm",
