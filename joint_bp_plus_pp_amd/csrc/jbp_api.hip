@@ -642,7 +642,13 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
         int rc0 = set_noise(ctx, pri.data(), plist, np, err, errlen);
         if (rc0) return rc0;
     }
-    if (cfg->block > 0) ctx->f64_block = cfg->block;
+    if (cfg->block > 0) {
+        if (cfg->block > 512 || cfg->block % 64) {
+            set_err(err, errlen, "block must be a multiple of 64 up to 512 (got %d)", cfg->block);
+            return JBP_ERR_ARG;
+        }
+        ctx->f64_block = cfg->block;
+    }
     if (cfg->slots > 0 && cfg->precision == JBP_FP64 && cfg->slots != ctx->f64_slots) {
         if (ctx->f64_scratch) (void)hipFree(ctx->f64_scratch);
         ctx->f64_scratch = nullptr;

@@ -54,6 +54,12 @@ struct PPBuf {
     int *idx = nullptr;         // slots whose marginals are gathered
     void *stage = nullptr;      // gather / write-back staging
     size_t stage_bytes = 0;
+    // pinned host copies (fast transfers): records, estimates, marginals,
+    // staging
+    int *h_rec = nullptr;
+    uint8_t *h_out = nullptr;
+    double *h_tot = nullptr;
+    void *h_stage = nullptr;
 };
 
 // What the post-processing stage decided for one trial.

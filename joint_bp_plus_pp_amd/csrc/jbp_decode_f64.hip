@@ -41,8 +41,11 @@ __device__ __forceinline__ SlotPtrs slot_ptrs(const DevGraph &g, double *scratch
     return s;
 }
 
+// Blocks of up to 512 threads (the tuner's largest), at least two resident
+// per compute unit: the register budget this leaves (up to 128 VGPRs) keeps
+// the check recursion's arrays in registers.
 template <int MAXC, int MAXV>
-__global__ void decode_f64_kernel(DevGraph g, TrialPlanes tp, int count, NoiseTab nt, int max_iter,
+__global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPlanes tp, int count, NoiseTab nt, int max_iter,
                                   double damping, double *scratch, size_t stride, int *counter) {
     __shared__ int s_trial;
     const int tid = threadIdx.x;

@@ -202,20 +202,11 @@ inline const int *col_rows(const Graph &g, int side, int v, int &count) {
 }
 
 // One table of ETS repair (jointbp_ets.cpp:3525-3642).
+// (diff = syndrome of est XOR target, whose two set checks form `key`)
 bool ets_one(const Graph &g, const EtsSide &es, int side, const uint8_t *target,
-             std::vector<uint8_t> &est) {
+             const std::vector<uint8_t> &diff, uint64_t key, std::vector<uint8_t> &est) {
     const int nv = es.nv;
     if (es.count() == 0 || nv <= 0 || nv > 32) return false;
-    const int m = g.m[side];
-    std::vector<uint8_t> diff = syndrome_of(g, side, est);
-    int unsat[3], nu = 0;
-    for (int c = 0; c < m; ++c) {
-        diff[c] ^= target[c];
-        if (diff[c] && nu < 3) unsat[nu++] = c;
-    }
-    if (nu != 2) return false;
-    const uint64_t key = (static_cast<uint64_t>(static_cast<uint32_t>(unsat[0])) << 32) |
-                         static_cast<uint32_t>(unsat[1]);
     auto it = es.by_pair.find(key);
     if (it == es.by_pair.end()) return false;
     std::vector<int> touched;
@@ -305,11 +296,23 @@ bool flip_pass(const Graph &g, const Params &prm, const Trial &tr, std::vector<u
                const uint8_t *sx, const uint8_t *sz) {
     bool any = false;
     for (int side = 0; side < 2; ++side) {
+        const uint8_t *target = side == 0 ? sx : sz;
         std::vector<int> uni;
         if (tr.hist_on && prm.hist_window > 0) uni = window_union(tr.win[side], g.n);
+        // est only changes when a set applies, which ends this side's loop
+        std::vector<uint8_t> diff;
+        const std::vector<int> *tried = nullptr;
         for (const std::vector<int> *set : flip_sets(uni, tr.win[side], tr.last[side])) {
             if (set->empty()) continue;
-            if (flip_fix(g, side, *set, side == 0 ? sx : sz, est)) {
+            // the same set again (the union of a one-entry history is that
+            // entry) fails again
+            if (tried && *tried == *set) continue;
+            if (diff.empty()) {
+                diff = syndrome_of(g, side, est);
+                for (int c = 0; c < g.m[side]; ++c) diff[c] ^= target[c];
+            }
+            tried = set;
+            if (flip_fix_diff(g, side, *set, target, diff, est)) {
                 any = true;
                 break;
             }
@@ -414,9 +417,21 @@ int unsatisfied(const Graph &g, int side, const uint8_t *est, const uint8_t *tar
 EtsFix ets_fix(const Graph &g, const Tables &t, int side, const uint8_t *target,
                std::vector<uint8_t> &est) {
     EtsFix r;
+    if (t.groups.empty()) return r;
+    // every table needs exactly two unsatisfied checks, and est only changes
+    // when a table applies (which ends the search)
+    std::vector<uint8_t> diff = syndrome_of(g, side, est);
+    int unsat[3], nu = 0;
+    for (int c = 0; c < g.m[side]; ++c) {
+        diff[c] ^= target[c];
+        if (diff[c] && nu < 3) unsat[nu++] = c;
+    }
+    if (nu != 2) return r;
+    const uint64_t key = (static_cast<uint64_t>(static_cast<uint32_t>(unsat[0])) << 32) |
+                         static_cast<uint32_t>(unsat[1]);
     for (const EtsGroup &grp : t.groups) {
         if (!grp.has[side]) continue;
-        if (ets_one(g, grp.side[side], side, target, est)) {
+        if (ets_one(g, grp.side[side], side, target, diff, key, est)) {
             r.applied = true;
             r.size = grp.size;
             r.label = grp.label;
@@ -428,6 +443,13 @@ EtsFix ets_fix(const Graph &g, const Tables &t, int side, const uint8_t *target,
 
 bool flip_fix(const Graph &g, int side, const std::vector<int> &vars_in, const uint8_t *target,
               std::vector<uint8_t> &est) {
+    std::vector<uint8_t> diff = syndrome_of(g, side, est);
+    for (int c = 0; c < g.m[side]; ++c) diff[c] ^= target[c];
+    return flip_fix_diff(g, side, vars_in, target, diff, est);
+}
+
+bool flip_fix_diff(const Graph &g, int side, const std::vector<int> &vars_in, const uint8_t *target,
+                   const std::vector<uint8_t> &diff, std::vector<uint8_t> &est) {
     std::vector<int> vars;
     for (int v : vars_in)
         if (v >= 0 && v < g.n) vars.push_back(v);
@@ -435,8 +457,6 @@ bool flip_fix(const Graph &g, int side, const std::vector<int> &vars_in, const u
     vars.erase(std::unique(vars.begin(), vars.end()), vars.end());
     if (vars.empty()) return false;
     const int m = g.m[side];
-    std::vector<uint8_t> diff = syndrome_of(g, side, est);
-    for (int c = 0; c < m; ++c) diff[c] ^= target[c];
     const int nv = static_cast<int>(vars.size());
     // more columns than rows cannot all be independent
     if (nv > m) return false;

@@ -84,6 +84,10 @@ EtsFix ets_fix(const Graph &g, const Tables &t, int side, const uint8_t *target,
 bool flip_fix(const Graph &g, int side, const std::vector<int> &vars, const uint8_t *target,
               std::vector<uint8_t> &est);
 
+// flip_fix with diff = syndrome of est XOR target already formed.
+bool flip_fix_diff(const Graph &g, int side, const std::vector<int> &vars, const uint8_t *target,
+                   const std::vector<uint8_t> &diff, std::vector<uint8_t> &est);
+
 // Ordered-statistics repair (jointbp_ets.cpp:4078-4386): with the variables
 // ordered by increasing |LLR|, the shortest prefix whose columns reach the
 // syndrome difference; applied when those columns are independent and the

@@ -77,7 +77,7 @@ constexpr int kRecInts = 4;
 // initial messages.  want_tot: keep the last iteration's marginals.  At the
 // end the block copies its estimate and previous estimate to out[b].
 template <int MAXC, int MAXV>
-__global__ void pp_step_kernel(DevGraph g, TrialPlanes tp, NoiseTab nt, const int *list,
+__global__ __launch_bounds__(256, 2) void pp_step_kernel(DevGraph g, TrialPlanes tp, NoiseTab nt, const int *list,
                                const int *slot_trial, int steps, int init, int want_tot,
                                double damping, char *state, size_t stride, int *rec, uint8_t *out) {
     __shared__ int s_cnt[2];
@@ -313,6 +313,11 @@ int ensure_pp_buffers(jbp_ctx *ctx, int slots, int rec_per_slot, char *err, size
     const size_t back = static_cast<size_t>(slots) * (8 * static_cast<size_t>(g.wn) + 8);
     pb.stage_bytes = std::max(syn, std::max(tot, back));
     HIPCHK(hipMalloc(&pb.stage, pb.stage_bytes));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&pb.h_rec),
+                         sizeof(int) * kRecInts * static_cast<size_t>(slots) * pb.rec_cap));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&pb.h_out), 2 * static_cast<size_t>(g.n) * slots));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&pb.h_tot), tot));
+    HIPCHK(hipHostMalloc(&pb.h_stage, std::max(syn, back)));
     return JBP_OK;
 }
 
@@ -328,6 +333,9 @@ void free_pp(jbp_ctx *ctx) {
                     static_cast<void *>(pb.slot_trial), static_cast<void *>(pb.rec),
                     static_cast<void *>(pb.out), static_cast<void *>(pb.idx), pb.stage})
         if (p) (void)hipFree(p);
+    for (void *p : {static_cast<void *>(pb.h_rec), static_cast<void *>(pb.h_out),
+                    static_cast<void *>(pb.h_tot), pb.h_stage})
+        if (p) (void)hipHostFree(p);
     pb = PPBuf();
 }
 
@@ -358,15 +366,16 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
         std::vector<uint8_t> sx(static_cast<size_t>(K) * mx), sz(static_cast<size_t>(K) * mz);
         for (int s = 0; s < 2; ++s) {
             const int wm = g.wm[s];
-            std::vector<uint32_t> w(static_cast<size_t>(K) * wm);
+            uint32_t *w = static_cast<uint32_t *>(pb.h_stage);
             const uint32_t *src = s == 0 ? ctx->tp.syn_x : ctx->tp.syn_z;
             if (wm % 2 == 0) {
                 launch_gather(src, wm / 2, 0, wm / 2, pb.slot_trial, K, pb.stage, st);
                 HIPCHK(hipGetLastError());
-                HIPCHK(hipMemcpyAsync(w.data(), pb.stage, 4 * w.size(), hipMemcpyDeviceToHost, st));
+                HIPCHK(hipMemcpyAsync(w, pb.stage, 4 * static_cast<size_t>(K) * wm,
+                                      hipMemcpyDeviceToHost, st));
             } else {
                 for (int i = 0; i < K; ++i)
-                    HIPCHK(hipMemcpyAsync(&w[static_cast<size_t>(i) * wm],
+                    HIPCHK(hipMemcpyAsync(w + static_cast<size_t>(i) * wm,
                                           src + static_cast<size_t>(slot_trial[i]) * wm, 4 * wm,
                                           hipMemcpyDeviceToHost, st));
             }
@@ -423,17 +432,16 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             });
             active.clear();
         }
-        std::vector<int> rec;
-        std::vector<uint8_t> out;
+        int *rec = pb.h_rec;
+        uint8_t *out = pb.h_out;
         if (H > 0 && !active.empty()) {
             HIPCHK(hipMemcpyAsync(pb.list, active.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
             launch_step(g, ctx->tp, nt, pb.list, K, pb.slot_trial, H, true, false, damping, pb.state,
                         pb.stride, pb.rec, pb.out, st);
             HIPCHK(hipGetLastError());
-            rec.resize(static_cast<size_t>(K) * H * kRecInts);
-            out.resize(static_cast<size_t>(K) * 2 * n);
-            HIPCHK(hipMemcpyAsync(rec.data(), pb.rec, sizeof(int) * rec.size(), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipMemcpyAsync(out.data(), pb.out, out.size(), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(rec, pb.rec, sizeof(int) * static_cast<size_t>(K) * H * kRecInts,
+                                  hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(out, pb.out, static_cast<size_t>(K) * 2 * n, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             pool.run(K, [&](int i) {
                 for (int t = 0; t < H; ++t) {
@@ -455,17 +463,17 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             active.erase(std::remove_if(active.begin(), active.end(), [&](int i) { return tr[i].done; }),
                          active.end());
         }
-        std::vector<double> tot, llr;
+        double *tot = pb.h_tot;
+        std::vector<double> llr;
         for (int t = std::max(H, 0); t < max_iter && !active.empty(); ++t) {
             const int A = static_cast<int>(active.size());
             HIPCHK(hipMemcpyAsync(pb.list, active.data(), sizeof(int) * A, hipMemcpyHostToDevice, st));
             launch_step(g, ctx->tp, nt, pb.list, A, pb.slot_trial, 1, t == 0, true, damping, pb.state,
                         pb.stride, pb.rec, pb.out, st);
             HIPCHK(hipGetLastError());
-            rec.resize(static_cast<size_t>(A) * kRecInts);
-            out.resize(static_cast<size_t>(A) * 2 * n);
-            HIPCHK(hipMemcpyAsync(rec.data(), pb.rec, sizeof(int) * rec.size(), hipMemcpyDeviceToHost, st));
-            HIPCHK(hipMemcpyAsync(out.data(), pb.out, out.size(), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(rec, pb.rec, sizeof(int) * static_cast<size_t>(A) * kRecInts,
+                                  hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(out, pb.out, static_cast<size_t>(A) * 2 * n, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             pool.run(A, [&](int a) {
                 const int i = active[a];
@@ -510,20 +518,19 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                 if (!tr[i].done && need_llr[i]) want.push_back(i);
             if (!want.empty()) {
                 const int W = static_cast<int>(want.size());
-                tot.resize(static_cast<size_t>(W) * 4 * n);
                 llr.resize(static_cast<size_t>(W) * 2 * n);
                 const size_t tot_off = 6 * static_cast<size_t>(g.side[0].E + g.side[1].E);
                 HIPCHK(hipMemcpyAsync(pb.idx, want.data(), sizeof(int) * W, hipMemcpyHostToDevice, st));
                 launch_gather(pb.state, pb.stride / 8, tot_off, 4 * n, pb.idx, W, pb.stage, st);
                 HIPCHK(hipGetLastError());
-                HIPCHK(hipMemcpyAsync(tot.data(), pb.stage, sizeof(double) * tot.size(),
+                HIPCHK(hipMemcpyAsync(tot, pb.stage, sizeof(double) * static_cast<size_t>(W) * 4 * n,
                                       hipMemcpyDeviceToHost, st));
                 HIPCHK(hipStreamSynchronize(st));
             }
             pool.run(static_cast<int>(want.size()), [&](int w) {
                 const int i = want[w];
                 double *lx = &llr[static_cast<size_t>(w) * 2 * n], *lz = lx + n;
-                pp::abs_llr_from_marginals(n, &tot[static_cast<size_t>(w) * 4 * n], lx, lz);
+                pp::abs_llr_from_marginals(n, tot + static_cast<size_t>(w) * 4 * n, lx, lz);
                 const uint8_t *tsx = &sx[static_cast<size_t>(i) * mx], *tsz = &sz[static_cast<size_t>(i) * mz];
                 if (att[i].want_osd) {
                     pp::attempt_osd(hg, att[i], tsx, tsz, lx, lz);
@@ -549,11 +556,13 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                          active.end());
         }
         // write back: estimate planes, iterations, syndrome flag
-        std::vector<uint32_t> planes(static_cast<size_t>(K) * 2 * g.wn, 0u);
-        std::vector<int> it(K);
-        std::vector<uint8_t> okb(K);
+        const size_t nplane = static_cast<size_t>(K) * 2 * g.wn;
+        uint32_t *planes = static_cast<uint32_t *>(pb.h_stage);
+        int *it = reinterpret_cast<int *>(planes + nplane);
+        uint8_t *okb = reinterpret_cast<uint8_t *>(it + K);
+        std::fill(planes, planes + nplane, 0u);
         for (int i = 0; i < K; ++i) {
-            uint32_t *px = &planes[static_cast<size_t>(i) * 2 * g.wn], *pz = px + g.wn;
+            uint32_t *px = planes + static_cast<size_t>(i) * 2 * g.wn, *pz = px + g.wn;
             for (int v = 0; v < n; ++v) {
                 px[v >> 5] |= static_cast<uint32_t>(est[i][v] & 1) << (v & 31);
                 pz[v >> 5] |= static_cast<uint32_t>((est[i][v] >> 1) & 1) << (v & 31);
@@ -565,11 +574,10 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
         {
             char *stage = static_cast<char *>(pb.stage);
             uint32_t *d_planes = reinterpret_cast<uint32_t *>(stage);
-            int *d_it = reinterpret_cast<int *>(stage + 4 * planes.size());
+            int *d_it = reinterpret_cast<int *>(stage + 4 * nplane);
             uint8_t *d_ok = reinterpret_cast<uint8_t *>(d_it + K);
-            HIPCHK(hipMemcpyAsync(d_planes, planes.data(), 4 * planes.size(), hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(d_it, it.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(d_ok, okb.data(), K, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(d_planes, planes, 4 * nplane + 5 * static_cast<size_t>(K),
+                                  hipMemcpyHostToDevice, st));
             const int bx = std::min(16, (g.wn + 255) / 256);
             hipLaunchKernelGGL(scatter_outcome_kernel, dim3(bx, K), dim3(256), 0, st, d_planes, d_it,
                                d_ok, pb.slot_trial, K, g.wn, ctx->tp);
