@@ -136,7 +136,7 @@ def pp_workload(jbp, dec, args, first):
         L.ref_ets_add.argtypes = [C.c_int, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_void_p]
         L.ref_simulate_pp.restype = C.c_double
         L.ref_simulate_pp.argtypes = [C.c_uint64, C.c_longlong, C.c_longlong, C.c_double, C.c_int,
-                                      C.c_int, C.c_int] + [C.c_void_p] * 7
+                                      C.c_double, C.c_int, C.c_int] + [C.c_void_p] * 7
         if L.ref_load_params(PARAMS.encode()) != 0:
             raise RuntimeError("reference could not read the params file")
         counts = (C.c_int * 2)()
@@ -147,7 +147,7 @@ def pp_workload(jbp, dec, args, first):
         bufs = [(C.c_int * m)()] + [(C.c_uint8 * m)() for _ in range(4)]
         ctr = (C.c_longlong * 10)()
         threads = cpu_threads()
-        secs = L.ref_simulate_pp(args.seed, first, m, p, max_iter, 5, threads, *bufs, None, ctr)
+        secs = L.ref_simulate_pp(args.seed, first, m, p, max_iter, 0.0, 5, threads, *bufs, None, ctr)
         L.ref_ets_clear()
         out["cpu_reference"] = {"trials_per_s": round(m / secs, 2), "cores": threads,
                                 "sample": "%d trials (indices %d..%d), %.1f s wall" %

@@ -16,6 +16,11 @@ struct DevSide {
     const int *edge_var = nullptr;  // E
     const int *var_off = nullptr;   // n+1
     const int *var_edge = nullptr;  // E: edge ids per variable, ascending check
+    // FP64 message rows, position-major: the i-th edge of check c lives in
+    // row i*m + c, so a wavefront walking checks c..c+63 at position i reads
+    // one contiguous span (rows = max check degree x m; unused rows idle)
+    int cap = 0;
+    const int *eslot = nullptr;     // E: row of each edge
 };
 
 struct DevGraph {

@@ -34,10 +34,10 @@ __device__ __forceinline__ SlotPtrs slot_ptrs(const DevGraph &g, double *scratch
     double *base = scratch + static_cast<size_t>(blockIdx.x) * stride;
     SlotPtrs s;
     s.v2c[0] = base;
-    s.v2c[1] = s.v2c[0] + 4 * static_cast<size_t>(g.side[0].E);
-    s.c2v[0] = s.v2c[1] + 4 * static_cast<size_t>(g.side[1].E);
-    s.c2v[1] = s.c2v[0] + 2 * static_cast<size_t>(g.side[0].E);
-    s.est = reinterpret_cast<uint8_t *>(s.c2v[1] + 2 * static_cast<size_t>(g.side[1].E));
+    s.v2c[1] = s.v2c[0] + 4 * static_cast<size_t>(g.side[0].cap);
+    s.c2v[0] = s.v2c[1] + 4 * static_cast<size_t>(g.side[1].cap);
+    s.c2v[1] = s.c2v[0] + 2 * static_cast<size_t>(g.side[0].cap);
+    s.est = reinterpret_cast<uint8_t *>(s.c2v[1] + 2 * static_cast<size_t>(g.side[1].cap));
     return s;
 }
 
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPla
 
         // init (jointbp_ets.cpp:1970-1982): v2c = prior, c2v = uniform
         for (int s = 0; s < 2; ++s) {
-            const int E = g.side[s].E;
+            const int E = g.side[s].cap;
             for (int e = tid; e < E; e += bs) {
                 double *m = sp.v2c[s] + 4 * static_cast<size_t>(e);
                 m[0] = prior.v[0]; m[1] = prior.v[1]; m[2] = prior.v[2]; m[3] = prior.v[3];
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPla
 }  // namespace
 
 size_t decode_f64_scratch_bytes(const DevGraph &g, int slots) {
-    size_t per = (6 * static_cast<size_t>(g.side[0].E + g.side[1].E)) * sizeof(double) +
+    size_t per = (6 * static_cast<size_t>(g.side[0].cap + g.side[1].cap)) * sizeof(double) +
                  ((static_cast<size_t>(g.n) + 7) / 8) * sizeof(double);
     return per * static_cast<size_t>(slots);
 }

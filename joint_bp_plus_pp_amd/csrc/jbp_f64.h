@@ -44,17 +44,17 @@ __device__ __forceinline__ M4 mix4(const M4 &nw, const M4 &old, double d) {
 template <int MAXC>
 __device__ void check_update(const DevSide &sd, int side, int c, int syn, const double *v2c,
                              double *c2v) {
-    const int off = sd.chk_off[c];
-    const int deg = sd.chk_off[c + 1] - off;
+    const int deg = sd.chk_off[c + 1] - sd.chk_off[c];
+    const size_t m = static_cast<size_t>(sd.m);
     double q0[MAXC], q1[MAXC], pe[MAXC + 1], po[MAXC + 1];
     pe[0] = 1.0;
     po[0] = 0.0;
 #pragma unroll
     for (int i = 0; i < MAXC; ++i) {
         if (i < deg) {
-            const double *m = v2c + 4 * static_cast<size_t>(off + i);
-            q0[i] = side == 0 ? m[0] + m[2] : m[0] + m[1];
-            q1[i] = side == 0 ? m[1] + m[3] : m[2] + m[3];
+            const double *mv = v2c + 4 * (i * m + c);
+            q0[i] = side == 0 ? mv[0] + mv[2] : mv[0] + mv[1];
+            q1[i] = side == 0 ? mv[1] + mv[3] : mv[2] + mv[3];
             pe[i + 1] = pe[i] * q0[i] + po[i] * q1[i];
             po[i + 1] = pe[i] * q1[i] + po[i] * q0[i];
         }
@@ -68,7 +68,7 @@ __device__ void check_update(const DevSide &sd, int side, int c, int syn, const 
             double v0 = syn == 0 ? p_even : p_odd;
             double v1 = syn == 0 ? p_odd : p_even;
             double sum = side == 0 ? ((v0 + v1) + v0) + v1 : ((v0 + v0) + v1) + v1;
-            double *o = c2v + 2 * static_cast<size_t>(off + i);
+            double *o = c2v + 2 * (i * m + c);
             if (sum <= 0.0) {
                 o[0] = 0.25;
                 o[1] = 0.25;
@@ -84,9 +84,10 @@ __device__ void check_update(const DevSide &sd, int side, int c, int syn, const 
     }
 }
 
-__device__ __forceinline__ M4 c2v_msg(const double *c2v, int e, int side) {
-    double a = c2v[2 * static_cast<size_t>(e)];
-    double b = c2v[2 * static_cast<size_t>(e) + 1];
+// (r: the edge's row)
+__device__ __forceinline__ M4 c2v_msg(const double *c2v, int r, int side) {
+    double a = c2v[2 * static_cast<size_t>(r)];
+    double b = c2v[2 * static_cast<size_t>(r) + 1];
     return side == 0 ? M4{{a, b, a, b}} : M4{{a, a, b, b}};
 }
 
@@ -111,7 +112,8 @@ __device__ int var_update(const DevGraph &g, int v, const M4 &prior, double damp
 #pragma unroll
         for (int i = 0; i < MAXV; ++i)
             if (i < deg[s])
-                pref[s][i + 1] = mul4(pref[s][i], c2v_msg(c2v[s], g.side[s].var_edge[b[s] + i], s));
+                pref[s][i + 1] = mul4(pref[s][i],
+                                      c2v_msg(c2v[s], g.side[s].eslot[g.side[s].var_edge[b[s] + i]], s));
     }
     M4 allx = ident, allz = ident;
 #pragma unroll
@@ -135,7 +137,7 @@ __device__ int var_update(const DevGraph &g, int v, const M4 &prior, double damp
 #pragma unroll
     for (int i = MAXV - 1; i >= 0; --i) {
         if (i < deg[0]) {
-            int e = g.side[0].var_edge[b[0] + i];
+            int e = g.side[0].eslot[g.side[0].var_edge[b[0] + i]];
             M4 out = mul4(mul4(prior, mul4(pref[0][i], suf)), allz);
             normalize4(out);
             double *slot = v2c[0] + 4 * static_cast<size_t>(e);
@@ -153,7 +155,7 @@ __device__ int var_update(const DevGraph &g, int v, const M4 &prior, double damp
 #pragma unroll
     for (int i = MAXV - 1; i >= 0; --i) {
         if (i < deg[1]) {
-            int e = g.side[1].var_edge[b[1] + i];
+            int e = g.side[1].eslot[g.side[1].var_edge[b[1] + i]];
             M4 out = mul4(px, mul4(pref[1][i], suf));
             normalize4(out);
             double *slot = v2c[1] + 4 * static_cast<size_t>(e);

@@ -39,8 +39,8 @@ namespace {
 using namespace f64;
 
 // Per-slot state, in bytes from the slot's base: v2c[2] (4 doubles per
-// edge), c2v[2] (2 per edge), marginals (4 doubles per qubit), then the
-// estimate and the previous estimate (one byte per qubit each).
+// message row), c2v[2] (2 per row), marginals (4 doubles per qubit), then
+// the estimate and the previous estimate (one byte per qubit each).
 struct StepSlot {
     double *v2c[2];
     double *c2v[2];
@@ -59,10 +59,10 @@ __device__ inline StepSlot step_slot(const DevGraph &g, char *state, size_t stri
     double *base = reinterpret_cast<double *>(state + static_cast<size_t>(slot) * stride);
     StepSlot s;
     s.v2c[0] = base;
-    s.v2c[1] = s.v2c[0] + 4 * static_cast<size_t>(g.side[0].E);
-    s.c2v[0] = s.v2c[1] + 4 * static_cast<size_t>(g.side[1].E);
-    s.c2v[1] = s.c2v[0] + 2 * static_cast<size_t>(g.side[0].E);
-    s.tot = s.c2v[1] + 2 * static_cast<size_t>(g.side[1].E);
+    s.v2c[1] = s.v2c[0] + 4 * static_cast<size_t>(g.side[0].cap);
+    s.c2v[0] = s.v2c[1] + 4 * static_cast<size_t>(g.side[1].cap);
+    s.c2v[1] = s.c2v[0] + 2 * static_cast<size_t>(g.side[0].cap);
+    s.tot = s.c2v[1] + 2 * static_cast<size_t>(g.side[1].cap);
     s.est = reinterpret_cast<uint8_t *>(s.tot + 4 * static_cast<size_t>(g.n));
     s.prev = s.est + g.n;
     return s;
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256, 2) void pp_step_kernel(DevGraph g, TrialPlanes
     const uint32_t *tsz = tp.syn_z + static_cast<size_t>(trial) * g.wm[1];
     if (init) {
         for (int s = 0; s < 2; ++s)
-            for (int e = tid; e < g.side[s].E; e += bs) {
+            for (int e = tid; e < g.side[s].cap; e += bs) {
                 double *m = sp.v2c[s] + 4 * static_cast<size_t>(e);
                 m[0] = prior.v[0]; m[1] = prior.v[1]; m[2] = prior.v[2]; m[3] = prior.v[3];
                 sp.c2v[s][2 * static_cast<size_t>(e)] = 0.25;
@@ -294,7 +294,7 @@ constexpr size_t kPPStateBytes = size_t(8) << 30;
 int ensure_pp_buffers(jbp_ctx *ctx, int slots, int rec_per_slot, char *err, size_t errlen) {
     PPBuf &pb = ctx->ppb;
     const DevGraph &g = ctx->dg;
-    const size_t stride = step_slot_bytes(g.side[0].E, g.side[1].E, g.n);
+    const size_t stride = step_slot_bytes(g.side[0].cap, g.side[1].cap, g.n);
     if (pb.cap >= slots && pb.rec_cap >= rec_per_slot && pb.stride == stride) return JBP_OK;
     free_pp(ctx);
     pb.cap = slots;
@@ -352,7 +352,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
     outs.assign(trials.size(), PPOutcome());
     Pool pool(pool_threads());
     hipStream_t st = ctx->stream;
-    const size_t stride = step_slot_bytes(g.side[0].E, g.side[1].E, n);
+    const size_t stride = step_slot_bytes(g.side[0].cap, g.side[1].cap, n);
     const size_t batch = std::max<size_t>(1, std::min<size_t>(kPPSlots, kPPStateBytes / stride));
     for (size_t b0 = 0; b0 < trials.size(); b0 += batch) {
         const int K = static_cast<int>(std::min<size_t>(batch, trials.size() - b0));
@@ -519,7 +519,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             if (!want.empty()) {
                 const int W = static_cast<int>(want.size());
                 llr.resize(static_cast<size_t>(W) * 2 * n);
-                const size_t tot_off = 6 * static_cast<size_t>(g.side[0].E + g.side[1].E);
+                const size_t tot_off = 6 * static_cast<size_t>(g.side[0].cap + g.side[1].cap);
                 HIPCHK(hipMemcpyAsync(pb.idx, want.data(), sizeof(int) * W, hipMemcpyHostToDevice, st));
                 launch_gather(pb.state, pb.stride / 8, tot_off, 4 * n, pb.idx, W, pb.stage, st);
                 HIPCHK(hipGetLastError());

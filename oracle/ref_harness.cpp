@@ -295,7 +295,8 @@ int ref_pp_repair(int method, int side, const int *syndrome, const double *abs_l
     return applied;
 }
 
-// The simulate loop with PP on (jointbp_ets.cpp:5931-6228): the decode with
+// The simulate loop with PP on (jointbp_ets.cpp:5931-6228), --damping as
+// given: the decode with
 // in-loop PP, then the post-decode ETS / flip / OSD passes and the
 // reference's counting.  Per trial: iterations, final syndrome match,
 // success, exact, flags (bit 0 BP match, 1 PP success, 2 ETS saved, 3 flip
@@ -304,7 +305,7 @@ int ref_pp_repair(int method, int side, const int *syndrome, const double *abs_l
 // osd_pp_success, stab_success, ets_used, ets_saved, ets6_used, ets6_saved,
 // ets12_used, ets12_saved.  Returns wall seconds.
 double ref_simulate_pp(uint64_t base, long long first_idx, long long count, double p, int max_iter,
-                       int flip_hist, int threads, int *iters, unsigned char *syn_ok,
+                       double damping, int flip_hist, int threads, int *iters, unsigned char *syn_ok,
                        unsigned char *success, unsigned char *exact, unsigned char *flags,
                        int *est_out, long long *counters) {
     Msg prior = make_prior(p);
@@ -324,7 +325,7 @@ double ref_simulate_pp(uint64_t base, long long first_idx, long long count, doub
             auto sz = compute_syndrome(g_ref.z_checks, err, false);
             JointBPResult res = joint_bp_decode(
                 g_ref.x_checks, g_ref.z_checks, g_ref.var_to_x, g_ref.var_to_z, sx, sz, prior,
-                max_iter, flip_hist, false, 0.0, false, false, true, &pp_ctx, &err,
+                max_iter, flip_hist, false, damping, false, false, true, &pp_ctx, &err,
                 &g_ref.hx_basis, &g_ref.hz_basis, nullptr, nullptr);
             bool bp_ok = res.bp_syndrome_match;
             bool used6 = res.pp_used_ets6;

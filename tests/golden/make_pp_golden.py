@@ -36,12 +36,13 @@ GROUPS = [(6, "6", "ets_6_2.txt"), (8, "8", "ets_8_2.txt"), (8, "8p4", "ets_8_2_
           (10, "10", "ets_10_2.txt"), (12, "12", "ets_12_2.txt"), (14, "14", "ets_14_2.txt"),
           (16, "16", "ets_16_2.txt"), (18, "18", "ets_18_2.txt"), (20, "20", "ets_20_2.txt"),
           (22, "22", "ets_22_2.txt")]
-# per-trial workloads: (name, p, max_iter, trials, seed, flip_hist)
-WORKLOADS = [("p0.04_it8", 0.04, 8, 256, 7, 5),
-             ("p0.03_it5", 0.03, 5, 256, 7, 5),
-             ("p0.04_it8_hist0", 0.04, 8, 128, 21, 0),
-             ("p0.02_it1", 0.02, 1, 128, 5, 5),
-             ("p0.05_it115", 0.05, 115, 512, 11, 5)]
+# per-trial workloads: (name, p, max_iter, trials, seed, flip_hist, damping)
+WORKLOADS = [("p0.04_it8", 0.04, 8, 256, 7, 5, 0.0),
+             ("p0.03_it5", 0.03, 5, 256, 7, 5, 0.0),
+             ("p0.04_it8_hist0", 0.04, 8, 128, 21, 0, 0.0),
+             ("p0.02_it1", 0.02, 1, 128, 5, 5, 0.0),
+             ("p0.05_it115", 0.05, 115, 512, 11, 5, 0.0),
+             ("p0.04_it10_damp0.3", 0.04, 10, 128, 13, 5, 0.3)]
 # a small hypergraph-product code (built by libjbp's seeded generator), no
 # trapping-set tables: (m, n, col_w, row_w, seed), p, max_iter, trials, seed
 HGP = ((15, 20, 3, 4, 5), 0.05, 6, 256, 3)
@@ -68,7 +69,7 @@ def load():
                                 C.c_void_p]
     h.ref_simulate_pp.restype = C.c_double
     h.ref_simulate_pp.argtypes = [C.c_uint64, C.c_longlong, C.c_longlong, C.c_double, C.c_int,
-                                  C.c_int, C.c_int] + [C.c_void_p] * 7
+                                  C.c_double, C.c_int, C.c_int] + [C.c_void_p] * 7
     assert h.ref_load_params(PARAMS.encode()) == 0
     return h
 
@@ -215,17 +216,17 @@ def main():
                         **{k: np.asarray(v) for k, v in cases.items()})
     print("repairs:", len(cases["method"]), "applied:", int(np.count_nonzero(cases["applied"])))
 
-    for name, p, max_iter, trials, seed, hist in WORKLOADS:
+    for name, p, max_iter, trials, seed, hist, damping in WORKLOADS:
         it = np.zeros(trials, np.int32)
         ok, su, ex, fl = (np.zeros(trials, np.uint8) for _ in range(4))
         est = np.zeros((trials, n), np.int32)
         ctr = np.zeros(10, np.int64)
-        sec = h.ref_simulate_pp(seed, 1, trials, p, max_iter, hist, 8, it.ctypes.data, ok.ctypes.data,
+        sec = h.ref_simulate_pp(seed, 1, trials, p, max_iter, damping, hist, 8, it.ctypes.data, ok.ctypes.data,
                                 su.ctypes.data, ex.ctypes.data, fl.ctypes.data, est.ctypes.data,
                                 ctr.ctypes.data)
         digests = np.array([est_digest(e) for e in est], dtype="S40")
         np.savez_compressed(os.path.join(HERE, "pp_trials_%s.npz" % name), p=p, max_iter=max_iter,
-                            seed=seed, flip_hist=hist, first_trial=1, iterations=it, syn_match=ok,
+                            seed=seed, flip_hist=hist, damping=damping, first_trial=1, iterations=it, syn_match=ok,
                             success=su, exact=ex, flags=fl, est_sha1=digests, counters=ctr)
         print(name, "%.1fs" % sec, "counters", ctr.tolist())
 
@@ -244,12 +245,12 @@ def main():
     ok, su, ex, fl = (np.zeros(trials, np.uint8) for _ in range(4))
     est = np.zeros((trials, code.n), np.int32)
     ctr = np.zeros(10, np.int64)
-    sec = h.ref_simulate_pp(seed, 1, trials, p, max_iter, 5, 8, it.ctypes.data, ok.ctypes.data,
+    sec = h.ref_simulate_pp(seed, 1, trials, p, max_iter, 0.0, 5, 8, it.ctypes.data, ok.ctypes.data,
                             su.ctypes.data, ex.ctypes.data, fl.ctypes.data, est.ctypes.data,
                             ctr.ctypes.data)
     digests = np.array([est_digest(e) for e in est], dtype="S40")
     np.savez_compressed(os.path.join(HERE, "pp_trials_hgp15x20.npz"), hgp=np.array([hm, hn, cw, rw, hseed]),
-                        p=p, max_iter=max_iter, seed=seed, flip_hist=5, first_trial=1, iterations=it,
+                        p=p, max_iter=max_iter, seed=seed, flip_hist=5, damping=0.0, first_trial=1, iterations=it,
                         syn_match=ok, success=su, exact=ex, flags=fl, est_sha1=digests, counters=ctr)
     print("hgp15x20", "%.1fs" % sec, "counters", ctr.tolist())
 

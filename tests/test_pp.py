@@ -11,7 +11,8 @@ from conftest import PARAMS, golden_npz
 
 jbp = pytest.importorskip("joint_bp_plus_pp_amd")
 
-WORKLOADS = ["p0.04_it8", "p0.03_it5", "p0.04_it8_hist0", "p0.02_it1", "p0.05_it115"]
+WORKLOADS = ["p0.04_it8", "p0.03_it5", "p0.04_it8_hist0", "p0.02_it1", "p0.05_it115",
+             "p0.04_it10_damp0.3"]
 COUNTERS = ["pp_success", "flip_pp_success", "osd_pp_success", "stab_success", "ets_used",
             "ets_saved", "ets6_used", "ets6_saved", "ets12_used", "ets12_saved"]
 
@@ -32,7 +33,7 @@ def run(dec, g, precision, **kw):
     return dec.simulate(int(g["seed"]), len(g["iterations"]), float(g["p"]),
                         max_iter=int(g["max_iter"]), first_trial=int(g["first_trial"]),
                         precision=precision, pp=True, flip_hist=int(g["flip_hist"]), want_est=True,
-                        **kw)
+                        damping=float(g["damping"]), **kw)
 
 
 def check_equal(stats, per, g):
