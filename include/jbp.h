@@ -237,6 +237,18 @@ int jbp_decode_batch_pp(jbp_ctx *ctx, int count, const uint32_t *sx_bits,
                         uint8_t *syn_match_out, uint8_t *pp_out, int *est_out, char *err,
                         size_t errlen);
 
+/* FP64 decode of count syndrome pairs (per decode the reference CPU
+ * decoder's iterations and estimate) that also returns the per-qubit |LLR|
+ * of the X and Z parts of the last iteration's marginals (abs_llr_x /
+ * abs_llr_z, jointbp_ets.cpp:2483-2493: the ordering OSD uses), formed with
+ * the C library's log from the device's FP64 marginals, so they are the
+ * reference's values bit for bit.  llr_out: count x 2 x nvars doubles (X
+ * then Z per decode); any output may be NULL.  max_iter >= 1. */
+int jbp_decode_llr(jbp_ctx *ctx, int count, const uint32_t *sx_bits, const uint32_t *sz_bits,
+                   const double prior[4], int max_iter, double damping, int *iterations_out,
+                   uint8_t *syn_match_out, int *est_out, double *llr_out, char *err,
+                   size_t errlen);
+
 /* Sample only: errors of trials first..first+count-1 (count x nvars states)
  * and their syndromes (count x mX, count x mZ ints; either may be NULL). */
 int jbp_sample(jbp_ctx *ctx, uint64_t base_seed, int64_t first_trial, int count, double p,

@@ -137,6 +137,7 @@ def lib():
         "jbp_set_pp_tables": (i32, [vp, vp, cp, sz]),
         "jbp_pp_repair": (i32, [C.POINTER(GraphDesc), vp, i32, i32, vp, vp, vp, i32, vp, vp, cp, sz]),
         "jbp_decode_batch_pp": (i32, [vp, i32, vp, vp, vp, i32, dbl, i32, i32, vp, vp, vp, vp, cp, sz]),
+        "jbp_decode_llr": (i32, [vp, i32, vp, vp, vp, i32, dbl, vp, vp, vp, vp, cp, sz]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -442,6 +443,24 @@ class Decoder:
         _check(lib().jbp_decode_batch(self._h, count, _ptr(bx), _ptr(bz), _ptr(pr), max_iter,
                                       damping, precision, _ptr(it), _ptr(ok), _ptr(est), e, 512), e)
         return it, ok, est
+
+    def decode_llr(self, sx, sz, prior, max_iter=50, damping=0.0):
+        """FP64 decodes (the reference's, per decode) with the |LLR| of the
+        last iteration: returns (iterations, syn_match, est, llr) with llr of
+        shape (count, 2, n), X then Z (jbp_decode_llr)."""
+        sx = np.atleast_2d(np.asarray(sx))
+        sz = np.atleast_2d(np.asarray(sz))
+        count = sx.shape[0]
+        bx, bz = pack_bits(sx), pack_bits(sz)
+        pr = np.ascontiguousarray(prior, dtype=np.float64)
+        it = np.zeros(count, np.int32)
+        ok = np.zeros(count, np.uint8)
+        est = np.zeros((count, self.code.n), np.int32)
+        llr = np.zeros((count, 2, self.code.n), np.float64)
+        e = _errbuf()
+        _check(lib().jbp_decode_llr(self._h, count, _ptr(bx), _ptr(bz), _ptr(pr), max_iter, damping,
+                                    _ptr(it), _ptr(ok), _ptr(est), _ptr(llr), e, 512), e)
+        return it, ok, est, llr
 
     def sample(self, base_seed, first_trial, count, p):
         """Device-sampled errors and syndromes of trials first..first+count-1."""

@@ -74,8 +74,9 @@ constexpr int kRecInts = 4;
 
 // Block b advances the trial of slot list[b] by up to `steps` iterations
 // (stopping when both syndromes match).  init: start from the reference's
-// initial messages.  want_tot: keep the last iteration's marginals.  At the
-// end the block copies its estimate and previous estimate to out[b].
+// initial messages.  want_tot: write every iteration's marginals (the slot
+// then holds the last iteration's).  At the end the block copies its
+// estimate and previous estimate to out[b].
 template <int MAXC, int MAXV>
 __global__ __launch_bounds__(256, 2) void pp_step_kernel(DevGraph g, TrialPlanes tp, NoiseTab nt, const int *list,
                                const int *slot_trial, int steps, int init, int want_tot,
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(256, 2) void pp_step_kernel(DevGraph g, TrialPlanes
             }
         }
         __syncthreads();
-        double *tot = (want_tot && j == steps - 1) ? sp.tot : nullptr;
+        double *tot = want_tot ? sp.tot : nullptr;
         int changed1 = 0, changed2 = 0;
         for (int v = tid; v < n; v += bs) {
             const int best = var_update<MAXV>(g, v, prior, damping, sp.v2c, sp.c2v, tot);
@@ -667,6 +668,78 @@ extern "C" int jbp_decode_batch_pp(jbp_ctx *ctx, int count, const uint32_t *sx_b
         HIPCHK(hipMemcpyAsync(est_out, ctx->d_states, static_cast<size_t>(count) * g.n * 4,
                               hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    return JBP_OK;
+}
+
+extern "C" int jbp_decode_llr(jbp_ctx *ctx, int count, const uint32_t *sx_bits, const uint32_t *sz_bits,
+                              const double prior[4], int max_iter, double damping,
+                              int *iterations_out, uint8_t *syn_match_out, int *est_out,
+                              double *llr_out, char *err, size_t errlen) {
+    using namespace jbp;
+    if (!ctx || count < 0 || !sx_bits || !sz_bits || !prior || max_iter < 1 || damping < 0.0) {
+        set_err(err, errlen, "bad arguments");
+        return JBP_ERR_ARG;
+    }
+    if (count == 0) return JBP_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = ensure_chunk(ctx, count, err, errlen);
+    if (rc) return rc;
+    const DevGraph &g = ctx->dg;
+    const int n = g.n;
+    HIPCHK(hipMemcpyAsync(ctx->tp.syn_x, sx_bits, static_cast<size_t>(count) * g.wm[0] * 4,
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->tp.syn_z, sz_bits, static_cast<size_t>(count) * g.wm[1] * 4,
+                          hipMemcpyHostToDevice, ctx->stream));
+    {
+        const Prior pri = make_prior(prior);
+        if ((rc = set_noise(ctx, &pri, nullptr, 1, err, errlen))) return rc;
+    }
+    hipStream_t st = ctx->stream;
+    const size_t batch = std::max<size_t>(
+        1, std::min<size_t>(kPPSlots, kPPStateBytes / step_slot_bytes(g.side[0].cap, g.side[1].cap, n)));
+    for (int b0 = 0; b0 < count; b0 += static_cast<int>(batch)) {
+        const int K = static_cast<int>(std::min<size_t>(batch, static_cast<size_t>(count - b0)));
+        if ((rc = ensure_pp_buffers(ctx, K, max_iter, err, errlen))) return rc;
+        PPBuf &pb = ctx->ppb;
+        std::vector<int> ids(K);
+        for (int i = 0; i < K; ++i) ids[i] = i;
+        std::vector<int> trials(K);
+        for (int i = 0; i < K; ++i) trials[i] = b0 + i;
+        HIPCHK(hipMemcpyAsync(pb.list, ids.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(pb.slot_trial, trials.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
+        // every iteration in one launch, marginals kept (the last one stays)
+        launch_step(g, ctx->tp, ctx->noise, pb.list, K, pb.slot_trial, max_iter, true, true, damping,
+                    pb.state, pb.stride, pb.rec, pb.out, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(pb.h_rec, pb.rec, sizeof(int) * kRecInts * static_cast<size_t>(K) * max_iter,
+                              hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(pb.h_out, pb.out, static_cast<size_t>(K) * 2 * n, hipMemcpyDeviceToHost, st));
+        launch_gather(pb.state, pb.stride / 8, 6 * static_cast<size_t>(g.side[0].cap + g.side[1].cap),
+                      4 * n, pb.list, K, pb.stage, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(pb.h_tot, pb.stage, sizeof(double) * static_cast<size_t>(K) * 4 * n,
+                              hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int i = 0; i < K; ++i) {
+            int iters = max_iter, match = 0;
+            for (int t = 0; t < max_iter; ++t) {
+                const int *r = pb.h_rec + (static_cast<size_t>(i) * max_iter + t) * kRecInts;
+                if (r[0] == 0 && r[1] == 0) {
+                    iters = t + 1;
+                    match = 1;
+                    break;
+                }
+            }
+            const size_t k = static_cast<size_t>(b0 + i);
+            if (iterations_out) iterations_out[k] = iters;
+            if (syn_match_out) syn_match_out[k] = static_cast<uint8_t>(match);
+            if (est_out)
+                for (int v = 0; v < n; ++v) est_out[k * n + v] = pb.h_out[static_cast<size_t>(i) * 2 * n + v];
+            if (llr_out)
+                pp::abs_llr_from_marginals(n, pb.h_tot + static_cast<size_t>(i) * 4 * n, llr_out + k * 2 * n,
+                                           llr_out + k * 2 * n + n);
+        }
     }
     return JBP_OK;
 }

@@ -243,6 +243,31 @@ double ref_simulate(uint64_t base, long long first_idx, long long count, double 
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// One decode with PP off that keeps the |LLR| of the last iteration run
+// (abs_llr_x / abs_llr_z, jointbp_ets.cpp:2483-2493): the per-iteration
+// diagnostics path (truth vector + iteration log) collects them at every
+// iteration, so a decode that converges early keeps its last iteration's.
+// llr_out: 2 x nvars (X then Z).  Returns iterations.
+int ref_decode_llr(const int *sx_in, const int *sz_in, const double *prior4, int max_iter,
+                   double damping, int *est_out, int *syn_match, double *llr_out) {
+    std::vector<int> sx(sx_in, sx_in + g_ref.x_checks.size());
+    std::vector<int> sz(sz_in, sz_in + g_ref.z_checks.size());
+    Msg prior{prior4[0], prior4[1], prior4[2], prior4[3]};
+    std::vector<int> truth(g_ref.nvars, 0);
+    std::vector<std::string> logs;
+    JointBPResult res = joint_bp_decode(g_ref.x_checks, g_ref.z_checks, g_ref.var_to_x, g_ref.var_to_z,
+                                        sx, sz, prior, max_iter, 5, false, damping, false, false, true,
+                                        nullptr, &truth, &g_ref.hx_basis, &g_ref.hz_basis, &logs, nullptr);
+    std::copy(res.est.begin(), res.est.end(), est_out);
+    *syn_match = res.syndrome_match ? 1 : 0;
+    const size_t n = static_cast<size_t>(g_ref.nvars);
+    for (size_t v = 0; v < n; ++v) {
+        llr_out[v] = res.abs_llr_x.size() == n ? res.abs_llr_x[v] : -1.0;
+        llr_out[n + v] = res.abs_llr_z.size() == n ? res.abs_llr_z[v] : -1.0;
+    }
+    return res.iterations;
+}
+
 // Loads one table group with read_ets_file (jointbp_ets.cpp:5238-5307 keeps a
 // side when its file reads and has entries).  counts: entries per side, 0
 // empty, -1 missing.

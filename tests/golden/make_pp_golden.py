@@ -67,6 +67,9 @@ def load():
     h.ref_ets_add.argtypes = [C.c_int, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_void_p]
     h.ref_pp_repair.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                 C.c_void_p]
+    h.ref_decode_llr.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                 C.c_void_p, C.c_void_p, C.c_void_p]
+    h.ref_prior.argtypes = [C.c_double, C.c_void_p]
     h.ref_simulate_pp.restype = C.c_double
     h.ref_simulate_pp.argtypes = [C.c_uint64, C.c_longlong, C.c_longlong, C.c_double, C.c_int,
                                   C.c_double, C.c_int, C.c_int] + [C.c_void_p] * 7
@@ -229,6 +232,33 @@ def main():
                             seed=seed, flip_hist=hist, damping=damping, first_trial=1, iterations=it, syn_match=ok,
                             success=su, exact=ex, flags=fl, est_sha1=digests, counters=ctr)
         print(name, "%.1fs" % sec, "counters", ctr.tolist())
+
+    # |LLR| of the last iteration of plain decodes (what OSD orders by)
+    llr_rec = {k: [] for k in ("max_iter", "trial", "iterations", "syn_match", "est_sha1",
+                               "llr_sha1", "llr_head")}
+    prior = np.zeros(4)
+    h.ref_prior(0.04, prior.ctypes.data)
+    for max_iter in (8, 50):
+        for trial in range(1, 7):
+            err = np.zeros(n, np.int32)
+            h.ref_sample(7, trial, 0.04, err.ctypes.data)
+            sx, sz = syndromes(err)
+            est = np.zeros(n, np.int32)
+            ok = C.c_int(0)
+            llr = np.zeros(2 * n)
+            it = h.ref_decode_llr(sx.ctypes.data, sz.ctypes.data, prior.ctypes.data, max_iter, 0.0,
+                                  est.ctypes.data, C.byref(ok), llr.ctypes.data)
+            llr_rec["max_iter"].append(max_iter)
+            llr_rec["trial"].append(trial)
+            llr_rec["iterations"].append(it)
+            llr_rec["syn_match"].append(ok.value)
+            llr_rec["est_sha1"].append(est_digest(est))
+            llr_rec["llr_sha1"].append(hashlib.sha1(llr.tobytes()).hexdigest())
+            llr_rec["llr_head"].append(llr[:8].copy())
+    np.savez_compressed(os.path.join(HERE, "llr_decodes.npz"), p=0.04, seed=7,
+                        **{k: np.asarray(v, dtype="S40" if k.endswith("sha1") else None)
+                           for k, v in llr_rec.items()})
+    print("llr:", list(zip(llr_rec["max_iter"], llr_rec["iterations"], llr_rec["syn_match"])))
 
     # small HGP code, no tables
     import sys

@@ -138,3 +138,25 @@ def test_decode_batch_pp_equals_simulate(pp_decoder):
     # where the decode succeeded the flags are the reference's
     succ = g["success"][:count] == 1
     np.testing.assert_array_equal(flags[succ], g["flags"][:count][succ])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("max_iter", [8, 50])
+def test_decode_llr_equals_reference(main_decoder, max_iter):
+    """|LLR| of the last iteration (what OSD orders by), bit for bit with the
+    reference's abs_llr_x / abs_llr_z, together with its iterations and
+    estimate."""
+    g = golden_npz("llr_decodes.npz")
+    sel = np.nonzero(g["max_iter"] == max_iter)[0]
+    trials = g["trial"][sel]
+    print("decode_llr", max_iter, flush=True)
+    err, sx, sz = main_decoder.sample(int(g["seed"]), int(trials[0]), len(trials), float(g["p"]))
+    assert list(trials) == list(range(int(trials[0]), int(trials[0]) + len(trials)))
+    it, ok, est, llr = main_decoder.decode_llr(sx, sz, jbp.prior_from_p(float(g["p"])),
+                                               max_iter=max_iter)
+    np.testing.assert_array_equal(it, g["iterations"][sel])
+    np.testing.assert_array_equal(ok, g["syn_match"][sel])
+    np.testing.assert_array_equal(np.array([digest(e) for e in est]), g["est_sha1"][sel])
+    np.testing.assert_array_equal(llr[:, 0, :8], g["llr_head"][sel])
+    got = np.array([hashlib.sha1(np.ascontiguousarray(l).tobytes()).hexdigest().encode() for l in llr])
+    np.testing.assert_array_equal(got, g["llr_sha1"][sel])
