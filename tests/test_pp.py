@@ -160,3 +160,35 @@ def test_decode_llr_equals_reference(main_decoder, max_iter):
     np.testing.assert_array_equal(llr[:, 0, :8], g["llr_head"][sel])
     got = np.array([hashlib.sha1(np.ascontiguousarray(l).tobytes()).hexdigest().encode() for l in llr])
     np.testing.assert_array_equal(got, g["llr_sha1"][sel])
+
+
+@pytest.mark.gpu
+def test_pp_interleaved_p_equals_single_p(pp_decoder):
+    """With a p list, trial idx uses p[idx % len]: its post-processed outcome
+    equals the same index of a single-p run at that p (channel prior of the
+    stage's FP64 decode included)."""
+    ps = [0.03, 0.04]
+    n = 192
+    print("pp interleaved", flush=True)
+    _, mixed = pp_decoder.simulate(5, n, ps, max_iter=8, precision=jbp.FP64, pp=True, want_est=True)
+    for k, p in enumerate(ps):
+        _, single = pp_decoder.simulate(5, n, p, max_iter=8, precision=jbp.FP64, pp=True,
+                                        want_est=True)
+        idx = np.arange(n)
+        sel = ((idx + 1) % len(ps)) == k  # trial index = first_trial + i = i + 1
+        for key in ("iterations", "syn_match", "success", "exact", "pp_flags"):
+            np.testing.assert_array_equal(mixed[key][sel], single[key][sel], err_msg=key)
+        np.testing.assert_array_equal(mixed["est"][sel], single["est"][sel])
+
+
+@pytest.mark.gpu
+def test_pp_after_hbm_engine_first_pass(pp_decoder):
+    """The HBM-resident FP32 engine as the first pass: the trials it leaves
+    go through the same FP64 stage (outcomes agree with the LDS engine's
+    first pass except where the two FP32 engines round differently)."""
+    g = golden_npz("pp_trials_p0.04_it8.npz")
+    print("pp hbm engine", flush=True)
+    s_h, h = run(pp_decoder, g, jbp.FP32, engine=jbp.ENGINE_HBM, slots=128)
+    s_l, l = run(pp_decoder, g, jbp.FP32, engine=jbp.ENGINE_LDS)
+    assert np.mean((h["success"] == l["success"]) & (h["pp_flags"] == l["pp_flags"])) >= 0.97
+    assert s_h["pp_trials"] > 0
