@@ -44,7 +44,7 @@ __device__ __forceinline__ SlotPtrs slot_ptrs(const DevGraph &g, double *scratch
 // Blocks of up to 512 threads (the tuner's largest), at least two resident
 // per compute unit: the register budget this leaves (up to 128 VGPRs) keeps
 // the check recursion's arrays in registers.
-template <int MAXC, int MAXV>
+template <int MAXC, int MAXV, bool PAIR>
 __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPlanes tp, int count, NoiseTab nt, int max_iter,
                                   double damping, double *scratch, size_t stride, int *counter) {
     __shared__ int s_trial;
@@ -70,8 +70,7 @@ __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPla
         for (int s = 0; s < 2; ++s) {
             const int E = g.side[s].cap;
             for (int e = tid; e < E; e += bs) {
-                double *m = sp.v2c[s] + 4 * static_cast<size_t>(e);
-                m[0] = prior.v[0]; m[1] = prior.v[1]; m[2] = prior.v[2]; m[3] = prior.v[3];
+                store_v2c<PAIR>(sp.v2c[s], static_cast<size_t>(e), s, prior);
                 sp.c2v[s][2 * static_cast<size_t>(e)] = 0.25;
                 sp.c2v[s][2 * static_cast<size_t>(e) + 1] = 0.25;
             }
@@ -84,17 +83,17 @@ __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPla
             for (int c = tid; c < mx + mz; c += bs) {
                 if (c < mx) {
                     int syn = (tsx[c >> 5] >> (c & 31)) & 1;
-                    check_update<MAXC>(g.side[0], 0, c, syn, sp.v2c[0], sp.c2v[0]);
+                    check_update<MAXC, PAIR>(g.side[0], 0, c, syn, sp.v2c[0], sp.c2v[0]);
                 } else {
                     int cz = c - mx;
                     int syn = (tsz[cz >> 5] >> (cz & 31)) & 1;
-                    check_update<MAXC>(g.side[1], 1, cz, syn, sp.v2c[1], sp.c2v[1]);
+                    check_update<MAXC, PAIR>(g.side[1], 1, cz, syn, sp.v2c[1], sp.c2v[1]);
                 }
             }
             __syncthreads();
             for (int v = tid; v < g.n; v += bs)
                 sp.est[v] = static_cast<uint8_t>(
-                    var_update<MAXV>(g, v, prior, damping, sp.v2c, sp.c2v, nullptr));
+                    var_update<MAXV, PAIR>(g, v, prior, damping, sp.v2c, sp.c2v, nullptr));
             __syncthreads();
             // both syndromes of est (jointbp_ets.cpp:2521-2525)
             int bad = 0;
@@ -158,13 +157,19 @@ void launch_decode_f64(const DevGraph &g, TrialPlanes tp, int count, const Noise
     if (count <= 0) return;
     size_t stride = decode_f64_scratch_bytes(g, 1) / sizeof(double);
     int grid = slots < count ? slots : count;
-    if (g.max_cdeg <= 16 && g.max_vdeg <= 4) {
-        hipLaunchKernelGGL((decode_f64_kernel<16, 4>), dim3(grid), dim3(block), 0, st, g, tp,
+    const bool small = g.max_cdeg <= 16 && g.max_vdeg <= 4, pair = damping <= 0.0;
+    if (small && pair)
+        hipLaunchKernelGGL((decode_f64_kernel<16, 4, true>), dim3(grid), dim3(block), 0, st, g, tp,
                            count, nt, max_iter, damping, scratch, stride, counter);
-    } else {
-        hipLaunchKernelGGL((decode_f64_kernel<32, 16>), dim3(grid), dim3(block), 0, st, g, tp,
+    else if (small)
+        hipLaunchKernelGGL((decode_f64_kernel<16, 4, false>), dim3(grid), dim3(block), 0, st, g, tp,
                            count, nt, max_iter, damping, scratch, stride, counter);
-    }
+    else if (pair)
+        hipLaunchKernelGGL((decode_f64_kernel<32, 16, true>), dim3(grid), dim3(block), 0, st, g, tp,
+                           count, nt, max_iter, damping, scratch, stride, counter);
+    else
+        hipLaunchKernelGGL((decode_f64_kernel<32, 16, false>), dim3(grid), dim3(block), 0, st, g, tp,
+                           count, nt, max_iter, damping, scratch, stride, counter);
 }
 
 }  // namespace jbp

@@ -40,8 +40,30 @@ __device__ __forceinline__ M4 mix4(const M4 &nw, const M4 &old, double d) {
     return out;
 }
 
+// Variable-to-check messages are read by the check update only through the
+// pair sums q0 = P(even), q1 = P(odd) of the side's Pauli bit (X side:
+// m0 + m2, m1 + m3; Z side: m0 + m1, m2 + m3).  Without damping the variable
+// update stores those two sums (PAIR: 2 doubles per row) instead of the
+// four components (4 per row, needed to mix old and new messages when
+// damping); the sums are formed in the same order either way.
+__device__ __forceinline__ void pair_sums(const double *m, int side, double &q0, double &q1) {
+    q0 = side == 0 ? m[0] + m[2] : m[0] + m[1];
+    q1 = side == 0 ? m[1] + m[3] : m[2] + m[3];
+}
+
+template <bool PAIR>
+__device__ __forceinline__ void store_v2c(double *v2c, size_t row, int side, const M4 &m) {
+    if (PAIR) {
+        double *o = v2c + 2 * row;
+        pair_sums(m.v, side, o[0], o[1]);
+    } else {
+        double *o = v2c + 4 * row;
+        o[0] = m.v[0]; o[1] = m.v[1]; o[2] = m.v[2]; o[3] = m.v[3];
+    }
+}
+
 // Check update of one check (jointbp_ets.cpp:2373-2449).
-template <int MAXC>
+template <int MAXC, bool PAIR>
 __device__ void check_update(const DevSide &sd, int side, int c, int syn, const double *v2c,
                              double *c2v) {
     const int deg = sd.chk_off[c + 1] - sd.chk_off[c];
@@ -52,9 +74,13 @@ __device__ void check_update(const DevSide &sd, int side, int c, int syn, const 
 #pragma unroll
     for (int i = 0; i < MAXC; ++i) {
         if (i < deg) {
-            const double *mv = v2c + 4 * (i * m + c);
-            q0[i] = side == 0 ? mv[0] + mv[2] : mv[0] + mv[1];
-            q1[i] = side == 0 ? mv[1] + mv[3] : mv[2] + mv[3];
+            if (PAIR) {
+                const double *mv = v2c + 2 * (i * m + c);
+                q0[i] = mv[0];
+                q1[i] = mv[1];
+            } else {
+                pair_sums(v2c + 4 * (i * m + c), side, q0[i], q1[i]);
+            }
             pe[i + 1] = pe[i] * q0[i] + po[i] * q1[i];
             po[i + 1] = pe[i] * q1[i] + po[i] * q0[i];
         }
@@ -98,7 +124,7 @@ __device__ __forceinline__ M4 c2v_msg(const double *c2v, int r, int side) {
 //
 // Messages: v2c[s] (4 doubles per edge), c2v[s] (2 per edge).  Returns the
 // hard decision; tot (optional) receives the normalised marginal.
-template <int MAXV>
+template <int MAXV, bool PAIR>
 __device__ int var_update(const DevGraph &g, int v, const M4 &prior, double damping,
                           double *const *v2c, const double *const *c2v, double *tot) {
     const M4 ident{{1.0, 1.0, 1.0, 1.0}};
@@ -140,12 +166,12 @@ __device__ int var_update(const DevGraph &g, int v, const M4 &prior, double damp
             int e = g.side[0].eslot[g.side[0].var_edge[b[0] + i]];
             M4 out = mul4(mul4(prior, mul4(pref[0][i], suf)), allz);
             normalize4(out);
-            double *slot = v2c[0] + 4 * static_cast<size_t>(e);
-            if (damping > 0.0) {
+            if (!PAIR && damping > 0.0) {
+                const double *slot = v2c[0] + 4 * static_cast<size_t>(e);
                 M4 old{{slot[0], slot[1], slot[2], slot[3]}};
                 out = mix4(out, old, damping);
             }
-            slot[0] = out.v[0]; slot[1] = out.v[1]; slot[2] = out.v[2]; slot[3] = out.v[3];
+            store_v2c<PAIR>(v2c[0], static_cast<size_t>(e), 0, out);
             suf = mul4(suf, c2v_msg(c2v[0], e, 0));
         }
     }
@@ -158,12 +184,12 @@ __device__ int var_update(const DevGraph &g, int v, const M4 &prior, double damp
             int e = g.side[1].eslot[g.side[1].var_edge[b[1] + i]];
             M4 out = mul4(px, mul4(pref[1][i], suf));
             normalize4(out);
-            double *slot = v2c[1] + 4 * static_cast<size_t>(e);
-            if (damping > 0.0) {
+            if (!PAIR && damping > 0.0) {
+                const double *slot = v2c[1] + 4 * static_cast<size_t>(e);
                 M4 old{{slot[0], slot[1], slot[2], slot[3]}};
                 out = mix4(out, old, damping);
             }
-            slot[0] = out.v[0]; slot[1] = out.v[1]; slot[2] = out.v[2]; slot[3] = out.v[3];
+            store_v2c<PAIR>(v2c[1], static_cast<size_t>(e), 1, out);
             suf = mul4(suf, c2v_msg(c2v[1], e, 1));
         }
     }

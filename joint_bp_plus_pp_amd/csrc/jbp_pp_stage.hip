@@ -77,7 +77,7 @@ constexpr int kRecInts = 4;
 // initial messages.  want_tot: write every iteration's marginals (the slot
 // then holds the last iteration's).  At the end the block copies its
 // estimate and previous estimate to out[b].
-template <int MAXC, int MAXV>
+template <int MAXC, int MAXV, bool PAIR>
 __global__ __launch_bounds__(256, 2) void pp_step_kernel(DevGraph g, TrialPlanes tp, NoiseTab nt, const int *list,
                                const int *slot_trial, int steps, int init, int want_tot,
                                double damping, char *state, size_t stride, int *rec, uint8_t *out) {
@@ -95,8 +95,7 @@ __global__ __launch_bounds__(256, 2) void pp_step_kernel(DevGraph g, TrialPlanes
     if (init) {
         for (int s = 0; s < 2; ++s)
             for (int e = tid; e < g.side[s].cap; e += bs) {
-                double *m = sp.v2c[s] + 4 * static_cast<size_t>(e);
-                m[0] = prior.v[0]; m[1] = prior.v[1]; m[2] = prior.v[2]; m[3] = prior.v[3];
+                store_v2c<PAIR>(sp.v2c[s], static_cast<size_t>(e), s, prior);
                 sp.c2v[s][2 * static_cast<size_t>(e)] = 0.25;
                 sp.c2v[s][2 * static_cast<size_t>(e) + 1] = 0.25;
             }
@@ -107,10 +106,11 @@ __global__ __launch_bounds__(256, 2) void pp_step_kernel(DevGraph g, TrialPlanes
         if (tid == 0) s_cnt[0] = s_cnt[1] = 0;
         for (int c = tid; c < mx + mz; c += bs) {
             if (c < mx) {
-                check_update<MAXC>(g.side[0], 0, c, (tsx[c >> 5] >> (c & 31)) & 1, sp.v2c[0], sp.c2v[0]);
+                check_update<MAXC, PAIR>(g.side[0], 0, c, (tsx[c >> 5] >> (c & 31)) & 1, sp.v2c[0],
+                                         sp.c2v[0]);
             } else {
                 const int cz = c - mx;
-                check_update<MAXC>(g.side[1], 1, cz, (tsz[cz >> 5] >> (cz & 31)) & 1, sp.v2c[1],
+                check_update<MAXC, PAIR>(g.side[1], 1, cz, (tsz[cz >> 5] >> (cz & 31)) & 1, sp.v2c[1],
                                    sp.c2v[1]);
             }
         }
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(256, 2) void pp_step_kernel(DevGraph g, TrialPlanes
         double *tot = want_tot ? sp.tot : nullptr;
         int changed1 = 0, changed2 = 0;
         for (int v = tid; v < n; v += bs) {
-            const int best = var_update<MAXV>(g, v, prior, damping, sp.v2c, sp.c2v, tot);
+            const int best = var_update<MAXV, PAIR>(g, v, prior, damping, sp.v2c, sp.c2v, tot);
             const int e1 = sp.est[v], e2 = sp.prev[v];
             changed1 |= best != e1;
             changed2 |= best != e2;
@@ -168,15 +168,20 @@ void launch_step(const DevGraph &g, const TrialPlanes &tp, const NoiseTab &nt, c
                  int count, const int *slot_trial, int steps, bool init, bool want_tot,
                  double damping, char *state, size_t stride, int *rec, uint8_t *out,
                  hipStream_t st) {
-    if (g.max_cdeg <= 16 && g.max_vdeg <= 4) {
-        hipLaunchKernelGGL((pp_step_kernel<16, 4>), dim3(count), dim3(256), 0, st, g, tp, nt, list,
-                           slot_trial, steps, init ? 1 : 0, want_tot ? 1 : 0, damping, state,
-                           stride, rec, out);
-    } else {
-        hipLaunchKernelGGL((pp_step_kernel<32, 16>), dim3(count), dim3(256), 0, st, g, tp, nt,
-                           list, slot_trial, steps, init ? 1 : 0, want_tot ? 1 : 0, damping, state,
-                           stride, rec, out);
-    }
+    const bool small = g.max_cdeg <= 16 && g.max_vdeg <= 4, pair = damping <= 0.0;
+    const int ini = init ? 1 : 0, wt = want_tot ? 1 : 0;
+    if (small && pair)
+        hipLaunchKernelGGL((pp_step_kernel<16, 4, true>), dim3(count), dim3(256), 0, st, g, tp, nt,
+                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
+    else if (small)
+        hipLaunchKernelGGL((pp_step_kernel<16, 4, false>), dim3(count), dim3(256), 0, st, g, tp, nt,
+                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
+    else if (pair)
+        hipLaunchKernelGGL((pp_step_kernel<32, 16, true>), dim3(count), dim3(256), 0, st, g, tp, nt,
+                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
+    else
+        hipLaunchKernelGGL((pp_step_kernel<32, 16, false>), dim3(count), dim3(256), 0, st, g, tp, nt,
+                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
 }
 
 // dst[k][0..len) = src[idx[k] * stride + off + 0..len), in 8-byte units
