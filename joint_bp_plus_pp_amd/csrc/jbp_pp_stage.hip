@@ -388,8 +388,9 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             HIPCHK(hipStreamSynchronize(st));
             const int m = s == 0 ? mx : mz;
             uint8_t *dst = s == 0 ? sx.data() : sz.data();
-            for (int i = 0; i < K; ++i)
+            pool.run(K, [&](int i) {
                 unpack_bits(&w[static_cast<size_t>(i) * wm], m, dst + static_cast<size_t>(i) * m);
+            });
         }
         std::vector<pp::Trial> tr(K);
         std::vector<std::vector<uint8_t>> est(K, std::vector<uint8_t>(n, 0));
@@ -566,17 +567,17 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
         uint32_t *planes = static_cast<uint32_t *>(pb.h_stage);
         int *it = reinterpret_cast<int *>(planes + nplane);
         uint8_t *okb = reinterpret_cast<uint8_t *>(it + K);
-        std::fill(planes, planes + nplane, 0u);
-        for (int i = 0; i < K; ++i) {
+        pool.run(K, [&](int i) {
             uint32_t *px = planes + static_cast<size_t>(i) * 2 * g.wn, *pz = px + g.wn;
+            std::fill(px, px + 2 * g.wn, 0u);
             for (int v = 0; v < n; ++v) {
                 px[v >> 5] |= static_cast<uint32_t>(est[i][v] & 1) << (v & 31);
                 pz[v >> 5] |= static_cast<uint32_t>((est[i][v] >> 1) & 1) << (v & 31);
             }
             it[i] = o[i].iterations;
             okb[i] = o[i].ok ? 1 : 0;
-            outs[b0 + i] = o[i];
-        }
+        });
+        for (int i = 0; i < K; ++i) outs[b0 + i] = o[i];
         {
             char *stage = static_cast<char *>(pb.stage);
             uint32_t *d_planes = reinterpret_cast<uint32_t *>(stage);
