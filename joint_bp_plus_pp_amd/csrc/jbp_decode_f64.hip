@@ -157,8 +157,17 @@ void launch_decode_f64(const DevGraph &g, TrialPlanes tp, int count, const Noise
     if (count <= 0) return;
     size_t stride = decode_f64_scratch_bytes(g, 1) / sizeof(double);
     int grid = slots < count ? slots : count;
+    // recursion arrays sized to the code (registers): degree 12 / 3 for the
+    // permutation-block codes, 16 / 4, or 32 / 16
+    const bool exact12 = g.max_cdeg <= 12 && g.max_vdeg <= 3;
     const bool small = g.max_cdeg <= 16 && g.max_vdeg <= 4, pair = damping <= 0.0;
-    if (small && pair)
+    if (exact12 && pair)
+        hipLaunchKernelGGL((decode_f64_kernel<12, 3, true>), dim3(grid), dim3(block), 0, st, g, tp,
+                           count, nt, max_iter, damping, scratch, stride, counter);
+    else if (exact12)
+        hipLaunchKernelGGL((decode_f64_kernel<12, 3, false>), dim3(grid), dim3(block), 0, st, g, tp,
+                           count, nt, max_iter, damping, scratch, stride, counter);
+    else if (small && pair)
         hipLaunchKernelGGL((decode_f64_kernel<16, 4, true>), dim3(grid), dim3(block), 0, st, g, tp,
                            count, nt, max_iter, damping, scratch, stride, counter);
     else if (small)

@@ -168,9 +168,16 @@ void launch_step(const DevGraph &g, const TrialPlanes &tp, const NoiseTab &nt, c
                  int count, const int *slot_trial, int steps, bool init, bool want_tot,
                  double damping, char *state, size_t stride, int *rec, uint8_t *out,
                  hipStream_t st) {
+    const bool exact12 = g.max_cdeg <= 12 && g.max_vdeg <= 3;
     const bool small = g.max_cdeg <= 16 && g.max_vdeg <= 4, pair = damping <= 0.0;
     const int ini = init ? 1 : 0, wt = want_tot ? 1 : 0;
-    if (small && pair)
+    if (exact12 && pair)
+        hipLaunchKernelGGL((pp_step_kernel<12, 3, true>), dim3(count), dim3(256), 0, st, g, tp, nt,
+                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
+    else if (exact12)
+        hipLaunchKernelGGL((pp_step_kernel<12, 3, false>), dim3(count), dim3(256), 0, st, g, tp, nt,
+                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
+    else if (small && pair)
         hipLaunchKernelGGL((pp_step_kernel<16, 4, true>), dim3(count), dim3(256), 0, st, g, tp, nt,
                            list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
     else if (small)
