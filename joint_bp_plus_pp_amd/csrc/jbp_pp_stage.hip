@@ -363,6 +363,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
     prm.max_iter = max_iter;
     prm.hist_window = hist_window;
     outs.assign(trials.size(), PPOutcome());
+    if (trials.empty()) return JBP_OK;  // the usual case with long decodes: nothing to start
     Pool pool(pool_threads());
     hipStream_t st = ctx->stream;
     const size_t stride = step_slot_bytes(g.side[0].cap, g.side[1].cap, n);
