@@ -191,6 +191,10 @@ namespace jbp {
 void free_chunk(jbp_ctx *ctx) {
     for (void *p : ctx->chunk_allocs) (void)hipFree(p);
     ctx->chunk_allocs.clear();
+    if (ctx->h_iters) (void)hipHostFree(ctx->h_iters);
+    if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
+    ctx->h_iters = nullptr;
+    ctx->h_flags = nullptr;
     ctx->chunk_cap = 0;
     ctx->tp = TrialPlanes();
     ctx->d_keys = nullptr;
@@ -216,9 +220,13 @@ int ensure_chunk(jbp_ctx *ctx, int count, char *err, size_t errlen) {
     if ((rc = alloc(c * g.wm[0] * 4, &p))) return rc; ctx->tp.syn_x = static_cast<uint32_t *>(p);
     if ((rc = alloc(c * g.wm[1] * 4, &p))) return rc; ctx->tp.syn_z = static_cast<uint32_t *>(p);
     if ((rc = alloc(c * 4, &p))) return rc; ctx->tp.iters = static_cast<int *>(p);
-    if ((rc = alloc(c, &p))) return rc; ctx->tp.syn_ok = static_cast<uint8_t *>(p);
-    if ((rc = alloc(c, &p))) return rc; ctx->tp.success = static_cast<uint8_t *>(p);
-    if ((rc = alloc(c, &p))) return rc; ctx->tp.exact = static_cast<uint8_t *>(p);
+    // the three outcome flags in one block, so one copy brings them back
+    if ((rc = alloc(3 * c, &p))) return rc;
+    ctx->tp.syn_ok = static_cast<uint8_t *>(p);
+    ctx->tp.success = ctx->tp.syn_ok + c;
+    ctx->tp.exact = ctx->tp.syn_ok + 2 * c;
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ctx->h_iters), c * 4));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ctx->h_flags), 3 * c));
     if ((rc = alloc(c * 4, &p))) return rc; ctx->d_keys = static_cast<int *>(p);
     if ((rc = alloc(c * 4, &p))) return rc; ctx->d_order = static_cast<int *>(p);
     ctx->chunk_cap = count;
@@ -406,7 +414,8 @@ int run_gen(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmu
 // Decode `count` trials whose syndrome planes are already in ctx->tp.
 int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmup, int interval,
                double damping, int precision, char *err, size_t errlen,
-               const LearnedInfer *learned, int engine, int slots, bool use_graph) {
+               const LearnedInfer *learned, int engine, int slots, bool use_graph,
+               bool defer_timing) {
     if (precision != JBP_FP32 && precision != JBP_FP64) {
         set_err(err, errlen, "precision must be 32 or 64");
         return JBP_ERR_ARG;
@@ -417,7 +426,8 @@ int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int wa
     }
     launch_fill_i32(ctx->d_counter, 0, 1, ctx->stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    hipEvent_t e0 = defer_timing ? ctx->evt[2] : ctx->ev0, e1 = defer_timing ? ctx->evt[3] : ctx->ev1;
+    HIPCHK(hipEventRecord(e0, ctx->stream));
     if (precision == JBP_FP32) {
         if (damping > 0.0) {
             set_err(err, errlen, "FP32 decoder runs damping 0 only; use precision 64");
@@ -447,7 +457,8 @@ int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int wa
                           ctx->f64_block, ctx->f64_scratch, ctx->d_counter, ctx->stream);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    if (defer_timing) return JBP_OK;
     HIPCHK(hipEventSynchronize(ctx->ev1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -680,8 +691,11 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
     const int chunk = static_cast<int>(std::min<int64_t>(std::max<int64_t>(total, 1), kDefaultChunk));
     int rc = ensure_chunk(ctx, chunk, err, errlen);
     if (rc) return rc;
-    std::vector<int> iters(chunk);
-    std::vector<uint8_t> ok(chunk), succ(chunk), exact(chunk);
+    // outcome copies land in pinned memory: iterations, and the three flag
+    // arrays (one block on the device) in one copy
+    const int cap = ctx->chunk_cap;
+    int *iters = ctx->h_iters;
+    uint8_t *ok = ctx->h_flags, *succ = ctx->h_flags + cap, *exact = ctx->h_flags + 2 * cap;
     double dev_ms = 0.0;
     bool stop = false;
     for (int64_t done = 0; done < total && !stop;) {
@@ -690,27 +704,25 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
         // trial index idx uses channel entry idx mod n_p
         NoiseTab nt = ctx->noise;
         nt.phase = static_cast<int>(((first % np) + np) % np);
-        HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+        // (no host wait between the launches of a chunk: the sample and
+        // decode timings are read after the chunk's one synchronisation)
+        HIPCHK(hipEventRecord(ctx->evt[0], ctx->stream));
         launch_sample(g, ctx->tp, cfg->base_seed, first, cnt, nt, ctx->stream);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-        HIPCHK(hipEventSynchronize(ctx->ev1));
-        float sms = 0.f;
-        HIPCHK(hipEventElapsedTime(&sms, ctx->ev0, ctx->ev1));
-        ctx->last_ms[0] = sms;
-        ctx->launches[0]++;
+        HIPCHK(hipEventRecord(ctx->evt[1], ctx->stream));
         if (cfg->schedule == 0) {
             launch_order_by_weight(g, ctx->tp, cnt, ctx->d_keys, ctx->d_order, ctx->stream);
             HIPCHK(hipGetLastError());
             ctx->tp.order = ctx->d_order;
         }
-        if (!pp_on || pp_h > 0) {
+        const bool decoded = !pp_on || pp_h > 0;
+        if (decoded) {
             rc = run_decode(ctx, cnt, nt, pp_on ? pp_h : cfg->max_iter, cfg->check_warmup,
                             cfg->check_interval, cfg->damping, cfg->precision, err, errlen, learned,
-                            cfg->engine, cfg->precision == JBP_FP32 ? cfg->slots : 0, cfg->graph != 0);
+                            cfg->engine, cfg->precision == JBP_FP32 ? cfg->slots : 0, cfg->graph != 0,
+                            true);
             ctx->tp.order = nullptr;
             if (rc) return rc;
-            dev_ms += ctx->last_ms[1];
         }
         ctx->tp.order = nullptr;
         // post-processing: the trials still unconverged after pp_h iterations
@@ -719,7 +731,7 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
         std::vector<int> pp_of;
         if (pp_on) {
             if (pp_h > 0) {
-                HIPCHK(hipMemcpyAsync(ok.data(), ctx->tp.syn_ok, cnt, hipMemcpyDeviceToHost, ctx->stream));
+                HIPCHK(hipMemcpyAsync(ok, ctx->tp.syn_ok, cnt, hipMemcpyDeviceToHost, ctx->stream));
                 HIPCHK(hipStreamSynchronize(ctx->stream));
             }
             for (int i = 0; i < cnt; ++i)
@@ -747,11 +759,23 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
             launch_logical(g, DevLogicals(), DevLogicals(), ctx->tp, cnt, ctx->stream);
             HIPCHK(hipGetLastError());
         }
-        HIPCHK(hipMemcpyAsync(iters.data(), ctx->tp.iters, cnt * 4, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipMemcpyAsync(ok.data(), ctx->tp.syn_ok, cnt, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipMemcpyAsync(succ.data(), ctx->tp.success, cnt, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipMemcpyAsync(exact.data(), ctx->tp.exact, cnt, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(iters, ctx->tp.iters, cnt * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ok, ctx->tp.syn_ok, 3 * static_cast<size_t>(cap), hipMemcpyDeviceToHost,
+                              ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
+        {
+            float sms = 0.f;
+            HIPCHK(hipEventElapsedTime(&sms, ctx->evt[0], ctx->evt[1]));
+            ctx->last_ms[0] = sms;
+            ctx->launches[0]++;
+            if (decoded) {
+                float dms = 0.f;
+                HIPCHK(hipEventElapsedTime(&dms, ctx->evt[2], ctx->evt[3]));
+                ctx->last_ms[1] = dms;
+                ctx->launches[1]++;
+                dev_ms += dms;
+            }
+        }
         const bool judged = ctx->have_logicals || ctx->have_basis;
         if (judged) {
             float rms = 0.f;
@@ -1009,7 +1033,9 @@ int jbp_create(const jbp_graph_desc *g, int device_id, jbp_ctx **out, char *err,
     }
     ctx->cus = prop.multiProcessorCount;
     if (hipStreamCreate(&ctx->stream) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
-        hipEventCreate(&ctx->ev1) != hipSuccess) {
+        hipEventCreate(&ctx->ev1) != hipSuccess || hipEventCreate(&ctx->evt[0]) != hipSuccess ||
+        hipEventCreate(&ctx->evt[1]) != hipSuccess || hipEventCreate(&ctx->evt[2]) != hipSuccess ||
+        hipEventCreate(&ctx->evt[3]) != hipSuccess) {
         set_err(err, errlen, "stream/event creation failed");
         return fail(JBP_ERR_DEVICE);
     }
@@ -1128,6 +1154,8 @@ void jbp_destroy(jbp_ctx *ctx) {
     if (ctx->d_ints) (void)hipFree(ctx->d_ints);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (hipEvent_t e : ctx->evt)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -112,6 +112,9 @@ struct jbp_ctx {
     int chunk_cap = 0;
     jbp::TrialPlanes tp;
     std::vector<void *> chunk_allocs;
+    int *h_iters = nullptr;     // pinned [chunk_cap]: outcome copies of a chunk
+    uint8_t *h_flags = nullptr; // pinned [3][chunk_cap]: syn_ok, success, exact
+    hipEvent_t evt[4] = {nullptr, nullptr, nullptr, nullptr};  // sample / decode start, end
     double *f64_scratch = nullptr;
     int f64_slots = 0;
     int f64_block = 256;
@@ -175,11 +178,13 @@ int set_noise(jbp_ctx *ctx, const Prior *priors, const double *p, int n, char *e
 void free_noise(jbp_ctx *ctx);
 // Decode `count` trials whose syndrome planes are already in ctx->tp, slot i
 // with channel entry nt.entry(i); records the decoder launch time (HIP
-// events) in ctx->last_ms[1].
+// events) in ctx->last_ms[1].  defer_timing: record the events
+// ctx->evt[2], evt[3] around the launch and return without waiting; the
+// caller reads them after its own synchronisation.
 int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmup, int interval,
                double damping, int precision, char *err, size_t errlen,
                const LearnedInfer *learned = nullptr, int engine = JBP_ENGINE_AUTO,
-               int slots = 0, bool use_graph = false);
+               int slots = 0, bool use_graph = false, bool defer_timing = false);
 void free_gen(jbp_ctx *ctx);
 void free_logicals(jbp_ctx *ctx);
 void free_pipe(jbp_ctx *ctx);
