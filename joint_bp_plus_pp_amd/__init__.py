@@ -513,7 +513,8 @@ class Decoder:
                  precision=FP32, check_warmup=0, check_interval=1, min_fails=0, slots=0, block=0,
                  per_trial=True, want_err=False, want_est=False, engine=0, schedule=0,
                  graph=False, pp=False, flip_hist=5, _model=None):
-        """The --simulate --no-pp inner loop on the device.  Returns (stats,
+        """The --simulate inner loop on the device (post-processing off unless
+        pp=True, i.e. the reference's --no-pp by default).  Returns (stats,
         per-trial dict or None).  p may be a sequence: trial index idx then
         uses p[idx % len(p)] (interleaved mixed-noise run).  engine selects
         the FP32 decoder (ENGINE_AUTO, ENGINE_LDS, ENGINE_HBM); slots is the
