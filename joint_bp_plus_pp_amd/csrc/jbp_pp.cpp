@@ -656,9 +656,8 @@ bool attempt_end(Trial &tr, Attempt &a, int t, std::vector<uint8_t> &est) {
     return true;
 }
 
-PostOut post_decode(const Graph &g, const Tables &t, const Params &prm, const Trial &tr,
-                    std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz,
-                    const double *llr_x, const double *llr_z) {
+PostOut post_begin(const Graph &g, const Tables &t, const Params &prm, const Trial &tr,
+                   std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz) {
     PostOut o;
     const EtsFix ex = ets_fix(g, t, 0, sx, est);
     const EtsFix ez = ets_fix(g, t, 1, sz, est);
@@ -667,11 +666,25 @@ PostOut post_decode(const Graph &g, const Tables &t, const Params &prm, const Tr
     o.used12 = ex.size == 12 || ez.size == 12;
     if (o.ets_applied) o.ets_ok = o.ok = both_match(g, est, sx, sz);
     if (!o.ok && flip_pass(g, prm, tr, est, sx, sz)) o.flip_ok = o.ok = both_match(g, est, sx, sz);
-    if (!o.ok && std::max(unsatisfied(g, 0, est.data(), sx), unsatisfied(g, 1, est.data(), sz)) <= kUsLimit) {
-        const bool ox = osd_fix(g, 0, llr_x, sx, est);
-        const bool oz = osd_fix(g, 1, llr_z, sz, est);
-        if (ox || oz) o.osd_ok = o.ok = both_match(g, est, sx, sz);
-    }
+    if (!o.ok)
+        o.want_osd = std::max(unsatisfied(g, 0, est.data(), sx), unsatisfied(g, 1, est.data(), sz)) <=
+                     kUsLimit;
+    return o;
+}
+
+void post_osd(const Graph &g, PostOut &o, std::vector<uint8_t> &est, const uint8_t *sx,
+              const uint8_t *sz, const double *llr_x, const double *llr_z) {
+    if (!o.want_osd) return;
+    const bool ox = osd_fix(g, 0, llr_x, sx, est);
+    const bool oz = osd_fix(g, 1, llr_z, sz, est);
+    if (ox || oz) o.osd_ok = o.ok = both_match(g, est, sx, sz);
+}
+
+PostOut post_decode(const Graph &g, const Tables &t, const Params &prm, const Trial &tr,
+                    std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz,
+                    const double *llr_x, const double *llr_z) {
+    PostOut o = post_begin(g, t, prm, tr, est, sx, sz);
+    post_osd(g, o, est, sx, sz, llr_x, llr_z);
     return o;
 }
 

@@ -169,10 +169,17 @@ struct PostOut {
     bool ok = false;
     bool ets_applied = false, ets_ok = false, flip_ok = false, osd_ok = false;
     bool used6 = false, used12 = false;
+    bool want_osd = false;  // (post_begin) OSD is to run: it needs |LLR|
 };
 PostOut post_decode(const Graph &g, const Tables &t, const Params &prm, const Trial &tr,
                     std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz,
                     const double *llr_x, const double *llr_z);
+// The same pass in two parts, so that |LLR| are fetched only when OSD runs:
+// post_begin does ETS and the flip sets and sets want_osd; post_osd finishes.
+PostOut post_begin(const Graph &g, const Tables &t, const Params &prm, const Trial &tr,
+                   std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz);
+void post_osd(const Graph &g, PostOut &o, std::vector<uint8_t> &est, const uint8_t *sx,
+              const uint8_t *sz, const double *llr_x, const double *llr_z);
 
 }  // namespace pp
 }  // namespace jbp

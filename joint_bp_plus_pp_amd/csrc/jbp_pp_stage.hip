@@ -521,9 +521,21 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                 }
                 if (t + 1 == max_iter) {
                     at_end[i] = 1;
-                    // the post-decode pass needs |LLR| only when it has to run
-                    // itself (see below)
-                    need_llr[i] |= !attempted[i];
+                    if (!attempted[i]) {
+                        // the post-decode pass runs itself (no attempt to reuse):
+                        // ETS and flip sets now, |LLR| only if OSD follows
+                        tr[i].iterations = max_iter;
+                        o[i].post_run = true;
+                        o[i].post = pp::post_begin(hg, tabs, prm, tr[i], est[i], &sx[static_cast<size_t>(i) * mx],
+                                                   &sz[static_cast<size_t>(i) * mz]);
+                        need_llr[i] = o[i].post.want_osd;
+                        if (!need_llr[i]) {
+                            tr[i].done = true;
+                            finish(i);
+                            o[i].ok = o[i].post.ok;
+                            return;
+                        }
+                    }
                 }
                 if (at_end[i] && !need_llr[i]) end_trial(i);
             });
@@ -560,9 +572,11 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                     } else {
                         tr[i].done = true;
                         tr[i].iterations = max_iter;
+                        const pp::PostOut begun = o[i].post;
                         finish(i);
                         o[i].post_run = true;
-                        o[i].post = pp::post_decode(hg, tabs, prm, tr[i], est[i], tsx, tsz, lx, lz);
+                        o[i].post = begun;
+                        pp::post_osd(hg, o[i].post, est[i], tsx, tsz, lx, lz);
                         o[i].ok = o[i].post.ok;
                     }
                 }
