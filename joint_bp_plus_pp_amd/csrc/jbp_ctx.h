@@ -150,7 +150,10 @@ struct jbp_ctx {
     // post-processing: host graph, trapping-set tables, device buffers
     jbp::pp::Graph pp_graph;
     jbp::pp::Tables pp_tables;
-    jbp::PPBuf ppb;
+    // two buffer sets and a second stream: one batch's host repairs overlap
+    // the next batch's first device phase
+    jbp::PPBuf ppb[2];
+    hipStream_t pp_stream = nullptr;
     double last_ms[3] = {0, 0, 0};
     int64_t launches[3] = {0, 0, 0};
 };

@@ -304,12 +304,22 @@ int pool_threads() {
 constexpr int kPPSlots = 1024;
 constexpr size_t kPPStateBytes = size_t(8) << 30;
 
-int ensure_pp_buffers(jbp_ctx *ctx, int slots, int rec_per_slot, char *err, size_t errlen) {
-    PPBuf &pb = ctx->ppb;
+void free_pp_buf(PPBuf &pb) {
+    for (void *p : {static_cast<void *>(pb.state), static_cast<void *>(pb.list),
+                    static_cast<void *>(pb.slot_trial), static_cast<void *>(pb.rec),
+                    static_cast<void *>(pb.out), static_cast<void *>(pb.idx), pb.stage})
+        if (p) (void)hipFree(p);
+    for (void *p : {static_cast<void *>(pb.h_rec), static_cast<void *>(pb.h_out),
+                    static_cast<void *>(pb.h_tot), pb.h_stage})
+        if (p) (void)hipHostFree(p);
+    pb = PPBuf();
+}
+
+int ensure_pp_buffers(jbp_ctx *ctx, PPBuf &pb, int slots, int rec_per_slot, char *err, size_t errlen) {
     const DevGraph &g = ctx->dg;
     const size_t stride = step_slot_bytes(g.side[0].cap, g.side[1].cap, g.n);
     if (pb.cap >= slots && pb.rec_cap >= rec_per_slot && pb.stride == stride) return JBP_OK;
-    free_pp(ctx);
+    free_pp_buf(pb);
     pb.cap = slots;
     pb.rec_cap = std::max(rec_per_slot, 1);
     pb.stride = stride;
@@ -341,16 +351,28 @@ void unpack_bits(const uint32_t *w, int count, uint8_t *out) {
 }  // namespace
 
 void free_pp(jbp_ctx *ctx) {
-    PPBuf &pb = ctx->ppb;
-    for (void *p : {static_cast<void *>(pb.state), static_cast<void *>(pb.list),
-                    static_cast<void *>(pb.slot_trial), static_cast<void *>(pb.rec),
-                    static_cast<void *>(pb.out), static_cast<void *>(pb.idx), pb.stage})
-        if (p) (void)hipFree(p);
-    for (void *p : {static_cast<void *>(pb.h_rec), static_cast<void *>(pb.h_out),
-                    static_cast<void *>(pb.h_tot), pb.h_stage})
-        if (p) (void)hipHostFree(p);
-    pb = PPBuf();
+    for (PPBuf &pb : ctx->ppb) free_pp_buf(pb);
+    if (ctx->pp_stream) (void)hipStreamDestroy(ctx->pp_stream);
+    ctx->pp_stream = nullptr;
 }
+
+// One batch of the stage: its trials' host state and the buffer set and
+// stream it uses.
+struct PPBatch {
+    size_t b0 = 0;
+    int K = 0;
+    PPBuf *pb = nullptr;
+    hipStream_t st = nullptr;
+    bool phase_a = false;  // phase A launched, results not yet read
+    std::vector<uint8_t> sx, sz;
+    std::vector<pp::Trial> tr;
+    std::vector<std::vector<uint8_t>> est;
+    std::vector<PPOutcome> o;
+    std::vector<int> active;
+    std::vector<pp::Attempt> att;
+    std::vector<uint8_t> need_llr, at_end, attempted;
+    std::vector<double> llr;
+};
 
 int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int hist_window,
            const std::vector<int> &trials, std::vector<PPOutcome> &outs, char *err, size_t errlen) {
@@ -364,20 +386,32 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
     prm.hist_window = hist_window;
     outs.assign(trials.size(), PPOutcome());
     if (trials.empty()) return JBP_OK;  // the usual case with long decodes: nothing to start
+    if (!ctx->pp_stream) HIPCHK(hipStreamCreate(&ctx->pp_stream));
     Pool pool(pool_threads());
-    hipStream_t st = ctx->stream;
     const size_t stride = step_slot_bytes(g.side[0].cap, g.side[1].cap, n);
     const size_t batch = std::max<size_t>(1, std::min<size_t>(kPPSlots, kPPStateBytes / stride));
-    for (size_t b0 = 0; b0 < trials.size(); b0 += batch) {
-        const int K = static_cast<int>(std::min<size_t>(batch, trials.size() - b0));
-        int rc = ensure_pp_buffers(ctx, K, std::max(H, 1), err, errlen);
+    const size_t nbatch = (trials.size() + batch - 1) / batch;
+    hipStream_t streams[2] = {ctx->stream, ctx->pp_stream};
+
+    // Syndromes to bytes, host state, and (H > 0) phase A launched without
+    // waiting for it.
+    auto start = [&](PPBatch &B, size_t bi) -> int {
+        B = PPBatch();
+        B.b0 = bi * batch;
+        B.K = static_cast<int>(std::min<size_t>(batch, trials.size() - B.b0));
+        B.pb = &ctx->ppb[bi & 1];
+        B.st = streams[bi & 1];
+        // the write-back of the batch that used this set before
+        HIPCHK(hipStreamSynchronize(B.st));
+        const int K = B.K;
+        int rc = ensure_pp_buffers(ctx, *B.pb, K, std::max(H, 1), err, errlen);
         if (rc) return rc;
-        PPBuf &pb = ctx->ppb;
-        // syndromes of the batch, gathered on the device and unpacked to bytes
-        std::vector<int> slot_trial(trials.begin() + b0, trials.begin() + b0 + K);
-        HIPCHK(hipMemcpyAsync(pb.slot_trial, slot_trial.data(), sizeof(int) * K,
-                              hipMemcpyHostToDevice, st));
-        std::vector<uint8_t> sx(static_cast<size_t>(K) * mx), sz(static_cast<size_t>(K) * mz);
+        PPBuf &pb = *B.pb;
+        hipStream_t st = B.st;
+        std::vector<int> slot_trial(trials.begin() + B.b0, trials.begin() + B.b0 + K);
+        HIPCHK(hipMemcpyAsync(pb.slot_trial, slot_trial.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
+        B.sx.resize(static_cast<size_t>(K) * mx);
+        B.sz.resize(static_cast<size_t>(K) * mz);
         for (int s = 0; s < 2; ++s) {
             const int wm = g.wm[s];
             uint32_t *w = static_cast<uint32_t *>(pb.h_stage);
@@ -385,8 +419,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             if (wm % 2 == 0) {
                 launch_gather(src, wm / 2, 0, wm / 2, pb.slot_trial, K, pb.stage, st);
                 HIPCHK(hipGetLastError());
-                HIPCHK(hipMemcpyAsync(w, pb.stage, 4 * static_cast<size_t>(K) * wm,
-                                      hipMemcpyDeviceToHost, st));
+                HIPCHK(hipMemcpyAsync(w, pb.stage, 4 * static_cast<size_t>(K) * wm, hipMemcpyDeviceToHost, st));
             } else {
                 for (int i = 0; i < K; ++i)
                     HIPCHK(hipMemcpyAsync(w + static_cast<size_t>(i) * wm,
@@ -395,16 +428,57 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             }
             HIPCHK(hipStreamSynchronize(st));
             const int m = s == 0 ? mx : mz;
-            uint8_t *dst = s == 0 ? sx.data() : sz.data();
+            uint8_t *dst = s == 0 ? B.sx.data() : B.sz.data();
             pool.run(K, [&](int i) {
                 unpack_bits(&w[static_cast<size_t>(i) * wm], m, dst + static_cast<size_t>(i) * m);
             });
         }
-        std::vector<pp::Trial> tr(K);
-        std::vector<std::vector<uint8_t>> est(K, std::vector<uint8_t>(n, 0));
-        std::vector<PPOutcome> o(K);
-        std::vector<int> active(K);
-        for (int i = 0; i < K; ++i) active[i] = i;
+        B.tr.assign(K, pp::Trial());
+        B.est.assign(K, std::vector<uint8_t>(n, 0));
+        B.o.assign(K, PPOutcome());
+        B.active.resize(K);
+        for (int i = 0; i < K; ++i) B.active[i] = i;
+        B.att.assign(K, pp::Attempt());
+        B.need_llr.assign(K, 0);
+        B.at_end.assign(K, 0);
+        B.attempted.assign(K, 0);
+        if (H > 0 && max_iter > 0) {
+            HIPCHK(hipMemcpyAsync(pb.list, B.active.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
+            launch_step(g, ctx->tp, nt, pb.list, K, pb.slot_trial, H, true, false, damping, pb.state,
+                        pb.stride, pb.rec, pb.out, st);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(pb.h_rec, pb.rec, sizeof(int) * static_cast<size_t>(K) * H * kRecInts,
+                                  hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(pb.h_out, pb.out, static_cast<size_t>(K) * 2 * n, hipMemcpyDeviceToHost, st));
+            B.phase_a = true;
+        }
+        return JBP_OK;
+    };
+
+    // Phase A's records, phase B, the repairs and the write-back.  kick()
+    // starts the next batch; it is called once this batch has no kernel
+    // left but its write-back, so the next batch's phase A (which fills
+    // the device) runs during this batch's last host repairs instead of
+    // holding this batch's kernels back.
+    auto finish_batch = [&](PPBatch &B, const std::function<int()> &kick) -> int {
+        bool kicked = false;
+        auto kick_once = [&]() -> int {
+            if (kicked) return JBP_OK;
+            kicked = true;
+            return kick();
+        };
+        const int K = B.K;
+        PPBuf &pb = *B.pb;
+        hipStream_t st = B.st;
+        auto &tr = B.tr;
+        auto &est = B.est;
+        auto &o = B.o;
+        auto &active = B.active;
+        auto &att = B.att;
+        auto &need_llr = B.need_llr;
+        auto &at_end = B.at_end;
+        auto &attempted = B.attempted;
+        const uint8_t *sx = B.sx.data(), *sz = B.sz.data();
         auto finish = [&](int i) {
             o[i].bp_ok = tr[i].bp_ok;
             o[i].ok = tr[i].ok;
@@ -416,10 +490,8 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             o[i].used6 = tr[i].used6;
             o[i].used12 = tr[i].used12;
         };
-        std::vector<pp::Attempt> att(K);
-        std::vector<uint8_t> need_llr(K), at_end(K), attempted(K);
-        // Trial i ends unconverged at max_iter after an in-loop attempt at that
-        // same iteration failed.  The reference's post-decode pass then
+        // Trial i ends unconverged at max_iter after an in-loop attempt at
+        // that same iteration failed.  The reference's post-decode pass then
         // repeats exactly that attempt (ETS, the same flip sets, OSD with the
         // same |LLR|) on the decoder's estimate, in place: it fails the same
         // way and leaves the estimate the attempt built.  Take that result
@@ -441,23 +513,18 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             pool.run(K, [&](int i) {
                 o[i].iterations = max_iter < 0 ? 0 : max_iter;
                 o[i].post_run = true;
-                o[i].post = pp::post_decode(hg, tabs, prm, tr[i], est[i], &sx[static_cast<size_t>(i) * mx],
-                                            &sz[static_cast<size_t>(i) * mz], nullptr, nullptr);
+                o[i].post = pp::post_decode(hg, tabs, prm, tr[i], est[i], sx + static_cast<size_t>(i) * mx,
+                                            sz + static_cast<size_t>(i) * mz, nullptr, nullptr);
                 o[i].ok = o[i].post.ok;
             });
             active.clear();
         }
+        int rc = JBP_OK;
         int *rec = pb.h_rec;
         uint8_t *out = pb.h_out;
-        if (H > 0 && !active.empty()) {
-            HIPCHK(hipMemcpyAsync(pb.list, active.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
-            launch_step(g, ctx->tp, nt, pb.list, K, pb.slot_trial, H, true, false, damping, pb.state,
-                        pb.stride, pb.rec, pb.out, st);
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(rec, pb.rec, sizeof(int) * static_cast<size_t>(K) * H * kRecInts,
-                                  hipMemcpyDeviceToHost, st));
-            HIPCHK(hipMemcpyAsync(out, pb.out, static_cast<size_t>(K) * 2 * n, hipMemcpyDeviceToHost, st));
+        if (B.phase_a) {
             HIPCHK(hipStreamSynchronize(st));
+            B.phase_a = false;
             pool.run(K, [&](int i) {
                 for (int t = 0; t < H; ++t) {
                     const int *r = &rec[(static_cast<size_t>(i) * H + t) * kRecInts];
@@ -479,7 +546,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                          active.end());
         }
         double *tot = pb.h_tot;
-        std::vector<double> llr;
+        auto &llr = B.llr;
         for (int t = std::max(H, 0); t < max_iter && !active.empty(); ++t) {
             const int A = static_cast<int>(active.size());
             HIPCHK(hipMemcpyAsync(pb.list, active.data(), sizeof(int) * A, hipMemcpyHostToDevice, st));
@@ -509,10 +576,10 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                     finish(i);
                     return;
                 }
+                const uint8_t *tsx = sx + static_cast<size_t>(i) * mx, *tsz = sz + static_cast<size_t>(i) * mz;
                 if (step == pp::Step::kAttempt) {
                     attempted[i] = 1;
-                    pp::attempt_begin(hg, tabs, prm, tr[i], est[i], &sx[static_cast<size_t>(i) * mx],
-                                      &sz[static_cast<size_t>(i) * mz], att[i]);
+                    pp::attempt_begin(hg, tabs, prm, tr[i], est[i], tsx, tsz, att[i]);
                     if (pp::attempt_end(tr[i], att[i], t, est[i])) {
                         finish(i);
                         return;
@@ -526,8 +593,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                         // ETS and flip sets now, |LLR| only if OSD follows
                         tr[i].iterations = max_iter;
                         o[i].post_run = true;
-                        o[i].post = pp::post_begin(hg, tabs, prm, tr[i], est[i], &sx[static_cast<size_t>(i) * mx],
-                                                   &sz[static_cast<size_t>(i) * mz]);
+                        o[i].post = pp::post_begin(hg, tabs, prm, tr[i], est[i], tsx, tsz);
                         need_llr[i] = o[i].post.want_osd;
                         if (!need_llr[i]) {
                             tr[i].done = true;
@@ -554,11 +620,12 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                                       hipMemcpyDeviceToHost, st));
                 HIPCHK(hipStreamSynchronize(st));
             }
+            if (t + 1 == max_iter && (rc = kick_once())) return rc;
             pool.run(static_cast<int>(want.size()), [&](int w) {
                 const int i = want[w];
                 double *lx = &llr[static_cast<size_t>(w) * 2 * n], *lz = lx + n;
                 pp::abs_llr_from_marginals(n, tot + static_cast<size_t>(w) * 4 * n, lx, lz);
-                const uint8_t *tsx = &sx[static_cast<size_t>(i) * mx], *tsz = &sz[static_cast<size_t>(i) * mz];
+                const uint8_t *tsx = sx + static_cast<size_t>(i) * mx, *tsz = sz + static_cast<size_t>(i) * mz;
                 if (att[i].want_osd) {
                     pp::attempt_osd(hg, att[i], tsx, tsz, lx, lz);
                     if (pp::attempt_end(tr[i], att[i], t, est[i])) {
@@ -571,11 +638,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                         end_trial(i);
                     } else {
                         tr[i].done = true;
-                        tr[i].iterations = max_iter;
-                        const pp::PostOut begun = o[i].post;
                         finish(i);
-                        o[i].post_run = true;
-                        o[i].post = begun;
                         pp::post_osd(hg, o[i].post, est[i], tsx, tsz, lx, lz);
                         o[i].ok = o[i].post.ok;
                     }
@@ -584,7 +647,11 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             active.erase(std::remove_if(active.begin(), active.end(), [&](int i) { return tr[i].done; }),
                          active.end());
         }
-        // write back: estimate planes, iterations, syndrome flag
+        if ((rc = kick_once())) return rc;
+        // write back: estimate planes, iterations, syndrome flag.  Queued
+        // without a wait: the host reads nothing it writes before the final
+        // synchronisation, and this buffer set's next use is ordered after
+        // it on the same stream.
         const size_t nplane = static_cast<size_t>(K) * 2 * g.wn;
         uint32_t *planes = static_cast<uint32_t *>(pb.h_stage);
         int *it = reinterpret_cast<int *>(planes + nplane);
@@ -599,21 +666,31 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             it[i] = o[i].iterations;
             okb[i] = o[i].ok ? 1 : 0;
         });
-        for (int i = 0; i < K; ++i) outs[b0 + i] = o[i];
-        {
-            char *stage = static_cast<char *>(pb.stage);
-            uint32_t *d_planes = reinterpret_cast<uint32_t *>(stage);
-            int *d_it = reinterpret_cast<int *>(stage + 4 * nplane);
-            uint8_t *d_ok = reinterpret_cast<uint8_t *>(d_it + K);
-            HIPCHK(hipMemcpyAsync(d_planes, planes, 4 * nplane + 5 * static_cast<size_t>(K),
-                                  hipMemcpyHostToDevice, st));
-            const int bx = std::min(16, (g.wn + 255) / 256);
-            hipLaunchKernelGGL(scatter_outcome_kernel, dim3(bx, K), dim3(256), 0, st, d_planes, d_it,
-                               d_ok, pb.slot_trial, K, g.wn, ctx->tp);
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipStreamSynchronize(st));
-        }
+        for (int i = 0; i < K; ++i) outs[B.b0 + i] = o[i];
+        char *stage = static_cast<char *>(pb.stage);
+        uint32_t *d_planes = reinterpret_cast<uint32_t *>(stage);
+        int *d_it = reinterpret_cast<int *>(stage + 4 * nplane);
+        uint8_t *d_ok = reinterpret_cast<uint8_t *>(d_it + K);
+        HIPCHK(hipMemcpyAsync(d_planes, planes, 4 * nplane + 5 * static_cast<size_t>(K), hipMemcpyHostToDevice, st));
+        const int bx = std::min(16, (g.wn + 255) / 256);
+        hipLaunchKernelGGL(scatter_outcome_kernel, dim3(bx, K), dim3(256), 0, st, d_planes, d_it, d_ok,
+                           pb.slot_trial, K, g.wn, ctx->tp);
+        HIPCHK(hipGetLastError());
+        return JBP_OK;
+    };
+
+    // Batches alternate between the two buffer sets and streams; batch
+    // b + 1 starts from inside batch b's finish (kick).
+    PPBatch cur, next;
+    int rc = start(cur, 0);
+    if (rc) return rc;
+    for (size_t bi = 0; bi < nbatch; ++bi) {
+        auto kick = [&]() -> int { return bi + 1 < nbatch ? start(next, bi + 1) : JBP_OK; };
+        if ((rc = finish_batch(cur, kick))) return rc;
+        std::swap(cur, next);
     }
+    HIPCHK(hipStreamSynchronize(streams[0]));
+    HIPCHK(hipStreamSynchronize(streams[1]));
     return JBP_OK;
 }
 
@@ -728,8 +805,8 @@ extern "C" int jbp_decode_llr(jbp_ctx *ctx, int count, const uint32_t *sx_bits, 
         1, std::min<size_t>(kPPSlots, kPPStateBytes / step_slot_bytes(g.side[0].cap, g.side[1].cap, n)));
     for (int b0 = 0; b0 < count; b0 += static_cast<int>(batch)) {
         const int K = static_cast<int>(std::min<size_t>(batch, static_cast<size_t>(count - b0)));
-        if ((rc = ensure_pp_buffers(ctx, K, max_iter, err, errlen))) return rc;
-        PPBuf &pb = ctx->ppb;
+        if ((rc = ensure_pp_buffers(ctx, ctx->ppb[0], K, max_iter, err, errlen))) return rc;
+        PPBuf &pb = ctx->ppb[0];
         std::vector<int> ids(K);
         for (int i = 0; i < K; ++i) ids[i] = i;
         std::vector<int> trials(K);
