@@ -60,6 +60,9 @@ struct PPBuf {
     uint8_t *h_out = nullptr;
     double *h_tot = nullptr;
     void *h_stage = nullptr;
+    // marginals reach the host in up to kTotChunks pieces, one event each
+    static constexpr int kTotChunks = 8;
+    hipEvent_t tot_ev[kTotChunks] = {};
 };
 
 // What the post-processing stage decided for one trial.

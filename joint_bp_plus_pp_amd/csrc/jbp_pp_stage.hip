@@ -305,6 +305,8 @@ constexpr int kPPSlots = 1024;
 constexpr size_t kPPStateBytes = size_t(8) << 30;
 
 void free_pp_buf(PPBuf &pb) {
+    for (hipEvent_t &e : pb.tot_ev)
+        if (e) (void)hipEventDestroy(e);
     for (void *p : {static_cast<void *>(pb.state), static_cast<void *>(pb.list),
                     static_cast<void *>(pb.slot_trial), static_cast<void *>(pb.rec),
                     static_cast<void *>(pb.out), static_cast<void *>(pb.idx), pb.stage})
@@ -319,6 +321,13 @@ int ensure_pp_buffers(jbp_ctx *ctx, PPBuf &pb, int slots, int rec_per_slot, char
     const DevGraph &g = ctx->dg;
     const size_t stride = step_slot_bytes(g.side[0].cap, g.side[1].cap, g.n);
     if (pb.cap >= slots && pb.rec_cap >= rec_per_slot && pb.stride == stride) return JBP_OK;
+    // Grow only (slots rounded up to 256, within the per-batch cap): a
+    // fresh pinned buffer's first copies run at a fraction of the warm
+    // rate, so calls of varying sizes keep the buffers they already have.
+    const bool same = pb.stride == stride;
+    const int batch_cap = static_cast<int>(std::max<size_t>(1, std::min<size_t>(kPPSlots, kPPStateBytes / stride)));
+    slots = std::max(std::min((slots + 255) / 256 * 256, std::max(slots, batch_cap)), same ? pb.cap : 0);
+    rec_per_slot = std::max(rec_per_slot, same ? pb.rec_cap : 0);
     free_pp_buf(pb);
     pb.cap = slots;
     pb.rec_cap = std::max(rec_per_slot, 1);
@@ -341,6 +350,7 @@ int ensure_pp_buffers(jbp_ctx *ctx, PPBuf &pb, int slots, int rec_per_slot, char
     HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&pb.h_out), 2 * static_cast<size_t>(g.n) * slots));
     HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&pb.h_tot), tot));
     HIPCHK(hipHostMalloc(&pb.h_stage, std::max(syn, back)));
+    for (hipEvent_t &e : pb.tot_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return JBP_OK;
 }
 
@@ -363,7 +373,8 @@ struct PPBatch {
     int K = 0;
     PPBuf *pb = nullptr;
     hipStream_t st = nullptr;
-    bool phase_a = false;  // phase A launched, results not yet read
+    bool syn_wait = false;  // syndromes queued for the host, not yet unpacked
+    bool phase_a = false;   // phase A launched, results not yet read
     std::vector<uint8_t> sx, sz;
     std::vector<pp::Trial> tr;
     std::vector<std::vector<uint8_t>> est;
@@ -371,7 +382,6 @@ struct PPBatch {
     std::vector<int> active;
     std::vector<pp::Attempt> att;
     std::vector<uint8_t> need_llr, at_end, attempted;
-    std::vector<double> llr;
 };
 
 int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int hist_window,
@@ -393,8 +403,8 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
     const size_t nbatch = (trials.size() + batch - 1) / batch;
     hipStream_t streams[2] = {ctx->stream, ctx->pp_stream};
 
-    // Syndromes to bytes, host state, and (H > 0) phase A launched without
-    // waiting for it.
+    // Host state, the syndromes' copy to the host and (H > 0) phase A, all
+    // queued without waiting; finish_batch waits for them.
     auto start = [&](PPBatch &B, size_t bi) -> int {
         B = PPBatch();
         B.b0 = bi * batch;
@@ -414,7 +424,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
         B.sz.resize(static_cast<size_t>(K) * mz);
         for (int s = 0; s < 2; ++s) {
             const int wm = g.wm[s];
-            uint32_t *w = static_cast<uint32_t *>(pb.h_stage);
+            uint32_t *w = static_cast<uint32_t *>(pb.h_stage) + (s == 0 ? 0 : static_cast<size_t>(K) * g.wm[0]);
             const uint32_t *src = s == 0 ? ctx->tp.syn_x : ctx->tp.syn_z;
             if (wm % 2 == 0) {
                 launch_gather(src, wm / 2, 0, wm / 2, pb.slot_trial, K, pb.stage, st);
@@ -426,13 +436,8 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                                           src + static_cast<size_t>(slot_trial[i]) * wm, 4 * wm,
                                           hipMemcpyDeviceToHost, st));
             }
-            HIPCHK(hipStreamSynchronize(st));
-            const int m = s == 0 ? mx : mz;
-            uint8_t *dst = s == 0 ? B.sx.data() : B.sz.data();
-            pool.run(K, [&](int i) {
-                unpack_bits(&w[static_cast<size_t>(i) * wm], m, dst + static_cast<size_t>(i) * m);
-            });
         }
+        B.syn_wait = true;
         B.tr.assign(K, pp::Trial());
         B.est.assign(K, std::vector<uint8_t>(n, 0));
         B.o.assign(K, PPOutcome());
@@ -508,6 +513,17 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             o[i].ok = false;
             est[i].swap(att[i].trial);
         };
+        // syndromes and phase A (queued by start)
+        HIPCHK(hipStreamSynchronize(st));
+        if (B.syn_wait) {
+            const uint32_t *w = static_cast<const uint32_t *>(pb.h_stage);
+            const uint32_t *wz = w + static_cast<size_t>(K) * g.wm[0];
+            pool.run(K, [&](int i) {
+                unpack_bits(w + static_cast<size_t>(i) * g.wm[0], mx, B.sx.data() + static_cast<size_t>(i) * mx);
+                unpack_bits(wz + static_cast<size_t>(i) * g.wm[1], mz, B.sz.data() + static_cast<size_t>(i) * mz);
+            });
+            B.syn_wait = false;
+        }
         if (max_iter <= 0) {
             // no iteration at all: the post-decode pass on the zero estimate
             pool.run(K, [&](int i) {
@@ -523,7 +539,6 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
         int *rec = pb.h_rec;
         uint8_t *out = pb.h_out;
         if (B.phase_a) {
-            HIPCHK(hipStreamSynchronize(st));
             B.phase_a = false;
             pool.run(K, [&](int i) {
                 for (int t = 0; t < H; ++t) {
@@ -546,7 +561,6 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                          active.end());
         }
         double *tot = pb.h_tot;
-        auto &llr = B.llr;
         for (int t = std::max(H, 0); t < max_iter && !active.empty(); ++t) {
             const int A = static_cast<int>(active.size());
             HIPCHK(hipMemcpyAsync(pb.list, active.data(), sizeof(int) * A, hipMemcpyHostToDevice, st));
@@ -609,21 +623,37 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             std::vector<int> want;
             for (int i : active)
                 if (!tr[i].done && need_llr[i]) want.push_back(i);
-            if (!want.empty()) {
-                const int W = static_cast<int>(want.size());
-                llr.resize(static_cast<size_t>(W) * 2 * n);
+            // One gather, then the copy to the host in chunks, each with an
+            // event: the repairs of a chunk start as soon as it has arrived,
+            // while the later chunks are still on their way.
+            const int W = static_cast<int>(want.size());
+            const int per = W > 0 ? (W + PPBuf::kTotChunks - 1) / PPBuf::kTotChunks : 1;
+            if (W > 0) {
                 const size_t tot_off = 6 * static_cast<size_t>(g.side[0].cap + g.side[1].cap);
+                const size_t row = 4 * static_cast<size_t>(n);
                 HIPCHK(hipMemcpyAsync(pb.idx, want.data(), sizeof(int) * W, hipMemcpyHostToDevice, st));
                 launch_gather(pb.state, pb.stride / 8, tot_off, 4 * n, pb.idx, W, pb.stage, st);
                 HIPCHK(hipGetLastError());
-                HIPCHK(hipMemcpyAsync(tot, pb.stage, sizeof(double) * static_cast<size_t>(W) * 4 * n,
-                                      hipMemcpyDeviceToHost, st));
-                HIPCHK(hipStreamSynchronize(st));
+                for (int c = 0; c * per < W; ++c) {
+                    const size_t w0 = static_cast<size_t>(c) * per, w1 = std::min<size_t>(W, w0 + per);
+                    HIPCHK(hipMemcpyAsync(tot + w0 * row, static_cast<const double *>(pb.stage) + w0 * row,
+                                          sizeof(double) * (w1 - w0) * row, hipMemcpyDeviceToHost, st));
+                    HIPCHK(hipEventRecord(pb.tot_ev[c], st));
+                }
             }
+            // this batch's last kernel is queued: the next batch may fill the device
             if (t + 1 == max_iter && (rc = kick_once())) return rc;
-            pool.run(static_cast<int>(want.size()), [&](int w) {
+            std::atomic<int> ev_err{0};
+            pool.run(W, [&](int w) {
+                if (hipEventSynchronize(pb.tot_ev[w / per]) != hipSuccess) {
+                    ev_err.store(1);
+                    return;
+                }
                 const int i = want[w];
-                double *lx = &llr[static_cast<size_t>(w) * 2 * n], *lz = lx + n;
+                // |LLR| lives only for this trial's OSD: one buffer per thread
+                thread_local std::vector<double> llr;
+                llr.resize(2 * static_cast<size_t>(n));
+                double *lx = llr.data(), *lz = lx + n;
                 pp::abs_llr_from_marginals(n, tot + static_cast<size_t>(w) * 4 * n, lx, lz);
                 const uint8_t *tsx = sx + static_cast<size_t>(i) * mx, *tsz = sz + static_cast<size_t>(i) * mz;
                 if (att[i].want_osd) {
@@ -644,6 +674,10 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                     }
                 }
             });
+            if (ev_err.load()) {
+                set_err(err, errlen, "waiting for the marginals' copy to the host failed");
+                return JBP_ERR_DEVICE;
+            }
             active.erase(std::remove_if(active.begin(), active.end(), [&](int i) { return tr[i].done; }),
                          active.end());
         }
