@@ -219,7 +219,9 @@ int jbp_set_pp_tables(jbp_ctx *ctx, const jbp_pp_tables *t, char *err, size_t er
 
 /* One repair on one estimate, on the host (no device): method 0 = ETS over
  * the groups of t, 1 = flip set `vars` (n_vars entries), 2 = OSD with the
- * per-qubit |LLR| abs_llr; side 0 repairs the X part against the X-check
+ * per-qubit |LLR| abs_llr, 3 = the same OSD with abs_llr holding the
+ * normalised marginals [nvars][4] the |LLR| come from (the form the device
+ * stage hands over); side 0 repairs the X part against the X-check
  * syndrome, 1 the Z part.  est (nvars states 0..3) is updated in place when
  * the repair applies; *applied = 1 then (for ETS, the group's size). */
 int jbp_pp_repair(const jbp_graph_desc *graph, const jbp_pp_tables *t, int method, int side,

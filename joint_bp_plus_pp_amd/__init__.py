@@ -79,6 +79,7 @@ PPF_BP_OK, PPF_PP_SUCCESS, PPF_ETS_SAVED, PPF_FLIP_SAVED = 1, 2, 4, 8
 PPF_OSD_SAVED, PPF_ETS_USED, PPF_ETS6, PPF_ETS12 = 16, 32, 64, 128
 # jbp_pp_repair methods
 PP_ETS, PP_FLIP, PP_OSD = 0, 1, 2
+PP_OSD_MARGINALS = 3  # OSD given the normalised marginals [n][4] instead of |LLR|
 # the reference's table groups in the order it loads and tries them
 # (size, label, file name), jointbp_ets.cpp:5229-5327
 ETS_GROUPS = [(6, "6", "ets_6_2.txt"), (8, "8", "ets_8_2.txt"), (8, "8p4", "ets_8_2_path4.txt"),

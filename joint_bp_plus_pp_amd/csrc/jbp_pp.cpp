@@ -490,9 +490,23 @@ bool flip_fix_diff(const Graph &g, int side, const std::vector<int> &vars_in, co
     return true;
 }
 
-bool osd_fix(const Graph &g, int side, const double *abs_llr, const uint8_t *target,
+namespace {
+
+// Clamped pair sums of a marginal: P(bit = 1), P(bit = 0) of the side's
+// Pauli bit, as jointbp_ets.cpp:2483-2493 forms them.
+inline void llr_pair(const double *t, int side, double &p1, double &p0) {
+    const double eps = 1e-300;
+    p1 = side == 0 ? t[1] + t[3] : t[2] + t[3];
+    p0 = side == 0 ? t[0] + t[2] : t[0] + t[1];
+    p1 = std::max(p1, eps);
+    p0 = std::max(p0, eps);
+}
+
+}  // namespace
+
+bool osd_fix(const Graph &g, int side, const LlrSide &llr, const uint8_t *target,
              std::vector<uint8_t> &est) {
-    if (!abs_llr || g.m[side] == 0) return false;
+    if (llr.empty() || g.m[side] == 0) return false;
     const int m = g.m[side], n = g.n;
     std::vector<uint8_t> diff = syndrome_of(g, side, est);
     bool any = false;
@@ -502,21 +516,63 @@ bool osd_fix(const Graph &g, int side, const double *abs_llr, const uint8_t *tar
     }
     if (!any) return false;
     // |LLR| order with ties by index (the reference's stable sort); only
-    // the prefix the walk reaches is sorted, in growing chunks
+    // the prefix the walk reaches is sorted, in growing chunks.
+    //
+    // From marginals, |LLR| = |log p1 - log p0| (the reference's value, with
+    // its std::log) is computed only for the candidates of each chunk.  The
+    // candidates come from the key k = max(p1, p0) / min(p1, p0): with p1, p0
+    // in [1e-300, 1e300] and logs within 2 ulp, |LLR| and ln k differ by less
+    // than E = 1e-12.  Take kt, the chunk's last key in key order, and as
+    // candidates every remaining variable with k <= kt (1 + 1e-9).  At least
+    // a chunk's worth of variables have k <= kt, so |LLR| <= ln kt + E; a
+    // variable with k above the bound has |LLR| > ln kt + 1e-9 - E, strictly
+    // more, and is not in the chunk.  Sorting the candidates by the exact
+    // value therefore gives the chunk exactly.
     std::vector<int> order(n);
     for (int v = 0; v < n; ++v) order[v] = v;
-    const auto before = [&](int a, int b) {
-        return abs_llr[a] < abs_llr[b] || (abs_llr[a] == abs_llr[b] && a < b);
+    std::vector<double> val, key;
+    const double *L = llr.abs_llr;
+    bool lazy = false;
+    auto exact = [&](int v) {
+        double p1, p0;
+        llr_pair(llr.tot4 + 4 * static_cast<size_t>(v), side, p1, p0);
+        return std::fabs(std::log(p1) - std::log(p0));
     };
+    if (!L) {
+        val.assign(n, -1.0);
+        key.resize(n);
+        lazy = true;
+        for (int v = 0; v < n && lazy; ++v) {
+            double p1, p0;
+            llr_pair(llr.tot4 + 4 * static_cast<size_t>(v), side, p1, p0);
+            key[v] = p1 >= p0 ? p1 / p0 : p0 / p1;
+            lazy = p1 <= 1e300 && p0 <= 1e300 && std::isfinite(key[v]);
+        }
+        if (!lazy)  // outside the bound's range: every value, as the reference
+            for (int v = 0; v < n; ++v) val[v] = exact(v);
+        L = val.data();
+    }
+    const auto before = [&](int a, int b) { return L[a] < L[b] || (L[a] == L[b] && a < b); };
     int sorted = 0;
+    auto extend = [&](int next) {
+        auto rb = order.begin() + sorted;
+        if (lazy) {
+            const int want = next - sorted;
+            std::nth_element(rb, rb + (want - 1), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+            const double lim = key[*(rb + (want - 1))] * (1.0 + 1e-9);
+            auto mid = std::partition(rb, order.end(), [&](int v) { return key[v] <= lim; });
+            for (auto it = rb; it != mid; ++it)
+                if (!(val[*it] >= 0.0)) val[*it] = exact(*it);
+            std::partial_sort(rb, order.begin() + next, mid, before);
+        } else {
+            std::partial_sort(rb, order.begin() + next, order.end(), before);
+        }
+        sorted = next;
+    };
     SpanBasis basis(m, std::min(n, m + 1));
     basis.set_target(diff.data());
     for (int j = 0; j < n; ++j) {
-        if (j == sorted) {
-            const int next = std::min(n, std::max(2 * sorted, sorted + 256));
-            std::partial_sort(order.begin() + sorted, order.begin() + next, order.end(), before);
-            sorted = next;
-        }
+        if (j == sorted) extend(std::min(n, std::max(2 * sorted, sorted + 256)));
         int cnt = 0;
         const int *rows = col_rows(g, side, order[j], cnt);
         if (!basis.add(rows, cnt, j)) return false;
@@ -635,7 +691,7 @@ void attempt_begin(const Graph &g, const Tables &t, const Params &prm, const Tri
 }
 
 void attempt_osd(const Graph &g, Attempt &a, const uint8_t *sx, const uint8_t *sz,
-                 const double *llr_x, const double *llr_z) {
+                 const LlrSide &llr_x, const LlrSide &llr_z) {
     if (!a.want_osd) return;
     const bool ox = osd_fix(g, 0, llr_x, sx, a.trial);
     const bool oz = osd_fix(g, 1, llr_z, sz, a.trial);
@@ -673,7 +729,7 @@ PostOut post_begin(const Graph &g, const Tables &t, const Params &prm, const Tri
 }
 
 void post_osd(const Graph &g, PostOut &o, std::vector<uint8_t> &est, const uint8_t *sx,
-              const uint8_t *sz, const double *llr_x, const double *llr_z) {
+              const uint8_t *sz, const LlrSide &llr_x, const LlrSide &llr_z) {
     if (!o.want_osd) return;
     const bool ox = osd_fix(g, 0, llr_x, sx, est);
     const bool oz = osd_fix(g, 1, llr_z, sz, est);
@@ -684,7 +740,7 @@ PostOut post_decode(const Graph &g, const Tables &t, const Params &prm, const Tr
                     std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz,
                     const double *llr_x, const double *llr_z) {
     PostOut o = post_begin(g, t, prm, tr, est, sx, sz);
-    post_osd(g, o, est, sx, sz, llr_x, llr_z);
+    post_osd(g, o, est, sx, sz, LlrSide::values(llr_x), LlrSide::values(llr_z));
     return o;
 }
 

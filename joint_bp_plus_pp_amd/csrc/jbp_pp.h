@@ -88,11 +88,22 @@ bool flip_fix(const Graph &g, int side, const std::vector<int> &vars, const uint
 bool flip_fix_diff(const Graph &g, int side, const std::vector<int> &vars, const uint8_t *target,
                    const std::vector<uint8_t> &diff, std::vector<uint8_t> &est);
 
+// One side's |LLR|, as values or as the normalised marginals [n][4] they
+// come from (jointbp_ets.cpp:2483-2493).  From marginals OSD computes the
+// exact values (std::log) only around the prefix of the order it walks.
+struct LlrSide {
+    const double *abs_llr = nullptr;
+    const double *tot4 = nullptr;
+    static LlrSide values(const double *v) { LlrSide l; l.abs_llr = v; return l; }
+    static LlrSide marginals(const double *t) { LlrSide l; l.tot4 = t; return l; }
+    bool empty() const { return !abs_llr && !tot4; }
+};
+
 // Ordered-statistics repair (jointbp_ets.cpp:4078-4386): with the variables
 // ordered by increasing |LLR|, the shortest prefix whose columns reach the
 // syndrome difference; applied when those columns are independent and the
 // correction has weight <= 30.
-bool osd_fix(const Graph &g, int side, const double *abs_llr, const uint8_t *target,
+bool osd_fix(const Graph &g, int side, const LlrSide &llr, const uint8_t *target,
              std::vector<uint8_t> &est);
 
 // The reference's sets of flip candidates (jointbp_ets.cpp:3670-3689):
@@ -159,7 +170,7 @@ void attempt_begin(const Graph &g, const Tables &t, const Params &prm, const Tri
                    const std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz,
                    Attempt &a);
 void attempt_osd(const Graph &g, Attempt &a, const uint8_t *sx, const uint8_t *sz,
-                 const double *llr_x, const double *llr_z);
+                 const LlrSide &llr_x, const LlrSide &llr_z);
 bool attempt_end(Trial &tr, Attempt &a, int t, std::vector<uint8_t> &est);
 
 // The post-decode pass over a trial that ended unconverged
@@ -179,7 +190,7 @@ PostOut post_decode(const Graph &g, const Tables &t, const Params &prm, const Tr
 PostOut post_begin(const Graph &g, const Tables &t, const Params &prm, const Trial &tr,
                    std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz);
 void post_osd(const Graph &g, PostOut &o, std::vector<uint8_t> &est, const uint8_t *sx,
-              const uint8_t *sz, const double *llr_x, const double *llr_z);
+              const uint8_t *sz, const LlrSide &llr_x, const LlrSide &llr_z);
 
 }  // namespace pp
 }  // namespace jbp

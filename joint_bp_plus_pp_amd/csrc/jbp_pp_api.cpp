@@ -78,8 +78,8 @@ int jbp_pp_repair(const jbp_graph_desc *graph, const jbp_pp_tables *t, int metho
                   const int *syndrome, const double *abs_llr, const int *vars, int n_vars,
                   int *est, int *applied, char *err, size_t errlen) {
     if (!desc_ok(graph) || !syndrome || !est || (side != 0 && side != 1) || method < 0 ||
-        method > 2 || (method == 0 && !t) || (method == 1 && n_vars > 0 && !vars) ||
-        (method == 2 && !abs_llr)) {
+        method > 3 || (method == 0 && !t) || (method == 1 && n_vars > 0 && !vars) ||
+        (method >= 2 && !abs_llr)) {
         say(err, errlen, "bad arguments");
         return JBP_ERR_ARG;
     }
@@ -102,7 +102,9 @@ int jbp_pp_repair(const jbp_graph_desc *graph, const jbp_pp_tables *t, int metho
         std::vector<int> set(vars, vars + (n_vars > 0 ? n_vars : 0));
         done = jbp::pp::flip_fix(g, side, set, syn.data(), e) ? 1 : 0;
     } else {
-        done = jbp::pp::osd_fix(g, side, abs_llr, syn.data(), e) ? 1 : 0;
+        const jbp::pp::LlrSide l =
+            method == 2 ? jbp::pp::LlrSide::values(abs_llr) : jbp::pp::LlrSide::marginals(abs_llr);
+        done = jbp::pp::osd_fix(g, side, l, syn.data(), e) ? 1 : 0;
     }
     if (done)
         for (int v = 0; v < n; ++v) est[v] = e[v];

@@ -461,10 +461,10 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
     };
 
     // Phase A's records, phase B, the repairs and the write-back.  kick()
-    // starts the next batch; it is called once this batch has no kernel
-    // left but its write-back, so the next batch's phase A (which fills
-    // the device) runs during this batch's last host repairs instead of
-    // holding this batch's kernels back.
+    // starts the next batch; it is called once this batch's last step has
+    // run and the marginals of its last iteration are gathered, so the
+    // next batch's phase A (which holds the device) runs during this
+    // batch's repairs, and the marginals travel to the host meanwhile.
     auto finish_batch = [&](PPBatch &B, const std::function<int()> &kick) -> int {
         bool kicked = false;
         auto kick_once = [&]() -> int {
@@ -561,6 +561,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                          active.end());
         }
         double *tot = pb.h_tot;
+        std::vector<int> pos(K);  // trial -> row of its marginals (last iteration)
         for (int t = std::max(H, 0); t < max_iter && !active.empty(); ++t) {
             const int A = static_cast<int>(active.size());
             HIPCHK(hipMemcpyAsync(pb.list, active.data(), sizeof(int) * A, hipMemcpyHostToDevice, st));
@@ -571,6 +572,33 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                                   hipMemcpyDeviceToHost, st));
             HIPCHK(hipMemcpyAsync(out, pb.out, static_cast<size_t>(A) * 2 * n, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
+            // Marginals reach the host in chunks, one event each, so that the
+            // repairs of a chunk start as soon as it has arrived.  In the
+            // last iteration every active trial's are gathered now, before
+            // the next batch takes the device (most of them are needed);
+            // before that only those of the trials that reach OSD, below.
+            const bool last = t + 1 == max_iter;
+            const size_t row = 4 * static_cast<size_t>(n);
+            const size_t tot_off = 6 * static_cast<size_t>(g.side[0].cap + g.side[1].cap);
+            auto queue_marginals = [&](const int *idx, int count, int &per) -> int {
+                per = std::max(1, (count + PPBuf::kTotChunks - 1) / PPBuf::kTotChunks);
+                if (count <= 0) return JBP_OK;
+                launch_gather(pb.state, pb.stride / 8, tot_off, 4 * n, idx, count, pb.stage, st);
+                HIPCHK(hipGetLastError());
+                for (int c = 0; c * per < count; ++c) {
+                    const size_t w0 = static_cast<size_t>(c) * per, w1 = std::min<size_t>(count, w0 + per);
+                    HIPCHK(hipMemcpyAsync(tot + w0 * row, static_cast<const double *>(pb.stage) + w0 * row,
+                                          sizeof(double) * (w1 - w0) * row, hipMemcpyDeviceToHost, st));
+                    HIPCHK(hipEventRecord(pb.tot_ev[c], st));
+                }
+                return JBP_OK;
+            };
+            int per = 1;
+            if (last) {
+                if ((rc = queue_marginals(pb.list, A, per))) return rc;
+                if ((rc = kick_once())) return rc;
+                for (int a = 0; a < A; ++a) pos[active[a]] = a;
+            }
             pool.run(A, [&](int a) {
                 const int i = active[a];
                 const int *r = &rec[static_cast<size_t>(a) * kRecInts];
@@ -623,38 +651,21 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             std::vector<int> want;
             for (int i : active)
                 if (!tr[i].done && need_llr[i]) want.push_back(i);
-            // One gather, then the copy to the host in chunks, each with an
-            // event: the repairs of a chunk start as soon as it has arrived,
-            // while the later chunks are still on their way.
             const int W = static_cast<int>(want.size());
-            const int per = W > 0 ? (W + PPBuf::kTotChunks - 1) / PPBuf::kTotChunks : 1;
-            if (W > 0) {
-                const size_t tot_off = 6 * static_cast<size_t>(g.side[0].cap + g.side[1].cap);
-                const size_t row = 4 * static_cast<size_t>(n);
+            if (!last && W > 0) {
                 HIPCHK(hipMemcpyAsync(pb.idx, want.data(), sizeof(int) * W, hipMemcpyHostToDevice, st));
-                launch_gather(pb.state, pb.stride / 8, tot_off, 4 * n, pb.idx, W, pb.stage, st);
-                HIPCHK(hipGetLastError());
-                for (int c = 0; c * per < W; ++c) {
-                    const size_t w0 = static_cast<size_t>(c) * per, w1 = std::min<size_t>(W, w0 + per);
-                    HIPCHK(hipMemcpyAsync(tot + w0 * row, static_cast<const double *>(pb.stage) + w0 * row,
-                                          sizeof(double) * (w1 - w0) * row, hipMemcpyDeviceToHost, st));
-                    HIPCHK(hipEventRecord(pb.tot_ev[c], st));
-                }
+                if ((rc = queue_marginals(pb.idx, W, per))) return rc;
             }
-            // this batch's last kernel is queued: the next batch may fill the device
-            if (t + 1 == max_iter && (rc = kick_once())) return rc;
             std::atomic<int> ev_err{0};
             pool.run(W, [&](int w) {
-                if (hipEventSynchronize(pb.tot_ev[w / per]) != hipSuccess) {
+                const int i = want[w];
+                const int r = last ? pos[i] : w;  // row of the trial's marginals
+                if (hipEventSynchronize(pb.tot_ev[r / per]) != hipSuccess) {
                     ev_err.store(1);
                     return;
                 }
-                const int i = want[w];
-                // |LLR| lives only for this trial's OSD: one buffer per thread
-                thread_local std::vector<double> llr;
-                llr.resize(2 * static_cast<size_t>(n));
-                double *lx = llr.data(), *lz = lx + n;
-                pp::abs_llr_from_marginals(n, tot + static_cast<size_t>(w) * 4 * n, lx, lz);
+                // OSD takes the marginals and computes the |LLR| it needs
+                const pp::LlrSide lx = pp::LlrSide::marginals(tot + static_cast<size_t>(r) * row), lz = lx;
                 const uint8_t *tsx = sx + static_cast<size_t>(i) * mx, *tsz = sz + static_cast<size_t>(i) * mz;
                 if (att[i].want_osd) {
                     pp::attempt_osd(hg, att[i], tsx, tsz, lx, lz);
