@@ -255,7 +255,9 @@ int ensure_ints(jbp_ctx *ctx, size_t elems, char *err, size_t errlen) {
 
 int ensure_f64(jbp_ctx *ctx, char *err, size_t errlen) {
     if (ctx->f64_scratch) return JBP_OK;
-    int slots = ctx->f64_slots > 0 ? ctx->f64_slots : 2 * std::max(1, ctx->cus);
+    // resident 256-thread blocks per compute unit: 3 for degree 12 / 3
+    const int per_cu = ctx->dg.max_cdeg <= 12 && ctx->dg.max_vdeg <= 3 ? 3 : 2;
+    int slots = ctx->f64_slots > 0 ? ctx->f64_slots : per_cu * std::max(1, ctx->cus);
     size_t bytes = decode_f64_scratch_bytes(ctx->dg, slots);
     HIPCHK(hipMalloc(&ctx->f64_scratch, bytes));
     ctx->f64_slots = slots;

@@ -41,9 +41,9 @@ __device__ __forceinline__ SlotPtrs slot_ptrs(const DevGraph &g, double *scratch
     return s;
 }
 
-// Blocks of up to 512 threads (the tuner's largest), at least two resident
-// per compute unit: the register budget this leaves (up to 128 VGPRs) keeps
-// the check recursion's arrays in registers.
+// Blocks of up to 512 threads (the tuner's largest); the second bound is
+// waves per SIMD (2: up to 256 VGPRs).  Degree 12 / 3 with pair sums needs
+// about 143, so three 256-thread blocks share a compute unit.
 template <int MAXC, int MAXV, bool PAIR>
 __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPlanes tp, int count, NoiseTab nt, int max_iter,
                                   double damping, double *scratch, size_t stride, int *counter) {
@@ -80,16 +80,14 @@ __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPla
         int iterations = max_iter;
         int matched = 0;
         for (int iter = 0; iter < max_iter; ++iter) {
-            for (int c = tid; c < mx + mz; c += bs) {
-                if (c < mx) {
-                    int syn = (tsx[c >> 5] >> (c & 31)) & 1;
-                    check_update<MAXC, PAIR>(g.side[0], 0, c, syn, sp.v2c[0], sp.c2v[0]);
-                } else {
-                    int cz = c - mx;
-                    int syn = (tsz[cz >> 5] >> (cz & 31)) & 1;
-                    check_update<MAXC, PAIR>(g.side[1], 1, cz, syn, sp.v2c[1], sp.c2v[1]);
-                }
-            }
+            // one loop per side: a single loop branching between the sides
+            // keeps both inlined recursions' registers live
+            #pragma unroll 1
+            for (int c = tid; c < mx; c += bs)
+                check_update<MAXC, PAIR>(g.side[0], 0, c, (tsx[c >> 5] >> (c & 31)) & 1, sp.v2c[0], sp.c2v[0]);
+            #pragma unroll 1
+            for (int c = tid; c < mz; c += bs)
+                check_update<MAXC, PAIR>(g.side[1], 1, c, (tsz[c >> 5] >> (c & 31)) & 1, sp.v2c[1], sp.c2v[1]);
             __syncthreads();
             for (int v = tid; v < g.n; v += bs)
                 sp.est[v] = static_cast<uint8_t>(

@@ -62,25 +62,35 @@ __device__ __forceinline__ void store_v2c(double *v2c, size_t row, int side, con
     }
 }
 
-// Check update of one check (jointbp_ets.cpp:2373-2449).
+// Check update of one check (jointbp_ets.cpp:2373-2449).  Every position's
+// incoming message is loaded before any is used (positions past the
+// check's degree load row 0 and are ignored): loads under the degree test
+// would each wait for the previous one.  Row i of a side's messages starts
+// at a block-uniform address, so each load is that base plus the check's
+// 32-bit offset.
 template <int MAXC, bool PAIR>
 __device__ void check_update(const DevSide &sd, int side, int c, int syn, const double *v2c,
                              double *c2v) {
     const int deg = sd.chk_off[c + 1] - sd.chk_off[c];
     const size_t m = static_cast<size_t>(sd.m);
+    constexpr int W = PAIR ? 2 : 4;  // doubles per v2c row
     double q0[MAXC], q1[MAXC], pe[MAXC + 1], po[MAXC + 1];
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+        const double *row = v2c + W * (i < deg ? i : 0) * m;
+        if (PAIR) {
+            const double2 mv = *reinterpret_cast<const double2 *>(row + 2 * c);
+            q0[i] = mv.x;
+            q1[i] = mv.y;
+        } else {
+            pair_sums(row + 4 * c, side, q0[i], q1[i]);
+        }
+    }
     pe[0] = 1.0;
     po[0] = 0.0;
 #pragma unroll
     for (int i = 0; i < MAXC; ++i) {
         if (i < deg) {
-            if (PAIR) {
-                const double *mv = v2c + 2 * (i * m + c);
-                q0[i] = mv[0];
-                q1[i] = mv[1];
-            } else {
-                pair_sums(v2c + 4 * (i * m + c), side, q0[i], q1[i]);
-            }
             pe[i + 1] = pe[i] * q0[i] + po[i] * q1[i];
             po[i + 1] = pe[i] * q1[i] + po[i] * q0[i];
         }
@@ -94,14 +104,15 @@ __device__ void check_update(const DevSide &sd, int side, int c, int syn, const 
             double v0 = syn == 0 ? p_even : p_odd;
             double v1 = syn == 0 ? p_odd : p_even;
             double sum = side == 0 ? ((v0 + v1) + v0) + v1 : ((v0 + v0) + v1) + v1;
-            double *o = c2v + 2 * (i * m + c);
+            double2 o;
             if (sum <= 0.0) {
-                o[0] = 0.25;
-                o[1] = 0.25;
+                o.x = 0.25;
+                o.y = 0.25;
             } else {
-                o[0] = v0 / sum;
-                o[1] = v1 / sum;
+                o.x = v0 / sum;
+                o.y = v1 / sum;
             }
+            *reinterpret_cast<double2 *>(c2v + 2 * i * m + 2 * c) = o;
             double ne = se * q0[i] + so * q1[i];
             double no = se * q1[i] + so * q0[i];
             se = ne;

@@ -77,8 +77,11 @@ constexpr int kRecInts = 4;
 // initial messages.  want_tot: write every iteration's marginals (the slot
 // then holds the last iteration's).  At the end the block copies its
 // estimate and previous estimate to out[b].
+// Waves per SIMD the register budget allows: 4 for degree 12 / 3 with pair
+// sums (128 VGPRs, every block of a 1024-trial batch resident at once), 2
+// for the larger recursions.
 template <int MAXC, int MAXV, bool PAIR>
-__global__ __launch_bounds__(256, 2) void pp_step_kernel(DevGraph g, TrialPlanes tp, NoiseTab nt, const int *list,
+__global__ __launch_bounds__(256, (MAXC <= 12 && PAIR) ? 4 : 2) void pp_step_kernel(DevGraph g, TrialPlanes tp, NoiseTab nt, const int *list,
                                const int *slot_trial, int steps, int init, int want_tot,
                                double damping, char *state, size_t stride, int *rec, uint8_t *out) {
     __shared__ int s_cnt[2];
@@ -104,16 +107,14 @@ __global__ __launch_bounds__(256, 2) void pp_step_kernel(DevGraph g, TrialPlanes
     }
     for (int j = 0; j < steps; ++j) {
         if (tid == 0) s_cnt[0] = s_cnt[1] = 0;
-        for (int c = tid; c < mx + mz; c += bs) {
-            if (c < mx) {
-                check_update<MAXC, PAIR>(g.side[0], 0, c, (tsx[c >> 5] >> (c & 31)) & 1, sp.v2c[0],
-                                         sp.c2v[0]);
-            } else {
-                const int cz = c - mx;
-                check_update<MAXC, PAIR>(g.side[1], 1, cz, (tsz[cz >> 5] >> (cz & 31)) & 1, sp.v2c[1],
-                                   sp.c2v[1]);
-            }
-        }
+        // one loop per side: a single loop branching between the sides
+        // keeps both inlined recursions' registers live
+        #pragma unroll 1
+        for (int c = tid; c < mx; c += bs)
+            check_update<MAXC, PAIR>(g.side[0], 0, c, (tsx[c >> 5] >> (c & 31)) & 1, sp.v2c[0], sp.c2v[0]);
+        #pragma unroll 1
+        for (int c = tid; c < mz; c += bs)
+            check_update<MAXC, PAIR>(g.side[1], 1, c, (tsz[c >> 5] >> (c & 31)) & 1, sp.v2c[1], sp.c2v[1]);
         __syncthreads();
         double *tot = want_tot ? sp.tot : nullptr;
         int changed1 = 0, changed2 = 0;

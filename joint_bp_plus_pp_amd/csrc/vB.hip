@@ -1,0 +1,893 @@
+// Post-processing stage of a simulate chunk: belief propagation for the
+// trials that are still unconverged when the reference's PP triggers first
+// become possible, one iteration per launch on the device, with the PP
+// decisions and repairs (jbp_pp.cpp) on host threads between launches.
+//
+// Why the split works: the reference only considers PP from iteration 100
+// on (or at the last iteration, jointbp_ets.cpp:2563), so a trial that BP
+// finishes in H = min(99, max_iter - 1) iterations never reaches it.  Those
+// trials keep the ordinary decoders' results.  The rest are re-decoded here
+// from iteration 1 in FP64 with the reference's operation order, so their
+// estimates, iteration counts and marginals are the reference's own:
+//   phase A: iterations 1..H in one launch per batch, recording per
+//            iteration the unsatisfied-check counts and whether the estimate
+//            equals the previous / the one before (all the trigger logic
+//            reads before a trigger can fire);
+//   phase B: from iteration H+1, one launch per iteration; the host reads
+//            the records and both latest estimates, runs the trigger state
+//            machine and, where it fires, ETS / flip / OSD (OSD's |LLR| come
+//            from the marginals the kernel leaves in device memory, copied
+//            only for the trials that need them); trials that end at
+//            max_iter get the post-decode pass.
+// Final estimates, iteration counts and syndrome flags go back into the
+// chunk's planes, so the residual test and counters that follow treat these
+// trials like any other.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
+
+#include "jbp_ctx.h"
+#include "jbp_f64.h"
+
+namespace jbp {
+namespace {
+
+using namespace f64;
+
+// Per-slot state, in bytes from the slot's base: v2c[2] (4 doubles per
+// message row), c2v[2] (2 per row), marginals (4 doubles per qubit), then
+// the estimate and the previous estimate (one byte per qubit each).
+struct StepSlot {
+    double *v2c[2];
+    double *c2v[2];
+    double *tot;
+    uint8_t *est;
+    uint8_t *prev;
+};
+
+__host__ __device__ inline size_t step_slot_bytes(int E0, int E1, int n) {
+    size_t b = (6 * static_cast<size_t>(E0 + E1) + 4 * static_cast<size_t>(n)) * sizeof(double) +
+               2 * static_cast<size_t>(n);
+    return (b + 255) & ~static_cast<size_t>(255);
+}
+
+__device__ inline StepSlot step_slot(const DevGraph &g, char *state, size_t stride, int slot) {
+    double *base = reinterpret_cast<double *>(state + static_cast<size_t>(slot) * stride);
+    StepSlot s;
+    s.v2c[0] = base;
+    s.v2c[1] = s.v2c[0] + 4 * static_cast<size_t>(g.side[0].cap);
+    s.c2v[0] = s.v2c[1] + 4 * static_cast<size_t>(g.side[1].cap);
+    s.c2v[1] = s.c2v[0] + 2 * static_cast<size_t>(g.side[0].cap);
+    s.tot = s.c2v[1] + 2 * static_cast<size_t>(g.side[1].cap);
+    s.est = reinterpret_cast<uint8_t *>(s.tot + 4 * static_cast<size_t>(g.n));
+    s.prev = s.est + g.n;
+    return s;
+}
+
+// Record of one iteration: {unsat X, unsat Z, flags, 0}; flags bit 0 = the
+// estimate equals the previous one, bit 1 = equals the one before that.
+constexpr int kRecInts = 4;
+
+// Block b advances the trial of slot list[b] by up to `steps` iterations
+// (stopping when both syndromes match).  init: start from the reference's
+// initial messages.  want_tot: write every iteration's marginals (the slot
+// then holds the last iteration's).  At the end the block copies its
+// estimate and previous estimate to out[b].
+template <int MAXC, int MAXV, bool PAIR>
+__global__ __launch_bounds__(256, 2) void pp_step_kernel(DevGraph g, TrialPlanes tp, NoiseTab nt, const int *list,
+                               const int *slot_trial, int steps, int init, int want_tot,
+                               double damping, char *state, size_t stride, int *rec, uint8_t *out) {
+    __shared__ int s_cnt[2];
+    const int tid = threadIdx.x, bs = blockDim.x;
+    const int b = blockIdx.x;
+    const int slot = list[b];
+    const int trial = slot_trial[slot];
+    const StepSlot sp = step_slot(g, state, stride, slot);
+    const int mx = g.side[0].m, mz = g.side[1].m, n = g.n;
+    const double *pq = nt.prior + 4 * nt.entry(trial);
+    const M4 prior{{pq[0], pq[1], pq[2], pq[3]}};
+    const uint32_t *tsx = tp.syn_x + static_cast<size_t>(trial) * g.wm[0];
+    const uint32_t *tsz = tp.syn_z + static_cast<size_t>(trial) * g.wm[1];
+    if (init) {
+        for (int s = 0; s < 2; ++s)
+            for (int e = tid; e < g.side[s].cap; e += bs) {
+                store_v2c<PAIR>(sp.v2c[s], static_cast<size_t>(e), s, prior);
+                sp.c2v[s][2 * static_cast<size_t>(e)] = 0.25;
+                sp.c2v[s][2 * static_cast<size_t>(e) + 1] = 0.25;
+            }
+        for (int v = tid; v < n; v += bs) sp.est[v] = sp.prev[v] = 0;
+        __syncthreads();
+    }
+    for (int j = 0; j < steps; ++j) {
+        if (tid == 0) s_cnt[0] = s_cnt[1] = 0;
+        // one loop per side: a single loop branching between the sides
+        // keeps both inlined recursions' registers live
+        #pragma unroll 1
+        for (int c = tid; c < mx; c += bs)
+            if (0) check_update<MAXC, PAIR>(g.side[0], 0, c, (tsx[c >> 5] >> (c & 31)) & 1, sp.v2c[0], sp.c2v[0]);
+        #pragma unroll 1
+        for (int c = tid; c < mz; c += bs)
+            if (0) check_update<MAXC, PAIR>(g.side[1], 1, c, (tsz[c >> 5] >> (c & 31)) & 1, sp.v2c[1], sp.c2v[1]);
+        __syncthreads();
+        double *tot = want_tot ? sp.tot : nullptr;
+        int changed1 = 0, changed2 = 0;
+        for (int v = tid; v < n; v += bs) {
+            const int best = var_update<MAXV, PAIR>(g, v, prior, damping, sp.v2c, sp.c2v, tot);
+            const int e1 = sp.est[v], e2 = sp.prev[v];
+            changed1 |= best != e1;
+            changed2 |= best != e2;
+            sp.prev[v] = static_cast<uint8_t>(e1);
+            sp.est[v] = static_cast<uint8_t>(best);
+        }
+        __syncthreads();
+        int ux = 0, uz = 0;
+        for (int c = tid; c < mx + mz; c += bs) {
+            const int s = c < mx ? 0 : 1;
+            const int cc = c < mx ? c : c - mx;
+            const DevSide &sd = g.side[s];
+            int par = 0;
+            for (int e = sd.chk_off[cc]; e < sd.chk_off[cc + 1]; ++e) {
+                const int st = sp.est[sd.edge_var[e]];
+                par ^= s == 0 ? (st & 1) : (st >> 1);
+            }
+            const uint32_t *t = s == 0 ? tsx : tsz;
+            if (par != static_cast<int>((t[cc >> 5] >> (cc & 31)) & 1u)) {
+                if (s == 0) ++ux;
+                else ++uz;
+            }
+        }
+        if (ux) atomicAdd(&s_cnt[0], ux);
+        if (uz) atomicAdd(&s_cnt[1], uz);
+        const int any1 = __syncthreads_or(changed1);
+        const int any2 = __syncthreads_or(changed2);
+        const int usx = s_cnt[0], usz = s_cnt[1];
+        if (tid == 0) {
+            int *r = rec + (static_cast<size_t>(b) * steps + j) * kRecInts;
+            r[0] = usx;
+            r[1] = usz;
+            r[2] = (any1 ? 0 : 1) | (any2 ? 0 : 2);
+            r[3] = 0;
+        }
+        __syncthreads();
+        if (usx + usz == 0) break;
+    }
+    uint8_t *o = out + static_cast<size_t>(b) * 2 * n;
+    for (int v = tid; v < n; v += bs) {
+        o[v] = sp.est[v];
+        o[n + v] = sp.prev[v];
+    }
+}
+
+void launch_step(const DevGraph &g, const TrialPlanes &tp, const NoiseTab &nt, const int *list,
+                 int count, const int *slot_trial, int steps, bool init, bool want_tot,
+                 double damping, char *state, size_t stride, int *rec, uint8_t *out,
+                 hipStream_t st) {
+    const bool exact12 = g.max_cdeg <= 12 && g.max_vdeg <= 3;
+    const bool small = g.max_cdeg <= 16 && g.max_vdeg <= 4, pair = damping <= 0.0;
+    const int ini = init ? 1 : 0, wt = want_tot ? 1 : 0;
+    if (exact12 && pair)
+        hipLaunchKernelGGL((pp_step_kernel<12, 3, true>), dim3(count), dim3(256), 0, st, g, tp, nt,
+                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
+    else if (exact12)
+        hipLaunchKernelGGL((pp_step_kernel<12, 3, false>), dim3(count), dim3(256), 0, st, g, tp, nt,
+                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
+    else if (small && pair)
+        hipLaunchKernelGGL((pp_step_kernel<16, 4, true>), dim3(count), dim3(256), 0, st, g, tp, nt,
+                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
+    else if (small)
+        hipLaunchKernelGGL((pp_step_kernel<16, 4, false>), dim3(count), dim3(256), 0, st, g, tp, nt,
+                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
+    else if (pair)
+        hipLaunchKernelGGL((pp_step_kernel<32, 16, true>), dim3(count), dim3(256), 0, st, g, tp, nt,
+                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
+    else
+        hipLaunchKernelGGL((pp_step_kernel<32, 16, false>), dim3(count), dim3(256), 0, st, g, tp, nt,
+                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
+}
+
+// dst[k][0..len) = src[idx[k] * stride + off + 0..len), in 8-byte units
+// (the chunk syndromes of listed trials, the marginals of listed slots).
+__global__ void gather_kernel(const uint64_t *src, size_t stride, size_t off, int len,
+                              const int *idx, int count, uint64_t *dst) {
+    const int k = blockIdx.y;
+    if (k >= count) return;
+    const uint64_t *s = src + static_cast<size_t>(idx[k]) * stride + off;
+    uint64_t *d = dst + static_cast<size_t>(k) * len;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < len; i += gridDim.x * blockDim.x) d[i] = s[i];
+}
+
+// Writes the stage's outcome of listed trial k back into the chunk planes:
+// est words (X then Z, wn each), iterations and syndrome flag.
+__global__ void scatter_outcome_kernel(const uint32_t *planes, const int *iters, const uint8_t *ok,
+                                       const int *idx, int count, int wn, TrialPlanes tp) {
+    const int k = blockIdx.y;
+    if (k >= count) return;
+    const size_t t = static_cast<size_t>(idx[k]);
+    const uint32_t *px = planes + static_cast<size_t>(k) * 2 * wn;
+    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < wn; w += gridDim.x * blockDim.x) {
+        tp.est_x[t * wn + w] = px[w];
+        tp.est_z[t * wn + w] = px[wn + w];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        tp.iters[t] = iters[k];
+        tp.syn_ok[t] = ok[k];
+    }
+}
+
+void launch_gather(const void *src, size_t stride, size_t off, int len, const int *idx, int count,
+                   void *dst, hipStream_t st) {
+    if (count <= 0 || len <= 0) return;
+    const int bx = std::min(64, (len + 255) / 256);
+    hipLaunchKernelGGL(gather_kernel, dim3(bx, count), dim3(256), 0, st,
+                       static_cast<const uint64_t *>(src), stride, off, len, idx, count,
+                       static_cast<uint64_t *>(dst));
+}
+
+// Host worker pool: run(count, fn) calls fn(0..count-1) across the workers
+// and the calling thread and returns when all calls are done.
+class Pool {
+public:
+    explicit Pool(int threads) {
+        for (int i = 1; i < threads; ++i) th_.emplace_back([this] { work(); });
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    void run(int count, const std::function<void(int)> &fn) {
+        if (count <= 0) return;
+        if (th_.empty() || count == 1) {
+            for (int i = 0; i < count; ++i) fn(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            fn_ = &fn;
+            count_ = count;
+            next_.store(0);
+            pending_ = static_cast<int>(th_.size());
+            ++round_;
+        }
+        cv_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [this] { return pending_ == 0; });
+        fn_ = nullptr;
+    }
+
+private:
+    void drain() {
+        for (int i; (i = next_.fetch_add(1)) < count_;) (*fn_)(i);
+    }
+    void work() {
+        long seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || round_ != seen; });
+                if (stop_) return;
+                seen = round_;
+            }
+            drain();
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)> *fn_ = nullptr;
+    int count_ = 0;
+    std::atomic<int> next_{0};
+    int pending_ = 0;
+    long round_ = 0;
+    bool stop_ = false;
+};
+
+int pool_threads() {
+    unsigned hc = std::thread::hardware_concurrency();
+    return static_cast<int>(std::max(1u, std::min(hc ? hc : 1u, 16u)));
+}
+
+// Trials per batch of the stage (each holds about 3 MB of FP64 state for
+// the included code, 22 MB for HGP(150x200)), and the state cap per batch.
+constexpr int kPPSlots = 1024;
+constexpr size_t kPPStateBytes = size_t(8) << 30;
+
+void free_pp_buf(PPBuf &pb) {
+    for (hipEvent_t &e : pb.tot_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (void *p : {static_cast<void *>(pb.state), static_cast<void *>(pb.list),
+                    static_cast<void *>(pb.slot_trial), static_cast<void *>(pb.rec),
+                    static_cast<void *>(pb.out), static_cast<void *>(pb.idx), pb.stage})
+        if (p) (void)hipFree(p);
+    for (void *p : {static_cast<void *>(pb.h_rec), static_cast<void *>(pb.h_out),
+                    static_cast<void *>(pb.h_tot), pb.h_stage})
+        if (p) (void)hipHostFree(p);
+    pb = PPBuf();
+}
+
+int ensure_pp_buffers(jbp_ctx *ctx, PPBuf &pb, int slots, int rec_per_slot, char *err, size_t errlen) {
+    const DevGraph &g = ctx->dg;
+    const size_t stride = step_slot_bytes(g.side[0].cap, g.side[1].cap, g.n);
+    if (pb.cap >= slots && pb.rec_cap >= rec_per_slot && pb.stride == stride) return JBP_OK;
+    // Grow only (slots rounded up to 256, within the per-batch cap): a
+    // fresh pinned buffer's first copies run at a fraction of the warm
+    // rate, so calls of varying sizes keep the buffers they already have.
+    const bool same = pb.stride == stride;
+    const int batch_cap = static_cast<int>(std::max<size_t>(1, std::min<size_t>(kPPSlots, kPPStateBytes / stride)));
+    slots = std::max(std::min((slots + 255) / 256 * 256, std::max(slots, batch_cap)), same ? pb.cap : 0);
+    rec_per_slot = std::max(rec_per_slot, same ? pb.rec_cap : 0);
+    free_pp_buf(pb);
+    pb.cap = slots;
+    pb.rec_cap = std::max(rec_per_slot, 1);
+    pb.stride = stride;
+    HIPCHK(hipMalloc(&pb.state, stride * slots));
+    HIPCHK(hipMalloc(&pb.list, sizeof(int) * slots));
+    HIPCHK(hipMalloc(&pb.slot_trial, sizeof(int) * slots));
+    HIPCHK(hipMalloc(&pb.rec, sizeof(int) * kRecInts * static_cast<size_t>(slots) * pb.rec_cap));
+    HIPCHK(hipMalloc(&pb.out, 2 * static_cast<size_t>(g.n) * slots));
+    HIPCHK(hipMalloc(&pb.idx, sizeof(int) * slots));
+    // staging: syndromes (as 8-byte units), marginals of every slot, or the
+    // write-back planes + iterations + flags, whichever is largest
+    const size_t syn = 8 * static_cast<size_t>(slots) * ((g.wm[0] + 1) / 2 + (g.wm[1] + 1) / 2);
+    const size_t tot = 32 * static_cast<size_t>(slots) * g.n;
+    const size_t back = static_cast<size_t>(slots) * (8 * static_cast<size_t>(g.wn) + 8);
+    pb.stage_bytes = std::max(syn, std::max(tot, back));
+    HIPCHK(hipMalloc(&pb.stage, pb.stage_bytes));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&pb.h_rec),
+                         sizeof(int) * kRecInts * static_cast<size_t>(slots) * pb.rec_cap));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&pb.h_out), 2 * static_cast<size_t>(g.n) * slots));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&pb.h_tot), tot));
+    HIPCHK(hipHostMalloc(&pb.h_stage, std::max(syn, back)));
+    for (hipEvent_t &e : pb.tot_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return JBP_OK;
+}
+
+void unpack_bits(const uint32_t *w, int count, uint8_t *out) {
+    for (int i = 0; i < count; ++i) out[i] = static_cast<uint8_t>((w[i >> 5] >> (i & 31)) & 1u);
+}
+
+}  // namespace
+
+void free_pp(jbp_ctx *ctx) {
+    for (PPBuf &pb : ctx->ppb) free_pp_buf(pb);
+    if (ctx->pp_stream) (void)hipStreamDestroy(ctx->pp_stream);
+    ctx->pp_stream = nullptr;
+}
+
+// One batch of the stage: its trials' host state and the buffer set and
+// stream it uses.
+struct PPBatch {
+    size_t b0 = 0;
+    int K = 0;
+    PPBuf *pb = nullptr;
+    hipStream_t st = nullptr;
+    bool syn_wait = false;  // syndromes queued for the host, not yet unpacked
+    bool phase_a = false;   // phase A launched, results not yet read
+    std::vector<uint8_t> sx, sz;
+    std::vector<pp::Trial> tr;
+    std::vector<std::vector<uint8_t>> est;
+    std::vector<PPOutcome> o;
+    std::vector<int> active;
+    std::vector<pp::Attempt> att;
+    std::vector<uint8_t> need_llr, at_end, attempted;
+};
+
+int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int hist_window,
+           const std::vector<int> &trials, std::vector<PPOutcome> &outs, char *err, size_t errlen) {
+    const DevGraph &g = ctx->dg;
+    const pp::Graph &hg = ctx->pp_graph;
+    const pp::Tables &tabs = ctx->pp_tables;
+    const int n = g.n, mx = g.side[0].m, mz = g.side[1].m;
+    const int H = std::min(99, max_iter - 1);  // iterations before any trigger can fire
+    pp::Params prm;
+    prm.max_iter = max_iter;
+    prm.hist_window = hist_window;
+    outs.assign(trials.size(), PPOutcome());
+    if (trials.empty()) return JBP_OK;  // the usual case with long decodes: nothing to start
+    if (!ctx->pp_stream) HIPCHK(hipStreamCreate(&ctx->pp_stream));
+    Pool pool(pool_threads());
+    const size_t stride = step_slot_bytes(g.side[0].cap, g.side[1].cap, n);
+    const size_t batch = std::max<size_t>(1, std::min<size_t>(kPPSlots, kPPStateBytes / stride));
+    const size_t nbatch = (trials.size() + batch - 1) / batch;
+    hipStream_t streams[2] = {ctx->stream, ctx->pp_stream};
+
+    // Host state, the syndromes' copy to the host and (H > 0) phase A, all
+    // queued without waiting; finish_batch waits for them.
+    auto start = [&](PPBatch &B, size_t bi) -> int {
+        B = PPBatch();
+        B.b0 = bi * batch;
+        B.K = static_cast<int>(std::min<size_t>(batch, trials.size() - B.b0));
+        B.pb = &ctx->ppb[bi & 1];
+        B.st = streams[bi & 1];
+        // the write-back of the batch that used this set before
+        HIPCHK(hipStreamSynchronize(B.st));
+        const int K = B.K;
+        int rc = ensure_pp_buffers(ctx, *B.pb, K, std::max(H, 1), err, errlen);
+        if (rc) return rc;
+        PPBuf &pb = *B.pb;
+        hipStream_t st = B.st;
+        std::vector<int> slot_trial(trials.begin() + B.b0, trials.begin() + B.b0 + K);
+        HIPCHK(hipMemcpyAsync(pb.slot_trial, slot_trial.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
+        B.sx.resize(static_cast<size_t>(K) * mx);
+        B.sz.resize(static_cast<size_t>(K) * mz);
+        for (int s = 0; s < 2; ++s) {
+            const int wm = g.wm[s];
+            uint32_t *w = static_cast<uint32_t *>(pb.h_stage) + (s == 0 ? 0 : static_cast<size_t>(K) * g.wm[0]);
+            const uint32_t *src = s == 0 ? ctx->tp.syn_x : ctx->tp.syn_z;
+            if (wm % 2 == 0) {
+                launch_gather(src, wm / 2, 0, wm / 2, pb.slot_trial, K, pb.stage, st);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(w, pb.stage, 4 * static_cast<size_t>(K) * wm, hipMemcpyDeviceToHost, st));
+            } else {
+                for (int i = 0; i < K; ++i)
+                    HIPCHK(hipMemcpyAsync(w + static_cast<size_t>(i) * wm,
+                                          src + static_cast<size_t>(slot_trial[i]) * wm, 4 * wm,
+                                          hipMemcpyDeviceToHost, st));
+            }
+        }
+        B.syn_wait = true;
+        B.tr.assign(K, pp::Trial());
+        B.est.assign(K, std::vector<uint8_t>(n, 0));
+        B.o.assign(K, PPOutcome());
+        B.active.resize(K);
+        for (int i = 0; i < K; ++i) B.active[i] = i;
+        B.att.assign(K, pp::Attempt());
+        B.need_llr.assign(K, 0);
+        B.at_end.assign(K, 0);
+        B.attempted.assign(K, 0);
+        if (H > 0 && max_iter > 0) {
+            HIPCHK(hipMemcpyAsync(pb.list, B.active.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
+            launch_step(g, ctx->tp, nt, pb.list, K, pb.slot_trial, H, true, false, damping, pb.state,
+                        pb.stride, pb.rec, pb.out, st);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(pb.h_rec, pb.rec, sizeof(int) * static_cast<size_t>(K) * H * kRecInts,
+                                  hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(pb.h_out, pb.out, static_cast<size_t>(K) * 2 * n, hipMemcpyDeviceToHost, st));
+            B.phase_a = true;
+        }
+        return JBP_OK;
+    };
+
+    // Phase A's records, phase B, the repairs and the write-back.  kick()
+    // starts the next batch; it is called once this batch's last step has
+    // run and the marginals of its last iteration are gathered, so the
+    // next batch's phase A (which holds the device) runs during this
+    // batch's repairs, and the marginals travel to the host meanwhile.
+    auto finish_batch = [&](PPBatch &B, const std::function<int()> &kick) -> int {
+        bool kicked = false;
+        auto kick_once = [&]() -> int {
+            if (kicked) return JBP_OK;
+            kicked = true;
+            return kick();
+        };
+        const int K = B.K;
+        PPBuf &pb = *B.pb;
+        hipStream_t st = B.st;
+        auto &tr = B.tr;
+        auto &est = B.est;
+        auto &o = B.o;
+        auto &active = B.active;
+        auto &att = B.att;
+        auto &need_llr = B.need_llr;
+        auto &at_end = B.at_end;
+        auto &attempted = B.attempted;
+        const uint8_t *sx = B.sx.data(), *sz = B.sz.data();
+        auto finish = [&](int i) {
+            o[i].bp_ok = tr[i].bp_ok;
+            o[i].ok = tr[i].ok;
+            o[i].iterations = tr[i].iterations;
+            o[i].pp_success = tr[i].pp_success;
+            o[i].pp_ets = tr[i].pp_ets;
+            o[i].pp_flip = tr[i].pp_flip;
+            o[i].used_any = tr[i].used_any;
+            o[i].used6 = tr[i].used6;
+            o[i].used12 = tr[i].used12;
+        };
+        // Trial i ends unconverged at max_iter after an in-loop attempt at
+        // that same iteration failed.  The reference's post-decode pass then
+        // repeats exactly that attempt (ETS, the same flip sets, OSD with the
+        // same |LLR|) on the decoder's estimate, in place: it fails the same
+        // way and leaves the estimate the attempt built.  Take that result
+        // instead of computing it twice.
+        auto end_trial = [&](int i) {
+            tr[i].done = true;
+            tr[i].iterations = max_iter;
+            finish(i);
+            o[i].post_run = true;
+            o[i].post = pp::PostOut();
+            o[i].post.ets_applied = att[i].ets_used;
+            o[i].post.used6 = att[i].ets6;
+            o[i].post.used12 = att[i].ets12;
+            o[i].ok = false;
+            est[i].swap(att[i].trial);
+        };
+        // syndromes and phase A (queued by start)
+        HIPCHK(hipStreamSynchronize(st));
+        if (B.syn_wait) {
+            const uint32_t *w = static_cast<const uint32_t *>(pb.h_stage);
+            const uint32_t *wz = w + static_cast<size_t>(K) * g.wm[0];
+            pool.run(K, [&](int i) {
+                unpack_bits(w + static_cast<size_t>(i) * g.wm[0], mx, B.sx.data() + static_cast<size_t>(i) * mx);
+                unpack_bits(wz + static_cast<size_t>(i) * g.wm[1], mz, B.sz.data() + static_cast<size_t>(i) * mz);
+            });
+            B.syn_wait = false;
+        }
+        if (max_iter <= 0) {
+            // no iteration at all: the post-decode pass on the zero estimate
+            pool.run(K, [&](int i) {
+                o[i].iterations = max_iter < 0 ? 0 : max_iter;
+                o[i].post_run = true;
+                o[i].post = pp::post_decode(hg, tabs, prm, tr[i], est[i], sx + static_cast<size_t>(i) * mx,
+                                            sz + static_cast<size_t>(i) * mz, nullptr, nullptr);
+                o[i].ok = o[i].post.ok;
+            });
+            active.clear();
+        }
+        int rc = JBP_OK;
+        int *rec = pb.h_rec;
+        uint8_t *out = pb.h_out;
+        if (B.phase_a) {
+            B.phase_a = false;
+            pool.run(K, [&](int i) {
+                for (int t = 0; t < H; ++t) {
+                    const int *r = &rec[(static_cast<size_t>(i) * H + t) * kRecInts];
+                    pp::IterRec ir;
+                    ir.us[0] = r[0];
+                    ir.us[1] = r[1];
+                    ir.syn_all = r[0] == 0 && r[1] == 0;
+                    ir.same = r[2] & 1;
+                    ir.osc = (r[2] >> 1) & 1;
+                    if (pp::observe(hg, prm, tr[i], t, ir, nullptr, nullptr) == pp::Step::kConverged) {
+                        std::copy(&out[static_cast<size_t>(i) * 2 * n], &out[static_cast<size_t>(i) * 2 * n] + n,
+                                  est[i].begin());
+                        finish(i);
+                        break;
+                    }
+                }
+            });
+            active.erase(std::remove_if(active.begin(), active.end(), [&](int i) { return tr[i].done; }),
+                         active.end());
+        }
+        double *tot = pb.h_tot;
+        std::vector<int> pos(K);  // trial -> row of its marginals (last iteration)
+        for (int t = std::max(H, 0); t < max_iter && !active.empty(); ++t) {
+            const int A = static_cast<int>(active.size());
+            HIPCHK(hipMemcpyAsync(pb.list, active.data(), sizeof(int) * A, hipMemcpyHostToDevice, st));
+            launch_step(g, ctx->tp, nt, pb.list, A, pb.slot_trial, 1, t == 0, true, damping, pb.state,
+                        pb.stride, pb.rec, pb.out, st);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(rec, pb.rec, sizeof(int) * static_cast<size_t>(A) * kRecInts,
+                                  hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(out, pb.out, static_cast<size_t>(A) * 2 * n, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            // Marginals reach the host in chunks, one event each, so that the
+            // repairs of a chunk start as soon as it has arrived.  In the
+            // last iteration every active trial's are gathered now, before
+            // the next batch takes the device (most of them are needed);
+            // before that only those of the trials that reach OSD, below.
+            const bool last = t + 1 == max_iter;
+            const size_t row = 4 * static_cast<size_t>(n);
+            const size_t tot_off = 6 * static_cast<size_t>(g.side[0].cap + g.side[1].cap);
+            auto queue_marginals = [&](const int *idx, int count, int &per) -> int {
+                per = std::max(1, (count + PPBuf::kTotChunks - 1) / PPBuf::kTotChunks);
+                if (count <= 0) return JBP_OK;
+                launch_gather(pb.state, pb.stride / 8, tot_off, 4 * n, idx, count, pb.stage, st);
+                HIPCHK(hipGetLastError());
+                for (int c = 0; c * per < count; ++c) {
+                    const size_t w0 = static_cast<size_t>(c) * per, w1 = std::min<size_t>(count, w0 + per);
+                    HIPCHK(hipMemcpyAsync(tot + w0 * row, static_cast<const double *>(pb.stage) + w0 * row,
+                                          sizeof(double) * (w1 - w0) * row, hipMemcpyDeviceToHost, st));
+                    HIPCHK(hipEventRecord(pb.tot_ev[c], st));
+                }
+                return JBP_OK;
+            };
+            int per = 1;
+            if (last) {
+                if ((rc = queue_marginals(pb.list, A, per))) return rc;
+                if ((rc = kick_once())) return rc;
+                for (int a = 0; a < A; ++a) pos[active[a]] = a;
+            }
+            pool.run(A, [&](int a) {
+                const int i = active[a];
+                const int *r = &rec[static_cast<size_t>(a) * kRecInts];
+                const uint8_t *e_now = &out[static_cast<size_t>(a) * 2 * n];
+                const uint8_t *e_prev = e_now + n;
+                pp::IterRec ir;
+                ir.us[0] = r[0];
+                ir.us[1] = r[1];
+                ir.syn_all = r[0] == 0 && r[1] == 0;
+                ir.same = r[2] & 1;
+                ir.osc = (r[2] >> 1) & 1;
+                need_llr[i] = at_end[i] = attempted[i] = 0;
+                att[i].want_osd = false;
+                std::copy(e_now, e_now + n, est[i].begin());
+                const pp::Step step = pp::observe(hg, prm, tr[i], t, ir, e_now, t > 0 ? e_prev : nullptr);
+                if (step == pp::Step::kConverged) {
+                    finish(i);
+                    return;
+                }
+                const uint8_t *tsx = sx + static_cast<size_t>(i) * mx, *tsz = sz + static_cast<size_t>(i) * mz;
+                if (step == pp::Step::kAttempt) {
+                    attempted[i] = 1;
+                    pp::attempt_begin(hg, tabs, prm, tr[i], est[i], tsx, tsz, att[i]);
+                    if (pp::attempt_end(tr[i], att[i], t, est[i])) {
+                        finish(i);
+                        return;
+                    }
+                    need_llr[i] = att[i].want_osd;
+                }
+                if (t + 1 == max_iter) {
+                    at_end[i] = 1;
+                    if (!attempted[i]) {
+                        // the post-decode pass runs itself (no attempt to reuse):
+                        // ETS and flip sets now, |LLR| only if OSD follows
+                        tr[i].iterations = max_iter;
+                        o[i].post_run = true;
+                        o[i].post = pp::post_begin(hg, tabs, prm, tr[i], est[i], tsx, tsz);
+                        need_llr[i] = o[i].post.want_osd;
+                        if (!need_llr[i]) {
+                            tr[i].done = true;
+                            finish(i);
+                            o[i].ok = o[i].post.ok;
+                            return;
+                        }
+                    }
+                }
+                if (at_end[i] && !need_llr[i]) end_trial(i);
+            });
+            // marginals of the trials that need |LLR| (OSD now or after the decode)
+            std::vector<int> want;
+            for (int i : active)
+                if (!tr[i].done && need_llr[i]) want.push_back(i);
+            const int W = static_cast<int>(want.size());
+            if (!last && W > 0) {
+                HIPCHK(hipMemcpyAsync(pb.idx, want.data(), sizeof(int) * W, hipMemcpyHostToDevice, st));
+                if ((rc = queue_marginals(pb.idx, W, per))) return rc;
+            }
+            std::atomic<int> ev_err{0};
+            pool.run(W, [&](int w) {
+                const int i = want[w];
+                const int r = last ? pos[i] : w;  // row of the trial's marginals
+                if (hipEventSynchronize(pb.tot_ev[r / per]) != hipSuccess) {
+                    ev_err.store(1);
+                    return;
+                }
+                // OSD takes the marginals and computes the |LLR| it needs
+                const pp::LlrSide lx = pp::LlrSide::marginals(tot + static_cast<size_t>(r) * row), lz = lx;
+                const uint8_t *tsx = sx + static_cast<size_t>(i) * mx, *tsz = sz + static_cast<size_t>(i) * mz;
+                if (att[i].want_osd) {
+                    pp::attempt_osd(hg, att[i], tsx, tsz, lx, lz);
+                    if (pp::attempt_end(tr[i], att[i], t, est[i])) {
+                        finish(i);
+                        return;
+                    }
+                }
+                if (at_end[i]) {
+                    if (attempted[i]) {
+                        end_trial(i);
+                    } else {
+                        tr[i].done = true;
+                        finish(i);
+                        pp::post_osd(hg, o[i].post, est[i], tsx, tsz, lx, lz);
+                        o[i].ok = o[i].post.ok;
+                    }
+                }
+            });
+            if (ev_err.load()) {
+                set_err(err, errlen, "waiting for the marginals' copy to the host failed");
+                return JBP_ERR_DEVICE;
+            }
+            active.erase(std::remove_if(active.begin(), active.end(), [&](int i) { return tr[i].done; }),
+                         active.end());
+        }
+        if ((rc = kick_once())) return rc;
+        // write back: estimate planes, iterations, syndrome flag.  Queued
+        // without a wait: the host reads nothing it writes before the final
+        // synchronisation, and this buffer set's next use is ordered after
+        // it on the same stream.
+        const size_t nplane = static_cast<size_t>(K) * 2 * g.wn;
+        uint32_t *planes = static_cast<uint32_t *>(pb.h_stage);
+        int *it = reinterpret_cast<int *>(planes + nplane);
+        uint8_t *okb = reinterpret_cast<uint8_t *>(it + K);
+        pool.run(K, [&](int i) {
+            uint32_t *px = planes + static_cast<size_t>(i) * 2 * g.wn, *pz = px + g.wn;
+            std::fill(px, px + 2 * g.wn, 0u);
+            for (int v = 0; v < n; ++v) {
+                px[v >> 5] |= static_cast<uint32_t>(est[i][v] & 1) << (v & 31);
+                pz[v >> 5] |= static_cast<uint32_t>((est[i][v] >> 1) & 1) << (v & 31);
+            }
+            it[i] = o[i].iterations;
+            okb[i] = o[i].ok ? 1 : 0;
+        });
+        for (int i = 0; i < K; ++i) outs[B.b0 + i] = o[i];
+        char *stage = static_cast<char *>(pb.stage);
+        uint32_t *d_planes = reinterpret_cast<uint32_t *>(stage);
+        int *d_it = reinterpret_cast<int *>(stage + 4 * nplane);
+        uint8_t *d_ok = reinterpret_cast<uint8_t *>(d_it + K);
+        HIPCHK(hipMemcpyAsync(d_planes, planes, 4 * nplane + 5 * static_cast<size_t>(K), hipMemcpyHostToDevice, st));
+        const int bx = std::min(16, (g.wn + 255) / 256);
+        hipLaunchKernelGGL(scatter_outcome_kernel, dim3(bx, K), dim3(256), 0, st, d_planes, d_it, d_ok,
+                           pb.slot_trial, K, g.wn, ctx->tp);
+        HIPCHK(hipGetLastError());
+        return JBP_OK;
+    };
+
+    // Batches alternate between the two buffer sets and streams; batch
+    // b + 1 starts from inside batch b's finish (kick).
+    PPBatch cur, next;
+    int rc = start(cur, 0);
+    if (rc) return rc;
+    for (size_t bi = 0; bi < nbatch; ++bi) {
+        auto kick = [&]() -> int { return bi + 1 < nbatch ? start(next, bi + 1) : JBP_OK; };
+        if ((rc = finish_batch(cur, kick))) return rc;
+        std::swap(cur, next);
+    }
+    HIPCHK(hipStreamSynchronize(streams[0]));
+    HIPCHK(hipStreamSynchronize(streams[1]));
+    return JBP_OK;
+}
+
+}  // namespace jbp
+
+extern "C" int jbp_set_pp_tables(jbp_ctx *ctx, const jbp_pp_tables *t, char *err, size_t errlen) {
+    if (!ctx) {
+        jbp::set_err(err, errlen, "bad arguments");
+        return JBP_ERR_ARG;
+    }
+    ctx->pp_tables = t ? t->t : jbp::pp::Tables();
+    return JBP_OK;
+}
+
+extern "C" int jbp_decode_batch_pp(jbp_ctx *ctx, int count, const uint32_t *sx_bits,
+                                   const uint32_t *sz_bits, const double prior[4], int max_iter,
+                                   double damping, int precision, int flip_hist,
+                                   int *iterations_out, uint8_t *syn_match_out, uint8_t *pp_out,
+                                   int *est_out, char *err, size_t errlen) {
+    using namespace jbp;
+    if (!ctx || count < 0 || !sx_bits || !sz_bits || !prior || max_iter < 0 || flip_hist < 0) {
+        set_err(err, errlen, "bad arguments");
+        return JBP_ERR_ARG;
+    }
+    if (count == 0) return JBP_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = ensure_chunk(ctx, count, err, errlen);
+    if (rc) return rc;
+    const DevGraph &g = ctx->dg;
+    HIPCHK(hipMemcpyAsync(ctx->tp.syn_x, sx_bits, static_cast<size_t>(count) * g.wm[0] * 4,
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->tp.syn_z, sz_bits, static_cast<size_t>(count) * g.wm[1] * 4,
+                          hipMemcpyHostToDevice, ctx->stream));
+    {
+        const Prior pri = make_prior(prior);
+        if ((rc = set_noise(ctx, &pri, nullptr, 1, err, errlen))) return rc;
+    }
+    const int h = std::min(99, max_iter - 1);
+    std::vector<uint8_t> ok(count, 0);
+    if (h > 0) {
+        rc = run_decode(ctx, count, ctx->noise, h, 0, 1, damping, precision, err, errlen);
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(ok.data(), ctx->tp.syn_ok, count, hipMemcpyDeviceToHost));
+    }
+    std::vector<int> list;
+    for (int i = 0; i < count; ++i)
+        if (h <= 0 || !ok[i]) list.push_back(i);
+    std::vector<PPOutcome> outs;
+    rc = run_pp(ctx, ctx->noise, max_iter, damping, flip_hist, list, outs, err, errlen);
+    if (rc) return rc;
+    if (pp_out) {
+        for (int i = 0; i < count; ++i) pp_out[i] = JBP_PPF_BP_OK;
+        for (size_t k = 0; k < list.size(); ++k) {
+            const PPOutcome &o = outs[k];
+            const bool ets = o.pp_ets || (o.post_run && o.post.ets_ok);
+            const bool flip = o.pp_flip || (o.post_run && o.post.flip_ok);
+            const bool osd = (o.post_run && o.post.osd_ok) || (o.pp_success && !o.pp_ets && !o.pp_flip);
+            const bool any = o.pp_success || ets || flip || osd;
+            const bool u6 = o.used6 || (o.post_run && o.post.used6);
+            const bool u12 = o.used12 || (o.post_run && o.post.used12);
+            const bool used = o.used_any || u6 || u12 || (o.post_run && o.post.ets_applied);
+            pp_out[list[k]] = static_cast<uint8_t>(
+                (o.bp_ok ? JBP_PPF_BP_OK : 0) | (any && o.ok ? JBP_PPF_PP_SUCCESS : 0) |
+                (ets && o.ok ? JBP_PPF_ETS_SAVED : 0) | (flip && o.ok ? JBP_PPF_FLIP_SAVED : 0) |
+                (osd && o.ok ? JBP_PPF_OSD_SAVED : 0) | (used ? JBP_PPF_ETS_USED : 0) |
+                (u6 ? JBP_PPF_ETS6 : 0) | (u12 ? JBP_PPF_ETS12 : 0));
+        }
+    }
+    if (iterations_out)
+        HIPCHK(hipMemcpy(iterations_out, ctx->tp.iters, static_cast<size_t>(count) * 4,
+                         hipMemcpyDeviceToHost));
+    if (syn_match_out)
+        HIPCHK(hipMemcpy(syn_match_out, ctx->tp.syn_ok, static_cast<size_t>(count),
+                         hipMemcpyDeviceToHost));
+    if (est_out) {
+        if ((rc = ensure_states(ctx, static_cast<size_t>(count) * g.n, err, errlen))) return rc;
+        launch_unpack_states(ctx->tp.est_x, ctx->tp.est_z, g.wn, g.n, count, ctx->d_states,
+                             ctx->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(est_out, ctx->d_states, static_cast<size_t>(count) * g.n * 4,
+                              hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    return JBP_OK;
+}
+
+extern "C" int jbp_decode_llr(jbp_ctx *ctx, int count, const uint32_t *sx_bits, const uint32_t *sz_bits,
+                              const double prior[4], int max_iter, double damping,
+                              int *iterations_out, uint8_t *syn_match_out, int *est_out,
+                              double *llr_out, char *err, size_t errlen) {
+    using namespace jbp;
+    if (!ctx || count < 0 || !sx_bits || !sz_bits || !prior || max_iter < 1 || damping < 0.0) {
+        set_err(err, errlen, "bad arguments");
+        return JBP_ERR_ARG;
+    }
+    if (count == 0) return JBP_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = ensure_chunk(ctx, count, err, errlen);
+    if (rc) return rc;
+    const DevGraph &g = ctx->dg;
+    const int n = g.n;
+    HIPCHK(hipMemcpyAsync(ctx->tp.syn_x, sx_bits, static_cast<size_t>(count) * g.wm[0] * 4,
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->tp.syn_z, sz_bits, static_cast<size_t>(count) * g.wm[1] * 4,
+                          hipMemcpyHostToDevice, ctx->stream));
+    {
+        const Prior pri = make_prior(prior);
+        if ((rc = set_noise(ctx, &pri, nullptr, 1, err, errlen))) return rc;
+    }
+    hipStream_t st = ctx->stream;
+    const size_t batch = std::max<size_t>(
+        1, std::min<size_t>(kPPSlots, kPPStateBytes / step_slot_bytes(g.side[0].cap, g.side[1].cap, n)));
+    for (int b0 = 0; b0 < count; b0 += static_cast<int>(batch)) {
+        const int K = static_cast<int>(std::min<size_t>(batch, static_cast<size_t>(count - b0)));
+        if ((rc = ensure_pp_buffers(ctx, ctx->ppb[0], K, max_iter, err, errlen))) return rc;
+        PPBuf &pb = ctx->ppb[0];
+        std::vector<int> ids(K);
+        for (int i = 0; i < K; ++i) ids[i] = i;
+        std::vector<int> trials(K);
+        for (int i = 0; i < K; ++i) trials[i] = b0 + i;
+        HIPCHK(hipMemcpyAsync(pb.list, ids.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(pb.slot_trial, trials.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
+        // every iteration in one launch, marginals kept (the last one stays)
+        launch_step(g, ctx->tp, ctx->noise, pb.list, K, pb.slot_trial, max_iter, true, true, damping,
+                    pb.state, pb.stride, pb.rec, pb.out, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(pb.h_rec, pb.rec, sizeof(int) * kRecInts * static_cast<size_t>(K) * max_iter,
+                              hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(pb.h_out, pb.out, static_cast<size_t>(K) * 2 * n, hipMemcpyDeviceToHost, st));
+        launch_gather(pb.state, pb.stride / 8, 6 * static_cast<size_t>(g.side[0].cap + g.side[1].cap),
+                      4 * n, pb.list, K, pb.stage, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(pb.h_tot, pb.stage, sizeof(double) * static_cast<size_t>(K) * 4 * n,
+                              hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int i = 0; i < K; ++i) {
+            int iters = max_iter, match = 0;
+            for (int t = 0; t < max_iter; ++t) {
+                const int *r = pb.h_rec + (static_cast<size_t>(i) * max_iter + t) * kRecInts;
+                if (r[0] == 0 && r[1] == 0) {
+                    iters = t + 1;
+                    match = 1;
+                    break;
+                }
+            }
+            const size_t k = static_cast<size_t>(b0 + i);
+            if (iterations_out) iterations_out[k] = iters;
+            if (syn_match_out) syn_match_out[k] = static_cast<uint8_t>(match);
+            if (est_out)
+                for (int v = 0; v < n; ++v) est_out[k * n + v] = pb.h_out[static_cast<size_t>(i) * 2 * n + v];
+            if (llr_out)
+                pp::abs_llr_from_marginals(n, pb.h_tot + static_cast<size_t>(i) * 4 * n, llr_out + k * 2 * n,
+                                           llr_out + k * 2 * n + n);
+        }
+    }
+    return JBP_OK;
+}
