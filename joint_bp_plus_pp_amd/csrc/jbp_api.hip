@@ -1069,10 +1069,11 @@ int jbp_create(const jbp_graph_desc *g, int device_id, jbp_ctx **out, char *err,
         const HostSide &h = ctx->hs[s];
         DevSide &d = dg.side[s];
         d.cap = std::max(1, dg.max_cdeg * h.m);
-        std::vector<int> row(std::max(h.E, 1), 0);
+        std::vector<int> row(std::max(h.E, 1), 0), vrow(std::max(h.E, 1), 0);
         for (int c = 0; c < h.m; ++c)
             for (int i = h.chk_off[c]; i < h.chk_off[c + 1]; ++i) row[i] = (i - h.chk_off[c]) * h.m + c;
-        if ((rc = dev_upload(ctx, row, &d.eslot, err, errlen))) return fail(rc);
+        for (int k = 0; k < h.E; ++k) vrow[k] = row[h.var_edge[k]];
+        if ((rc = dev_upload(ctx, vrow, &d.vrow, err, errlen))) return fail(rc);
     }
     dg.vpad_w = dg.max_vdeg <= 4 ? 4 : dg.max_vdeg <= 8 ? 8 : 16;
     dg.cpad_w = dg.max_cdeg <= 8 ? 8 : dg.max_cdeg <= 12 ? 12 : dg.max_cdeg <= 16 ? 16 : 32;

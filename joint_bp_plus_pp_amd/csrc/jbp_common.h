@@ -20,7 +20,7 @@ struct DevSide {
     // row i*m + c, so a wavefront walking checks c..c+63 at position i reads
     // one contiguous span (rows = max check degree x m; unused rows idle)
     int cap = 0;
-    const int *eslot = nullptr;     // E: row of each edge
+    const int *vrow = nullptr;      // E: row of each entry of the variable lists (var_edge order)
 };
 
 struct DevGraph {

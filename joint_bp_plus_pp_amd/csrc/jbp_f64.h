@@ -128,29 +128,60 @@ __device__ __forceinline__ M4 c2v_msg(const double *c2v, int r, int side) {
     return side == 0 ? M4{{a, b, a, b}} : M4{{a, a, b, b}};
 }
 
+__device__ __forceinline__ M4 c2v_pair(double a, double b, int side) {
+    return side == 0 ? M4{{a, b, a, b}} : M4{{a, a, b, b}};
+}
+
 // Joint variable update + hard decision (jointbp_ets.cpp:2451-2519).  The
 // suffix products are formed on the fly while walking the edges backwards;
 // each outgoing message uses the same operands as the reference's
 // pref[i] (*) suf[i+1].
 //
 // Messages: v2c[s] (4 doubles per edge), c2v[s] (2 per edge).  Returns the
-// hard decision; tot (optional) receives the normalised marginal.
+// hard decision; tot (optional) receives the normalised marginal.  Up to
+// degree 4 the incoming messages are all loaded first (positions past the
+// degree load the side's row 0 and are ignored) and kept in registers.
 template <int MAXV, bool PAIR>
 __device__ int var_update(const DevGraph &g, int v, const M4 &prior, double damping,
                           double *const *v2c, const double *const *c2v, double *tot) {
+    constexpr bool kBatch = MAXV <= 4;
     const M4 ident{{1.0, 1.0, 1.0, 1.0}};
+    constexpr int NB = kBatch ? MAXV : 1;
     int b[2], deg[2];
-    M4 pref[2][MAXV + 1];
+    int row[2][NB];
+    double ca[2][NB], cb[2][NB];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
         b[s] = g.side[s].var_off[v];
         deg[s] = g.side[s].var_off[v + 1] - b[s];
+    }
+    if (kBatch) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int i = 0; i < NB; ++i) row[s][i] = g.side[s].vrow[i < deg[s] ? b[s] + i : 0];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                const double2 m = *reinterpret_cast<const double2 *>(c2v[s] + 2 * static_cast<size_t>(row[s][i]));
+                ca[s][i] = m.x;
+                cb[s][i] = m.y;
+            }
+    }
+    // row and incoming message of position i on side s
+    auto erow = [&](int s, int i) -> int { return kBatch ? row[s][kBatch ? i : 0] : g.side[s].vrow[b[s] + i]; };
+    auto msg = [&](int s, int i) -> M4 {
+        if (kBatch) return c2v_pair(ca[s][kBatch ? i : 0], cb[s][kBatch ? i : 0], s);
+        return c2v_msg(c2v[s], erow(s, i), s);
+    };
+    M4 pref[2][MAXV + 1];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
         pref[s][0] = ident;
 #pragma unroll
         for (int i = 0; i < MAXV; ++i)
-            if (i < deg[s])
-                pref[s][i + 1] = mul4(pref[s][i],
-                                      c2v_msg(c2v[s], g.side[s].eslot[g.side[s].var_edge[b[s] + i]], s));
+            if (i < deg[s]) pref[s][i + 1] = mul4(pref[s][i], msg(s, i));
     }
     M4 allx = ident, allz = ident;
 #pragma unroll
@@ -174,7 +205,7 @@ __device__ int var_update(const DevGraph &g, int v, const M4 &prior, double damp
 #pragma unroll
     for (int i = MAXV - 1; i >= 0; --i) {
         if (i < deg[0]) {
-            int e = g.side[0].eslot[g.side[0].var_edge[b[0] + i]];
+            const int e = erow(0, i);
             M4 out = mul4(mul4(prior, mul4(pref[0][i], suf)), allz);
             normalize4(out);
             if (!PAIR && damping > 0.0) {
@@ -183,7 +214,7 @@ __device__ int var_update(const DevGraph &g, int v, const M4 &prior, double damp
                 out = mix4(out, old, damping);
             }
             store_v2c<PAIR>(v2c[0], static_cast<size_t>(e), 0, out);
-            suf = mul4(suf, c2v_msg(c2v[0], e, 0));
+            suf = mul4(suf, msg(0, i));
         }
     }
     // Z edges: ((prior (*) x_all) (*) z_excl)
@@ -192,7 +223,7 @@ __device__ int var_update(const DevGraph &g, int v, const M4 &prior, double damp
 #pragma unroll
     for (int i = MAXV - 1; i >= 0; --i) {
         if (i < deg[1]) {
-            int e = g.side[1].eslot[g.side[1].var_edge[b[1] + i]];
+            const int e = erow(1, i);
             M4 out = mul4(px, mul4(pref[1][i], suf));
             normalize4(out);
             if (!PAIR && damping > 0.0) {
@@ -201,7 +232,7 @@ __device__ int var_update(const DevGraph &g, int v, const M4 &prior, double damp
                 out = mix4(out, old, damping);
             }
             store_v2c<PAIR>(v2c[1], static_cast<size_t>(e), 1, out);
-            suf = mul4(suf, c2v_msg(c2v[1], e, 1));
+            suf = mul4(suf, msg(1, i));
         }
     }
     return best;
