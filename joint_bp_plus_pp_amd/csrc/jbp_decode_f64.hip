@@ -95,18 +95,10 @@ __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPla
             __syncthreads();
             // both syndromes of est (jointbp_ets.cpp:2521-2525)
             int bad = 0;
-            for (int c = tid; c < mx + mz && !bad; c += bs) {
-                int s = c < mx ? 0 : 1;
-                int cc = c < mx ? c : c - mx;
-                const DevSide &sd = g.side[s];
-                int par = 0;
-                for (int e = sd.chk_off[cc]; e < sd.chk_off[cc + 1]; ++e) {
-                    int st = sp.est[sd.edge_var[e]];
-                    par ^= s == 0 ? (st & 1) : (st >> 1);
-                }
-                const uint32_t *t = s == 0 ? tsx : tsz;
-                bad |= par != static_cast<int>((t[cc >> 5] >> (cc & 31)) & 1u);
-            }
+            for (int c = tid; c < mx && !bad; c += bs)
+                bad |= check_parity<MAXC>(g.side[0], 0, c, sp.est) != static_cast<int>((tsx[c >> 5] >> (c & 31)) & 1u);
+            for (int c = tid; c < mz && !bad; c += bs)
+                bad |= check_parity<MAXC>(g.side[1], 1, c, sp.est) != static_cast<int>((tsz[c >> 5] >> (c & 31)) & 1u);
             if (!__syncthreads_or(bad)) {
                 iterations = iter + 1;
                 matched = 1;

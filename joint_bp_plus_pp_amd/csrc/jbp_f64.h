@@ -238,5 +238,20 @@ __device__ int var_update(const DevGraph &g, int v, const M4 &prior, double damp
     return best;
 }
 
+// Parity of check c of one side over est (X checks read the X bit, Z checks
+// the Z bit), its variables' states loaded together.
+template <int MAXC>
+__device__ __forceinline__ int check_parity(const DevSide &sd, int side, int c, const uint8_t *est) {
+    const int b = sd.chk_off[c], deg = sd.chk_off[c + 1] - b;
+    int st[MAXC];
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) st[i] = est[sd.edge_var[i < deg ? b + i : 0]];
+    int par = 0;
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i)
+        if (i < deg) par ^= side == 0 ? (st[i] & 1) : (st[i] >> 1);
+    return par;
+}
+
 }  // namespace f64
 }  // namespace jbp

@@ -128,21 +128,10 @@ __global__ __launch_bounds__(256, (MAXC <= 12 && PAIR) ? 4 : 2) void pp_step_ker
         }
         __syncthreads();
         int ux = 0, uz = 0;
-        for (int c = tid; c < mx + mz; c += bs) {
-            const int s = c < mx ? 0 : 1;
-            const int cc = c < mx ? c : c - mx;
-            const DevSide &sd = g.side[s];
-            int par = 0;
-            for (int e = sd.chk_off[cc]; e < sd.chk_off[cc + 1]; ++e) {
-                const int st = sp.est[sd.edge_var[e]];
-                par ^= s == 0 ? (st & 1) : (st >> 1);
-            }
-            const uint32_t *t = s == 0 ? tsx : tsz;
-            if (par != static_cast<int>((t[cc >> 5] >> (cc & 31)) & 1u)) {
-                if (s == 0) ++ux;
-                else ++uz;
-            }
-        }
+        for (int c = tid; c < mx; c += bs)
+            ux += check_parity<MAXC>(g.side[0], 0, c, sp.est) != static_cast<int>((tsx[c >> 5] >> (c & 31)) & 1u);
+        for (int c = tid; c < mz; c += bs)
+            uz += check_parity<MAXC>(g.side[1], 1, c, sp.est) != static_cast<int>((tsz[c >> 5] >> (c & 31)) & 1u);
         if (ux) atomicAdd(&s_cnt[0], ux);
         if (uz) atomicAdd(&s_cnt[1], uz);
         const int any1 = __syncthreads_or(changed1);
