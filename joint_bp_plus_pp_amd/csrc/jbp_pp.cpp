@@ -74,28 +74,6 @@ bool node_id(const std::string &tok, char tag, long long P, long long &id) {
     return true;
 }
 
-std::vector<uint8_t> syndrome_of(const Graph &g, int side, const std::vector<uint8_t> &est) {
-    std::vector<uint8_t> s(g.m[side]);
-    syndrome(g, side, est.data(), s.data());
-    return s;
-}
-
-bool matches(const Graph &g, int side, const std::vector<uint8_t> &est, const uint8_t *target) {
-    const auto &off = g.coff[side];
-    const auto &var = g.cvar[side];
-    for (int c = 0; c < g.m[side]; ++c) {
-        int par = 0;
-        for (int e = off[c]; e < off[c + 1]; ++e) par ^= part(side, est[var[e]]);
-        if (par != target[c]) return false;
-    }
-    return true;
-}
-
-bool both_match(const Graph &g, const std::vector<uint8_t> &est, const uint8_t *sx,
-                const uint8_t *sz) {
-    return matches(g, 0, est, sx) && matches(g, 1, est, sz);
-}
-
 // Solution of rows * x = rhs over GF(2) for at most 32 unknowns (bit i of a
 // row = unknown i), free unknowns set to 0; false when inconsistent.
 bool solve32(std::vector<uint32_t> rows, std::vector<uint8_t> rhs, int nv, uint32_t &x) {
@@ -201,10 +179,22 @@ inline const int *col_rows(const Graph &g, int side, int v, int &count) {
     return g.vchk[side].data() + g.voff[side][v];
 }
 
+// Variable v's side bit toggled: the bits of its checks flip in the
+// syndrome difference.
+inline void diff_toggle(const Graph &g, int side, SynDiff &sd, int v) {
+    int cnt = 0;
+    const int *rows = col_rows(g, side, v, cnt);
+    for (int k = 0; k < cnt; ++k) {
+        uint8_t &b = sd.d[rows[k]];
+        b ^= 1;
+        sd.ones += b ? 1 : -1;
+    }
+}
+
 // One table of ETS repair (jointbp_ets.cpp:3525-3642).
-// (diff = syndrome of est XOR target, whose two set checks form `key`)
-bool ets_one(const Graph &g, const EtsSide &es, int side, const uint8_t *target,
-             const std::vector<uint8_t> &diff, uint64_t key, std::vector<uint8_t> &est) {
+// (diff: est's syndrome difference, whose two set checks form `key`)
+bool ets_one(const Graph &g, const EtsSide &es, int side, SynDiff &diff, uint64_t key,
+             std::vector<uint8_t> &est) {
     const int nv = es.nv;
     if (es.count() == 0 || nv <= 0 || nv > 32) return false;
     auto it = es.by_pair.find(key);
@@ -239,17 +229,20 @@ bool ets_one(const Graph &g, const EtsSide &es, int side, const uint8_t *target,
             }
             if (!mask) continue;
             rows.push_back(mask);
-            rhs.push_back(diff[c]);
+            rhs.push_back(diff.d[c]);
         }
         uint32_t x = 0;
         if (!solve32(rows, rhs, nv, x)) continue;
-        std::vector<uint8_t> trial = est;
+        // the entry's flips must clear the whole difference
         for (int i = 0; i < nv; ++i)
-            if (x & (1u << i)) toggle(side, trial[vars[i]]);
-        if (matches(g, side, trial, target)) {
-            est.swap(trial);
+            if (x & (1u << i)) diff_toggle(g, side, diff, vars[i]);
+        if (diff.ones == 0) {
+            for (int i = 0; i < nv; ++i)
+                if (x & (1u << i)) toggle(side, est[vars[i]]);
             return true;
         }
+        for (int i = 0; i < nv; ++i)
+            if (x & (1u << i)) diff_toggle(g, side, diff, vars[i]);
     }
     return false;
 }
@@ -293,26 +286,20 @@ std::vector<int> window_union(const std::deque<std::vector<int>> &win, int n) {
 // The flip-set loops of both sides (jointbp_ets.cpp:2951-2978 and
 // 6144-6166): the first set that repairs a side ends that side's loop.
 bool flip_pass(const Graph &g, const Params &prm, const Trial &tr, std::vector<uint8_t> &est,
-               const uint8_t *sx, const uint8_t *sz) {
+               SynDiff (&diff)[2]) {
     bool any = false;
     for (int side = 0; side < 2; ++side) {
-        const uint8_t *target = side == 0 ? sx : sz;
         std::vector<int> uni;
         if (tr.hist_on && prm.hist_window > 0) uni = window_union(tr.win[side], g.n);
         // est only changes when a set applies, which ends this side's loop
-        std::vector<uint8_t> diff;
         const std::vector<int> *tried = nullptr;
         for (const std::vector<int> *set : flip_sets(uni, tr.win[side], tr.last[side])) {
             if (set->empty()) continue;
             // the same set again (the union of a one-entry history is that
             // entry) fails again
             if (tried && *tried == *set) continue;
-            if (diff.empty()) {
-                diff = syndrome_of(g, side, est);
-                for (int c = 0; c < g.m[side]; ++c) diff[c] ^= target[c];
-            }
             tried = set;
-            if (flip_fix_diff(g, side, *set, target, diff, est)) {
+            if (flip_fix_diff(g, side, *set, diff[side], est)) {
                 any = true;
                 break;
             }
@@ -414,24 +401,40 @@ int unsatisfied(const Graph &g, int side, const uint8_t *est, const uint8_t *tar
     return count;
 }
 
+void syn_diff_init(const Graph &g, int side, const uint8_t *est, const uint8_t *target, SynDiff &sd) {
+    const int m = g.m[side];
+    sd.d.resize(m);
+    syndrome(g, side, est, sd.d.data());
+    int ones = 0;
+    for (int c = 0; c < m; ++c) {
+        sd.d[c] ^= target[c] ? 1 : 0;
+        ones += sd.d[c];
+    }
+    sd.ones = ones;
+}
+
 EtsFix ets_fix(const Graph &g, const Tables &t, int side, const uint8_t *target,
                std::vector<uint8_t> &est) {
+    if (t.groups.empty()) return EtsFix();
+    SynDiff sd;
+    syn_diff_init(g, side, est.data(), target, sd);
+    return ets_fix_diff(g, t, side, sd, est);
+}
+
+EtsFix ets_fix_diff(const Graph &g, const Tables &t, int side, SynDiff &diff, std::vector<uint8_t> &est) {
     EtsFix r;
     if (t.groups.empty()) return r;
     // every table needs exactly two unsatisfied checks, and est only changes
     // when a table applies (which ends the search)
-    std::vector<uint8_t> diff = syndrome_of(g, side, est);
-    int unsat[3], nu = 0;
-    for (int c = 0; c < g.m[side]; ++c) {
-        diff[c] ^= target[c];
-        if (diff[c] && nu < 3) unsat[nu++] = c;
-    }
-    if (nu != 2) return r;
+    if (diff.ones != 2) return r;
+    int unsat[2], nu = 0;
+    for (int c = 0; c < g.m[side] && nu < 2; ++c)
+        if (diff.d[c]) unsat[nu++] = c;
     const uint64_t key = (static_cast<uint64_t>(static_cast<uint32_t>(unsat[0])) << 32) |
                          static_cast<uint32_t>(unsat[1]);
     for (const EtsGroup &grp : t.groups) {
         if (!grp.has[side]) continue;
-        if (ets_one(g, grp.side[side], side, target, diff, key, est)) {
+        if (ets_one(g, grp.side[side], side, diff, key, est)) {
             r.applied = true;
             r.size = grp.size;
             r.label = grp.label;
@@ -443,13 +446,13 @@ EtsFix ets_fix(const Graph &g, const Tables &t, int side, const uint8_t *target,
 
 bool flip_fix(const Graph &g, int side, const std::vector<int> &vars_in, const uint8_t *target,
               std::vector<uint8_t> &est) {
-    std::vector<uint8_t> diff = syndrome_of(g, side, est);
-    for (int c = 0; c < g.m[side]; ++c) diff[c] ^= target[c];
-    return flip_fix_diff(g, side, vars_in, target, diff, est);
+    SynDiff sd;
+    syn_diff_init(g, side, est.data(), target, sd);
+    return flip_fix_diff(g, side, vars_in, sd, est);
 }
 
-bool flip_fix_diff(const Graph &g, int side, const std::vector<int> &vars_in, const uint8_t *target,
-                   const std::vector<uint8_t> &diff, std::vector<uint8_t> &est) {
+bool flip_fix_diff(const Graph &g, int side, const std::vector<int> &vars_in, SynDiff &diff,
+                   std::vector<uint8_t> &est) {
     std::vector<int> vars;
     for (int v : vars_in)
         if (v >= 0 && v < g.n) vars.push_back(v);
@@ -471,22 +474,26 @@ bool flip_fix_diff(const Graph &g, int side, const std::vector<int> &vars_in, co
             for (int k = 0; k < cnt; ++k) hit[rows[k]] = 1;
         }
         for (int c = 0; c < m; ++c)
-            if (diff[c] && !hit[c]) return false;
+            if (diff.d[c] && !hit[c]) return false;
     }
     SpanBasis basis(m, nv);
-    basis.set_target(diff.data());
+    basis.set_target(diff.d.data());
     for (int j = 0; j < nv; ++j) {
         int cnt = 0;
         const int *rows = col_rows(g, side, vars[j], cnt);
         if (!basis.add(rows, cnt, j)) return false;  // not unique
     }
     if (!basis.target_reached()) return false;  // not solvable
-    std::vector<uint8_t> trial = est;
     const auto &comb = basis.target_combination();
     for (int j = 0; j < nv; ++j)
-        if ((comb[j >> 6] >> (j & 63)) & 1) toggle(side, trial[vars[j]]);
-    if (!matches(g, side, trial, target)) return false;
-    est.swap(trial);
+        if ((comb[j >> 6] >> (j & 63)) & 1) diff_toggle(g, side, diff, vars[j]);
+    if (diff.ones != 0) {  // (the combination reaches the difference: never)
+        for (int j = 0; j < nv; ++j)
+            if ((comb[j >> 6] >> (j & 63)) & 1) diff_toggle(g, side, diff, vars[j]);
+        return false;
+    }
+    for (int j = 0; j < nv; ++j)
+        if ((comb[j >> 6] >> (j & 63)) & 1) toggle(side, est[vars[j]]);
     return true;
 }
 
@@ -507,14 +514,16 @@ inline void llr_pair(const double *t, int side, double &p1, double &p0) {
 bool osd_fix(const Graph &g, int side, const LlrSide &llr, const uint8_t *target,
              std::vector<uint8_t> &est) {
     if (llr.empty() || g.m[side] == 0) return false;
+    SynDiff sd;
+    syn_diff_init(g, side, est.data(), target, sd);
+    return osd_fix_diff(g, side, llr, sd, est);
+}
+
+bool osd_fix_diff(const Graph &g, int side, const LlrSide &llr, SynDiff &diff,
+                  std::vector<uint8_t> &est) {
+    if (llr.empty() || g.m[side] == 0) return false;
     const int m = g.m[side], n = g.n;
-    std::vector<uint8_t> diff = syndrome_of(g, side, est);
-    bool any = false;
-    for (int c = 0; c < m; ++c) {
-        diff[c] ^= target[c];
-        any |= diff[c] != 0;
-    }
-    if (!any) return false;
+    if (diff.ones == 0) return false;
     // |LLR| order with ties by index (the reference's stable sort); only
     // the prefix the walk reaches is sorted, in growing chunks.
     //
@@ -570,7 +579,7 @@ bool osd_fix(const Graph &g, int side, const LlrSide &llr, const uint8_t *target
         sorted = next;
     };
     SpanBasis basis(m, std::min(n, m + 1));
-    basis.set_target(diff.data());
+    basis.set_target(diff.d.data());
     for (int j = 0; j < n; ++j) {
         if (j == sorted) extend(std::min(n, std::max(2 * sorted, sorted + 256)));
         int cnt = 0;
@@ -579,15 +588,19 @@ bool osd_fix(const Graph &g, int side, const LlrSide &llr, const uint8_t *target
         if (!basis.target_reached()) continue;
         // k = j + 1 columns reach the difference and are independent
         const auto &comb = basis.target_combination();
-        std::vector<uint8_t> trial = est;
+        auto used = [&](int i) { return (comb[i >> 6] >> (i & 63)) & 1; };
         int weight = 0;
+        for (int i = 0; i <= j; ++i) weight += used(i) ? 1 : 0;
+        if (weight > kOsdWeightLimit) return false;
         for (int i = 0; i <= j; ++i)
-            if ((comb[i >> 6] >> (i & 63)) & 1) {
-                ++weight;
-                toggle(side, trial[order[i]]);
-            }
-        if (weight > kOsdWeightLimit || !matches(g, side, trial, target)) return false;
-        est.swap(trial);
+            if (used(i)) diff_toggle(g, side, diff, order[i]);
+        if (diff.ones != 0) {  // (the combination reaches the difference: never)
+            for (int i = 0; i <= j; ++i)
+                if (used(i)) diff_toggle(g, side, diff, order[i]);
+            return false;
+        }
+        for (int i = 0; i <= j; ++i)
+            if (used(i)) toggle(side, est[order[i]]);
         return true;
     }
     return false;
@@ -673,29 +686,34 @@ Step observe(const Graph &g, const Params &prm, Trial &tr, int t, const IterRec 
     return Step::kContinue;
 }
 
+namespace {
+
+bool all_clear(const SynDiff (&diff)[2]) { return diff[0].ones == 0 && diff[1].ones == 0; }
+
+}  // namespace
+
 void attempt_begin(const Graph &g, const Tables &t, const Params &prm, const Trial &tr,
                    const std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz,
                    Attempt &a) {
-    a = Attempt();
+    a.ok = a.want_osd = a.ets_used = a.ets6 = a.ets12 = a.ets_ok = a.flip_ok = false;
     a.trial = est;
-    const EtsFix ex = ets_fix(g, t, 0, sx, a.trial);
-    const EtsFix ez = ets_fix(g, t, 1, sz, a.trial);
+    syn_diff_init(g, 0, a.trial.data(), sx, a.diff[0]);
+    syn_diff_init(g, 1, a.trial.data(), sz, a.diff[1]);
+    const EtsFix ex = ets_fix_diff(g, t, 0, a.diff[0], a.trial);
+    const EtsFix ez = ets_fix_diff(g, t, 1, a.diff[1], a.trial);
     a.ets_used = ex.applied || ez.applied;
     a.ets6 = ex.size == 6 || ez.size == 6;
     a.ets12 = ex.size == 12 || ez.size == 12;
-    if (a.ets_used) a.ets_ok = a.ok = both_match(g, a.trial, sx, sz);
-    if (!a.ok && flip_pass(g, prm, tr, a.trial, sx, sz)) a.flip_ok = a.ok = both_match(g, a.trial, sx, sz);
-    if (!a.ok)
-        a.want_osd = std::max(unsatisfied(g, 0, a.trial.data(), sx),
-                              unsatisfied(g, 1, a.trial.data(), sz)) <= kUsLimit;
+    if (a.ets_used) a.ets_ok = a.ok = all_clear(a.diff);
+    if (!a.ok && flip_pass(g, prm, tr, a.trial, a.diff)) a.flip_ok = a.ok = all_clear(a.diff);
+    if (!a.ok) a.want_osd = std::max(a.diff[0].ones, a.diff[1].ones) <= kUsLimit;
 }
 
-void attempt_osd(const Graph &g, Attempt &a, const uint8_t *sx, const uint8_t *sz,
-                 const LlrSide &llr_x, const LlrSide &llr_z) {
+void attempt_osd(const Graph &g, Attempt &a, const LlrSide &llr_x, const LlrSide &llr_z) {
     if (!a.want_osd) return;
-    const bool ox = osd_fix(g, 0, llr_x, sx, a.trial);
-    const bool oz = osd_fix(g, 1, llr_z, sz, a.trial);
-    if (ox || oz) a.ok = both_match(g, a.trial, sx, sz);
+    const bool ox = osd_fix_diff(g, 0, llr_x, a.diff[0], a.trial);
+    const bool oz = osd_fix_diff(g, 1, llr_z, a.diff[1], a.trial);
+    if (ox || oz) a.ok = all_clear(a.diff);
 }
 
 bool attempt_end(Trial &tr, Attempt &a, int t, std::vector<uint8_t> &est) {
@@ -713,34 +731,35 @@ bool attempt_end(Trial &tr, Attempt &a, int t, std::vector<uint8_t> &est) {
 }
 
 PostOut post_begin(const Graph &g, const Tables &t, const Params &prm, const Trial &tr,
-                   std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz) {
+                   std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz, SynDiff (&diff)[2]) {
     PostOut o;
-    const EtsFix ex = ets_fix(g, t, 0, sx, est);
-    const EtsFix ez = ets_fix(g, t, 1, sz, est);
+    syn_diff_init(g, 0, est.data(), sx, diff[0]);
+    syn_diff_init(g, 1, est.data(), sz, diff[1]);
+    const EtsFix ex = ets_fix_diff(g, t, 0, diff[0], est);
+    const EtsFix ez = ets_fix_diff(g, t, 1, diff[1], est);
     o.ets_applied = ex.applied || ez.applied;
     o.used6 = ex.size == 6 || ez.size == 6;
     o.used12 = ex.size == 12 || ez.size == 12;
-    if (o.ets_applied) o.ets_ok = o.ok = both_match(g, est, sx, sz);
-    if (!o.ok && flip_pass(g, prm, tr, est, sx, sz)) o.flip_ok = o.ok = both_match(g, est, sx, sz);
-    if (!o.ok)
-        o.want_osd = std::max(unsatisfied(g, 0, est.data(), sx), unsatisfied(g, 1, est.data(), sz)) <=
-                     kUsLimit;
+    if (o.ets_applied) o.ets_ok = o.ok = all_clear(diff);
+    if (!o.ok && flip_pass(g, prm, tr, est, diff)) o.flip_ok = o.ok = all_clear(diff);
+    if (!o.ok) o.want_osd = std::max(diff[0].ones, diff[1].ones) <= kUsLimit;
     return o;
 }
 
-void post_osd(const Graph &g, PostOut &o, std::vector<uint8_t> &est, const uint8_t *sx,
-              const uint8_t *sz, const LlrSide &llr_x, const LlrSide &llr_z) {
+void post_osd(const Graph &g, PostOut &o, std::vector<uint8_t> &est, SynDiff (&diff)[2],
+              const LlrSide &llr_x, const LlrSide &llr_z) {
     if (!o.want_osd) return;
-    const bool ox = osd_fix(g, 0, llr_x, sx, est);
-    const bool oz = osd_fix(g, 1, llr_z, sz, est);
-    if (ox || oz) o.osd_ok = o.ok = both_match(g, est, sx, sz);
+    const bool ox = osd_fix_diff(g, 0, llr_x, diff[0], est);
+    const bool oz = osd_fix_diff(g, 1, llr_z, diff[1], est);
+    if (ox || oz) o.osd_ok = o.ok = all_clear(diff);
 }
 
 PostOut post_decode(const Graph &g, const Tables &t, const Params &prm, const Trial &tr,
                     std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz,
                     const double *llr_x, const double *llr_z) {
-    PostOut o = post_begin(g, t, prm, tr, est, sx, sz);
-    post_osd(g, o, est, sx, sz, LlrSide::values(llr_x), LlrSide::values(llr_z));
+    SynDiff diff[2];
+    PostOut o = post_begin(g, t, prm, tr, est, sx, sz, diff);
+    post_osd(g, o, est, diff, LlrSide::values(llr_x), LlrSide::values(llr_z));
     return o;
 }
 

@@ -72,11 +72,23 @@ struct EtsFix {
 void syndrome(const Graph &g, int side, const uint8_t *est, uint8_t *out);
 int unsatisfied(const Graph &g, int side, const uint8_t *est, const uint8_t *target);
 
+// The syndrome difference of an estimate on one side, d[c] = syndrome[c] XOR
+// target[c], with its count of ones (the unsatisfied checks).  The repairs
+// keep it in step with the estimate: toggling a variable flips the bits of
+// its checks, so a candidate is verified without a syndrome pass.
+struct SynDiff {
+    std::vector<uint8_t> d;
+    int ones = 0;
+};
+void syn_diff_init(const Graph &g, int side, const uint8_t *est, const uint8_t *target, SynDiff &sd);
+
 // ETS repair of one side (jointbp_ets.cpp:3525-3668): applies to est the
 // first table entry, over the groups in order, whose variables explain the
 // two unsatisfied checks.
 EtsFix ets_fix(const Graph &g, const Tables &t, int side, const uint8_t *target,
                std::vector<uint8_t> &est);
+// ets_fix on the estimate's syndrome difference (updated when it applies).
+EtsFix ets_fix_diff(const Graph &g, const Tables &t, int side, SynDiff &diff, std::vector<uint8_t> &est);
 
 // Flip-set repair (jointbp_ets.cpp:3726-3880): flips the variables of `vars`
 // that make est match the target syndrome, when that set of flips exists
@@ -84,9 +96,9 @@ EtsFix ets_fix(const Graph &g, const Tables &t, int side, const uint8_t *target,
 bool flip_fix(const Graph &g, int side, const std::vector<int> &vars, const uint8_t *target,
               std::vector<uint8_t> &est);
 
-// flip_fix with diff = syndrome of est XOR target already formed.
-bool flip_fix_diff(const Graph &g, int side, const std::vector<int> &vars, const uint8_t *target,
-                   const std::vector<uint8_t> &diff, std::vector<uint8_t> &est);
+// flip_fix on the estimate's syndrome difference (updated when it applies).
+bool flip_fix_diff(const Graph &g, int side, const std::vector<int> &vars, SynDiff &diff,
+                   std::vector<uint8_t> &est);
 
 // One side's |LLR|, as values or as the normalised marginals [n][4] they
 // come from (jointbp_ets.cpp:2483-2493).  From marginals OSD computes the
@@ -105,6 +117,8 @@ struct LlrSide {
 // correction has weight <= 30.
 bool osd_fix(const Graph &g, int side, const LlrSide &llr, const uint8_t *target,
              std::vector<uint8_t> &est);
+bool osd_fix_diff(const Graph &g, int side, const LlrSide &llr, SynDiff &diff,
+                  std::vector<uint8_t> &est);
 
 // The reference's sets of flip candidates (jointbp_ets.cpp:3670-3689):
 // the union of the recent flip history, each recent iteration's flips
@@ -163,14 +177,14 @@ Step observe(const Graph &g, const Params &prm, Trial &tr, int t, const IterRec 
 // successful repair (est receives it, the trial is done).
 struct Attempt {
     std::vector<uint8_t> trial;
+    SynDiff diff[2];  // of trial, X and Z
     bool ok = false, want_osd = false;
     bool ets_used = false, ets6 = false, ets12 = false, ets_ok = false, flip_ok = false;
 };
 void attempt_begin(const Graph &g, const Tables &t, const Params &prm, const Trial &tr,
                    const std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz,
                    Attempt &a);
-void attempt_osd(const Graph &g, Attempt &a, const uint8_t *sx, const uint8_t *sz,
-                 const LlrSide &llr_x, const LlrSide &llr_z);
+void attempt_osd(const Graph &g, Attempt &a, const LlrSide &llr_x, const LlrSide &llr_z);
 bool attempt_end(Trial &tr, Attempt &a, int t, std::vector<uint8_t> &est);
 
 // The post-decode pass over a trial that ended unconverged
@@ -187,10 +201,11 @@ PostOut post_decode(const Graph &g, const Tables &t, const Params &prm, const Tr
                     const double *llr_x, const double *llr_z);
 // The same pass in two parts, so that |LLR| are fetched only when OSD runs:
 // post_begin does ETS and the flip sets and sets want_osd; post_osd finishes.
+// diff: the caller's scratch, carried from one to the other.
 PostOut post_begin(const Graph &g, const Tables &t, const Params &prm, const Trial &tr,
-                   std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz);
-void post_osd(const Graph &g, PostOut &o, std::vector<uint8_t> &est, const uint8_t *sx,
-              const uint8_t *sz, const LlrSide &llr_x, const LlrSide &llr_z);
+                   std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz, SynDiff (&diff)[2]);
+void post_osd(const Graph &g, PostOut &o, std::vector<uint8_t> &est, SynDiff (&diff)[2],
+              const LlrSide &llr_x, const LlrSide &llr_z);
 
 }  // namespace pp
 }  // namespace jbp

@@ -625,7 +625,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                         // ETS and flip sets now, |LLR| only if OSD follows
                         tr[i].iterations = max_iter;
                         o[i].post_run = true;
-                        o[i].post = pp::post_begin(hg, tabs, prm, tr[i], est[i], tsx, tsz);
+                        o[i].post = pp::post_begin(hg, tabs, prm, tr[i], est[i], tsx, tsz, att[i].diff);
                         need_llr[i] = o[i].post.want_osd;
                         if (!need_llr[i]) {
                             tr[i].done = true;
@@ -656,9 +656,8 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                 }
                 // OSD takes the marginals and computes the |LLR| it needs
                 const pp::LlrSide lx = pp::LlrSide::marginals(tot + static_cast<size_t>(r) * row), lz = lx;
-                const uint8_t *tsx = sx + static_cast<size_t>(i) * mx, *tsz = sz + static_cast<size_t>(i) * mz;
                 if (att[i].want_osd) {
-                    pp::attempt_osd(hg, att[i], tsx, tsz, lx, lz);
+                    pp::attempt_osd(hg, att[i], lx, lz);
                     if (pp::attempt_end(tr[i], att[i], t, est[i])) {
                         finish(i);
                         return;
@@ -670,7 +669,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                     } else {
                         tr[i].done = true;
                         finish(i);
-                        pp::post_osd(hg, o[i].post, est[i], tsx, tsz, lx, lz);
+                        pp::post_osd(hg, o[i].post, est[i], att[i].diff, lx, lz);
                         o[i].ok = o[i].post.ok;
                     }
                 }
