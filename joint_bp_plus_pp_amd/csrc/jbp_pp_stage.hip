@@ -393,10 +393,12 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
     const size_t nbatch = (trials.size() + batch - 1) / batch;
     hipStream_t streams[2] = {ctx->stream, ctx->pp_stream};
 
-    // Host state, the syndromes' copy to the host and (H > 0) phase A, all
-    // queued without waiting; finish_batch waits for them.
+    // The syndromes' copy to the host and (H > 0) phase A, queued without
+    // waiting; finish_batch sets up the batch's host state and then waits
+    // for them.
+    std::vector<int> iota(batch);
+    for (size_t i = 0; i < batch; ++i) iota[i] = static_cast<int>(i);
     auto start = [&](PPBatch &B, size_t bi) -> int {
-        B = PPBatch();
         B.b0 = bi * batch;
         B.K = static_cast<int>(std::min<size_t>(batch, trials.size() - B.b0));
         B.pb = &ctx->ppb[bi & 1];
@@ -408,10 +410,8 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
         if (rc) return rc;
         PPBuf &pb = *B.pb;
         hipStream_t st = B.st;
-        std::vector<int> slot_trial(trials.begin() + B.b0, trials.begin() + B.b0 + K);
-        HIPCHK(hipMemcpyAsync(pb.slot_trial, slot_trial.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
-        B.sx.resize(static_cast<size_t>(K) * mx);
-        B.sz.resize(static_cast<size_t>(K) * mz);
+        const int *slot_trial = trials.data() + B.b0;
+        HIPCHK(hipMemcpyAsync(pb.slot_trial, slot_trial, sizeof(int) * K, hipMemcpyHostToDevice, st));
         for (int s = 0; s < 2; ++s) {
             const int wm = g.wm[s];
             uint32_t *w = static_cast<uint32_t *>(pb.h_stage) + (s == 0 ? 0 : static_cast<size_t>(K) * g.wm[0]);
@@ -428,17 +428,9 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             }
         }
         B.syn_wait = true;
-        B.tr.assign(K, pp::Trial());
-        B.est.assign(K, std::vector<uint8_t>(n, 0));
-        B.o.assign(K, PPOutcome());
-        B.active.resize(K);
-        for (int i = 0; i < K; ++i) B.active[i] = i;
-        B.att.assign(K, pp::Attempt());
-        B.need_llr.assign(K, 0);
-        B.at_end.assign(K, 0);
-        B.attempted.assign(K, 0);
+        B.phase_a = false;
         if (H > 0 && max_iter > 0) {
-            HIPCHK(hipMemcpyAsync(pb.list, B.active.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(pb.list, iota.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
             launch_step(g, ctx->tp, nt, pb.list, K, pb.slot_trial, H, true, false, damping, pb.state,
                         pb.stride, pb.rec, pb.out, st);
             HIPCHK(hipGetLastError());
@@ -463,6 +455,20 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             return kick();
         };
         const int K = B.K;
+        // host state (while the device runs phase A); vectors keep their
+        // allocations from the batch that used this object before
+        B.sx.resize(static_cast<size_t>(K) * mx);
+        B.sz.resize(static_cast<size_t>(K) * mz);
+        B.tr.assign(K, pp::Trial());
+        B.est.resize(K);
+        for (auto &e : B.est) e.assign(n, 0);
+        B.o.assign(K, PPOutcome());
+        B.active.resize(K);
+        for (int i = 0; i < K; ++i) B.active[i] = i;
+        B.att.resize(K);  // (every attempt resets its fields when it starts)
+        B.need_llr.assign(K, 0);
+        B.at_end.assign(K, 0);
+        B.attempted.assign(K, 0);
         PPBuf &pb = *B.pb;
         hipStream_t st = B.st;
         auto &tr = B.tr;
