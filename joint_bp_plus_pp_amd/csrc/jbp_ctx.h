@@ -157,6 +157,8 @@ struct jbp_ctx {
     // the next batch's first device phase
     jbp::PPBuf ppb[2];
     hipStream_t pp_stream = nullptr;
+    int *pp_idx = nullptr;  // the stage's slot list 0..batch-1, then its trial indices
+    size_t pp_idx_cap = 0;
     double last_ms[3] = {0, 0, 0};
     int64_t launches[3] = {0, 0, 0};
 };

@@ -352,6 +352,9 @@ void unpack_bits(const uint32_t *w, int count, uint8_t *out) {
 
 void free_pp(jbp_ctx *ctx) {
     for (PPBuf &pb : ctx->ppb) free_pp_buf(pb);
+    if (ctx->pp_idx) (void)hipFree(ctx->pp_idx);
+    ctx->pp_idx = nullptr;
+    ctx->pp_idx_cap = 0;
     if (ctx->pp_stream) (void)hipStreamDestroy(ctx->pp_stream);
     ctx->pp_stream = nullptr;
 }
@@ -361,6 +364,7 @@ void free_pp(jbp_ctx *ctx) {
 struct PPBatch {
     size_t b0 = 0;
     int K = 0;
+    const int *slot_trial = nullptr;  // device: trial index of each slot
     PPBuf *pb = nullptr;
     hipStream_t st = nullptr;
     bool syn_wait = false;  // syndromes queued for the host, not yet unpacked
@@ -396,8 +400,30 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
     // The syndromes' copy to the host and (H > 0) phase A, queued without
     // waiting; finish_batch sets up the batch's host state and then waits
     // for them.
-    std::vector<int> iota(batch);
-    for (size_t i = 0; i < batch; ++i) iota[i] = static_cast<int>(i);
+    // The slot list 0..batch-1 and every trial index of the stage, uploaded
+    // once: a batch's first launches then have no copy ahead of them in
+    // their stream (a copy waits behind the previous batch's marginals on
+    // the copy engine, and the launches would wait with it).
+    {
+        const size_t nidx = batch + trials.size();
+        if (ctx->pp_idx_cap < nidx) {
+            if (ctx->pp_idx) (void)hipFree(ctx->pp_idx);
+            ctx->pp_idx = nullptr;
+            ctx->pp_idx_cap = 0;
+            HIPCHK(hipMalloc(&ctx->pp_idx, sizeof(int) * nidx));
+            ctx->pp_idx_cap = nidx;
+        }
+        std::vector<int> idx(nidx);
+        for (size_t i = 0; i < batch; ++i) idx[i] = static_cast<int>(i);
+        std::copy(trials.begin(), trials.end(), idx.begin() + batch);
+        HIPCHK(hipMemcpy(ctx->pp_idx, idx.data(), sizeof(int) * nidx, hipMemcpyHostToDevice));
+    }
+    const int *d_slots = ctx->pp_idx, *d_trials = ctx->pp_idx + batch;
+    // where each side's syndrome words of a batch of K start in the staging
+    // buffers (32-bit words; the Z side 8-byte aligned for the gather)
+    auto syn_off = [&](int K, int s) -> size_t {
+        return s == 0 ? 0 : ((static_cast<size_t>(K) * g.wm[0] + 1) & ~static_cast<size_t>(1));
+    };
     auto start = [&](PPBatch &B, size_t bi) -> int {
         B.b0 = bi * batch;
         B.K = static_cast<int>(std::min<size_t>(batch, trials.size() - B.b0));
@@ -410,34 +436,41 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
         if (rc) return rc;
         PPBuf &pb = *B.pb;
         hipStream_t st = B.st;
-        const int *slot_trial = trials.data() + B.b0;
-        HIPCHK(hipMemcpyAsync(pb.slot_trial, slot_trial, sizeof(int) * K, hipMemcpyHostToDevice, st));
+        B.slot_trial = d_trials + B.b0;
+        // the syndromes gathered (both sides side by side in the staging
+        // buffer), phase A, then the copies to the host
+        for (int s = 0; s < 2; ++s)
+            if (g.wm[s] % 2 == 0) {
+                launch_gather(s == 0 ? ctx->tp.syn_x : ctx->tp.syn_z, g.wm[s] / 2, 0, g.wm[s] / 2,
+                              B.slot_trial, K, static_cast<uint32_t *>(pb.stage) + syn_off(K, s), st);
+                HIPCHK(hipGetLastError());
+            }
+        B.phase_a = false;
+        if (H > 0 && max_iter > 0) {
+            launch_step(g, ctx->tp, nt, d_slots, K, B.slot_trial, H, true, false, damping, pb.state,
+                        pb.stride, pb.rec, pb.out, st);
+            HIPCHK(hipGetLastError());
+            B.phase_a = true;
+        }
         for (int s = 0; s < 2; ++s) {
             const int wm = g.wm[s];
-            uint32_t *w = static_cast<uint32_t *>(pb.h_stage) + (s == 0 ? 0 : static_cast<size_t>(K) * g.wm[0]);
-            const uint32_t *src = s == 0 ? ctx->tp.syn_x : ctx->tp.syn_z;
+            uint32_t *w = static_cast<uint32_t *>(pb.h_stage) + syn_off(K, s);
             if (wm % 2 == 0) {
-                launch_gather(src, wm / 2, 0, wm / 2, pb.slot_trial, K, pb.stage, st);
-                HIPCHK(hipGetLastError());
-                HIPCHK(hipMemcpyAsync(w, pb.stage, 4 * static_cast<size_t>(K) * wm, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipMemcpyAsync(w, static_cast<const uint32_t *>(pb.stage) + syn_off(K, s),
+                                      4 * static_cast<size_t>(K) * wm, hipMemcpyDeviceToHost, st));
             } else {
+                const uint32_t *src = s == 0 ? ctx->tp.syn_x : ctx->tp.syn_z;
                 for (int i = 0; i < K; ++i)
                     HIPCHK(hipMemcpyAsync(w + static_cast<size_t>(i) * wm,
-                                          src + static_cast<size_t>(slot_trial[i]) * wm, 4 * wm,
+                                          src + static_cast<size_t>(trials[B.b0 + i]) * wm, 4 * wm,
                                           hipMemcpyDeviceToHost, st));
             }
         }
         B.syn_wait = true;
-        B.phase_a = false;
-        if (H > 0 && max_iter > 0) {
-            HIPCHK(hipMemcpyAsync(pb.list, iota.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
-            launch_step(g, ctx->tp, nt, pb.list, K, pb.slot_trial, H, true, false, damping, pb.state,
-                        pb.stride, pb.rec, pb.out, st);
-            HIPCHK(hipGetLastError());
+        if (B.phase_a) {
             HIPCHK(hipMemcpyAsync(pb.h_rec, pb.rec, sizeof(int) * static_cast<size_t>(K) * H * kRecInts,
                                   hipMemcpyDeviceToHost, st));
             HIPCHK(hipMemcpyAsync(pb.h_out, pb.out, static_cast<size_t>(K) * 2 * n, hipMemcpyDeviceToHost, st));
-            B.phase_a = true;
         }
         return JBP_OK;
     };
@@ -513,7 +546,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
         HIPCHK(hipStreamSynchronize(st));
         if (B.syn_wait) {
             const uint32_t *w = static_cast<const uint32_t *>(pb.h_stage);
-            const uint32_t *wz = w + static_cast<size_t>(K) * g.wm[0];
+            const uint32_t *wz = w + syn_off(K, 1);
             pool.run(K, [&](int i) {
                 unpack_bits(w + static_cast<size_t>(i) * g.wm[0], mx, B.sx.data() + static_cast<size_t>(i) * mx);
                 unpack_bits(wz + static_cast<size_t>(i) * g.wm[1], mz, B.sz.data() + static_cast<size_t>(i) * mz);
@@ -561,7 +594,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
         for (int t = std::max(H, 0); t < max_iter && !active.empty(); ++t) {
             const int A = static_cast<int>(active.size());
             HIPCHK(hipMemcpyAsync(pb.list, active.data(), sizeof(int) * A, hipMemcpyHostToDevice, st));
-            launch_step(g, ctx->tp, nt, pb.list, A, pb.slot_trial, 1, t == 0, true, damping, pb.state,
+            launch_step(g, ctx->tp, nt, pb.list, A, B.slot_trial, 1, t == 0, true, damping, pb.state,
                         pb.stride, pb.rec, pb.out, st);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(rec, pb.rec, sizeof(int) * static_cast<size_t>(A) * kRecInts,
@@ -714,7 +747,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
         HIPCHK(hipMemcpyAsync(d_planes, planes, 4 * nplane + 5 * static_cast<size_t>(K), hipMemcpyHostToDevice, st));
         const int bx = std::min(16, (g.wn + 255) / 256);
         hipLaunchKernelGGL(scatter_outcome_kernel, dim3(bx, K), dim3(256), 0, st, d_planes, d_it, d_ok,
-                           pb.slot_trial, K, g.wn, ctx->tp);
+                           B.slot_trial, K, g.wn, ctx->tp);
         HIPCHK(hipGetLastError());
         return JBP_OK;
     };
