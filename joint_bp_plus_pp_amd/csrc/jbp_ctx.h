@@ -60,9 +60,11 @@ struct PPBuf {
     uint8_t *h_out = nullptr;
     double *h_tot = nullptr;
     void *h_stage = nullptr;
+    void *d_hstage = nullptr;  // h_stage's device address (kernels read it directly)
     // marginals reach the host in up to kTotChunks pieces, one event each
     static constexpr int kTotChunks = 8;
     hipEvent_t tot_ev[kTotChunks] = {};
+    hipEvent_t ready_ev = nullptr;  // a batch's first device phase and its copies done
 };
 
 // What the post-processing stage decided for one trial.

@@ -74,8 +74,9 @@ constexpr int kRecInts = 4;
 
 // Block b advances the trial of slot list[b] by up to `steps` iterations
 // (stopping when both syndromes match).  init: start from the reference's
-// initial messages.  want_tot: write every iteration's marginals (the slot
-// then holds the last iteration's).  At the end the block copies its
+// initial messages.  want_tot: 1 writes every iteration's marginals (the slot
+// then holds the last iteration's), 2 only the launch's last iteration's (a
+// trial that stops earlier leaves none).  At the end the block copies its
 // estimate and previous estimate to out[b].
 // Waves per SIMD the register budget allows: 4 for degree 12 / 3 with pair
 // sums (128 VGPRs, every block of a 1024-trial batch resident at once), 2
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(256, (MAXC <= 12 && PAIR) ? 4 : 2) void pp_step_ker
         for (int c = tid; c < mz; c += bs)
             check_update<MAXC, PAIR>(g.side[1], 1, c, (tsz[c >> 5] >> (c & 31)) & 1, sp.v2c[1], sp.c2v[1]);
         __syncthreads();
-        double *tot = want_tot ? sp.tot : nullptr;
+        double *tot = (want_tot == 1 || (want_tot == 2 && j == steps - 1)) ? sp.tot : nullptr;
         int changed1 = 0, changed2 = 0;
         for (int v = tid; v < n; v += bs) {
             const int best = var_update<MAXV, PAIR>(g, v, prior, damping, sp.v2c, sp.c2v, tot);
@@ -155,12 +156,12 @@ __global__ __launch_bounds__(256, (MAXC <= 12 && PAIR) ? 4 : 2) void pp_step_ker
 }
 
 void launch_step(const DevGraph &g, const TrialPlanes &tp, const NoiseTab &nt, const int *list,
-                 int count, const int *slot_trial, int steps, bool init, bool want_tot,
+                 int count, const int *slot_trial, int steps, bool init, int want_tot,
                  double damping, char *state, size_t stride, int *rec, uint8_t *out,
                  hipStream_t st) {
     const bool exact12 = g.max_cdeg <= 12 && g.max_vdeg <= 3;
     const bool small = g.max_cdeg <= 16 && g.max_vdeg <= 4, pair = damping <= 0.0;
-    const int ini = init ? 1 : 0, wt = want_tot ? 1 : 0;
+    const int ini = init ? 1 : 0, wt = want_tot;
     if (exact12 && pair)
         hipLaunchKernelGGL((pp_step_kernel<12, 3, true>), dim3(count), dim3(256), 0, st, g, tp, nt,
                            list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
@@ -297,6 +298,7 @@ constexpr size_t kPPStateBytes = size_t(8) << 30;
 void free_pp_buf(PPBuf &pb) {
     for (hipEvent_t &e : pb.tot_ev)
         if (e) (void)hipEventDestroy(e);
+    if (pb.ready_ev) (void)hipEventDestroy(pb.ready_ev);
     for (void *p : {static_cast<void *>(pb.state), static_cast<void *>(pb.list),
                     static_cast<void *>(pb.slot_trial), static_cast<void *>(pb.rec),
                     static_cast<void *>(pb.out), static_cast<void *>(pb.idx), pb.stage})
@@ -340,7 +342,9 @@ int ensure_pp_buffers(jbp_ctx *ctx, PPBuf &pb, int slots, int rec_per_slot, char
     HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&pb.h_out), 2 * static_cast<size_t>(g.n) * slots));
     HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&pb.h_tot), tot));
     HIPCHK(hipHostMalloc(&pb.h_stage, std::max(syn, back)));
+    HIPCHK(hipHostGetDevicePointer(&pb.d_hstage, pb.h_stage, 0));
     for (hipEvent_t &e : pb.tot_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&pb.ready_ev, hipEventDisableTiming));
     return JBP_OK;
 }
 
@@ -367,6 +371,7 @@ struct PPBatch {
     const int *slot_trial = nullptr;  // device: trial index of each slot
     PPBuf *pb = nullptr;
     hipStream_t st = nullptr;
+    int per_all = 1;        // slots per chunk of the first phase's marginal copies
     bool syn_wait = false;  // syndromes queued for the host, not yet unpacked
     bool phase_a = false;   // phase A launched, results not yet read
     std::vector<uint8_t> sx, sz;
@@ -397,9 +402,6 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
     const size_t nbatch = (trials.size() + batch - 1) / batch;
     hipStream_t streams[2] = {ctx->stream, ctx->pp_stream};
 
-    // The syndromes' copy to the host and (H > 0) phase A, queued without
-    // waiting; finish_batch sets up the batch's host state and then waits
-    // for them.
     // The slot list 0..batch-1 and every trial index of the stage, uploaded
     // once: a batch's first launches then have no copy ahead of them in
     // their stream (a copy waits behind the previous batch's marginals on
@@ -424,33 +426,65 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
     auto syn_off = [&](int K, int s) -> size_t {
         return s == 0 ? 0 : ((static_cast<size_t>(K) * g.wm[0] + 1) & ~static_cast<size_t>(1));
     };
+    // Iterations of the first device phase: H, then iteration H itself (the
+    // first at which a trigger can fire; nothing the host decides before it
+    // changes it, since a trial that converges earlier stops on the device).
+    const int HA = H + 1;
+    const size_t row = 4 * static_cast<size_t>(n);
+    const size_t tot_off = 6 * static_cast<size_t>(g.side[0].cap + g.side[1].cap);
+    // Marginals of the listed slots gathered, then copied to the host in
+    // chunks, one event each, so that the repairs of a chunk start as soon
+    // as it has arrived.
+    auto queue_marginals = [&](PPBuf &pb, hipStream_t st, const int *idx, int count, int &per) -> int {
+        per = std::max(1, (count + PPBuf::kTotChunks - 1) / PPBuf::kTotChunks);
+        if (count <= 0) return JBP_OK;
+        launch_gather(pb.state, pb.stride / 8, tot_off, 4 * n, idx, count, pb.stage, st);
+        HIPCHK(hipGetLastError());
+        return JBP_OK;
+    };
+    auto queue_marginal_copies = [&](PPBuf &pb, hipStream_t st, int count, int per) -> int {
+        for (int c = 0; c * per < count; ++c) {
+            const size_t w0 = static_cast<size_t>(c) * per, w1 = std::min<size_t>(count, w0 + per);
+            HIPCHK(hipMemcpyAsync(pb.h_tot + w0 * row, static_cast<const double *>(pb.stage) + w0 * row,
+                                  sizeof(double) * (w1 - w0) * row, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipEventRecord(pb.tot_ev[c], st));
+        }
+        return JBP_OK;
+    };
+
+    // Queues a batch without waiting: the syndromes' gather, the first
+    // device phase (iterations 0..H), the copies of the syndromes, records
+    // and estimates, the gather of every slot's iteration-H marginals (ready
+    // event), and their copies.  No copy goes ahead of the launches in the
+    // stream (a copy waits behind the previous batch's marginals on the copy
+    // engine, and the launches would wait with it).
     auto start = [&](PPBatch &B, size_t bi) -> int {
         B.b0 = bi * batch;
         B.K = static_cast<int>(std::min<size_t>(batch, trials.size() - B.b0));
         B.pb = &ctx->ppb[bi & 1];
         B.st = streams[bi & 1];
-        // the write-back of the batch that used this set before
-        HIPCHK(hipStreamSynchronize(B.st));
         const int K = B.K;
-        int rc = ensure_pp_buffers(ctx, *B.pb, K, std::max(H, 1), err, errlen);
+        // The batch that used this set before may still be writing back: its
+        // copies and kernel precede everything queued here in the stream,
+        // and the host writes nothing they read, so only growing the buffers
+        // (freeing the old ones) waits for it.
+        if (B.pb->cap < K || B.pb->rec_cap < HA || B.pb->stride != stride) HIPCHK(hipStreamSynchronize(B.st));
+        int rc = ensure_pp_buffers(ctx, *B.pb, K, HA, err, errlen);
         if (rc) return rc;
         PPBuf &pb = *B.pb;
         hipStream_t st = B.st;
         B.slot_trial = d_trials + B.b0;
-        // the syndromes gathered (both sides side by side in the staging
-        // buffer), phase A, then the copies to the host
         for (int s = 0; s < 2; ++s)
             if (g.wm[s] % 2 == 0) {
                 launch_gather(s == 0 ? ctx->tp.syn_x : ctx->tp.syn_z, g.wm[s] / 2, 0, g.wm[s] / 2,
                               B.slot_trial, K, static_cast<uint32_t *>(pb.stage) + syn_off(K, s), st);
                 HIPCHK(hipGetLastError());
             }
-        B.phase_a = false;
-        if (H > 0 && max_iter > 0) {
-            launch_step(g, ctx->tp, nt, d_slots, K, B.slot_trial, H, true, false, damping, pb.state,
+        B.phase_a = max_iter > 0;
+        if (B.phase_a) {
+            launch_step(g, ctx->tp, nt, d_slots, K, B.slot_trial, HA, true, 2, damping, pb.state,
                         pb.stride, pb.rec, pb.out, st);
             HIPCHK(hipGetLastError());
-            B.phase_a = true;
         }
         for (int s = 0; s < 2; ++s) {
             const int wm = g.wm[s];
@@ -467,19 +501,25 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             }
         }
         B.syn_wait = true;
+        B.per_all = 1;
         if (B.phase_a) {
-            HIPCHK(hipMemcpyAsync(pb.h_rec, pb.rec, sizeof(int) * static_cast<size_t>(K) * H * kRecInts,
+            HIPCHK(hipMemcpyAsync(pb.h_rec, pb.rec, sizeof(int) * static_cast<size_t>(K) * HA * kRecInts,
                                   hipMemcpyDeviceToHost, st));
             HIPCHK(hipMemcpyAsync(pb.h_out, pb.out, static_cast<size_t>(K) * 2 * n, hipMemcpyDeviceToHost, st));
+            // (the gather reuses the staging buffer after the syndromes' copy)
+            if ((rc = queue_marginals(pb, st, d_slots, K, B.per_all))) return rc;
         }
+        HIPCHK(hipEventRecord(pb.ready_ev, st));
+        if (B.phase_a && (rc = queue_marginal_copies(pb, st, K, B.per_all))) return rc;
         return JBP_OK;
     };
 
-    // Phase A's records, phase B, the repairs and the write-back.  kick()
-    // starts the next batch; it is called once this batch's last step has
-    // run and the marginals of its last iteration are gathered, so the
-    // next batch's phase A (which holds the device) runs during this
-    // batch's repairs, and the marginals travel to the host meanwhile.
+    // The first phase's records and iteration H, further iterations one
+    // launch each, the repairs and the write-back.  kick() starts the next
+    // batch, whose first phase then holds every slot a step kernel could
+    // use: it is called as soon as this batch has no step left, i.e. right
+    // after the first phase when iteration H is the last, else after the
+    // last step (with that iteration's marginals gathered first).
     auto finish_batch = [&](PPBatch &B, const std::function<int()> &kick) -> int {
         bool kicked = false;
         auto kick_once = [&]() -> int {
@@ -542,8 +582,10 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             o[i].ok = false;
             est[i].swap(att[i].trial);
         };
-        // syndromes and phase A (queued by start)
-        HIPCHK(hipStreamSynchronize(st));
+        int rc = JBP_OK;
+        // the first phase, its copies and the marginals' gather (queued by start)
+        HIPCHK(hipEventSynchronize(pb.ready_ev));
+        if (HA >= max_iter && (rc = kick_once())) return rc;
         if (B.syn_wait) {
             const uint32_t *w = static_cast<const uint32_t *>(pb.h_stage);
             const uint32_t *wz = w + syn_off(K, 1);
@@ -564,14 +606,13 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             });
             active.clear();
         }
-        int rc = JBP_OK;
         int *rec = pb.h_rec;
         uint8_t *out = pb.h_out;
         if (B.phase_a) {
-            B.phase_a = false;
+            // iterations before H: bookkeeping only (no trigger can fire)
             pool.run(K, [&](int i) {
                 for (int t = 0; t < H; ++t) {
-                    const int *r = &rec[(static_cast<size_t>(i) * H + t) * kRecInts];
+                    const int *r = &rec[(static_cast<size_t>(i) * HA + t) * kRecInts];
                     pp::IterRec ir;
                     ir.us[0] = r[0];
                     ir.us[1] = r[1];
@@ -590,48 +631,45 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                          active.end());
         }
         double *tot = pb.h_tot;
-        std::vector<int> pos(K);  // trial -> row of its marginals (last iteration)
-        for (int t = std::max(H, 0); t < max_iter && !active.empty(); ++t) {
+        std::vector<int> pos(K);  // trial -> row of its marginals (a step's last iteration)
+        for (int t = H; B.phase_a && t < max_iter && !active.empty(); ++t) {
             const int A = static_cast<int>(active.size());
-            HIPCHK(hipMemcpyAsync(pb.list, active.data(), sizeof(int) * A, hipMemcpyHostToDevice, st));
-            launch_step(g, ctx->tp, nt, pb.list, A, B.slot_trial, 1, t == 0, true, damping, pb.state,
-                        pb.stride, pb.rec, pb.out, st);
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(rec, pb.rec, sizeof(int) * static_cast<size_t>(A) * kRecInts,
-                                  hipMemcpyDeviceToHost, st));
-            HIPCHK(hipMemcpyAsync(out, pb.out, static_cast<size_t>(A) * 2 * n, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            // Marginals reach the host in chunks, one event each, so that the
-            // repairs of a chunk start as soon as it has arrived.  In the
-            // last iteration every active trial's are gathered now, before
-            // the next batch takes the device (most of them are needed);
-            // before that only those of the trials that reach OSD, below.
+            const bool first = t == H;  // iteration H: computed by the first phase
             const bool last = t + 1 == max_iter;
-            const size_t row = 4 * static_cast<size_t>(n);
-            const size_t tot_off = 6 * static_cast<size_t>(g.side[0].cap + g.side[1].cap);
-            auto queue_marginals = [&](const int *idx, int count, int &per) -> int {
-                per = std::max(1, (count + PPBuf::kTotChunks - 1) / PPBuf::kTotChunks);
-                if (count <= 0) return JBP_OK;
-                launch_gather(pb.state, pb.stride / 8, tot_off, 4 * n, idx, count, pb.stage, st);
+            int per = B.per_all;
+            if (!first) {
+                HIPCHK(hipMemcpyAsync(pb.list, active.data(), sizeof(int) * A, hipMemcpyHostToDevice, st));
+                launch_step(g, ctx->tp, nt, pb.list, A, B.slot_trial, 1, false, 1, damping, pb.state,
+                            pb.stride, pb.rec, pb.out, st);
                 HIPCHK(hipGetLastError());
-                for (int c = 0; c * per < count; ++c) {
-                    const size_t w0 = static_cast<size_t>(c) * per, w1 = std::min<size_t>(count, w0 + per);
-                    HIPCHK(hipMemcpyAsync(tot + w0 * row, static_cast<const double *>(pb.stage) + w0 * row,
-                                          sizeof(double) * (w1 - w0) * row, hipMemcpyDeviceToHost, st));
-                    HIPCHK(hipEventRecord(pb.tot_ev[c], st));
+                HIPCHK(hipMemcpyAsync(rec, pb.rec, sizeof(int) * static_cast<size_t>(A) * kRecInts,
+                                      hipMemcpyDeviceToHost, st));
+                HIPCHK(hipMemcpyAsync(out, pb.out, static_cast<size_t>(A) * 2 * n, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                // In the last iteration every active trial's marginals are
+                // gathered now, before the next batch takes the device (most
+                // of them are needed); before that only those of the trials
+                // that reach OSD, below.
+                if (last) {
+                    if ((rc = queue_marginals(pb, st, pb.list, A, per))) return rc;
+                    if ((rc = kick_once())) return rc;
+                    if ((rc = queue_marginal_copies(pb, st, A, per))) return rc;
+                    for (int a = 0; a < A; ++a) pos[active[a]] = a;
                 }
-                return JBP_OK;
-            };
-            int per = 1;
-            if (last) {
-                if ((rc = queue_marginals(pb.list, A, per))) return rc;
-                if ((rc = kick_once())) return rc;
-                for (int a = 0; a < A; ++a) pos[active[a]] = a;
             }
+            // iteration t's records and estimates: by slot after the first
+            // phase, by position in the list after a step
+            auto rec_of = [&](int a) -> const int * {
+                return first ? &rec[(static_cast<size_t>(active[a]) * HA + H) * kRecInts]
+                             : &rec[static_cast<size_t>(a) * kRecInts];
+            };
+            auto out_of = [&](int a) -> const uint8_t * {
+                return &out[static_cast<size_t>(first ? active[a] : a) * 2 * n];
+            };
             pool.run(A, [&](int a) {
                 const int i = active[a];
-                const int *r = &rec[static_cast<size_t>(a) * kRecInts];
-                const uint8_t *e_now = &out[static_cast<size_t>(a) * 2 * n];
+                const int *r = rec_of(a);
+                const uint8_t *e_now = out_of(a);
                 const uint8_t *e_prev = e_now + n;
                 pp::IterRec ir;
                 ir.us[0] = r[0];
@@ -657,7 +695,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                     }
                     need_llr[i] = att[i].want_osd;
                 }
-                if (t + 1 == max_iter) {
+                if (last) {
                     at_end[i] = 1;
                     if (!attempted[i]) {
                         // the post-decode pass runs itself (no attempt to reuse):
@@ -681,14 +719,16 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             for (int i : active)
                 if (!tr[i].done && need_llr[i]) want.push_back(i);
             const int W = static_cast<int>(want.size());
-            if (!last && W > 0) {
+            if (!first && !last && W > 0) {
                 HIPCHK(hipMemcpyAsync(pb.idx, want.data(), sizeof(int) * W, hipMemcpyHostToDevice, st));
-                if ((rc = queue_marginals(pb.idx, W, per))) return rc;
+                if ((rc = queue_marginals(pb, st, pb.idx, W, per))) return rc;
+                if ((rc = queue_marginal_copies(pb, st, W, per))) return rc;
             }
             std::atomic<int> ev_err{0};
             pool.run(W, [&](int w) {
                 const int i = want[w];
-                const int r = last ? pos[i] : w;  // row of the trial's marginals
+                // row of the trial's marginals
+                const int r = first ? i : last ? pos[i] : w;
                 if (hipEventSynchronize(pb.tot_ev[r / per]) != hipSuccess) {
                     ev_err.store(1);
                     return;
@@ -724,7 +764,9 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
         // write back: estimate planes, iterations, syndrome flag.  Queued
         // without a wait: the host reads nothing it writes before the final
         // synchronisation, and this buffer set's next use is ordered after
-        // it on the same stream.
+        // it on the same stream.  The kernel reads the pinned buffer
+        // directly: a copy would wait behind the marginal copies on the
+        // copy engine, and the next batch queued behind it with it.
         const size_t nplane = static_cast<size_t>(K) * 2 * g.wn;
         uint32_t *planes = static_cast<uint32_t *>(pb.h_stage);
         int *it = reinterpret_cast<int *>(planes + nplane);
@@ -740,11 +782,9 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             okb[i] = o[i].ok ? 1 : 0;
         });
         for (int i = 0; i < K; ++i) outs[B.b0 + i] = o[i];
-        char *stage = static_cast<char *>(pb.stage);
-        uint32_t *d_planes = reinterpret_cast<uint32_t *>(stage);
-        int *d_it = reinterpret_cast<int *>(stage + 4 * nplane);
-        uint8_t *d_ok = reinterpret_cast<uint8_t *>(d_it + K);
-        HIPCHK(hipMemcpyAsync(d_planes, planes, 4 * nplane + 5 * static_cast<size_t>(K), hipMemcpyHostToDevice, st));
+        const uint32_t *d_planes = static_cast<const uint32_t *>(pb.d_hstage);
+        const int *d_it = reinterpret_cast<const int *>(d_planes + nplane);
+        const uint8_t *d_ok = reinterpret_cast<const uint8_t *>(d_it + K);
         const int bx = std::min(16, (g.wn + 255) / 256);
         hipLaunchKernelGGL(scatter_outcome_kernel, dim3(bx, K), dim3(256), 0, st, d_planes, d_it, d_ok,
                            B.slot_trial, K, g.wn, ctx->tp);
@@ -887,7 +927,7 @@ extern "C" int jbp_decode_llr(jbp_ctx *ctx, int count, const uint32_t *sx_bits, 
         HIPCHK(hipMemcpyAsync(pb.list, ids.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(pb.slot_trial, trials.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
         // every iteration in one launch, marginals kept (the last one stays)
-        launch_step(g, ctx->tp, ctx->noise, pb.list, K, pb.slot_trial, max_iter, true, true, damping,
+        launch_step(g, ctx->tp, ctx->noise, pb.list, K, pb.slot_trial, max_iter, true, 1, damping,
                     pb.state, pb.stride, pb.rec, pb.out, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(pb.h_rec, pb.rec, sizeof(int) * kRecInts * static_cast<size_t>(K) * max_iter,
