@@ -201,9 +201,23 @@ __global__ void scatter_outcome_kernel(const uint32_t *planes, const int *iters,
     if (k >= count) return;
     const size_t t = static_cast<size_t>(idx[k]);
     const uint32_t *px = planes + static_cast<size_t>(k) * 2 * wn;
-    for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < wn; w += gridDim.x * blockDim.x) {
-        tp.est_x[t * wn + w] = px[w];
-        tp.est_z[t * wn + w] = px[wn + w];
+    const int stride = gridDim.x * blockDim.x;
+    if ((wn & 3) == 0) {
+        // 16 bytes per load: the planes may sit in host memory, where every
+        // load is a bus round trip (rows 16-byte aligned: 8 wn bytes apart,
+        // from aligned bases)
+        const int w4 = wn >> 2;
+        const uint4 *sx = reinterpret_cast<const uint4 *>(px), *sz = sx + w4;
+        uint4 *dx = reinterpret_cast<uint4 *>(tp.est_x + t * wn), *dz = reinterpret_cast<uint4 *>(tp.est_z + t * wn);
+        for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < w4; w += stride) {
+            dx[w] = sx[w];
+            dz[w] = sz[w];
+        }
+    } else {
+        for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < wn; w += stride) {
+            tp.est_x[t * wn + w] = px[w];
+            tp.est_z[t * wn + w] = px[wn + w];
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         tp.iters[t] = iters[k];
