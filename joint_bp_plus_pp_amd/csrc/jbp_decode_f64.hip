@@ -138,6 +138,7 @@ __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPla
 size_t decode_f64_scratch_bytes(const DevGraph &g, int slots) {
     size_t per = (6 * static_cast<size_t>(g.side[0].cap + g.side[1].cap)) * sizeof(double) +
                  ((static_cast<size_t>(g.n) + 7) / 8) * sizeof(double);
+    per = (per + 255) / 256 * 256;  // every slot 256-byte aligned (16-byte message pairs)
     return per * static_cast<size_t>(slots);
 }
 

@@ -54,8 +54,9 @@ __device__ __forceinline__ void pair_sums(const double *m, int side, double &q0,
 template <bool PAIR>
 __device__ __forceinline__ void store_v2c(double *v2c, size_t row, int side, const M4 &m) {
     if (PAIR) {
-        double *o = v2c + 2 * row;
-        pair_sums(m.v, side, o[0], o[1]);
+        double2 o;  // one 16-byte store: a wavefront writes whole cache lines
+        pair_sums(m.v, side, o.x, o.y);
+        *reinterpret_cast<double2 *>(v2c + 2 * row) = o;
     } else {
         double *o = v2c + 4 * row;
         o[0] = m.v[0]; o[1] = m.v[1]; o[2] = m.v[2]; o[3] = m.v[3];
