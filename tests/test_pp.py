@@ -192,3 +192,29 @@ def test_pp_after_hbm_engine_first_pass(pp_decoder):
     s_l, l = run(pp_decoder, g, jbp.FP32, engine=jbp.ENGINE_LDS)
     assert np.mean((h["success"] == l["success"]) & (h["pp_flags"] == l["pp_flags"])) >= 0.97
     assert s_h["pp_trials"] > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("p,max_iter", [(0.04, 8), (0.08, 103)], ids=["it8", "it103"])
+def test_pp_many_batches_equal_single_batch_calls(pp_decoder, p, max_iter):
+    """One call whose stage runs three batches of up to 1,024 trials (two
+    buffer sets and streams, the next batch queued right after the first
+    device phase at max-iter 8, after the last step at max-iter 103) gives,
+    trial for trial, what calls small enough for one batch give on the same
+    trial indices.  The one-batch path is the one the reference fixtures pin."""
+    n, part = 2600, 650
+    kw = dict(max_iter=max_iter, precision=jbp.FP64, pp=True, want_est=True)
+    print("pp batches", p, max_iter, flush=True)
+    s_all, big = pp_decoder.simulate(41, n, p, first_trial=1, **kw)
+    assert s_all["pp_trials"] > 2048  # three batches
+    pieces, stats = [], []
+    for k in range(0, n, part):
+        s, per = pp_decoder.simulate(41, part, p, first_trial=1 + k, **kw)
+        assert s["pp_trials"] < 1024
+        pieces.append(per)
+        stats.append(s)
+    for key in ("iterations", "syn_match", "success", "exact", "pp_flags"):
+        np.testing.assert_array_equal(big[key], np.concatenate([q[key] for q in pieces]), err_msg=key)
+    np.testing.assert_array_equal(big["est"], np.concatenate([q["est"] for q in pieces]))
+    for key in COUNTERS + ["failures", "bp_failures", "pp_trials"]:
+        assert int(s_all[key]) == sum(int(s[key]) for s in stats), key
