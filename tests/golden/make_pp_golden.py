@@ -43,9 +43,12 @@ WORKLOADS = [("p0.04_it8", 0.04, 8, 256, 7, 5, 0.0),
              ("p0.02_it1", 0.02, 1, 128, 5, 5, 0.0),
              ("p0.05_it115", 0.05, 115, 512, 11, 5, 0.0),
              ("p0.04_it10_damp0.3", 0.04, 10, 128, 13, 5, 0.3)]
-# a small hypergraph-product code (built by libjbp's seeded generator), no
-# trapping-set tables: (m, n, col_w, row_w, seed), p, max_iter, trials, seed
-HGP = ((15, 20, 3, 4, 5), 0.05, 6, 256, 3)
+# small hypergraph-product codes (built by libjbp's seeded generator), no
+# trapping-set tables: name, (m, n, col_w, row_w, seed), p, max_iter, trials,
+# seed.  HGP(21x28) has 588 checks per side: 19 words per syndrome, an odd
+# count (the stage then copies syndromes trial by trial).
+HGPS = [("hgp15x20", (15, 20, 3, 4, 5), 0.05, 6, 256, 3),
+        ("hgp21x28", (21, 28, 3, 4, 2), 0.04, 9, 256, 4)]
 
 
 def pack2(states):
@@ -100,6 +103,34 @@ def read_entries(name, side):
         vars_ = [int(a[1:].split(":")[0]) * P + int(a.split(":")[1]) for a in v]
         out.append(vars_)
     return out
+
+
+def hgp_fixtures(h):
+    """Per-trial PP outcomes of the reference on the small HGP codes."""
+    import sys
+    sys.path.insert(0, ROOT)
+    import joint_bp_plus_pp_amd as jbp
+    h.ref_set_checks.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    for name, (hm, hn, cw, rw, hseed), p, max_iter, trials, seed in HGPS:
+        code = jbp.Code.hgp(hm, hn, cw, rw, hseed)
+        xs, zs = code.checks(0), code.checks(1)
+        h.ref_set_checks(code.n, xs.shape[0], xs.shape[1], np.ascontiguousarray(xs).ctypes.data,
+                         zs.shape[0], zs.shape[1], np.ascontiguousarray(zs).ctypes.data)
+        h.ref_ets_clear()
+        it = np.zeros(trials, np.int32)
+        ok, su, ex, fl = (np.zeros(trials, np.uint8) for _ in range(4))
+        est = np.zeros((trials, code.n), np.int32)
+        ctr = np.zeros(10, np.int64)
+        sec = h.ref_simulate_pp(seed, 1, trials, p, max_iter, 0.0, 5, 8, it.ctypes.data, ok.ctypes.data,
+                                su.ctypes.data, ex.ctypes.data, fl.ctypes.data, est.ctypes.data,
+                                ctr.ctypes.data)
+        digests = np.array([est_digest(e) for e in est], dtype="S40")
+        np.savez_compressed(os.path.join(HERE, "pp_trials_%s.npz" % name),
+                            hgp=np.array([hm, hn, cw, rw, hseed]), p=p, max_iter=max_iter, seed=seed,
+                            flip_hist=5, damping=0.0, first_trial=1, iterations=it, syn_match=ok,
+                            success=su, exact=ex, flags=fl, est_sha1=digests, counters=ctr)
+        print(name, "%.1fs" % sec, "counters", ctr.tolist(), "bp failures",
+              int(np.count_nonzero((fl & 1) == 0)))
 
 
 def main():
@@ -260,29 +291,7 @@ def main():
                            for k, v in llr_rec.items()})
     print("llr:", list(zip(llr_rec["max_iter"], llr_rec["iterations"], llr_rec["syn_match"])))
 
-    # small HGP code, no tables
-    import sys
-    sys.path.insert(0, ROOT)
-    import joint_bp_plus_pp_amd as jbp
-    (hm, hn, cw, rw, hseed), p, max_iter, trials, seed = HGP
-    code = jbp.Code.hgp(hm, hn, cw, rw, hseed)
-    xs, zs = code.checks(0), code.checks(1)
-    h.ref_set_checks.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
-    h.ref_set_checks(code.n, xs.shape[0], xs.shape[1], np.ascontiguousarray(xs).ctypes.data,
-                     zs.shape[0], zs.shape[1], np.ascontiguousarray(zs).ctypes.data)
-    h.ref_ets_clear()
-    it = np.zeros(trials, np.int32)
-    ok, su, ex, fl = (np.zeros(trials, np.uint8) for _ in range(4))
-    est = np.zeros((trials, code.n), np.int32)
-    ctr = np.zeros(10, np.int64)
-    sec = h.ref_simulate_pp(seed, 1, trials, p, max_iter, 0.0, 5, 8, it.ctypes.data, ok.ctypes.data,
-                            su.ctypes.data, ex.ctypes.data, fl.ctypes.data, est.ctypes.data,
-                            ctr.ctypes.data)
-    digests = np.array([est_digest(e) for e in est], dtype="S40")
-    np.savez_compressed(os.path.join(HERE, "pp_trials_hgp15x20.npz"), hgp=np.array([hm, hn, cw, rw, hseed]),
-                        p=p, max_iter=max_iter, seed=seed, flip_hist=5, damping=0.0, first_trial=1, iterations=it,
-                        syn_match=ok, success=su, exact=ex, flags=fl, est_sha1=digests, counters=ctr)
-    print("hgp15x20", "%.1fs" % sec, "counters", ctr.tolist())
+    hgp_fixtures(h)
 
     # the reference program itself, PP on: its ETS Load table, a Run Summary
     # and a single --err decode that PP repairs.  It runs in a scratch
@@ -324,4 +333,8 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    import sys
+    if sys.argv[1:] == ["hgp"]:  # only the HGP fixtures
+        hgp_fixtures(load())
+    else:
+        main()

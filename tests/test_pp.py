@@ -60,11 +60,14 @@ def test_pp_fp64_equals_reference_per_trial(pp_decoder, name):
 
 
 @pytest.mark.gpu
-def test_pp_fp64_hgp_equals_reference_per_trial():
+@pytest.mark.parametrize("name", ["hgp15x20", "hgp21x28"])
+def test_pp_fp64_hgp_equals_reference_per_trial(name):
+    """Small HGP codes without tables; HGP(21x28) has 19 syndrome words per
+    side, an odd count (the stage copies its syndromes trial by trial)."""
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
-    g = golden_npz("pp_trials_hgp15x20.npz")
+    g = golden_npz("pp_trials_%s.npz" % name)
     m, n, cw, rw, seed = (int(v) for v in g["hgp"])
     code = jbp.Code.hgp(m, n, cw, rw, seed)
     with jbp.Decoder(code, 0) as dec:
