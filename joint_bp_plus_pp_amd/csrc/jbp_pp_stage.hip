@@ -440,9 +440,10 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
     auto syn_off = [&](int K, int s) -> size_t {
         return s == 0 ? 0 : ((static_cast<size_t>(K) * g.wm[0] + 1) & ~static_cast<size_t>(1));
     };
-    // Iterations of the first device phase: H, then iteration H itself (the
-    // first at which a trigger can fire; nothing the host decides before it
-    // changes it, since a trial that converges earlier stops on the device).
+    // Iterations of the first device phase: the H iterations 0..H-1 before
+    // any trigger can fire, plus iteration H, the first at which one can
+    // (nothing the host decides before it changes it: a trial that
+    // converges earlier stops on the device).
     const int HA = H + 1;
     const size_t row = 4 * static_cast<size_t>(n);
     const size_t tot_off = 6 * static_cast<size_t>(g.side[0].cap + g.side[1].cap);
