@@ -1,27 +1,28 @@
 // Post-processing stage of a simulate chunk: belief propagation for the
 // trials that are still unconverged when the reference's PP triggers first
-// become possible, one iteration per launch on the device, with the PP
-// decisions and repairs (jbp_pp.cpp) on host threads between launches.
+// become possible, on the device, with the PP decisions and repairs
+// (jbp_pp.cpp) on host threads.
 //
 // Why the split works: the reference only considers PP from iteration 100
 // on (or at the last iteration, jointbp_ets.cpp:2563), so a trial that BP
 // finishes in H = min(99, max_iter - 1) iterations never reaches it.  Those
 // trials keep the ordinary decoders' results.  The rest are re-decoded here
-// from iteration 1 in FP64 with the reference's operation order, so their
-// estimates, iteration counts and marginals are the reference's own:
-//   phase A: iterations 1..H in one launch per batch, recording per
-//            iteration the unsatisfied-check counts and whether the estimate
-//            equals the previous / the one before (all the trigger logic
-//            reads before a trigger can fire);
-//   phase B: from iteration H+1, one launch per iteration; the host reads
+// from the first iteration in FP64 with the reference's operation order, so
+// their estimates, iteration counts and marginals are the reference's own:
+//   first phase: iterations 0..H in one launch per batch of up to 1,024
+//            trials, recording per iteration the unsatisfied-check counts and
+//            whether the estimate equals the previous / the one before (all
+//            the trigger logic reads), and iteration H's marginals, which go
+//            to the host with the records;
+//   later iterations (max_iter above 100): one launch each; the host reads
 //            the records and both latest estimates, runs the trigger state
 //            machine and, where it fires, ETS / flip / OSD (OSD's |LLR| come
-//            from the marginals the kernel leaves in device memory, copied
-//            only for the trials that need them); trials that end at
-//            max_iter get the post-decode pass.
-// Final estimates, iteration counts and syndrome flags go back into the
-// chunk's planes, so the residual test and counters that follow treat these
-// trials like any other.
+//            from the marginals); trials that end at max_iter get the
+//            post-decode pass.
+// Batches alternate between two buffer sets and streams, so that the host
+// repairs of one overlap the next one's launch.  Final estimates, iteration
+// counts and syndrome flags go back into the chunk's planes, so the residual
+// test and counters that follow treat these trials like any other.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -415,6 +416,10 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
     const size_t batch = std::max<size_t>(1, std::min<size_t>(kPPSlots, kPPStateBytes / stride));
     const size_t nbatch = (trials.size() + batch - 1) / batch;
     hipStream_t streams[2] = {ctx->stream, ctx->pp_stream};
+    // Every return, an error's included, leaves both streams drained: queued
+    // copies and kernels use the context's buffers, which a later call may
+    // grow or free.
+    auto body = [&]() -> int {
 
     // The slot list 0..batch-1 and every trial index of the stage, uploaded
     // once: a batch's first launches then have no copy ahead of them in
@@ -820,6 +825,13 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
     HIPCHK(hipStreamSynchronize(streams[0]));
     HIPCHK(hipStreamSynchronize(streams[1]));
     return JBP_OK;
+    };
+    const int status = body();
+    if (status != JBP_OK) {
+        (void)hipStreamSynchronize(streams[0]);
+        (void)hipStreamSynchronize(streams[1]);
+    }
+    return status;
 }
 
 }  // namespace jbp
