@@ -1096,8 +1096,22 @@ int jbp_create(const jbp_graph_desc *g, int device_id, jbp_ctx **out, char *err,
                 pad[static_cast<size_t>(v) * W + (i - h.var_off[v])] = h.var_edge[i];
         if ((rc = dev_upload(ctx, pad, &dg.vpad[s], err, errlen))) return fail(rc);
     }
-    ctx->qc = detect_qc(ctx->hs, ctx->n, ctx->qcp) &&
-              qc_f32_supported(ctx->qcp, device_id, nullptr, 0);
+    const bool is_qc = detect_qc(ctx->hs, ctx->n, ctx->qcp);
+    ctx->qc = is_qc && qc_f32_supported(ctx->qcp, device_id, nullptr, 0);
+    if (is_qc) {
+        const int JL = ctx->qcp.J * ctx->qcp.L;
+        std::vector<int> tab(6 * JL);
+        for (int s = 0; s < 2; ++s)
+            for (int i = 0; i < JL; ++i) {
+                tab[(3 * s + 0) * JL + i] = ctx->qcp.a[s][i];
+                tab[(3 * s + 1) * JL + i] = ctx->qcp.b[s][i];
+                tab[(3 * s + 2) * JL + i] = ctx->qcp.ainv[s][i];
+            }
+        if ((rc = dev_upload(ctx, tab, &dg.qc_tab, err, errlen))) return fail(rc);
+        dg.qc_P = ctx->qcp.P;
+        dg.qc_J = ctx->qcp.J;
+        dg.qc_L = ctx->qcp.L;
+    }
     if (ctx->qc) {
         const int JL = ctx->qcp.J * ctx->qcp.L;
         std::vector<int> tab(4 * JL);

@@ -40,6 +40,12 @@ struct DevGraph {
     // side (rows 16-byte aligned)
     int cpad_w = 0;
     const int *cpad[2] = {nullptr, nullptr};
+    // quasi-cyclic codes (null otherwise): per side, the block maps a, b,
+    // ainv of check r*P + x meeting variable cc*P + (a*x + b mod P), as
+    // [side][a | b | ainv][r*L + cc]; the FP64 kernels form a variable's
+    // message rows from them instead of loading row tables
+    const int *qc_tab = nullptr;
+    int qc_P = 0, qc_J = 0, qc_L = 0;
 };
 
 // Quasi-cyclic structure: check (r*P + x) of side g meets variable

@@ -44,12 +44,17 @@ __device__ __forceinline__ SlotPtrs slot_ptrs(const DevGraph &g, double *scratch
 // Blocks of up to 512 threads (the tuner's largest); the second bound is
 // waves per SIMD (2: up to 256 VGPRs).  Degree 12 / 3 with pair sums needs
 // about 143, so three 256-thread blocks share a compute unit.
-template <int MAXC, int MAXV, bool PAIR>
+template <int MAXC, int MAXV, bool PAIR, bool QC>
 __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPlanes tp, int count, NoiseTab nt, int max_iter,
                                   double damping, double *scratch, size_t stride, int *counter) {
     __shared__ int s_trial;
+    __shared__ int s_qc[QC ? 6 * kQcMaxBlocks : 1];  // the block maps (QC codes)
     const int tid = threadIdx.x;
     const int bs = blockDim.x;
+    if (QC) {
+        for (int i = tid; i < 6 * g.qc_J * g.qc_L; i += bs) s_qc[i] = g.qc_tab[i];
+        __syncthreads();
+    }
     const SlotPtrs sp = slot_ptrs(g, scratch, stride);
     const int mx = g.side[0].m, mz = g.side[1].m;
     for (;;) {
@@ -84,14 +89,14 @@ __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPla
             // keeps both inlined recursions' registers live
             #pragma unroll 1
             for (int c = tid; c < mx; c += bs)
-                check_update<MAXC, PAIR>(g.side[0], 0, c, (tsx[c >> 5] >> (c & 31)) & 1, sp.v2c[0], sp.c2v[0]);
+                check_update<MAXC, PAIR, QC>(g.side[0], 0, c, (tsx[c >> 5] >> (c & 31)) & 1, sp.v2c[0], sp.c2v[0]);
             #pragma unroll 1
             for (int c = tid; c < mz; c += bs)
-                check_update<MAXC, PAIR>(g.side[1], 1, c, (tsz[c >> 5] >> (c & 31)) & 1, sp.v2c[1], sp.c2v[1]);
+                check_update<MAXC, PAIR, QC>(g.side[1], 1, c, (tsz[c >> 5] >> (c & 31)) & 1, sp.v2c[1], sp.c2v[1]);
             __syncthreads();
             for (int v = tid; v < g.n; v += bs)
                 sp.est[v] = static_cast<uint8_t>(
-                    var_update<MAXV, PAIR>(g, v, prior, damping, sp.v2c, sp.c2v, nullptr));
+                    var_update<MAXV, PAIR, QC>(g, v, prior, damping, sp.v2c, sp.c2v, nullptr, s_qc));
             __syncthreads();
             // both syndromes of est (jointbp_ets.cpp:2521-2525)
             int bad = 0;
@@ -152,24 +157,20 @@ void launch_decode_f64(const DevGraph &g, TrialPlanes tp, int count, const Noise
     // permutation-block codes, 16 / 4, or 32 / 16
     const bool exact12 = g.max_cdeg <= 12 && g.max_vdeg <= 3;
     const bool small = g.max_cdeg <= 16 && g.max_vdeg <= 4, pair = damping <= 0.0;
-    if (exact12 && pair)
-        hipLaunchKernelGGL((decode_f64_kernel<12, 3, true>), dim3(grid), dim3(block), 0, st, g, tp,
-                           count, nt, max_iter, damping, scratch, stride, counter);
-    else if (exact12)
-        hipLaunchKernelGGL((decode_f64_kernel<12, 3, false>), dim3(grid), dim3(block), 0, st, g, tp,
-                           count, nt, max_iter, damping, scratch, stride, counter);
-    else if (small && pair)
-        hipLaunchKernelGGL((decode_f64_kernel<16, 4, true>), dim3(grid), dim3(block), 0, st, g, tp,
-                           count, nt, max_iter, damping, scratch, stride, counter);
-    else if (small)
-        hipLaunchKernelGGL((decode_f64_kernel<16, 4, false>), dim3(grid), dim3(block), 0, st, g, tp,
-                           count, nt, max_iter, damping, scratch, stride, counter);
-    else if (pair)
-        hipLaunchKernelGGL((decode_f64_kernel<32, 16, true>), dim3(grid), dim3(block), 0, st, g, tp,
-                           count, nt, max_iter, damping, scratch, stride, counter);
-    else
-        hipLaunchKernelGGL((decode_f64_kernel<32, 16, false>), dim3(grid), dim3(block), 0, st, g, tp,
-                           count, nt, max_iter, damping, scratch, stride, counter);
+    // quasi-cyclic with 3 x 12 blocks: rows computed from the block maps
+    const bool qc = g.qc_tab && g.qc_J == 3 && g.qc_L == 12 && exact12;
+#define JBP_F64(C, V, PR, Q)                                                                              \
+    hipLaunchKernelGGL((decode_f64_kernel<C, V, PR, Q>), dim3(grid), dim3(block), 0, st, g, tp, count, nt, \
+                       max_iter, damping, scratch, stride, counter)
+    if (qc && pair) JBP_F64(12, 3, true, true);
+    else if (qc) JBP_F64(12, 3, false, true);
+    else if (exact12 && pair) JBP_F64(12, 3, true, false);
+    else if (exact12) JBP_F64(12, 3, false, false);
+    else if (small && pair) JBP_F64(16, 4, true, false);
+    else if (small) JBP_F64(16, 4, false, false);
+    else if (pair) JBP_F64(32, 16, true, false);
+    else JBP_F64(32, 16, false, false);
+#undef JBP_F64
 }
 
 }  // namespace jbp

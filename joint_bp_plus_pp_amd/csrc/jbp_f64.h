@@ -69,10 +69,11 @@ __device__ __forceinline__ void store_v2c(double *v2c, size_t row, int side, con
 // would each wait for the previous one.  Row i of a side's messages starts
 // at a block-uniform address, so each load is that base plus the check's
 // 32-bit offset.
-template <int MAXC, bool PAIR>
+// (QC: every check has degree MAXC, no offsets read)
+template <int MAXC, bool PAIR, bool QC = false>
 __device__ void check_update(const DevSide &sd, int side, int c, int syn, const double *v2c,
                              double *c2v) {
-    const int deg = sd.chk_off[c + 1] - sd.chk_off[c];
+    const int deg = QC ? MAXC : sd.chk_off[c + 1] - sd.chk_off[c];
     const size_t m = static_cast<size_t>(sd.m);
     constexpr int W = PAIR ? 2 : 4;  // doubles per v2c row
     double q0[MAXC], q1[MAXC], pe[MAXC + 1], po[MAXC + 1];
@@ -133,6 +134,15 @@ __device__ __forceinline__ M4 c2v_pair(double a, double b, int side) {
     return side == 0 ? M4{{a, b, a, b}} : M4{{a, a, b, b}};
 }
 
+// q = a / P for 0 <= a < 2^22 through the float reciprocal, corrected to the
+// exact quotient (the estimate is off by at most one).
+__device__ __forceinline__ int div_small(int a, int P, float inv_p) {
+    int q = static_cast<int>(static_cast<float>(a) * inv_p);
+    const int r = a - q * P;
+    q += r >= P ? 1 : (r < 0 ? -1 : 0);
+    return q;
+}
+
 // Joint variable update + hard decision (jointbp_ets.cpp:2451-2519).  The
 // suffix products are formed on the fly while walking the edges backwards;
 // each outgoing message uses the same operands as the reference's
@@ -142,25 +152,52 @@ __device__ __forceinline__ M4 c2v_pair(double a, double b, int side) {
 // hard decision; tot (optional) receives the normalised marginal.  Up to
 // degree 4 the incoming messages are all loaded first (positions past the
 // degree load the side's row 0 and are ignored) and kept in registers.
-template <int MAXV, bool PAIR>
+//
+// QC (a quasi-cyclic code with MAXV row blocks; qc = its block maps, as in
+// DevGraph::qc_tab): variable v = cc*P + y meets, in row block r, check
+// r*P + x with x = ainv*(y - b) mod P, at position cc of that check, so its
+// rows are computed instead of read from the offset and row tables.
+template <int MAXV, bool PAIR, bool QC = false>
 __device__ int var_update(const DevGraph &g, int v, const M4 &prior, double damping,
-                          double *const *v2c, const double *const *c2v, double *tot) {
-    constexpr bool kBatch = MAXV <= 4;
+                          double *const *v2c, const double *const *c2v, double *tot,
+                          const int *qc = nullptr) {
+    constexpr bool kBatch = MAXV <= 4 || QC;
     const M4 ident{{1.0, 1.0, 1.0, 1.0}};
     constexpr int NB = kBatch ? MAXV : 1;
     int b[2], deg[2];
     int row[2][NB];
     double ca[2][NB], cb[2][NB];
+    if (QC) {
+        const int P = g.qc_P, JL = MAXV * g.qc_L;
+        const float inv_p = 1.0f / static_cast<float>(P);
+        const int cc = div_small(v, P, inv_p), y = v - cc * P;
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        b[s] = g.side[s].var_off[v];
-        deg[s] = g.side[s].var_off[v + 1] - b[s];
+        for (int s = 0; s < 2; ++s) {
+            b[s] = 0;
+            deg[s] = MAXV;
+            const int *t = qc + 3 * s * JL;
+#pragma unroll
+            for (int r = 0; r < NB; ++r) {
+                const int blk = r * g.qc_L + cc;
+                const int u = t[2 * JL + blk] * (y - t[JL + blk] + P);  // ainv * (y - b + P)
+                const int x = u - div_small(u, P, inv_p) * P;
+                row[s][r] = cc * g.side[s].m + r * P + x;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            b[s] = g.side[s].var_off[v];
+            deg[s] = g.side[s].var_off[v + 1] - b[s];
+        }
     }
     if (kBatch) {
+        if (!QC) {
 #pragma unroll
-        for (int s = 0; s < 2; ++s)
+            for (int s = 0; s < 2; ++s)
 #pragma unroll
-            for (int i = 0; i < NB; ++i) row[s][i] = g.side[s].vrow[i < deg[s] ? b[s] + i : 0];
+                for (int i = 0; i < NB; ++i) row[s][i] = g.side[s].vrow[i < deg[s] ? b[s] + i : 0];
+        }
 #pragma unroll
         for (int s = 0; s < 2; ++s)
 #pragma unroll

@@ -82,12 +82,17 @@ constexpr int kRecInts = 4;
 // Waves per SIMD the register budget allows: 4 for degree 12 / 3 with pair
 // sums (128 VGPRs, every block of a 1024-trial batch resident at once), 2
 // for the larger recursions.
-template <int MAXC, int MAXV, bool PAIR>
+template <int MAXC, int MAXV, bool PAIR, bool QC>
 __global__ __launch_bounds__(256, (MAXC <= 12 && PAIR) ? 4 : 2) void pp_step_kernel(DevGraph g, TrialPlanes tp, NoiseTab nt, const int *list,
                                const int *slot_trial, int steps, int init, int want_tot,
                                double damping, char *state, size_t stride, int *rec, uint8_t *out) {
     __shared__ int s_cnt[2];
+    __shared__ int s_qc[QC ? 6 * kQcMaxBlocks : 1];  // the block maps (QC codes)
     const int tid = threadIdx.x, bs = blockDim.x;
+    if (QC) {
+        for (int i = tid; i < 6 * g.qc_J * g.qc_L; i += bs) s_qc[i] = g.qc_tab[i];
+        __syncthreads();
+    }
     const int b = blockIdx.x;
     const int slot = list[b];
     const int trial = slot_trial[slot];
@@ -113,15 +118,15 @@ __global__ __launch_bounds__(256, (MAXC <= 12 && PAIR) ? 4 : 2) void pp_step_ker
         // keeps both inlined recursions' registers live
         #pragma unroll 1
         for (int c = tid; c < mx; c += bs)
-            check_update<MAXC, PAIR>(g.side[0], 0, c, (tsx[c >> 5] >> (c & 31)) & 1, sp.v2c[0], sp.c2v[0]);
+            check_update<MAXC, PAIR, QC>(g.side[0], 0, c, (tsx[c >> 5] >> (c & 31)) & 1, sp.v2c[0], sp.c2v[0]);
         #pragma unroll 1
         for (int c = tid; c < mz; c += bs)
-            check_update<MAXC, PAIR>(g.side[1], 1, c, (tsz[c >> 5] >> (c & 31)) & 1, sp.v2c[1], sp.c2v[1]);
+            check_update<MAXC, PAIR, QC>(g.side[1], 1, c, (tsz[c >> 5] >> (c & 31)) & 1, sp.v2c[1], sp.c2v[1]);
         __syncthreads();
         double *tot = (want_tot == 1 || (want_tot == 2 && j == steps - 1)) ? sp.tot : nullptr;
         int changed1 = 0, changed2 = 0;
         for (int v = tid; v < n; v += bs) {
-            const int best = var_update<MAXV, PAIR>(g, v, prior, damping, sp.v2c, sp.c2v, tot);
+            const int best = var_update<MAXV, PAIR, QC>(g, v, prior, damping, sp.v2c, sp.c2v, tot, s_qc);
             const int e1 = sp.est[v], e2 = sp.prev[v];
             changed1 |= best != e1;
             changed2 |= best != e2;
@@ -162,25 +167,20 @@ void launch_step(const DevGraph &g, const TrialPlanes &tp, const NoiseTab &nt, c
                  hipStream_t st) {
     const bool exact12 = g.max_cdeg <= 12 && g.max_vdeg <= 3;
     const bool small = g.max_cdeg <= 16 && g.max_vdeg <= 4, pair = damping <= 0.0;
+    const bool qc = g.qc_tab && g.qc_J == 3 && g.qc_L == 12 && exact12;
     const int ini = init ? 1 : 0, wt = want_tot;
-    if (exact12 && pair)
-        hipLaunchKernelGGL((pp_step_kernel<12, 3, true>), dim3(count), dim3(256), 0, st, g, tp, nt,
-                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
-    else if (exact12)
-        hipLaunchKernelGGL((pp_step_kernel<12, 3, false>), dim3(count), dim3(256), 0, st, g, tp, nt,
-                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
-    else if (small && pair)
-        hipLaunchKernelGGL((pp_step_kernel<16, 4, true>), dim3(count), dim3(256), 0, st, g, tp, nt,
-                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
-    else if (small)
-        hipLaunchKernelGGL((pp_step_kernel<16, 4, false>), dim3(count), dim3(256), 0, st, g, tp, nt,
-                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
-    else if (pair)
-        hipLaunchKernelGGL((pp_step_kernel<32, 16, true>), dim3(count), dim3(256), 0, st, g, tp, nt,
-                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
-    else
-        hipLaunchKernelGGL((pp_step_kernel<32, 16, false>), dim3(count), dim3(256), 0, st, g, tp, nt,
-                           list, slot_trial, steps, ini, wt, damping, state, stride, rec, out);
+#define JBP_STEP(C, V, PR, Q)                                                                           \
+    hipLaunchKernelGGL((pp_step_kernel<C, V, PR, Q>), dim3(count), dim3(256), 0, st, g, tp, nt, list, \
+                       slot_trial, steps, ini, wt, damping, state, stride, rec, out)
+    if (qc && pair) JBP_STEP(12, 3, true, true);
+    else if (qc) JBP_STEP(12, 3, false, true);
+    else if (exact12 && pair) JBP_STEP(12, 3, true, false);
+    else if (exact12) JBP_STEP(12, 3, false, false);
+    else if (small && pair) JBP_STEP(16, 4, true, false);
+    else if (small) JBP_STEP(16, 4, false, false);
+    else if (pair) JBP_STEP(32, 16, true, false);
+    else JBP_STEP(32, 16, false, false);
+#undef JBP_STEP
 }
 
 // dst[k][0..len) = src[idx[k] * stride + off + 0..len), in 8-byte units
