@@ -87,12 +87,17 @@ __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPla
         for (int iter = 0; iter < max_iter; ++iter) {
             // one loop per side: a single loop branching between the sides
             // keeps both inlined recursions' registers live
-            #pragma unroll 1
-            for (int c = tid; c < mx; c += bs)
-                check_update<MAXC, PAIR, QC>(g.side[0], 0, c, (tsx[c >> 5] >> (c & 31)) & 1, sp.v2c[0], sp.c2v[0]);
-            #pragma unroll 1
-            for (int c = tid; c < mz; c += bs)
-                check_update<MAXC, PAIR, QC>(g.side[1], 1, c, (tsz[c >> 5] >> (c & 31)) & 1, sp.v2c[1], sp.c2v[1]);
+            if (QC && PAIR) {
+                check_pass_pipelined<MAXC>(g.side[0], 0, tid, bs, tsx, sp.v2c[0], sp.c2v[0]);
+                check_pass_pipelined<MAXC>(g.side[1], 1, tid, bs, tsz, sp.v2c[1], sp.c2v[1]);
+            } else {
+#pragma unroll 1
+                for (int c = tid; c < mx; c += bs)
+                    check_update<MAXC, PAIR, QC>(g.side[0], 0, c, (tsx[c >> 5] >> (c & 31)) & 1, sp.v2c[0], sp.c2v[0]);
+#pragma unroll 1
+                for (int c = tid; c < mz; c += bs)
+                    check_update<MAXC, PAIR, QC>(g.side[1], 1, c, (tsz[c >> 5] >> (c & 31)) & 1, sp.v2c[1], sp.c2v[1]);
+            }
             __syncthreads();
             for (int v = tid; v < g.n; v += bs)
                 sp.est[v] = static_cast<uint8_t>(

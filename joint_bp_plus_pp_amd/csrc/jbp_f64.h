@@ -123,6 +123,82 @@ __device__ void check_update(const DevSide &sd, int side, int c, int syn, const 
     }
 }
 
+// The check update split in two for a software-pipelined pass over a
+// thread's checks (quasi-cyclic codes, pair sums): check_load fetches a
+// check's twelve incoming pairs, check_finish computes and stores its
+// outgoing messages exactly as check_update does.
+template <int MAXC>
+__device__ __forceinline__ void check_load(const DevSide &sd, int c, const double *v2c, double (&q0)[MAXC],
+                                           double (&q1)[MAXC]) {
+    const size_t m = static_cast<size_t>(sd.m);
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+        const double2 mv = *reinterpret_cast<const double2 *>(v2c + 2 * i * m + 2 * c);
+        q0[i] = mv.x;
+        q1[i] = mv.y;
+    }
+}
+
+template <int MAXC>
+__device__ __forceinline__ void check_finish(const DevSide &sd, int side, int c, int syn, const double (&q0)[MAXC],
+                                             const double (&q1)[MAXC], double *c2v) {
+    const size_t m = static_cast<size_t>(sd.m);
+    double pe[MAXC + 1], po[MAXC + 1];
+    pe[0] = 1.0;
+    po[0] = 0.0;
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+        pe[i + 1] = pe[i] * q0[i] + po[i] * q1[i];
+        po[i + 1] = pe[i] * q1[i] + po[i] * q0[i];
+    }
+    double se = 1.0, so = 0.0;
+#pragma unroll
+    for (int i = MAXC - 1; i >= 0; --i) {
+        double p_even = pe[i] * se + po[i] * so;
+        double p_odd = pe[i] * so + po[i] * se;
+        double v0 = syn == 0 ? p_even : p_odd;
+        double v1 = syn == 0 ? p_odd : p_even;
+        double sum = side == 0 ? ((v0 + v1) + v0) + v1 : ((v0 + v0) + v1) + v1;
+        double2 o;
+        if (sum <= 0.0) {
+            o.x = 0.25;
+            o.y = 0.25;
+        } else {
+            o.x = v0 / sum;
+            o.y = v1 / sum;
+        }
+        *reinterpret_cast<double2 *>(c2v + 2 * i * m + 2 * c) = o;
+        double ne = se * q0[i] + so * q1[i];
+        double no = se * q1[i] + so * q0[i];
+        se = ne;
+        so = no;
+    }
+}
+
+// A thread's checks c0, c0 + step, ... of one side: the next check's loads
+// are issued before the current one is computed (two register sets, no copy
+// between them, so the loads stay in flight during the arithmetic).
+template <int MAXC>
+__device__ void check_pass_pipelined(const DevSide &sd, int side, int c0, int step, const uint32_t *ts,
+                                     const double *v2c, double *c2v) {
+    const int m = sd.m;
+    if (c0 >= m) return;
+    double a0[MAXC], a1[MAXC], b0[MAXC], b1[MAXC];
+    int c = c0;
+    check_load<MAXC>(sd, c, v2c, a0, a1);
+    for (;;) {
+        const int c1 = c + step;
+        if (c1 < m) check_load<MAXC>(sd, c1, v2c, b0, b1);
+        check_finish<MAXC>(sd, side, c, (ts[c >> 5] >> (c & 31)) & 1, a0, a1, c2v);
+        if (c1 >= m) break;
+        const int c2 = c1 + step;
+        if (c2 < m) check_load<MAXC>(sd, c2, v2c, a0, a1);
+        check_finish<MAXC>(sd, side, c1, (ts[c1 >> 5] >> (c1 & 31)) & 1, b0, b1, c2v);
+        if (c2 >= m) break;
+        c = c2;
+    }
+}
+
 // (r: the edge's row)
 __device__ __forceinline__ M4 c2v_msg(const double *c2v, int r, int side) {
     double a = c2v[2 * static_cast<size_t>(r)];

@@ -81,9 +81,10 @@ constexpr int kRecInts = 4;
 // estimate and previous estimate to out[b].
 // Waves per SIMD the register budget allows: 4 for degree 12 / 3 with pair
 // sums (128 VGPRs, every block of a 1024-trial batch resident at once), 2
-// for the larger recursions.
+// for the larger recursions and for the quasi-cyclic pair path, whose
+// pipelined check pass holds two checks' messages.
 template <int MAXC, int MAXV, bool PAIR, bool QC>
-__global__ __launch_bounds__(256, (MAXC <= 12 && PAIR) ? 4 : 2) void pp_step_kernel(DevGraph g, TrialPlanes tp, NoiseTab nt, const int *list,
+__global__ __launch_bounds__(256, (MAXC <= 12 && PAIR && !QC) ? 4 : 2) void pp_step_kernel(DevGraph g, TrialPlanes tp, NoiseTab nt, const int *list,
                                const int *slot_trial, int steps, int init, int want_tot,
                                double damping, char *state, size_t stride, int *rec, uint8_t *out) {
     __shared__ int s_cnt[2];
@@ -116,12 +117,17 @@ __global__ __launch_bounds__(256, (MAXC <= 12 && PAIR) ? 4 : 2) void pp_step_ker
         if (tid == 0) s_cnt[0] = s_cnt[1] = 0;
         // one loop per side: a single loop branching between the sides
         // keeps both inlined recursions' registers live
-        #pragma unroll 1
-        for (int c = tid; c < mx; c += bs)
-            check_update<MAXC, PAIR, QC>(g.side[0], 0, c, (tsx[c >> 5] >> (c & 31)) & 1, sp.v2c[0], sp.c2v[0]);
-        #pragma unroll 1
-        for (int c = tid; c < mz; c += bs)
-            check_update<MAXC, PAIR, QC>(g.side[1], 1, c, (tsz[c >> 5] >> (c & 31)) & 1, sp.v2c[1], sp.c2v[1]);
+        if (QC && PAIR) {
+            check_pass_pipelined<MAXC>(g.side[0], 0, tid, bs, tsx, sp.v2c[0], sp.c2v[0]);
+            check_pass_pipelined<MAXC>(g.side[1], 1, tid, bs, tsz, sp.v2c[1], sp.c2v[1]);
+        } else {
+#pragma unroll 1
+            for (int c = tid; c < mx; c += bs)
+                check_update<MAXC, PAIR, QC>(g.side[0], 0, c, (tsx[c >> 5] >> (c & 31)) & 1, sp.v2c[0], sp.c2v[0]);
+#pragma unroll 1
+            for (int c = tid; c < mz; c += bs)
+                check_update<MAXC, PAIR, QC>(g.side[1], 1, c, (tsz[c >> 5] >> (c & 31)) & 1, sp.v2c[1], sp.c2v[1]);
+        }
         __syncthreads();
         double *tot = (want_tot == 1 || (want_tot == 2 && j == steps - 1)) ? sp.tot : nullptr;
         int changed1 = 0, changed2 = 0;
