@@ -53,6 +53,9 @@ struct DevGraph {
 // ainv/b are kept per (g, r, cc) for the variable-side inverse map
 // x = ainv*(y - b) mod P.
 constexpr int kQcMaxBlocks = 64;
+// Largest LDS staging area of the FP64 kernels' quasi-cyclic variable pass:
+// two workgroups still share a compute unit's 160 KB.
+constexpr size_t kF64MaxLds = 78 * 1024;
 struct QcParams {
     int P = 0, J = 0, L = 0;
     int a[2][kQcMaxBlocks];     // [side][r*L + cc]

@@ -8,14 +8,18 @@
 // Layout: one workgroup decodes one trial at a time and pulls the next trial
 // index from a device counter when it finishes (persistent, refill on
 // completion).  Its messages live in a private slice of a global scratch
-// buffer:
-//   v2c[side][E][4]  variable-to-check, four Pauli components
-//   c2v[side][E][2]  check-to-variable; the reference's X-side message is
-//                    {a, b, a, b} and Z-side {a, a, b, b}, so two values
-//                    carry it without loss
-//   est[n]           hard decision, one byte per qubit
-// Checks and variables are walked over the generic check-major CSR, so the
-// kernel serves any code, including ones without permutation structure.
+// buffer, position-major (row i*m + c: position i of check c, so a check
+// pass over consecutive checks reads and writes contiguous rows):
+//   v2c[side][rows][2 or 4]  variable-to-check: the two pair sums the check
+//                            update reads, or all four Pauli components
+//                            when damping mixes old and new messages
+//   c2v[side][rows][2]       check-to-variable; the reference's X-side
+//                            message is {a, b, a, b} and Z-side
+//                            {a, a, b, b}, so two values carry it
+//   est[n]                   hard decision, one byte per qubit
+// Any code is served through its CSR tables; quasi-cyclic codes with pair
+// sums compute their rows from the block maps instead and stage the
+// variable pass through LDS one column block at a time.
 #include "jbp_common.h"
 #include "jbp_f64.h"
 
@@ -43,12 +47,14 @@ __device__ __forceinline__ SlotPtrs slot_ptrs(const DevGraph &g, double *scratch
 
 // Blocks of up to 512 threads (the tuner's largest); the second bound is
 // waves per SIMD (2: up to 256 VGPRs).  Degree 12 / 3 with pair sums needs
-// about 143, so three 256-thread blocks share a compute unit.
+// about 143, so three 256-thread blocks share a compute unit; the
+// quasi-cyclic pair path (pipelined check pass, 72 KB of LDS) runs two.
 template <int MAXC, int MAXV, bool PAIR, bool QC>
 __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPlanes tp, int count, NoiseTab nt, int max_iter,
                                   double damping, double *scratch, size_t stride, int *counter) {
     __shared__ int s_trial;
     __shared__ int s_qc[QC ? 6 * kQcMaxBlocks : 1];  // the block maps (QC codes)
+    extern __shared__ double2 s_msg[];               // QC pair sums: a column block's rows
     const int tid = threadIdx.x;
     const int bs = blockDim.x;
     if (QC) {
@@ -99,9 +105,14 @@ __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPla
                     check_update<MAXC, PAIR, QC>(g.side[1], 1, c, (tsz[c >> 5] >> (c & 31)) & 1, sp.v2c[1], sp.c2v[1]);
             }
             __syncthreads();
-            for (int v = tid; v < g.n; v += bs)
-                sp.est[v] = static_cast<uint8_t>(
-                    var_update<MAXV, PAIR, QC>(g, v, prior, damping, sp.v2c, sp.c2v, nullptr, s_qc));
+            if (QC && PAIR) {
+                var_pass_qc_lds<MAXV>(g, prior, sp.v2c, sp.c2v, s_qc, s_msg,
+                                      [&](int v, int best, const M4 &) { sp.est[v] = static_cast<uint8_t>(best); });
+            } else {
+                for (int v = tid; v < g.n; v += bs)
+                    sp.est[v] = static_cast<uint8_t>(
+                        var_update<MAXV, PAIR, QC>(g, v, prior, damping, sp.v2c, sp.c2v, nullptr, s_qc));
+            }
             __syncthreads();
             // both syndromes of est (jointbp_ets.cpp:2521-2525)
             int bad = 0;
@@ -162,13 +173,19 @@ void launch_decode_f64(const DevGraph &g, TrialPlanes tp, int count, const Noise
     // permutation-block codes, 16 / 4, or 32 / 16
     const bool exact12 = g.max_cdeg <= 12 && g.max_vdeg <= 3;
     const bool small = g.max_cdeg <= 16 && g.max_vdeg <= 4, pair = damping <= 0.0;
-    // quasi-cyclic with 3 x 12 blocks: rows computed from the block maps
+    // quasi-cyclic with 3 x 12 blocks: rows computed from the block maps;
+    // with pair sums the variable pass stages a column block's rows in LDS
     const bool qc = g.qc_tab && g.qc_J == 3 && g.qc_L == 12 && exact12;
-#define JBP_F64(C, V, PR, Q)                                                                              \
-    hipLaunchKernelGGL((decode_f64_kernel<C, V, PR, Q>), dim3(grid), dim3(block), 0, st, g, tp, count, nt, \
-                       max_iter, damping, scratch, stride, counter)
-    if (qc && pair) JBP_F64(12, 3, true, true);
-    else if (qc) JBP_F64(12, 3, false, true);
+    const size_t lds = f64::var_pass_lds_bytes(g.side[0].m);
+    const bool qc_lds = qc && pair && lds <= kF64MaxLds;
+#define JBP_F64(C, V, PR, Q)                                                                                \
+    hipLaunchKernelGGL((decode_f64_kernel<C, V, PR, Q>), dim3(grid), dim3(block), (PR && Q) ? lds : 0, st, g, \
+                       tp, count, nt, max_iter, damping, scratch, stride, counter)
+    if (qc_lds) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(decode_f64_kernel<12, 3, true, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        JBP_F64(12, 3, true, true);
+    } else if (qc && !pair) JBP_F64(12, 3, false, true);
     else if (exact12 && pair) JBP_F64(12, 3, true, false);
     else if (exact12) JBP_F64(12, 3, false, false);
     else if (small && pair) JBP_F64(16, 4, true, false);

@@ -199,6 +199,10 @@ __device__ void check_pass_pipelined(const DevSide &sd, int side, int c0, int st
     }
 }
 
+__device__ __forceinline__ const double2 *c2v_rows(const double *c2v) {
+    return reinterpret_cast<const double2 *>(c2v);
+}
+
 // (r: the edge's row)
 __device__ __forceinline__ M4 c2v_msg(const double *c2v, int r, int side) {
     double a = c2v[2 * static_cast<size_t>(r)];
@@ -350,6 +354,136 @@ __device__ int var_update(const DevGraph &g, int v, const M4 &prior, double damp
         }
     }
     return best;
+}
+
+// var_update's arithmetic for one variable of degree MAXV on both sides with
+// pair sums, on messages already loaded (same operands, same order):
+// ca/cb[s][i] is the incoming pair of position i of side s; o[s][i] receives
+// the outgoing pair sums and total the normalised marginal.
+template <int MAXV>
+__device__ __forceinline__ int var_core_pair(const M4 &prior, const double (&ca)[2][MAXV], const double (&cb)[2][MAXV],
+                                             double2 (&o)[2][MAXV], M4 &total) {
+    const M4 ident{{1.0, 1.0, 1.0, 1.0}};
+    M4 pref[2][MAXV + 1];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        pref[s][0] = ident;
+#pragma unroll
+        for (int i = 0; i < MAXV; ++i) pref[s][i + 1] = mul4(pref[s][i], c2v_pair(ca[s][i], cb[s][i], s));
+    }
+    const M4 allx = pref[0][MAXV], allz = pref[1][MAXV];
+    total = mul4(prior, mul4(allx, allz));
+    normalize4(total);
+    int best = 0;
+    double bv = total.v[0];
+#pragma unroll
+    for (int k = 1; k < 4; ++k)
+        if (total.v[k] > bv) {
+            best = k;
+            bv = total.v[k];
+        }
+    M4 suf = ident;
+#pragma unroll
+    for (int i = MAXV - 1; i >= 0; --i) {
+        M4 out = mul4(mul4(prior, mul4(pref[0][i], suf)), allz);
+        normalize4(out);
+        pair_sums(out.v, 0, o[0][i].x, o[0][i].y);
+        suf = mul4(suf, c2v_pair(ca[0][i], cb[0][i], 0));
+    }
+    suf = ident;
+    const M4 px = mul4(prior, allx);
+#pragma unroll
+    for (int i = MAXV - 1; i >= 0; --i) {
+        M4 out = mul4(px, mul4(pref[1][i], suf));
+        normalize4(out);
+        pair_sums(out.v, 1, o[1][i].x, o[1][i].y);
+        suf = mul4(suf, c2v_pair(ca[1][i], cb[1][i], 1));
+    }
+    return best;
+}
+
+// Bytes of LDS var_pass_qc_lds needs: both sides' rows of one column block.
+inline size_t var_pass_lds_bytes(int m) { return 2 * static_cast<size_t>(m) * sizeof(double2); }
+
+// Variable pass of a quasi-cyclic code (MAXV row blocks of size P, m = MAXV*P
+// checks per side) with pair sums, staged through LDS.  The variables of
+// column block cc meet every check at position cc, so all their messages
+// lie in rows cc*m + [0, m) of each side, the row of check r*P + x being
+// cc*m + r*P + x.  The workgroup copies those rows of c2v into LDS with
+// coalesced loads, updates the block's P variables against LDS (each
+// outgoing pair overwrites the incoming pair of the same edge, which no
+// other variable touches), and copies the rows out to v2c.  lds holds 2*m
+// pairs (side 0 then side 1).  visit(v, best, total) receives each
+// variable's hard decision and normalised marginal.  The caller synchronises
+// the workgroup before (c2v complete) and after (v2c complete).  Workgroup
+// sizes are multiples of 64.
+template <int MAXV, class Visit>
+__device__ void var_pass_qc_lds(const DevGraph &g, const M4 &prior, double *const *v2c, const double *const *c2v,
+                                const int *qc, double2 *lds, Visit visit) {
+    const int tid = threadIdx.x, bs = blockDim.x;
+    const int P = g.qc_P, L = g.qc_L, JL = MAXV * L, m = g.side[0].m, rows = 2 * m;
+    const float inv_p = 1.0f / static_cast<float>(P);
+    // LDS row k of column block cc from c2v: 16-byte loads written to LDS by
+    // the load itself (lane l of a wavefront writes at the wavefront's LDS
+    // base + 16*l, so thread tid owns LDS rows tid + j*bs, as in the copy out)
+    auto fetch = [&](int cc) {
+        const size_t base = static_cast<size_t>(cc) * m;
+        for (int k = tid; k < rows; k += bs) {
+            const double2 *src = k < m ? c2v_rows(c2v[0]) + base + k : c2v_rows(c2v[1]) + base + (k - m);
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                             (__attribute__((address_space(3))) void *)(lds + (k & ~63)), 16, 0, 0);
+        }
+    };
+    fetch(0);
+    for (int cc = 0; cc < L; ++cc) {
+        __builtin_amdgcn_s_waitcnt(0);  // this thread's LDS loads have landed
+        __syncthreads();
+        for (int y = tid; y < P; y += bs) {
+            int idx[2][MAXV];
+            double ca[2][MAXV], cb[2][MAXV];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int *t = qc + 3 * s * JL;
+#pragma unroll
+                for (int r = 0; r < MAXV; ++r) {
+                    const int blk = r * L + cc;
+                    const int u = t[2 * JL + blk] * (y - t[JL + blk] + P);  // ainv * (y - b + P)
+                    idx[s][r] = s * m + r * P + (u - div_small(u, P, inv_p) * P);
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int r = 0; r < MAXV; ++r) {
+                    const double2 mv = lds[idx[s][r]];
+                    ca[s][r] = mv.x;
+                    cb[s][r] = mv.y;
+                }
+            double2 o[2][MAXV];
+            M4 total;
+            const int best = var_core_pair<MAXV>(prior, ca, cb, o, total);
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int r = 0; r < MAXV; ++r) lds[idx[s][r]] = o[s][r];
+            visit(cc * P + y, best, total);
+        }
+        __syncthreads();
+        // copy out (each thread its own LDS rows), then the next block's rows
+        // in over the same LDS rows once the reads have returned
+        const size_t base = static_cast<size_t>(cc) * m;
+        double2 *out0 = reinterpret_cast<double2 *>(v2c[0]) + base;
+        double2 *out1 = reinterpret_cast<double2 *>(v2c[1]) + base;
+        for (int k = tid; k < rows; k += bs) {
+            const double2 mv = lds[k];
+            if (k < m) out0[k] = mv;
+            else out1[k - m] = mv;
+        }
+        if (cc + 1 < L) {
+            __builtin_amdgcn_s_waitcnt(0);
+            fetch(cc + 1);
+        }
+    }
 }
 
 // Parity of check c of one side over est (X checks read the X bit, Z checks

@@ -82,13 +82,15 @@ constexpr int kRecInts = 4;
 // Waves per SIMD the register budget allows: 4 for degree 12 / 3 with pair
 // sums (128 VGPRs, every block of a 1024-trial batch resident at once), 2
 // for the larger recursions and for the quasi-cyclic pair path, whose
-// pipelined check pass holds two checks' messages.
+// pipelined check pass holds two checks' messages and whose variable pass
+// stages a column block's rows in LDS (72 KB: two blocks per compute unit).
 template <int MAXC, int MAXV, bool PAIR, bool QC>
 __global__ __launch_bounds__(256, (MAXC <= 12 && PAIR && !QC) ? 4 : 2) void pp_step_kernel(DevGraph g, TrialPlanes tp, NoiseTab nt, const int *list,
                                const int *slot_trial, int steps, int init, int want_tot,
                                double damping, char *state, size_t stride, int *rec, uint8_t *out) {
     __shared__ int s_cnt[2];
     __shared__ int s_qc[QC ? 6 * kQcMaxBlocks : 1];  // the block maps (QC codes)
+    extern __shared__ double2 s_msg[];               // QC pair sums: a column block's rows
     const int tid = threadIdx.x, bs = blockDim.x;
     if (QC) {
         for (int i = tid; i < 6 * g.qc_J * g.qc_L; i += bs) s_qc[i] = g.qc_tab[i];
@@ -131,13 +133,24 @@ __global__ __launch_bounds__(256, (MAXC <= 12 && PAIR && !QC) ? 4 : 2) void pp_s
         __syncthreads();
         double *tot = (want_tot == 1 || (want_tot == 2 && j == steps - 1)) ? sp.tot : nullptr;
         int changed1 = 0, changed2 = 0;
-        for (int v = tid; v < n; v += bs) {
-            const int best = var_update<MAXV, PAIR, QC>(g, v, prior, damping, sp.v2c, sp.c2v, tot, s_qc);
+        auto record = [&](int v, int best) {
             const int e1 = sp.est[v], e2 = sp.prev[v];
             changed1 |= best != e1;
             changed2 |= best != e2;
             sp.prev[v] = static_cast<uint8_t>(e1);
             sp.est[v] = static_cast<uint8_t>(best);
+        };
+        if (QC && PAIR) {
+            var_pass_qc_lds<MAXV>(g, prior, sp.v2c, sp.c2v, s_qc, s_msg, [&](int v, int best, const M4 &total) {
+                if (tot) {
+                    double *t = tot + 4 * static_cast<size_t>(v);
+                    t[0] = total.v[0]; t[1] = total.v[1]; t[2] = total.v[2]; t[3] = total.v[3];
+                }
+                record(v, best);
+            });
+        } else {
+            for (int v = tid; v < n; v += bs)
+                record(v, var_update<MAXV, PAIR, QC>(g, v, prior, damping, sp.v2c, sp.c2v, tot, s_qc));
         }
         __syncthreads();
         int ux = 0, uz = 0;
@@ -174,12 +187,17 @@ void launch_step(const DevGraph &g, const TrialPlanes &tp, const NoiseTab &nt, c
     const bool exact12 = g.max_cdeg <= 12 && g.max_vdeg <= 3;
     const bool small = g.max_cdeg <= 16 && g.max_vdeg <= 4, pair = damping <= 0.0;
     const bool qc = g.qc_tab && g.qc_J == 3 && g.qc_L == 12 && exact12;
+    const size_t lds = f64::var_pass_lds_bytes(g.side[0].m);
+    const bool qc_lds = qc && pair && lds <= kF64MaxLds;
     const int ini = init ? 1 : 0, wt = want_tot;
-#define JBP_STEP(C, V, PR, Q)                                                                           \
-    hipLaunchKernelGGL((pp_step_kernel<C, V, PR, Q>), dim3(count), dim3(256), 0, st, g, tp, nt, list, \
-                       slot_trial, steps, ini, wt, damping, state, stride, rec, out)
-    if (qc && pair) JBP_STEP(12, 3, true, true);
-    else if (qc) JBP_STEP(12, 3, false, true);
+#define JBP_STEP(C, V, PR, Q)                                                                                    \
+    hipLaunchKernelGGL((pp_step_kernel<C, V, PR, Q>), dim3(count), dim3(256), (PR && Q) ? lds : 0, st, g, tp, nt, \
+                       list, slot_trial, steps, ini, wt, damping, state, stride, rec, out)
+    if (qc_lds) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(pp_step_kernel<12, 3, true, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        JBP_STEP(12, 3, true, true);
+    } else if (qc && !pair) JBP_STEP(12, 3, false, true);
     else if (exact12 && pair) JBP_STEP(12, 3, true, false);
     else if (exact12) JBP_STEP(12, 3, false, false);
     else if (small && pair) JBP_STEP(16, 4, true, false);
