@@ -389,9 +389,11 @@ def main():
     value = done / elapsed
     launches = max(1, dec.last_kernel_ms(1)[1] - launches0)
     kern_s = sum(dev_ms) / 1e3  # sum of the decoder launches' HIP-event durations
-    achieved = sum(step_iters) * BYTES_PER_ITER / kern_s / 1e9
+    # FP64: 64 B of message traffic per edge per iteration (pair sums)
+    per_iter = BYTES_PER_ITER if args.precision == 32 else 64 * EDGES_PER_ITER
+    achieved = sum(step_iters) * per_iter / kern_s / 1e9
     traffic = None
-    if os.path.exists(TRAFFIC_FILE):
+    if args.precision == 32 and os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
         traffic = t.get("hbm_bytes_per_launch")
@@ -473,7 +475,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "decode_qc_f32_kernel",
+            "kernel": "decode_qc_f32_kernel" if args.precision == 32 else "decode_f64_kernel",
             "kernel_ms_per_launch": round(sum(dev_ms) / launches, 4),
             "launches": launches,
         },
