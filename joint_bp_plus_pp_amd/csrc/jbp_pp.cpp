@@ -413,6 +413,18 @@ void syn_diff_init(const Graph &g, int side, const uint8_t *est, const uint8_t *
     sd.ones = ones;
 }
 
+void syn_diff_from_bits(const Graph &g, int side, const uint32_t *bits, SynDiff &sd) {
+    const int m = g.m[side];
+    sd.d.resize(m);
+    int ones = 0;
+    for (int c = 0; c < m; ++c) {
+        const uint8_t b = static_cast<uint8_t>((bits[c >> 5] >> (c & 31)) & 1u);
+        sd.d[c] = b;
+        ones += b;
+    }
+    sd.ones = ones;
+}
+
 EtsFix ets_fix(const Graph &g, const Tables &t, int side, const uint8_t *target,
                std::vector<uint8_t> &est) {
     if (t.groups.empty()) return EtsFix();
@@ -694,11 +706,16 @@ bool all_clear(const SynDiff (&diff)[2]) { return diff[0].ones == 0 && diff[1].o
 
 void attempt_begin(const Graph &g, const Tables &t, const Params &prm, const Trial &tr,
                    const std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz,
-                   Attempt &a) {
+                   Attempt &a, const uint32_t *const *dbits) {
     a.ok = a.want_osd = a.ets_used = a.ets6 = a.ets12 = a.ets_ok = a.flip_ok = false;
     a.trial = est;
-    syn_diff_init(g, 0, a.trial.data(), sx, a.diff[0]);
-    syn_diff_init(g, 1, a.trial.data(), sz, a.diff[1]);
+    if (dbits) {
+        syn_diff_from_bits(g, 0, dbits[0], a.diff[0]);
+        syn_diff_from_bits(g, 1, dbits[1], a.diff[1]);
+    } else {
+        syn_diff_init(g, 0, a.trial.data(), sx, a.diff[0]);
+        syn_diff_init(g, 1, a.trial.data(), sz, a.diff[1]);
+    }
     const EtsFix ex = ets_fix_diff(g, t, 0, a.diff[0], a.trial);
     const EtsFix ez = ets_fix_diff(g, t, 1, a.diff[1], a.trial);
     a.ets_used = ex.applied || ez.applied;
@@ -731,10 +748,16 @@ bool attempt_end(Trial &tr, Attempt &a, int t, std::vector<uint8_t> &est) {
 }
 
 PostOut post_begin(const Graph &g, const Tables &t, const Params &prm, const Trial &tr,
-                   std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz, SynDiff (&diff)[2]) {
+                   std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz, SynDiff (&diff)[2],
+                   const uint32_t *const *dbits) {
     PostOut o;
-    syn_diff_init(g, 0, est.data(), sx, diff[0]);
-    syn_diff_init(g, 1, est.data(), sz, diff[1]);
+    if (dbits) {
+        syn_diff_from_bits(g, 0, dbits[0], diff[0]);
+        syn_diff_from_bits(g, 1, dbits[1], diff[1]);
+    } else {
+        syn_diff_init(g, 0, est.data(), sx, diff[0]);
+        syn_diff_init(g, 1, est.data(), sz, diff[1]);
+    }
     const EtsFix ex = ets_fix_diff(g, t, 0, diff[0], est);
     const EtsFix ez = ets_fix_diff(g, t, 1, diff[1], est);
     o.ets_applied = ex.applied || ez.applied;

@@ -81,6 +81,9 @@ struct SynDiff {
     int ones = 0;
 };
 void syn_diff_init(const Graph &g, int side, const uint8_t *est, const uint8_t *target, SynDiff &sd);
+// The same from bit words already holding syndrome(est) XOR target (bit c of
+// word c / 32), as a device computes them beside the estimate.
+void syn_diff_from_bits(const Graph &g, int side, const uint32_t *bits, SynDiff &sd);
 
 // ETS repair of one side (jointbp_ets.cpp:3525-3668): applies to est the
 // first table entry, over the groups in order, whose variables explain the
@@ -181,9 +184,11 @@ struct Attempt {
     bool ok = false, want_osd = false;
     bool ets_used = false, ets6 = false, ets12 = false, ets_ok = false, flip_ok = false;
 };
+// (dbits: optional syndrome differences of est, X and Z, as bit words; else
+// they are computed from sx, sz)
 void attempt_begin(const Graph &g, const Tables &t, const Params &prm, const Trial &tr,
                    const std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz,
-                   Attempt &a);
+                   Attempt &a, const uint32_t *const *dbits = nullptr);
 void attempt_osd(const Graph &g, Attempt &a, const LlrSide &llr_x, const LlrSide &llr_z);
 bool attempt_end(Trial &tr, Attempt &a, int t, std::vector<uint8_t> &est);
 
@@ -203,7 +208,8 @@ PostOut post_decode(const Graph &g, const Tables &t, const Params &prm, const Tr
 // post_begin does ETS and the flip sets and sets want_osd; post_osd finishes.
 // diff: the caller's scratch, carried from one to the other.
 PostOut post_begin(const Graph &g, const Tables &t, const Params &prm, const Trial &tr,
-                   std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz, SynDiff (&diff)[2]);
+                   std::vector<uint8_t> &est, const uint8_t *sx, const uint8_t *sz, SynDiff (&diff)[2],
+                   const uint32_t *const *dbits = nullptr);
 void post_osd(const Graph &g, PostOut &o, std::vector<uint8_t> &est, SynDiff (&diff)[2],
               const LlrSide &llr_x, const LlrSide &llr_z);
 

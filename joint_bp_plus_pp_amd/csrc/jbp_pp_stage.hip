@@ -50,6 +50,27 @@ struct StepSlot {
     uint8_t *prev;
 };
 
+// Per-block output of a step launch, in bytes: the estimate and the previous
+// estimate (one byte per qubit each), then the launch's last syndrome
+// differences, syndrome(est) XOR the trial's syndrome, as bit words (X
+// checks, then Z checks; bit c of word c / 32).
+__host__ __device__ inline size_t step_out_bytes(const DevGraph &g) {
+    return ((2 * static_cast<size_t>(g.n) + 3) & ~static_cast<size_t>(3)) + 4 * static_cast<size_t>(g.wm[0] + g.wm[1]);
+}
+__host__ __device__ inline size_t step_out_diff(const DevGraph &g) {
+    return (2 * static_cast<size_t>(g.n) + 3) & ~static_cast<size_t>(3);
+}
+
+// Bits of one wavefront's 64 consecutive checks c - lane .. into words
+// (lane 0 is active whenever any lane is: c grows with the lane).
+__device__ __forceinline__ void put_diff_bits(uint32_t *w, int c, int bad, int wm) {
+    const uint64_t bits = __ballot(bad);
+    if ((threadIdx.x & 63) == 0) {
+        w[c >> 5] = static_cast<uint32_t>(bits);
+        if ((c >> 5) + 1 < wm) w[(c >> 5) + 1] = static_cast<uint32_t>(bits >> 32);
+    }
+}
+
 __host__ __device__ inline size_t step_slot_bytes(int E0, int E1, int n) {
     size_t b = (6 * static_cast<size_t>(E0 + E1) + 4 * static_cast<size_t>(n)) * sizeof(double) +
                2 * static_cast<size_t>(n);
@@ -115,6 +136,8 @@ __global__ __launch_bounds__(256, (MAXC <= 12 && PAIR && !QC) ? 4 : 2) void pp_s
         for (int v = tid; v < n; v += bs) sp.est[v] = sp.prev[v] = 0;
         __syncthreads();
     }
+    uint8_t *o = out + static_cast<size_t>(b) * step_out_bytes(g);
+    uint32_t *dw = reinterpret_cast<uint32_t *>(o + step_out_diff(g));
     for (int j = 0; j < steps; ++j) {
         if (tid == 0) s_cnt[0] = s_cnt[1] = 0;
         // one loop per side: a single loop branching between the sides
@@ -154,10 +177,16 @@ __global__ __launch_bounds__(256, (MAXC <= 12 && PAIR && !QC) ? 4 : 2) void pp_s
         }
         __syncthreads();
         int ux = 0, uz = 0;
-        for (int c = tid; c < mx; c += bs)
-            ux += check_parity<MAXC>(g.side[0], 0, c, sp.est) != static_cast<int>((tsx[c >> 5] >> (c & 31)) & 1u);
-        for (int c = tid; c < mz; c += bs)
-            uz += check_parity<MAXC>(g.side[1], 1, c, sp.est) != static_cast<int>((tsz[c >> 5] >> (c & 31)) & 1u);
+        for (int c = tid; c < mx; c += bs) {
+            const int bad = check_parity<MAXC>(g.side[0], 0, c, sp.est) != static_cast<int>((tsx[c >> 5] >> (c & 31)) & 1u);
+            put_diff_bits(dw, c, bad, g.wm[0]);
+            ux += bad;
+        }
+        for (int c = tid; c < mz; c += bs) {
+            const int bad = check_parity<MAXC>(g.side[1], 1, c, sp.est) != static_cast<int>((tsz[c >> 5] >> (c & 31)) & 1u);
+            put_diff_bits(dw + g.wm[0], c, bad, g.wm[1]);
+            uz += bad;
+        }
         if (ux) atomicAdd(&s_cnt[0], ux);
         if (uz) atomicAdd(&s_cnt[1], uz);
         const int any1 = __syncthreads_or(changed1);
@@ -173,7 +202,6 @@ __global__ __launch_bounds__(256, (MAXC <= 12 && PAIR && !QC) ? 4 : 2) void pp_s
         __syncthreads();
         if (usx + usz == 0) break;
     }
-    uint8_t *o = out + static_cast<size_t>(b) * 2 * n;
     for (int v = tid; v < n; v += bs) {
         o[v] = sp.est[v];
         o[n + v] = sp.prev[v];
@@ -367,7 +395,7 @@ int ensure_pp_buffers(jbp_ctx *ctx, PPBuf &pb, int slots, int rec_per_slot, char
     HIPCHK(hipMalloc(&pb.list, sizeof(int) * slots));
     HIPCHK(hipMalloc(&pb.slot_trial, sizeof(int) * slots));
     HIPCHK(hipMalloc(&pb.rec, sizeof(int) * kRecInts * static_cast<size_t>(slots) * pb.rec_cap));
-    HIPCHK(hipMalloc(&pb.out, 2 * static_cast<size_t>(g.n) * slots));
+    HIPCHK(hipMalloc(&pb.out, step_out_bytes(g) * slots));
     HIPCHK(hipMalloc(&pb.idx, sizeof(int) * slots));
     // staging: syndromes (as 8-byte units), marginals of every slot, or the
     // write-back planes + iterations + flags, whichever is largest
@@ -378,7 +406,7 @@ int ensure_pp_buffers(jbp_ctx *ctx, PPBuf &pb, int slots, int rec_per_slot, char
     HIPCHK(hipMalloc(&pb.stage, pb.stage_bytes));
     HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&pb.h_rec),
                          sizeof(int) * kRecInts * static_cast<size_t>(slots) * pb.rec_cap));
-    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&pb.h_out), 2 * static_cast<size_t>(g.n) * slots));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&pb.h_out), step_out_bytes(g) * slots));
     HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&pb.h_tot), tot));
     HIPCHK(hipHostMalloc(&pb.h_stage, std::max(syn, back)));
     HIPCHK(hipHostGetDevicePointer(&pb.d_hstage, pb.h_stage, 0));
@@ -428,6 +456,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
     const pp::Graph &hg = ctx->pp_graph;
     const pp::Tables &tabs = ctx->pp_tables;
     const int n = g.n, mx = g.side[0].m, mz = g.side[1].m;
+    const size_t ostride = step_out_bytes(g), odiff = step_out_diff(g);  // step output per block
     const int H = std::min(99, max_iter - 1);  // iterations before any trigger can fire
     pp::Params prm;
     prm.max_iter = max_iter;
@@ -549,7 +578,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
         if (B.phase_a) {
             HIPCHK(hipMemcpyAsync(pb.h_rec, pb.rec, sizeof(int) * static_cast<size_t>(K) * HA * kRecInts,
                                   hipMemcpyDeviceToHost, st));
-            HIPCHK(hipMemcpyAsync(pb.h_out, pb.out, static_cast<size_t>(K) * 2 * n, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(pb.h_out, pb.out, static_cast<size_t>(K) * ostride, hipMemcpyDeviceToHost, st));
             // (the gather reuses the staging buffer after the syndromes' copy)
             if ((rc = queue_marginals(pb, st, d_slots, K, B.per_all))) return rc;
         }
@@ -664,7 +693,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                     ir.same = r[2] & 1;
                     ir.osc = (r[2] >> 1) & 1;
                     if (pp::observe(hg, prm, tr[i], t, ir, nullptr, nullptr) == pp::Step::kConverged) {
-                        std::copy(&out[static_cast<size_t>(i) * 2 * n], &out[static_cast<size_t>(i) * 2 * n] + n,
+                        std::copy(&out[static_cast<size_t>(i) * ostride], &out[static_cast<size_t>(i) * ostride] + n,
                                   est[i].begin());
                         finish(i);
                         break;
@@ -688,7 +717,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                 HIPCHK(hipGetLastError());
                 HIPCHK(hipMemcpyAsync(rec, pb.rec, sizeof(int) * static_cast<size_t>(A) * kRecInts,
                                       hipMemcpyDeviceToHost, st));
-                HIPCHK(hipMemcpyAsync(out, pb.out, static_cast<size_t>(A) * 2 * n, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipMemcpyAsync(out, pb.out, static_cast<size_t>(A) * ostride, hipMemcpyDeviceToHost, st));
                 HIPCHK(hipStreamSynchronize(st));
                 // In the last iteration every active trial's marginals are
                 // gathered now, before the next batch takes the device (most
@@ -708,7 +737,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                              : &rec[static_cast<size_t>(a) * kRecInts];
             };
             auto out_of = [&](int a) -> const uint8_t * {
-                return &out[static_cast<size_t>(first ? active[a] : a) * 2 * n];
+                return &out[static_cast<size_t>(first ? active[a] : a) * ostride];
             };
             pool.run(A, [&](int a) {
                 const int i = active[a];
@@ -730,9 +759,12 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                     return;
                 }
                 const uint8_t *tsx = sx + static_cast<size_t>(i) * mx, *tsz = sz + static_cast<size_t>(i) * mz;
+                // the device's syndrome differences of this estimate
+                const uint32_t *dbits[2] = {reinterpret_cast<const uint32_t *>(e_now + odiff),
+                                            reinterpret_cast<const uint32_t *>(e_now + odiff) + g.wm[0]};
                 if (step == pp::Step::kAttempt) {
                     attempted[i] = 1;
-                    pp::attempt_begin(hg, tabs, prm, tr[i], est[i], tsx, tsz, att[i]);
+                    pp::attempt_begin(hg, tabs, prm, tr[i], est[i], tsx, tsz, att[i], dbits);
                     if (pp::attempt_end(tr[i], att[i], t, est[i])) {
                         finish(i);
                         return;
@@ -746,7 +778,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                         // ETS and flip sets now, |LLR| only if OSD follows
                         tr[i].iterations = max_iter;
                         o[i].post_run = true;
-                        o[i].post = pp::post_begin(hg, tabs, prm, tr[i], est[i], tsx, tsz, att[i].diff);
+                        o[i].post = pp::post_begin(hg, tabs, prm, tr[i], est[i], tsx, tsz, att[i].diff, dbits);
                         need_llr[i] = o[i].post.want_osd;
                         if (!need_llr[i]) {
                             tr[i].done = true;
@@ -983,7 +1015,7 @@ extern "C" int jbp_decode_llr(jbp_ctx *ctx, int count, const uint32_t *sx_bits, 
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(pb.h_rec, pb.rec, sizeof(int) * kRecInts * static_cast<size_t>(K) * max_iter,
                               hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(pb.h_out, pb.out, static_cast<size_t>(K) * 2 * n, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(pb.h_out, pb.out, static_cast<size_t>(K) * step_out_bytes(g), hipMemcpyDeviceToHost, st));
         launch_gather(pb.state, pb.stride / 8, 6 * static_cast<size_t>(g.side[0].cap + g.side[1].cap),
                       4 * n, pb.list, K, pb.stage, st);
         HIPCHK(hipGetLastError());
@@ -1004,7 +1036,7 @@ extern "C" int jbp_decode_llr(jbp_ctx *ctx, int count, const uint32_t *sx_bits, 
             if (iterations_out) iterations_out[k] = iters;
             if (syn_match_out) syn_match_out[k] = static_cast<uint8_t>(match);
             if (est_out)
-                for (int v = 0; v < n; ++v) est_out[k * n + v] = pb.h_out[static_cast<size_t>(i) * 2 * n + v];
+                for (int v = 0; v < n; ++v) est_out[k * n + v] = pb.h_out[static_cast<size_t>(i) * step_out_bytes(g) + v];
             if (llr_out)
                 pp::abs_llr_from_marginals(n, pb.h_tot + static_cast<size_t>(i) * 4 * n, llr_out + k * 2 * n,
                                            llr_out + k * 2 * n + n);
