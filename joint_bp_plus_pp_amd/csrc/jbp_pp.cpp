@@ -19,6 +19,7 @@
 #include <cctype>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 
 namespace jbp {
@@ -679,10 +680,25 @@ Step observe(const Graph &g, const Params &prm, Trial &tr, int t, const IterRec 
         }
     }
     if (prev && est) {
+        // qubits whose X part (bit 0) or Z part (bit 1) changed, ascending;
+        // eight qubits per word compare, most words unchanged
+        tr.last[0].clear();
+        tr.last[1].clear();
+        auto changed = [&](int v, unsigned d) {
+            if (d & 1u) tr.last[0].push_back(v);
+            if (d & 2u) tr.last[1].push_back(v);
+        };
+        int v = 0;
+        for (; v + 8 <= g.n; v += 8) {
+            uint64_t a, b;
+            std::memcpy(&a, prev + v, 8);
+            std::memcpy(&b, est + v, 8);
+            const uint64_t x = (a ^ b) & 0x0303030303030303ull;
+            if (x == 0) continue;
+            for (int k = 0; k < 8; ++k) changed(v + k, static_cast<unsigned>(x >> (8 * k)) & 3u);
+        }
+        for (; v < g.n; ++v) changed(v, static_cast<unsigned>(prev[v] ^ est[v]) & 3u);
         for (int s = 0; s < 2; ++s) {
-            tr.last[s].clear();
-            for (int v = 0; v < g.n; ++v)
-                if (part(s, prev[v]) != part(s, est[v])) tr.last[s].push_back(v);
             if (tr.hist_on) {
                 tr.win[s].push_back(tr.last[s]);
                 if (static_cast<int>(tr.win[s].size()) > prm.hist_window) tr.win[s].pop_front();

@@ -27,6 +27,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstring>
 #include <functional>
 #include <mutex>
 #include <thread>
@@ -850,9 +851,19 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
         pool.run(K, [&](int i) {
             uint32_t *px = planes + static_cast<size_t>(i) * 2 * g.wn, *pz = px + g.wn;
             std::fill(px, px + 2 * g.wn, 0u);
-            for (int v = 0; v < n; ++v) {
-                px[v >> 5] |= static_cast<uint32_t>(est[i][v] & 1) << (v & 31);
-                pz[v >> 5] |= static_cast<uint32_t>((est[i][v] >> 1) & 1) << (v & 31);
+            const uint8_t *e = est[i].data();
+            int v = 0;
+            for (; v + 8 <= n; v += 8) {
+                // bit 0 (X) and bit 1 (Z) of eight states, gathered into a byte each
+                constexpr uint64_t kLow = 0x0101010101010101ull, kGather = 0x0102040810204080ull;
+                uint64_t x;
+                std::memcpy(&x, e + v, 8);
+                px[v >> 5] |= static_cast<uint32_t>(((x & kLow) * kGather) >> 56) << (v & 31);
+                pz[v >> 5] |= static_cast<uint32_t>((((x >> 1) & kLow) * kGather) >> 56) << (v & 31);
+            }
+            for (; v < n; ++v) {
+                px[v >> 5] |= static_cast<uint32_t>(e[v] & 1) << (v & 31);
+                pz[v >> 5] |= static_cast<uint32_t>((e[v] >> 1) & 1) << (v & 31);
             }
             it[i] = o[i].iterations;
             okb[i] = o[i].ok ? 1 : 0;
