@@ -159,6 +159,18 @@ def test_tune_picks_a_candidate_and_keeps_results(small_hgp):
     assert per_trial_equal(a, b)
 
 
+def test_fp64_workgroup_sizes_keep_results(main_decoder):
+    """On the included (quasi-cyclic) code the FP64 decoder stages its
+    variable pass through LDS, each thread copying its own rows in and out;
+    every workgroup size it accepts decodes every trial the same."""
+    dec = main_decoder
+    _, a = dec.simulate(41, 96, 0.05, max_iter=30, precision=jbp.FP64, want_est=True)
+    for bs in (64, 128, 512, 256):  # (the context keeps the last size: 256, its default)
+        print("fp64 block", bs, flush=True)
+        _, b = dec.simulate(41, 96, 0.05, max_iter=30, precision=jbp.FP64, block=bs, want_est=True)
+        assert per_trial_equal(a, b) and np.array_equal(a["est"], b["est"])
+
+
 def test_tune_lds_engine_has_nothing_to_tune(main_decoder):
     assert main_decoder.tune(1, 256, 0.05, max_iter=20, precision=jbp.FP32)[:2] == (0, 0)
 
