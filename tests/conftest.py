@@ -18,6 +18,24 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")
 
 
+_HIP_DEVICE = []
+
+
+def hip_device_available():
+    """Whether a HIP device is visible, probed once per session (through torch,
+    before any test's decoder opens the device: probed afterwards the answer
+    can come back false)."""
+    if not _HIP_DEVICE:
+        import torch  # device probe only
+        _HIP_DEVICE.append(torch.cuda.is_available())
+    return _HIP_DEVICE[0]
+
+
+def pytest_collection_finish(session):
+    if any(item.get_closest_marker("gpu") for item in session.items):
+        hip_device_available()
+
+
 def golden_npz(name):
     with np.load(os.path.join(GOLDEN, name), allow_pickle=False) as z:
         return {k: z[k] for k in z.files}
@@ -56,14 +74,12 @@ def main_code_oracle(oracle):
 
 @pytest.fixture(scope="session")
 def gpu_available():
-    import torch  # device probe only
-    return torch.cuda.is_available()
+    return hip_device_available()
 
 
 @pytest.fixture(scope="session")
 def main_decoder():
-    import torch
-    if not torch.cuda.is_available():
+    if not hip_device_available():
         pytest.skip("no HIP device")
     import joint_bp_plus_pp_amd as jbp
     code = jbp.Code.from_params(PARAMS)

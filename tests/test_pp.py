@@ -7,7 +7,7 @@ import hashlib
 import numpy as np
 import pytest
 
-from conftest import PARAMS, golden_npz
+from conftest import PARAMS, golden_npz, hip_device_available
 
 jbp = pytest.importorskip("joint_bp_plus_pp_amd")
 
@@ -64,8 +64,7 @@ def test_pp_fp64_equals_reference_per_trial(pp_decoder, name):
 def test_pp_fp64_hgp_equals_reference_per_trial(name):
     """Small HGP codes without tables; HGP(21x28) has 19 syndrome words per
     side, an odd count (the stage copies its syndromes trial by trial)."""
-    import torch
-    if not torch.cuda.is_available():
+    if not hip_device_available():
         pytest.skip("no HIP device")
     g = golden_npz("pp_trials_%s.npz" % name)
     m, n, cw, rw, seed = (int(v) for v in g["hgp"])
