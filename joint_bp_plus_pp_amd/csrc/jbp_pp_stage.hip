@@ -439,7 +439,8 @@ struct PPBatch {
     const int *slot_trial = nullptr;  // device: trial index of each slot
     PPBuf *pb = nullptr;
     hipStream_t st = nullptr;
-    int per_all = 1;        // slots per chunk of the first phase's marginal copies
+    int per_all = 1;        // rows per chunk of the first phase's marginal copies
+    std::vector<int> live, live_row;  // slots running after iteration H; slot -> marginal row or -1
     bool syn_wait = false;  // syndromes queued for the host, not yet unpacked
     bool phase_a = false;   // phase A launched, results not yet read
     std::vector<uint8_t> sx, sz;
@@ -528,8 +529,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
 
     // Queues a batch without waiting: the syndromes' gather, the first
     // device phase (iterations 0..H), the copies of the syndromes, records
-    // and estimates, the gather of every slot's iteration-H marginals (ready
-    // event), and their copies.  No copy goes ahead of the launches in the
+    // and estimates (ready event).  No copy goes ahead of the launches in the
     // stream (a copy waits behind the previous batch's marginals on the copy
     // engine, and the launches would wait with it).
     auto start = [&](PPBatch &B, size_t bi) -> int {
@@ -580,11 +580,8 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             HIPCHK(hipMemcpyAsync(pb.h_rec, pb.rec, sizeof(int) * static_cast<size_t>(K) * HA * kRecInts,
                                   hipMemcpyDeviceToHost, st));
             HIPCHK(hipMemcpyAsync(pb.h_out, pb.out, static_cast<size_t>(K) * ostride, hipMemcpyDeviceToHost, st));
-            // (the gather reuses the staging buffer after the syndromes' copy)
-            if ((rc = queue_marginals(pb, st, d_slots, K, B.per_all))) return rc;
         }
         HIPCHK(hipEventRecord(pb.ready_ev, st));
-        if (B.phase_a && (rc = queue_marginal_copies(pb, st, K, B.per_all))) return rc;
         return JBP_OK;
     };
 
@@ -659,6 +656,31 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
         int rc = JBP_OK;
         // the first phase, its copies and the marginals' gather (queued by start)
         HIPCHK(hipEventSynchronize(pb.ready_ev));
+        if (B.phase_a) {
+            // Iteration H's marginals of the slots still running after it
+            // (the converged ones need none), gathered and their copies
+            // queued before the next batch takes the device and the copy
+            // engine.
+            B.live.clear();
+            B.live_row.assign(K, -1);
+            for (int i = 0; i < K; ++i) {
+                bool conv = false;
+                for (int t = 0; t <= H && !conv; ++t) {
+                    const int *r = &pb.h_rec[(static_cast<size_t>(i) * HA + t) * kRecInts];
+                    conv = r[0] == 0 && r[1] == 0;
+                }
+                if (!conv) {
+                    B.live_row[i] = static_cast<int>(B.live.size());
+                    B.live.push_back(i);
+                }
+            }
+            const int L = static_cast<int>(B.live.size());
+            if (L > 0) {
+                HIPCHK(hipMemcpyAsync(pb.idx, B.live.data(), sizeof(int) * L, hipMemcpyHostToDevice, st));
+                if ((rc = queue_marginals(pb, st, pb.idx, L, B.per_all))) return rc;
+                if ((rc = queue_marginal_copies(pb, st, L, B.per_all))) return rc;
+            }
+        }
         if (HA >= max_iter && (rc = kick_once())) return rc;
         if (B.syn_wait) {
             const uint32_t *w = static_cast<const uint32_t *>(pb.h_stage);
@@ -805,7 +827,11 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             pool.run(W, [&](int w) {
                 const int i = want[w];
                 // row of the trial's marginals
-                const int r = first ? i : last ? pos[i] : w;
+                const int r = first ? B.live_row[i] : last ? pos[i] : w;
+                if (r < 0) {  // (a trial that needs them is running: never)
+                    ev_err.store(2);
+                    return;
+                }
                 if (hipEventSynchronize(pb.tot_ev[r / per]) != hipSuccess) {
                     ev_err.store(1);
                     return;
@@ -831,7 +857,8 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                 }
             });
             if (ev_err.load()) {
-                set_err(err, errlen, "waiting for the marginals' copy to the host failed");
+                set_err(err, errlen, ev_err.load() == 2 ? "marginals missing for a trial that needs them"
+                                                        : "waiting for the marginals' copy to the host failed");
                 return JBP_ERR_DEVICE;
             }
             active.erase(std::remove_if(active.begin(), active.end(), [&](int i) { return tr[i].done; }),
