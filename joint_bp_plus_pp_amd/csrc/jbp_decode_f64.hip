@@ -90,12 +90,12 @@ __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPla
 
         int iterations = max_iter;
         int matched = 0;
+        int res = -1;  // QC pair path: the column block resident in LDS
         for (int iter = 0; iter < max_iter; ++iter) {
             // one loop per side: a single loop branching between the sides
             // keeps both inlined recursions' registers live
             if (QC && PAIR) {
-                check_pass_pipelined<MAXC>(g.side[0], 0, tid, bs, tsx, sp.v2c[0], sp.c2v[0]);
-                check_pass_pipelined<MAXC>(g.side[1], 1, tid, bs, tsz, sp.v2c[1], sp.c2v[1]);
+                check_passes_qc<MAXC>(g, res, tsx, tsz, sp.v2c, sp.c2v, s_msg);
             } else {
 #pragma unroll 1
                 for (int c = tid; c < mx; c += bs)
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPla
             }
             __syncthreads();
             if (QC && PAIR) {
-                var_pass_qc_lds<MAXV>(g, prior, sp.v2c, sp.c2v, s_qc, s_msg,
+                var_pass_qc_lds<MAXV>(g, prior, sp.v2c, sp.c2v, s_qc, s_msg, res,
                                       [&](int v, int best, const M4 &) { sp.est[v] = static_cast<uint8_t>(best); });
             } else {
                 for (int v = tid; v < g.n; v += bs)

@@ -126,22 +126,24 @@ __device__ void check_update(const DevSide &sd, int side, int c, int syn, const 
 // The check update split in two for a software-pipelined pass over a
 // thread's checks (quasi-cyclic codes, pair sums): check_load fetches a
 // check's twelve incoming pairs, check_finish computes and stores its
-// outgoing messages exactly as check_update does.
-template <int MAXC>
-__device__ __forceinline__ void check_load(const DevSide &sd, int c, const double *v2c, double (&q0)[MAXC],
-                                           double (&q1)[MAXC]) {
+// outgoing messages exactly as check_update does.  RES >= 0: position RES's
+// messages are resident in LDS (res[c], the column block the variable pass
+// left there) and are read and written there instead of in HBM.
+template <int MAXC, int RES>
+__device__ __forceinline__ void check_load(const DevSide &sd, int c, const double *v2c, const double2 *res,
+                                           double (&q0)[MAXC], double (&q1)[MAXC]) {
     const size_t m = static_cast<size_t>(sd.m);
 #pragma unroll
     for (int i = 0; i < MAXC; ++i) {
-        const double2 mv = *reinterpret_cast<const double2 *>(v2c + 2 * i * m + 2 * c);
+        const double2 mv = i == RES ? res[c] : *reinterpret_cast<const double2 *>(v2c + 2 * i * m + 2 * c);
         q0[i] = mv.x;
         q1[i] = mv.y;
     }
 }
 
-template <int MAXC>
+template <int MAXC, int RES>
 __device__ __forceinline__ void check_finish(const DevSide &sd, int side, int c, int syn, const double (&q0)[MAXC],
-                                             const double (&q1)[MAXC], double *c2v) {
+                                             const double (&q1)[MAXC], double *c2v, double2 *res) {
     const size_t m = static_cast<size_t>(sd.m);
     double pe[MAXC + 1], po[MAXC + 1];
     pe[0] = 1.0;
@@ -167,7 +169,8 @@ __device__ __forceinline__ void check_finish(const DevSide &sd, int side, int c,
             o.x = v0 / sum;
             o.y = v1 / sum;
         }
-        *reinterpret_cast<double2 *>(c2v + 2 * i * m + 2 * c) = o;
+        if (i == RES) res[c] = o;
+        else *reinterpret_cast<double2 *>(c2v + 2 * i * m + 2 * c) = o;
         double ne = se * q0[i] + so * q1[i];
         double no = se * q1[i] + so * q0[i];
         se = ne;
@@ -178,24 +181,42 @@ __device__ __forceinline__ void check_finish(const DevSide &sd, int side, int c,
 // A thread's checks c0, c0 + step, ... of one side: the next check's loads
 // are issued before the current one is computed (two register sets, no copy
 // between them, so the loads stay in flight during the arithmetic).
-template <int MAXC>
+template <int MAXC, int RES>
 __device__ void check_pass_pipelined(const DevSide &sd, int side, int c0, int step, const uint32_t *ts,
-                                     const double *v2c, double *c2v) {
+                                     const double *v2c, double *c2v, double2 *res) {
     const int m = sd.m;
     if (c0 >= m) return;
     double a0[MAXC], a1[MAXC], b0[MAXC], b1[MAXC];
     int c = c0;
-    check_load<MAXC>(sd, c, v2c, a0, a1);
+    check_load<MAXC, RES>(sd, c, v2c, res, a0, a1);
     for (;;) {
         const int c1 = c + step;
-        if (c1 < m) check_load<MAXC>(sd, c1, v2c, b0, b1);
-        check_finish<MAXC>(sd, side, c, (ts[c >> 5] >> (c & 31)) & 1, a0, a1, c2v);
+        if (c1 < m) check_load<MAXC, RES>(sd, c1, v2c, res, b0, b1);
+        check_finish<MAXC, RES>(sd, side, c, (ts[c >> 5] >> (c & 31)) & 1, a0, a1, c2v, res);
         if (c1 >= m) break;
         const int c2 = c1 + step;
-        if (c2 < m) check_load<MAXC>(sd, c2, v2c, a0, a1);
-        check_finish<MAXC>(sd, side, c1, (ts[c1 >> 5] >> (c1 & 31)) & 1, b0, b1, c2v);
+        if (c2 < m) check_load<MAXC, RES>(sd, c2, v2c, res, a0, a1);
+        check_finish<MAXC, RES>(sd, side, c1, (ts[c1 >> 5] >> (c1 & 31)) & 1, b0, b1, c2v, res);
         if (c2 >= m) break;
         c = c2;
+    }
+}
+
+// Both sides' check passes, with the column block `res` (-1: none, 0 or
+// MAXC - 1) resident in LDS (lds: 2*m pairs, side 0 then side 1).
+template <int MAXC>
+__device__ void check_passes_qc(const DevGraph &g, int res, const uint32_t *tsx, const uint32_t *tsz,
+                                double *const *v2c, double *const *c2v, double2 *lds) {
+    const int tid = threadIdx.x, bs = blockDim.x, m = g.side[0].m;
+    if (res < 0) {
+        check_pass_pipelined<MAXC, -1>(g.side[0], 0, tid, bs, tsx, v2c[0], c2v[0], lds);
+        check_pass_pipelined<MAXC, -1>(g.side[1], 1, tid, bs, tsz, v2c[1], c2v[1], lds + m);
+    } else if (res == 0) {
+        check_pass_pipelined<MAXC, 0>(g.side[0], 0, tid, bs, tsx, v2c[0], c2v[0], lds);
+        check_pass_pipelined<MAXC, 0>(g.side[1], 1, tid, bs, tsz, v2c[1], c2v[1], lds + m);
+    } else {
+        check_pass_pipelined<MAXC, MAXC - 1>(g.side[0], 0, tid, bs, tsx, v2c[0], c2v[0], lds);
+        check_pass_pipelined<MAXC, MAXC - 1>(g.side[1], 1, tid, bs, tsz, v2c[1], c2v[1], lds + m);
     }
 }
 
@@ -402,6 +423,21 @@ __device__ __forceinline__ int var_core_pair(const M4 &prior, const double (&ca)
     return best;
 }
 
+// Copies column block cc's variable-to-check rows from LDS to v2c, each
+// thread its own rows (tid + j * blockDim.x); cc < 0: nothing.
+__device__ __forceinline__ void var_pass_flush(const DevGraph &g, int cc, double *const *v2c, const double2 *lds) {
+    if (cc < 0) return;
+    const int m = g.side[0].m, rows = 2 * m;
+    const size_t base = static_cast<size_t>(cc) * m;
+    double2 *out0 = reinterpret_cast<double2 *>(v2c[0]) + base;
+    double2 *out1 = reinterpret_cast<double2 *>(v2c[1]) + base;
+    for (int k = threadIdx.x; k < rows; k += blockDim.x) {
+        const double2 mv = lds[k];
+        if (k < m) out0[k] = mv;
+        else out1[k - m] = mv;
+    }
+}
+
 // Bytes of LDS var_pass_qc_lds needs: both sides' rows of one column block.
 inline size_t var_pass_lds_bytes(int m) { return 2 * static_cast<size_t>(m) * sizeof(double2); }
 
@@ -417,9 +453,16 @@ inline size_t var_pass_lds_bytes(int m) { return 2 * static_cast<size_t>(m) * si
 // variable's hard decision and normalised marginal.  The caller synchronises
 // the workgroup before (c2v complete) and after (v2c complete).  Workgroup
 // sizes are multiples of 64.
+//
+// res: on entry the column block already in LDS (its check messages, as the
+// check pass left them) or -1; on exit the block left there (its
+// variable-to-check messages, not copied out: the next check pass uses them
+// in LDS, or var_pass_flush writes them back).  The blocks are walked from
+// the resident end (ascending from 0, descending from L - 1), so each pass
+// starts with the block the previous one ended on.
 template <int MAXV, class Visit>
 __device__ void var_pass_qc_lds(const DevGraph &g, const M4 &prior, double *const *v2c, const double *const *c2v,
-                                const int *qc, double2 *lds, Visit visit) {
+                                const int *qc, double2 *lds, int &res, Visit visit) {
     const int tid = threadIdx.x, bs = blockDim.x;
     const int P = g.qc_P, L = g.qc_L, JL = MAXV * L, m = g.side[0].m, rows = 2 * m;
     const float inv_p = 1.0f / static_cast<float>(P);
@@ -434,8 +477,11 @@ __device__ void var_pass_qc_lds(const DevGraph &g, const M4 &prior, double *cons
                                              (__attribute__((address_space(3))) void *)(lds + (k & ~63)), 16, 0, 0);
         }
     };
-    fetch(0);
-    for (int cc = 0; cc < L; ++cc) {
+    const bool down = L > 1 && res == L - 1;
+    const int first = down ? L - 1 : 0, dir = down ? -1 : 1;
+    if (res != first) fetch(first);
+    for (int k = 0; k < L; ++k) {
+        const int cc = first + dir * k;
         __builtin_amdgcn_s_waitcnt(0);  // this thread's LDS loads have landed
         __syncthreads();
         for (int y = tid; y < P; y += bs) {
@@ -469,21 +515,14 @@ __device__ void var_pass_qc_lds(const DevGraph &g, const M4 &prior, double *cons
             visit(cc * P + y, best, total);
         }
         __syncthreads();
+        if (k + 1 == L) break;  // the last block stays resident
         // copy out (each thread its own LDS rows), then the next block's rows
         // in over the same LDS rows once the reads have returned
-        const size_t base = static_cast<size_t>(cc) * m;
-        double2 *out0 = reinterpret_cast<double2 *>(v2c[0]) + base;
-        double2 *out1 = reinterpret_cast<double2 *>(v2c[1]) + base;
-        for (int k = tid; k < rows; k += bs) {
-            const double2 mv = lds[k];
-            if (k < m) out0[k] = mv;
-            else out1[k - m] = mv;
-        }
-        if (cc + 1 < L) {
-            __builtin_amdgcn_s_waitcnt(0);
-            fetch(cc + 1);
-        }
+        var_pass_flush(g, cc, v2c, lds);
+        __builtin_amdgcn_s_waitcnt(0);
+        fetch(cc + dir);
     }
+    res = first + dir * (L - 1);
 }
 
 // Parity of check c of one side over est (X checks read the X bit, Z checks

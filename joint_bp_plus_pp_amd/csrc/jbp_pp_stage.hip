@@ -139,13 +139,13 @@ __global__ __launch_bounds__(256, (MAXC <= 12 && PAIR && !QC) ? 4 : 2) void pp_s
     }
     uint8_t *o = out + static_cast<size_t>(b) * step_out_bytes(g);
     uint32_t *dw = reinterpret_cast<uint32_t *>(o + step_out_diff(g));
+    int res = -1;  // QC pair path: the column block resident in LDS
     for (int j = 0; j < steps; ++j) {
         if (tid == 0) s_cnt[0] = s_cnt[1] = 0;
         // one loop per side: a single loop branching between the sides
         // keeps both inlined recursions' registers live
         if (QC && PAIR) {
-            check_pass_pipelined<MAXC>(g.side[0], 0, tid, bs, tsx, sp.v2c[0], sp.c2v[0]);
-            check_pass_pipelined<MAXC>(g.side[1], 1, tid, bs, tsz, sp.v2c[1], sp.c2v[1]);
+            check_passes_qc<MAXC>(g, res, tsx, tsz, sp.v2c, sp.c2v, s_msg);
         } else {
 #pragma unroll 1
             for (int c = tid; c < mx; c += bs)
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256, (MAXC <= 12 && PAIR && !QC) ? 4 : 2) void pp_s
             sp.est[v] = static_cast<uint8_t>(best);
         };
         if (QC && PAIR) {
-            var_pass_qc_lds<MAXV>(g, prior, sp.v2c, sp.c2v, s_qc, s_msg, [&](int v, int best, const M4 &total) {
+            var_pass_qc_lds<MAXV>(g, prior, sp.v2c, sp.c2v, s_qc, s_msg, res, [&](int v, int best, const M4 &total) {
                 if (tot) {
                     double *t = tot + 4 * static_cast<size_t>(v);
                     t[0] = total.v[0]; t[1] = total.v[1]; t[2] = total.v[2]; t[3] = total.v[3];
@@ -203,6 +203,8 @@ __global__ __launch_bounds__(256, (MAXC <= 12 && PAIR && !QC) ? 4 : 2) void pp_s
         __syncthreads();
         if (usx + usz == 0) break;
     }
+    // the slot's messages complete in HBM for the next launch
+    if (QC && PAIR) var_pass_flush(g, res, sp.v2c, s_msg);
     for (int v = tid; v < n; v += bs) {
         o[v] = sp.est[v];
         o[n + v] = sp.prev[v];
