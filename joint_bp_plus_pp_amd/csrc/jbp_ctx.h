@@ -62,7 +62,7 @@ struct PPBuf {
     void *h_stage = nullptr;
     void *d_hstage = nullptr;  // h_stage's device address (kernels read it directly)
     // marginals reach the host in up to kTotChunks pieces, one event each
-    static constexpr int kTotChunks = 8;
+    static constexpr int kTotChunks = 16;
     hipEvent_t tot_ev[kTotChunks] = {};
     hipEvent_t ready_ev = nullptr;  // a batch's first device phase and its copies done
 };
