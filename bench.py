@@ -53,18 +53,26 @@ def wilson(n, k, z=1.96):
 
 
 def cpu_threads():
+    """Host threads for the CPU baseline: every core of this process's
+    affinity mask, capped at the job's CPU share when the environment states
+    one (OMP_NUM_THREADS; 16 per GPU on the GPU hosts).  Returns (threads,
+    facts about the host)."""
     try:
         avail = len(os.sched_getaffinity(0))
     except AttributeError:
         avail = os.cpu_count() or 1
-    return max(1, min(16, avail))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    threads = avail
+    if share.isdigit() and int(share) > 0:
+        threads = min(threads, int(share))
+    facts = {"host_cpu_count": os.cpu_count(), "affinity_cpus": avail,
+             "cpu_share": int(share) if share.isdigit() else None}
+    return max(1, threads), facts
 
 
-def cpu_baseline(args, first_trial):
-    """Reference decoder (or, without it, the oracle restatement) on a bounded
-    sample of the workload: the first `cpu_trials` trial indices."""
-    threads = cpu_threads()
-    count = args.cpu_trials
+def _run_cpu(count, first_trial, args, threads):
+    """(seconds, iterations, successes, kind) of the reference decoder (or,
+    without it, the oracle restatement) on trials first..first+count-1."""
     harness = os.path.join(ROOT, "oracle", "_ref", "libref_harness.so")
     n = count
     it = (C.c_int * n)()
@@ -93,12 +101,31 @@ def cpu_baseline(args, first_trial):
         secs = L.orc_simulate(h, args.seed, first_trial, count, args.p, args.max_iter, 0.0, threads,
                               it, ok, su, ex, None)
         kind = "port"
-    iters = sum(it)
-    return {"value": round(count / secs, 3), "unit": "trials/s", "cores": threads, "kind": kind,
-            "sample": "%d trials (indices %d..%d) of the same workload, p=%g, max-iter %d, FP64 "
-                      "reference decoder, one trial per thread; %.1f s wall, avg_iter %.2f, "
-                      "failures %d" % (count, first_trial, first_trial + count - 1, args.p,
-                                       args.max_iter, secs, iters / count, count - sum(su))}
+    return secs, sum(it), sum(su), kind
+
+
+def cpu_baseline(args, first_trial):
+    """Reference decoder (or, without it, the oracle restatement) on a bounded
+    sample of the workload: the first `cpu_trials` trial indices on every
+    host thread of the job, and the first `cpu_trials_1core` on one thread
+    (the reference program itself is single-threaded, jointbp_ets.cpp:5931)."""
+    threads, facts = cpu_threads()
+    count = args.cpu_trials
+    secs, iters, succ, kind = _run_cpu(count, first_trial, args, threads)
+    out = {"value": round(count / secs, 3), "unit": "trials/s", "cores": threads, "kind": kind,
+           "sample": "%d trials (indices %d..%d) of the same workload, p=%g, max-iter %d, FP64 "
+                     "reference decoder, one trial per thread on %d threads; %.1f s wall, "
+                     "avg_iter %.2f, failures %d" % (count, first_trial, first_trial + count - 1,
+                                                    args.p, args.max_iter, threads, secs,
+                                                    iters / count, count - succ)}
+    out.update(facts)
+    m = args.cpu_trials_1core
+    if m > 0:
+        s1, it1, _, _ = _run_cpu(m, first_trial, args, 1)
+        out["one_core"] = {"value": round(m / s1, 3), "unit": "trials/s",
+                           "sample": "%d trials (indices %d..%d), one thread, %.1f s, avg_iter %.2f"
+                                     % (m, first_trial, first_trial + m - 1, s1, it1 / m)}
+    return out
 
 
 def pp_workload(jbp, dec, args, first):
@@ -146,7 +173,7 @@ def pp_workload(jbp, dec, args, first):
         m = args.pp_cpu_trials
         bufs = [(C.c_int * m)()] + [(C.c_uint8 * m)() for _ in range(4)]
         ctr = (C.c_longlong * 10)()
-        threads = cpu_threads()
+        threads, _ = cpu_threads()
         secs = L.ref_simulate_pp(args.seed, first, m, p, max_iter, 0.0, 5, threads, *bufs, None, ctr)
         L.ref_ets_clear()
         out["cpu_reference"] = {"trials_per_s": round(m / secs, 2), "cores": threads,
@@ -327,6 +354,8 @@ def main():
     ap.add_argument("--seed", type=int, default=106)
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--cpu-trials", type=int, default=2048)
+    ap.add_argument("--cpu-trials-1core", type=int, default=128,
+                    help="trials of the CPU baseline also timed on one thread (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--learned-steps", type=int, default=40,
                     help="steps of the learned decoder on the first timed trial indices (0: skip)")

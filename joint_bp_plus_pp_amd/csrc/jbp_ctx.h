@@ -61,6 +61,12 @@ struct PPBuf {
     double *h_tot = nullptr;
     void *h_stage = nullptr;
     void *d_hstage = nullptr;  // h_stage's device address (kernels read it directly)
+    // pinned index lists copied to the device (slots whose marginals are
+    // gathered, slots a step advances, trial of each slot): 3 x cap ints.
+    // Copies from pinned memory read their source when they run, so a list
+    // stays here (owned by the buffer set, not by a local vector) until the
+    // stream has passed the copy.
+    int *h_list = nullptr;
     // marginals reach the host in up to kTotChunks pieces, one event each
     static constexpr int kTotChunks = 16;
     hipEvent_t tot_ev[kTotChunks] = {};

@@ -374,7 +374,7 @@ void free_pp_buf(PPBuf &pb) {
                     static_cast<void *>(pb.out), static_cast<void *>(pb.idx), pb.stage})
         if (p) (void)hipFree(p);
     for (void *p : {static_cast<void *>(pb.h_rec), static_cast<void *>(pb.h_out),
-                    static_cast<void *>(pb.h_tot), pb.h_stage})
+                    static_cast<void *>(pb.h_tot), pb.h_stage, static_cast<void *>(pb.h_list)})
         if (p) (void)hipHostFree(p);
     pb = PPBuf();
 }
@@ -413,8 +413,27 @@ int ensure_pp_buffers(jbp_ctx *ctx, PPBuf &pb, int slots, int rec_per_slot, char
     HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&pb.h_tot), tot));
     HIPCHK(hipHostMalloc(&pb.h_stage, std::max(syn, back)));
     HIPCHK(hipHostGetDevicePointer(&pb.d_hstage, pb.h_stage, 0));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&pb.h_list), 3 * sizeof(int) * static_cast<size_t>(slots)));
     for (hipEvent_t &e : pb.tot_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&pb.ready_ev, hipEventDisableTiming));
+    return JBP_OK;
+}
+
+// Copies list[0..count) to dev through region `which` (0..2) of the buffer
+// set's pinned list array.  The caller must not refill that region before
+// the stream has passed this copy (each region is reused only after a
+// synchronisation that follows it).
+int copy_list(PPBuf &pb, int which, const int *list, int count, int *dev, hipStream_t st,
+              char *err, size_t errlen) {
+    if (count <= 0) return JBP_OK;
+    if (count > pb.cap || which < 0 || which > 2 || !pb.h_list) {
+        set_err(err, errlen, "post-processing list of %d entries exceeds the buffer set (%d)", count,
+                pb.cap);
+        return JBP_ERR_ARG;
+    }
+    int *h = pb.h_list + static_cast<size_t>(which) * pb.cap;
+    std::copy(list, list + count, h);
+    HIPCHK(hipMemcpyAsync(dev, h, sizeof(int) * static_cast<size_t>(count), hipMemcpyHostToDevice, st));
     return JBP_OK;
 }
 
@@ -678,7 +697,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             }
             const int L = static_cast<int>(B.live.size());
             if (L > 0) {
-                HIPCHK(hipMemcpyAsync(pb.idx, B.live.data(), sizeof(int) * L, hipMemcpyHostToDevice, st));
+                if ((rc = copy_list(pb, 0, B.live.data(), L, pb.idx, st, err, errlen))) return rc;
                 if ((rc = queue_marginals(pb, st, pb.idx, L, B.per_all))) return rc;
                 if ((rc = queue_marginal_copies(pb, st, L, B.per_all))) return rc;
             }
@@ -736,7 +755,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
             const bool last = t + 1 == max_iter;
             int per = B.per_all;
             if (!first) {
-                HIPCHK(hipMemcpyAsync(pb.list, active.data(), sizeof(int) * A, hipMemcpyHostToDevice, st));
+                if ((rc = copy_list(pb, 1, active.data(), A, pb.list, st, err, errlen))) return rc;
                 launch_step(g, ctx->tp, nt, pb.list, A, B.slot_trial, 1, false, 1, damping, pb.state,
                             pb.stride, pb.rec, pb.out, st);
                 HIPCHK(hipGetLastError());
@@ -821,7 +840,7 @@ int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int h
                 if (!tr[i].done && need_llr[i]) want.push_back(i);
             const int W = static_cast<int>(want.size());
             if (!first && !last && W > 0) {
-                HIPCHK(hipMemcpyAsync(pb.idx, want.data(), sizeof(int) * W, hipMemcpyHostToDevice, st));
+                if ((rc = copy_list(pb, 2, want.data(), W, pb.idx, st, err, errlen))) return rc;
                 if ((rc = queue_marginals(pb, st, pb.idx, W, per))) return rc;
                 if ((rc = queue_marginal_copies(pb, st, W, per))) return rc;
             }
@@ -1047,8 +1066,8 @@ extern "C" int jbp_decode_llr(jbp_ctx *ctx, int count, const uint32_t *sx_bits, 
         for (int i = 0; i < K; ++i) ids[i] = i;
         std::vector<int> trials(K);
         for (int i = 0; i < K; ++i) trials[i] = b0 + i;
-        HIPCHK(hipMemcpyAsync(pb.list, ids.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(pb.slot_trial, trials.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
+        if ((rc = copy_list(pb, 0, ids.data(), K, pb.list, st, err, errlen))) return rc;
+        if ((rc = copy_list(pb, 1, trials.data(), K, pb.slot_trial, st, err, errlen))) return rc;
         // every iteration in one launch, marginals kept (the last one stays)
         launch_step(g, ctx->tp, ctx->noise, pb.list, K, pb.slot_trial, max_iter, true, 1, damping,
                     pb.state, pb.stride, pb.rec, pb.out, st);

@@ -460,9 +460,9 @@ int main(int argc, char **argv) {
         }
     }
 
-    if (enable_pp && freeze_syn) {
-        std::cerr << "--freeze-syn with post-processing runs on the reference's CPU decoder only; "
-                     "add --no-pp.\n";
+    if (freeze_syn) {
+        std::cerr << "--freeze-syn runs on the reference's CPU decoder only (no CPU decoder in "
+                     "this build).\n";
         return 1;
     }
     jbp_ctx *ctx = nullptr;
