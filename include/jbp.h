@@ -102,6 +102,26 @@ int jbp_decode_batch(jbp_ctx *ctx, int count, const uint32_t *sx_bits, const uin
                      int *iterations_out, uint8_t *syn_match_out, int *est_out, char *err,
                      size_t errlen);
 
+/* Allocation-free form of jbp_decode_batch (FP32, LDS-resident decoder;
+ * permutation-block codes only).  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory when they decode (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Inputs and outputs are the caller's DEVICE buffers: d_sx_bits / d_sz_bits
+ * count x ceil(mX/32) / ceil(mZ/32) uint32 words (bit c of word c/32);
+ * d_iterations, d_syn_match count entries; d_est_x / d_est_z the estimate's
+ * X and Z bit planes, count x ceil(nvars/32) words.  Per decode the results
+ * equal jbp_decode_batch's (damping 0).  Work is queued on `stream` (a
+ * hipStream_t; NULL = the null stream) and the call returns without waiting.
+ * It calls no allocator: the work counter, channel constants and decode order
+ * live in a per-device static array (512 B + 8 B per trial, set up by
+ * jbp_create), so count is at most 65,536 (larger: JBP_ERR_ARG).  Calls on
+ * one GPU share that array: calls that overlap on one GPU race and must not
+ * be issued. */
+int jbp_decode_batch_noalloc(jbp_ctx *ctx, int count, const uint32_t *d_sx_bits,
+                             const uint32_t *d_sz_bits, const double prior[4], int max_iter,
+                             int *d_iterations, uint8_t *d_syn_match, uint32_t *d_est_x,
+                             uint32_t *d_est_z, void *stream, char *err, size_t errlen);
+
 /* Monte-Carlo configuration: the inner loop of the reference's --simulate
  * path (jointbp_ets.cpp:5931-6207), post-processing off or on. */
 typedef struct jbp_sim_cfg {

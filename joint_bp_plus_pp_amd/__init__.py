@@ -107,6 +107,7 @@ def lib():
         "jbp_decode": (i32, [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, dbl, i32,
                              C.POINTER(DecodeResult), vp, cp, sz]),
         "jbp_decode_batch": (i32, [vp, i32, vp, vp, vp, i32, dbl, i32, vp, vp, vp, cp, sz]),
+        "jbp_decode_batch_noalloc": (i32, [vp, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, cp, sz]),
         "jbp_simulate": (i32, [vp, C.POINTER(SimCfg), C.POINTER(TrialOut), C.POINTER(RunStats), cp, sz]),
         "jbp_sample": (i32, [vp, u64, i64, i32, dbl, vp, vp, vp, cp, sz]),
         "jbp_in_row_space": (i32, [vp, i32, i32, vp, vp, cp, sz]),

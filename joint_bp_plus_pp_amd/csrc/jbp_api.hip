@@ -1006,6 +1006,10 @@ int jbp_create(const jbp_graph_desc *g, int device_id, jbp_ctx **out, char *err,
         return JBP_ERR_ARG;
     }
     HIPCHK(hipSetDevice(device_id));
+    {
+        int rc0 = noalloc_setup(device_id, err, errlen);
+        if (rc0) return rc0;
+    }
     auto *ctx = new jbp_ctx();
     ctx->device = device_id;
     ctx->n = g->nvars;

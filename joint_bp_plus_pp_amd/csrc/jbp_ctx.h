@@ -213,6 +213,10 @@ void free_pp(jbp_ctx *ctx);
 int run_pp(jbp_ctx *ctx, const NoiseTab &nt, int max_iter, double damping, int hist_window,
            const std::vector<int> &trials, std::vector<PPOutcome> &outs, char *err, size_t errlen);
 
+// Looks up this device's address of the allocation-free decode's scratch
+// array (jbp_noalloc.hip), once per device; the device must be current.
+int noalloc_setup(int device, char *err, size_t errlen);
+
 int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_run_stats *stats,
                   const LearnedInfer *learned, char *err, size_t errlen);
 
