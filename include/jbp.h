@@ -337,8 +337,24 @@ int jbp_info(jbp_ctx *ctx, int64_t out[8]);
 typedef struct jbp_learned jbp_learned;
 
 /* T learned iterations (1..64); parameters start at weights 1 and an MLP
- * whose damping is close to 0. */
+ * whose damping is close to 0.  The kind is JBP_LEARNED_SUM_PRODUCT. */
 int jbp_learned_create(jbp_ctx *ctx, int T, jbp_learned **out, char *err, size_t errlen);
+
+/* Model kinds.  SUM_PRODUCT: the reference's joint BP (jointbp_ets.cpp:
+ * 2373-2519) with the weights and damping above.  MIN_SUM: the same model
+ * with max-product updates (check: sign product times the minimum
+ * magnitude over the other edges, clamped at 64 bits; joint variable term
+ * max(ln a + L, ln b) - max(ln c + L, ln d) for ln((a e^L + b) / (c e^L +
+ * d)); hard decisions from the signs of the X / Z marginals): the weights
+ * act as learned normalisation, and the inference decoder runs it on FP16
+ * pairs with no transcendental function, several times cheaper per
+ * iteration than sum-product BP.  Plain BP stays the fallback. */
+#define JBP_LEARNED_SUM_PRODUCT 0
+#define JBP_LEARNED_MIN_SUM 1
+int jbp_learned_create_kind(jbp_ctx *ctx, int T, int kind, jbp_learned **out, char *err,
+                            size_t errlen);
+/* The model's kind (JBP_LEARNED_*), -1 for NULL. */
+int jbp_learned_kind(const jbp_learned *model);
 /* Destroy a model before the context it was created on. */
 void jbp_learned_destroy(jbp_learned *model);
 int jbp_learned_param_count(const jbp_learned *model);

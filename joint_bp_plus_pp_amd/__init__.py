@@ -9,6 +9,7 @@ device, single decodes, and the batched Monte-Carlo call.
 There is no CPU fallback here: if libjbp.so is missing, importing the
 bindings raises.
 """
+import atexit
 import ctypes as C
 import os
 import re
@@ -28,6 +29,7 @@ FP64 = 64
 STATUS = {0: "ok", 1: "invalid argument", 2: "device error", 3: "unsupported", 4: "out of memory"}
 STATUS_UNSUPPORTED = 3
 ENGINE_AUTO, ENGINE_LDS, ENGINE_HBM = 0, 1, 2
+LEARNED_SUM_PRODUCT, LEARNED_MIN_SUM = 0, 1
 
 
 class JbpError(RuntimeError):
@@ -114,6 +116,8 @@ def lib():
         "jbp_last_kernel_ms": (dbl, [vp, i32, C.POINTER(i64)]),
         "jbp_info": (i32, [vp, vp]),
         "jbp_learned_create": (i32, [vp, i32, C.POINTER(vp), cp, sz]),
+        "jbp_learned_create_kind": (i32, [vp, i32, i32, C.POINTER(vp), cp, sz]),
+        "jbp_learned_kind": (i32, [vp]),
         "jbp_learned_destroy": (None, [vp]),
         "jbp_learned_param_count": (i32, [vp]),
         "jbp_learned_set_params": (i32, [vp, vp, i32, cp, sz]),
@@ -342,6 +346,22 @@ def pp_repair(code, method, side, syndrome, est, tables=None, abs_llr=None, flip
     return out, applied.value
 
 
+# Decoders still open when the interpreter exits are destroyed (with their
+# learned models) before module teardown, while the library handle and this
+# module's names are still usable, instead of leaving device objects to the
+# runtime's own exit.
+_LIVE = weakref.WeakSet()
+
+
+@atexit.register
+def _destroy_live_decoders():
+    for d in list(_LIVE):
+        try:
+            d.destroy()
+        except Exception:
+            pass
+
+
 class Decoder:
     """A decoder context on one device: cuda_bp_create / cuda_bp_decode /
     cuda_bp_destroy (jointbp_cuda.h:48-63) plus the batched calls."""
@@ -353,6 +373,7 @@ class Decoder:
         _check(lib().jbp_create(C.byref(self._graph), device, C.byref(h), e, 512), e)
         self._h = h
         self._models = weakref.WeakSet()  # learned models bound to this context
+        _LIVE.add(self)
         # hypergraph-product codes carry their logical operators: the
         # success test uses them (the dense row basis is too large to build)
         lx, lz = code.logicals(0), code.logicals(1)
@@ -567,15 +588,18 @@ class Decoder:
 
 class LearnedModel:
     """Learned-BP parameters on a decoder's device (include/jbp.h,
-    jbp_learned_*).  Layout: wc[2][nb][T] | wv[2][nb][T] | mlp[65]."""
+    jbp_learned_*).  Layout: wc[2][nb][T] | wv[2][nb][T] | mlp[65].  kind:
+    LEARNED_SUM_PRODUCT (the reference's joint BP with weights) or
+    LEARNED_MIN_SUM (normalised min-sum, the fast inference decoder)."""
 
     MLP_PARAMS = 65
 
-    def __init__(self, decoder, T=25):
+    def __init__(self, decoder, T=25, kind=0):
         self.decoder = decoder
         self.T = T
+        self.kind = int(kind)
         h, e = C.c_void_p(), _errbuf()
-        _check(lib().jbp_learned_create(decoder._h, T, C.byref(h), e, 512), e)
+        _check(lib().jbp_learned_create_kind(decoder._h, T, self.kind, C.byref(h), e, 512), e)
         self._h = h
         decoder._models.add(self)
         self.count = lib().jbp_learned_param_count(h)

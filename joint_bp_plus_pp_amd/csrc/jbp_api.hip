@@ -199,6 +199,7 @@ void free_chunk(jbp_ctx *ctx) {
     ctx->tp = TrialPlanes();
     ctx->d_keys = nullptr;
     ctx->d_order = nullptr;
+    ctx->d_fb = nullptr;
 }
 
 int ensure_chunk(jbp_ctx *ctx, int count, char *err, size_t errlen) {
@@ -229,6 +230,7 @@ int ensure_chunk(jbp_ctx *ctx, int count, char *err, size_t errlen) {
     HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ctx->h_flags), 3 * c));
     if ((rc = alloc(c * 4, &p))) return rc; ctx->d_keys = static_cast<int *>(p);
     if ((rc = alloc(c * 4, &p))) return rc; ctx->d_order = static_cast<int *>(p);
+    if ((rc = alloc((c + 4) * 4, &p))) return rc; ctx->d_fb = static_cast<int *>(p);
     ctx->chunk_cap = count;
     return JBP_OK;
 }
@@ -447,7 +449,7 @@ int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int wa
         if (lds) {
             launch_decode_qc_f32(ctx->dg, ctx->qcp, ctx->d_qc_tables, ctx->tp, count, nt, max_iter,
                                  warmup, interval, std::max(1, ctx->cus), ctx->d_counter, ctx->stream,
-                                 learned);
+                                 learned, ctx->d_fb);
         } else {
             int rc = run_gen(ctx, count, nt, max_iter, warmup, interval, slots, use_graph, err, errlen);
             if (rc) return rc;
@@ -512,6 +514,7 @@ int ensure_pipe(jbp_ctx *ctx, int cap, char *err, size_t errlen) {
         if ((rc = alloc(4, &p))) return rc; b.counter = static_cast<int *>(p);
         if ((rc = alloc(c * 4, &p))) return rc; b.keys = static_cast<int *>(p);
         if ((rc = alloc(c * 4, &p))) return rc; b.order = static_cast<int *>(p);
+        if ((rc = alloc((c + 4) * 4, &p))) return rc; b.fb = static_cast<int *>(p);
         HIPCHK(hipStreamCreate(&b.st));
         for (auto &e : b.ev) HIPCHK(hipEventCreate(&e));
         HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&b.h_iters), c * 4));
@@ -556,7 +559,7 @@ int simulate_pipelined(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_run_stats &st, 
         launch_fill_i32(b.counter, 0, 1, b.st);
         HIPCHK(hipEventRecord(b.ev[1], b.st));
         launch_decode_qc_f32(g, ctx->qcp, ctx->d_qc_tables, tpo, cnt, nt, cfg->max_iter, warmup,
-                             interval, std::max(1, ctx->cus), b.counter, b.st, learned);
+                             interval, std::max(1, ctx->cus), b.counter, b.st, learned, b.fb);
         HIPCHK(hipEventRecord(b.ev[2], b.st));
         if (ctx->have_logicals) launch_logical(g, ctx->lg[0], ctx->lg[1], b.tp, cnt, b.st);
         else if (ctx->have_basis) launch_residual(g, ctx->basis[0], ctx->basis[1], b.tp, cnt, b.st);

@@ -153,11 +153,13 @@ size_t decode_f64_scratch_bytes(const DevGraph &g, int slots);
 // d_tables: device ints [2][J][L] ainv followed by [2][J][L] b.
 struct LearnedInfer;
 // learned == nullptr: plain BP.  Otherwise learned->T learned iterations,
-// then plain BP restarted on trials that did not converge.
+// then plain BP restarted on trials that did not converge.  The min-sum
+// kind runs as two launches (the learned phase queues its unconverged
+// trials in fb, count + 4 ints of device scratch, for the plain one).
 void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_tables,
                           TrialPlanes tp, int count, const NoiseTab &nt, int max_iter, int check_warmup,
                           int check_interval, int slots, int *counter, hipStream_t st,
-                          const LearnedInfer *learned = nullptr);
+                          const LearnedInfer *learned = nullptr, int *fb = nullptr);
 bool qc_f32_supported(const QcParams &qc, int device, char *why, size_t len);
 void launch_fill_i32(int *ptr, int value, int count, hipStream_t st);
 void launch_unpack_states(const uint32_t *xbits, const uint32_t *zbits, int words, int n,

@@ -38,6 +38,7 @@ struct PipeBuf {
     uint8_t *h_flags = nullptr;  // pinned: ok | success | exact, cap each
     int *keys = nullptr;      // decode-order scratch
     int *order = nullptr;
+    int *fb = nullptr;        // min-sum decoder's fallback list (cap + 4 ints)
 };
 constexpr int kMaxPList = JBP_MAX_P_LIST;
 
@@ -132,6 +133,7 @@ struct jbp_ctx {
     int *d_counter = nullptr;
     int *d_keys = nullptr;    // decode order of the chunk (chunk_cap each)
     int *d_order = nullptr;
+    int *d_fb = nullptr;      // min-sum decoder's fallback list (chunk_cap + 4)
     int *d_states = nullptr;  // unpack buffer
     size_t d_states_cap = 0;
     int *d_ints = nullptr;    // pack input buffer

@@ -317,34 +317,240 @@ __device__ __forceinline__ int qc_phase(float *B, float *GZ, int *s_cnt, float *
     }
 }
 
-// LEARNED: the first li.T iterations use the learned weights and damping; a
-// trial that has not converged by then restarts plain BP from the channel
-// prior for up to max_iter iterations, and its iteration count is li.T plus
-// the plain iterations.
-template <int P, int J, int L, bool LEARNED, bool DAMPED>
-__global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, TrialPlanes tp,
-                                                         int count, NoiseTab nt, int max_iter,
-                                                         int warmup, int interval, int *counter,
-                                                         LearnedInfer li) {
-    static_assert(P % 64 == 0 && P <= 1024, "one thread per block row, whole wavefronts");
-    extern __shared__ float B[];  // [J][L][P] exchange buffer, then GZ[L][P]
-    float *GZ = B + J * L * P;   // Z-side extrinsic base per variable, parked between passes
-    __shared__ int s_trial;
-    __shared__ int s_cnt;        // learned: unsatisfied checks of the estimate under test
-    __shared__ float s_d;        // learned: damping of the current iteration
-    __shared__ float s_w[LEARNED ? 2 * 4 * J * L : 1];  // learned: weights, double buffer
+
+// ---------------------------------------------------------------------------
+// Learned min-sum phase (the fast learned decoder).
+//
+// Normalised min-sum with the max-log joint variable node (the max-product
+// form of the updates above), learned weights per (side, block, iteration)
+// on both message directions and optional MLP damping; the training kernels
+// (jbp_learned.hip, kind 1) define it.  In base 2 (all updates are
+// homogeneous, so the weights are the same in any base):
+//   S      = sum_r wc[r] * lambda[r]                (per side)
+//   F      = max(lp_I + S', lp_Z) - max(lp_X + S', lp_Y)   (X side; S' = the
+//            other side's S; Z side with X and Z swapped)
+//   est    = sign bits of G = S + F
+//   mu[r]  = wv[r] * (G - wc[r] * lambda[r])
+//   lambda = (-1)^syn prod_{k != e} sign(mu_k) * min(64, min_{k != e} |mu_k|)
+// X and Z travel together as FP16 pairs: word B[r][cc][x] holds, in its low
+// half, the message of X check (r, x) at position cc and, in its high half,
+// that of Z check (r, x), so a check owner updates its X and Z checks with
+// the same packed instructions, and a variable owner keeps its 36 incoming
+// (X, Z) pairs in 36 registers.  The estimate bits ride in the least
+// significant bit of each half of mu (bit 0 and bit 16 of the word).
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) _Float16 lds_h16;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+__device__ __forceinline__ h2 as_h2(uint32_t u) { return __builtin_bit_cast(h2, u); }
+__device__ __forceinline__ uint32_t h2_bits(h2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t u) { return __builtin_bit_cast(u16x2, u); }
+__device__ __forceinline__ uint32_t u16x2_bits(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ h2 hmax2(h2 a, h2 b) { return __builtin_elementwise_max(a, b); }
+// magnitudes (sign bits clear) compare as unsigned integers
+__device__ __forceinline__ u16x2 umin2(u16x2 a, u16x2 b) { return __builtin_elementwise_min(a, b); }
+
+constexpr uint32_t kMsClampPair = 0x54005400u;  // (64.0, 64.0) in FP16
+
+template <int P, int J, int L, bool DAMPED>
+__device__ __forceinline__ int qc_phase_ms(uint32_t *Bw, int *s_cnt, float *s_d, uint32_t *s_w,
+                                           uint32_t (&aw)[2][J][(L + 1) / 2], uint32_t synx,
+                                           uint32_t synz, const PriorF &pr, int kmax,
+                                           const LearnedInfer &li, float u0, uint32_t &estx,
+                                           uint32_t &estz, int &matched) {
+    constexpr int HB = J * L / 2;
     const int t = threadIdx.x;
     const int lane = t & 63;
+    // Byte address of the word holding variable (cc, t)'s message in block
+    // (r, cc) of side s (its Z half is 2 bytes further on), without the
+    // block's offset inside its half of B, which the LDS instructions take as
+    // their immediate.  Unpacked once per trial into 72 registers; the packed
+    // copy is rebuilt from them on exit, so it holds no registers meanwhile.
+    uint32_t ad[2][J][L];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int r = 0; r < J; ++r)
+#pragma unroll
+            for (int cc = 0; cc < L; ++cc)
+                ad[s][r][cc] = (cc & 1) ? (aw[s][r][cc >> 1] >> 16) : (aw[s][r][cc >> 1] & 0xffffu);
+    auto repack = [&]() {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int r = 0; r < J; ++r)
+#pragma unroll
+                for (int w = 0; w < (L + 1) / 2; ++w)
+                    aw[s][r][w] = ad[s][r][2 * w] | (2 * w + 1 < L ? ad[s][r][2 * w + 1] << 16 : 0u);
+    };
+    auto at = [&](int s, int r, int cc) -> uint32_t {
+        return ad[s][r][cc] + static_cast<uint32_t>(((r * L + cc) % HB) * P * 4) + (s ? 2u : 0u);
+    };
+    auto ldh = [](uint32_t a) -> _Float16 { return *reinterpret_cast<lds_h16 *>(static_cast<uintptr_t>(a)); };
+    auto sth = [](uint32_t a, _Float16 v) { *reinterpret_cast<lds_h16 *>(static_cast<uintptr_t>(a)) = v; };
+    auto stage = [&](int kk) {
+        if (t < kMsRec && kk <= kmax) s_w[(kk & 1) * kMsRec + t] = li.ms_rec[kk * kMsRec + t];
+    };
+    // channel constants as (X side, Z side) pairs
+    const h2 cA = {(_Float16)pr.lp[0], (_Float16)pr.lp[0]};
+    const h2 cB = {(_Float16)pr.lp[2], (_Float16)pr.lp[1]};
+    const h2 cC = {(_Float16)pr.lp[1], (_Float16)pr.lp[2]};
+    const h2 cD = {(_Float16)pr.lp[3], (_Float16)pr.lp[3]};
+    // damped: the previous check messages, mixed into the new ones
+    h2 lam[DAMPED ? J : 1][DAMPED ? L : 1];
+    if (DAMPED) {
+#pragma unroll
+        for (int r = 0; r < J; ++r)
+#pragma unroll
+            for (int cc = 0; cc < L; ++cc) lam[r][cc] = h2{(_Float16)0.0f, (_Float16)0.0f};
+    }
+    stage(0);
+    if (DAMPED && t == 0) {
+        s_cnt[0] = 0;
+        s_cnt[1] = 0;
+    }
+    // lambda_0 = 0: clear the exchange words the first variable pass reads
+    {
+        lds_u32 *bw = reinterpret_cast<lds_u32 *>(reinterpret_cast<uintptr_t>(Bw));
+#pragma unroll
+        for (int i = 0; i < J * L; ++i) bw[i * P + t] = 0u;
+    }
+    float ukm1 = u0;
+    float dprev = 0.0f;  // damping of the check messages being read (iteration k)
+    int k = 0;
+    __syncthreads();
+    for (;;) {
+        stage(k + 1);
+        const uint32_t *rec = s_w + (k & 1) * kMsRec;
+        // Variable pass k: read lambda_k (left in place by the check pass;
+        // zeros before the first), form the estimate bits and mu, write mu
+        // back to the same halves.
+        uint32_t acc_est = 0;
+        // software-pipelined over column blocks: the next block's check
+        // messages are in flight while this block's arithmetic runs
+        h2 nxt[J];
+#pragma unroll
+        for (int r = 0; r < J; ++r) nxt[r] = h2{ldh(at(0, r, 0)), ldh(at(1, r, 0))};
+#pragma unroll
+        for (int cc = 0; cc < L; ++cc) {
+            h2 lv[J];
+#pragma unroll
+            for (int r = 0; r < J; ++r) lv[r] = nxt[r];
+            if (cc + 1 < L) {
+#pragma unroll
+                for (int r = 0; r < J; ++r) nxt[r] = h2{ldh(at(0, r, cc + 1)), ldh(at(1, r, cc + 1))};
+            }
+            const uint4 w0 = *reinterpret_cast<const uint4 *>(rec + cc * kMsRecCC);
+            const uint4 w1 = *reinterpret_cast<const uint4 *>(rec + cc * kMsRecCC + 4);
+            const h2 wc[3] = {as_h2(w0.x), as_h2(w0.y), as_h2(w0.z)};
+            const h2 wv[3] = {as_h2(w0.w), as_h2(w1.x), as_h2(w1.y)};
+            h2 S;
+            if (DAMPED) {
+                // lambda_k = (1 - d_k) wc_{k-1} CN + d_k lambda_{k-1}
+                const h2 dd = {(_Float16)dprev, (_Float16)dprev};
+                const h2 c1 = {(_Float16)(1.0f - dprev), (_Float16)(1.0f - dprev)};
+#pragma unroll
+                for (int r = 0; r < J; ++r) {
+                    lam[r][cc] = __builtin_elementwise_fma(c1 * wc[r], lv[r], dd * lam[r][cc]);
+                    lv[r] = lam[r][cc];
+                }
+                S = lv[0] + lv[1] + lv[2];
+            } else {
+                S = wc[0] * lv[0];
+#pragma unroll
+                for (int r = 1; r < J; ++r) S = __builtin_elementwise_fma(wc[r], lv[r], S);
+            }
+            const h2 Sw = __builtin_shufflevector(S, S, 1, 0);
+            const h2 G = S + (hmax2(Sw + cA, cB) - hmax2(Sw + cC, cD));
+            const uint32_t eb = (h2_bits(G) >> 15) & 0x00010001u;
+            acc_est |= eb << cc;
+#pragma unroll
+            for (int r = 0; r < J; ++r) {
+                const h2 m = DAMPED ? (G - lv[r]) : __builtin_elementwise_fma(-wc[r], lv[r], G);
+                const h2 mb = as_h2((h2_bits(wv[r] * m) & 0xfffefffeu) | eb);
+                sth(at(0, r, cc), mb.x);
+                sth(at(1, r, cc), mb.y);
+            }
+        }
+        if (DAMPED && t < 64) {
+            const float d = mlp_damping_wave(li.params + 4 * J * L * li.T,
+                                             static_cast<float>(k + 1) / static_cast<float>(li.T),
+                                             ukm1, lane);
+            if (t == 0) *s_d = d;
+        }
+        __syncthreads();
+        if (DAMPED && t == 0) s_cnt[(k + 1) & 1] = 0;
+        // check pass: X check (r, t) in the low halves, Z check (r, t) in the high
+        uint32_t bad = 0;
+        int nbad = 0;
+#pragma unroll 1
+        for (int r = 0; r < J; ++r) {
+            lds_u32 *row = reinterpret_cast<lds_u32 *>(reinterpret_cast<uintptr_t>(Bw)) + r * L * P + t;
+            const uint32_t synw = ((synx >> r) & 1u) | (((synz >> r) & 1u) << 16);
+            uint32_t u[L];
+            u16x2 pre[L];
+            u16x2 run = as_u16x2(kMsClampPair);
+            uint32_t acc = 0;
+#pragma unroll
+            for (int i = 0; i < L; ++i) {
+                u[i] = row[i * P];
+                acc ^= u[i];
+                pre[i] = run;
+                run = umin2(run, as_u16x2(u[i] & 0x7fff7fffu));
+            }
+            const uint32_t par = (acc ^ synw) & 0x00010001u;  // estimate parity vs syndrome
+            bad |= par;
+            if (DAMPED) nbad += __popc(par);
+            const uint32_t sall = acc ^ (synw << 15);
+            u16x2 suf = as_u16x2(kMsClampPair);
+#pragma unroll
+            for (int i = L - 1; i >= 0; --i) {
+                const uint32_t mag = u16x2_bits(umin2(pre[i], suf));
+                suf = umin2(suf, as_u16x2(u[i] & 0x7fff7fffu));
+                row[i * P] = mag | ((sall ^ u[i]) & 0x80008000u);
+            }
+        }
+        if (DAMPED) {
+            // unsatisfied checks of est_k: a wave sum, one LDS atomic per wave
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) nbad += __shfl_xor(nbad, m);
+            if (lane == 0 && nbad) atomicAdd(&s_cnt[k & 1], nbad);
+        }
+        const int any_bad = __syncthreads_or(static_cast<int>(bad));
+        estx = acc_est & 0xffffu;
+        estz = acc_est >> 16;
+        if (k >= 1 && !any_bad) {
+            matched = 1;
+            repack();
+            return k;
+        }
+        if (k >= kmax) {
+            matched = 0;
+            repack();
+            return k;
+        }
+        if (DAMPED) {
+            ukm1 = static_cast<float>(s_cnt[k & 1]) / static_cast<float>(2 * J * P);
+            dprev = *s_d;
+        }
+        ++k;
+    }
+}
 
-    // Row index x = ainv*(t - b) mod P of this thread's variable (cc, t) in
-    // block (side, r, cc); its exchange slot is B[(r*L + cc)*P + x], kept as
-    // a 16-bit LDS address (two per register, see qc_phase).
+// Exchange-slot addresses of thread t's variables (cc, t): row index
+// x = ainv*(t - b) mod P in block (side, r, cc), whose slot is
+// B[(r*L + cc)*P + x], kept as a 16-bit LDS address (two per register: see
+// qc_phase); blocks of the upper half of B carry the half's offset.
+template <int P, int J, int L>
+__device__ __forceinline__ void qc_slot_addresses(const QcDev &qc, const float *B,
+                                                  uint32_t (&aw)[2][J][(L + 1) / 2]) {
     constexpr int HB = J * L / 2;
     static_assert((P - 1) * 4 + HB * P * 4 + 4096 < 65536 && (HB - 1) * P * 4 < 65536,
                   "exchange addresses must fit 16 bits and the LDS immediate offset");
-    const uint32_t bbase = static_cast<uint32_t>((uintptr_t)((lds_float *)B));  // LDS address of B
+    const int t = threadIdx.x;
+    const uint32_t bbase = static_cast<uint32_t>((uintptr_t)((const lds_float *)B));  // LDS address of B
     constexpr int W = (L + 1) / 2;
-    uint32_t aw[2][J][W];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -366,84 +572,188 @@ __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, 
                     bbase + x * 4u + (blk >= HB ? static_cast<uint32_t>(HB * P * 4) : 0u);
                 aw[s][r][cc >> 1] |= a16 << (16 * (cc & 1));
             }
+}
+
+// Channel constants of one trial, uniform across the workgroup.
+__device__ __forceinline__ PriorF trial_prior(const NoiseTab &nt, int trial) {
+    PriorF pr;
+    const int ent = __builtin_amdgcn_readfirstlane(nt.entry(trial));
+    const float *q = nt.prior32 + kPrior32 * ent;
+    auto uni = [](float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); };
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        pr.lp[i] = uni(q[i]);
+        pr.fx[i] = uni(q[4 + i]);
+        pr.fz[i] = uni(q[8 + i]);
+    }
+    return pr;
+}
+
+// Estimate bit planes and outcome of one trial: variable (cc, t) is bit
+// cc*P + t; P % 64 == 0 makes each wavefront's 64 variables one aligned
+// 64-bit span.
+template <int P, int L>
+__device__ __forceinline__ void write_outcome(const DevGraph &g, const TrialPlanes &tp, int trial,
+                                              uint32_t estx, uint32_t estz, int iterations,
+                                              int matched) {
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    uint32_t *ox = tp.est_x + static_cast<size_t>(trial) * g.wn;
+    uint32_t *oz = tp.est_z + static_cast<size_t>(trial) * g.wn;
+#pragma unroll
+    for (int cc = 0; cc < L; ++cc) {
+        uint64_t bx = __ballot((estx >> cc) & 1u);
+        uint64_t bz = __ballot((estz >> cc) & 1u);
+        if (lane == 0) {
+            int w = (cc * P + t) >> 5;
+            ox[w] = static_cast<uint32_t>(bx);
+            ox[w + 1] = static_cast<uint32_t>(bx >> 32);
+            oz[w] = static_cast<uint32_t>(bz);
+            oz[w + 1] = static_cast<uint32_t>(bz >> 32);
+        }
+    }
+    if (t == 0) {
+        tp.iters[trial] = iterations;
+        tp.syn_ok[trial] = static_cast<uint8_t>(matched);
+    }
+}
+
+// Syndrome bits of checks (r, t): bit r.
+template <int P, int J>
+__device__ __forceinline__ void trial_syndromes(const DevGraph &g, const TrialPlanes &tp, int trial,
+                                                uint32_t &synx, uint32_t &synz) {
+    const int t = threadIdx.x;
+    synx = 0;
+    synz = 0;
+#pragma unroll
+    for (int r = 0; r < J; ++r) {
+        int c = r * P + t;
+        synx |= ((tp.syn_x[static_cast<size_t>(trial) * g.wm[0] + (c >> 5)] >> (c & 31)) & 1u) << r;
+        synz |= ((tp.syn_z[static_cast<size_t>(trial) * g.wm[1] + (c >> 5)] >> (c & 31)) & 1u) << r;
+    }
+}
+
+// Plain BP (LEARNED = false) or the sum-product learned decoder: the first
+// li.T iterations use the learned weights and damping; a trial that has not
+// converged by then restarts plain BP from the channel prior for up to
+// max_iter iterations, and its iteration count is li.T plus the plain
+// iterations.  count_ptr (plain only, may be null): the trial count lives on
+// the device (the fallback list of the min-sum decoder), and iter_offset is
+// added to every reported iteration count.
+template <int P, int J, int L, bool LEARNED, bool DAMPED>
+__global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, TrialPlanes tp,
+                                                         int count, NoiseTab nt, int max_iter,
+                                                         int warmup, int interval, int *counter,
+                                                         LearnedInfer li, const int *count_ptr,
+                                                         int iter_offset) {
+    static_assert(P % 64 == 0 && P <= 1024, "one thread per block row, whole wavefronts");
+    extern __shared__ float B[];  // [J][L][P] exchange buffer, then GZ[L][P]
+    float *GZ = B + J * L * P;   // Z-side extrinsic base per variable, parked between passes
+    __shared__ int s_trial;
+    __shared__ int s_cnt[2];     // learned: unsatisfied checks of the estimate under test
+    __shared__ float s_d;        // learned: damping of the current iteration
+    __shared__ float s_w[LEARNED ? 2 * 4 * J * L : 1];  // learned: weights, double buffer
+    const int t = threadIdx.x;
+    if (count_ptr) count = *count_ptr;
+    uint32_t aw[2][J][(L + 1) / 2];
+    qc_slot_addresses<P, J, L>(qc, B, aw);
 
     for (;;) {
         if (t == 0) {
+            // the k-th trial handed out: order[k] (a permutation of the
+            // chunk, or the list of trials a previous launch left over)
             const int k = atomicAdd(counter, 1);
-            s_trial = (tp.order && k < count) ? tp.order[k] : k;
+            s_trial = k < count ? (tp.order ? tp.order[k] : k) : -1;
         }
         __syncthreads();
         const int trial = s_trial;
         __syncthreads();
-        if (trial >= count) return;
-        // this trial's channel constants, uniform across the workgroup
-        PriorF pr;
-        {
-            const int ent = __builtin_amdgcn_readfirstlane(nt.entry(trial));
-            const float *q = nt.prior32 + kPrior32 * ent;
-            auto uni = [](float x) {
-                return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
-            };
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                pr.lp[i] = uni(q[i]);
-                pr.fx[i] = uni(q[4 + i]);
-                pr.fz[i] = uni(q[8 + i]);
-            }
-        }
-
-        uint32_t synx = 0, synz = 0;  // bit r: syndrome of check (r, t)
-#pragma unroll
-        for (int r = 0; r < J; ++r) {
-            int c = r * P + t;
-            synx |= ((tp.syn_x[static_cast<size_t>(trial) * g.wm[0] + (c >> 5)] >> (c & 31)) & 1u) << r;
-            synz |= ((tp.syn_z[static_cast<size_t>(trial) * g.wm[1] + (c >> 5)] >> (c & 31)) & 1u) << r;
-        }
+        if (trial < 0) return;
+        const PriorF pr = trial_prior(nt, trial);
+        uint32_t synx, synz;  // bit r: syndrome of check (r, t)
+        trial_syndromes<P, J>(g, tp, trial, synx, synz);
         float lam[2][J][L];
         uint32_t estx = 0, estz = 0;  // bit cc: hard decision of variable (cc, t)
         int matched = 0, iterations = 0;
         if (LEARNED) {
             // unsatisfied fraction of est_0 (all identity): the syndrome weight
-            if (t == 0) s_cnt = 0;
+            if (t == 0) s_cnt[0] = 0;
             __syncthreads();
-            atomicAdd(&s_cnt, __popc(synx) + __popc(synz));
+            atomicAdd(&s_cnt[0], __popc(synx) + __popc(synz));
             __syncthreads();
-            const float u0 = static_cast<float>(s_cnt) / static_cast<float>(2 * J * P);
-            int k = qc_phase<P, J, L, true, DAMPED>(B, GZ, &s_cnt, &s_d, s_w, aw, lam, synx, synz, pr, li.T, 0,
-                                            1, li, u0, estx, estz, matched);
+            const float u0 = static_cast<float>(s_cnt[0]) / static_cast<float>(2 * J * P);
+            __syncthreads();
+            int k = qc_phase<P, J, L, true, DAMPED>(B, GZ, s_cnt, &s_d, s_w, aw, lam, synx, synz, pr,
+                                                    li.T, 0, 1, li, u0, estx, estz, matched);
             if (matched) {
                 iterations = k;
             } else {
-                k = qc_phase<P, J, L, false, false>(B, GZ, &s_cnt, &s_d, s_w, aw, lam, synx, synz, pr,
-                                             max_iter, warmup, interval, li, 0.0f, estx, estz,
-                                             matched);
+                k = qc_phase<P, J, L, false, false>(B, GZ, s_cnt, &s_d, s_w, aw, lam, synx, synz, pr,
+                                                    max_iter, warmup, interval, li, 0.0f, estx, estz,
+                                                    matched);
                 iterations = li.T + k;
             }
         } else {
-            iterations = qc_phase<P, J, L, false, false>(B, GZ, &s_cnt, &s_d, s_w, aw, lam, synx, synz, pr,
-                                                  max_iter, warmup, interval, li, 0.0f, estx,
-                                                  estz, matched);
+            iterations = qc_phase<P, J, L, false, false>(B, GZ, s_cnt, &s_d, s_w, aw, lam, synx, synz,
+                                                         pr, max_iter, warmup, interval, li, 0.0f,
+                                                         estx, estz, matched) + iter_offset;
         }
+        write_outcome<P, L>(g, tp, trial, estx, estz, iterations, matched);
+        __syncthreads();
+    }
+}
 
-        // est bit planes: variable (cc, t) is bit cc*P + t; P % 64 == 0
-        // makes each wavefront's 64 variables one aligned 64-bit span.
-        uint32_t *ox = tp.est_x + static_cast<size_t>(trial) * g.wn;
-        uint32_t *oz = tp.est_z + static_cast<size_t>(trial) * g.wn;
-#pragma unroll
-        for (int cc = 0; cc < L; ++cc) {
-            uint64_t bx = __ballot((estx >> cc) & 1u);
-            uint64_t bz = __ballot((estz >> cc) & 1u);
-            if (lane == 0) {
-                int w = (cc * P + t) >> 5;
-                ox[w] = static_cast<uint32_t>(bx);
-                ox[w + 1] = static_cast<uint32_t>(bx >> 32);
-                oz[w] = static_cast<uint32_t>(bz);
-                oz[w + 1] = static_cast<uint32_t>(bz >> 32);
-            }
-        }
+// The learned min-sum decoder (li.T iterations of qc_phase_ms).  A trial
+// that converges is finished here; the others are appended to fb_list
+// (fb_ctr[0] of them) for plain BP, launched next on the same stream, which
+// decodes them from the channel prior (reported iterations li.T + its own).
+// Its exchange buffer is B only (J*L*P words).
+template <int P, int J, int L, bool DAMPED>
+__global__ void __launch_bounds__(P) decode_qc_ms_kernel(DevGraph g, QcDev qc, TrialPlanes tp,
+                                                        int count, NoiseTab nt, int *counter,
+                                                        LearnedInfer li, int *fb_list, int *fb_ctr) {
+    static_assert(P % 64 == 0 && P <= 1024, "one thread per block row, whole wavefronts");
+    extern __shared__ float B[];  // [J][L][P] words: (X, Z) FP16 pairs
+    __shared__ int s_trial;
+    __shared__ int s_cnt[2];
+    __shared__ float s_d;
+    __shared__ __align__(16) uint32_t s_w[2 * kMsRec];
+    const int t = threadIdx.x;
+    uint32_t aw[2][J][(L + 1) / 2];
+    qc_slot_addresses<P, J, L>(qc, B, aw);
+    for (;;) {
         if (t == 0) {
-            tp.iters[trial] = iterations;
-            tp.syn_ok[trial] = static_cast<uint8_t>(matched);
+            // the k-th trial handed out: order[k] (a permutation of the
+            // chunk, or the list of trials a previous launch left over)
+            const int k = atomicAdd(counter, 1);
+            s_trial = k < count ? (tp.order ? tp.order[k] : k) : -1;
+        }
+        __syncthreads();
+        const int trial = s_trial;
+        __syncthreads();
+        if (trial < 0) return;
+        const PriorF pr = trial_prior(nt, trial);
+        uint32_t synx, synz;
+        trial_syndromes<P, J>(g, tp, trial, synx, synz);
+        float u0 = 0.0f;
+        if (DAMPED) {
+            // unsatisfied fraction of est_0 (all identity): the syndrome weight
+            if (t == 0) s_cnt[0] = 0;
+            __syncthreads();
+            atomicAdd(&s_cnt[0], __popc(synx) + __popc(synz));
+            __syncthreads();
+            u0 = static_cast<float>(s_cnt[0]) / static_cast<float>(2 * J * P);
+            __syncthreads();
+        }
+        uint32_t estx = 0, estz = 0;
+        int matched = 0;
+        const int k = qc_phase_ms<P, J, L, DAMPED>(reinterpret_cast<uint32_t *>(B), s_cnt, &s_d, s_w,
+                                                   aw, synx, synz, pr, li.T, li, u0, estx, estz,
+                                                   matched);
+        if (matched) {
+            write_outcome<P, L>(g, tp, trial, estx, estz, k, 1);
+        } else if (t == 0) {
+            fb_list[atomicAdd(fb_ctr, 1)] = trial;
         }
         __syncthreads();
     }
@@ -480,7 +790,7 @@ bool qc_f32_supported(const QcParams &qc, int device, char *why, size_t len) {
 void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_tables,
                           TrialPlanes tp, int count, const NoiseTab &nt, int max_iter, int check_warmup,
                           int check_interval, int slots, int *counter, hipStream_t st,
-                          const LearnedInfer *learned) {
+                          const LearnedInfer *learned, int *fb) {
     if (count <= 0) return;
     QcDev dev;
     dev.P = qc.P;
@@ -492,17 +802,36 @@ void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_ta
     int warmup = check_warmup > 0 ? check_warmup : 0;
     LearnedInfer li;
     if (learned) li = *learned;
-    auto launch = [&](auto kern) {
+    auto launch = [&](auto kern, const int *count_ptr, int offset, int *ctr, TrialPlanes planes) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(768), lds, st, g, dev, tp, count, nt, max_iter,
-                           warmup, interval, counter, li);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(768), lds, st, g, dev, planes, count, nt, max_iter,
+                           warmup, interval, ctr, li, count_ptr, offset);
     };
-    if (learned && learned->T > 0) {
-        if (learned->use_mlp) launch(decode_qc_f32_kernel<768, 3, 12, true, true>);
-        else launch(decode_qc_f32_kernel<768, 3, 12, true, false>);
+    if (learned && learned->T > 0 && learned->kind == kLearnedMS) {
+        // min-sum phase, then plain BP over the trials it left unconverged
+        // (fb: [0] their count, [1] the plain decoder's work counter, [2..]
+        // none; fb + 4: the list, count entries)
+        int *fb_list = fb + 4;
+        launch_fill_i32(fb, 0, 2, st);
+        const size_t ms_lds = static_cast<size_t>(qc.J) * qc.L * qc.P * sizeof(uint32_t);
+        auto ms = [&](auto kern) {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(ms_lds));
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(768), ms_lds, st, g, dev, tp, count, nt, counter, li,
+                               fb_list, fb);
+        };
+        if (learned->use_mlp) ms(decode_qc_ms_kernel<768, 3, 12, true>);
+        else ms(decode_qc_ms_kernel<768, 3, 12, false>);
+        TrialPlanes fbp = tp;
+        fbp.order = fb_list;
+        li = LearnedInfer();
+        launch(decode_qc_f32_kernel<768, 3, 12, false, false>, fb, learned->T, fb + 1, fbp);
+    } else if (learned && learned->T > 0) {
+        if (learned->use_mlp) launch(decode_qc_f32_kernel<768, 3, 12, true, true>, nullptr, 0, counter, tp);
+        else launch(decode_qc_f32_kernel<768, 3, 12, true, false>, nullptr, 0, counter, tp);
     } else {
-        launch(decode_qc_f32_kernel<768, 3, 12, false, false>);
+        launch(decode_qc_f32_kernel<768, 3, 12, false, false>, nullptr, 0, counter, tp);
     }
 }
 

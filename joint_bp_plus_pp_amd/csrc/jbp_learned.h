@@ -19,8 +19,13 @@ constexpr int kMlpParams = 4 * kMlpHidden + 1;  // 65
 constexpr int kMaxLearnedT = 64;
 constexpr int kMaxLearnedBlocks = 64;
 
+// Model kinds: sum-product (the reference's joint BP with weights) and
+// normalised min-sum (max-product updates, the fast learned decoder).
+constexpr int kLearnedSP = 0;
+constexpr int kLearnedMS = 1;
+
 struct LearnedDevParams {
-    int nb = 0, T = 0, use_mlp = 1;
+    int nb = 0, T = 0, use_mlp = 1, kind = kLearnedSP;
     const int *blk0 = nullptr;  // block of each X edge (device)
     const int *blk1 = nullptr;  // block of each Z edge (device)
 };
@@ -48,6 +53,15 @@ struct LearnedInfer {
     const float *params = nullptr;
     int T = 0;
     int use_mlp = 1;
+    int kind = kLearnedSP;
+    // min-sum kind: per variable pass k = 0..T one record of kMsRec words,
+    // for each column block cc: (wc X, wc Z) of iteration k-1 (1 at k = 0)
+    // for r = 0..J-1, then (wv X, wv Z) of iteration k (1 at k = T), as FP16
+    // pairs, two words of padding (device, (T+1) x kMsRec uint32)
+    const uint32_t *ms_rec = nullptr;
 };
+// Words of one min-sum weight record: 12 column blocks x (3 + 3 + 2).
+constexpr int kMsRecCC = 8;
+constexpr int kMsRec = 12 * kMsRecCC;
 
 }  // namespace jbp

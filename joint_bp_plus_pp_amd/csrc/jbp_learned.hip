@@ -22,6 +22,13 @@
 // h_e = dL/dlambda_e * sign * cosh^2(lambda_e / 2),
 //   dL/dmu_j = sech^2(mu_j / 2) * dQ/dt_j,   Q = sum_e h_e prod_{k != e} t_k,
 // and all dQ/dt_j come from one prefix and one suffix recursion.
+//
+// KIND = 1 is the normalised min-sum form of the same model (the
+// production learned decoder): CN_e = (-1)^syn prod_{k != e} sign(mu_k) *
+// min_{k != e} |mu_k| and F(L) = max(ln a + L, ln b) - max(ln c + L, ln d),
+// hard decisions from the signs of the X / Z marginals.  Its check adjoint
+// routes each edge's gradient to the input that attains its minimum: the
+// overall arg-minimum i1 for every edge but i1 itself, the second i2 for i1.
 #include "jbp_common.h"
 #include "jbp_learned.h"
 
@@ -69,6 +76,28 @@ struct Model {
     const int *blk0, *blk1;  // block of each edge, per side
     int nb, T, use_mlp;
 };
+
+// Channel constants of one trial: (a, b, c, d) of each side's joint term and
+// their logarithms (the min-sum form uses the logarithms).
+template <typename R>
+struct Chan {
+    R fx[4], fz[4], lfx[4], lfz[4], lp[4];
+};
+
+// KIND 0: F(L) = ln((a e^L + b) / (c e^L + d)); KIND 1: its max-log form.
+template <int KIND, typename R>
+__device__ __forceinline__ R fterm_k(R L, const R *k, const R *lk) {
+    if (KIND == 1) {
+        const R u = lk[0] + L, v = lk[2] + L;
+        return (u > lk[1] ? u : lk[1]) - (v > lk[3] ? v : lk[3]);
+    }
+    return fterm(L, k);
+}
+template <int KIND, typename R>
+__device__ __forceinline__ R fterm_kd(R L, const R *k, const R *lk) {
+    if (KIND == 1) return (lk[0] + L > lk[1] ? R(1) : R(0)) - (lk[2] + L > lk[3] ? R(1) : R(0));
+    return fterm_d(L, k);
+}
 
 template <typename R>
 struct Tape {
@@ -136,6 +165,35 @@ __device__ void check_node(const R *mu, int deg, int syn, R *raw, R *out) {
     }
 }
 
+// Min-sum check node of one check: raw (unclamped) and clamped outputs.
+template <typename R>
+__device__ void check_node_ms(const R *mu, int deg, int syn, R *raw, R *out) {
+    const R lmax = R(64.0 * 0.69314718055994530942);
+    R m1 = R(HUGE_VAL), m2 = R(HUGE_VAL);
+    int i1 = -1;
+    uint32_t neg = 0, negall = static_cast<uint32_t>(syn & 1);
+    for (int i = 0; i < deg; ++i) {
+        const R m = mu[i];
+        const R a = fabs(m);
+        const uint32_t ng = m < R(0) ? 1u : 0u;
+        neg |= ng << i;
+        negall ^= ng;
+        if (a < m1) {
+            m2 = m1;
+            m1 = a;
+            i1 = i;
+        } else if (a < m2) {
+            m2 = a;
+        }
+    }
+    for (int i = 0; i < deg; ++i) {
+        const R mag = i == i1 ? m2 : m1;
+        const R sg = ((negall ^ (neg >> i)) & 1u) ? R(-1) : R(1);
+        raw[i] = sg * mag;
+        out[i] = sg * (mag < lmax ? mag : lmax);
+    }
+}
+
 template <typename R>
 struct Slot {
     R *mu, *a, *raw, *out, *gc, *gp, *S;  // Et, Et, Et, Et, Et, Et, 2n
@@ -159,10 +217,15 @@ __device__ Slot<R> slot_of(const DevGraph &g, const Tape<R> &tb) {
 }
 
 template <typename R>
-__device__ void priors(const double *p4, R *fx, R *fz, R *lp) {
-    fx[0] = R(p4[0]); fx[1] = R(p4[2]); fx[2] = R(p4[1]); fx[3] = R(p4[3]);
-    fz[0] = R(p4[0]); fz[1] = R(p4[1]); fz[2] = R(p4[2]); fz[3] = R(p4[3]);
-    for (int i = 0; i < 4; ++i) lp[i] = R(log(p4[i]));
+__device__ void priors(const double *p4, Chan<R> &ch) {
+    const int ox[4] = {0, 2, 1, 3};
+    for (int i = 0; i < 4; ++i) {
+        ch.fx[i] = R(p4[ox[i]]);
+        ch.fz[i] = R(p4[i]);
+        ch.lfx[i] = R(log(p4[ox[i]]));
+        ch.lfz[i] = R(log(p4[i]));
+        ch.lp[i] = R(log(p4[i]));
+    }
 }
 
 __device__ __forceinline__ int syn_bit(const DevGraph &g, const TrialPlanes &tp, int b, int s, int c) {
@@ -176,9 +239,9 @@ __device__ __forceinline__ int err_state(const DevGraph &g, const TrialPlanes &t
 }
 
 // Variable pass of iteration t from lambda(t-1) = prev: S sums, then a and mu.
-template <typename R>
-__device__ void var_pass(const DevGraph &g, const Model<R> &M, int t, const R *prev, const R *fx,
-                         const R *fz, Slot<R> &sl) {
+template <int KIND, typename R>
+__device__ void var_pass(const DevGraph &g, const Model<R> &M, int t, const R *prev,
+                         const Chan<R> &ch, Slot<R> &sl) {
     const int n = g.n, Ex = g.side[0].E;
     for (int v = threadIdx.x; v < n; v += kBlock) {
         R sx = R(0), sz = R(0);
@@ -186,7 +249,7 @@ __device__ void var_pass(const DevGraph &g, const Model<R> &M, int t, const R *p
         for (int k = g.side[1].var_off[v]; k < g.side[1].var_off[v + 1]; ++k) sz += prev[Ex + g.side[1].var_edge[k]];
         sl.S[v] = sx;
         sl.S[n + v] = sz;
-        const R f0 = fterm(sz, fx), f1 = fterm(sx, fz);
+        const R f0 = fterm_k<KIND>(sz, ch.fx, ch.lfx), f1 = fterm_k<KIND>(sx, ch.fz, ch.lfz);
         for (int k = g.side[0].var_off[v]; k < g.side[0].var_off[v + 1]; ++k) {
             int e = g.side[0].var_edge[k];
             R a = sx - prev[e] + f0;
@@ -203,7 +266,7 @@ __device__ void var_pass(const DevGraph &g, const Model<R> &M, int t, const R *p
 }
 
 // Check pass: raw and clamped check outputs of every edge from mu.
-template <typename R>
+template <int KIND, typename R>
 __device__ void check_pass(const DevGraph &g, const TrialPlanes &tp, int b, Slot<R> &sl) {
     const int mx = g.side[0].m, mz = g.side[1].m, Ex = g.side[0].E;
     for (int c = threadIdx.x; c < mx + mz; c += kBlock) {
@@ -211,12 +274,16 @@ __device__ void check_pass(const DevGraph &g, const TrialPlanes &tp, int b, Slot
         const int cc = c < mx ? c : c - mx;
         const int base = s == 0 ? 0 : Ex;
         const int off = g.side[s].chk_off[cc], deg = g.side[s].chk_off[cc + 1] - off;
-        check_node(sl.mu + base + off, deg, syn_bit(g, tp, b, s, cc), sl.raw + base + off,
-                   sl.out + base + off);
+        if (KIND == 1)
+            check_node_ms(sl.mu + base + off, deg, syn_bit(g, tp, b, s, cc), sl.raw + base + off,
+                          sl.out + base + off);
+        else
+            check_node(sl.mu + base + off, deg, syn_bit(g, tp, b, s, cc), sl.raw + base + off,
+                       sl.out + base + off);
     }
 }
 
-template <typename R>
+template <typename R, int KIND>
 __global__ void __launch_bounds__(kBlock) learned_fwd_kernel(DevGraph g, Model<R> M, TrialPlanes tp,
                                                             Tape<R> tb, int batch, int *counter) {
     __shared__ int s_trial;
@@ -237,8 +304,8 @@ __global__ void __launch_bounds__(kBlock) learned_fwd_kernel(DevGraph g, Model<R
         const int b = s_trial;
         __syncthreads();
         if (b >= batch) return;
-        R fx[4], fz[4], lp[4];
-        priors(tb.prior + 4 * static_cast<size_t>(b), fx, fz, lp);
+        Chan<R> ch;
+        priors(tb.prior + 4 * static_cast<size_t>(b), ch);
         R *lam = tb.act + static_cast<size_t>(b) * (T + 1) * Et;
         for (size_t e = threadIdx.x; e < Et; e += kBlock) lam[e] = R(0);
         if (threadIdx.x == 0) {
@@ -257,7 +324,7 @@ __global__ void __launch_bounds__(kBlock) learned_fwd_kernel(DevGraph g, Model<R
         for (int t = 1; t <= T; ++t) {
             const R *prev = lam + static_cast<size_t>(t - 1) * Et;
             R *cur = lam + static_cast<size_t>(t) * Et;
-            var_pass(g, M, t, prev, fx, fz, sl);
+            var_pass<KIND>(g, M, t, prev, ch, sl);
             if (threadIdx.x == 0) {
                 R h[kMlpHidden];
                 R x0 = R(t) / R(T), x1 = s_ubar[t >= 2 ? t - 2 : 0];
@@ -268,7 +335,7 @@ __global__ void __launch_bounds__(kBlock) learned_fwd_kernel(DevGraph g, Model<R
                 s_cnt = 0;
             }
             __syncthreads();
-            check_pass(g, tp, b, sl);
+            check_pass<KIND>(g, tp, b, sl);
             __syncthreads();
             const R d = s_d;
             for (size_t e = threadIdx.x; e < Et; e += kBlock) {
@@ -280,10 +347,15 @@ __global__ void __launch_bounds__(kBlock) learned_fwd_kernel(DevGraph g, Model<R
                 R sx = R(0), sz = R(0);
                 for (int k = g.side[0].var_off[v]; k < g.side[0].var_off[v + 1]; ++k) sx += cur[g.side[0].var_edge[k]];
                 for (int k = g.side[1].var_off[v]; k < g.side[1].var_off[v + 1]; ++k) sz += cur[Ex + g.side[1].var_edge[k]];
-                const R lx = sx + fterm(sz, fx), lz = sz + fterm(sx, fz);
+                const R lx = sx + fterm_k<KIND>(sz, ch.fx, ch.lfx);
+                const R lz = sz + fterm_k<KIND>(sx, ch.fz, ch.lfz);
                 const int st = err_state(g, tp, b, v);
                 loss_local += softplus(R(2 * (st & 1) - 1) * lx) + softplus(R(2 * (st >> 1) - 1) * lz);
-                R l0 = lp[0] + sx + sz, l1 = lp[1] + sz, l2 = lp[2] + sx, l3 = lp[3];
+                if (KIND == 1) {  // per-bit decisions from the marginals' signs
+                    sl.est[v] = (lx < R(0) ? 1 : 0) | (lz < R(0) ? 2 : 0);
+                    continue;
+                }
+                R l0 = ch.lp[0] + sx + sz, l1 = ch.lp[1] + sz, l2 = ch.lp[2] + sx, l3 = ch.lp[3];
                 int best = 0;
                 R bv = l0;
                 if (l1 > bv) { best = 1; bv = l1; }
@@ -319,7 +391,7 @@ __global__ void __launch_bounds__(kBlock) learned_fwd_kernel(DevGraph g, Model<R
     }
 }
 
-template <typename R>
+template <typename R, int KIND>
 __global__ void __launch_bounds__(kBlock) learned_bwd_kernel(DevGraph g, Model<R> M, TrialPlanes tp,
                                                             Tape<R> tb, int batch, int *counter) {
     __shared__ int s_trial;
@@ -338,8 +410,8 @@ __global__ void __launch_bounds__(kBlock) learned_bwd_kernel(DevGraph g, Model<R
         const int b = s_trial;
         __syncthreads();
         if (b >= batch) return;
-        R fx[4], fz[4], lp[4];
-        priors(tb.prior + 4 * static_cast<size_t>(b), fx, fz, lp);
+        Chan<R> ch;
+        priors(tb.prior + 4 * static_cast<size_t>(b), ch);
         const R *lam = tb.act + static_cast<size_t>(b) * (T + 1) * Et;
         R *gc = sl.gc, *gp = sl.gp;
         for (size_t e = threadIdx.x; e < Et; e += kBlock) gc[e] = R(0);
@@ -353,20 +425,21 @@ __global__ void __launch_bounds__(kBlock) learned_bwd_kernel(DevGraph g, Model<R
                 R sx = R(0), sz = R(0);
                 for (int k = g.side[0].var_off[v]; k < g.side[0].var_off[v + 1]; ++k) sx += cur[g.side[0].var_edge[k]];
                 for (int k = g.side[1].var_off[v]; k < g.side[1].var_off[v + 1]; ++k) sz += cur[Ex + g.side[1].var_edge[k]];
-                const R lx = sx + fterm(sz, fx), lz = sz + fterm(sx, fz);
+                const R lx = sx + fterm_k<KIND>(sz, ch.fx, ch.lfx);
+                const R lz = sz + fterm_k<KIND>(sx, ch.fz, ch.lfz);
                 const int st = err_state(g, tp, b, v);
                 const R kx = R(2 * (st & 1) - 1), kz = R(2 * (st >> 1) - 1);
                 const R glx = scale * kx * sigm(kx * lx), glz = scale * kz * sigm(kz * lz);
-                const R gsx = glx + glz * fterm_d(sx, fz);
-                const R gsz = glz + glx * fterm_d(sz, fx);
+                const R gsx = glx + glz * fterm_kd<KIND>(sx, ch.fz, ch.lfz);
+                const R gsz = glz + glx * fterm_kd<KIND>(sz, ch.fx, ch.lfx);
                 for (int k = g.side[0].var_off[v]; k < g.side[0].var_off[v + 1]; ++k) gc[g.side[0].var_edge[k]] += gsx;
                 for (int k = g.side[1].var_off[v]; k < g.side[1].var_off[v + 1]; ++k) gc[Ex + g.side[1].var_edge[k]] += gsz;
             }
             // (2) recompute the variable pass, (3) the check pass
-            var_pass(g, M, t, prev, fx, fz, sl);
+            var_pass<KIND>(g, M, t, prev, ch, sl);
             if (threadIdx.x == 0) s_gd = R(0);
             __syncthreads();
-            check_pass(g, tp, b, sl);
+            check_pass<KIND>(g, tp, b, sl);
             __syncthreads();
             // (4) damping and check weights
             const R d = tb.tape_d[static_cast<size_t>(b) * T + t - 1];
@@ -399,7 +472,30 @@ __global__ void __launch_bounds__(kBlock) learned_bwd_kernel(DevGraph g, Model<R
                 }
             }
             // (5) check-node adjoint: gc (= dL/dlambda-hat) -> dL/dmu into out
-            for (int c = threadIdx.x; c < mx + mz; c += kBlock) {
+            for (int c = threadIdx.x; KIND == 1 && c < mx + mz; c += kBlock) {
+                const int s = c < mx ? 0 : 1;
+                const int cc = c < mx ? c : c - mx;
+                const int base = (s == 0 ? 0 : Ex) + g.side[s].chk_off[cc];
+                const int deg = g.side[s].chk_off[cc + 1] - g.side[s].chk_off[cc];
+                // arg-minima of |mu| (lowest index among equals)
+                int i1 = 0;
+                for (int i = 1; i < deg; ++i)
+                    if (fabs(sl.mu[base + i]) < fabs(sl.mu[base + i1])) i1 = i;
+                int i2 = i1 == 0 ? 1 : 0;
+                for (int i = 0; i < deg; ++i)
+                    if (i != i1 && fabs(sl.mu[base + i]) < fabs(sl.mu[base + i2])) i2 = i;
+                R to1 = R(0), to2 = R(0);
+                for (int i = 0; i < deg; ++i) {
+                    const R ge = gc[base + i];
+                    const R se = sl.raw[base + i] < R(0) ? -ge : ge;
+                    if (i == i1) to2 += se;
+                    else to1 += se;
+                }
+                for (int i = 0; i < deg; ++i) sl.out[base + i] = R(0);
+                sl.out[base + i1] = sl.mu[base + i1] < R(0) ? -to1 : to1;
+                if (deg > 1) sl.out[base + i2] = sl.mu[base + i2] < R(0) ? -to2 : to2;
+            }
+            for (int c = threadIdx.x; KIND == 0 && c < mx + mz; c += kBlock) {
                 const int s = c < mx ? 0 : 1;
                 const int cc = c < mx ? c : c - mx;
                 const int base = (s == 0 ? 0 : Ex) + g.side[s].chk_off[cc];
@@ -452,8 +548,8 @@ __global__ void __launch_bounds__(kBlock) learned_bwd_kernel(DevGraph g, Model<R
                     gsum[s] = acc;
                 }
                 const R sx = sl.S[v], sz = sl.S[n + v];
-                const R fdx = fterm_d(sz, fx);  // d a_x / d S_z
-                const R fdz = fterm_d(sx, fz);  // d a_z / d S_x
+                const R fdx = fterm_kd<KIND>(sz, ch.fx, ch.lfx);  // d a_x / d S_z
+                const R fdz = fterm_kd<KIND>(sx, ch.fz, ch.lfz);  // d a_z / d S_x
                 for (int k = g.side[0].var_off[v]; k < g.side[0].var_off[v + 1]; ++k)
                     gp[g.side[0].var_edge[k]] += gsum[0] + gsum[1] * fdz;
                 for (int k = g.side[1].var_off[v]; k < g.side[1].var_off[v + 1]; ++k)
@@ -522,6 +618,7 @@ void launch_learned_train(const DevGraph &g, const LearnedDevParams &lp, const R
                           TrialPlanes tp, int batch, R *act, R *scratch, int slots, R *tape_d,
                           R *tape_u, const double *prior, R *loss, R *grad, int *conv,
                           int *counters, bool backward, hipStream_t st) {
+    const bool ms = lp.kind == 1;
     Model<R> M = model_of<R>(lp, params);
     Tape<R> tb;
     tb.act = act;
@@ -537,10 +634,17 @@ void launch_learned_train(const DevGraph &g, const LearnedDevParams &lp, const R
     tb.conv = conv;
     const int grid = slots < batch ? slots : batch;
     launch_fill_i32(counters, 0, 2, st);
-    hipLaunchKernelGGL((learned_fwd_kernel<R>), dim3(grid), dim3(kBlock), 0, st, g, M, tp, tb,
-                       batch, counters);
-    if (backward)
-        hipLaunchKernelGGL((learned_bwd_kernel<R>), dim3(grid), dim3(kBlock), 0, st, g, M, tp, tb,
+    if (ms)
+        hipLaunchKernelGGL((learned_fwd_kernel<R, 1>), dim3(grid), dim3(kBlock), 0, st, g, M, tp, tb,
+                           batch, counters);
+    else
+        hipLaunchKernelGGL((learned_fwd_kernel<R, 0>), dim3(grid), dim3(kBlock), 0, st, g, M, tp, tb,
+                           batch, counters);
+    if (backward && ms)
+        hipLaunchKernelGGL((learned_bwd_kernel<R, 1>), dim3(grid), dim3(kBlock), 0, st, g, M, tp, tb,
+                           batch, counters + 1);
+    else if (backward)
+        hipLaunchKernelGGL((learned_bwd_kernel<R, 0>), dim3(grid), dim3(kBlock), 0, st, g, M, tp, tb,
                            batch, counters + 1);
 }
 

@@ -14,6 +14,8 @@ using namespace jbp;
 struct jbp_learned {
     jbp_ctx *ctx = nullptr;
     int T = 0, nb = 0, use_mlp = 1, count = 0;
+    int kind = kLearnedSP;
+    uint32_t *d_ms_rec = nullptr;      // min-sum decoder: FP16 weight records
     std::vector<double> params;        // host copy of the master parameters
     double *d_p64 = nullptr;           // device master (double)
     float *d_p32 = nullptr;            // device copy read by the FP32 training kernels
@@ -102,6 +104,7 @@ LearnedDevParams dev_params(const jbp_learned *m) {
     lp.nb = m->nb;
     lp.T = m->T;
     lp.use_mlp = m->use_mlp;
+    lp.kind = m->kind;
     lp.blk0 = m->d_blk[0];
     lp.blk1 = m->d_blk[1];
     return lp;
@@ -132,6 +135,15 @@ int run_train(jbp_learned *m, int batch, bool backward, char *err, size_t errlen
 extern "C" {
 
 int jbp_learned_create(jbp_ctx *ctx, int T, jbp_learned **out, char *err, size_t errlen) {
+    return jbp_learned_create_kind(ctx, T, JBP_LEARNED_SUM_PRODUCT, out, err, errlen);
+}
+
+int jbp_learned_create_kind(jbp_ctx *ctx, int T, int kind, jbp_learned **out, char *err,
+                            size_t errlen) {
+    if (kind != JBP_LEARNED_SUM_PRODUCT && kind != JBP_LEARNED_MIN_SUM) {
+        set_err(err, errlen, "unknown learned-decoder kind %d", kind);
+        return JBP_ERR_ARG;
+    }
     if (!ctx || !out || T < 1 || T > kMaxLearnedT) {
         set_err(err, errlen, "learned model needs a context and 1 <= T <= %d", kMaxLearnedT);
         return JBP_ERR_ARG;
@@ -150,6 +162,7 @@ int jbp_learned_create(jbp_ctx *ctx, int T, jbp_learned **out, char *err, size_t
     HIPCHK(hipSetDevice(ctx->device));
     auto *m = new jbp_learned();
     m->ctx = ctx;
+    m->kind = kind == JBP_LEARNED_MIN_SUM ? kLearnedMS : kLearnedSP;
     m->T = T;
     m->nb = nb;
     m->count = learned_param_count(nb, T);
@@ -173,6 +186,9 @@ int jbp_learned_create(jbp_ctx *ctx, int T, jbp_learned **out, char *err, size_t
     m->d_p32 = static_cast<float *>(p);
     if (hipMalloc(&p, m->count * sizeof(float)) != hipSuccess) return fail(JBP_ERR_NOMEM);
     m->d_infer = static_cast<float *>(p);
+    if (hipMalloc(&p, static_cast<size_t>(T + 1) * kMsRec * sizeof(uint32_t)) != hipSuccess)
+        return fail(JBP_ERR_NOMEM);
+    m->d_ms_rec = static_cast<uint32_t *>(p);
     if (hipMalloc(&p, m->count * sizeof(double)) != hipSuccess) return fail(JBP_ERR_NOMEM);
     m->d_m = static_cast<double *>(p);
     if (hipMalloc(&p, m->count * sizeof(double)) != hipSuccess) return fail(JBP_ERR_NOMEM);
@@ -203,13 +219,17 @@ void jbp_learned_destroy(jbp_learned *m) {
     free_bufs(m);
     for (void *p : {static_cast<void *>(m->d_p64), static_cast<void *>(m->d_p32),
                     static_cast<void *>(m->d_m), static_cast<void *>(m->d_v),
-                    static_cast<void *>(m->d_infer),
+                    static_cast<void *>(m->d_infer), static_cast<void *>(m->d_ms_rec),
                     static_cast<void *>(m->d_blk[0]), static_cast<void *>(m->d_blk[1])})
         if (p) (void)hipFree(p);
     delete m;
 }
 
 int jbp_learned_param_count(const jbp_learned *m) { return m ? m->count : -1; }
+
+int jbp_learned_kind(const jbp_learned *m) {
+    return !m ? -1 : m->kind == kLearnedMS ? JBP_LEARNED_MIN_SUM : JBP_LEARNED_SUM_PRODUCT;
+}
 
 int jbp_learned_set_params(jbp_learned *m, const double *params, int use_mlp, char *err,
                            size_t errlen) {
@@ -355,6 +375,36 @@ int jbp_simulate_learned(jbp_ctx *ctx, const jbp_learned *m, const jbp_sim_cfg *
     li.params = m->d_infer;
     li.T = m->T;
     li.use_mlp = m->use_mlp;
+    li.kind = m->kind;
+    if (m->kind == kLearnedMS) {
+        // variable pass k: (wc X, wc Z) of iteration k-1 per row block, then
+        // (wv X, wv Z) of iteration k, FP16 pairs (1 where there is none)
+        if (ctx->qcp.J != 3 || ctx->qcp.L != 12) {
+            set_err(err, errlen, "the min-sum decoder is built for 3 x 12 blocks");
+            return JBP_ERR_UNSUPPORTED;
+        }
+        const int J = ctx->qcp.J, L = ctx->qcp.L;
+        auto h16 = [](double v) -> uint32_t {
+            const _Float16 h = static_cast<_Float16>(static_cast<float>(v));
+            return __builtin_bit_cast(uint16_t, h);
+        };
+        auto pair = [&](double x, double z) { return h16(x) | (h16(z) << 16); };
+        std::vector<uint32_t> rec(static_cast<size_t>(T + 1) * kMsRec, 0u);
+        for (int k = 0; k <= T; ++k)
+            for (int cc = 0; cc < L; ++cc)
+                for (int r = 0; r < J; ++r) {
+                    const int b = r * L + cc;
+                    uint32_t *w = &rec[static_cast<size_t>(k) * kMsRec + cc * kMsRecCC];
+                    w[r] = k == 0 ? pair(1.0, 1.0)
+                                  : pair(m->params[(static_cast<size_t>(0) * nb + b) * T + k - 1],
+                                         m->params[(static_cast<size_t>(1) * nb + b) * T + k - 1]);
+                    w[3 + r] = k == T ? pair(1.0, 1.0)
+                                      : pair(m->params[(static_cast<size_t>(2) * nb + b) * T + k],
+                                             m->params[(static_cast<size_t>(3) * nb + b) * T + k]);
+                }
+        HIPCHK(hipMemcpy(m->d_ms_rec, rec.data(), rec.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        li.ms_rec = m->d_ms_rec;
+    }
     return simulate_impl(ctx, cfg, per_trial, stats, &li, err, errlen);
 }
 

@@ -17,15 +17,17 @@ from . import FP32, LearnedModel
 
 
 def train(decoder, T=25, steps=200, batch=4096, lr=1e-2, p_lo=0.03, p_hi=0.05, seed=2024,
-          first_trial=1, log_every=20, lr_decay=0.0, log=None, use_mlp=True):
+          first_trial=1, log_every=20, lr_decay=0.0, log=None, use_mlp=True, kind=0, init=None):
     """Train a LearnedModel on device-sampled trials (p uniform in
     [p_lo, p_hi] per trial, consecutive trial indices per step).  Returns
     (model, history) with one (step, loss, seconds) entry per logged step.
     use_mlp=False trains the message weights alone (damping 0), which the
     inference decoder runs without the damping state."""
-    model = LearnedModel(decoder, T)
-    if not use_mlp:
-        model.set_params(model.get_params(), use_mlp=False)
+    model = LearnedModel(decoder, T, kind=kind)
+    params = model.get_params()
+    if init is not None:
+        params = np.asarray(init, dtype=np.float64).copy()
+    model.set_params(params, use_mlp=use_mlp)
     history = []
     t0 = time.time()
     nxt = first_trial
@@ -42,7 +44,8 @@ def train(decoder, T=25, steps=200, batch=4096, lr=1e-2, p_lo=0.03, p_hi=0.05, s
 
 def save(model, path, meta=None):
     d = {"params": model.get_params(), "T": np.array(model.T), "nb": np.array(model.nb),
-         "use_mlp": np.array(int(getattr(model, "use_mlp", True)))}
+         "use_mlp": np.array(int(getattr(model, "use_mlp", True))),
+         "kind": np.array(int(getattr(model, "kind", 0)))}
     for k, v in (meta or {}).items():
         d["meta_" + k] = np.asarray(v)
     np.savez(path, **d)
@@ -54,7 +57,8 @@ def load(decoder, path, use_mlp=None):
         T = int(z["T"])
         if use_mlp is None:
             use_mlp = bool(int(z["use_mlp"])) if "use_mlp" in z.files else True
-        model = LearnedModel(decoder, T)
+        kind = int(z["kind"]) if "kind" in z.files else 0
+        model = LearnedModel(decoder, T, kind=kind)
         params = z["params"]
         if params.size != model.count:
             raise ValueError("%s holds %d parameters, the code needs %d" % (path, params.size,
@@ -94,13 +98,24 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--no-mlp", action="store_true", help="message weights only, no damping MLP")
+    ap.add_argument("--kind", choices=("sum-product", "min-sum"), default="min-sum",
+                    help="check / variable updates of the learned decoder")
+    ap.add_argument("--init-weight", type=float, default=1.0,
+                    help="starting value of every message weight")
     a = ap.parse_args(argv)
     dec = Decoder(Code.from_params(a.params), a.device)
+    kind = 1 if a.kind == "min-sum" else 0
+    probe = LearnedModel(dec, a.T, kind=kind)
+    init = probe.get_params()
+    probe.destroy()
+    init[:4 * probe.nb * a.T] = a.init_weight
     model, hist = train(dec, T=a.T, steps=a.steps, batch=a.batch, lr=a.lr, p_lo=a.p_lo,
                         p_hi=a.p_hi, seed=a.seed, lr_decay=a.lr_decay,
-                        log=lambda m: print(m, flush=True), use_mlp=not a.no_mlp)
+                        log=lambda m: print(m, flush=True), use_mlp=not a.no_mlp, kind=kind,
+                        init=init)
     save(model, a.out, meta={"steps": a.steps, "batch": a.batch, "lr": a.lr, "p_lo": a.p_lo,
-                             "p_hi": a.p_hi, "seed": a.seed, "loss": hist[-1][1]})
+                             "p_hi": a.p_hi, "seed": a.seed, "loss": hist[-1][1],
+                             "init_weight": a.init_weight})
     print("wrote %s (final loss %.6f, %.1f s)" % (a.out, hist[-1][1], hist[-1][2]), flush=True)
     model.destroy()
     dec.destroy()

@@ -21,6 +21,20 @@
 //
 // Check-node derivative: written as the O(deg^3) product over all other
 // inputs, independent of the O(deg) recursion the device adjoint uses.
+//
+// Second kind (kind = 1, normalised min-sum): the same model with the
+// max-product (min-sum) form of both node updates,
+//   check:  CN_e(mu) = (-1)^syn prod_{k != e} sign(mu_k) * min_{k != e} |mu_k|
+//   joint variable term:  F(L) = max(ln a + L, ln b) - max(ln c + L, ln d)
+//           in place of ln((a e^L + b) / (c e^L + d)),
+// hard decisions from the signs of the per-qubit X / Z marginals, and the
+// BCE loss on those marginals.  With unit weights and no damping it is
+// max-product BP, exact on a cycle-free graph (pinned against brute-force
+// maximisation over all error patterns on small tree codes).  The loss is
+// piecewise smooth in the parameters; orl_loss2 also returns a hash of every
+// discrete choice the forward pass made (arg-minima, signs, clamps, max
+// branches, estimates), so a finite-difference test can confirm that both
+// of its points lie on the same smooth piece.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -59,7 +73,66 @@ struct Params {
     const double *wv;   // [2][nb][T]
     const double *mlp;  // W1[16][2] | b1[16] | W2[16] | b2
     bool use_mlp;
+    int kind;           // 0 sum-product, 1 min-sum
 };
+
+// Hash of the discrete choices of a min-sum forward pass (FNV-1a).
+struct Sig {
+    uint64_t h = 1469598103934665603ull;
+    void add(uint64_t v) {
+        h ^= v;
+        h *= 1099511628211ull;
+    }
+};
+
+// Max-log joint-variable term F(L) = max(ln k0 + L, ln k1) - max(ln k2 + L, ln k3)
+// and its derivative (0, +-1) away from the branch points.
+inline double fterm_ms(double L, const double k[4], Sig *sig = nullptr) {
+    const double a = std::log(k[0]) + L, b = std::log(k[1]);
+    const double c = std::log(k[2]) + L, d = std::log(k[3]);
+    if (sig) sig->add((a > b ? 1u : 0u) | (c > d ? 2u : 0u));
+    return std::max(a, b) - std::max(c, d);
+}
+inline double fterm_ms_d(double L, const double k[4]) {
+    return (std::log(k[0]) + L > std::log(k[1]) ? 1.0 : 0.0) -
+           (std::log(k[2]) + L > std::log(k[3]) ? 1.0 : 0.0);
+}
+inline double fterm_k(int kind, double L, const double k[4], Sig *sig = nullptr) {
+    return kind == 1 ? fterm_ms(L, k, sig) : fterm(L, k);
+}
+inline double fterm_kd(int kind, double L, const double k[4]) {
+    return kind == 1 ? fterm_ms_d(L, k) : fterm_d(L, k);
+}
+
+// Min-sum check update, written from its definition: for each edge the
+// minimum and the sign product over the other edges.
+void check_node_ms(const double *mu, int deg, int syn, double *raw, double *out, Sig *sig) {
+    uint64_t negmask = 0;
+    for (int i = 0; i < deg; ++i) negmask |= static_cast<uint64_t>(mu[i] < 0 ? 1 : 0) << i;
+    for (int i = 0; i < deg; ++i) {
+        double mag = HUGE_VAL;
+        int neg = syn & 1;
+        for (int j = 0; j < deg; ++j) {
+            if (j == i) continue;
+            mag = std::min(mag, std::fabs(mu[j]));
+            neg ^= mu[j] < 0 ? 1 : 0;
+        }
+        raw[i] = neg ? -mag : mag;
+        out[i] = neg ? -std::min(mag, kLlrMax) : std::min(mag, kLlrMax);
+    }
+    if (sig) {
+        int i1 = 0;
+        for (int j = 1; j < deg; ++j)
+            if (std::fabs(mu[j]) < std::fabs(mu[i1])) i1 = j;
+        int i2 = i1 == 0 ? 1 : 0;
+        for (int j = 0; j < deg; ++j)
+            if (j != i1 && std::fabs(mu[j]) < std::fabs(mu[i2])) i2 = j;
+        sig->add(negmask);
+        sig->add(static_cast<uint64_t>(i1) | static_cast<uint64_t>(i2) << 8 |
+                 static_cast<uint64_t>(std::fabs(mu[i1]) > kLlrMax) << 16 |
+                 static_cast<uint64_t>(std::fabs(mu[i2]) > kLlrMax) << 17);
+    }
+}
 
 double mlp_forward(const double *mlp, double x0, double x1, double *h) {
     double z = mlp[kHidden * 2 + kHidden + kHidden];
@@ -71,6 +144,7 @@ double mlp_forward(const double *mlp, double x0, double x1, double *h) {
 }
 
 struct TrialTape {
+    std::vector<double> marg;                  // last iteration's X then Z marginal LLRs (2n)
     std::vector<std::vector<double>> lam[2];   // [t][e], t = 0..T
     std::vector<double> d, x0, x1;             // per iteration 1..T (index t-1)
     std::vector<int> est;                      // last estimate
@@ -111,7 +185,8 @@ void check_node(const double *mu, int deg, int syn, double *raw, double *out) {
 // One trial forward; fills the tape; returns the trial's loss sum (not yet
 // averaged).  err: states 0..3; prior: 4-ary normalized.
 double trial_forward(const Graph &g, const Params &P, int T, const int *err, const double *prior,
-                     TrialTape &tape) {
+                     TrialTape &tape, Sig *sig = nullptr) {
+    const int kind = P.kind;
     const double fx[4] = {prior[0], prior[2], prior[1], prior[3]};
     const double fz[4] = {prior[0], prior[1], prior[2], prior[3]};
     const double lp[4] = {std::log(prior[0]), std::log(prior[1]), std::log(prior[2]), std::log(prior[3])};
@@ -140,6 +215,7 @@ double trial_forward(const Graph &g, const Params &P, int T, const int *err, con
     std::vector<int> est(g.n, 0);
     double loss = 0.0;
     tape.converged_at = 0;
+    tape.marg.assign(2 * static_cast<size_t>(g.n), 0.0);
     for (int t = 1; t <= T; ++t) {
         const std::vector<double> *lp_prev[2] = {&tape.lam[0][t - 1], &tape.lam[1][t - 1]};
         for (int s = 0; s < 2; ++s) {
@@ -150,7 +226,7 @@ double trial_forward(const Graph &g, const Params &P, int T, const int *err, con
         for (int s = 0; s < 2; ++s) {
             mu[s].assign(g.edge_var[s].size(), 0.0);
             for (int v = 0; v < g.n; ++v) {
-                double f = s == 0 ? fterm(S[1][v], fx) : fterm(S[0][v], fz);
+                double f = s == 0 ? fterm_k(kind, S[1][v], fx, sig) : fterm_k(kind, S[0][v], fz, sig);
                 for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) {
                     int e = g.var_edge[s][k];
                     double a = S[s][v] - (*lp_prev[s])[e] + f;
@@ -170,7 +246,8 @@ double trial_forward(const Graph &g, const Params &P, int T, const int *err, con
                 int off = g.chk_off[s][c], deg = g.chk_off[s][c + 1] - off;
                 raw.assign(deg, 0.0);
                 out.assign(deg, 0.0);
-                check_node(&mu[s][off], deg, syn[s][c], raw.data(), out.data());
+                if (kind == 1) check_node_ms(&mu[s][off], deg, syn[s][c], raw.data(), out.data(), sig);
+                else check_node(&mu[s][off], deg, syn[s][c], raw.data(), out.data());
                 for (int i = 0; i < deg; ++i) {
                     int e = off + i;
                     double w = P.wc[(s * g.nb + g.blk[s][e]) * T + (t - 1)];
@@ -185,10 +262,17 @@ double trial_forward(const Graph &g, const Params &P, int T, const int *err, con
                 for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) S[s][v] += tape.lam[s][t][g.var_edge[s][k]];
         }
         for (int v = 0; v < g.n; ++v) {
-            double lx = S[0][v] + fterm(S[1][v], fx);
-            double lz = S[1][v] + fterm(S[0][v], fz);
+            double lx = S[0][v] + fterm_k(kind, S[1][v], fx, sig);
+            double lz = S[1][v] + fterm_k(kind, S[0][v], fz, sig);
             int yx = err[v] & 1, yz = err[v] >> 1;
             loss += softplus((2 * yx - 1) * lx) + softplus((2 * yz - 1) * lz);
+            tape.marg[v] = lx;
+            tape.marg[g.n + v] = lz;
+            if (kind == 1) {  // per-bit decisions from the marginals' signs
+                est[v] = (lx < 0 ? 1 : 0) | (lz < 0 ? 2 : 0);
+                if (sig) sig->add(static_cast<uint64_t>(est[v]));
+                continue;
+            }
             double l0 = lp[0] + S[0][v] + S[1][v], l1 = lp[1] + S[1][v], l2 = lp[2] + S[0][v], l3 = lp[3];
             int best = 0;
             double bv = l0;
@@ -217,6 +301,7 @@ double trial_forward(const Graph &g, const Params &P, int T, const int *err, con
 // Adjoint of trial_forward.  scale multiplies the trial's loss sum.
 void trial_backward(const Graph &g, const Params &P, int T, const int *err, const double *prior,
                     const TrialTape &tape, double scale, double *gwc, double *gwv, double *gmlp) {
+    const int kind = P.kind;
     const double fx[4] = {prior[0], prior[2], prior[1], prior[3]};
     const double fz[4] = {prior[0], prior[1], prior[2], prior[3]};
     std::vector<int> syn[2];
@@ -242,13 +327,13 @@ void trial_backward(const Graph &g, const Params &P, int T, const int *err, cons
                 for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) S[s][v] += lam_t[s][t][g.var_edge[s][k]];
         }
         for (int v = 0; v < g.n; ++v) {
-            double lx = S[0][v] + fterm(S[1][v], fx);
-            double lz = S[1][v] + fterm(S[0][v], fz);
+            double lx = S[0][v] + fterm_k(kind, S[1][v], fx);
+            double lz = S[1][v] + fterm_k(kind, S[0][v], fz);
             double kx = 2 * (err[v] & 1) - 1, kz = 2 * (err[v] >> 1) - 1;
             double glx = scale * kx * sigmoid(kx * lx);
             double glz = scale * kz * sigmoid(kz * lz);
-            double gsx = glx + glz * fterm_d(S[0][v], fz);
-            double gsz = glz + glx * fterm_d(S[1][v], fx);
+            double gsx = glx + glz * fterm_kd(kind, S[0][v], fz);
+            double gsz = glz + glx * fterm_kd(kind, S[1][v], fx);
             for (int k = g.var_off[0][v]; k < g.var_off[0][v + 1]; ++k) G[0][g.var_edge[0][k]] += gsx;
             for (int k = g.var_off[1][v]; k < g.var_off[1][v + 1]; ++k) G[1][g.var_edge[1][k]] += gsz;
         }
@@ -262,7 +347,7 @@ void trial_backward(const Graph &g, const Params &P, int T, const int *err, cons
             mu[s].assign(g.edge_var[s].size(), 0.0);
             a[s].assign(g.edge_var[s].size(), 0.0);
             for (int v = 0; v < g.n; ++v) {
-                double f = s == 0 ? fterm(Sp[1][v], fx) : fterm(Sp[0][v], fz);
+                double f = s == 0 ? fterm_k(kind, Sp[1][v], fx) : fterm_k(kind, Sp[0][v], fz);
                 for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) {
                     int e = g.var_edge[s][k];
                     a[s][e] = Sp[s][v] - lam_t[s][t - 1][e] + f;
@@ -273,7 +358,8 @@ void trial_backward(const Graph &g, const Params &P, int T, const int *err, cons
             out[s].assign(g.edge_var[s].size(), 0.0);
             for (int c = 0; c < g.m[s]; ++c) {
                 int off = g.chk_off[s][c], deg = g.chk_off[s][c + 1] - off;
-                check_node(&mu[s][off], deg, syn[s][c], &raw[s][off], &out[s][off]);
+                if (kind == 1) check_node_ms(&mu[s][off], deg, syn[s][c], &raw[s][off], &out[s][off], nullptr);
+                else check_node(&mu[s][off], deg, syn[s][c], &raw[s][off], &out[s][off]);
             }
         }
         const double d = tape.d[t - 1];
@@ -305,9 +391,24 @@ void trial_backward(const Graph &g, const Params &P, int T, const int *err, cons
             }
         }
         // check-node adjoint: d lambda_e / d mu_j = sign_c * P_ej sech^2(mu_j/2) / sech^2(raw_e/2)
+        // (min-sum: d raw_e / d mu_j = sign(raw_e) sign(mu_j) for the j that
+        // attains the minimum over the other edges, 0 for the rest)
         for (int s = 0; s < 2; ++s) {
             Gmu[s].assign(g.edge_var[s].size(), 0.0);
-            for (int c = 0; c < g.m[s]; ++c) {
+            for (int c = 0; kind == 1 && c < g.m[s]; ++c) {
+                int off = g.chk_off[s][c], deg = g.chk_off[s][c + 1] - off;
+                for (int e = 0; e < deg; ++e) {
+                    double ge = G[s][off + e];
+                    if (ge == 0.0) continue;
+                    int js = -1;
+                    for (int j = 0; j < deg; ++j)
+                        if (j != e && (js < 0 || std::fabs(mu[s][off + j]) < std::fabs(mu[s][off + js]))) js = j;
+                    double se = raw[s][off + e] < 0 ? -1.0 : 1.0;
+                    double sj = mu[s][off + js] < 0 ? -1.0 : 1.0;
+                    Gmu[s][off + js] += ge * se * sj;
+                }
+            }
+            for (int c = 0; kind == 0 && c < g.m[s]; ++c) {
                 int off = g.chk_off[s][c], deg = g.chk_off[s][c + 1] - off;
                 double sgn = syn[s][c] ? -1.0 : 1.0;
                 for (int e = 0; e < deg; ++e) {
@@ -340,7 +441,7 @@ void trial_backward(const Graph &g, const Params &P, int T, const int *err, cons
                 }
                 for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) Gp[s][g.var_edge[s][k]] += ga_sum;
                 // F_s(S_o) term: d a / d S_o = F_s'(S_o)
-                double fd = s == 0 ? fterm_d(Sp[1][v], fx) : fterm_d(Sp[0][v], fz);
+                double fd = s == 0 ? fterm_kd(kind, Sp[1][v], fx) : fterm_kd(kind, Sp[0][v], fz);
                 for (int k = g.var_off[o][v]; k < g.var_off[o][v + 1]; ++k) Gp[o][g.var_edge[o][k]] += ga_sum * fd;
             }
         }
@@ -389,13 +490,19 @@ int orl_mlp_params() { return kMlpParams; }
 // when grad_* are non-null, its gradient.  Per trial, conv_out (may be
 // null) receives the first iteration whose estimate satisfies both
 // syndromes (0 = none) and est_out (may be null) the final estimate.
-double orl_loss(int n, int mx, int mz, const int *xoff, const int *xvar, const int *zoff,
-                const int *zvar, const int *xblk, const int *zblk, int nb, int T, const double *wc,
-                const double *wv, const double *mlp, int use_mlp, int batch, const int *err,
-                const double *p, double *grad_wc, double *grad_wv, double *grad_mlp, int *conv_out,
-                int *est_out) {
+// kind 0: sum-product, 1: min-sum (header).  signature (may be null): hash
+// of the min-sum forward pass's discrete choices over the whole batch.
+// marg_out (may be null): per trial the last iteration's per-qubit X then Z
+// marginal log-likelihood ratios (batch x 2n).
+double orl_loss2(int kind, int n, int mx, int mz, const int *xoff, const int *xvar,
+                 const int *zoff, const int *zvar, const int *xblk, const int *zblk, int nb, int T,
+                 const double *wc, const double *wv, const double *mlp, int use_mlp, int batch,
+                 const int *err, const double *p, double *grad_wc, double *grad_wv,
+                 double *grad_mlp, int *conv_out, int *est_out, uint64_t *signature,
+                 double *marg_out) {
     Graph g = make_graph(n, mx, mz, xoff, xvar, zoff, zvar, xblk, zblk, nb);
-    Params P{wc, wv, mlp, use_mlp != 0};
+    Params P{wc, wv, mlp, use_mlp != 0, kind == 1 ? 1 : 0};
+    Sig sig;
     const double scale = 1.0 / (static_cast<double>(batch) * T * 2.0 * n);
     if (grad_wc) std::memset(grad_wc, 0, sizeof(double) * 2 * nb * T);
     if (grad_wv) std::memset(grad_wv, 0, sizeof(double) * 2 * nb * T);
@@ -406,15 +513,27 @@ double orl_loss(int n, int mx, int mz, const int *xoff, const int *xvar, const i
         double prior[4];
         prior_of(p[b], prior);
         TrialTape tape;
-        total += trial_forward(g, P, T, err + static_cast<size_t>(b) * n, prior, tape);
+        total += trial_forward(g, P, T, err + static_cast<size_t>(b) * n, prior, tape,
+                               signature ? &sig : nullptr);
         if (conv_out) conv_out[b] = tape.converged_at;
         if (est_out) std::copy(tape.est.begin(), tape.est.end(), est_out + static_cast<size_t>(b) * n);
+        if (marg_out) std::copy(tape.marg.begin(), tape.marg.end(), marg_out + static_cast<size_t>(b) * 2 * n);
         if (grad_wc || grad_wv || grad_mlp)
             trial_backward(g, P, T, err + static_cast<size_t>(b) * n, prior, tape, scale,
                            grad_wc ? grad_wc : dummy_wc.data(), grad_wv ? grad_wv : dummy_wv.data(),
                            grad_mlp ? grad_mlp : dummy_mlp.data());
     }
+    if (signature) *signature = sig.h;
     return total * scale;
+}
+
+double orl_loss(int n, int mx, int mz, const int *xoff, const int *xvar, const int *zoff,
+                const int *zvar, const int *xblk, const int *zblk, int nb, int T, const double *wc,
+                const double *wv, const double *mlp, int use_mlp, int batch, const int *err,
+                const double *p, double *grad_wc, double *grad_wv, double *grad_mlp, int *conv_out,
+                int *est_out) {
+    return orl_loss2(0, n, mx, mz, xoff, xvar, zoff, zvar, xblk, zblk, nb, T, wc, wv, mlp, use_mlp,
+                     batch, err, p, grad_wc, grad_wv, grad_mlp, conv_out, est_out, nullptr, nullptr);
 }
 
 }  // extern "C"

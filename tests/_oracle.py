@@ -138,18 +138,28 @@ class Oracle:
 
 
 class LearnedOracle:
-    """FP64 learned-BP forward / loss / adjoint (oracle/learned_oracle.cpp)."""
+    """FP64 learned-BP forward / loss / adjoint (oracle/learned_oracle.cpp).
+    kind 0: sum-product, 1: min-sum.  Weight blocks: edge at position i of
+    check r*P + x is block r*L + i, or explicit per-edge blocks (blk, nb)."""
 
-    def __init__(self, code, P=768, L=12, path=ORACLE_SO):
+    def __init__(self, code, P=768, L=12, path=ORACLE_SO, blk=None, nb=None):
         L_ = C.CDLL(path)
         vp = C.c_void_p
         L_.orl_loss.restype = C.c_double
         L_.orl_loss.argtypes = [C.c_int] * 3 + [vp] * 6 + [C.c_int, C.c_int, vp, vp, vp, C.c_int,
                                                            C.c_int, vp, vp, vp, vp, vp, vp, vp]
+        L_.orl_loss2.restype = C.c_double
+        L_.orl_loss2.argtypes = [C.c_int] * 4 + [vp] * 6 + [C.c_int, C.c_int, vp, vp, vp, C.c_int,
+                                                            C.c_int, vp, vp, vp, vp, vp, vp, vp,
+                                                            vp, vp]
         self.lib = L_
         self.code = code
         self.xoff, self.xvar = code.csr(0)
         self.zoff, self.zvar = code.csr(1)
+        if blk is not None:
+            self.blk = [np.ascontiguousarray(b, dtype=np.int32) for b in blk]
+            self.nb = int(nb)
+            return
         self.nb = (code.mx // P) * L
         self.blk = []
         for off in (self.xoff, self.zoff):
@@ -159,7 +169,9 @@ class LearnedOracle:
                 b[off[c]:off[c + 1]] = r * L + np.arange(off[c + 1] - off[c])
             self.blk.append(b)
 
-    def loss(self, T, wc, wv, mlp, use_mlp, err, p, grad=False):
+    def loss(self, T, wc, wv, mlp, use_mlp, err, p, grad=False, kind=0, extra=False):
+        """(loss, [grads], conv, est) and, with extra=True, a dict with the
+        forward pass's signature hash and the last marginals (min-sum)."""
         err = np.ascontiguousarray(err, dtype=np.int32)
         p = np.ascontiguousarray(np.broadcast_to(p, (err.shape[0],)), dtype=np.float64)
         wc = np.ascontiguousarray(wc, dtype=np.float64)
@@ -168,9 +180,15 @@ class LearnedOracle:
         g = [np.zeros_like(wc), np.zeros_like(wv), np.zeros_like(mlp)] if grad else [None] * 3
         conv = np.zeros(err.shape[0], np.int32)
         est = np.zeros_like(err)
+        sig = np.zeros(1, np.uint64)
+        marg = np.zeros((err.shape[0], 2 * err.shape[1]), np.float64)
         c = self.code
-        val = self.lib.orl_loss(c.n, c.mx, c.mz, _p(self.xoff), _p(self.xvar), _p(self.zoff),
-                                _p(self.zvar), _p(self.blk[0]), _p(self.blk[1]), self.nb, T, _p(wc),
-                                _p(wv), _p(mlp), int(use_mlp), err.shape[0], _p(err), _p(p),
-                                _p(g[0]), _p(g[1]), _p(g[2]), _p(conv), _p(est))
-        return (val, g, conv, est) if grad else (val, conv, est)
+        val = self.lib.orl_loss2(int(kind), c.n, c.mx, c.mz, _p(self.xoff), _p(self.xvar),
+                                 _p(self.zoff), _p(self.zvar), _p(self.blk[0]), _p(self.blk[1]),
+                                 self.nb, T, _p(wc), _p(wv), _p(mlp), int(use_mlp), err.shape[0],
+                                 _p(err), _p(p), _p(g[0]), _p(g[1]), _p(g[2]), _p(conv), _p(est),
+                                 _p(sig), _p(marg))
+        out = (val, g, conv, est) if grad else (val, conv, est)
+        if extra:
+            out = out + ({"signature": int(sig[0]), "marginals": marg},)
+        return out

@@ -58,3 +58,117 @@ def test_unit_weights_without_damping_are_plain_bp(lo, main_code_oracle):
     _, conv, est = lo.loss(T, ones, ones, np.zeros(65), 0, errs, 0.05)
     want = np.where(z["syn_ok"][:N] == 1, z["iterations"][:N], 0)
     assert np.array_equal(conv, want)
+
+
+# ---------------------------------------------------------------------------
+# Min-sum kind (kind = 1): max-product BP with learned weights.
+
+def tree_code(oracle):
+    """A cycle-free joint factor graph: X checks {0,1,2}, {2,3}; Z checks
+    {1,4,5}, {3,6,7} (qubits 1, 2, 3 join the checks into one tree)."""
+    xs = np.array([[0, 1, 2], [2, 3, -1]], np.int32)
+    zs = np.array([[1, 4, 5], [3, 6, 7]], np.int32)
+    return oracle.code_from_checks(8, xs, zs), xs, zs
+
+
+def brute_force_max_marginals(n, xs, zs, sx, sz, prior):
+    """Per qubit: max log P(e) over errors with the syndromes and x_v = 0,
+    minus the same with x_v = 1 (and for z_v); states 0 I, 1 X, 2 Z, 3 Y."""
+    lp = np.log(prior)
+    states = np.array(np.meshgrid(*[np.arange(4)] * n, indexing="ij")).reshape(n, -1).T
+    xb, zb = states & 1, states >> 1
+    ok = np.ones(len(states), bool)
+    for c, row in enumerate(xs):
+        q = row[row >= 0]
+        ok &= (xb[:, q].sum(1) & 1) == sx[c]
+    for c, row in enumerate(zs):
+        q = row[row >= 0]
+        ok &= (zb[:, q].sum(1) & 1) == sz[c]
+    w = lp[states[ok]].sum(1)
+    out = np.zeros(2 * n)
+    for v in range(n):
+        for side, bits in ((0, xb[ok, v]), (1, zb[ok, v])):
+            out[side * n + v] = w[bits == 0].max() - w[bits == 1].max()
+    return out
+
+
+def test_minsum_unit_weights_are_exact_max_product_on_a_tree(oracle):
+    # max-product BP is exact on a cycle-free graph: after enough iterations
+    # the min-sum marginals equal the brute-force max-marginal log ratios
+    code, xs, zs = tree_code(oracle)
+    offs = [code.csr(s)[0] for s in (0, 1)]
+    blk = [np.zeros(off[-1], np.int32) for off in offs]
+    lo = LearnedOracle_(code, blk=blk, nb=1)
+    T = 8
+    ones = np.ones((2, 1, T))
+    rng = np.random.default_rng(5)
+    for trial in range(12):
+        err = rng.integers(0, 4, size=8).astype(np.int32)
+        err[rng.random(8) < 0.5] = 0
+        p = [0.05, 0.12, 0.2][trial % 3]
+        sx, sz = code.syndrome(err)
+        _, conv, est, ex = lo.loss(T, ones, ones, np.zeros(65), 0, err[None], p, kind=1, extra=True)
+        want = brute_force_max_marginals(8, xs, zs, sx, sz, oracle.prior(p))
+        assert np.allclose(ex["marginals"][0], want, rtol=0, atol=1e-9), (trial, err)
+
+
+def LearnedOracle_(code, **kw):
+    from _oracle import LearnedOracle
+    return LearnedOracle(code, **kw)
+
+
+def test_minsum_adjoint_matches_central_differences_on_smooth_pieces(oracle):
+    # The min-sum loss is piecewise smooth.  Central differences (eps = 1e-3)
+    # are compared where the forward pass makes the same discrete choices at
+    # theta - eps, theta and theta + eps (the oracle's signature hash), which
+    # must happen for at least 4 parameters of each group (check weights,
+    # variable weights, MLP).  The code is a small hypergraph product, so
+    # that such parameters are common.
+    import joint_bp_plus_pp_amd as jbp
+    c = jbp.Code.hgp(15, 20, 3, 4, seed=2)
+    oc = oracle.code_from_checks(c.n, c.checks(0), c.checks(1))
+    blk = []
+    for s in (0, 1):
+        off = oc.csr(s)[0]
+        b = np.zeros(off[-1], np.int32)
+        for k in range(len(off) - 1):
+            b[off[k]:off[k + 1]] = np.arange(off[k + 1] - off[k])
+        blk.append(b)
+    nb = int(max(b.max() for b in blk)) + 1
+    lo = LearnedOracle_(oc, blk=blk, nb=nb)
+    T = 5
+    rng = np.random.default_rng(1)
+    wc = 0.8 + 0.1 * rng.standard_normal((2, nb, T))
+    wv = 1 + 0.1 * rng.standard_normal((2, nb, T))
+    mlp = 0.3 * rng.standard_normal(65)
+    mlp[64] = -1.0
+    errs = np.stack([oc.sample(7, t, 0.05) for t in (1, 2, 3)])
+    p = np.array([0.05, 0.04, 0.06])
+    _, (gwc, gwv, gm), _, _, ex = lo.loss(T, wc, wv, mlp, 1, errs, p, grad=True, kind=1, extra=True)
+    eps = 1e-3
+    found = {"wc": 0, "wv": 0, "mlp": 0}
+    worst = 0.0
+    for attempt in range(300):
+        name = ("wc", "wv", "mlp")[attempt % 3]
+        if found[name] >= 4:
+            continue
+        arr = {"wc": wc, "wv": wv, "mlp": mlp}[name]
+        g = {"wc": gwc, "wv": gwv, "mlp": gm}[name]
+        ix = tuple(int(rng.integers(0, s)) for s in arr.shape)
+        old = arr[ix]
+        arr[ix] = old + eps
+        lp, _, _, ep = lo.loss(T, wc, wv, mlp, 1, errs, p, kind=1, extra=True)
+        arr[ix] = old - eps
+        lm, _, _, em = lo.loss(T, wc, wv, mlp, 1, errs, p, kind=1, extra=True)
+        arr[ix] = old
+        if not (ep["signature"] == em["signature"] == ex["signature"]):
+            continue
+        fd = (lp - lm) / (2 * eps)
+        if abs(fd) < 1e-12 and abs(g[ix]) < 1e-12:
+            continue  # parameters that do not reach the loss
+        rel = abs(g[ix] - fd) / abs(fd)
+        worst = max(worst, rel)
+        found[name] += 1
+        assert rel < 1e-3, (name, ix, g[ix], fd)
+    print("smooth pieces checked", found, "worst relative error %.2e" % worst)
+    assert all(v >= 4 for v in found.values()), found
