@@ -1,24 +1,31 @@
 #!/usr/bin/env python3
 """Throughput of the --simulate --no-pp path on one MI355X.
 
-Workload (the batched GPU configuration): the included girth-8 (3,12)-regular
-[[9216,4612]] code, i.i.d. depolarizing noise at p = 0.04, 8192 seeded trials
-per step = one persistent decoder launch (each compute unit pulls the next
-trial when its current one stops), max-iter 1000 with per-trial early
-termination, FP32 messages.  A step samples its trials on the device
-(seed = splitmix64(base ^ idx*phi), mt19937_64), computes syndromes,
-decodes, and runs the stabilizer-residual test; consecutive steps take
-consecutive trial indices.  config.pipelined reports calls of 65,536 trials
-(eight launches alternating two streams, the next launch's trials sampled
-while the current one decodes); their launches overlap, so the roofline is
-taken from the one-launch steps.
+Headline workload: the included girth-8 (3,12)-regular [[9216,4612]] code,
+i.i.d. depolarizing noise at p = 0.04, max-iter 1000, FP32.  A step is one
+call over --trials (65,536) seeded trials: each is sampled on the device
+(seed = splitmix64(base ^ idx*phi), mt19937_64), its syndromes formed, then
+decoded by the learned min-sum decoder (T = 25 learned iterations; shipped
+weights trained by this project's own kernels), with every trial it leaves
+unconverged decoded again by plain FP32 BP from the prior up to max-iter
+1000, and judged by the stabilizer-residual test.  Launches of --chunk
+(16,384) trials alternate two streams.  The same trial indices are then run
+by plain batched FP32 BP: `value` is the learned decoder's trials/s, with
+plain BP's trials/s, both FERs and their Wilson intervals ("matched FER":
+the learned FER no higher than the upper end of plain BP's 95% interval;
+"fer_below_plain_ci" when it is under the interval's lower end) beside it.
 
-Prints one JSON line (last line of stdout).  `value` is decoded trials/s over
-the timed steps.  `roofline` is for the decoder kernel: algorithmic bytes
-(16 B per edge per trial-iteration, 884,736 B per trial-iteration) over its
-HIP-event duration.  `cpu_baseline` times the reference decoder (built from
-its own source under oracle/_ref/) on a bounded sample of the same workload
-on this host's cores.
+Prints one JSON line (last line of stdout).  `roofline` is for the learned
+decoder's kernel (decode_qc_ms_kernel): it keeps every message on chip, so
+HBM does not bind it; the vector ALU does.  Its per-trial-iteration issue
+slots (wave64 VALU instructions at their issue cost) come from the rocprofv3
+counter profile committed for this kernel source (profiles/, checked by the
+source's SHA-256: a stale profile is not used); `achieved` is that count
+times the kernel's trial-iterations per second, measured here with HIP
+events over single-launch steps; `peak` is the MI355X issue rate (256 CUs x
+4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction).  `cpu_baseline` times
+the reference decoder (built from its own source under oracle/_ref/) on a
+bounded sample of the same workload on this host's cores.
 """
 import argparse
 import ctypes as C
@@ -28,6 +35,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 PARAMS = os.path.join(ROOT, "joint_bp_plus_pp_amd", "codes",
@@ -36,9 +45,14 @@ METRIC = "decoded trials/s at p=0.04 at matched FER (learned vs plain BP), % HBM
 EDGES_PER_ITER = 2 * 27648
 BYTES_PER_ITER = 16 * EDGES_PER_ITER  # 884,736
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "decode_qc_f32_p0.04_traffic.json")
-LEARNED_FILES = [os.path.join(ROOT, "joint_bp_plus_pp_amd", "codes", f)
-                 for f in ("learned_T25_P768_J3_L12.npz", "learned_T25_nomlp_P768_J3_L12.npz")]
+LEARNED_MS_FILE = os.path.join(ROOT, "joint_bp_plus_pp_amd", "codes", "learned_ms_T25_P768_J3_L12.npz")
+LEARNED_SP_FILES = [os.path.join(ROOT, "joint_bp_plus_pp_amd", "codes", f)
+                    for f in ("learned_T25_P768_J3_L12.npz", "learned_T25_nomlp_P768_J3_L12.npz")]
+KERNEL_SOURCE = os.path.join(ROOT, "joint_bp_plus_pp_amd", "csrc", "jbp_decode_f32.hip")
+VALU_PROFILE = os.path.join(ROOT, "profiles", "decode_qc_f32_kernels_p0.04_counters.json")
+# MI355X: 256 compute units x 4 SIMDs, 2.4 GHz peak engine clock, one wave64
+# VALU instruction per SIMD every 4 cycles
+VALU_PEAK_GSLOTS = 256 * 4 * 2.4 / 4.0
 MIXED_PS = (0.01, 0.025, 0.04, 0.05)
 
 
@@ -183,40 +197,50 @@ def pp_workload(jbp, dec, args, first):
     return out
 
 
-def learned_vs_plain(jbp, dec, args, first, plain):
-    """Each shipped learned model (T learned iterations, then plain BP
-    restarted from the prior) on the first timed trial indices, beside plain
-    BP on the same indices."""
+def sp_learned_leg(jbp, dec, args, first):
+    """The sum-product learned models (weights on the reference's joint BP),
+    for comparison: trials/s and FER over --sp-learned-steps steps."""
     from joint_bp_plus_pp_amd import learned as jl
-    p_done, p_fails, p_el = plain
-    plo, phi = wilson(p_done, p_fails)
-    out = {"plain_same_trials": {"trials": p_done, "trials_per_s": round(p_done / p_el, 2),
-                                 "fer": p_fails / p_done, "fer_ci95": [plo, phi]}}
-    for path in LEARNED_FILES:
-        if not os.path.exists(path):
+    out = {}
+    for path in LEARNED_SP_FILES:
+        if not os.path.exists(path) or args.sp_learned_steps <= 0:
             continue
         model = jl.load(dec, path)
-        nxt, done, fails, iters, dev = first, 0, 0, 0, 0.0
-        l0 = dec.last_kernel_ms(1)[1]
+        nxt, done, fails, iters = first, 0, 0, 0
         t0 = time.perf_counter()
-        for _ in range(args.learned_steps):
+        for _ in range(args.sp_learned_steps):
             st, _ = dec.simulate_learned(model, args.seed, args.trials, args.p,
-                                         max_iter=args.max_iter, first_trial=nxt, per_trial=False)
+                                         max_iter=args.max_iter, first_trial=nxt, per_trial=False,
+                                         chunk=args.chunk)
             nxt += args.trials
             done += st["done"]
             fails += st["failures"]
             iters += st["total_iters"]
-            dev += st["device_ms"]
         el = time.perf_counter() - t0
-        nl = max(1, dec.last_kernel_ms(1)[1] - l0)
-        lo, hi = wilson(done, fails)
-        key = "mlp_damping" if model.use_mlp else "weights_only"
-        out[key] = {"weights": os.path.relpath(path, ROOT), "T": model.T, "trials": done,
-                    "trials_per_s": round(done / el, 2), "fer": fails / done, "fer_ci95": [lo, hi],
-                    "avg_iter": round(iters / done, 4), "decode_ms_per_launch": round(dev / nl, 4),
-                    "speedup_vs_plain": round((done / el) / (p_done / p_el), 4)}
+        key = "sum_product_mlp_damping" if model.use_mlp else "sum_product_weights_only"
+        out[key] = {"weights": os.path.relpath(path, ROOT), "trials": done,
+                    "trials_per_s": round(done / el, 2), "fer": fails / done,
+                    "avg_iter": round(iters / done, 4)}
         model.destroy()
-    return out if len(out) > 1 else None
+    return out or None
+
+
+def source_sha256(path):
+    import hashlib
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def kernel_profile():
+    """Counter profile committed for the current kernel source, or None when
+    there is none or it was taken on another version of the source."""
+    if not os.path.exists(VALU_PROFILE):
+        return None, "no counter profile"
+    with open(VALU_PROFILE) as f:
+        prof = json.load(f)
+    if prof.get("source_sha256") != source_sha256(KERNEL_SOURCE):
+        return None, "counter profile taken on another version of %s" % os.path.relpath(KERNEL_SOURCE, ROOT)
+    return prof, None
 
 
 def mixed_workload(jbp, qc_dec, args):
@@ -336,29 +360,111 @@ def copy_calibration(mib):
     print("copy calibration: %d MiB device-to-device" % mib, file=sys.stderr)
 
 
+def profile_run(jbp, args):
+    """The workload rocprofv3 counter passes run (--profile-run FILE):
+    --roofline-steps single launches of --chunk trials of the learned
+    decoder, then of plain BP on the same indices.  Writes the trial
+    iterations each kernel did, which the counter totals are divided by."""
+    from joint_bp_plus_pp_amd import learned as jl
+    code = jbp.Code.from_params(PARAMS)
+    dec = jbp.Decoder(code, 0)
+    model = jl.load(dec, LEARNED_MS_FILE)
+    T, n, R = model.T, args.chunk, args.roofline_steps
+    ms_it = fb_it = pl_it = 0
+    for k in range(R):
+        _, r = dec.simulate_learned(model, args.seed, n, args.p, max_iter=args.max_iter,
+                                    first_trial=1 + k * n, chunk=n)
+        ms_it += int(np.minimum(r["iterations"], T).sum())
+        fb_it += int(np.maximum(r["iterations"] - T, 0).sum())
+        st, _ = dec.simulate(args.seed, n, args.p, max_iter=args.max_iter, first_trial=1 + k * n,
+                             precision=32, per_trial=False, chunk=n)
+        pl_it += st["total_iters"]
+    out = {"launches": R, "trials_per_launch": n, "p": args.p, "max_iter": args.max_iter,
+           "ms_trial_iterations": ms_it, "plain_trial_iterations": pl_it + fb_it,
+           "fallback_trial_iterations": fb_it}
+    with open(args.profile_run, "w") as f:
+        json.dump(out, f)
+    model.destroy()
+    dec.destroy()
+    print("profile run: %s" % json.dumps(out), flush=True)
+
+
+def summarize_counters(args):
+    """Per-kernel totals of rocprofv3 counter CSVs over a --profile-run
+    workload, per trial-iteration, into a JSON summary (--summarize OUT)
+    stamped with the kernel source's SHA-256.  FETCH_SIZE and WRITE_SIZE are
+    scaled by what they read for a device-to-device copy of known size (a
+    dispatch of the runtime's copy kernel in the same run)."""
+    import csv
+    import collections
+    with open(args.profile_log) as f:
+        log = json.load(f)
+    tot = {"ms": collections.defaultdict(float), "plain": collections.defaultdict(float)}
+    disp = {"ms": set(), "plain": set()}
+    copy = collections.defaultdict(list)
+    for path in args.counters:
+        for r in csv.DictReader(open(path)):
+            name, c, v = r["Kernel_Name"], r["Counter_Name"], float(r["Counter_Value"])
+            if "decode_qc_ms_kernel" in name:
+                key = "ms"
+            elif "decode_qc_f32_kernel<768, 3, 12, false, false>" in name:
+                key = "plain"
+            else:
+                if "copyBuffer" in name:
+                    copy[c].append(v)
+                continue
+            tot[key][c] += v
+            disp[key].add((path, r["Dispatch_Id"]))
+    out = {"source": os.path.relpath(KERNEL_SOURCE, ROOT), "source_sha256": source_sha256(KERNEL_SOURCE),
+           "workload": log, "command": args.command}
+    # copy calibration: the largest copy dispatch moved copy_bytes each way
+    scale = {}
+    cb = args.copy_calibration_mb << 20
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        if copy.get(c) and cb:
+            scale[c] = cb / (max(copy[c]) * 1024.0)
+    for key, its in (("ms", log["ms_trial_iterations"]), ("plain", log["plain_trial_iterations"])):
+        t = tot[key]
+        d = {"dispatches_per_pass": len(disp[key]) // max(1, len(args.counters)),
+             "trial_iterations": its}
+        for c, v in sorted(t.items()):
+            d[c + "_per_trial_iteration"] = v / its
+        slots = t.get("SQ_INSTS_VALU", 0.0) + t.get("SQ_INSTS_VALU_TRANS_F32", 0.0)
+        d["valu_slots_per_trial_iteration"] = round(slots / its, 1)
+        launches = log["launches"]
+        if "FETCH_SIZE" in t and "WRITE_SIZE" in t and scale:
+            # counters in KiB; scaled by the copy calibration
+            d["hbm_bytes_per_launch"] = round(1024.0 * (t["FETCH_SIZE"] * scale["FETCH_SIZE"] +
+                                                        t["WRITE_SIZE"] * scale["WRITE_SIZE"]) / launches)
+        out[key] = d
+    out["copy_scale"] = scale
+    with open(args.summarize, "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote %s" % args.summarize, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", choices=("qc", "hgp"), default="qc",
                     help="qc: the included code (default, the headline); hgp: the seeded "
                          "hypergraph-product code HGP(150x200) on the HBM-resident decoder")
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--trials", type=int, default=8192,
-                    help="trials per step: one persistent decoder launch")
-    ap.add_argument("--pipelined-steps", type=int, default=8,
-                    help="also time this many calls of 65,536 trials (eight launches each, "
-                         "pipelined over two streams); 0 = skip")
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--trials", type=int, default=65536, help="trials per step (one call)")
+    ap.add_argument("--chunk", type=int, default=16384,
+                    help="trials per persistent launch; launches alternate two streams")
+    ap.add_argument("--roofline-steps", type=int, default=20,
+                    help="single-launch steps of each decoder timed for the roofline")
     ap.add_argument("--p", type=float, default=0.04)
     ap.add_argument("--max-iter", type=int, default=1000)
     ap.add_argument("--seed", type=int, default=106)
-    ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--cpu-trials", type=int, default=2048)
     ap.add_argument("--cpu-trials-1core", type=int, default=128,
                     help="trials of the CPU baseline also timed on one thread (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--learned-steps", type=int, default=40,
-                    help="steps of the learned decoder on the first timed trial indices (0: skip)")
+    ap.add_argument("--sp-learned-steps", type=int, default=0,
+                    help="steps of the sum-product learned models, for comparison (0: skip)")
     ap.add_argument("--mixed-trials", type=int, default=4096, help="trials per p of the mixed run")
     ap.add_argument("--mixed-max-iter", type=int, default=200)
     ap.add_argument("--hgp-trials", type=int, default=2048)
@@ -370,102 +476,136 @@ def main():
     ap.add_argument("--pp-max-iter", type=int, default=8)
     ap.add_argument("--pp-cpu-trials", type=int, default=128,
                     help="trials of the reference's PP loop timed beside it (0: skip)")
+    ap.add_argument("--profile-run", default="",
+                    help="run only the counter-profiling workload and write its trial iterations "
+                         "to this JSON file")
+    ap.add_argument("--summarize", default="",
+                    help="write a per-kernel counter summary (with --counters, --profile-log)")
+    ap.add_argument("--counters", nargs="*", default=[], help="rocprofv3 counter_collection CSVs")
+    ap.add_argument("--profile-log", default="", help="the --profile-run JSON of those passes")
+    ap.add_argument("--command", default="", help="command recorded in the summary")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a plain copy "
                          "kernel of known size, to scale profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: the decoder shards trials, and this build "
-              "measures the single-GPU share (got --gpus %d)" % args.gpus, file=sys.stderr)
+        print("bench.py runs on one GPU only: trials shard over GPUs by index range, and the "
+              "figure here is one GPU's share (got --gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.summarize:
+        return summarize_counters(args)
 
     import joint_bp_plus_pp_amd as jbp
 
     if args.copy_calibration_mb > 0:
         copy_calibration(args.copy_calibration_mb)
+    if args.profile_run:
+        return profile_run(jbp, args)
     if args.workload == "hgp":
         return bench_hgp(jbp, args)
+    from joint_bp_plus_pp_amd import learned as jl
     code = jbp.Code.from_params(PARAMS)
     dec = jbp.Decoder(code, 0)
     info = dec.info()
-    nxt = 1
-    for _ in range(args.warmup):
-        dec.simulate(args.seed, args.trials, args.p, max_iter=args.max_iter, first_trial=nxt,
-                     precision=args.precision, per_trial=False)
-        nxt += args.trials
-    first_timed = nxt
-    launches0 = dec.last_kernel_ms(1)[1]
-    done = fails = iters = 0
-    dev_ms = []
-    step_iters = []
-    head = [0, 0, 0.0]  # plain trials, failures, seconds over the first learned_steps steps
-    t0 = time.perf_counter()
-    for step in range(args.steps):
-        ts = time.perf_counter()
-        st, _ = dec.simulate(args.seed, args.trials, args.p, max_iter=args.max_iter, first_trial=nxt,
-                             precision=args.precision, per_trial=False)
-        nxt += args.trials
-        done += st["done"]
-        fails += st["failures"]
-        iters += st["total_iters"]
-        dev_ms.append(st["device_ms"])
-        step_iters.append(st["total_iters"])
-        if step < args.learned_steps:
-            head[0] += st["done"]
-            head[1] += st["failures"]
-            head[2] += time.perf_counter() - ts
-    elapsed = time.perf_counter() - t0
-    value = done / elapsed
-    launches = max(1, dec.last_kernel_ms(1)[1] - launches0)
-    kern_s = sum(dev_ms) / 1e3  # sum of the decoder launches' HIP-event durations
-    # FP64: 64 B of message traffic per edge per iteration (pair sums)
-    per_iter = BYTES_PER_ITER if args.precision == 32 else 64 * EDGES_PER_ITER
-    achieved = sum(step_iters) * per_iter / kern_s / 1e9
-    traffic = None
-    if args.precision == 32 and os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
-            t = json.load(f)
-        traffic = t.get("hbm_bytes_per_launch")
-    lo, hi = wilson(done, fails)
-    print("bench: %d trials in %.3f s; avg_iter %.3f; FER %.3g ci95 [%.3g, %.3g]; decode kernel "
-          "%.3f ms/launch (%d launches); %.3g trial-iter/s; info %s"
-          % (done, elapsed, iters / done, fails / done, lo, hi, sum(dev_ms) / launches, launches,
-             iters / elapsed, info), file=sys.stderr)
-    pipelined = None
-    if args.pipelined_steps > 0 and args.precision == 32:
-        pn = 65536
-        pdone = pfails = piters = 0
-        tp0 = time.perf_counter()
-        for _ in range(args.pipelined_steps):
-            st, _ = dec.simulate(args.seed, pn, args.p, max_iter=args.max_iter, first_trial=nxt,
-                                 precision=args.precision, per_trial=False)
-            nxt += pn
-            pdone += st["done"]
-            pfails += st["failures"]
-            piters += st["total_iters"]
-        pel = time.perf_counter() - tp0
-        plo, phi = wilson(pdone, pfails)
-        pipelined = {"trials_per_call": pn, "calls": args.pipelined_steps,
-                     "trials_per_s": round(pdone / pel, 2),
-                     "trial_iterations_per_s": round(piters / pel, 1),
-                     "fer": pfails / pdone, "fer_ci95": [plo, phi]}
-        print("bench: pipelined %s" % json.dumps(pipelined), file=sys.stderr)
-    learned = None
-    if args.learned_steps > 0 and args.precision == 32:
-        learned = learned_vs_plain(jbp, dec, args, first_timed, head)
-        if learned:
-            print("bench: learned %s" % json.dumps(learned), file=sys.stderr)
+    model = jl.load(dec, LEARNED_MS_FILE)
+    T = model.T
+    N = args.trials
+    # warm-up: both decoders, indices outside the timed range
+    wfirst = 1 + (args.steps + 8) * N
+    for w in range(args.warmup):
+        dec.simulate_learned(model, args.seed, N, args.p, max_iter=args.max_iter,
+                             first_trial=wfirst + w * N, per_trial=False, chunk=args.chunk)
+        dec.simulate(args.seed, N, args.p, max_iter=args.max_iter, first_trial=wfirst + w * N,
+                     precision=32, per_trial=False, chunk=args.chunk)
+
+    def run(learned):
+        done = fails = iters = 0
+        nxt = 1
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            if learned:
+                st, _ = dec.simulate_learned(model, args.seed, N, args.p, max_iter=args.max_iter,
+                                             first_trial=nxt, per_trial=False, chunk=args.chunk)
+            else:
+                st, _ = dec.simulate(args.seed, N, args.p, max_iter=args.max_iter, first_trial=nxt,
+                                     precision=32, per_trial=False, chunk=args.chunk)
+            nxt += N
+            done += st["done"]
+            fails += st["failures"]
+            iters += st["total_iters"]
+        return done, fails, iters, time.perf_counter() - t0
+
+    l_done, l_fails, l_iters, l_el = run(True)
+    p_done, p_fails, p_iters, p_el = run(False)
+    value = l_done / l_el
+    llo, lhi = wilson(l_done, l_fails)
+    plo, phi = wilson(p_done, p_fails)
+    # matched: the learned FER is no worse than plain BP's (at most the upper
+    # end of plain's 95% interval); below: under the interval's lower end
+    matched = l_fails / l_done <= phi
+    below = l_fails / l_done < plo
+    print("bench: learned %d trials %.3f s (%.0f/s, avg_iter %.3f, FER %.3g [%.3g, %.3g]); plain "
+          "%.3f s (%.0f/s, avg_iter %.3f, FER %.3g [%.3g, %.3g]); matched %s"
+          % (l_done, l_el, value, l_iters / l_done, l_fails / l_done, llo, lhi, p_el, p_done / p_el,
+             p_iters / p_done, p_fails / p_done, plo, phi, matched), file=sys.stderr)
+
+    # roofline leg: single launches (one chunk per call, no stream overlap)
+    # so that each kernel's HIP-event duration is its own
+    rfirst = wfirst + args.warmup * N
+    ms_ms, ms_iters, pl_ms, pl_iters, n_r = [], [], [], [], args.roofline_steps
+    for k in range(n_r):
+        st, r = dec.simulate_learned(model, args.seed, args.chunk, args.p, max_iter=args.max_iter,
+                                     first_trial=rfirst + k * args.chunk, chunk=args.chunk)
+        ms_ms.append(dec.last_kernel_ms(4)[0])
+        ms_iters.append(int(np.minimum(r["iterations"], T).sum()))
+        st, _ = dec.simulate(args.seed, args.chunk, args.p, max_iter=args.max_iter,
+                             first_trial=rfirst + k * args.chunk, precision=32, per_trial=False,
+                             chunk=args.chunk)
+        pl_ms.append(dec.last_kernel_ms(1)[0])
+        pl_iters.append(st["total_iters"])
+    ms_rate = sum(ms_iters) / (sum(ms_ms) / 1e3)     # learned-kernel trial-iterations/s
+    pl_rate = sum(pl_iters) / (sum(pl_ms) / 1e3)     # plain-kernel trial-iterations/s
+    prof, why = kernel_profile()
+    roofline = {"bound": "valu", "unit": "Gslots/s", "peak": VALU_PEAK_GSLOTS,
+                "kernel": "decode_qc_ms_kernel (learned min-sum, FP16 pairs)",
+                "kernel_ms_per_launch": round(sum(ms_ms) / n_r, 4), "launches": n_r,
+                "trial_iterations_per_s": round(ms_rate, 1),
+                "streaming_equivalent_gbs": round(ms_rate * BYTES_PER_ITER / 1e9, 1)}
+    plain_roof = {"kernel": "decode_qc_f32_kernel (plain sum-product, FP32)",
+                  "kernel_ms_per_launch": round(sum(pl_ms) / n_r, 4),
+                  "trial_iterations_per_s": round(pl_rate, 1),
+                  "streaming_equivalent_gbs": round(pl_rate * BYTES_PER_ITER / 1e9, 1)}
+    if prof:
+        v = prof["ms"]["valu_slots_per_trial_iteration"]
+        roofline.update({"achieved": round(v * ms_rate / 1e9, 2),
+                         "frac": round(v * ms_rate / 1e9 / VALU_PEAK_GSLOTS, 4),
+                         "valu_slots_per_trial_iteration": v,
+                         "traffic": prof["ms"].get("hbm_bytes_per_launch"),
+                         "profile": os.path.relpath(VALU_PROFILE, ROOT)})
+        vp = prof["plain"]["valu_slots_per_trial_iteration"]
+        plain_roof.update({"valu_achieved": round(vp * pl_rate / 1e9, 2),
+                           "valu_frac": round(vp * pl_rate / 1e9 / VALU_PEAK_GSLOTS, 4),
+                           "valu_slots_per_trial_iteration": vp,
+                           "traffic": prof["plain"].get("hbm_bytes_per_launch")})
+    else:
+        roofline.update({"achieved": None, "frac": None, "traffic": None, "note": why})
+    print("bench: roofline %s; plain %s" % (json.dumps(roofline), json.dumps(plain_roof)),
+          file=sys.stderr)
+
+    after = 1 + (args.steps + 8 + args.warmup) * N + n_r * args.chunk
+    sp_learned = sp_learned_leg(jbp, dec, args, after) if args.sp_learned_steps > 0 else None
     mixed = None
-    if not args.no_mixed and args.precision == 32:
+    if not args.no_mixed:
         mixed = mixed_workload(jbp, dec, args)
         print("bench: mixed %s" % json.dumps(mixed), file=sys.stderr)
     pp = None
-    if args.pp_trials > 0 and args.precision == 32:
-        pp = pp_workload(jbp, dec, args, nxt)
+    if args.pp_trials > 0:
+        pp = pp_workload(jbp, dec, args, after)
         print("bench: pp %s" % json.dumps(pp), file=sys.stderr)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args, first_timed)
+        cpu = cpu_baseline(args, 1)
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -473,43 +613,43 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "ms_per_step": round(1e3 * l_el / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32" if args.precision == 32 else "f64",
+        "dtype": "fp16",
         "data": "synthetic",
         "config": {
-            "workload": "plain joint BP, batched, included [[9216,4612]] code, p=%g, %d trials per "
-                        "step (one persistent launch), max-iter %d, early termination, seed %d"
-                        % (args.p, args.trials, args.max_iter, args.seed),
-            "decoder": "plain FP32 LDS-resident" if args.precision == 32 else "plain FP64 parity",
-            "trials_per_step": args.trials,
-            "p": args.p,
-            "max_iter": args.max_iter,
-            "avg_iter": round(iters / done, 4),
-            "fer": fails / done,
-            "fer_ci95": [lo, hi],
-            "trial_iterations_per_s": round(iters / elapsed, 1),
-            "edge_updates_per_s": round(iters * EDGES_PER_ITER / elapsed, 1),
-            "pipelined": pipelined,
-            "learned": learned,
+            "workload": "learned min-sum BP (T=%d, shipped weights) + plain FP32 BP fallback to "
+                        "max-iter %d, included [[9216,4612]] code, p=%g, %d trials per step in "
+                        "launches of %d over two streams, seed %d; plain FP32 BP on the same "
+                        "trial indices beside it" % (T, args.max_iter, args.p, N, args.chunk,
+                                                     args.seed),
+            "decoder": "learned min-sum, FP16 (X, Z) message pairs, LDS-resident; fallback plain "
+                       "FP32 sum-product",
+            "weights": os.path.relpath(LEARNED_MS_FILE, ROOT),
+            "trials_per_step": N, "launch_trials": args.chunk, "p": args.p,
+            "max_iter": args.max_iter, "T": T,
+            "trials": l_done, "avg_iter": round(l_iters / l_done, 4),
+            "fer": l_fails / l_done, "fer_ci95": [llo, lhi],
+            "trial_iterations_per_s": round(l_iters / l_el, 1),
+            "edge_updates_per_s": round(l_iters * EDGES_PER_ITER / l_el, 1),
+            "plain_same_trials": {"trials": p_done, "trials_per_s": round(p_done / p_el, 2),
+                                  "avg_iter": round(p_iters / p_done, 4), "fer": p_fails / p_done,
+                                  "fer_ci95": [plo, phi],
+                                  "trial_iterations_per_s": round(p_iters / p_el, 1),
+                                  "roofline": plain_roof},
+            "speedup_vs_plain": round(value / (p_done / p_el), 4),
+            "fer_matched": matched,
+            "fer_below_plain_ci": below,
+            "sp_learned": sp_learned,
             "mixed": mixed,
             "pp": pp,
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "kernel": "decode_qc_f32_kernel" if args.precision == 32 else "decode_f64_kernel",
-            "kernel_ms_per_launch": round(sum(dev_ms) / launches, 4),
-            "launches": launches,
-        },
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    model.destroy()
     dec.destroy()
     print(json.dumps(line), flush=True)
 

@@ -85,9 +85,10 @@ void jbp_destroy(jbp_ctx *ctx);
  * over (I, X, Z, Y).  est_out receives nvars states 0..3.  precision is
  * JBP_FP32 or JBP_FP64.  check_warmup / check_interval skip syndrome tests as
  * the reference's flags do (FP32 only; FP64 tests every iteration).
- * freeze_syn is not run on the GPU: it returns JBP_ERR_UNSUPPORTED so that
- * the caller falls back to its CPU decoder, as the reference caller does on
- * any decode failure (jointbp_ets.cpp:5961-5985). */
+ * freeze_syn (the reference's --freeze-syn, jointbp_ets.cpp:1884-1901,
+ * 2526-2535) runs on the FP64 decoder; with FP32 it returns
+ * JBP_ERR_UNSUPPORTED so that the caller falls back, as the reference
+ * caller does on any decode failure (jointbp_ets.cpp:5961-5985). */
 int jbp_decode(jbp_ctx *ctx, const int *sx, const int *sz, const double prior[4], int max_iter,
                int check_warmup, int check_interval, int measure_costs, int use_graph,
                int freeze_syn, double damping, int precision, jbp_decode_result *out,
@@ -151,6 +152,11 @@ typedef struct jbp_sim_cfg {
                              OSD, default on there; off = --no-pp) with the
                              tables installed by jbp_set_pp_tables          */
     int flip_hist;        /* --flip-hist (reference default 5)              */
+    int chunk;            /* LDS engine, runs longer than one launch: trials
+                             per persistent launch (0 = 8192, at most 65536);
+                             launches alternate two streams                  */
+    int freeze_syn;       /* --freeze-syn (jointbp_ets.cpp:1884-1901,
+                             2526-2535): FP64, post-processing off           */
 } jbp_sim_cfg;
 
 #define JBP_MAX_P_LIST 4096
@@ -158,7 +164,10 @@ typedef struct jbp_sim_cfg {
 /* FP32 decoders.  LDS: one trial per compute unit with all messages on chip,
  * for the 768 x (3 x 12) permutation-block codes.  HBM: any code; 64 trials
  * interleaved per wavefront, messages in device memory (slots = trials
- * resident, a multiple of 64).  AUTO picks LDS when it applies. */
+ * resident, a multiple of 64).  AUTO picks LDS when it applies.  Damping
+ * (the reference's scaled_mix of each variable-to-check message with the
+ * previous one, jointbp_ets.cpp:1903-1911) runs on the HBM decoder, which
+ * keeps the previous messages; AUTO picks it when damping > 0. */
 #define JBP_ENGINE_AUTO 0
 #define JBP_ENGINE_LDS 1
 #define JBP_ENGINE_HBM 2
@@ -315,7 +324,10 @@ int jbp_in_row_space(jbp_ctx *ctx, int side, int count, const int *bits, uint8_t
  * profiling: which = 0 sample, 1 decode, 2 residual check.  Returns ms and
  * the launch count since the context was created.  which = 3: the decode
  * time again, with the number of engine steps (one flooding iteration of
- * every resident trial) of the last generic-decoder call in *launches. */
+ * every resident trial) of the last generic-decoder call in *launches.
+ * which = 4: the learned min-sum decoder's own launch within the last
+ * decode (without the plain-BP launch over the trials it left), and the
+ * count of such launches. */
 double jbp_last_kernel_ms(jbp_ctx *ctx, int which, int64_t *launches);
 
 /* Context facts: out[0] = 1 when the LDS-resident FP32 decoder applies,

@@ -58,7 +58,8 @@ class SimCfg(C.Structure):
                 ("precision", C.c_int), ("check_warmup", C.c_int), ("check_interval", C.c_int),
                 ("slots", C.c_int), ("block", C.c_int), ("min_fails", C.c_int64),
                 ("p_list", C.POINTER(C.c_double)), ("n_p", C.c_int), ("engine", C.c_int),
-                ("schedule", C.c_int), ("graph", C.c_int), ("pp", C.c_int), ("flip_hist", C.c_int)]
+                ("schedule", C.c_int), ("graph", C.c_int), ("pp", C.c_int), ("flip_hist", C.c_int),
+                ("chunk", C.c_int), ("freeze_syn", C.c_int)]
 
 
 class TrialOut(C.Structure):
@@ -523,19 +524,19 @@ class Decoder:
 
     def simulate_learned(self, model, base_seed, n_trials, p, max_iter=1000, first_trial=1,
                          check_warmup=0, check_interval=1, min_fails=0, per_trial=True,
-                         want_err=False, want_est=False):
+                         want_err=False, want_est=False, chunk=0):
         """jbp_simulate with the learned decoder for model.T iterations, then
         plain FP32 BP from the prior on trials that did not converge."""
         return self.simulate(base_seed, n_trials, p, max_iter=max_iter, first_trial=first_trial,
                              precision=FP32, check_warmup=check_warmup,
                              check_interval=check_interval, min_fails=min_fails,
                              per_trial=per_trial, want_err=want_err, want_est=want_est,
-                             _model=model)
+                             chunk=chunk, _model=model)
 
     def simulate(self, base_seed, n_trials, p, max_iter=50, first_trial=1, damping=0.0,
                  precision=FP32, check_warmup=0, check_interval=1, min_fails=0, slots=0, block=0,
                  per_trial=True, want_err=False, want_est=False, engine=0, schedule=0,
-                 graph=False, pp=False, flip_hist=5, _model=None):
+                 graph=False, pp=False, flip_hist=5, chunk=0, freeze_syn=False, _model=None):
         """The --simulate inner loop on the device (post-processing off unless
         pp=True, i.e. the reference's --no-pp by default).  Returns (stats,
         per-trial dict or None).  p may be a sequence: trial index idx then
@@ -546,7 +547,9 @@ class Decoder:
         syndrome first (results are the same); graph=True replays the HBM
         engine's step batches from a captured graph.  pp=True runs the
         reference's post-processing with the tables of set_pp_tables
-        (flip_hist: --flip-hist); per-trial "pp_flags" then holds PPF_* bits."""
+        (flip_hist: --flip-hist); per-trial "pp_flags" then holds PPF_* bits.
+        chunk: trials per persistent launch of the LDS engine when a run
+        spans several launches (0 = 8192)."""
         plist = None
         if np.ndim(p) > 0:
             plist = np.ascontiguousarray(p, dtype=np.float64)
@@ -555,7 +558,7 @@ class Decoder:
                      check_interval, slots, block, min_fails,
                      plist.ctypes.data_as(C.POINTER(C.c_double)) if plist is not None else None,
                      int(plist.size) if plist is not None else 0, engine, schedule, int(graph),
-                     int(pp), flip_hist)
+                     int(pp), flip_hist, chunk, int(freeze_syn))
         out = None
         arrays = {}
         if per_trial:

@@ -377,7 +377,7 @@ int default_gen_slots(const jbp_ctx *ctx) {
 }
 
 int run_gen(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmup, int interval,
-            int slots, bool use_graph, char *err, size_t errlen) {
+            int slots, bool use_graph, double damping, char *err, size_t errlen) {
     const DevGraph &g = ctx->dg;
     if (!gen_supported(g)) {
         set_err(err, errlen, "generic FP32 decoder supports check degree <= 32");
@@ -386,7 +386,8 @@ int run_gen(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmu
     int S = slots > 0 ? slots : default_gen_slots(ctx);
     S = std::min(S, ((count + 63) / 64) * 64);
     const int groups = std::max(1, (S + 63) / 64);
-    const size_t bytes = gen_carve_bytes(g, groups);
+    const bool damped = damping > 0.0;
+    const size_t bytes = gen_carve_bytes(g, groups, damped);
     if (bytes > ctx->gen_cap) {
         if (ctx->gen_mem) (void)hipFree(ctx->gen_mem);
         ctx->gen_mem = nullptr;
@@ -397,7 +398,8 @@ int run_gen(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmu
     if (!ctx->h_ctrl) HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ctx->h_ctrl), 2 * sizeof(int)));
     for (auto &e : ctx->gen_ev)
         if (!e) HIPCHK(hipEventCreate(&e));
-    gen_carve(g, groups, ctx->gen_mem, ctx->gen);
+    gen_carve(g, groups, ctx->gen_mem, ctx->gen, damped);
+    ctx->gen.damping = static_cast<float>(damped ? damping : 0.0);
     int64_t steps = 0;
     const int rc = gen_decode(g, ctx->gen, ctx->tp, count, nt, max_iter, warmup, interval,
                               ctx->h_ctrl, ctx->gen_ev, ctx->stream, &steps, use_graph);
@@ -415,11 +417,15 @@ int run_gen(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmu
 
 }  // namespace
 
+// The learned min-sum decoder runs as its own launch (timed apart).
+static bool learned_ms(const LearnedInfer *li) { return li && li->T > 0 && li->kind == kLearnedMS; }
+
 // Decode `count` trials whose syndrome planes are already in ctx->tp.
 int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmup, int interval,
                double damping, int precision, char *err, size_t errlen,
                const LearnedInfer *learned, int engine, int slots, bool use_graph,
-               bool defer_timing) {
+               bool defer_timing,
+               bool freeze) {
     if (precision != JBP_FP32 && precision != JBP_FP64) {
         set_err(err, errlen, "precision must be 32 or 64");
         return JBP_ERR_ARG;
@@ -432,12 +438,19 @@ int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int wa
     HIPCHK(hipGetLastError());
     hipEvent_t e0 = defer_timing ? ctx->evt[2] : ctx->ev0, e1 = defer_timing ? ctx->evt[3] : ctx->ev1;
     HIPCHK(hipEventRecord(e0, ctx->stream));
+    if (freeze && precision != JBP_FP64) {
+        set_err(err, errlen, "freeze-syn runs on the FP64 decoder (precision 64)");
+        return JBP_ERR_UNSUPPORTED;
+    }
     if (precision == JBP_FP32) {
-        if (damping > 0.0) {
-            set_err(err, errlen, "FP32 decoder runs damping 0 only; use precision 64");
+        // damping mixes each variable-to-check message with the previous
+        // one, which the HBM engine keeps and the LDS engine has no room for
+        if (damping > 0.0 && (engine == JBP_ENGINE_LDS || learned)) {
+            set_err(err, errlen, "FP32 damping runs on the HBM-resident decoder (engine AUTO or HBM)");
             return JBP_ERR_UNSUPPORTED;
         }
-        const bool lds = engine == JBP_ENGINE_LDS || (engine == JBP_ENGINE_AUTO && (ctx->qc || learned));
+        const bool lds = engine == JBP_ENGINE_LDS ||
+                         (engine == JBP_ENGINE_AUTO && damping <= 0.0 && (ctx->qc || learned));
         if (lds && !ctx->qc) {
             set_err(err, errlen, "the LDS-resident FP32 decoder needs a 768 x (3 x 12) permutation-block code");
             return JBP_ERR_UNSUPPORTED;
@@ -449,16 +462,17 @@ int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int wa
         if (lds) {
             launch_decode_qc_f32(ctx->dg, ctx->qcp, ctx->d_qc_tables, ctx->tp, count, nt, max_iter,
                                  warmup, interval, std::max(1, ctx->cus), ctx->d_counter, ctx->stream,
-                                 learned, ctx->d_fb);
+                                 learned, ctx->d_fb, learned_ms(learned) ? ctx->evt[4] : nullptr);
         } else {
-            int rc = run_gen(ctx, count, nt, max_iter, warmup, interval, slots, use_graph, err, errlen);
+            int rc = run_gen(ctx, count, nt, max_iter, warmup, interval, slots, use_graph, damping, err,
+                             errlen);
             if (rc) return rc;
         }
     } else {
         int rc = ensure_f64(ctx, err, errlen);
         if (rc) return rc;
         launch_decode_f64(ctx->dg, ctx->tp, count, nt, max_iter, damping, ctx->f64_slots,
-                          ctx->f64_block, ctx->f64_scratch, ctx->d_counter, ctx->stream);
+                          ctx->f64_block, ctx->f64_scratch, ctx->d_counter, ctx->stream, freeze);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e1, ctx->stream));
@@ -468,6 +482,11 @@ int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int wa
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_ms[1] = ms;
     ctx->launches[1]++;
+    if (learned_ms(learned) && precision == JBP_FP32) {
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->evt[4]));
+        ctx->last_ms[4] = ms;
+        ctx->launches[4]++;
+    }
     return JBP_OK;
 }
 
@@ -534,7 +553,7 @@ int simulate_pipelined(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_run_stats &st, 
                        const LearnedInfer *learned, jbp_trial_out *pt, char *err, size_t errlen) {
     const DevGraph &g = ctx->dg;
     const int64_t total = cfg->n_trials;
-    const int C = kPipeChunk;
+    const int C = cfg->chunk > 0 ? cfg->chunk : kPipeChunk;
     int rc = ensure_pipe(ctx, C, err, errlen);
     if (rc) return rc;
     const int64_t nch = (total + C - 1) / C;
@@ -559,14 +578,16 @@ int simulate_pipelined(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_run_stats &st, 
         launch_fill_i32(b.counter, 0, 1, b.st);
         HIPCHK(hipEventRecord(b.ev[1], b.st));
         launch_decode_qc_f32(g, ctx->qcp, ctx->d_qc_tables, tpo, cnt, nt, cfg->max_iter, warmup,
-                             interval, std::max(1, ctx->cus), b.counter, b.st, learned, b.fb);
+                             interval, std::max(1, ctx->cus), b.counter, b.st, learned, b.fb,
+                             learned_ms(learned) ? b.ev[4] : nullptr);
         HIPCHK(hipEventRecord(b.ev[2], b.st));
         if (ctx->have_logicals) launch_logical(g, ctx->lg[0], ctx->lg[1], b.tp, cnt, b.st);
         else if (ctx->have_basis) launch_residual(g, ctx->basis[0], ctx->basis[1], b.tp, cnt, b.st);
         else launch_logical(g, DevLogicals(), DevLogicals(), b.tp, cnt, b.st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(b.h_iters, b.tp.iters, cnt * 4, hipMemcpyDeviceToHost, b.st));
-        HIPCHK(hipMemcpyAsync(b.h_flags, b.tp.syn_ok, 3 * static_cast<size_t>(C), hipMemcpyDeviceToHost, b.st));
+        // the three flag arrays sit b.cap apart (the buffers may be larger than C)
+        HIPCHK(hipMemcpyAsync(b.h_flags, b.tp.syn_ok, 3 * static_cast<size_t>(b.cap), hipMemcpyDeviceToHost, b.st));
         HIPCHK(hipEventRecord(b.ev[3], b.st));
         return JBP_OK;
     };
@@ -589,8 +610,14 @@ int simulate_pipelined(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_run_stats &st, 
         ctx->launches[0]++;
         ctx->launches[1]++;
         ctx->launches[2]++;
+        if (learned_ms(learned)) {
+            float lms = 0.f;
+            HIPCHK(hipEventElapsedTime(&lms, b.ev[1], b.ev[4]));
+            ctx->last_ms[4] = lms;
+            ctx->launches[4]++;
+        }
         dev_ms += dms;
-        const uint8_t *ok = b.h_flags, *succ = b.h_flags + C, *exact = b.h_flags + 2 * C;
+        const uint8_t *ok = b.h_flags, *succ = b.h_flags + b.cap, *exact = b.h_flags + 2 * b.cap;
         int take = cnt;
         for (int i = 0; i < cnt; ++i) {
             const uint8_t s = judged ? succ[i] : static_cast<uint8_t>(ok[i] && exact[i]);
@@ -658,6 +685,10 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
         int rc0 = set_noise(ctx, pri.data(), plist, np, err, errlen);
         if (rc0) return rc0;
     }
+    if (cfg->chunk < 0 || cfg->chunk > kDefaultChunk) {
+        set_err(err, errlen, "chunk must be 0..%d (got %d)", kDefaultChunk, cfg->chunk);
+        return JBP_ERR_ARG;
+    }
     if (cfg->block > 0) {
         if (cfg->block > 512 || cfg->block % 64) {
             set_err(err, errlen, "block must be a multiple of 64 up to 512 (got %d)", cfg->block);
@@ -676,6 +707,10 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
         set_err(err, errlen, "post-processing is not combined with the learned decoder");
         return JBP_ERR_UNSUPPORTED;
     }
+    if (cfg->freeze_syn && (pp_on || learned || cfg->precision != JBP_FP64)) {
+        set_err(err, errlen, "freeze-syn runs on the FP64 decoder without post-processing");
+        return JBP_ERR_UNSUPPORTED;
+    }
     if (pp_on && cfg->flip_hist < 0) {
         set_err(err, errlen, "flip_hist must be 0 or more");
         return JBP_ERR_ARG;
@@ -685,7 +720,7 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
     const bool lds = cfg->precision == JBP_FP32 && cfg->damping <= 0.0 && ctx->qc &&
                      (cfg->engine == JBP_ENGINE_LDS ||
                       (cfg->engine == JBP_ENGINE_AUTO) || learned);
-    if (lds && total > kPipeChunk && !pp_on && !(pt && (pt->err || pt->est))) {
+    if (lds && total > (cfg->chunk > 0 ? cfg->chunk : kPipeChunk) && !pp_on && !(pt && (pt->err || pt->est))) {
         int rc0 = simulate_pipelined(ctx, cfg, st, np, learned, pt, err, errlen);
         if (rc0) return rc0;
         st.elapsed_ms =
@@ -725,7 +760,7 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
             rc = run_decode(ctx, cnt, nt, pp_on ? pp_h : cfg->max_iter, cfg->check_warmup,
                             cfg->check_interval, cfg->damping, cfg->precision, err, errlen, learned,
                             cfg->engine, cfg->precision == JBP_FP32 ? cfg->slots : 0, cfg->graph != 0,
-                            true);
+                            true, cfg->freeze_syn != 0);
             ctx->tp.order = nullptr;
             if (rc) return rc;
         }
@@ -778,6 +813,12 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
                 HIPCHK(hipEventElapsedTime(&dms, ctx->evt[2], ctx->evt[3]));
                 ctx->last_ms[1] = dms;
                 ctx->launches[1]++;
+                if (learned_ms(learned) && cfg->precision == JBP_FP32) {
+                    float lms = 0.f;
+                    HIPCHK(hipEventElapsedTime(&lms, ctx->evt[2], ctx->evt[4]));
+                    ctx->last_ms[4] = lms;
+                    ctx->launches[4]++;
+                }
                 dev_ms += dms;
             }
         }
@@ -1044,7 +1085,7 @@ int jbp_create(const jbp_graph_desc *g, int device_id, jbp_ctx **out, char *err,
     if (hipStreamCreate(&ctx->stream) != hipSuccess || hipEventCreate(&ctx->ev0) != hipSuccess ||
         hipEventCreate(&ctx->ev1) != hipSuccess || hipEventCreate(&ctx->evt[0]) != hipSuccess ||
         hipEventCreate(&ctx->evt[1]) != hipSuccess || hipEventCreate(&ctx->evt[2]) != hipSuccess ||
-        hipEventCreate(&ctx->evt[3]) != hipSuccess) {
+        hipEventCreate(&ctx->evt[3]) != hipSuccess || hipEventCreate(&ctx->evt[4]) != hipSuccess) {
         set_err(err, errlen, "stream/event creation failed");
         return fail(JBP_ERR_DEVICE);
     }
@@ -1198,7 +1239,7 @@ int jbp_info(jbp_ctx *ctx, int64_t out[8]) {
 }
 
 double jbp_last_kernel_ms(jbp_ctx *ctx, int which, int64_t *launches) {
-    if (!ctx || which < 0 || which > 3) return -1.0;
+    if (!ctx || which < 0 || which > 4) return -1.0;
     if (which == 3) {  // generic decoder: engine steps of its last decode call
         if (launches) *launches = ctx->gen_steps;
         return ctx->last_ms[1];
@@ -1309,8 +1350,8 @@ int jbp_decode(jbp_ctx *ctx, const int *sx, const int *sz, const double prior[4]
         set_err(err, errlen, "bad arguments");
         return JBP_ERR_ARG;
     }
-    if (freeze_syn) {
-        set_err(err, errlen, "freeze-syn is not supported on the GPU");
+    if (freeze_syn && precision != JBP_FP64) {
+        set_err(err, errlen, "freeze-syn runs on the FP64 decoder (precision 64)");
         return JBP_ERR_UNSUPPORTED;
     }
     if (fault("decode")) {
@@ -1336,7 +1377,7 @@ int jbp_decode(jbp_ctx *ctx, const int *sx, const int *sz, const double prior[4]
     // Graph replay applies to the single-launch decoders (the LDS-resident
     // FP32 decoder and the FP64 decoder); the generic FP32 decoder's host
     // loop cannot be captured, so it runs eagerly.
-    if (use_graph && (precision == JBP_FP64 || (ctx->qc && damping <= 0.0)) &&
+    if (use_graph && !freeze_syn && (precision == JBP_FP64 || (ctx->qc && damping <= 0.0)) &&
         (precision == JBP_FP32 || precision == JBP_FP64)) {
         rc = decode_graph(ctx, bx, bz, max_iter, warmup, interval, damping, precision, out, est_out,
                           t0, err, errlen);
@@ -1347,7 +1388,7 @@ int jbp_decode(jbp_ctx *ctx, const int *sx, const int *sz, const double prior[4]
     HIPCHK(hipMemcpy(ctx->tp.syn_z, bz.data(), bz.size() * 4, hipMemcpyHostToDevice));
     auto m1 = std::chrono::steady_clock::now();
     rc = run_decode(ctx, 1, ctx->noise, max_iter, warmup, interval, damping, precision, err,
-                    errlen);
+                    errlen, nullptr, JBP_ENGINE_AUTO, 0, false, false, freeze_syn != 0);
     if (rc) return rc;
     out->kernel_ms = ctx->last_ms[1];
     auto m2 = std::chrono::steady_clock::now();

@@ -146,9 +146,10 @@ void launch_residual(const DevGraph &g, DevBasis bx, DevBasis bz, TrialPlanes tp
 // lz against the Z part.
 void launch_logical(const DevGraph &g, DevLogicals lx, DevLogicals lz, TrialPlanes tp, int count,
                     hipStream_t st);
+// freeze: the reference's --freeze-syn
 void launch_decode_f64(const DevGraph &g, TrialPlanes tp, int count, const NoiseTab &nt, int max_iter,
                        double damping, int slots, int block, double *scratch, int *counter,
-                       hipStream_t st);
+                       hipStream_t st, bool freeze = false);
 size_t decode_f64_scratch_bytes(const DevGraph &g, int slots);
 // d_tables: device ints [2][J][L] ainv followed by [2][J][L] b.
 struct LearnedInfer;
@@ -159,7 +160,8 @@ struct LearnedInfer;
 void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_tables,
                           TrialPlanes tp, int count, const NoiseTab &nt, int max_iter, int check_warmup,
                           int check_interval, int slots, int *counter, hipStream_t st,
-                          const LearnedInfer *learned = nullptr, int *fb = nullptr);
+                          const LearnedInfer *learned = nullptr, int *fb = nullptr,
+                          hipEvent_t learned_end = nullptr);
 bool qc_f32_supported(const QcParams &qc, int device, char *why, size_t len);
 void launch_fill_i32(int *ptr, int value, int count, hipStream_t st);
 void launch_unpack_states(const uint32_t *xbits, const uint32_t *zbits, int words, int n,

@@ -33,7 +33,7 @@ struct PipeBuf {
     std::vector<void *> allocs;
     int *counter = nullptr;
     hipStream_t st = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // sample, decode start, decode end, done
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // sample, decode start, decode end, done, learned-phase end
     int *h_iters = nullptr;   // pinned
     uint8_t *h_flags = nullptr;  // pinned: ok | success | exact, cap each
     int *keys = nullptr;      // decode-order scratch
@@ -126,7 +126,7 @@ struct jbp_ctx {
     std::vector<void *> chunk_allocs;
     int *h_iters = nullptr;     // pinned [chunk_cap]: outcome copies of a chunk
     uint8_t *h_flags = nullptr; // pinned [3][chunk_cap]: syn_ok, success, exact
-    hipEvent_t evt[4] = {nullptr, nullptr, nullptr, nullptr};  // sample / decode start, end
+    hipEvent_t evt[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // sample / decode start, end; [4] learned-phase end
     double *f64_scratch = nullptr;
     int f64_slots = 0;
     int f64_block = 256;
@@ -169,8 +169,8 @@ struct jbp_ctx {
     hipStream_t pp_stream = nullptr;
     int *pp_idx = nullptr;  // the stage's slot list 0..batch-1, then its trial indices
     size_t pp_idx_cap = 0;
-    double last_ms[3] = {0, 0, 0};
-    int64_t launches[3] = {0, 0, 0};
+    double last_ms[5] = {0, 0, 0, 0, 0};
+    int64_t launches[5] = {0, 0, 0, 0, 0};
 };
 
 #define HIPCHK(call)                                                                    \
@@ -202,7 +202,8 @@ void free_noise(jbp_ctx *ctx);
 int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmup, int interval,
                double damping, int precision, char *err, size_t errlen,
                const LearnedInfer *learned = nullptr, int engine = JBP_ENGINE_AUTO,
-               int slots = 0, bool use_graph = false, bool defer_timing = false);
+               int slots = 0, bool use_graph = false, bool defer_timing = false,
+               bool freeze = false);
 void free_gen(jbp_ctx *ctx);
 void free_logicals(jbp_ctx *ctx);
 void free_pipe(jbp_ctx *ctx);

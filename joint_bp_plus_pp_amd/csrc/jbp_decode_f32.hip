@@ -790,7 +790,7 @@ bool qc_f32_supported(const QcParams &qc, int device, char *why, size_t len) {
 void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_tables,
                           TrialPlanes tp, int count, const NoiseTab &nt, int max_iter, int check_warmup,
                           int check_interval, int slots, int *counter, hipStream_t st,
-                          const LearnedInfer *learned, int *fb) {
+                          const LearnedInfer *learned, int *fb, hipEvent_t learned_end) {
     if (count <= 0) return;
     QcDev dev;
     dev.P = qc.P;
@@ -823,6 +823,7 @@ void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_ta
         };
         if (learned->use_mlp) ms(decode_qc_ms_kernel<768, 3, 12, true>);
         else ms(decode_qc_ms_kernel<768, 3, 12, false>);
+        if (learned_end) (void)hipEventRecord(learned_end, st);
         TrialPlanes fbp = tp;
         fbp.order = fb_list;
         li = LearnedInfer();

@@ -49,9 +49,14 @@ __device__ __forceinline__ SlotPtrs slot_ptrs(const DevGraph &g, double *scratch
 // waves per SIMD (2: up to 256 VGPRs).  Degree 12 / 3 with pair sums needs
 // about 143, so three 256-thread blocks share a compute unit; the
 // quasi-cyclic pair path (pipelined check pass, 72 KB of LDS) runs two.
-template <int MAXC, int MAXV, bool PAIR, bool QC>
+// FREEZE: the reference's --freeze-syn (jointbp_ets.cpp:1884-1901,
+// 2526-2535): once a side's syndrome matches, its check pass stops and its
+// check-to-variable messages become the deterministic messages of the
+// estimate's bits at that iteration (generic rows only, QC = false).
+template <int MAXC, int MAXV, bool PAIR, bool QC, bool FREEZE = false>
 __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPlanes tp, int count, NoiseTab nt, int max_iter,
                                   double damping, double *scratch, size_t stride, int *counter) {
+    static_assert(!(FREEZE && QC), "freeze runs on the generic rows");
     __shared__ int s_trial;
     __shared__ int s_qc[QC ? 6 * kQcMaxBlocks : 1];  // the block maps (QC codes)
     extern __shared__ double2 s_msg[];               // QC pair sums: a column block's rows
@@ -91,6 +96,7 @@ __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPla
         int iterations = max_iter;
         int matched = 0;
         int res = -1;  // QC pair path: the column block resident in LDS
+        bool frozen[2] = {false, false};
         for (int iter = 0; iter < max_iter; ++iter) {
             // one loop per side: a single loop branching between the sides
             // keeps both inlined recursions' registers live
@@ -98,10 +104,10 @@ __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPla
                 check_passes_qc<MAXC>(g, res, tsx, tsz, sp.v2c, sp.c2v, s_msg);
             } else {
 #pragma unroll 1
-                for (int c = tid; c < mx; c += bs)
+                for (int c = tid; c < mx && !frozen[0]; c += bs)
                     check_update<MAXC, PAIR, QC>(g.side[0], 0, c, (tsx[c >> 5] >> (c & 31)) & 1, sp.v2c[0], sp.c2v[0]);
 #pragma unroll 1
-                for (int c = tid; c < mz; c += bs)
+                for (int c = tid; c < mz && !frozen[1]; c += bs)
                     check_update<MAXC, PAIR, QC>(g.side[1], 1, c, (tsz[c >> 5] >> (c & 31)) & 1, sp.v2c[1], sp.c2v[1]);
             }
             __syncthreads();
@@ -115,6 +121,42 @@ __global__ __launch_bounds__(512, 2) void decode_f64_kernel(DevGraph g, TrialPla
             }
             __syncthreads();
             // both syndromes of est (jointbp_ets.cpp:2521-2525)
+            if (FREEZE) {
+                int badx = 0, badz = 0;
+                for (int c = tid; c < mx && !badx; c += bs)
+                    badx |= check_parity<MAXC>(g.side[0], 0, c, sp.est) != static_cast<int>((tsx[c >> 5] >> (c & 31)) & 1u);
+                for (int c = tid; c < mz && !badz; c += bs)
+                    badz |= check_parity<MAXC>(g.side[1], 1, c, sp.est) != static_cast<int>((tsz[c >> 5] >> (c & 31)) & 1u);
+                const bool okx = !__syncthreads_or(badx), okz = !__syncthreads_or(badz);
+                // freeze_side_msgs: every check-to-variable message of a side
+                // whose syndrome now matches becomes the estimate bit's
+                // deterministic message ({.5,0,.5,0} / {0,.5,0,.5} on the X
+                // side, {.5,.5,0,0} / {0,0,.5,.5} on the Z side)
+                for (int s = 0; s < 2; ++s) {
+                    if (!(s == 0 ? okx : okz) || frozen[s]) continue;
+                    frozen[s] = true;
+                    const DevSide &sd = g.side[s];
+                    const size_t m = static_cast<size_t>(sd.m);
+                    for (int c = tid; c < sd.m; c += bs) {
+                        const int b = sd.chk_off[c], deg = sd.chk_off[c + 1] - b;
+                        for (int i = 0; i < deg; ++i) {
+                            const int st = sp.est[sd.edge_var[b + i]];
+                            const int bit = s == 0 ? (st & 1) : (st >> 1);
+                            double2 o;
+                            o.x = bit ? 0.0 : 0.5;
+                            o.y = bit ? 0.5 : 0.0;
+                            *reinterpret_cast<double2 *>(sp.c2v[s] + 2 * i * m + 2 * c) = o;
+                        }
+                    }
+                }
+                __syncthreads();
+                if (okx && okz) {
+                    iterations = iter + 1;
+                    matched = 1;
+                    break;
+                }
+                continue;
+            }
             int bad = 0;
             for (int c = tid; c < mx && !bad; c += bs)
                 bad |= check_parity<MAXC>(g.side[0], 0, c, sp.est) != static_cast<int>((tsx[c >> 5] >> (c & 31)) & 1u);
@@ -165,7 +207,7 @@ size_t decode_f64_scratch_bytes(const DevGraph &g, int slots) {
 
 void launch_decode_f64(const DevGraph &g, TrialPlanes tp, int count, const NoiseTab &nt, int max_iter,
                        double damping, int slots, int block, double *scratch, int *counter,
-                       hipStream_t st) {
+                       hipStream_t st, bool freeze) {
     if (count <= 0) return;
     size_t stride = decode_f64_scratch_bytes(g, 1) / sizeof(double);
     int grid = slots < count ? slots : count;
@@ -181,6 +223,19 @@ void launch_decode_f64(const DevGraph &g, TrialPlanes tp, int count, const Noise
 #define JBP_F64(C, V, PR, Q)                                                                                \
     hipLaunchKernelGGL((decode_f64_kernel<C, V, PR, Q>), dim3(grid), dim3(block), (PR && Q) ? lds : 0, st, g, \
                        tp, count, nt, max_iter, damping, scratch, stride, counter)
+    if (freeze) {
+#define JBP_F64F(C, V, PR)                                                                             \
+    hipLaunchKernelGGL((decode_f64_kernel<C, V, PR, false, true>), dim3(grid), dim3(block), 0, st, g, tp, \
+                       count, nt, max_iter, damping, scratch, stride, counter)
+        if (exact12 && pair) JBP_F64F(12, 3, true);
+        else if (exact12) JBP_F64F(12, 3, false);
+        else if (small && pair) JBP_F64F(16, 4, true);
+        else if (small) JBP_F64F(16, 4, false);
+        else if (pair) JBP_F64F(32, 16, true);
+        else JBP_F64F(32, 16, false);
+#undef JBP_F64F
+        return;
+    }
     if (qc_lds) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(decode_f64_kernel<12, 3, true, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));

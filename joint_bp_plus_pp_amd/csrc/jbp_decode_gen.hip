@@ -173,6 +173,20 @@ __device__ __forceinline__ void check_update(float (&mu)[MAXC], int deg, uint32_
     }
 }
 
+// Damping of a variable-to-check message, the reference's scaled_mix
+// (jointbp_ets.cpp:1903-1911): normalize((1 - d) new + d old) of the 4-ary
+// messages.  A check reads only the pair sums of its side, and mixing
+// commutes with forming them, so on the binary probabilities
+// (q0, q1) = (1 - d) (n0, n1) + d (o0, o1), returned as log2(q0 / q1).
+__device__ __forceinline__ float mix_llr(float mn, float mo, float d) {
+    const float en = fast_exp2(-fabsf(mn)), eo = fast_exp2(-fabsf(mo));
+    const float rn = __builtin_amdgcn_rcpf(1.0f + en), ro = __builtin_amdgcn_rcpf(1.0f + eo);
+    const float n0 = mn >= 0.0f ? rn : en * rn, n1 = mn >= 0.0f ? en * rn : rn;
+    const float o0 = mo >= 0.0f ? ro : eo * ro, o1 = mo >= 0.0f ? eo * ro : ro;
+    const float q0 = (1.0f - d) * n0 + d * o0, q1 = (1.0f - d) * n1 + d * o1;
+    return fast_log2(q0) - fast_log2(q1);
+}
+
 // One wavefront per group (lane = slot): decides which slots stop, records
 // their outcome, and hands the next trials to stopped and empty slots.
 __device__ void book_group(const GenState &s, const TrialPlanes &tp, int count, const NoiseTab &nt,
@@ -274,6 +288,14 @@ __device__ __forceinline__ void check_chunk(const DevGraph &g, const GenState &s
     const float *gb = (side == 0 ? s.gb[0] : s.gb[1]) + static_cast<size_t>(group) * g.n * 64 + lane;
     const uint64_t *ew = (side == 0 ? s.estw[0] : s.estw[1]) + static_cast<size_t>(group) * g.n;
     const uint64_t *sw = (side == 0 ? s.synw[0] : s.synw[1]) + static_cast<size_t>(group) * sd.m;
+    // damping: a slot's first step sends the prior message unmixed
+    float *mp = s.mprev[side] ? s.mprev[side] + static_cast<size_t>(group) * (sd.E + 1) * 64 + lane : nullptr;
+    auto damp = [&](float m, size_t e) {
+        if (!mp) return m;
+        const float v = fresh ? m : mix_llr(m, mp[e * 64], s.damping);
+        mp[e * 64] = v;
+        return v;
+    };
     uint64_t bad = 0;
     if constexpr (DEG > 0) {
         // Load phase for all CPW checks, then update and store: stores of one
@@ -299,7 +321,7 @@ __device__ __forceinline__ void check_chunk(const DevGraph &g, const GenState &s
                 pj ^= ew[v];
                 const float old = __uint_as_float(
                     __float_as_uint(lam[static_cast<size_t>(e0s[j] + i) * 64]) & keep);
-                mu[j][i] = gb[static_cast<size_t>(v) * 64] - old;
+                mu[j][i] = damp(gb[static_cast<size_t>(v) * 64] - old, static_cast<size_t>(e0s[j] + i));
             }
             par[j] = pj;
         }
@@ -326,7 +348,7 @@ __device__ __forceinline__ void check_chunk(const DevGraph &g, const GenState &s
                     par ^= ew[v];
                     const float old = __uint_as_float(
                         __float_as_uint(lam[static_cast<size_t>(e0 + i) * 64]) & keep);
-                    mu[i] = gb[static_cast<size_t>(v) * 64] - old;
+                    mu[i] = damp(gb[static_cast<size_t>(v) * 64] - old, static_cast<size_t>(e0 + i));
                 }
             }
             const uint64_t syn = sw[c];
@@ -475,10 +497,10 @@ void launch_var(const DevGraph &g, const GenState &s, hipStream_t st) {
 
 }  // namespace
 
-size_t gen_state_bytes(const DevGraph &g, int groups) {
+size_t gen_state_bytes(const DevGraph &g, int groups, bool damped) {
     const size_t G = static_cast<size_t>(groups), S = 64 * G;
     size_t b = 0;
-    b += S * (static_cast<size_t>(g.side[0].E) + g.side[1].E + 2) * sizeof(float);  // lam (+ zero edge)
+    b += (damped ? 2 : 1) * S * (static_cast<size_t>(g.side[0].E) + g.side[1].E + 2) * sizeof(float);  // lam (+ zero edge), mprev
     b += 2 * S * static_cast<size_t>(g.n) * sizeof(float);                      // gb
     b += 2 * G * static_cast<size_t>(g.n) * 8;                                   // estw
     b += G * (static_cast<size_t>(g.side[0].m) + g.side[1].m) * 8;               // synw
@@ -488,7 +510,7 @@ size_t gen_state_bytes(const DevGraph &g, int groups) {
 
 bool gen_supported(const DevGraph &g) { return g.max_cdeg <= 32 && g.n > 0; }
 
-void gen_carve(const DevGraph &g, int groups, void *base, GenState &s) {
+void gen_carve(const DevGraph &g, int groups, void *base, GenState &s, bool damped) {
     const size_t G = static_cast<size_t>(groups), S = 64 * G;
     char *p = static_cast<char *>(base);
     auto take = [&](size_t bytes) {
@@ -511,11 +533,13 @@ void gen_carve(const DevGraph &g, int groups, void *base, GenState &s) {
     s.prior32 = reinterpret_cast<float *>(take(S * kPrior32 * 4));
     s.fin = reinterpret_cast<int2 *>(take(S * 8));
     s.ctrl = reinterpret_cast<int *>(take(64));
+    for (int k = 0; k < 2; ++k)
+        s.mprev[k] = damped ? reinterpret_cast<float *>(take(S * (g.side[k].E + 1) * 4)) : nullptr;
 }
 
-size_t gen_carve_bytes(const DevGraph &g, int groups) {
+size_t gen_carve_bytes(const DevGraph &g, int groups, bool damped) {
     // gen_carve rounds every array up to 256 B
-    return gen_state_bytes(g, groups) + 16 * 256;
+    return gen_state_bytes(g, groups, damped) + 18 * 256;
 }
 
 int gen_decode(const DevGraph &g, const GenState &s, TrialPlanes tp, int count, const NoiseTab &nt,

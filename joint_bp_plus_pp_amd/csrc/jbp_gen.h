@@ -26,11 +26,15 @@ struct GenState {
     float *prior32 = nullptr;    // [groups][kPrior32][64]
     int2 *fin = nullptr;         // [S] (trial, slot) stopped this step
     int *ctrl = nullptr;         // [0] next trial, [1] trials finished, [2] fin entries
+    // damping (the reference's scaled_mix): each side's previous
+    // variable-to-check messages, [groups][E_side + 1][64]; null when off
+    float *mprev[2] = {nullptr, nullptr};
+    float damping = 0.0f;
 };
 
 bool gen_supported(const DevGraph &g);
-size_t gen_carve_bytes(const DevGraph &g, int groups);
-void gen_carve(const DevGraph &g, int groups, void *base, GenState &s);
+size_t gen_carve_bytes(const DevGraph &g, int groups, bool damped = false);
+void gen_carve(const DevGraph &g, int groups, void *base, GenState &s, bool damped = false);
 
 // Decode trials 0..count-1 of tp (syndrome planes in place) to est planes,
 // iterations and syn_ok.  h_ctrl: pinned host ints [2]; evs: two events.

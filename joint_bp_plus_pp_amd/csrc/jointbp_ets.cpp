@@ -144,7 +144,8 @@ void usage(const char *prog) {
               << "  --sx FILE --sz FILE   decode one syndrome pair (one 0/1 per line)\n"
               << "  --err FILE            decode the syndromes of one error (0=I,1=X,2=Z,3=Y per line)\n"
               << "Decoder:\n"
-              << "  --p N (0.05)  --max-iter N (50)  --damping R (0.0, FP64 only)\n"
+              << "  --p N (0.05)  --max-iter N (50)  --damping R (0.0)\n"
+              << "  --freeze-syn          freeze a side once its syndrome matches (FP64, --no-pp)\n"
               << "  --precision 32|64     FP32 messages (default) or the FP64 decoder that\n"
               << "                        reproduces the reference CPU decoder trial for trial\n"
               << "  --no-pp               disable post-processing (ETS / flip / OSD)\n"
@@ -408,10 +409,6 @@ int main(int argc, char **argv) {
                      "available on the device decoder.\n";
         return 1;
     }
-    if (damping > 0.0 && precision == JBP_FP32) {
-        std::cerr << "--damping runs on the FP64 decoder; add --precision 64.\n";
-        return 1;
-    }
     if (simulate && !err_path.empty()) {
         std::cerr << "--err cannot be used with --simulate\n";
         return 1;
@@ -460,9 +457,9 @@ int main(int argc, char **argv) {
         }
     }
 
-    if (freeze_syn) {
-        std::cerr << "--freeze-syn runs on the reference's CPU decoder only (no CPU decoder in "
-                     "this build).\n";
+    if (freeze_syn && (enable_pp || precision != JBP_FP64)) {
+        std::cerr << "--freeze-syn runs on the FP64 decoder with post-processing off; add "
+                     "--precision 64 --no-pp.\n";
         return 1;
     }
     jbp_ctx *ctx = nullptr;
@@ -599,6 +596,7 @@ int main(int argc, char **argv) {
                            warmup, interval, 0, 0, min_fails > 0 ? min_fails - pr.failures : 0};
         cfg.pp = enable_pp ? 1 : 0;
         cfg.flip_hist = flip_hist;
+        cfg.freeze_syn = freeze_syn ? 1 : 0;
         jbp_run_stats st;
         if (jbp_simulate(ctx, &cfg, &out, &st, err, sizeof(err)) != JBP_OK) {
             std::cerr << "CUDA decode failed: " << err << " (no CPU decoder in this build)\n";

@@ -144,10 +144,54 @@ def ref_cli_summary(args):
     return {"args": args, "returncode": out.returncode, "lines": keep}
 
 
+def xonly_decodes(lib, base, count, p, max_iter, damping):
+    """Decodes of X-only errors (each sampled error with its Z components
+    dropped: Y -> X, Z -> I) with --freeze-syn on and off.  The Z syndrome is
+    then zero and matches at the first iteration, so the Z side freezes while
+    the X side still decodes: the branch --freeze-syn changes."""
+    n = 9216
+    err = np.zeros(n, np.int32)
+    sx = np.zeros(2304, np.int32)
+    sz = np.zeros(2304, np.int32)
+    est = np.zeros(n, np.int32)
+    prior = np.zeros(4, np.float64)
+    lib.ref_prior(p, prior.ctypes.data)
+    rec = {k: [] for k in ("err_hash", "iterations", "syn_ok", "est_hash", "iterations_nofreeze",
+                           "est_hash_nofreeze")}
+    for t in range(1, count + 1):
+        lib.ref_sample(base, t, p, err.ctypes.data)
+        err &= 1
+        lib.ref_syndrome(err.ctypes.data, sx.ctypes.data, sz.ctypes.data)
+        for freeze, sfx in ((1, ""), (0, "_nofreeze")):
+            ok = C.c_int(0)
+            it = lib.ref_decode(sx.ctypes.data, sz.ctypes.data, prior.ctypes.data, max_iter, damping,
+                                freeze, est.ctypes.data, C.byref(ok))
+            rec["iterations" + sfx].append(int(it))
+            rec["est_hash" + sfx].append(fnv1a(est))
+            if freeze:
+                rec["syn_ok"].append(int(ok.value))
+        rec["err_hash"].append(fnv1a(err))
+    arrays = {k: np.array(v, dtype=np.uint64 if "hash" in k else np.int32) for k, v in rec.items()}
+    arrays["meta"] = np.array([base, 1, count, max_iter], dtype=np.int64)
+    arrays["p"] = np.array([p, damping], dtype=np.float64)
+    arrays["prior"] = prior.copy()
+    return arrays
+
+
+def freeze_fixtures(lib):
+    """--freeze-syn (jointbp_ets.cpp:1884-1901, 2526-2535) on X-only errors."""
+    np.savez_compressed(os.path.join(HERE, "decodes_xonly_p0.04_it60_freeze.npz"),
+                        **xonly_decodes(lib, 106, 48, 0.04, 60, 0.0))
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference checker first: make -C oracle ref")
     lib = load()
+    if sys.argv[1:] == ["freeze"]:
+        freeze_fixtures(lib)
+        print("freeze fixtures written to", HERE)
+        return
     with open(os.path.join(HERE, "code_summary.json"), "w") as f:
         json.dump(code_summary(lib), f, indent=1)
     np.savez_compressed(os.path.join(HERE, "trials_p0.05_it50.npz"),
@@ -158,6 +202,7 @@ def main():
                         **per_trial(lib, 3, 1, 64, 0.025, 200))
     np.savez_compressed(os.path.join(HERE, "trials_p0.05_it50_damp0.3.npz"),
                         **per_trial(lib, 106, 1, 64, 0.05, 50, damping=0.3))
+    freeze_fixtures(lib)
     agg = {"cfg1_p0.05_it50_seed106_1000": aggregate(lib, 106, 1, 1000, 0.05, 50),
            "p0.04_it1000_seed106_2000": aggregate(lib, 106, 1, 2000, 0.04, 1000)}
     # The reference program's own Run Summary on the first workload, and a

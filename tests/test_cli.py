@@ -54,8 +54,10 @@ def test_argument_errors():
     assert run(["--no-pp", "--trials", "3"]).returncode == 1
     assert run(["--simulate", "--no-pp", "--trial-index", "4", "--trials", "2"]).returncode == 1
     assert run(["--bogus"]).returncode == 1
-    # --freeze-syn is refused in every mode, never silently dropped
-    for mode in (["--simulate", "--no-pp", "--trials", "2"], ["--simulate", "--trials", "2"]):
+    # --freeze-syn runs on the FP64 decoder with post-processing off; other
+    # modes refuse it (never a silent drop)
+    for mode in (["--simulate", "--no-pp", "--trials", "2"], ["--simulate", "--trials", "2"],
+                 ["--simulate", "--trials", "2", "--precision", "64"]):
         out = run(mode + ["--freeze-syn"])
         assert out.returncode == 1 and "--freeze-syn" in out.stderr, mode
     bad = subprocess.run([CLI, "--params", "/nonexistent"], capture_output=True, text=True)

@@ -250,3 +250,37 @@ def test_hbm_engine_graph_replay_equals_eager(main_decoder, small_hgp):
             _, e = d.simulate(51 + rep, n, p, max_iter=80, precision=jbp.FP32,
                               engine=jbp.ENGINE_HBM, slots=128, want_est=True)
             assert per_trial_equal(g, e) and np.array_equal(g["est"], e["est"])
+
+
+def test_fp32_damping_matches_fp64_fer(main_decoder):
+    # damping 0.3 (the reference's scaled_mix, here on the HBM engine's
+    # binary message pairs) against the FP64 decoder, which is per-trial the
+    # reference's (trials_p0.05_it50_damp0.3 fixture), on the same indices
+    import math
+    from conftest import golden_npz
+    z = golden_npz("trials_p0.05_it50_damp0.3.npz")
+    n = 4096
+    print("fp32 damping 0.3", flush=True)
+    s32, r32 = main_decoder.simulate(106, n, 0.05, max_iter=50, damping=0.3, precision=jbp.FP32)
+    s64, r64 = main_decoder.simulate(106, n, 0.05, max_iter=50, damping=0.3, precision=jbp.FP64)
+    m = len(z["success"])
+    assert np.array_equal(r64["success"][:m], z["success"])
+    assert np.array_equal(r64["iterations"][:m], z["iterations"])
+    k = s64["failures"]
+    phat, zz = k / n, 1.96
+    den = 1 + zz * zz / n
+    c = (phat + zz * zz / (2 * n)) / den
+    h = zz * math.sqrt((phat * (1 - phat) + zz * zz / (4 * n)) / n) / den
+    fer32 = s32["failures"] / n
+    agree = float(np.mean(r32["success"] == r64["success"]))
+    same_it = float(np.mean(r32["iterations"] == r64["iterations"]))
+    print("damping 0.3: FER fp32 %.4f fp64 %.4f ci95 [%.4f, %.4f]; agreement %.4f, same "
+          "iterations %.4f; avg_iter %.3f vs %.3f" % (fer32, k / n, c - h, c + h, agree, same_it,
+                                                      s32["total_iters"] / n, s64["total_iters"] / n),
+          flush=True)
+    assert c - h <= fer32 <= c + h
+    assert agree >= 0.98 and same_it >= 0.8
+    # the LDS engine keeps no previous messages: it refuses damping
+    with pytest.raises(jbp.JbpError):
+        main_decoder.simulate(106, 64, 0.05, max_iter=50, damping=0.3, precision=jbp.FP32,
+                              engine=jbp.ENGINE_LDS)

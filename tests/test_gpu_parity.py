@@ -238,3 +238,42 @@ def test_pipelined_run_equals_serial_run(main_decoder):
     sb, b = main_decoder.simulate(106, n, 0.05, max_iter=100, precision=jbp.FP32, min_fails=limit)
     assert sa["done"] == sb["done"] < n and sa["failures"] == sb["failures"] == limit
     assert np.array_equal(a["success"], b["success"])
+
+
+def test_fp64_freeze_syn_matches_reference(main_decoder, main_code_oracle):
+    # --freeze-syn (jointbp_ets.cpp:1884-1901, 2526-2535) on the FP64 decoder:
+    # per decode the reference's iterations, match and estimate, on X-only
+    # errors (the Z side matches and freezes first); freezing changes some
+    # of these decodes, and FP32 refuses it
+    z = golden_npz("decodes_xonly_p0.04_it60_freeze.npz")
+    base, _, count, max_iter = map(int, z["meta"])
+    p, damping = z["p"]
+    prior = jbp.prior_from_p(p)
+    for i in range(count):
+        err = main_code_oracle.sample(base, 1 + i, p) & 1
+        sx, sz = main_code_oracle.syndrome(err)
+        print("freeze decode", i, flush=True)
+        est, it, ok, _ = main_decoder.decode(sx, sz, prior, max_iter=max_iter, precision=jbp.FP64,
+                                             freeze_syn=True, damping=damping)
+        assert (it, int(ok), fnv1a(est)) == (int(z["iterations"][i]), int(z["syn_ok"][i]),
+                                             int(z["est_hash"][i])), i
+        est, it, ok, _ = main_decoder.decode(sx, sz, prior, max_iter=max_iter, precision=jbp.FP64,
+                                             damping=damping)
+        assert (it, fnv1a(est)) == (int(z["iterations_nofreeze"][i]), int(z["est_hash_nofreeze"][i]))
+    with pytest.raises(jbp.JbpError):
+        main_decoder.decode(sx, sz, prior, max_iter=10, precision=jbp.FP32, freeze_syn=True)
+
+
+def test_fp64_freeze_syn_simulate_matches_oracle(main_decoder, main_code_oracle):
+    # the --simulate path with --freeze-syn: per trial the oracle's decoder
+    # with the same flag
+    n, p, it = 48, 0.05, 50
+    _, r = main_decoder.simulate(106, n, p, max_iter=it, precision=jbp.FP64, freeze_syn=True,
+                                 want_est=True)
+    prior = jbp.prior_from_p(p)
+    for t in range(n):
+        err = main_code_oracle.sample(106, 1 + t, p)
+        sx, sz = main_code_oracle.syndrome(err)
+        est, k, ok = main_code_oracle.decode(sx, sz, prior, it, 0.0, freeze_syn=True)
+        assert (r["iterations"][t], bool(r["syn_match"][t])) == (k, ok)
+        assert np.array_equal(r["est"][t], est)

@@ -182,3 +182,26 @@ def test_planted_stabilizer_is_success(main_code_oracle):
     est2 = err.copy()
     est2[row[:-1]] ^= 1  # drop one qubit: not a stabilizer
     assert not c.stab_equiv(err, est2)
+
+
+def test_freeze_syn_matches_reference(oracle, main_code_oracle):
+    # --freeze-syn (jointbp_ets.cpp:1884-1901, 2526-2535) on X-only errors:
+    # per decode the reference's iterations, match and estimate, with and
+    # without freezing (the fixture holds decodes where freezing changes them)
+    z = golden_npz("decodes_xonly_p0.04_it60_freeze.npz")
+    base, _, count, max_iter = map(int, z["meta"])
+    p, damping = z["p"]
+    prior = oracle.prior(p)
+    changed = 0
+    for i in range(count):
+        err = main_code_oracle.sample(base, 1 + i, p) & 1
+        assert fnv1a(err) == int(z["err_hash"][i])
+        sx, sz = main_code_oracle.syndrome(err)
+        est, it, ok = main_code_oracle.decode(sx, sz, prior, max_iter, damping, freeze_syn=True)
+        assert (it, int(ok), fnv1a(est)) == (int(z["iterations"][i]), int(z["syn_ok"][i]),
+                                             int(z["est_hash"][i])), i
+        est, it, ok = main_code_oracle.decode(sx, sz, prior, max_iter, damping, freeze_syn=False)
+        assert (it, fnv1a(est)) == (int(z["iterations_nofreeze"][i]), int(z["est_hash_nofreeze"][i]))
+        changed += int(z["iterations"][i] != z["iterations_nofreeze"][i] or
+                       z["est_hash"][i] != z["est_hash_nofreeze"][i])
+    assert changed >= 2
