@@ -2,14 +2,16 @@
 """Throughput of the --simulate --no-pp path on one MI355X.
 
 Headline workload: the included girth-8 (3,12)-regular [[9216,4612]] code,
-i.i.d. depolarizing noise at p = 0.04, max-iter 1000, FP32.  A step is one
-call over --trials (65,536) seeded trials: each is sampled on the device
+i.i.d. depolarizing noise at p = 0.04, max-iter 1000, FP32.  A step is a
+batch of --trials (65,536) seeded trials: each is sampled on the device
 (seed = splitmix64(base ^ idx*phi), mt19937_64), its syndromes formed, then
 decoded by the learned min-sum decoder (T = 25 learned iterations; shipped
 weights trained by this project's own kernels), with every trial it leaves
 unconverged decoded again by plain FP32 BP from the prior up to max-iter
-1000, and judged by the stabilizer-residual test.  Launches of --chunk
-(16,384) trials alternate two streams.  The same trial indices are then run
+1000, and judged by the stabilizer-residual test.  The timed steps run as
+one simulate call (--steps-per-call to split them), in launches of --chunk
+(16,384) trials pipelined over four streams (sampling, two decode streams,
+fallback/residual/copies) and eight buffer sets.  The same trial indices are then run
 by plain batched FP32 BP: `value` is the learned decoder's trials/s, with
 plain BP's trials/s, both FERs and their Wilson intervals ("matched FER":
 the learned FER no higher than the upper end of plain BP's 95% interval;
@@ -451,7 +453,9 @@ def main():
                          "hypergraph-product code HGP(150x200) on the HBM-resident decoder")
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--trials", type=int, default=65536, help="trials per step (one call)")
+    ap.add_argument("--trials", type=int, default=65536, help="trials per step")
+    ap.add_argument("--steps-per-call", type=int, default=0,
+                    help="steps per simulate call in the timed region (0: all steps in one call)")
     ap.add_argument("--chunk", type=int, default=16384,
                     help="trials per persistent launch; launches alternate two streams")
     ap.add_argument("--roofline-steps", type=int, default=20,
@@ -519,18 +523,26 @@ def main():
         dec.simulate(args.seed, N, args.p, max_iter=args.max_iter, first_trial=wfirst + w * N,
                      precision=32, per_trial=False, chunk=args.chunk)
 
+    # the timed steps run as calls of --steps-per-call steps each (default:
+    # one call over all of them, as the reference's --simulate --trials runs
+    # its whole trial count in one loop)
+    per_call = args.steps if args.steps_per_call <= 0 else min(args.steps_per_call, args.steps)
+
     def run(learned):
         done = fails = iters = 0
         nxt = 1
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        left = args.steps
+        while left > 0:
+            k = min(per_call, left)
             if learned:
-                st, _ = dec.simulate_learned(model, args.seed, N, args.p, max_iter=args.max_iter,
+                st, _ = dec.simulate_learned(model, args.seed, k * N, args.p, max_iter=args.max_iter,
                                              first_trial=nxt, per_trial=False, chunk=args.chunk)
             else:
-                st, _ = dec.simulate(args.seed, N, args.p, max_iter=args.max_iter, first_trial=nxt,
+                st, _ = dec.simulate(args.seed, k * N, args.p, max_iter=args.max_iter, first_trial=nxt,
                                      precision=32, per_trial=False, chunk=args.chunk)
-            nxt += N
+            nxt += k * N
+            left -= k
             done += st["done"]
             fails += st["failures"]
             iters += st["total_iters"]
@@ -572,8 +584,16 @@ def main():
                 "kernel_ms_per_launch": round(sum(ms_ms) / n_r, 4), "launches": n_r,
                 "trial_iterations_per_s": round(ms_rate, 1),
                 "streaming_equivalent_gbs": round(ms_rate * BYTES_PER_ITER / 1e9, 1)}
+    # the sustained (pipelined) plain rate against one launch at the median
+    # launch time: what stragglers and the sampling/residual work around the
+    # decoder cost the long run
+    med_rate = args.chunk / (float(np.median(pl_ms)) / 1e3)
     plain_roof = {"kernel": "decode_qc_f32_kernel (plain sum-product, FP32)",
                   "kernel_ms_per_launch": round(sum(pl_ms) / n_r, 4),
+                  "median_launch_ms": round(float(np.median(pl_ms)), 4),
+                  "max_launch_ms": round(float(np.max(pl_ms)), 4),
+                  "median_launch_trials_per_s": round(med_rate, 1),
+                  "sustained_over_median_launch": round((p_done / p_el) / med_rate, 4),
                   "trial_iterations_per_s": round(pl_rate, 1),
                   "streaming_equivalent_gbs": round(pl_rate * BYTES_PER_ITER / 1e9, 1)}
     if prof:
@@ -621,14 +641,14 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "learned min-sum BP (T=%d, shipped weights) + plain FP32 BP fallback to "
-                        "max-iter %d, included [[9216,4612]] code, p=%g, %d trials per step in "
-                        "launches of %d over two streams, seed %d; plain FP32 BP on the same "
-                        "trial indices beside it" % (T, args.max_iter, args.p, N, args.chunk,
-                                                     args.seed),
+                        "max-iter %d, included [[9216,4612]] code, p=%g, %d trials per step, "
+                        "%d steps per simulate call, in launches of %d trials pipelined over "
+                        "four streams, seed %d; plain FP32 BP on the same trial indices beside "
+                        "it" % (T, args.max_iter, args.p, N, per_call, args.chunk, args.seed),
             "decoder": "learned min-sum, FP16 (X, Z) message pairs, LDS-resident; fallback plain "
                        "FP32 sum-product",
             "weights": os.path.relpath(LEARNED_MS_FILE, ROOT),
-            "trials_per_step": N, "launch_trials": args.chunk, "p": args.p,
+            "trials_per_step": N, "steps_per_call": per_call, "launch_trials": args.chunk, "p": args.p,
             "max_iter": args.max_iter, "T": T,
             "trials": l_done, "avg_iter": round(l_iters / l_done, 4),
             "fer": l_fails / l_done, "fer_ci95": [llo, lhi],

@@ -235,6 +235,20 @@ int ensure_chunk(jbp_ctx *ctx, int count, char *err, size_t errlen) {
     return JBP_OK;
 }
 
+int ensure_mt(jbp_ctx *ctx, int count, char *err, size_t errlen) {
+    // generator states for up to one largest chunk at a time (longer
+    // sampling calls run in pieces)
+    const int want = std::max(1, std::min(count, kDefaultChunk));
+    if (want <= ctx->mt_cap) return JBP_OK;
+    if (ctx->d_mt) (void)hipFree(ctx->d_mt);
+    ctx->d_mt = nullptr;
+    ctx->mt_cap = 0;
+    HIPCHK(hipMalloc(reinterpret_cast<void **>(&ctx->d_mt),
+                     static_cast<size_t>(want) * kMtWords * sizeof(uint64_t)));
+    ctx->mt_cap = want;
+    return JBP_OK;
+}
+
 int ensure_states(jbp_ctx *ctx, size_t elems, char *err, size_t errlen) {
     if (elems <= ctx->d_states_cap) return JBP_OK;
     if (ctx->d_states) (void)hipFree(ctx->d_states);
@@ -491,16 +505,20 @@ int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int wa
 }
 
 void free_pipe(jbp_ctx *ctx) {
+    for (hipStream_t &st : ctx->pipe_st)
+        if (st) (void)hipStreamSynchronize(st);
     for (PipeBuf &b : ctx->pipe) {
-        if (b.st) (void)hipStreamSynchronize(b.st);
         for (void *p : b.allocs) (void)hipFree(p);
         b.allocs.clear();
         if (b.h_iters) (void)hipHostFree(b.h_iters);
         if (b.h_flags) (void)hipHostFree(b.h_flags);
         for (auto &e : b.ev)
             if (e) (void)hipEventDestroy(e);
-        if (b.st) (void)hipStreamDestroy(b.st);
         b = PipeBuf();
+    }
+    for (hipStream_t &st : ctx->pipe_st) {
+        if (st) (void)hipStreamDestroy(st);
+        st = nullptr;
     }
 }
 
@@ -534,21 +552,32 @@ int ensure_pipe(jbp_ctx *ctx, int cap, char *err, size_t errlen) {
         if ((rc = alloc(c * 4, &p))) return rc; b.keys = static_cast<int *>(p);
         if ((rc = alloc(c * 4, &p))) return rc; b.order = static_cast<int *>(p);
         if ((rc = alloc((c + 4) * 4, &p))) return rc; b.fb = static_cast<int *>(p);
-        HIPCHK(hipStreamCreate(&b.st));
         for (auto &e : b.ev) HIPCHK(hipEventCreate(&e));
         HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&b.h_iters), c * 4));
         HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&b.h_flags), 3 * c));
         b.cap = cap;
     }
+    for (hipStream_t &st : ctx->pipe_st) HIPCHK(hipStreamCreate(&st));
     return JBP_OK;
 }
 
-// Runs of the LDS-resident FP32 decoder longer than one launch: chunks of
-// kPipeChunk trials alternate between two streams and two buffer sets, so
-// chunk j+1 is sampled while chunk j decodes and its decoder's workgroups
-// take over compute units as chunk j's last trials finish.  Outcomes are
-// reduced on the host in chunk order (min_fails cuts at the same index as
-// the serial loop).
+// Runs of the LDS-resident FP32 decoder longer than one launch, as chunks of
+// C trials over kPipeDepth buffer sets and three streams (three hardware
+// queues of the four a process gets by default, the fourth left to the
+// runtime's own copies):
+//   sampling   (ctx->stream): seed, sample, weigh and order chunk j; these
+//              one-wavefront workgroups run beside the decoder workgroup
+//              of the chunk before
+//   decode     (pipe_st[0]): after chunk j is sampled, its decoder launch
+//   finishing  (pipe_st[1]): after chunk j's decoder launch, the min-sum
+//              decoder's fallback launch, the residual test and the outcome
+//              copies
+// The host reduces outcomes in chunk order (min_fails cuts at the same index
+// as the serial loop) and queues chunk j+kPipeDepth once chunk j's outcomes
+// are read, so a chunk whose slowest trial runs long (the fallback of a
+// trial that reaches max-iter holds one compute unit for tens of ms) delays
+// nothing but its own outcome: the decode stream keeps up to kPipeDepth - 1
+// later chunks going meanwhile.
 int simulate_pipelined(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_run_stats &st, int np,
                        const LearnedInfer *learned, jbp_trial_out *pt, char *err, size_t errlen) {
     const DevGraph &g = ctx->dg;
@@ -556,61 +585,81 @@ int simulate_pipelined(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_run_stats &st, 
     const int C = cfg->chunk > 0 ? cfg->chunk : kPipeChunk;
     int rc = ensure_pipe(ctx, C, err, errlen);
     if (rc) return rc;
+    if ((rc = ensure_mt(ctx, C, err, errlen))) return rc;
     const int64_t nch = (total + C - 1) / C;
     const bool judged = ctx->have_logicals || ctx->have_basis;
     const int interval = std::max(1, cfg->check_interval);
     const int warmup = std::max(0, cfg->check_warmup);
+    const bool ms_kind = learned_ms(learned);
+    hipStream_t sst = ctx->stream, dst = ctx->pipe_st[0], fst = ctx->pipe_st[1];
+    // plain BP: the decoder launch stops at the first scheduled syndrome test
+    // at or after kStragglerIter iterations, and the finishing stream decodes
+    // the trials still unconverged there again from the prior up to max_iter
+    // (so a trial that runs to max_iter holds up only its own chunk's
+    // outcome, not the decode stream)
+    int cap = std::max(kStragglerIter, warmup + interval);
+    if (cap > warmup) cap = warmup + interval * ((cap - warmup + interval - 1) / interval);
+    if (cap >= cfg->max_iter) cap = 0;
     // the previous call's work on the context stream (e.g. table uploads) first
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipStreamSynchronize(sst));
     auto enqueue = [&](int64_t j) -> int {
-        PipeBuf &b = ctx->pipe[j & 1];
+        PipeBuf &b = ctx->pipe[j % kPipeDepth];
         const int cnt = static_cast<int>(std::min<int64_t>(C, total - j * C));
         const int64_t first = cfg->first_trial + j * C;
         NoiseTab nt = ctx->noise;
         nt.phase = static_cast<int>(((first % np) + np) % np);
-        HIPCHK(hipEventRecord(b.ev[0], b.st));
-        launch_sample(g, b.tp, cfg->base_seed, first, cnt, nt, b.st);
+        HIPCHK(hipEventRecord(b.ev[0], sst));
+        launch_sample(g, b.tp, cfg->base_seed, first, cnt, nt, sst, ctx->d_mt, ctx->mt_cap);
         TrialPlanes tpo = b.tp;
         if (cfg->schedule == 0) {
-            launch_order_by_weight(g, b.tp, cnt, b.keys, b.order, b.st);
+            launch_order_by_weight(g, b.tp, cnt, b.keys, b.order, sst);
             tpo.order = b.order;
         }
-        launch_fill_i32(b.counter, 0, 1, b.st);
-        HIPCHK(hipEventRecord(b.ev[1], b.st));
+        launch_fill_i32(b.counter, 0, 1, sst);
+        HIPCHK(hipEventRecord(b.ev[5], sst));
+        HIPCHK(hipStreamWaitEvent(dst, b.ev[5], 0));
+        HIPCHK(hipEventRecord(b.ev[1], dst));
         launch_decode_qc_f32(g, ctx->qcp, ctx->d_qc_tables, tpo, cnt, nt, cfg->max_iter, warmup,
-                             interval, std::max(1, ctx->cus), b.counter, b.st, learned, b.fb,
-                             learned_ms(learned) ? b.ev[4] : nullptr);
-        HIPCHK(hipEventRecord(b.ev[2], b.st));
-        if (ctx->have_logicals) launch_logical(g, ctx->lg[0], ctx->lg[1], b.tp, cnt, b.st);
-        else if (ctx->have_basis) launch_residual(g, ctx->basis[0], ctx->basis[1], b.tp, cnt, b.st);
-        else launch_logical(g, DevLogicals(), DevLogicals(), b.tp, cnt, b.st);
+                             interval, std::max(1, ctx->cus), b.counter, dst, learned, b.fb,
+                             b.ev[4], fst, learned ? 0 : cap);
+        if (!ms_kind) {
+            HIPCHK(hipEventRecord(b.ev[4], dst));
+            HIPCHK(hipStreamWaitEvent(fst, b.ev[4], 0));
+        }
+        HIPCHK(hipEventRecord(b.ev[2], fst));
+        if (ctx->have_logicals) launch_logical(g, ctx->lg[0], ctx->lg[1], b.tp, cnt, fst);
+        else if (ctx->have_basis) launch_residual(g, ctx->basis[0], ctx->basis[1], b.tp, cnt, fst);
+        else launch_logical(g, DevLogicals(), DevLogicals(), b.tp, cnt, fst);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(b.h_iters, b.tp.iters, cnt * 4, hipMemcpyDeviceToHost, b.st));
+        HIPCHK(hipMemcpyAsync(b.h_iters, b.tp.iters, cnt * 4, hipMemcpyDeviceToHost, fst));
         // the three flag arrays sit b.cap apart (the buffers may be larger than C)
-        HIPCHK(hipMemcpyAsync(b.h_flags, b.tp.syn_ok, 3 * static_cast<size_t>(b.cap), hipMemcpyDeviceToHost, b.st));
-        HIPCHK(hipEventRecord(b.ev[3], b.st));
+        HIPCHK(hipMemcpyAsync(b.h_flags, b.tp.syn_ok, 3 * static_cast<size_t>(b.cap), hipMemcpyDeviceToHost, fst));
+        HIPCHK(hipEventRecord(b.ev[3], fst));
         return JBP_OK;
     };
     double dev_ms = 0.0;
     bool stop = false;
     int64_t done = 0;
-    for (int64_t j = 0; j < std::min<int64_t>(2, nch); ++j)
+    for (int64_t j = 0; j < std::min<int64_t>(kPipeDepth, nch); ++j)
         if ((rc = enqueue(j))) return rc;
     for (int64_t j = 0; j < nch && !stop; ++j) {
-        PipeBuf &b = ctx->pipe[j & 1];
+        PipeBuf &b = ctx->pipe[j % kPipeDepth];
         const int cnt = static_cast<int>(std::min<int64_t>(C, total - j * C));
         HIPCHK(hipEventSynchronize(b.ev[3]));
-        float sms = 0.f, dms = 0.f, rms = 0.f;
-        HIPCHK(hipEventElapsedTime(&sms, b.ev[0], b.ev[1]));
+        float sms = 0.f, dms = 0.f;
+        HIPCHK(hipEventElapsedTime(&sms, b.ev[0], b.ev[5]));
         HIPCHK(hipEventElapsedTime(&dms, b.ev[1], b.ev[2]));
-        HIPCHK(hipEventElapsedTime(&rms, b.ev[2], b.ev[3]));
         ctx->last_ms[0] = sms;
         ctx->last_ms[1] = dms;
-        ctx->last_ms[2] = rms;
         ctx->launches[0]++;
         ctx->launches[1]++;
+        if (judged) {
+            float rms = 0.f;
+            HIPCHK(hipEventElapsedTime(&rms, b.ev[2], b.ev[3]));
+            ctx->last_ms[2] = rms;
+        }
         ctx->launches[2]++;
-        if (learned_ms(learned)) {
+        if (ms_kind) {
             float lms = 0.f;
             HIPCHK(hipEventElapsedTime(&lms, b.ev[1], b.ev[4]));
             ctx->last_ms[4] = lms;
@@ -643,9 +692,12 @@ int simulate_pipelined(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_run_stats &st, 
             }
         }
         done += take;
-        if (!stop && j + 2 < nch && (rc = enqueue(j + 2))) return rc;
+        if (!stop && j + kPipeDepth < nch && (rc = enqueue(j + kPipeDepth))) return rc;
     }
-    for (PipeBuf &b : ctx->pipe) HIPCHK(hipStreamSynchronize(b.st));
+    // chunks still in flight after an early stop finish before the buffers
+    // are reused
+    HIPCHK(hipStreamSynchronize(sst));
+    for (hipStream_t s : ctx->pipe_st) HIPCHK(hipStreamSynchronize(s));
     st.done = done;
     st.device_ms = dev_ms;
     return JBP_OK;
@@ -731,6 +783,7 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
     const int chunk = static_cast<int>(std::min<int64_t>(std::max<int64_t>(total, 1), kDefaultChunk));
     int rc = ensure_chunk(ctx, chunk, err, errlen);
     if (rc) return rc;
+    if ((rc = ensure_mt(ctx, chunk, err, errlen))) return rc;
     // outcome copies land in pinned memory: iterations, and the three flag
     // arrays (one block on the device) in one copy
     const int cap = ctx->chunk_cap;
@@ -747,7 +800,7 @@ int simulate_impl(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_r
         // (no host wait between the launches of a chunk: the sample and
         // decode timings are read after the chunk's one synchronisation)
         HIPCHK(hipEventRecord(ctx->evt[0], ctx->stream));
-        launch_sample(g, ctx->tp, cfg->base_seed, first, cnt, nt, ctx->stream);
+        launch_sample(g, ctx->tp, cfg->base_seed, first, cnt, nt, ctx->stream, ctx->d_mt, ctx->mt_cap);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(ctx->evt[1], ctx->stream));
         if (cfg->schedule == 0) {
@@ -1216,6 +1269,7 @@ void jbp_destroy(jbp_ctx *ctx) {
     if (ctx->f64_scratch) (void)hipFree(ctx->f64_scratch);
     if (ctx->d_counter) (void)hipFree(ctx->d_counter);
     if (ctx->d_states) (void)hipFree(ctx->d_states);
+    if (ctx->d_mt) (void)hipFree(ctx->d_mt);
     if (ctx->d_ints) (void)hipFree(ctx->d_ints);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -1258,13 +1312,15 @@ int jbp_sample(jbp_ctx *ctx, uint64_t base_seed, int64_t first_trial, int count,
     HIPCHK(hipSetDevice(ctx->device));
     int rc = ensure_chunk(ctx, count, err, errlen);
     if (rc) return rc;
+    if ((rc = ensure_mt(ctx, count, err, errlen))) return rc;
     const DevGraph &g = ctx->dg;
     {
         const Prior pri = prior_from_p(p);
         if ((rc = set_noise(ctx, &pri, &p, 1, err, errlen))) return rc;
     }
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    launch_sample(g, ctx->tp, base_seed, first_trial, count, ctx->noise, ctx->stream);
+    launch_sample(g, ctx->tp, base_seed, first_trial, count, ctx->noise, ctx->stream, ctx->d_mt,
+                  ctx->mt_cap);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     if (err_out) {

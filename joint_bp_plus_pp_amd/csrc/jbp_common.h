@@ -132,8 +132,11 @@ struct NoiseTab {
 };
 
 // Launchers (defined in the .hip files).
+// mt_state: scratch of mt_cap x 312 64-bit words for the generators'
+// seeded states (count above mt_cap runs in pieces).
 void launch_sample(const DevGraph &g, TrialPlanes tp, uint64_t base_seed, int64_t first_trial,
-                   int count, const NoiseTab &nt, hipStream_t st);
+                   int count, const NoiseTab &nt, hipStream_t st, uint64_t *mt_state, int mt_cap);
+constexpr int kMtWords = 312;  // 64-bit words of one generator state
 void launch_syndromes(const DevGraph &g, TrialPlanes tp, int count, hipStream_t st);
 // order[0..count) = trial indices by decreasing syndrome weight (a counting
 // sort on a 12-bit key; keys: count ints of scratch).  Long decodes tend to
@@ -156,12 +159,17 @@ struct LearnedInfer;
 // learned == nullptr: plain BP.  Otherwise learned->T learned iterations,
 // then plain BP restarted on trials that did not converge.  The min-sum
 // kind runs as two launches (the learned phase queues its unconverged
-// trials in fb, count + 4 ints of device scratch, for the plain one).
+// trials in fb, count + 4 ints of device scratch, for the plain one).  Plain
+// BP with 0 < cap_iter < max_iter (and fb) also runs as two: every trial up
+// to cap_iter iterations, then the ones still unconverged from the prior up
+// to max_iter.  With fb_st the second launch goes on that stream, after
+// phase_end (recorded on st when the first launch ends; required then).
 void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_tables,
                           TrialPlanes tp, int count, const NoiseTab &nt, int max_iter, int check_warmup,
                           int check_interval, int slots, int *counter, hipStream_t st,
                           const LearnedInfer *learned = nullptr, int *fb = nullptr,
-                          hipEvent_t learned_end = nullptr);
+                          hipEvent_t phase_end = nullptr, hipStream_t fb_st = nullptr,
+                          int cap_iter = 0);
 bool qc_f32_supported(const QcParams &qc, int device, char *why, size_t len);
 void launch_fill_i32(int *ptr, int value, int count, hipStream_t st);
 void launch_unpack_states(const uint32_t *xbits, const uint32_t *zbits, int words, int n,

@@ -22,18 +22,25 @@ constexpr size_t kMaxBasisBytes = size_t(256) << 20;
 constexpr double kMaxBasisWork = 2e10;
 constexpr int kDefaultChunk = 65536;
 // Trials per persistent launch of the LDS-resident decoder when a run is
-// pipelined over two streams.
+// pipelined.
 constexpr int kPipeChunk = 8192;
+// Chunks in flight in a pipelined run (buffer sets).
+constexpr int kPipeDepth = 8;
+// Iterations after which a pipelined plain-BP launch hands a trial over to
+// the finishing stream (rounded up to a scheduled syndrome test).
+constexpr int kStragglerIter = 32;
 
-// One half of the two-stream pipeline: its own trial planes, decoder
-// counter, stream, events and pinned outcome arrays.
+// One chunk's buffer set in the pipeline: its own trial planes, decoder
+// counter, events and pinned outcome arrays.
 struct PipeBuf {
     int cap = 0;
     TrialPlanes tp;
     std::vector<void *> allocs;
     int *counter = nullptr;
-    hipStream_t st = nullptr;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // sample, decode start, decode end, done, learned-phase end
+    // [0] sampling start, [1] decode start, [2] decode end (fallback
+    // included), [3] outcomes copied, [4] first launch of the decode ended,
+    // [5] sampled and ordered
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     int *h_iters = nullptr;   // pinned
     uint8_t *h_flags = nullptr;  // pinned: ok | success | exact, cap each
     int *keys = nullptr;      // decode-order scratch
@@ -132,6 +139,8 @@ struct jbp_ctx {
     int f64_block = 256;
     int *d_counter = nullptr;
     int *d_keys = nullptr;    // decode order of the chunk (chunk_cap each)
+    uint64_t *d_mt = nullptr; // sampler generator states, mt_cap x 312 words
+    int mt_cap = 0;
     int *d_order = nullptr;
     int *d_fb = nullptr;      // min-sum decoder's fallback list (chunk_cap + 4)
     int *d_states = nullptr;  // unpack buffer
@@ -159,7 +168,11 @@ struct jbp_ctx {
     // it uses (buffers that a later call may reallocate)
     std::vector<uintptr_t> g1_key;
     uint32_t *h_stage = nullptr;   // pinned: syndrome words in, outcome out
-    jbp::PipeBuf pipe[2];
+    jbp::PipeBuf pipe[jbp::kPipeDepth];
+    // pipelined runs: sampling goes on `stream`, decoder launches on
+    // pipe_st[0]; fallback launches, the residual test and outcome copies on
+    // pipe_st[1]
+    hipStream_t pipe_st[2] = {nullptr, nullptr};
     // post-processing: host graph, trapping-set tables, device buffers
     jbp::pp::Graph pp_graph;
     jbp::pp::Tables pp_tables;
@@ -186,6 +199,9 @@ namespace jbp {
 
 void free_chunk(jbp_ctx *ctx);
 int ensure_chunk(jbp_ctx *ctx, int count, char *err, size_t errlen);
+// the sampler's generator-state scratch (ctx->d_mt), for count trials or
+// the largest chunk, whichever is smaller
+int ensure_mt(jbp_ctx *ctx, int count, char *err, size_t errlen);
 int ensure_states(jbp_ctx *ctx, size_t elems, char *err, size_t errlen);
 int ensure_ints(jbp_ctx *ctx, size_t elems, char *err, size_t errlen);
 Prior make_prior(const double p4[4]);

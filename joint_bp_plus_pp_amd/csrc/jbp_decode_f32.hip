@@ -638,14 +638,17 @@ __device__ __forceinline__ void trial_syndromes(const DevGraph &g, const TrialPl
 // converged by then restarts plain BP from the channel prior for up to
 // max_iter iterations, and its iteration count is li.T plus the plain
 // iterations.  count_ptr (plain only, may be null): the trial count lives on
-// the device (the fallback list of the min-sum decoder), and iter_offset is
-// added to every reported iteration count.
+// the device (a list of trials left over by an earlier launch), and
+// iter_offset is added to every reported iteration count.  fb_list (plain
+// only, may be null): a trial still unconverged after max_iter iterations is
+// appended there (fb_ctr[0] of them) instead of finishing, for a later
+// launch with a larger max_iter to decode again from the prior.
 template <int P, int J, int L, bool LEARNED, bool DAMPED>
 __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, TrialPlanes tp,
                                                          int count, NoiseTab nt, int max_iter,
                                                          int warmup, int interval, int *counter,
                                                          LearnedInfer li, const int *count_ptr,
-                                                         int iter_offset) {
+                                                         int iter_offset, int *fb_list, int *fb_ctr) {
     static_assert(P % 64 == 0 && P <= 1024, "one thread per block row, whole wavefronts");
     extern __shared__ float B[];  // [J][L][P] exchange buffer, then GZ[L][P]
     float *GZ = B + J * L * P;   // Z-side extrinsic base per variable, parked between passes
@@ -697,6 +700,11 @@ __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, 
             iterations = qc_phase<P, J, L, false, false>(B, GZ, s_cnt, &s_d, s_w, aw, lam, synx, synz,
                                                          pr, max_iter, warmup, interval, li, 0.0f,
                                                          estx, estz, matched) + iter_offset;
+            if (fb_list && !matched) {
+                if (t == 0) fb_list[atomicAdd(fb_ctr, 1)] = trial;
+                __syncthreads();
+                continue;
+            }
         }
         write_outcome<P, L>(g, tp, trial, estx, estz, iterations, matched);
         __syncthreads();
@@ -790,7 +798,8 @@ bool qc_f32_supported(const QcParams &qc, int device, char *why, size_t len) {
 void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_tables,
                           TrialPlanes tp, int count, const NoiseTab &nt, int max_iter, int check_warmup,
                           int check_interval, int slots, int *counter, hipStream_t st,
-                          const LearnedInfer *learned, int *fb, hipEvent_t learned_end) {
+                          const LearnedInfer *learned, int *fb, hipEvent_t phase_end,
+                          hipStream_t fb_st, int cap_iter) {
     if (count <= 0) return;
     QcDev dev;
     dev.P = qc.P;
@@ -802,37 +811,62 @@ void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_ta
     int warmup = check_warmup > 0 ? check_warmup : 0;
     LearnedInfer li;
     if (learned) li = *learned;
-    auto launch = [&](auto kern, const int *count_ptr, int offset, int *ctr, TrialPlanes planes) {
+    auto launch = [&](auto kern, const int *count_ptr, int offset, int *ctr, TrialPlanes planes,
+                      hipStream_t on, int iters, int *list, int *list_ctr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(768), lds, st, g, dev, planes, count, nt, max_iter,
-                           warmup, interval, ctr, li, count_ptr, offset);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(768), lds, on, g, dev, planes, count, nt, iters,
+                           warmup, interval, ctr, li, count_ptr, offset, list, list_ctr);
+    };
+    // the second launch of a two-phase decode: plain BP from the prior over
+    // the list the first launch left in fb (fb: [0] its count, [1] the work
+    // counter, fb + 4 the list), on fb_st after phase_end when given
+    auto second = [&](int offset) {
+        if (phase_end) (void)hipEventRecord(phase_end, st);
+        hipStream_t on = st;
+        if (fb_st && phase_end) {
+            (void)hipStreamWaitEvent(fb_st, phase_end, 0);
+            on = fb_st;
+        }
+        TrialPlanes fbp = tp;
+        fbp.order = fb + 4;
+        li = LearnedInfer();
+        launch(decode_qc_f32_kernel<768, 3, 12, false, false>, fb, offset, fb + 1, fbp, on, max_iter,
+               nullptr, nullptr);
     };
     if (learned && learned->T > 0 && learned->kind == kLearnedMS) {
         // min-sum phase, then plain BP over the trials it left unconverged
-        // (fb: [0] their count, [1] the plain decoder's work counter, [2..]
-        // none; fb + 4: the list, count entries)
-        int *fb_list = fb + 4;
         launch_fill_i32(fb, 0, 2, st);
         const size_t ms_lds = static_cast<size_t>(qc.J) * qc.L * qc.P * sizeof(uint32_t);
         auto ms = [&](auto kern) {
             (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(ms_lds));
             hipLaunchKernelGGL(kern, dim3(grid), dim3(768), ms_lds, st, g, dev, tp, count, nt, counter, li,
-                               fb_list, fb);
+                               fb + 4, fb);
         };
         if (learned->use_mlp) ms(decode_qc_ms_kernel<768, 3, 12, true>);
         else ms(decode_qc_ms_kernel<768, 3, 12, false>);
-        if (learned_end) (void)hipEventRecord(learned_end, st);
-        TrialPlanes fbp = tp;
-        fbp.order = fb_list;
-        li = LearnedInfer();
-        launch(decode_qc_f32_kernel<768, 3, 12, false, false>, fb, learned->T, fb + 1, fbp);
+        second(learned->T);
     } else if (learned && learned->T > 0) {
-        if (learned->use_mlp) launch(decode_qc_f32_kernel<768, 3, 12, true, true>, nullptr, 0, counter, tp);
-        else launch(decode_qc_f32_kernel<768, 3, 12, true, false>, nullptr, 0, counter, tp);
+        if (learned->use_mlp)
+            launch(decode_qc_f32_kernel<768, 3, 12, true, true>, nullptr, 0, counter, tp, st, max_iter,
+                   nullptr, nullptr);
+        else
+            launch(decode_qc_f32_kernel<768, 3, 12, true, false>, nullptr, 0, counter, tp, st, max_iter,
+                   nullptr, nullptr);
+    } else if (cap_iter > 0 && cap_iter < max_iter && fb) {
+        // plain BP in two launches: up to cap_iter iterations for every
+        // trial, then, for those still unconverged, the whole decode again
+        // from the prior up to max_iter.  BP is deterministic, so each trial
+        // ends exactly as in one launch; the first launch ends at the pace of
+        // the bulk instead of its slowest trial.
+        launch_fill_i32(fb, 0, 2, st);
+        launch(decode_qc_f32_kernel<768, 3, 12, false, false>, nullptr, 0, counter, tp, st, cap_iter,
+               fb + 4, fb);
+        second(0);
     } else {
-        launch(decode_qc_f32_kernel<768, 3, 12, false, false>, nullptr, 0, counter, tp);
+        launch(decode_qc_f32_kernel<768, 3, 12, false, false>, nullptr, 0, counter, tp, st, max_iter,
+               nullptr, nullptr);
     }
 }
 

@@ -333,7 +333,8 @@ int jbp_learned_train_step(jbp_learned *m, uint64_t base_seed, int64_t first_tri
     NoiseTab nt;
     nt.th = m->d_th;
     nt.n = batch;
-    launch_sample(ctx->dg, ctx->tp, base_seed, first_trial, batch, nt, ctx->stream);
+    if ((rc = ensure_mt(ctx, batch, err, errlen))) return rc;
+    launch_sample(ctx->dg, ctx->tp, base_seed, first_trial, batch, nt, ctx->stream, ctx->d_mt, ctx->mt_cap);
     HIPCHK(hipGetLastError());
     if ((rc = run_train<float>(m, batch, true, err, errlen))) return rc;
     launch_to_double(static_cast<const float *>(m->d_grad), m->d_grad64, m->count, ctx->stream);

@@ -8,10 +8,12 @@
 // values that round to 1.0 clamped to the largest double below 1), then the
 // depolarizing thresholds in double precision.
 //
-// One wavefront per trial: the 312-word Mersenne-Twister state lives in
-// LDS, the twist runs in three dependency phases across the 64 lanes, and
-// each lane tempers and classifies five of every 312 outputs.  Error and
-// syndrome bits are assembled with wave ballots into bit planes.
+// Seeding (312 dependent 64-bit steps per trial) runs first, one lane per
+// trial, into a scratch array of initial states.  Then one wavefront per
+// trial: the 312-word Mersenne-Twister state lives in LDS, the twist runs in
+// three dependency phases across the 64 lanes, and each lane tempers and
+// classifies five of every 312 outputs.  Error bits are set in LDS bit
+// planes by atomic OR; syndrome bits are assembled with wave ballots.
 //
 // Classification works on the raw 64-bit draw d.  The map d -> u (round to
 // double, scale by 2^-64, clamp below 1) is non-decreasing, and so are
@@ -21,6 +23,8 @@
 // arithmetic (first_draw_above below), so the device compares integers and the
 // result is identical to evaluating the doubles draw by draw.
 #include "jbp_common.h"
+
+#include <algorithm>
 
 namespace jbp {
 namespace {
@@ -49,15 +53,6 @@ __device__ __forceinline__ uint64_t temper(uint64_t z) {
     z ^= (z << 37) & 0xfff7eee000000000ull;
     z ^= (z >> 43);
     return z;
-}
-
-// OR a 64-bit ballot covering bits q0 .. q0+63 into an LDS bit plane.
-__device__ __forceinline__ void or_ballot(uint32_t *plane, int words, int q0, uint64_t m) {
-    int w = q0 >> 5;
-    uint32_t sh = static_cast<uint32_t>(q0 & 31);
-    plane[w] |= static_cast<uint32_t>(m << sh);
-    if (w + 1 < words) plane[w + 1] |= static_cast<uint32_t>(m >> (32 - sh));
-    if (sh && w + 2 < words) plane[w + 2] |= static_cast<uint32_t>(m >> (64 - sh));
 }
 
 // Syndromes (jointbp_ets.cpp:1834-1849) of one trial from its error bit
@@ -111,29 +106,41 @@ __global__ void __launch_bounds__(64) syndrome_kernel(DevGraph g, TrialPlanes tp
     write_syndromes<CW>(g, tp, trial, ex, ez);
 }
 
-// grid = count trials, block = 64 (one wavefront), dynamic LDS = 2*wn words.
+// std::mt19937_64 seeding of trials first_trial + t, t < count: state[0] =
+// splitmix64(base ^ idx*phi), state[i] = 6364136223846793005 *
+// (state[i-1] ^ (state[i-1] >> 62)) + i, into rows of kMtN words.  The chain
+// is serial within a trial, so each lane seeds its own trial.
+__global__ void __launch_bounds__(64) seed_kernel(uint64_t base_seed, int64_t first_trial, int count,
+                                                  uint64_t *state) {
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= count) return;
+    const uint64_t idx = static_cast<uint64_t>(first_trial + t);
+    uint64_t x = splitmix64(base_seed ^ (idx * 0x9e3779b97f4a7c15ull));
+    uint64_t *row = state + static_cast<size_t>(t) * kMtN;
+    row[0] = x;
+    for (int i = 1; i < kMtN; ++i) {
+        x = 6364136223846793005ull * (x ^ (x >> 62)) + static_cast<uint64_t>(i);
+        row[i] = x;
+    }
+}
+
+// Trials trial0 + blockIdx.x of the planes, from their seeded states
+// (state row blockIdx.x); grid = trials, block = 64 (one wavefront),
+// dynamic LDS = 2*(wn+1) words.
 template <int CW>
-__global__ void __launch_bounds__(64) sample_kernel(DevGraph g, TrialPlanes tp, uint64_t base_seed,
-                                                    int64_t first_trial, NoiseTab nt) {
+__global__ void __launch_bounds__(64) sample_kernel(DevGraph g, TrialPlanes tp, const uint64_t *state,
+                                                    int trial0, NoiseTab nt) {
     __shared__ uint64_t mt[kMtN];
     extern __shared__ uint32_t planes[];  // err_x | err_z, wn + 1 words each
     const int lane = threadIdx.x;
-    const int trial = blockIdx.x;
+    const int trial = trial0 + static_cast<int>(blockIdx.x);
     const int n = g.n, wn = g.wn;
     const DrawThresholds th = nt.th[nt.entry(trial)];
     uint32_t *ex = planes;
     uint32_t *ez = planes + wn + 1;
     for (int w = lane; w < 2 * (wn + 1); w += 64) planes[w] = 0u;
-
-    if (lane == 0) {
-        uint64_t idx = static_cast<uint64_t>(first_trial + trial);
-        uint64_t x = splitmix64(base_seed ^ (idx * 0x9e3779b97f4a7c15ull));
-        mt[0] = x;
-        for (int i = 1; i < kMtN; ++i) {
-            x = 6364136223846793005ull * (x ^ (x >> 62)) + static_cast<uint64_t>(i);
-            mt[i] = x;
-        }
-    }
+    const uint64_t *row = state + static_cast<size_t>(blockIdx.x) * kMtN;
+    for (int k = lane; k < kMtN; k += 64) mt[k] = row[k];
     __syncthreads();
 
     for (int base = 0; base < n; base += kMtN) {
@@ -168,24 +175,21 @@ __global__ void __launch_bounds__(64) sample_kernel(DevGraph g, TrialPlanes tp, 
         if (lane == 0) mt[kMtN - 1] = twist_word(mt[kMtN - 1], mt[0], mt[kMtM - 1]);
         __syncthreads();
 
+        // each lane sets its own qubits' bits (LDS atomic OR, no return
+        // value to wait for; most draws set nothing)
 #pragma unroll
         for (int j = 0; j < 5; ++j) {
-            int k = lane + 64 * j;
-            int q = base + k;
-            bool live = (k < kMtN) && (q < n);
-            int s = 0;
-            if (live) {
+            const int k = lane + 64 * j;
+            const int q = base + k;
+            if (k < kMtN && q < n) {
                 const uint64_t z = temper(mt[k]);
                 const bool b0 = (th.all_below & 1u) || z < th.t[0];
                 const bool b1 = (th.all_below & 2u) || z < th.t[1];
                 const bool b2 = (th.all_below & 4u) || z < th.t[2];
-                s = b0 ? 0 : (b1 ? 1 : (b2 ? 2 : 3));
-            }
-            uint64_t bx = __ballot(live && (s & 1));
-            uint64_t bz = __ballot(live && (s >> 1));
-            if (lane == 0 && base + 64 * j < n) {
-                or_ballot(ex, wn, base + 64 * j, bx);
-                or_ballot(ez, wn, base + 64 * j, bz);
+                const int s = b0 ? 0 : (b1 ? 1 : (b2 ? 2 : 3));
+                const uint32_t bit = 1u << (q & 31);
+                if (s & 1) atomicOr(&ex[q >> 5], bit);
+                if (s >> 1) atomicOr(&ez[q >> 5], bit);
             }
         }
         __syncthreads();
@@ -319,8 +323,11 @@ __global__ void __launch_bounds__(64) weight_kernel(DevGraph g, TrialPlanes tp, 
     if (lane == 0) keys[trial] = min(w >> shift, 4095);
 }
 
-// One block: counting sort of the keys, largest first.
+// One block: counting sort of the keys, largest first.  A pipelined run
+// orders chunk j+1 between the end of its sampling and the start of its
+// decoder launch, so the block is wide (16 wavefronts) to be short.
 constexpr int kOrderBlock = 1024;
+constexpr int kOrderKeys = 4096 / kOrderBlock;  // bins per thread in the scan
 __global__ void __launch_bounds__(kOrderBlock) order_kernel(const int *keys, int count, int *order) {
     __shared__ int bins[4096];
     __shared__ int part[kOrderBlock];
@@ -330,13 +337,13 @@ __global__ void __launch_bounds__(kOrderBlock) order_kernel(const int *keys, int
     for (int t = tid; t < count; t += kOrderBlock) atomicAdd(&bins[keys[t]], 1);
     __syncthreads();
     // start of key k in descending order = number of trials with key > k;
-    // thread i owns keys 4095-4i .. 4092-4i
-    int loc[4];
+    // thread i owns keys 4095 - kOrderKeys*i down to 4096 - kOrderKeys*(i+1)
+    int loc[kOrderKeys];
     int sum = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < kOrderKeys; ++j) {
         loc[j] = sum;
-        sum += bins[4095 - (4 * tid + j)];
+        sum += bins[4095 - (kOrderKeys * tid + j)];
     }
     part[tid] = sum;
     __syncthreads();
@@ -349,12 +356,12 @@ __global__ void __launch_bounds__(kOrderBlock) order_kernel(const int *keys, int
     const int before = part[tid] - sum;
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 4; ++j) bins[4095 - (4 * tid + j)] = before + loc[j];
+    for (int j = 0; j < kOrderKeys; ++j) bins[4095 - (kOrderKeys * tid + j)] = before + loc[j];
     __syncthreads();
     for (int t = tid; t < count; t += kOrderBlock) order[atomicAdd(&bins[keys[t]], 1)] = t;
 }
 
-__global__ void fill_i32_kernel(int *p, int v, int count) {
+__global__ void __launch_bounds__(64) fill_i32_kernel(int *p, int v, int count) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = v;
 }
@@ -436,14 +443,20 @@ DrawThresholds draw_thresholds(double p) {
 }
 
 void launch_sample(const DevGraph &g, TrialPlanes tp, uint64_t base_seed, int64_t first_trial,
-                   int count, const NoiseTab &nt, hipStream_t st) {
-    if (count <= 0) return;
+                   int count, const NoiseTab &nt, hipStream_t st, uint64_t *mt_state, int mt_cap) {
+    if (count <= 0 || mt_cap <= 0) return;
     const size_t lds = static_cast<size_t>(2) * (g.wn + 1) * sizeof(uint32_t);
-    switch (g.cpad_w) {
-        case 8: hipLaunchKernelGGL((sample_kernel<8>), dim3(count), dim3(64), lds, st, g, tp, base_seed, first_trial, nt); break;
-        case 12: hipLaunchKernelGGL((sample_kernel<12>), dim3(count), dim3(64), lds, st, g, tp, base_seed, first_trial, nt); break;
-        case 16: hipLaunchKernelGGL((sample_kernel<16>), dim3(count), dim3(64), lds, st, g, tp, base_seed, first_trial, nt); break;
-        default: hipLaunchKernelGGL((sample_kernel<32>), dim3(count), dim3(64), lds, st, g, tp, base_seed, first_trial, nt); break;
+    // pieces of at most mt_cap trials: seed their states, then sample them
+    for (int t0 = 0; t0 < count; t0 += mt_cap) {
+        const int k = std::min(mt_cap, count - t0);
+        hipLaunchKernelGGL(seed_kernel, dim3((k + 63) / 64), dim3(64), 0, st, base_seed, first_trial + t0,
+                           k, mt_state);
+        switch (g.cpad_w) {
+            case 8: hipLaunchKernelGGL((sample_kernel<8>), dim3(k), dim3(64), lds, st, g, tp, mt_state, t0, nt); break;
+            case 12: hipLaunchKernelGGL((sample_kernel<12>), dim3(k), dim3(64), lds, st, g, tp, mt_state, t0, nt); break;
+            case 16: hipLaunchKernelGGL((sample_kernel<16>), dim3(k), dim3(64), lds, st, g, tp, mt_state, t0, nt); break;
+            default: hipLaunchKernelGGL((sample_kernel<32>), dim3(k), dim3(64), lds, st, g, tp, mt_state, t0, nt); break;
+        }
     }
 }
 
@@ -483,7 +496,9 @@ void launch_order_by_weight(const DevGraph &g, TrialPlanes tp, int count, int *k
 
 void launch_fill_i32(int *ptr, int value, int count, hipStream_t st) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(fill_i32_kernel, dim3((count + 255) / 256), dim3(256), 0, st, ptr, value,
+    // one-wavefront blocks: a fill queued between decoder launches starts on
+    // whatever compute unit has a wavefront slot free
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((count + 63) / 64), dim3(64), 0, st, ptr, value,
                        count);
 }
 

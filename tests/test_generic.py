@@ -26,6 +26,22 @@ def big_hgp():
     dec.destroy()
 
 
+def test_sampling_runs_in_pieces_past_the_generator_scratch(small_hgp):
+    # the sampler seeds at most 65,536 generator states at a time; a longer
+    # call runs in pieces, and every trial is still drawn from its own
+    # index's stream (the trials on both sides of the piece boundary equal a
+    # short call over the same indices, which the oracle tests pin)
+    code, dec = small_hgp
+    n = 65536 + 40
+    print("sampling %d trials in pieces" % n, flush=True)
+    err, sx, sz = dec.sample(7, 3, n, 0.05)
+    e1, x1, z1 = dec.sample(7, 3 + 65536 - 20, 60, 0.05)
+    assert np.array_equal(err[65516:], e1) and np.array_equal(sx[65516:], x1) and np.array_equal(sz[65516:], z1)
+    e0, _, _ = dec.sample(7, 3, 16, 0.05)
+    assert np.array_equal(err[:16], e0)
+    assert 0.02 < np.mean(err != 0) < 0.08
+
+
 def per_trial_equal(a, b, keys=("iterations", "syn_match", "success", "exact")):
     return all(np.array_equal(a[k], b[k]) for k in keys)
 

@@ -100,18 +100,21 @@ def test_fp16_inference_follows_the_training_forward_pass(main_decoder):
     model.destroy()
 
 
-@pytest.mark.parametrize("n", [3 * 8192 + 5, 3000])
-def test_trials_left_unconverged_finish_as_plain_bp(main_decoder, n):
+@pytest.mark.parametrize("n,chunk", [(3 * 8192 + 5, 0), (3 * 8192 + 5, 2048), (3000, 0)])
+def test_trials_left_unconverged_finish_as_plain_bp(main_decoder, n, chunk):
     # unit weights (plain min-sum) leave most trials unconverged after T
     # iterations: each is decoded again by plain BP from the prior, in a
     # second launch over the list of such trials, and reports T plus plain
-    # BP's own count and plain BP's outcome (pipelined chunks and one chunk)
+    # BP's own count and plain BP's outcome (pipelined chunks, the fallback
+    # launches on the finishing stream; 13 chunks over the eight buffer
+    # sets; one chunk)
     T = 25
     model = jbp.LearnedModel(main_decoder, T, kind=jbp.LEARNED_MIN_SUM)
     params = model.get_params()
     params[:4 * model.nb * T] = 1.0
     model.set_params(params, use_mlp=False)
-    _, rl = main_decoder.simulate_learned(model, 106, n, 0.05, max_iter=60)
+    print("fallback case n=%d chunk=%d" % (n, chunk), flush=True)
+    _, rl = main_decoder.simulate_learned(model, 106, n, 0.05, max_iter=60, chunk=chunk)
     _, rp = main_decoder.simulate(106, n, 0.05, max_iter=60, precision=jbp.FP32)
     fb = rl["iterations"] > T
     print("%d of %d trials handed to plain BP" % (int(fb.sum()), n), flush=True)
