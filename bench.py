@@ -227,6 +227,37 @@ def sp_learned_leg(jbp, dec, args, first):
     return out or None
 
 
+def training_leg(jbp, dec, args):
+    """The training workload: --train-steps steps (after one untimed) of the
+    min-sum model at T = 25, batch --train-batch, p uniform in [0.03, 0.05],
+    each a device-sampled batch, the forward pass with its tape, the adjoint
+    pass and the Adam update.  The forward tape (the check messages of every
+    iteration, (T + 1) x 55,296 FP32 values per trial) is written once and
+    read once per step: that is the step's streaming traffic."""
+    T, B = 25, args.train_batch
+    model = jbp.LearnedModel(dec, T, kind=jbp.LEARNED_MIN_SUM)
+    params = model.get_params()
+    params[:4 * model.nb * T] = 1.0
+    params[:2 * model.nb * T] = 0.75
+    model.set_params(params, use_mlp=False)
+    model.train_step(args.seed, 1, B, 0.03, 0.05, lr=1e-3)
+    t0 = time.perf_counter()
+    losses = []
+    for k in range(args.train_steps):
+        losses.append(model.train_step(args.seed, 1 + (k + 1) * B, B, 0.03, 0.05, lr=1e-3))
+    el = (time.perf_counter() - t0) / args.train_steps
+    tape = B * (T + 1) * EDGES_PER_ITER * 4
+    model.destroy()
+    return {"workload": "min-sum model, T=%d, batch %d, p ~ U[0.03, 0.05], FP32 forward + adjoint + "
+                        "Adam" % (T, B),
+            "steps": args.train_steps, "ms_per_step": round(1e3 * el, 2),
+            "trial_iterations_per_s": round(B * T / el, 1),
+            "tape_bytes_per_step": tape,
+            "tape_traffic_gbs": round(2 * tape / el / 1e9, 1),
+            "tape_traffic_frac_hbm": round(2 * tape / el / 1e9 / HBM_PEAK_GBS, 4),
+            "loss": [round(x, 6) for x in losses]}
+
+
 def source_sha256(path):
     import hashlib
     with open(path, "rb") as f:
@@ -469,6 +500,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sp-learned-steps", type=int, default=0,
                     help="steps of the sum-product learned models, for comparison (0: skip)")
+    ap.add_argument("--train-steps", type=int, default=2,
+                    help="timed training steps of the min-sum model (0: skip)")
+    ap.add_argument("--train-batch", type=int, default=4096)
+    ap.add_argument("--train-only", action="store_true",
+                    help="run only the training leg (for profiling its kernels)")
     ap.add_argument("--mixed-trials", type=int, default=4096, help="trials per p of the mixed run")
     ap.add_argument("--mixed-max-iter", type=int, default=200)
     ap.add_argument("--hgp-trials", type=int, default=2048)
@@ -506,6 +542,12 @@ def main():
         copy_calibration(args.copy_calibration_mb)
     if args.profile_run:
         return profile_run(jbp, args)
+    if args.train_only:
+        code = jbp.Code.from_params(PARAMS)
+        dec = jbp.Decoder(code, 0)
+        print(json.dumps(training_leg(jbp, dec, args)), flush=True)
+        dec.destroy()
+        return
     if args.workload == "hgp":
         return bench_hgp(jbp, args)
     from joint_bp_plus_pp_amd import learned as jl
@@ -615,6 +657,10 @@ def main():
 
     after = 1 + (args.steps + 8 + args.warmup) * N + n_r * args.chunk
     sp_learned = sp_learned_leg(jbp, dec, args, after) if args.sp_learned_steps > 0 else None
+    training = None
+    if args.train_steps > 0:
+        training = training_leg(jbp, dec, args)
+        print("bench: training %s" % json.dumps(training), file=sys.stderr)
     mixed = None
     if not args.no_mixed:
         mixed = mixed_workload(jbp, dec, args)
@@ -663,6 +709,7 @@ def main():
             "fer_matched": matched,
             "fer_below_plain_ci": below,
             "sp_learned": sp_learned,
+            "training": training,
             "mixed": mixed,
             "pp": pp,
         },

@@ -569,9 +569,12 @@ int ensure_pipe(jbp_ctx *ctx, int cap, char *err, size_t errlen) {
 //              one-wavefront workgroups run beside the decoder workgroup
 //              of the chunk before
 //   decode     (pipe_st[0]): after chunk j is sampled, its decoder launch
-//   finishing  (pipe_st[1]): after chunk j's decoder launch, the min-sum
-//              decoder's fallback launch, the residual test and the outcome
+//   finishing  (pipe_st[1]): after chunk j's decoder launch, the second
+//              launch of a two-phase decode (the min-sum decoder's fallback,
+//              or plain BP's stragglers), the residual test and the outcome
 //              copies
+// (Plain BP over fewer than kOffloadChunks chunks instead decodes each chunk
+// in one launch, the decode and finishing streams taking turns.)
 // The host reduces outcomes in chunk order (min_fails cuts at the same index
 // as the serial loop) and queues chunk j+kPipeDepth once chunk j's outcomes
 // are read, so a chunk whose slowest trial runs long (the fallback of a
@@ -593,13 +596,24 @@ int simulate_pipelined(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_run_stats &st, 
     const bool ms_kind = learned_ms(learned);
     hipStream_t sst = ctx->stream, dst = ctx->pipe_st[0], fst = ctx->pipe_st[1];
     // plain BP: the decoder launch stops at the first scheduled syndrome test
-    // at or after kStragglerIter iterations, and the finishing stream decodes
-    // the trials still unconverged there again from the prior up to max_iter
-    // (so a trial that runs to max_iter holds up only its own chunk's
-    // outcome, not the decode stream)
-    int cap = std::max(kStragglerIter, warmup + interval);
-    if (cap > warmup) cap = warmup + interval * ((cap - warmup + interval - 1) / interval);
-    if (cap >= cfg->max_iter) cap = 0;
+    // at or after max(kStragglerIter, 3 x the mean iterations of the chunks
+    // reduced so far), and the finishing stream decodes the trials still
+    // unconverged there again from the prior up to max_iter (so a trial that
+    // runs to max_iter holds up only its own chunk's outcome, not the decode
+    // stream, and few trials are decoded twice at any p)
+    auto cap_for = [&](double mean_iters) {
+        int c = std::max({kStragglerIter, warmup + interval, static_cast<int>(std::ceil(3.0 * mean_iters))});
+        if (c > warmup) c = warmup + interval * ((c - warmup + interval - 1) / interval);
+        return c >= cfg->max_iter ? 0 : c;
+    };
+    // A run of few chunks has little for the decode stream to go on with
+    // while the finishing stream decodes stragglers again; there plain BP
+    // decodes each chunk in one launch, on the two streams in turn (chunk
+    // j+1's workgroups take over compute units as chunk j's last trials
+    // finish).
+    const bool offload = !learned && nch >= kOffloadChunks && cap_for(0.0) > 0;
+    const bool alternate = !learned && !offload;
+    int cap = offload ? cap_for(0.0) : 0;
     // the previous call's work on the context stream (e.g. table uploads) first
     HIPCHK(hipStreamSynchronize(sst));
     auto enqueue = [&](int64_t j) -> int {
@@ -617,24 +631,28 @@ int simulate_pipelined(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_run_stats &st, 
         }
         launch_fill_i32(b.counter, 0, 1, sst);
         HIPCHK(hipEventRecord(b.ev[5], sst));
-        HIPCHK(hipStreamWaitEvent(dst, b.ev[5], 0));
-        HIPCHK(hipEventRecord(b.ev[1], dst));
+        // decode and finishing streams of this chunk: with two streams taking
+        // turns, each chunk finishes on its own decode stream
+        hipStream_t ds = dst, fs = fst;
+        if (alternate) ds = fs = (j & 1) ? fst : dst;
+        HIPCHK(hipStreamWaitEvent(ds, b.ev[5], 0));
+        HIPCHK(hipEventRecord(b.ev[1], ds));
         launch_decode_qc_f32(g, ctx->qcp, ctx->d_qc_tables, tpo, cnt, nt, cfg->max_iter, warmup,
-                             interval, std::max(1, ctx->cus), b.counter, dst, learned, b.fb,
-                             b.ev[4], fst, learned ? 0 : cap);
+                             interval, std::max(1, ctx->cus), b.counter, ds, learned, b.fb,
+                             b.ev[4], fs, learned ? 0 : cap);
         if (!ms_kind) {
-            HIPCHK(hipEventRecord(b.ev[4], dst));
-            HIPCHK(hipStreamWaitEvent(fst, b.ev[4], 0));
+            HIPCHK(hipEventRecord(b.ev[4], ds));
+            HIPCHK(hipStreamWaitEvent(fs, b.ev[4], 0));
         }
-        HIPCHK(hipEventRecord(b.ev[2], fst));
-        if (ctx->have_logicals) launch_logical(g, ctx->lg[0], ctx->lg[1], b.tp, cnt, fst);
-        else if (ctx->have_basis) launch_residual(g, ctx->basis[0], ctx->basis[1], b.tp, cnt, fst);
-        else launch_logical(g, DevLogicals(), DevLogicals(), b.tp, cnt, fst);
+        HIPCHK(hipEventRecord(b.ev[2], fs));
+        if (ctx->have_logicals) launch_logical(g, ctx->lg[0], ctx->lg[1], b.tp, cnt, fs);
+        else if (ctx->have_basis) launch_residual(g, ctx->basis[0], ctx->basis[1], b.tp, cnt, fs);
+        else launch_logical(g, DevLogicals(), DevLogicals(), b.tp, cnt, fs);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(b.h_iters, b.tp.iters, cnt * 4, hipMemcpyDeviceToHost, fst));
+        HIPCHK(hipMemcpyAsync(b.h_iters, b.tp.iters, cnt * 4, hipMemcpyDeviceToHost, fs));
         // the three flag arrays sit b.cap apart (the buffers may be larger than C)
-        HIPCHK(hipMemcpyAsync(b.h_flags, b.tp.syn_ok, 3 * static_cast<size_t>(b.cap), hipMemcpyDeviceToHost, fst));
-        HIPCHK(hipEventRecord(b.ev[3], fst));
+        HIPCHK(hipMemcpyAsync(b.h_flags, b.tp.syn_ok, 3 * static_cast<size_t>(b.cap), hipMemcpyDeviceToHost, fs));
+        HIPCHK(hipEventRecord(b.ev[3], fs));
         return JBP_OK;
     };
     double dev_ms = 0.0;
@@ -692,6 +710,7 @@ int simulate_pipelined(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_run_stats &st, 
             }
         }
         done += take;
+        if (offload && done > 0) cap = cap_for(static_cast<double>(st.total_iters) / static_cast<double>(done));
         if (!stop && j + kPipeDepth < nch && (rc = enqueue(j + kPipeDepth))) return rc;
     }
     // chunks still in flight after an early stop finish before the buffers

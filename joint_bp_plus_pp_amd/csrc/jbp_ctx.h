@@ -29,6 +29,8 @@ constexpr int kPipeDepth = 8;
 // Iterations after which a pipelined plain-BP launch hands a trial over to
 // the finishing stream (rounded up to a scheduled syndrome test).
 constexpr int kStragglerIter = 32;
+// Fewest chunks for which a pipelined plain-BP run offloads stragglers.
+constexpr int kOffloadChunks = 4;
 
 // One chunk's buffer set in the pipeline: its own trial planes, decoder
 // counter, events and pinned outcome arrays.

@@ -221,23 +221,24 @@ def test_single_decode_graph_replay(main_decoder, main_code_oracle, precision):
             assert (g_it, g_ok) == (w_it, w_ok) and np.array_equal(g_est, w_est)
 
 
-@pytest.mark.parametrize("chunk,warmup,interval", [(0, 0, 1), (1024, 0, 1), (0, 4, 3)])
-def test_pipelined_run_equals_serial_run(main_decoder, chunk, warmup, interval):
+@pytest.mark.parametrize("n,chunk,warmup,interval", [(3 * 8192 + 77, 0, 0, 1), (3 * 8192 + 77, 1024, 0, 1),
+                                                    (3 * 8192 + 77, 0, 4, 3), (2 * 8192 + 77, 0, 0, 1)])
+def test_pipelined_run_equals_serial_run(main_decoder, n, chunk, warmup, interval):
     # Runs longer than one launch are pipelined: a sampling stream, a decode
     # stream and a finishing stream over eight buffer sets (chunk 1024: 25
-    # chunks, so the buffer sets are reused three times over).  The decode
-    # launch stops at the first scheduled syndrome test from 32 iterations on
-    # (34 with warmup 4, interval 3) and the finishing stream decodes what is
-    # left from the prior to max-iter.  The per-trial outcome must equal the
-    # serial path (one launch, taken when estimates are requested), and
-    # min_fails must cut at the same trial index.
-    n = 3 * 8192 + 77
-    print("pipelined vs serial, chunk", chunk, warmup, interval, flush=True)
+    # chunks, so the buffer sets are reused three times over).  From four
+    # chunks on, the decode launch stops at the first scheduled syndrome
+    # test from 32 iterations on (34 with warmup 4, interval 3) and the
+    # finishing stream decodes what is left from the prior to max-iter;
+    # three chunks decode whole on the two streams in turn.  The per-trial
+    # outcome must equal the serial path (one launch, taken when estimates
+    # are requested), and min_fails must cut at the same trial index.
+    print("pipelined vs serial", n, chunk, warmup, interval, flush=True)
     kw = dict(check_warmup=warmup, check_interval=interval)
     sa, a = main_decoder.simulate(106, n, 0.05, max_iter=100, precision=jbp.FP32, want_est=True, **kw)
     sb, b = main_decoder.simulate(106, n, 0.05, max_iter=100, precision=jbp.FP32, chunk=chunk, **kw)
     # trials handed to the finishing stream, some of them running to max-iter
-    assert np.sum(b["iterations"] > 34) > 20 and np.sum(b["iterations"] == 100) > 0
+    assert np.sum(b["iterations"] > 34) > 10 and np.sum(b["iterations"] == 100) > 0
     for k in ("iterations", "syn_match", "success", "exact"):
         assert np.array_equal(a[k], b[k]), k
     assert (sa["done"], sa["failures"], sa["total_iters"]) == (sb["done"], sb["failures"], sb["total_iters"])
