@@ -154,7 +154,9 @@ typedef struct jbp_sim_cfg {
     int flip_hist;        /* --flip-hist (reference default 5)              */
     int chunk;            /* LDS engine, runs longer than one launch: trials
                              per persistent launch (0 = 8192, at most 65536);
-                             launches alternate two streams                  */
+                             launches are pipelined over three streams, and
+                             plain BP decodes a long run's stragglers again
+                             on the third (per-trial results unchanged)      */
     int freeze_syn;       /* --freeze-syn (jointbp_ets.cpp:1884-1901,
                              2526-2535): FP64, post-processing off           */
 } jbp_sim_cfg;

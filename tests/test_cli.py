@@ -155,3 +155,55 @@ def test_err_decode_with_pp_matches_reference(tmp_path):
         assert line in out.stdout.splitlines(), line
     est = np.loadtxt(es, dtype=np.int64)
     assert hashlib.sha1(est.astype(np.uint8).tobytes()).hexdigest() == ref["est_sha1"]
+
+
+REF_HOST = os.path.join(ROOT, "oracle", "_ref", "jointbp_ets_jbp")
+
+
+def run_ref_host(tmp_path, extra):
+    """The reference's own jointbp_ets.cpp, built with -DUSE_CUDA against
+    joint_bp_plus_pp_amd/refhost/jointbp_cuda_jbp.cpp (its cuda_bp_* calls,
+    jointbp_cuda.h:48-63, answered by libjbp), decoding the reference's
+    trial 7 from an --err file."""
+    if not os.path.exists(REF_HOST):
+        pytest.skip("reference host on libjbp not built (needs the reference source)")
+    from _oracle import Oracle
+    err = Oracle().code_from_params(PARAMS).sample(106, 7, 0.05)
+    ep, es = tmp_path / "err.txt", tmp_path / "est.txt"
+    np.savetxt(ep, err, fmt="%d")
+    out = subprocess.run([REF_HOST, "--params", PARAMS, "--err", str(ep), "--p", "0.05", "--max-iter",
+                          "50", "--no-pp", "--est", str(es)] + extra, capture_output=True, text=True,
+                         timeout=300, env=dict(os.environ, NO_COLOR="1"))
+    return out, es
+
+
+def test_reference_host_without_device_falls_back_to_its_cpu_decoder(tmp_path):
+    # no device: cuda_bp_create returns null with libjbp's message, the
+    # reference prints it and decodes on its own CPU decoder
+    from conftest import hip_device_available
+    if hip_device_available():
+        pytest.skip("a device is present")
+    ref = golden_json("aggregates.json")["cli_err_trial7"]
+    out, es = run_ref_host(tmp_path, ["--cuda"])
+    assert out.returncode == 0, out.stderr
+    assert "CUDA init failed" in out.stderr and "falling back to CPU" in out.stderr
+    for line in ref["lines"][4:]:
+        assert line in out.stdout, line
+    assert fnv1a(np.loadtxt(es, dtype=np.int64)) == ref["est_hash"]
+
+
+@pytest.mark.gpu
+def test_reference_host_on_libjbp_matches_reference(tmp_path):
+    # the reference program's --err decode through its CUDA path, run by
+    # libjbp's FP64 decoder: the same Decode Result lines and estimate as the
+    # reference's CPU run, and no fallback
+    ref = golden_json("aggregates.json")["cli_err_trial7"]
+    out, es = run_ref_host(tmp_path, ["--cuda"])
+    assert out.returncode == 0, out.stderr
+    assert "falling back" not in out.stderr, out.stderr
+    for line in ref["lines"][4:]:
+        assert line in out.stdout, line
+    assert fnv1a(np.loadtxt(es, dtype=np.int64)) == ref["est_hash"]
+    # --freeze-syn goes to the device too (FP64 runs it)
+    out, _ = run_ref_host(tmp_path, ["--cuda", "--freeze-syn"])
+    assert out.returncode == 0 and "falling back" not in out.stderr, out.stderr
