@@ -279,7 +279,9 @@ def kernel_profile():
 def mixed_workload(jbp, qc_dec, args):
     """Interleaved p over the included code (trial idx uses p[idx % 4]) and
     the seeded hypergraph-product code HGP(150 x 200, (3,4)-regular),
-    n = 62,500, on the HBM-resident decoder (slots autotuned first)."""
+    n = 62,500, on the HBM-resident decoder (slots autotuned first): each as
+    its own call, both codes in one engine, and both codes' own engines at
+    once from two host threads."""
     out = {}
     n_qc = 4 * args.mixed_trials
     t0 = time.perf_counter()
@@ -322,6 +324,47 @@ def mixed_workload(jbp, qc_dec, args):
                                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                                "bytes_per_trial_iteration": bytes_iter,
                                "kernels": "gen_var_kernel + gen_check_kernel (+ book/emit/load)"}}
+    sep_ms = 1e3 * (out["qc_interleaved"]["trials"] / out["qc_interleaved"]["trials_per_s"] +
+                    n_h / out["hgp"]["trials_per_s"])
+    # both codes in one engine (jbp_simulate_mixed): every step's kernels
+    # cover the included code's trials (HBM engine, --mixed-qc-slots) and
+    # the HGP trials (tuned slots); one untimed run first
+    runs = [(qc_dec, dict(base_seed=args.seed, n_trials=n_qc, p=list(MIXED_PS),
+                          max_iter=args.mixed_max_iter, slots=args.mixed_qc_slots)),
+            (hdec, dict(base_seed=args.seed, n_trials=n_h, p=args.hgp_p, max_iter=args.mixed_max_iter,
+                        slots=slots))]
+    jbp.simulate_mixed(runs, per_trial=False)
+    t0 = time.perf_counter()
+    (sq, _), (sh, _) = jbp.simulate_mixed(runs, per_trial=False)
+    el = time.perf_counter() - t0
+    out["one_engine"] = {"trials": n_qc + n_h, "trials_per_s": round((n_qc + n_h) / el, 2),
+                         "ms": round(1e3 * el, 2), "device_ms": round(sq["device_ms"], 2),
+                         "qc_slots": args.mixed_qc_slots, "hgp_slots": slots,
+                         "qc_failures": sq["failures"], "hgp_failures": sh["failures"],
+                         "separate_calls_ms": round(sep_ms, 2)}
+    # the two codes' own engines at once: one host thread per context
+    import threading
+    box = {}
+
+    def go(key, fn):
+        t = time.perf_counter()
+        fn()
+        box[key] = time.perf_counter() - t
+    ths = [threading.Thread(target=go, args=("qc", lambda: qc_dec.simulate(
+               args.seed, n_qc, list(MIXED_PS), max_iter=args.mixed_max_iter, first_trial=1,
+               precision=32, per_trial=False))),
+           threading.Thread(target=go, args=("hgp", lambda: hdec.simulate(
+               args.seed, n_h, args.hgp_p, max_iter=args.mixed_max_iter, first_trial=1, precision=32,
+               per_trial=False)))]
+    t0 = time.perf_counter()
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    el = time.perf_counter() - t0
+    out["concurrent_engines"] = {"trials": n_qc + n_h, "trials_per_s": round((n_qc + n_h) / el, 2),
+                                 "ms": round(1e3 * el, 2),
+                                 "qc_ms": round(1e3 * box["qc"], 2), "hgp_ms": round(1e3 * box["hgp"], 2)}
     hdec.destroy()
     code.close()
     return out
@@ -507,6 +550,8 @@ def main():
                     help="run only the training leg (for profiling its kernels)")
     ap.add_argument("--mixed-trials", type=int, default=4096, help="trials per p of the mixed run")
     ap.add_argument("--mixed-max-iter", type=int, default=200)
+    ap.add_argument("--mixed-qc-slots", type=int, default=1024,
+                    help="resident trials of the included code in the one-engine mixed run")
     ap.add_argument("--hgp-trials", type=int, default=2048)
     ap.add_argument("--hgp-p", type=float, default=0.03)
     ap.add_argument("--no-mixed", action="store_true")

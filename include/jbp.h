@@ -216,6 +216,19 @@ typedef struct jbp_run_stats {
 int jbp_simulate(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *per_trial,
                  jbp_run_stats *stats, char *err, size_t errlen);
 
+/* jbp_simulate for up to two codes in one engine: code c's trials (cfgs[c],
+ * context ctxs[c], one context per code, all on one device) are sampled, then
+ * decoded together by the HBM-resident FP32 decoder, each engine step's
+ * kernels covering every code's resident trials, then judged by each code's
+ * success test.  Per trial the outcome equals jbp_simulate of that code with
+ * engine JBP_ENGINE_HBM.  FP32 only, at most 65,536 trials per code, no
+ * post-processing, freeze-syn or min_fails.  per_trial (may be NULL) and
+ * stats are arrays of n_codes; device_ms is the shared decode's time.  No
+ * counterpart in the reference (its decoder is generic over check lists,
+ * jointbp_ets.cpp:1943-1948, but runs one code per process). */
+int jbp_simulate_mixed(jbp_ctx *const *ctxs, const jbp_sim_cfg *cfgs, int n_codes,
+                       jbp_trial_out *per_trial, jbp_run_stats *stats, char *err, size_t errlen);
+
 /* ---------------------------------------------------------------------------
  * Post-processing (PP) of unconverged decodes: the reference's ETS lookup,
  * flip-set solve and ordered-statistics repair (jointbp_ets.cpp:2736-3074 in

@@ -112,6 +112,8 @@ def lib():
         "jbp_decode_batch": (i32, [vp, i32, vp, vp, vp, i32, dbl, i32, vp, vp, vp, cp, sz]),
         "jbp_decode_batch_noalloc": (i32, [vp, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp, cp, sz]),
         "jbp_simulate": (i32, [vp, C.POINTER(SimCfg), C.POINTER(TrialOut), C.POINTER(RunStats), cp, sz]),
+        "jbp_simulate_mixed": (i32, [C.POINTER(vp), C.POINTER(SimCfg), i32, C.POINTER(TrialOut),
+                                     C.POINTER(RunStats), cp, sz]),
         "jbp_sample": (i32, [vp, u64, i64, i32, dbl, vp, vp, vp, cp, sz]),
         "jbp_in_row_space": (i32, [vp, i32, i32, vp, vp, cp, sz]),
         "jbp_last_kernel_ms": (dbl, [vp, i32, C.POINTER(i64)]),
@@ -655,6 +657,53 @@ class LearnedModel:
         _check(lib().jbp_learned_train_step(self._h, base_seed, first_trial, batch, p_lo, p_hi, lr,
                                             C.byref(loss), e, 512), e)
         return loss.value
+
+
+def simulate_mixed(runs, per_trial=True):
+    """jbp_simulate_mixed: up to two (decoder, settings) pairs, each a
+    different code's Decoder on the same device, decoded together by the
+    HBM-resident FP32 engine (every engine step's kernels cover both codes'
+    trials).  settings: base_seed, n_trials, p (a number or a list: trial
+    index idx uses p[idx % len(p)]), and optionally max_iter (50),
+    first_trial (1), damping, check_warmup, check_interval, slots,
+    schedule.  Per trial the outcome equals that decoder's simulate() with
+    engine=ENGINE_HBM.  Returns [(stats, per-trial dict or None), ...]."""
+    n = len(runs)
+    cfgs = (SimCfg * n)()
+    ctxs = (C.c_void_p * n)()
+    outs = (TrialOut * n)()
+    stats = (RunStats * n)()
+    keep, arrays = [], []
+    for i, (dec, kw) in enumerate(runs):
+        p = kw["p"]
+        plist = None
+        if np.ndim(p) > 0:
+            plist = np.ascontiguousarray(p, dtype=np.float64)
+            keep.append(plist)
+            p = float(plist[0]) if plist.size else 0.0
+        nt = int(kw["n_trials"])
+        cfgs[i] = SimCfg(kw["base_seed"], kw.get("first_trial", 1), nt, p, kw.get("max_iter", 50),
+                         kw.get("damping", 0.0), FP32, kw.get("check_warmup", 0),
+                         kw.get("check_interval", 1), kw.get("slots", 0), 0, 0,
+                         plist.ctypes.data_as(C.POINTER(C.c_double)) if plist is not None else None,
+                         int(plist.size) if plist is not None else 0, ENGINE_HBM,
+                         kw.get("schedule", 0), 0, 0, 5, 0, 0)
+        ctxs[i] = dec._h
+        a = {}
+        if per_trial:
+            a = {"iterations": np.zeros(nt, np.int32), "syn_match": np.zeros(nt, np.uint8),
+                 "success": np.zeros(nt, np.uint8), "exact": np.zeros(nt, np.uint8),
+                 "pp_flags": np.zeros(nt, np.uint8)}
+            for k, v in a.items():
+                setattr(outs[i], k, v.ctypes.data_as(C.POINTER(C.c_int if v.dtype == np.int32 else C.c_uint8)))
+        arrays.append(a)
+    e = _errbuf()
+    _check(lib().jbp_simulate_mixed(ctxs, cfgs, n, outs if per_trial else None, stats, e, 512), e)
+    res = []
+    for i in range(n):
+        st = {k: getattr(stats[i], k) for k, _ in RunStats._fields_}
+        res.append((st, arrays[i] if per_trial else None))
+    return res
 
 
 def pack_bits(a):

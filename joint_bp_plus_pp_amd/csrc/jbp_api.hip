@@ -371,6 +371,8 @@ void free_gen(jbp_ctx *ctx) {
     ctx->gen = GenState();
     if (ctx->h_ctrl) (void)hipHostFree(ctx->h_ctrl);
     ctx->h_ctrl = nullptr;
+    if (ctx->h_ctrl_mix) (void)hipHostFree(ctx->h_ctrl_mix);
+    ctx->h_ctrl_mix = nullptr;
     for (auto &e : ctx->gen_ev)
         if (e) (void)hipEventDestroy(e);
     ctx->gen_ev[0] = ctx->gen_ev[1] = nullptr;
@@ -1529,6 +1531,191 @@ int jbp_in_row_space(jbp_ctx *ctx, int side, int count, const int *bits, uint8_t
 int jbp_simulate(jbp_ctx *ctx, const jbp_sim_cfg *cfg, jbp_trial_out *pt, jbp_run_stats *stats,
                  char *err, size_t errlen) {
     return jbp::simulate_impl(ctx, cfg, pt, stats, nullptr, err, errlen);
+}
+
+int jbp_simulate_mixed(jbp_ctx *const *ctxs, const jbp_sim_cfg *cfgs, int n_codes,
+                       jbp_trial_out *per_trial, jbp_run_stats *stats, char *err, size_t errlen) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (!ctxs || !cfgs || !stats || n_codes < 1 || n_codes > kMaxMixCodes) {
+        set_err(err, errlen, "bad arguments (1..%d codes)", kMaxMixCodes);
+        return JBP_ERR_ARG;
+    }
+    for (int c = 0; c < n_codes; ++c) {
+        const jbp_sim_cfg &cf = cfgs[c];
+        if (!ctxs[c] || cf.n_trials < 0 || cf.max_iter < 0 || cf.damping < 0.0) {
+            set_err(err, errlen, "bad arguments for code %d", c);
+            return JBP_ERR_ARG;
+        }
+        if (ctxs[c]->device != ctxs[0]->device) {
+            set_err(err, errlen, "mixed codes must share one device");
+            return JBP_ERR_ARG;
+        }
+        for (int d = 0; d < c; ++d)
+            if (ctxs[d] == ctxs[c]) {
+                set_err(err, errlen, "each code of a mixed run needs its own context");
+                return JBP_ERR_ARG;
+            }
+        if (cf.precision != JBP_FP32 || cf.pp || cf.freeze_syn || cf.min_fails > 0) {
+            set_err(err, errlen, "a mixed run is FP32 BP without post-processing, freeze-syn or min_fails");
+            return JBP_ERR_UNSUPPORTED;
+        }
+        if (cf.n_trials > kDefaultChunk) {
+            set_err(err, errlen, "at most %d trials per code in a mixed run", kDefaultChunk);
+            return JBP_ERR_ARG;
+        }
+        if (cf.n_p < 0 || (cf.n_p > 0 && !cf.p_list) || cf.n_p > kMaxPList) {
+            set_err(err, errlen, "p_list needs 1..%d entries", kMaxPList);
+            return JBP_ERR_ARG;
+        }
+        if (!gen_supported(ctxs[c]->dg)) {
+            set_err(err, errlen, "generic FP32 decoder supports check degree <= 32");
+            return JBP_ERR_UNSUPPORTED;
+        }
+    }
+    jbp_ctx *c0 = ctxs[0];
+    HIPCHK(hipSetDevice(c0->device));
+    hipStream_t st = c0->stream;
+    for (int c = 0; c < n_codes; ++c) HIPCHK(hipStreamSynchronize(ctxs[c]->stream));
+    GenMix mix;
+    int map[kMaxMixCodes];  // mix entry of code c, -1: no trials
+    for (int c = 0; c < n_codes; ++c) {
+        jbp_ctx *ctx = ctxs[c];
+        const jbp_sim_cfg &cf = cfgs[c];
+        map[c] = -1;
+        const int np = cf.n_p > 0 ? cf.n_p : 1;
+        const double *plist = cf.n_p > 0 ? cf.p_list : &cf.p;
+        std::vector<Prior> pri(np);
+        for (int i = 0; i < np; ++i) {
+            if (!(plist[i] >= 0.0 && plist[i] <= 1.0)) {
+                set_err(err, errlen, "p = %g is outside [0, 1]", plist[i]);
+                return JBP_ERR_ARG;
+            }
+            pri[i] = prior_from_p(plist[i]);
+        }
+        int rc = set_noise(ctx, pri.data(), plist, np, err, errlen);
+        if (rc) return rc;
+        const int cnt = static_cast<int>(cf.n_trials);
+        if (cnt == 0) continue;
+        if ((rc = ensure_chunk(ctx, cnt, err, errlen))) return rc;
+        if ((rc = ensure_mt(ctx, cnt, err, errlen))) return rc;
+        NoiseTab nt = ctx->noise;
+        nt.phase = static_cast<int>(((cf.first_trial % np) + np) % np);
+        launch_sample(ctx->dg, ctx->tp, cf.base_seed, cf.first_trial, cnt, nt, st, ctx->d_mt, ctx->mt_cap);
+        TrialPlanes tpo = ctx->tp;
+        if (cf.schedule == 0) {
+            launch_order_by_weight(ctx->dg, ctx->tp, cnt, ctx->d_keys, ctx->d_order, st);
+            tpo.order = ctx->d_order;
+        }
+        HIPCHK(hipGetLastError());
+        // this code's resident slots and state, as its own engine would have them
+        int S = cf.slots > 0 ? cf.slots : default_gen_slots(ctx);
+        S = std::min(S, ((cnt + 63) / 64) * 64);
+        const int groups = std::max(1, (S + 63) / 64);
+        const bool damped = cf.damping > 0.0;
+        const size_t bytes = gen_carve_bytes(ctx->dg, groups, damped);
+        if (bytes > ctx->gen_cap) {
+            if (ctx->gen_mem) (void)hipFree(ctx->gen_mem);
+            ctx->gen_mem = nullptr;
+            ctx->gen_cap = 0;
+            HIPCHK(hipMalloc(&ctx->gen_mem, bytes));
+            ctx->gen_cap = bytes;
+        }
+        gen_carve(ctx->dg, groups, ctx->gen_mem, ctx->gen, damped);
+        ctx->gen.damping = static_cast<float>(damped ? cf.damping : 0.0);
+        GenCode &k = mix.c[mix.n];
+        k.g = ctx->dg;
+        k.s = ctx->gen;
+        k.tp = tpo;
+        k.nt = nt;
+        k.count = cnt;
+        k.max_iter = cf.max_iter;
+        k.warmup = cf.check_warmup;
+        k.interval = cf.check_interval;
+        map[c] = mix.n++;
+    }
+    double dev_ms = 0.0;
+    if (mix.n > 0) {
+        if (!c0->h_ctrl_mix)
+            HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&c0->h_ctrl_mix), 2 * kMaxMixCodes * sizeof(int)));
+        for (auto &e : c0->gen_ev)
+            if (!e) HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventRecord(c0->evt[2], st));
+        int64_t steps = 0;
+        const int rc = gen_decode_mix(mix, c0->h_ctrl_mix, c0->gen_ev, st, &steps);
+        if (rc == -1) {
+            set_err(err, errlen, "mixed decoder exceeded its step bound");
+            return JBP_ERR_DEVICE;
+        }
+        if (rc != 0) {
+            set_err(err, errlen, "mixed decoder: %s",
+                    rc == -2 ? "empty mix" : hipGetErrorString(static_cast<hipError_t>(rc)));
+            return JBP_ERR_DEVICE;
+        }
+        HIPCHK(hipEventRecord(c0->evt[3], st));
+        HIPCHK(hipEventSynchronize(c0->evt[3]));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c0->evt[2], c0->evt[3]));
+        dev_ms = ms;
+        c0->last_ms[1] = ms;
+        c0->launches[1]++;
+        c0->gen_steps = steps;
+    }
+    // each code's success test and outcomes
+    for (int c = 0; c < n_codes; ++c) {
+        jbp_ctx *ctx = ctxs[c];
+        jbp_run_stats &rs = stats[c];
+        std::memset(&rs, 0, sizeof(rs));
+        rs.device_ms = dev_ms;
+        if (map[c] < 0) continue;
+        const int cnt = static_cast<int>(cfgs[c].n_trials);
+        const DevGraph &g = ctx->dg;
+        const bool judged = ctx->have_logicals || ctx->have_basis;
+        if (ctx->have_logicals) launch_logical(g, ctx->lg[0], ctx->lg[1], ctx->tp, cnt, st);
+        else if (ctx->have_basis) launch_residual(g, ctx->basis[0], ctx->basis[1], ctx->tp, cnt, st);
+        else launch_logical(g, DevLogicals(), DevLogicals(), ctx->tp, cnt, st);
+        HIPCHK(hipGetLastError());
+        const int cap = ctx->chunk_cap;
+        HIPCHK(hipMemcpyAsync(ctx->h_iters, ctx->tp.iters, cnt * 4, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(ctx->h_flags, ctx->tp.syn_ok, 3 * static_cast<size_t>(cap),
+                              hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        jbp_trial_out *pt = per_trial ? &per_trial[c] : nullptr;
+        if (pt && (pt->err || pt->est)) {
+            int rc = ensure_states(ctx, static_cast<size_t>(cnt) * g.n, err, errlen);
+            if (rc) return rc;
+            for (int which = 0; which < 2; ++which) {
+                int *dst = which == 0 ? pt->err : pt->est;
+                if (!dst) continue;
+                const uint32_t *xb = which == 0 ? ctx->tp.err_x : ctx->tp.est_x;
+                const uint32_t *zb = which == 0 ? ctx->tp.err_z : ctx->tp.est_z;
+                launch_unpack_states(xb, zb, g.wn, g.n, cnt, ctx->d_states, st);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(dst, ctx->d_states, static_cast<size_t>(cnt) * g.n * 4,
+                                      hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+            }
+        }
+        const uint8_t *ok = ctx->h_flags, *succ = ctx->h_flags + cap, *exact = ctx->h_flags + 2 * cap;
+        for (int i = 0; i < cnt; ++i) {
+            const uint8_t sc = judged ? succ[i] : static_cast<uint8_t>(ok[i] && exact[i]);
+            rs.total_iters += ctx->h_iters[i];
+            rs.failures += sc ? 0 : 1;
+            rs.bp_failures += ok[i] ? 0 : 1;
+            rs.exact += exact[i] ? 1 : 0;
+            rs.stab_success += (sc && !exact[i] && judged) ? 1 : 0;
+            if (pt) {
+                if (pt->pp_flags) pt->pp_flags[i] = ok[i] ? JBP_PPF_BP_OK : 0;
+                if (pt->iterations) pt->iterations[i] = ctx->h_iters[i];
+                if (pt->syn_match) pt->syn_match[i] = ok[i];
+                if (pt->success) pt->success[i] = sc;
+                if (pt->exact) pt->exact[i] = exact[i];
+            }
+        }
+        rs.done = cnt;
+    }
+    const double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (int c = 0; c < n_codes; ++c) stats[c].elapsed_ms = el;
+    return JBP_OK;
 }
 
 int jbp_set_logicals(jbp_ctx *ctx, int side, int64_t n_rows, const int64_t *row_off,

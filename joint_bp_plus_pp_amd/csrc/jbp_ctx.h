@@ -162,6 +162,7 @@ struct jbp_ctx {
     size_t gen_cap = 0;
     int gen_slots = 0;        // default resident trials (0: not chosen yet)
     int *h_ctrl = nullptr;    // pinned [2]
+    int *h_ctrl_mix = nullptr;  // pinned [2][kMaxMixCodes] (mixed runs led by this context)
     hipEvent_t gen_ev[2] = {nullptr, nullptr};
     int64_t gen_steps = 0;    // engine steps of the last generic decode
     // single-trial decode replayed from a captured graph (use_graph)

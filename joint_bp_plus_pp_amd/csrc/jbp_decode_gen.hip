@@ -61,23 +61,19 @@ __device__ __forceinline__ PriorF slot_prior(const GenState &s, int group, int l
     return pr;
 }
 
-// grid (ceil(n / (4*VPW)), groups), block 256: wave w handles VPW
+// Variable pass of block (bx, group), block 256: wave w handles VPW
 // variables; their edge lists come from the padded table (MAXV entries per
 // variable and side, -1 = none), so every gather of a variable is issued at
 // once.
 template <int VPW, int MAXV>
-__global__ void __launch_bounds__(256) gen_var_kernel(DevGraph g, GenState s) {
-    // the previous step's stopped-trial list has been emitted: clear it for
-    // this step's book pass
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) s.ctrl[2] = 0;
-    const int group = blockIdx.y;
+__device__ __forceinline__ void var_block(const DevGraph &g, const GenState &s, int bx, int group) {
     const uint64_t act = uni64(s.active[group]);
     if (!act) return;
     const int lane = threadIdx.x & 63;
     const bool fresh = (uni64(s.fresh[group]) >> lane) & 1u;
     const uint32_t keep = fresh ? 0u : ~0u;  // a fresh slot reads its old messages as 0
     const PriorF pr = slot_prior(s, group, lane);
-    const int v0 = (blockIdx.x * 4 + wave_id()) * VPW;
+    const int v0 = (bx * 4 + wave_id()) * VPW;
     const int v1 = min(v0 + VPW, g.n);
     const size_t E0 = g.side[0].E, E1 = g.side[1].E;
     const float *lam0 = s.lam[0] + static_cast<size_t>(group) * (E0 + 1) * 64 + lane;
@@ -134,6 +130,15 @@ __global__ void __launch_bounds__(256) gen_var_kernel(DevGraph g, GenState s) {
             }
         }
     }
+}
+
+// grid (ceil(n / (4*VPW)), groups), block 256.
+template <int VPW, int MAXV>
+__global__ void __launch_bounds__(256) gen_var_kernel(DevGraph g, GenState s) {
+    // the previous step's stopped-trial list has been emitted: clear it for
+    // this step's book pass
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) s.ctrl[2] = 0;
+    var_block<VPW, MAXV>(g, s, blockIdx.x, blockIdx.y);
 }
 
 // Check update of deg <= MAXC inputs mu[] in place (the pair recursion of
@@ -462,6 +467,118 @@ struct BookArgs {
     int max_iter, warmup, interval;
 };
 
+// ---- several codes in one engine (gen_decode_mix) ----
+// Engine groups [c.group0, c.group0 + c.s.groups) belong to code c: every
+// kernel of a step covers all codes, each block serving the code its group
+// (or, in the post pass, its block range) falls in.  Each code keeps its
+// own state, trials, channel table and counters, so a trial's decode is the
+// same as in that code's own engine.  The code index of a kernel body is a
+// template argument, so each code's fields are read from the kernel
+// arguments at a fixed offset.
+
+// Check-pass variants of a mixed engine.  Every variant compiled into a
+// kernel costs that kernel its registers (a kernel holds the most any of its
+// bodies needs), so the mixed kernels are instantiated per pair of codes'
+// variants, from few: exact degree 7 (hypergraph products of (3,4)-regular
+// codes), exact degree 12 (the included code) and a generic body for any
+// degree up to 32; likewise the variable pass per pair of padded widths.
+enum { kMixChk7, kMixChk12, kMixChkAny };
+inline int mix_check_variant(const DevGraph &g, int &cpw) {
+    if (g.min_cdeg == g.max_cdeg && g.max_cdeg == 7) {
+        cpw = 4;
+        return kMixChk7;
+    }
+    if (g.min_cdeg == g.max_cdeg && g.max_cdeg == 12) {
+        cpw = 2;
+        return kMixChk12;
+    }
+    cpw = 4;
+    return kMixChkAny;
+}
+
+template <int C, int W>
+__device__ __forceinline__ void var_code(const GenMix &m, int bx, int group) {
+    const GenCode &k = m.c[C];
+    if (bx == 0 && group == 0 && threadIdx.x == 0) k.s.ctrl[2] = 0;
+    if (bx >= k.vx) return;
+    var_block<16, W>(k.g, k.s, bx, group);
+}
+
+template <int C, int K>
+__device__ __forceinline__ void check_code(const GenMix &m, int group, int q) {
+    const GenCode &k = m.c[C];
+    const uint64_t act = uni64(k.s.active[group]);
+    if (!act || q >= k.nch) return;
+    if constexpr (K == kMixChk7) check_chunk<7, 4, 7>(k.g, k.s, group, q, k.nch0);
+    else if constexpr (K == kMixChk12) check_chunk<12, 2, 12>(k.g, k.s, group, q, k.nch0);
+    else check_chunk<32, 4, 0>(k.g, k.s, group, q, k.nch0);
+}
+
+template <int C>
+__device__ __forceinline__ void post_code(const GenMix &m, int b) {
+    const GenCode &k = m.c[C];
+    if (b < k.ex * kEmitRows) emit_block(k.g, k.s, k.tp, b % k.ex, b / k.ex);
+    else {
+        b -= k.ex * kEmitRows;
+        load_block(k.g, k.s, k.tp, b % k.lx, b / k.lx);
+    }
+}
+
+__device__ __forceinline__ int code_of_group(const GenMix &m, int gy) {
+    return (m.n > 1 && gy >= m.c[1].group0) ? 1 : 0;
+}
+
+// Variable and check passes run on one-dimensional grids: code c owns the
+// blocks [c.var_b0, + vx * groups) (and [c.chk_b0, + ceil(nch / 4) * groups)),
+// group-major, so a code whose graph needs fewer blocks per group launches
+// no idle ones.
+
+// grid = both codes' var blocks, block 256; W0 / W1: the codes' padded widths
+template <int W0, int W1>
+__global__ void __launch_bounds__(256) gen_var_mix_kernel(GenMix m) {
+    const int b = blockIdx.x;
+    if (m.n > 1 && b >= m.c[1].var_b0) {
+        const int l = b - m.c[1].var_b0;
+        var_code<1, W1>(m, l % m.c[1].vx, l / m.c[1].vx);
+    } else {
+        const int l = b - m.c[0].var_b0;
+        var_code<0, W0>(m, l % m.c[0].vx, l / m.c[0].vx);
+    }
+}
+
+// grid = both codes' check blocks, block 256; K0 / K1: the codes' check
+// variants
+template <int K0, int K1>
+__global__ void __launch_bounds__(256) gen_check_mix_kernel(GenMix m) {
+    const int b = blockIdx.x;
+    if (m.n > 1 && b >= m.c[1].chk_b0) {
+        const int l = b - m.c[1].chk_b0, cx = (m.c[1].nch + 3) / 4;
+        check_code<1, K1>(m, l / cx, (l % cx) * 4 + wave_id());
+    } else {
+        const int l = b - m.c[0].chk_b0, cx = (m.c[0].nch + 3) / 4;
+        check_code<0, K0>(m, l / cx, (l % cx) * 4 + wave_id());
+    }
+}
+
+// grid = all groups, block 64
+__global__ void __launch_bounds__(64) gen_book_mix_kernel(GenMix m) {
+    const int gy = blockIdx.x;
+    if (code_of_group(m, gy) == 0) {
+        const GenCode &k = m.c[0];
+        book_group(k.s, k.tp, k.count, k.nt, k.max_iter, k.warmup, k.interval, gy - k.group0, threadIdx.x);
+    } else {
+        const GenCode &k = m.c[1];
+        book_group(k.s, k.tp, k.count, k.nt, k.max_iter, k.warmup, k.interval, gy - k.group0, threadIdx.x);
+    }
+}
+
+// grid = the codes' post blocks one after another, block 256
+__global__ void __launch_bounds__(256) gen_post_mix_kernel(GenMix m) {
+    const int b = blockIdx.x;
+    if (m.n > 1 && b >= m.c[1].post0) post_code<1>(m, b - m.c[1].post0);
+    else post_code<0>(m, b - m.c[0].post0);
+}
+
 template <int MAXC, int DEG>
 void launch_check(const DevGraph &g, const GenState &s, const BookArgs &b, hipStream_t st) {
     constexpr int CPW = (DEG == 0 || DEG <= 8) ? 4 : 2;  // registers: CPW*DEG messages in flight
@@ -628,6 +745,94 @@ int gen_decode(const DevGraph &g, const GenState &s, TrialPlanes tp, int count, 
     cleanup();
     *steps_out = steps;
     return static_cast<int>(e);
+}
+
+int gen_decode_mix(GenMix &m, int *h_ctrl, hipEvent_t *evs, hipStream_t st, int64_t *steps_out) {
+    *steps_out = 0;
+    if (m.n < 1 || m.n > kMaxMixCodes) return -2;
+    int groups = 0, vblocks = 0, cblocks = 0, post = 0;
+    int64_t limit = 0;
+    constexpr int kBatch = 4;
+    for (int c = 0; c < m.n; ++c) {
+        GenCode &k = m.c[c];
+        if (k.count <= 0 || k.s.groups <= 0) return -2;
+        k.group0 = groups;
+        groups += k.s.groups;
+        k.vx = (k.g.n + 4 * 16 - 1) / (4 * 16);
+        k.var_b0 = vblocks;
+        vblocks += k.vx * k.s.groups;
+        k.cvar = mix_check_variant(k.g, k.cpw);
+        k.nch0 = (k.g.side[0].m + k.cpw - 1) / k.cpw;
+        k.nch = k.nch0 + (k.g.side[1].m + k.cpw - 1) / k.cpw;
+        k.chk_b0 = cblocks;
+        cblocks += (k.nch + 3) / 4 * k.s.groups;
+        k.lx = (std::max(k.g.side[0].m, k.g.side[1].m) + 255) / 256;
+        k.ex = ((k.g.n + 63) / 64 + 3) / 4;
+        k.post0 = post;
+        post += k.ex * kEmitRows + k.lx * k.s.groups;
+        k.warmup = std::max(0, k.warmup);
+        k.interval = std::max(1, k.interval);
+        // reset: every slot empty, counters zero (fill kernels)
+        launch_fill_i32(reinterpret_cast<int *>(k.s.bad), 0, 2 * (kBadWays + 3) * k.s.groups, st);
+        launch_fill_i32(k.s.ctrl, 0, 16, st);
+        hipLaunchKernelGGL(gen_zero_kernel, dim3(k.s.groups), dim3(64), 0, st, k.g, k.s);
+        // each trial occupies its slot for at most max_iter + 1 steps
+        const int64_t waves = (k.count + 64LL * k.s.groups - 1) / (64LL * k.s.groups);
+        limit = std::max(limit, (waves + 1) * (static_cast<int64_t>(k.max_iter) + 1) + 4 * kBatch);
+    }
+    hipLaunchKernelGGL(gen_book_mix_kernel, dim3(groups), dim3(64), 0, st, m);
+    hipLaunchKernelGGL(gen_post_mix_kernel, dim3(post), dim3(256), 0, st, m);
+    // the instantiations for this pair of codes (a lone code fills both)
+    const GenCode &k0 = m.c[0], &k1 = m.c[m.n > 1 ? 1 : 0];
+    auto wsel = [](int w) { return w == 4 ? 0 : (w == 8 ? 1 : 2); };
+    using Kern = void (*)(GenMix);
+    static const Kern var_tab[3][3] = {
+        {gen_var_mix_kernel<4, 4>, gen_var_mix_kernel<4, 8>, gen_var_mix_kernel<4, 16>},
+        {gen_var_mix_kernel<8, 4>, gen_var_mix_kernel<8, 8>, gen_var_mix_kernel<8, 16>},
+        {gen_var_mix_kernel<16, 4>, gen_var_mix_kernel<16, 8>, gen_var_mix_kernel<16, 16>}};
+    static const Kern chk_tab[3][3] = {
+        {gen_check_mix_kernel<kMixChk7, kMixChk7>, gen_check_mix_kernel<kMixChk7, kMixChk12>,
+         gen_check_mix_kernel<kMixChk7, kMixChkAny>},
+        {gen_check_mix_kernel<kMixChk12, kMixChk7>, gen_check_mix_kernel<kMixChk12, kMixChk12>,
+         gen_check_mix_kernel<kMixChk12, kMixChkAny>},
+        {gen_check_mix_kernel<kMixChkAny, kMixChk7>, gen_check_mix_kernel<kMixChkAny, kMixChk12>,
+         gen_check_mix_kernel<kMixChkAny, kMixChkAny>}};
+    const Kern var_k = var_tab[wsel(k0.g.vpad_w)][wsel(k1.g.vpad_w)];
+    const Kern chk_k = chk_tab[k0.cvar][k1.cvar];
+    int64_t steps = 0;
+    int pend = -1;
+    for (;;) {
+        steps += kBatch;
+        for (int b = 0; b < kBatch; ++b) {
+            hipLaunchKernelGGL(var_k, dim3(vblocks), dim3(256), 0, st, m);
+            hipLaunchKernelGGL(chk_k, dim3(cblocks), dim3(256), 0, st, m);
+            hipLaunchKernelGGL(gen_book_mix_kernel, dim3(groups), dim3(64), 0, st, m);
+            hipLaunchKernelGGL(gen_post_mix_kernel, dim3(post), dim3(256), 0, st, m);
+        }
+        hipError_t e = hipGetLastError();
+        // each code's finished count after this batch, read back one batch
+        // later (two copies and events alternate)
+        const int slot = static_cast<int>((steps / kBatch) & 1);
+        for (int c = 0; c < m.n && e == hipSuccess; ++c)
+            e = hipMemcpyAsync(h_ctrl + slot * kMaxMixCodes + c, m.c[c].s.ctrl + 1, sizeof(int),
+                               hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipEventRecord(evs[slot], st);
+        if (e != hipSuccess) return static_cast<int>(e);
+        if (pend >= 0) {
+            e = hipEventSynchronize(evs[pend]);
+            if (e != hipSuccess) return static_cast<int>(e);
+            bool all = true;
+            for (int c = 0; c < m.n; ++c) all = all && h_ctrl[pend * kMaxMixCodes + c] >= m.c[c].count;
+            if (all) break;
+        }
+        pend = slot;
+        if (steps > limit) {
+            (void)hipStreamSynchronize(st);
+            return -1;
+        }
+    }
+    *steps_out = steps;
+    return static_cast<int>(hipStreamSynchronize(st));
 }
 
 }  // namespace jbp

@@ -44,4 +44,32 @@ int gen_decode(const DevGraph &g, const GenState &s, TrialPlanes tp, int count, 
                int max_iter, int warmup, int interval, int *h_ctrl, hipEvent_t *evs,
                hipStream_t st, int64_t *steps_out, bool use_graph);
 
+// One code's share of a mixed engine: its graph, its own resident groups and
+// state, its trials and their channel table.  gen_decode_mix fills the
+// launch geometry (group0 ..).
+struct GenCode {
+    DevGraph g;
+    GenState s;
+    TrialPlanes tp;
+    NoiseTab nt;
+    int count = 0, max_iter = 0, warmup = 0, interval = 1;
+    int group0 = 0;                 // first engine group of this code
+    int vx = 0;                     // variable-pass blocks per group
+    int var_b0 = 0, chk_b0 = 0;     // first variable-pass and check-pass block
+    int cvar = 0, cpw = 4;          // check-pass variant (mixed kernels), checks per wavefront
+    int nch0 = 0, nch = 0;          // check chunks of side 0, of both sides
+    int ex = 0, lx = 0, post0 = 0;  // post-pass emit and load blocks, first block
+};
+constexpr int kMaxMixCodes = 2;
+struct GenMix {
+    GenCode c[kMaxMixCodes];
+    int n = 0;
+};
+
+// Decode every code's trials 0..count-1 in one engine: each step's kernels
+// cover all codes (launched, not graph-replayed).  h_ctrl: pinned host ints
+// [2][kMaxMixCodes]; evs: two events.  Returns 0, a hipError_t value, -1
+// when the step bound is exceeded, or -2 for an empty or oversized mix.
+int gen_decode_mix(GenMix &m, int *h_ctrl, hipEvent_t *evs, hipStream_t st, int64_t *steps_out);
+
 }  // namespace jbp

@@ -125,3 +125,27 @@ def test_hgp_workload_full_size(hgp150):
     assert s32["failures"] / n <= hi
     assert both >= 0.98
     assert agree >= 0.9
+
+
+def test_mixed_codes_in_one_engine_full_size(main_decoder, hgp150):
+    # cfg3 in one engine: the included code's 4 x 4,096 interleaved-p trials
+    # and HGP(150 x 200)'s 2,048 trials at p = 0.03, max-iter 200, decoded
+    # together (every engine step's kernels cover both codes); every trial
+    # equals the same trial of that code's own HBM-engine run
+    code, hdec = hgp150
+    n_qc, n_h, it = 4 * 4096, 2048, 200
+    print("mixed codes, one engine", flush=True)
+    (sq, rq), (sh, rh) = jbp.simulate_mixed([
+        (main_decoder, dict(base_seed=106, n_trials=n_qc, p=PS, max_iter=it)),
+        (hdec, dict(base_seed=1, n_trials=n_h, p=0.03, max_iter=it))])
+    print("separate runs", flush=True)
+    _, aq = main_decoder.simulate(106, n_qc, PS, max_iter=it, engine=jbp.ENGINE_HBM)
+    _, ah = hdec.simulate(1, n_h, 0.03, max_iter=it, engine=jbp.ENGINE_HBM)
+    for k in ("iterations", "syn_match", "success", "exact"):
+        assert np.array_equal(rq[k], aq[k]), ("qc", k)
+        assert np.array_equal(rh[k], ah[k]), ("hgp", k)
+    assert sq["done"] == n_qc and sh["done"] == n_h
+    assert sq["failures"] == int(np.sum(1 - rq["success"])) and sh["failures"] == int(np.sum(1 - rh["success"]))
+    print("included code: failures %d avg_iter %.3f; HGP: failures %d avg_iter %.3f; one engine %.1f ms"
+          % (sq["failures"], sq["total_iters"] / n_qc, sh["failures"], sh["total_iters"] / n_h,
+             sq["device_ms"]), flush=True)

@@ -42,6 +42,31 @@ def test_sampling_runs_in_pieces_past_the_generator_scratch(small_hgp):
     assert 0.02 < np.mean(err != 0) < 0.08
 
 
+@pytest.mark.parametrize("order", [0, 1])
+def test_mixed_codes_equal_separate_hbm_runs(main_decoder, small_hgp, order):
+    # two codes in one engine (either code first; the HGP code with damping
+    # 0.3, the included one with an interleaved p list and a check schedule):
+    # per trial the same as each code's own HBM-engine run
+    code, hdec = small_hgp
+    qc = (main_decoder, dict(base_seed=106, n_trials=1500, p=PS, max_iter=60, first_trial=7,
+                             check_warmup=2, check_interval=3))
+    hg = (hdec, dict(base_seed=3, n_trials=300, p=0.06, max_iter=40, damping=0.3, slots=128))
+    runs = [qc, hg] if order == 0 else [hg, qc]
+    print("mixed codes, order", order, flush=True)
+    res = jbp.simulate_mixed(runs)
+    for (dec, kw), (st, r) in zip(runs, res):
+        _, a = dec.simulate(kw["base_seed"], kw["n_trials"], kw["p"], max_iter=kw["max_iter"],
+                            first_trial=kw.get("first_trial", 1), damping=kw.get("damping", 0.0),
+                            check_warmup=kw.get("check_warmup", 0),
+                            check_interval=kw.get("check_interval", 1), engine=jbp.ENGINE_HBM)
+        assert per_trial_equal(r, a), kw
+        assert st["done"] == kw["n_trials"]
+    # a code with no trials takes no part
+    (s0, r0), (s1, r1) = jbp.simulate_mixed([(main_decoder, dict(base_seed=106, n_trials=0, p=0.04)),
+                                             (hdec, dict(base_seed=3, n_trials=64, p=0.06, max_iter=40))])
+    assert s0["done"] == 0 and s1["done"] == 64
+
+
 def per_trial_equal(a, b, keys=("iterations", "syn_match", "success", "exact")):
     return all(np.array_equal(a[k], b[k]) for k in keys)
 

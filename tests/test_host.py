@@ -139,3 +139,17 @@ def test_pack_bits_roundtrip():
     assert b.shape == (3, 3)
     back = ((b[:, :, None] >> np.arange(32, dtype=np.uint32)) & 1).reshape(3, -1)[:, :77]
     assert np.array_equal(back, a)
+
+
+def test_simulate_mixed_rejects_bad_arguments():
+    # checked before any device call (1: JBP_ERR_ARG)
+    L = jbp.lib()
+    e = C.create_string_buffer(512)
+    cfgs = (jbp.SimCfg * 3)()
+    ctxs = (C.c_void_p * 3)()
+    stats = (jbp.RunStats * 3)()
+    assert L.jbp_simulate_mixed(ctxs, cfgs, 0, None, stats, e, 512) == 1
+    assert L.jbp_simulate_mixed(ctxs, cfgs, 3, None, stats, e, 512) == 1
+    assert b"codes" in e.value
+    assert L.jbp_simulate_mixed(ctxs, cfgs, 1, None, stats, e, 512) == 1
+    assert b"code 0" in e.value
