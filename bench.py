@@ -48,6 +48,7 @@ EDGES_PER_ITER = 2 * 27648
 BYTES_PER_ITER = 16 * EDGES_PER_ITER  # 884,736
 HBM_PEAK_GBS = 8000.0
 LEARNED_MS_FILE = os.path.join(ROOT, "joint_bp_plus_pp_amd", "codes", "learned_ms_T25_P768_J3_L12.npz")
+LEARNED_LMS_FILE = os.path.join(ROOT, "joint_bp_plus_pp_amd", "codes", "learned_lms_T25_P768_J3_L12.npz")
 LEARNED_SP_FILES = [os.path.join(ROOT, "joint_bp_plus_pp_amd", "codes", f)
                     for f in ("learned_T25_P768_J3_L12.npz", "learned_T25_nomlp_P768_J3_L12.npz")]
 KERNEL_SOURCE = os.path.join(ROOT, "joint_bp_plus_pp_amd", "csrc", "jbp_decode_f32.hip")
@@ -56,6 +57,8 @@ VALU_PROFILE = os.path.join(ROOT, "profiles", "decode_qc_f32_kernels_p0.04_count
 # VALU instruction per SIMD every 4 cycles
 VALU_PEAK_GSLOTS = 256 * 4 * 2.4 / 4.0
 MIXED_PS = (0.01, 0.025, 0.04, 0.05)
+LEARNED_KINDS = {0: "learned sum-product", 1: "learned flooding min-sum",
+                 2: "learned layered min-sum"}
 
 
 def wilson(n, k, z=1.96):
@@ -436,31 +439,47 @@ def copy_calibration(mib):
     print("copy calibration: %d MiB device-to-device" % mib, file=sys.stderr)
 
 
+def ms_kernel_name(model):
+    """Instance of decode_qc_ms_kernel<P, J, L, DAMPED, LAYERED> that runs a
+    learned model on the included code."""
+    return "decode_qc_ms_kernel<768, 3, 12, %s, %s>" % (
+        "true" if (model.kind == 1 and model.use_mlp) else "false",
+        "true" if model.kind == 2 else "false")
+
+
 def profile_run(jbp, args):
     """The workload rocprofv3 counter passes run (--profile-run FILE):
     --roofline-steps single launches of --chunk trials of the learned
-    decoder, then of plain BP on the same indices.  Writes the trial
-    iterations each kernel did, which the counter totals are divided by."""
+    decoder (--weights), of the second learned decoder (--compare-weights)
+    and of plain BP, each on the same indices.  Writes the trial iterations
+    each kernel did, which the counter totals are divided by."""
     from joint_bp_plus_pp_amd import learned as jl
     code = jbp.Code.from_params(PARAMS)
     dec = jbp.Decoder(code, 0)
-    model = jl.load(dec, LEARNED_MS_FILE)
-    T, n, R = model.T, args.chunk, args.roofline_steps
-    ms_it = fb_it = pl_it = 0
+    models = [("ms", jl.load(dec, args.weights), args.weights)]
+    if args.compare_weights:
+        models.append(("ms_other", jl.load(dec, args.compare_weights), args.compare_weights))
+    n, R = args.chunk, args.roofline_steps
+    its = {key: 0 for key, _, _ in models}
+    fb_it = pl_it = 0
     for k in range(R):
-        _, r = dec.simulate_learned(model, args.seed, n, args.p, max_iter=args.max_iter,
-                                    first_trial=1 + k * n, chunk=n)
-        ms_it += int(np.minimum(r["iterations"], T).sum())
-        fb_it += int(np.maximum(r["iterations"] - T, 0).sum())
+        for key, model, _ in models:
+            _, r = dec.simulate_learned(model, args.seed, n, args.p, max_iter=args.max_iter,
+                                        first_trial=1 + k * n, chunk=n)
+            its[key] += int(np.minimum(r["iterations"], model.T).sum())
+            fb_it += int(np.maximum(r["iterations"] - model.T, 0).sum())
         st, _ = dec.simulate(args.seed, n, args.p, max_iter=args.max_iter, first_trial=1 + k * n,
                              precision=32, per_trial=False, chunk=n)
         pl_it += st["total_iters"]
     out = {"launches": R, "trials_per_launch": n, "p": args.p, "max_iter": args.max_iter,
-           "ms_trial_iterations": ms_it, "plain_trial_iterations": pl_it + fb_it,
-           "fallback_trial_iterations": fb_it}
+           "plain_trial_iterations": pl_it + fb_it, "fallback_trial_iterations": fb_it,
+           "learned": {key: {"weights": os.path.relpath(path, ROOT), "kind": int(model.kind),
+                             "kernel": ms_kernel_name(model), "trial_iterations": its[key]}
+                       for key, model, path in models}}
     with open(args.profile_run, "w") as f:
         json.dump(out, f)
-    model.destroy()
+    for _, model, _ in models:
+        model.destroy()
     dec.destroy()
     print("profile run: %s" % json.dumps(out), flush=True)
 
@@ -475,21 +494,24 @@ def summarize_counters(args):
     import collections
     with open(args.profile_log) as f:
         log = json.load(f)
-    tot = {"ms": collections.defaultdict(float), "plain": collections.defaultdict(float)}
-    disp = {"ms": set(), "plain": set()}
+    kernels = {key: d["kernel"] for key, d in log["learned"].items()}
+    kernels["plain"] = "decode_qc_f32_kernel<768, 3, 12, false, false>"
+    iters = {key: d["trial_iterations"] for key, d in log["learned"].items()}
+    iters["plain"] = log["plain_trial_iterations"]
+    tot = {key: collections.defaultdict(float) for key in kernels}
+    passes = collections.defaultdict(set)  # the CSVs that hold each counter
+    disp = {key: set() for key in kernels}
     copy = collections.defaultdict(list)
     for path in args.counters:
         for r in csv.DictReader(open(path)):
             name, c, v = r["Kernel_Name"], r["Counter_Name"], float(r["Counter_Value"])
-            if "decode_qc_ms_kernel" in name:
-                key = "ms"
-            elif "decode_qc_f32_kernel<768, 3, 12, false, false>" in name:
-                key = "plain"
-            else:
+            key = next((k for k, kn in kernels.items() if kn in name), None)
+            if key is None:
                 if "copyBuffer" in name:
                     copy[c].append(v)
                 continue
             tot[key][c] += v
+            passes[c].add(path)
             disp[key].add((path, r["Dispatch_Id"]))
     out = {"source": os.path.relpath(KERNEL_SOURCE, ROOT), "source_sha256": source_sha256(KERNEL_SOURCE),
            "workload": log, "command": args.command}
@@ -499,9 +521,11 @@ def summarize_counters(args):
     for c in ("FETCH_SIZE", "WRITE_SIZE"):
         if copy.get(c) and cb:
             scale[c] = cb / (max(copy[c]) * 1024.0)
-    for key, its in (("ms", log["ms_trial_iterations"]), ("plain", log["plain_trial_iterations"])):
-        t = tot[key]
-        d = {"dispatches_per_pass": len(disp[key]) // max(1, len(args.counters)),
+    for key, its in iters.items():
+        # a counter collected in several passes: the mean over those passes
+        t = {c: v / len(passes[c]) for c, v in tot[key].items()}
+        d = {"kernel": kernels[key],
+             "dispatches_per_pass": len(disp[key]) // max(1, len(args.counters)),
              "trial_iterations": its}
         for c, v in sorted(t.items()):
             d[c + "_per_trial_iteration"] = v / its
@@ -530,6 +554,10 @@ def main():
     ap.add_argument("--trials", type=int, default=65536, help="trials per step")
     ap.add_argument("--steps-per-call", type=int, default=0,
                     help="steps per simulate call in the timed region (0: all steps in one call)")
+    ap.add_argument("--weights", default=LEARNED_LMS_FILE,
+                    help="learned decoder of the headline (default: the layered min-sum model)")
+    ap.add_argument("--compare-weights", default=LEARNED_MS_FILE,
+                    help="second learned decoder timed on the same trials ('' to skip)")
     ap.add_argument("--chunk", type=int, default=16384,
                     help="trials per persistent launch; launches alternate two streams")
     ap.add_argument("--roofline-steps", type=int, default=20,
@@ -599,7 +627,8 @@ def main():
     code = jbp.Code.from_params(PARAMS)
     dec = jbp.Decoder(code, 0)
     info = dec.info()
-    model = jl.load(dec, LEARNED_MS_FILE)
+    model = jl.load(dec, args.weights)
+    other = jl.load(dec, args.compare_weights) if args.compare_weights else None
     T = model.T
     N = args.trials
     # warm-up: both decoders, indices outside the timed range
@@ -615,7 +644,7 @@ def main():
     # its whole trial count in one loop)
     per_call = args.steps if args.steps_per_call <= 0 else min(args.steps_per_call, args.steps)
 
-    def run(learned):
+    def run(learned, m=None):
         done = fails = iters = 0
         nxt = 1
         t0 = time.perf_counter()
@@ -623,8 +652,9 @@ def main():
         while left > 0:
             k = min(per_call, left)
             if learned:
-                st, _ = dec.simulate_learned(model, args.seed, k * N, args.p, max_iter=args.max_iter,
-                                             first_trial=nxt, per_trial=False, chunk=args.chunk)
+                st, _ = dec.simulate_learned(m or model, args.seed, k * N, args.p,
+                                             max_iter=args.max_iter, first_trial=nxt,
+                                             per_trial=False, chunk=args.chunk)
             else:
                 st, _ = dec.simulate(args.seed, k * N, args.p, max_iter=args.max_iter, first_trial=nxt,
                                      precision=32, per_trial=False, chunk=args.chunk)
@@ -637,6 +667,18 @@ def main():
 
     l_done, l_fails, l_iters, l_el = run(True)
     p_done, p_fails, p_iters, p_el = run(False)
+    compare = None
+    if other is not None:
+        for w in range(args.warmup):
+            dec.simulate_learned(other, args.seed, N, args.p, max_iter=args.max_iter,
+                                 first_trial=wfirst + w * N, per_trial=False, chunk=args.chunk)
+        o_done, o_fails, o_iters, o_el = run(True, other)
+        compare = {"weights": os.path.relpath(args.compare_weights, ROOT),
+                   "decoder": LEARNED_KINDS[other.kind], "trials": o_done,
+                   "trials_per_s": round(o_done / o_el, 2), "avg_iter": round(o_iters / o_done, 4),
+                   "fer": o_fails / o_done, "fer_ci95": list(wilson(o_done, o_fails)),
+                   "speedup_vs_plain": round((o_done / o_el) / (p_done / p_el), 4)}
+        print("bench: compare %s" % json.dumps(compare), file=sys.stderr)
     value = l_done / l_el
     llo, lhi = wilson(l_done, l_fails)
     plo, phi = wilson(p_done, p_fails)
@@ -666,8 +708,15 @@ def main():
     ms_rate = sum(ms_iters) / (sum(ms_ms) / 1e3)     # learned-kernel trial-iterations/s
     pl_rate = sum(pl_iters) / (sum(pl_ms) / 1e3)     # plain-kernel trial-iterations/s
     prof, why = kernel_profile()
+    # the profile's entry for the headline decoder's kernel instance
+    mine = None
+    if prof:
+        mine = next((d for k, d in prof.items() if isinstance(d, dict) and
+                     d.get("kernel") == ms_kernel_name(model)), None)
+        if mine is None:
+            prof, why = None, "counter profile has no entry for %s" % ms_kernel_name(model)
     roofline = {"bound": "valu", "unit": "Gslots/s", "peak": VALU_PEAK_GSLOTS,
-                "kernel": "decode_qc_ms_kernel (learned min-sum, FP16 pairs)",
+                "kernel": "%s (%s, FP16 pairs)" % (ms_kernel_name(model), LEARNED_KINDS[model.kind]),
                 "kernel_ms_per_launch": round(sum(ms_ms) / n_r, 4), "launches": n_r,
                 "trial_iterations_per_s": round(ms_rate, 1),
                 "streaming_equivalent_gbs": round(ms_rate * BYTES_PER_ITER / 1e9, 1)}
@@ -684,11 +733,11 @@ def main():
                   "trial_iterations_per_s": round(pl_rate, 1),
                   "streaming_equivalent_gbs": round(pl_rate * BYTES_PER_ITER / 1e9, 1)}
     if prof:
-        v = prof["ms"]["valu_slots_per_trial_iteration"]
+        v = mine["valu_slots_per_trial_iteration"]
         roofline.update({"achieved": round(v * ms_rate / 1e9, 2),
                          "frac": round(v * ms_rate / 1e9 / VALU_PEAK_GSLOTS, 4),
                          "valu_slots_per_trial_iteration": v,
-                         "traffic": prof["ms"].get("hbm_bytes_per_launch"),
+                         "traffic": mine.get("hbm_bytes_per_launch"),
                          "profile": os.path.relpath(VALU_PROFILE, ROOT)})
         vp = prof["plain"]["valu_slots_per_trial_iteration"]
         plain_roof.update({"valu_achieved": round(vp * pl_rate / 1e9, 2),
@@ -731,14 +780,15 @@ def main():
         "dtype": "fp16",
         "data": "synthetic",
         "config": {
-            "workload": "learned min-sum BP (T=%d, shipped weights) + plain FP32 BP fallback to "
+            "workload": "%s BP (T=%d, shipped weights) + plain FP32 BP fallback to "
                         "max-iter %d, included [[9216,4612]] code, p=%g, %d trials per step, "
                         "%d steps per simulate call, in launches of %d trials pipelined over "
-                        "four streams, seed %d; plain FP32 BP on the same trial indices beside "
-                        "it" % (T, args.max_iter, args.p, N, per_call, args.chunk, args.seed),
-            "decoder": "learned min-sum, FP16 (X, Z) message pairs, LDS-resident; fallback plain "
-                       "FP32 sum-product",
-            "weights": os.path.relpath(LEARNED_MS_FILE, ROOT),
+                        "three streams, seed %d; plain FP32 BP on the same trial indices beside "
+                        "it" % (LEARNED_KINDS[model.kind], T, args.max_iter, args.p, N, per_call,
+                                args.chunk, args.seed),
+            "decoder": "%s, FP16 (X, Z) message pairs, LDS-resident; fallback plain "
+                       "FP32 sum-product" % LEARNED_KINDS[model.kind],
+            "weights": os.path.relpath(args.weights, ROOT),
             "trials_per_step": N, "steps_per_call": per_call, "launch_trials": args.chunk, "p": args.p,
             "max_iter": args.max_iter, "T": T,
             "trials": l_done, "avg_iter": round(l_iters / l_done, 4),
@@ -751,6 +801,7 @@ def main():
                                   "trial_iterations_per_s": round(p_iters / p_el, 1),
                                   "roofline": plain_roof},
             "speedup_vs_plain": round(value / (p_done / p_el), 4),
+            "other_learned_same_trials": compare,
             "fer_matched": matched,
             "fer_below_plain_ci": below,
             "sp_learned": sp_learned,
@@ -762,6 +813,8 @@ def main():
         "cpu_baseline": cpu,
     }
     model.destroy()
+    if other is not None:
+        other.destroy()
     dec.destroy()
     print(json.dumps(line), flush=True)
 

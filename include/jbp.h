@@ -375,9 +375,15 @@ int jbp_learned_create(jbp_ctx *ctx, int T, jbp_learned **out, char *err, size_t
  * d)); hard decisions from the signs of the X / Z marginals): the weights
  * act as learned normalisation, and the inference decoder runs it on FP16
  * pairs with no transcendental function, several times cheaper per
- * iteration than sum-product BP.  Plain BP stays the fallback. */
+ * iteration than sum-product BP.  LAYERED_MIN_SUM: the min-sum model on a
+ * layered schedule (block row r of both sides is layer r; an iteration
+ * updates the layers in turn, each from the messages the layers before it
+ * left), which converges in fewer iterations; no damping (use_mlp must be
+ * 0); an iteration counts as converged when its layers all start from the
+ * same estimate and every check accepts it.  Plain BP stays the fallback. */
 #define JBP_LEARNED_SUM_PRODUCT 0
 #define JBP_LEARNED_MIN_SUM 1
+#define JBP_LEARNED_LAYERED_MIN_SUM 2
 int jbp_learned_create_kind(jbp_ctx *ctx, int T, int kind, jbp_learned **out, char *err,
                             size_t errlen);
 /* The model's kind (JBP_LEARNED_*), -1 for NULL. */

@@ -29,7 +29,7 @@ FP64 = 64
 STATUS = {0: "ok", 1: "invalid argument", 2: "device error", 3: "unsupported", 4: "out of memory"}
 STATUS_UNSUPPORTED = 3
 ENGINE_AUTO, ENGINE_LDS, ENGINE_HBM = 0, 1, 2
-LEARNED_SUM_PRODUCT, LEARNED_MIN_SUM = 0, 1
+LEARNED_SUM_PRODUCT, LEARNED_MIN_SUM, LEARNED_LAYERED_MIN_SUM = 0, 1, 2
 
 
 class JbpError(RuntimeError):
@@ -594,8 +594,9 @@ class Decoder:
 class LearnedModel:
     """Learned-BP parameters on a decoder's device (include/jbp.h,
     jbp_learned_*).  Layout: wc[2][nb][T] | wv[2][nb][T] | mlp[65].  kind:
-    LEARNED_SUM_PRODUCT (the reference's joint BP with weights) or
-    LEARNED_MIN_SUM (normalised min-sum, the fast inference decoder)."""
+    LEARNED_SUM_PRODUCT (the reference's joint BP with weights),
+    LEARNED_MIN_SUM (normalised min-sum) or LEARNED_LAYERED_MIN_SUM (the
+    min-sum model on a layered schedule: fewer iterations; no damping)."""
 
     MLP_PARAMS = 65
 
@@ -609,7 +610,7 @@ class LearnedModel:
         decoder._models.add(self)
         self.count = lib().jbp_learned_param_count(h)
         self.nb = (self.count - self.MLP_PARAMS) // (4 * T)
-        self.use_mlp = True
+        self.use_mlp = self.kind != LEARNED_LAYERED_MIN_SUM
 
     def destroy(self):
         if self._h and self._h.value:

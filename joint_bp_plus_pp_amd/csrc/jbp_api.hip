@@ -434,7 +434,9 @@ int run_gen(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmu
 }  // namespace
 
 // The learned min-sum decoder runs as its own launch (timed apart).
-static bool learned_ms(const LearnedInfer *li) { return li && li->T > 0 && li->kind == kLearnedMS; }
+static bool learned_ms(const LearnedInfer *li) {
+    return li && li->T > 0 && (li->kind == kLearnedMS || li->kind == kLearnedLMS);
+}
 
 // Decode `count` trials whose syndrome planes are already in ctx->tp.
 int run_decode(jbp_ctx *ctx, int count, const NoiseTab &nt, int max_iter, int warmup, int interval,

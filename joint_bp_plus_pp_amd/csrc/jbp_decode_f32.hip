@@ -328,7 +328,8 @@ __device__ __forceinline__ int qc_phase(float *B, float *GZ, int *s_cnt, float *
 // homogeneous, so the weights are the same in any base):
 //   S      = sum_r wc[r] * lambda[r]                (per side)
 //   F      = max(lp_I + S', lp_Z) - max(lp_X + S', lp_Y)   (X side; S' = the
-//            other side's S; Z side with X and Z swapped)
+//            other side's S; Z side with X and Z swapped), evaluated as the
+//            clamp it equals (ms_marginal)
 //   est    = sign bits of G = S + F
 //   mu[r]  = wv[r] * (G - wc[r] * lambda[r])
 //   lambda = (-1)^syn prod_{k != e} sign(mu_k) * min(64, min_{k != e} |mu_k|)
@@ -347,11 +348,47 @@ __device__ __forceinline__ h2 as_h2(uint32_t u) { return __builtin_bit_cast(h2, 
 __device__ __forceinline__ uint32_t h2_bits(h2 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ u16x2 as_u16x2(uint32_t u) { return __builtin_bit_cast(u16x2, u); }
 __device__ __forceinline__ uint32_t u16x2_bits(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
-__device__ __forceinline__ h2 hmax2(h2 a, h2 b) { return __builtin_elementwise_max(a, b); }
 // magnitudes (sign bits clear) compare as unsigned integers
 __device__ __forceinline__ u16x2 umin2(u16x2 a, u16x2 b) { return __builtin_elementwise_min(a, b); }
 
 constexpr uint32_t kMsClampPair = 0x54005400u;  // (64.0, 64.0) in FP16
+
+// The channel term F(L) = max(a + L, b) - max(c + L, d) of one side (a..d
+// its log-prior terms) is a clamp of a line of slope +-1: when
+// b - a <= d - c it is min(max(L + (a - d), b - d), a - c), otherwise
+// min(max((b - c) - L, a - c), b - d).  Per half of an (X, Z) pair: the
+// slope s, offset k and bounds lo <= hi, so the marginal G = S + F(S') is
+// four packed operations (S' = S with its halves swapped).
+struct MsChan {
+    h2 s, k, lo, hi;
+};
+// one half: (s, k, lo, hi)
+__device__ __forceinline__ void ms_chan_half(float a, float b, float c, float d, float (&v)[4]) {
+    const bool rising = b - a <= d - c;
+    v[0] = rising ? 1.0f : -1.0f;
+    v[1] = rising ? a - d : b - c;
+    v[2] = rising ? b - d : a - c;
+    v[3] = rising ? a - c : b - d;
+}
+__device__ __forceinline__ MsChan ms_chan(const PriorF &pr) {
+    // X half: the Z-side sum enters with (a, b, c, d) = (I, Z, X, Y) terms;
+    // Z half: (I, X, Z, Y)
+    float x[4], z[4];
+    ms_chan_half(pr.lp[0], pr.lp[2], pr.lp[1], pr.lp[3], x);
+    ms_chan_half(pr.lp[0], pr.lp[1], pr.lp[2], pr.lp[3], z);
+    MsChan m;
+    m.s = h2{(_Float16)x[0], (_Float16)z[0]};
+    m.k = h2{(_Float16)x[1], (_Float16)z[1]};
+    m.lo = h2{(_Float16)x[2], (_Float16)z[2]};
+    m.hi = h2{(_Float16)x[3], (_Float16)z[3]};
+    return m;
+}
+__device__ __forceinline__ h2 ms_marginal(h2 S, const MsChan &m) {
+    const h2 Sw = __builtin_shufflevector(S, S, 1, 0);
+    const h2 F = __builtin_elementwise_min(
+        __builtin_elementwise_max(__builtin_elementwise_fma(Sw, m.s, m.k), m.lo), m.hi);
+    return S + F;
+}
 
 template <int P, int J, int L, bool DAMPED>
 __device__ __forceinline__ int qc_phase_ms(uint32_t *Bw, int *s_cnt, float *s_d, uint32_t *s_w,
@@ -392,11 +429,7 @@ __device__ __forceinline__ int qc_phase_ms(uint32_t *Bw, int *s_cnt, float *s_d,
     auto stage = [&](int kk) {
         if (t < kMsRec && kk <= kmax) s_w[(kk & 1) * kMsRec + t] = li.ms_rec[kk * kMsRec + t];
     };
-    // channel constants as (X side, Z side) pairs
-    const h2 cA = {(_Float16)pr.lp[0], (_Float16)pr.lp[0]};
-    const h2 cB = {(_Float16)pr.lp[2], (_Float16)pr.lp[1]};
-    const h2 cC = {(_Float16)pr.lp[1], (_Float16)pr.lp[2]};
-    const h2 cD = {(_Float16)pr.lp[3], (_Float16)pr.lp[3]};
+    const MsChan ch = ms_chan(pr);  // channel term, (X side, Z side) pairs
     // damped: the previous check messages, mixed into the new ones
     h2 lam[DAMPED ? J : 1][DAMPED ? L : 1];
     if (DAMPED) {
@@ -461,8 +494,7 @@ __device__ __forceinline__ int qc_phase_ms(uint32_t *Bw, int *s_cnt, float *s_d,
 #pragma unroll
                 for (int r = 1; r < J; ++r) S = __builtin_elementwise_fma(wc[r], lv[r], S);
             }
-            const h2 Sw = __builtin_shufflevector(S, S, 1, 0);
-            const h2 G = S + (hmax2(Sw + cA, cB) - hmax2(Sw + cC, cD));
+            const h2 G = ms_marginal(S, ch);
             const uint32_t eb = (h2_bits(G) >> 15) & 0x00010001u;
             acc_est |= eb << cc;
 #pragma unroll
@@ -535,6 +567,120 @@ __device__ __forceinline__ int qc_phase_ms(uint32_t *Bw, int *s_cnt, float *s_d,
             dprev = *s_d;
         }
         ++k;
+    }
+}
+
+// Layered learned min-sum phase (kind kLearnedLMS, the training kernels'
+// learned_*_lms_kernel in base 2): block row r of both sides is layer r;
+// iteration k updates the layers in turn.  Thread t keeps the weighted check
+// messages of its variables (cc, t) for all three rows in registers (36
+// (X, Z) pairs); layer r is
+//   var step:  S = sum_r' lambda[r'],  G = S + F(S'),  est = sign bits of G,
+//              mu = wv[k][r] * (G - lambda[r]) written to row r's slots
+//   check step (row r, thread t's X and Z checks): min-sum as above
+//   read back: lambda[r] = wc[k][r] * (row r's new check outputs)
+// so each layer starts from the messages the layers before it left.  An
+// iteration converges when its three layers start from the same estimate
+// (est) and every row's parity (taken from the estimate bits the layer's mu
+// carry) matches the syndrome.  The packed slot addresses are laundered at
+// each layer so that only the current layer's are unpacked at a time.
+template <int P, int J, int L>
+__device__ __forceinline__ int qc_phase_lms(uint32_t *Bw, uint32_t *s_w, uint32_t (&aw)[2][J][(L + 1) / 2],
+                                            uint32_t synx, uint32_t synz, const PriorF &pr, int kmax,
+                                            const LearnedInfer &li, uint32_t &estx, uint32_t &estz,
+                                            int &matched) {
+    constexpr int HB = J * L / 2;
+    const int t = threadIdx.x;
+    auto launder = [&]() {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int r = 0; r < J; ++r)
+#pragma unroll
+                for (int w = 0; w < (L + 1) / 2; ++w) asm volatile("" : "+v"(aw[s][r][w]));
+    };
+    auto at = [&](int s, int r, int cc) -> uint32_t {
+        const uint32_t a16 = (cc & 1) ? (aw[s][r][cc >> 1] >> 16) : (aw[s][r][cc >> 1] & 0xffffu);
+        return a16 + static_cast<uint32_t>(((r * L + cc) % HB) * P * 4) + (s ? 2u : 0u);
+    };
+    auto ldh = [](uint32_t a) -> _Float16 { return *reinterpret_cast<lds_h16 *>(static_cast<uintptr_t>(a)); };
+    auto sth = [](uint32_t a, _Float16 v) { *reinterpret_cast<lds_h16 *>(static_cast<uintptr_t>(a)) = v; };
+    auto stage = [&](int kk) {
+        if (t < kMsRec && kk < kmax) s_w[(kk & 1) * kMsRec + t] = li.ms_rec[kk * kMsRec + t];
+    };
+    const MsChan ch = ms_chan(pr);
+    h2 lam[J][L];
+#pragma unroll
+    for (int r = 0; r < J; ++r)
+#pragma unroll
+        for (int cc = 0; cc < L; ++cc) lam[r][cc] = h2{(_Float16)0.0f, (_Float16)0.0f};
+    stage(0);
+    int k = 0;
+    __syncthreads();
+    for (;;) {
+        stage(k + 1);
+        const uint32_t *rec = s_w + (k & 1) * kMsRec;
+        uint32_t est0 = 0, estchg = 0, bad = 0;
+#pragma unroll
+        for (int r = 0; r < J; ++r) {
+            uint32_t est_r = 0;
+            launder();
+#pragma unroll
+            for (int cc = 0; cc < L; ++cc) {
+                const h2 wv = as_h2(rec[cc * kMsRecCC + 3 + r]);
+                const h2 S = lam[0][cc] + lam[1][cc] + lam[2][cc];
+                const h2 G = ms_marginal(S, ch);
+                const uint32_t eb = (h2_bits(G) >> 15) & 0x00010001u;
+                est_r |= eb << cc;
+                const h2 mb = as_h2((h2_bits(wv * (G - lam[r][cc])) & 0xfffefffeu) | eb);
+                sth(at(0, r, cc), mb.x);
+                sth(at(1, r, cc), mb.y);
+            }
+            if (r == 0) est0 = est_r;
+            else estchg |= est_r ^ est0;
+            __syncthreads();
+            {
+                lds_u32 *row = reinterpret_cast<lds_u32 *>(reinterpret_cast<uintptr_t>(Bw)) + r * L * P + t;
+                const uint32_t synw = ((synx >> r) & 1u) | (((synz >> r) & 1u) << 16);
+                uint32_t u[L];
+                u16x2 pre[L];
+                u16x2 run = as_u16x2(kMsClampPair);
+                uint32_t acc = 0;
+#pragma unroll
+                for (int i = 0; i < L; ++i) {
+                    u[i] = row[i * P];
+                    acc ^= u[i];
+                    pre[i] = run;
+                    run = umin2(run, as_u16x2(u[i] & 0x7fff7fffu));
+                }
+                bad |= (acc ^ synw) & 0x00010001u;  // estimate parity vs syndrome
+                const uint32_t sall = acc ^ (synw << 15);
+                u16x2 suf = as_u16x2(kMsClampPair);
+#pragma unroll
+                for (int i = L - 1; i >= 0; --i) {
+                    const uint32_t mag = u16x2_bits(umin2(pre[i], suf));
+                    suf = umin2(suf, as_u16x2(u[i] & 0x7fff7fffu));
+                    row[i * P] = mag | ((sall ^ u[i]) & 0x80008000u);
+                }
+            }
+            __syncthreads();
+            launder();
+#pragma unroll
+            for (int cc = 0; cc < L; ++cc)
+                lam[r][cc] = as_h2(rec[cc * kMsRecCC + r]) * h2{ldh(at(0, r, cc)), ldh(at(1, r, cc))};
+        }
+        ++k;
+        const int any_bad = __syncthreads_or(static_cast<int>(bad | estchg));
+        estx = est0 & 0xffffu;
+        estz = est0 >> 16;
+        if (!any_bad) {
+            matched = 1;
+            return k;
+        }
+        if (k >= kmax) {
+            matched = 0;
+            return k;
+        }
     }
 }
 
@@ -716,7 +862,7 @@ __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, 
 // (fb_ctr[0] of them) for plain BP, launched next on the same stream, which
 // decodes them from the channel prior (reported iterations li.T + its own).
 // Its exchange buffer is B only (J*L*P words).
-template <int P, int J, int L, bool DAMPED>
+template <int P, int J, int L, bool DAMPED, bool LAYERED>
 __global__ void __launch_bounds__(P) decode_qc_ms_kernel(DevGraph g, QcDev qc, TrialPlanes tp,
                                                         int count, NoiseTab nt, int *counter,
                                                         LearnedInfer li, int *fb_list, int *fb_ctr) {
@@ -755,9 +901,13 @@ __global__ void __launch_bounds__(P) decode_qc_ms_kernel(DevGraph g, QcDev qc, T
         }
         uint32_t estx = 0, estz = 0;
         int matched = 0;
-        const int k = qc_phase_ms<P, J, L, DAMPED>(reinterpret_cast<uint32_t *>(B), s_cnt, &s_d, s_w,
-                                                   aw, synx, synz, pr, li.T, li, u0, estx, estz,
-                                                   matched);
+        int k;
+        if constexpr (LAYERED)
+            k = qc_phase_lms<P, J, L>(reinterpret_cast<uint32_t *>(B), s_w, aw, synx, synz, pr, li.T, li,
+                                      estx, estz, matched);
+        else
+            k = qc_phase_ms<P, J, L, DAMPED>(reinterpret_cast<uint32_t *>(B), s_cnt, &s_d, s_w, aw, synx,
+                                             synz, pr, li.T, li, u0, estx, estz, matched);
         if (matched) {
             write_outcome<P, L>(g, tp, trial, estx, estz, k, 1);
         } else if (t == 0) {
@@ -834,7 +984,7 @@ void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_ta
         launch(decode_qc_f32_kernel<768, 3, 12, false, false>, fb, offset, fb + 1, fbp, on, max_iter,
                nullptr, nullptr);
     };
-    if (learned && learned->T > 0 && learned->kind == kLearnedMS) {
+    if (learned && learned->T > 0 && (learned->kind == kLearnedMS || learned->kind == kLearnedLMS)) {
         // min-sum phase, then plain BP over the trials it left unconverged
         launch_fill_i32(fb, 0, 2, st);
         const size_t ms_lds = static_cast<size_t>(qc.J) * qc.L * qc.P * sizeof(uint32_t);
@@ -844,8 +994,9 @@ void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_ta
             hipLaunchKernelGGL(kern, dim3(grid), dim3(768), ms_lds, st, g, dev, tp, count, nt, counter, li,
                                fb + 4, fb);
         };
-        if (learned->use_mlp) ms(decode_qc_ms_kernel<768, 3, 12, true>);
-        else ms(decode_qc_ms_kernel<768, 3, 12, false>);
+        if (learned->kind == kLearnedLMS) ms(decode_qc_ms_kernel<768, 3, 12, false, true>);
+        else if (learned->use_mlp) ms(decode_qc_ms_kernel<768, 3, 12, true, false>);
+        else ms(decode_qc_ms_kernel<768, 3, 12, false, false>);
         second(learned->T);
     } else if (learned && learned->T > 0) {
         if (learned->use_mlp)

@@ -19,13 +19,17 @@ constexpr int kMlpParams = 4 * kMlpHidden + 1;  // 65
 constexpr int kMaxLearnedT = 64;
 constexpr int kMaxLearnedBlocks = 64;
 
-// Model kinds: sum-product (the reference's joint BP with weights) and
-// normalised min-sum (max-product updates, the fast learned decoder).
+// Model kinds: sum-product (the reference's joint BP with weights),
+// normalised min-sum (max-product updates) and layered min-sum (the same
+// updates on a layered schedule: block row r of both sides is layer r, the
+// layers of an iteration updated in turn; no damping).
 constexpr int kLearnedSP = 0;
 constexpr int kLearnedMS = 1;
+constexpr int kLearnedLMS = 2;
 
 struct LearnedDevParams {
     int nb = 0, T = 0, use_mlp = 1, kind = kLearnedSP;
+    int lpb = 1;                // weight blocks per layer (the layer of block b is b / lpb)
     const int *blk0 = nullptr;  // block of each X edge (device)
     const int *blk1 = nullptr;  // block of each Z edge (device)
 };
@@ -57,7 +61,9 @@ struct LearnedInfer {
     // min-sum kind: per variable pass k = 0..T one record of kMsRec words,
     // for each column block cc: (wc X, wc Z) of iteration k-1 (1 at k = 0)
     // for r = 0..J-1, then (wv X, wv Z) of iteration k (1 at k = T), as FP16
-    // pairs, two words of padding (device, (T+1) x kMsRec uint32)
+    // pairs, two words of padding (device, (T+1) x kMsRec uint32).
+    // Layered min-sum kind: record k = 0..T-1 holds iteration k+1's own
+    // (wc X, wc Z) and (wv X, wv Z) per (r, cc) in the same places.
     const uint32_t *ms_rec = nullptr;
 };
 // Words of one min-sum weight record: 12 column blocks x (3 + 3 + 2).

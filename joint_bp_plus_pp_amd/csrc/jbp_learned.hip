@@ -75,6 +75,7 @@ struct Model {
     const R *wc, *wv, *mlp;  // [2][nb][T], [2][nb][T], [65]
     const int *blk0, *blk1;  // block of each edge, per side
     int nb, T, use_mlp;
+    int lpb;                 // layered kind: blocks per layer
 };
 
 // Channel constants of one trial: (a, b, c, d) of each side's joint term and
@@ -571,6 +572,309 @@ __global__ void __launch_bounds__(kBlock) learned_bwd_kernel(DevGraph g, Model<R
     }
 }
 
+// ---- layered min-sum (kind 2) ----
+// One workgroup per trial as above.  Layer of an edge: its weight block /
+// M.lpb (block row r of the permutation-block code).  Forward: tape row t
+// starts as a copy of row t-1 and each layer's update overwrites its edges
+// in place, so row t ends as iteration t's messages.  The estimate a layer
+// starts from is tested as the device decoder does: iteration t converges
+// when all its layers start from the same estimate and every check accepts
+// it.
+
+template <typename R>
+__device__ __forceinline__ int edge_layer(const Model<R> &M, int s, int e) {
+    return (s == 0 ? M.blk0[e] : M.blk1[e]) / M.lpb;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) learned_fwd_lms_kernel(DevGraph g, Model<R> M, TrialPlanes tp,
+                                                                Tape<R> tb, int batch, int *counter) {
+    __shared__ int s_trial;
+    __shared__ R s_loss;
+    __shared__ int s_unstable, s_bad;
+    const int T = M.T, n = g.n;
+    const int Ex = g.side[0].E;
+    const size_t Et = static_cast<size_t>(Ex) + g.side[1].E;
+    const int mx = g.side[0].m, mz = g.side[1].m;
+    const int nlay = M.nb / M.lpb;
+    const R scale = R(1) / (R(batch) * R(T) * R(2 * n));
+    Slot<R> sl = slot_of(g, tb);
+    for (;;) {
+        if (threadIdx.x == 0) s_trial = atomicAdd(counter, 1);
+        __syncthreads();
+        const int b = s_trial;
+        __syncthreads();
+        if (b >= batch) return;
+        Chan<R> ch;
+        priors(tb.prior + 4 * static_cast<size_t>(b), ch);
+        R *lam = tb.act + static_cast<size_t>(b) * (T + 1) * Et;
+        for (size_t e = threadIdx.x; e < Et; e += kBlock) lam[e] = R(0);
+        if (threadIdx.x == 0) s_loss = R(0);
+        __syncthreads();
+        int conv = 0;
+        R loss_local = R(0);
+        for (int t = 1; t <= T; ++t) {
+            R *cur = lam + static_cast<size_t>(t) * Et;
+            const R *prev = cur - Et;
+            for (size_t e = threadIdx.x; e < Et; e += kBlock) cur[e] = prev[e];
+            if (threadIdx.x == 0) {
+                s_unstable = 0;
+                s_bad = 0;
+            }
+            __syncthreads();
+            for (int r = 0; r < nlay; ++r) {
+                // sums and the estimate this layer starts from
+                int unstable = 0;
+                for (int v = threadIdx.x; v < n; v += kBlock) {
+                    R sx = R(0), sz = R(0);
+                    for (int k = g.side[0].var_off[v]; k < g.side[0].var_off[v + 1]; ++k) sx += cur[g.side[0].var_edge[k]];
+                    for (int k = g.side[1].var_off[v]; k < g.side[1].var_off[v + 1]; ++k) sz += cur[Ex + g.side[1].var_edge[k]];
+                    sl.S[v] = sx;
+                    sl.S[n + v] = sz;
+                    const R lx = sx + fterm_k<1>(sz, ch.fx, ch.lfx), lz = sz + fterm_k<1>(sx, ch.fz, ch.lfz);
+                    const int est = (lx < R(0) ? 1 : 0) | (lz < R(0) ? 2 : 0);
+                    if (r == 0) sl.est[v] = est;
+                    else unstable |= est != sl.est[v];
+                }
+                if (unstable) atomicOr(&s_unstable, 1);
+                __syncthreads();
+                // this layer's checks against the starting estimate (when the
+                // layers' estimates differ the iteration cannot converge)
+                int bad = 0;
+                for (int c = threadIdx.x; c < mx + mz; c += kBlock) {
+                    const int s = c < mx ? 0 : 1;
+                    const int cc = c < mx ? c : c - mx;
+                    const int e0 = g.side[s].chk_off[cc], e1 = g.side[s].chk_off[cc + 1];
+                    if (e1 == e0 || edge_layer(M, s, e0) != r) continue;
+                    int par = 0;
+                    for (int e = e0; e < e1; ++e) {
+                        const int st = sl.est[g.side[s].edge_var[e]];
+                        par ^= s == 0 ? (st & 1) : (st >> 1);
+                    }
+                    bad |= par != syn_bit(g, tp, b, s, cc);
+                }
+                if (bad) atomicOr(&s_bad, 1);
+                // mu of this layer's edges
+                for (int v = threadIdx.x; v < n; v += kBlock) {
+                    const R sx = sl.S[v], sz = sl.S[n + v];
+                    const R f0 = fterm_k<1>(sz, ch.fx, ch.lfx), f1 = fterm_k<1>(sx, ch.fz, ch.lfz);
+                    for (int k = g.side[0].var_off[v]; k < g.side[0].var_off[v + 1]; ++k) {
+                        const int e = g.side[0].var_edge[k];
+                        if (edge_layer(M, 0, e) != r) continue;
+                        const R a = sx - cur[e] + f0;
+                        sl.a[e] = a;
+                        sl.mu[e] = M.wv[M.blk0[e] * T + (t - 1)] * a;
+                    }
+                    for (int k = g.side[1].var_off[v]; k < g.side[1].var_off[v + 1]; ++k) {
+                        const int e = g.side[1].var_edge[k];
+                        if (edge_layer(M, 1, e) != r) continue;
+                        const R a = sz - cur[Ex + e] + f1;
+                        sl.a[Ex + e] = a;
+                        sl.mu[Ex + e] = M.wv[(M.nb + M.blk1[e]) * T + (t - 1)] * a;
+                    }
+                }
+                __syncthreads();
+                // this layer's check update, weighted into the state
+                for (int c = threadIdx.x; c < mx + mz; c += kBlock) {
+                    const int s = c < mx ? 0 : 1;
+                    const int cc = c < mx ? c : c - mx;
+                    const int off = g.side[s].chk_off[cc], deg = g.side[s].chk_off[cc + 1] - off;
+                    if (deg == 0 || edge_layer(M, s, off) != r) continue;
+                    const int base = s == 0 ? 0 : Ex;
+                    check_node_ms(sl.mu + base + off, deg, syn_bit(g, tp, b, s, cc), sl.raw + base + off,
+                                  sl.out + base + off);
+                    const int *blk = s == 0 ? M.blk0 : M.blk1;
+                    for (int i = 0; i < deg; ++i)
+                        cur[base + off + i] = M.wc[(s * M.nb + blk[off + i]) * T + (t - 1)] * sl.out[base + off + i];
+                }
+                __syncthreads();
+            }
+            if (s_unstable == 0 && s_bad == 0 && conv == 0) conv = t;
+            // loss on the marginals at the end of the iteration
+            for (int v = threadIdx.x; v < n; v += kBlock) {
+                R sx = R(0), sz = R(0);
+                for (int k = g.side[0].var_off[v]; k < g.side[0].var_off[v + 1]; ++k) sx += cur[g.side[0].var_edge[k]];
+                for (int k = g.side[1].var_off[v]; k < g.side[1].var_off[v + 1]; ++k) sz += cur[Ex + g.side[1].var_edge[k]];
+                const R lx = sx + fterm_k<1>(sz, ch.fx, ch.lfx);
+                const R lz = sz + fterm_k<1>(sx, ch.fz, ch.lfz);
+                const int st = err_state(g, tp, b, v);
+                loss_local += softplus(R(2 * (st & 1) - 1) * lx) + softplus(R(2 * (st >> 1) - 1) * lz);
+            }
+            __syncthreads();
+        }
+        block_add(&s_loss, loss_local);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            atomicAdd(tb.loss, s_loss * scale);
+            tb.conv[b] = conv;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kBlock) learned_bwd_lms_kernel(DevGraph g, Model<R> M, TrialPlanes tp,
+                                                                Tape<R> tb, int batch, int *counter) {
+    __shared__ int s_trial;
+    __shared__ R s_gw[4 * kMaxLearnedBlocks];  // wc side0 | wc side1 | wv side0 | wv side1
+    const int T = M.T, n = g.n, nb = M.nb;
+    const int Ex = g.side[0].E;
+    const size_t Et = static_cast<size_t>(Ex) + g.side[1].E;
+    const int mx = g.side[0].m, mz = g.side[1].m;
+    const int nlay = nb / M.lpb;
+    const R scale = R(1) / (R(batch) * R(T) * R(2 * n));
+    const R lmax = R(64.0 * 0.69314718055994530942);
+    Slot<R> sl = slot_of(g, tb);
+    for (;;) {
+        if (threadIdx.x == 0) s_trial = atomicAdd(counter, 1);
+        __syncthreads();
+        const int b = s_trial;
+        __syncthreads();
+        if (b >= batch) return;
+        Chan<R> ch;
+        priors(tb.prior + 4 * static_cast<size_t>(b), ch);
+        const R *lam = tb.act + static_cast<size_t>(b) * (T + 1) * Et;
+        R *gc = sl.gc;  // dL/d(messages) of the state after the step being undone
+        for (size_t e = threadIdx.x; e < Et; e += kBlock) gc[e] = R(0);
+        for (int i = threadIdx.x; i < 4 * nb; i += kBlock) s_gw[i] = R(0);
+        __syncthreads();
+        for (int t = T; t >= 1; --t) {
+            const R *prev = lam + static_cast<size_t>(t - 1) * Et;
+            const R *cur = lam + static_cast<size_t>(t) * Et;
+            // loss term at the end of iteration t
+            for (int v = threadIdx.x; v < n; v += kBlock) {
+                R sx = R(0), sz = R(0);
+                for (int k = g.side[0].var_off[v]; k < g.side[0].var_off[v + 1]; ++k) sx += cur[g.side[0].var_edge[k]];
+                for (int k = g.side[1].var_off[v]; k < g.side[1].var_off[v + 1]; ++k) sz += cur[Ex + g.side[1].var_edge[k]];
+                const R lx = sx + fterm_k<1>(sz, ch.fx, ch.lfx);
+                const R lz = sz + fterm_k<1>(sx, ch.fz, ch.lfz);
+                const int st = err_state(g, tp, b, v);
+                const R kx = R(2 * (st & 1) - 1), kz = R(2 * (st >> 1) - 1);
+                const R glx = scale * kx * sigm(kx * lx), glz = scale * kz * sigm(kz * lz);
+                const R gsx = glx + glz * fterm_kd<1>(sx, ch.fz, ch.lfz);
+                const R gsz = glz + glx * fterm_kd<1>(sz, ch.fx, ch.lfx);
+                for (int k = g.side[0].var_off[v]; k < g.side[0].var_off[v + 1]; ++k) gc[g.side[0].var_edge[k]] += gsx;
+                for (int k = g.side[1].var_off[v]; k < g.side[1].var_off[v + 1]; ++k) gc[Ex + g.side[1].var_edge[k]] += gsz;
+            }
+            __syncthreads();
+            for (int r = nlay - 1; r >= 0; --r) {
+                // the state before step (t, r): iteration t's messages on the
+                // layers before r, iteration t-1's on the rest; its sums, and
+                // a and mu of layer r's edges
+                for (int v = threadIdx.x; v < n; v += kBlock) {
+                    R sx = R(0), sz = R(0);
+                    for (int k = g.side[0].var_off[v]; k < g.side[0].var_off[v + 1]; ++k) {
+                        const int e = g.side[0].var_edge[k];
+                        sx += edge_layer(M, 0, e) < r ? cur[e] : prev[e];
+                    }
+                    for (int k = g.side[1].var_off[v]; k < g.side[1].var_off[v + 1]; ++k) {
+                        const int e = g.side[1].var_edge[k];
+                        sz += edge_layer(M, 1, e) < r ? cur[Ex + e] : prev[Ex + e];
+                    }
+                    sl.S[v] = sx;
+                    sl.S[n + v] = sz;
+                    const R f0 = fterm_k<1>(sz, ch.fx, ch.lfx), f1 = fterm_k<1>(sx, ch.fz, ch.lfz);
+                    for (int k = g.side[0].var_off[v]; k < g.side[0].var_off[v + 1]; ++k) {
+                        const int e = g.side[0].var_edge[k];
+                        if (edge_layer(M, 0, e) != r) continue;
+                        const R a = sx - prev[e] + f0;
+                        sl.a[e] = a;
+                        sl.mu[e] = M.wv[M.blk0[e] * T + (t - 1)] * a;
+                    }
+                    for (int k = g.side[1].var_off[v]; k < g.side[1].var_off[v + 1]; ++k) {
+                        const int e = g.side[1].var_edge[k];
+                        if (edge_layer(M, 1, e) != r) continue;
+                        const R a = sz - prev[Ex + e] + f1;
+                        sl.a[Ex + e] = a;
+                        sl.mu[Ex + e] = M.wv[(nb + M.blk1[e]) * T + (t - 1)] * a;
+                    }
+                }
+                __syncthreads();
+                // the layer's check update again, its weights' gradient, and
+                // the check adjoint (gradient to the input at each edge's
+                // minimum over the other edges) into sl.out
+                for (int c = threadIdx.x; c < mx + mz; c += kBlock) {
+                    const int s = c < mx ? 0 : 1;
+                    const int cc = c < mx ? c : c - mx;
+                    const int off = g.side[s].chk_off[cc], deg = g.side[s].chk_off[cc + 1] - off;
+                    if (deg == 0 || edge_layer(M, s, off) != r) continue;
+                    const int base = (s == 0 ? 0 : Ex) + off;
+                    const int *blk = s == 0 ? M.blk0 : M.blk1;
+                    check_node_ms(sl.mu + base, deg, syn_bit(g, tp, b, s, cc), sl.raw + base, sl.out + base);
+                    for (int i = 0; i < deg; ++i) {
+                        const int wi = s * nb + blk[off + i];
+                        const R ge = gc[base + i];
+                        atomicAdd(&s_gw[wi], ge * sl.out[base + i]);
+                        gc[base + i] = fabs(sl.raw[base + i]) > lmax ? R(0) : M.wc[wi * T + t - 1] * ge;
+                    }
+                    int i1 = 0;
+                    for (int i = 1; i < deg; ++i)
+                        if (fabs(sl.mu[base + i]) < fabs(sl.mu[base + i1])) i1 = i;
+                    int i2 = i1 == 0 ? 1 : 0;
+                    for (int i = 0; i < deg; ++i)
+                        if (i != i1 && fabs(sl.mu[base + i]) < fabs(sl.mu[base + i2])) i2 = i;
+                    R to1 = R(0), to2 = R(0);
+                    for (int i = 0; i < deg; ++i) {
+                        const R ge = gc[base + i];
+                        const R se = sl.raw[base + i] < R(0) ? -ge : ge;
+                        if (i == i1) to2 += se;
+                        else to1 += se;
+                    }
+                    for (int i = 0; i < deg; ++i) sl.out[base + i] = R(0);
+                    sl.out[base + i1] = sl.mu[base + i1] < R(0) ? -to1 : to1;
+                    if (deg > 1) sl.out[base + i2] = sl.mu[base + i2] < R(0) ? -to2 : to2;
+                }
+                __syncthreads();
+                // variable side: gradient of the state before the step
+                for (int v = threadIdx.x; v < n; v += kBlock) {
+                    R A[2];
+                    for (int s = 0; s < 2; ++s) {
+                        const int base = s == 0 ? 0 : Ex;
+                        const int *blk = s == 0 ? M.blk0 : M.blk1;
+                        R acc = R(0);
+                        for (int k = g.side[s].var_off[v]; k < g.side[s].var_off[v + 1]; ++k) {
+                            const int e = g.side[s].var_edge[k];
+                            if (edge_layer(M, s, e) != r) continue;
+                            const int wi = s * nb + blk[e];
+                            const R gmu = sl.out[base + e];
+                            atomicAdd(&s_gw[2 * nb + wi], gmu * sl.a[base + e]);
+                            acc += M.wv[wi * T + t - 1] * gmu;
+                        }
+                        A[s] = acc;
+                    }
+                    const R sx = sl.S[v], sz = sl.S[n + v];
+                    // d S_x: a_x directly, a_z through F_z(S_x); d S_z likewise
+                    const R dsx = A[0] + A[1] * fterm_kd<1>(sx, ch.fz, ch.lfz);
+                    const R dsz = A[1] + A[0] * fterm_kd<1>(sz, ch.fx, ch.lfx);
+                    for (int s = 0; s < 2; ++s) {
+                        const int base = s == 0 ? 0 : Ex;
+                        const int *blk = s == 0 ? M.blk0 : M.blk1;
+                        const R ds = s == 0 ? dsx : dsz;
+                        for (int k = g.side[s].var_off[v]; k < g.side[s].var_off[v + 1]; ++k) {
+                            const int e = g.side[s].var_edge[k];
+                            if (edge_layer(M, s, e) == r) {
+                                const R ga = M.wv[(s * nb + blk[e]) * T + t - 1] * sl.out[base + e];
+                                gc[base + e] = ds - ga;
+                            } else {
+                                gc[base + e] += ds;
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            for (int i = threadIdx.x; i < 4 * nb; i += kBlock) {
+                const int which = i / (2 * nb);  // 0 wc, 1 wv
+                const int wi = i % (2 * nb);
+                R *dst = which == 0 ? tb.gwc : tb.gwv;
+                if (s_gw[i] != R(0)) atomicAdd(&dst[wi * T + t - 1], s_gw[i]);
+                s_gw[i] = R(0);
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // Adam on the double master copy; refreshes the float copy the FP32
 // kernels read.
 __global__ void adam_kernel(double *p, const double *grad, double *m, double *v, float *pf,
@@ -603,6 +907,7 @@ Model<R> model_of(const LearnedDevParams &lp, const R *params) {
     M.mlp = params + 4 * lp.nb * lp.T;
     M.blk0 = lp.blk0;
     M.blk1 = lp.blk1;
+    M.lpb = lp.lpb > 0 ? lp.lpb : 1;
     return M;
 }
 
@@ -634,6 +939,14 @@ void launch_learned_train(const DevGraph &g, const LearnedDevParams &lp, const R
     tb.conv = conv;
     const int grid = slots < batch ? slots : batch;
     launch_fill_i32(counters, 0, 2, st);
+    if (lp.kind == kLearnedLMS) {
+        hipLaunchKernelGGL((learned_fwd_lms_kernel<R>), dim3(grid), dim3(kBlock), 0, st, g, M, tp, tb,
+                           batch, counters);
+        if (backward)
+            hipLaunchKernelGGL((learned_bwd_lms_kernel<R>), dim3(grid), dim3(kBlock), 0, st, g, M, tp,
+                               tb, batch, counters + 1);
+        return;
+    }
     if (ms)
         hipLaunchKernelGGL((learned_fwd_kernel<R, 1>), dim3(grid), dim3(kBlock), 0, st, g, M, tp, tb,
                            batch, counters);

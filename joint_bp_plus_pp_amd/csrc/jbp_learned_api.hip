@@ -105,6 +105,7 @@ LearnedDevParams dev_params(const jbp_learned *m) {
     lp.T = m->T;
     lp.use_mlp = m->use_mlp;
     lp.kind = m->kind;
+    lp.lpb = m->ctx->qcp.L;
     lp.blk0 = m->d_blk[0];
     lp.blk1 = m->d_blk[1];
     return lp;
@@ -140,7 +141,8 @@ int jbp_learned_create(jbp_ctx *ctx, int T, jbp_learned **out, char *err, size_t
 
 int jbp_learned_create_kind(jbp_ctx *ctx, int T, int kind, jbp_learned **out, char *err,
                             size_t errlen) {
-    if (kind != JBP_LEARNED_SUM_PRODUCT && kind != JBP_LEARNED_MIN_SUM) {
+    if (kind != JBP_LEARNED_SUM_PRODUCT && kind != JBP_LEARNED_MIN_SUM &&
+        kind != JBP_LEARNED_LAYERED_MIN_SUM) {
         set_err(err, errlen, "unknown learned-decoder kind %d", kind);
         return JBP_ERR_ARG;
     }
@@ -162,7 +164,9 @@ int jbp_learned_create_kind(jbp_ctx *ctx, int T, int kind, jbp_learned **out, ch
     HIPCHK(hipSetDevice(ctx->device));
     auto *m = new jbp_learned();
     m->ctx = ctx;
-    m->kind = kind == JBP_LEARNED_MIN_SUM ? kLearnedMS : kLearnedSP;
+    m->kind = kind == JBP_LEARNED_MIN_SUM ? kLearnedMS
+              : kind == JBP_LEARNED_LAYERED_MIN_SUM ? kLearnedLMS : kLearnedSP;
+    if (m->kind == kLearnedLMS) m->use_mlp = 0;
     m->T = T;
     m->nb = nb;
     m->count = learned_param_count(nb, T);
@@ -228,7 +232,10 @@ void jbp_learned_destroy(jbp_learned *m) {
 int jbp_learned_param_count(const jbp_learned *m) { return m ? m->count : -1; }
 
 int jbp_learned_kind(const jbp_learned *m) {
-    return !m ? -1 : m->kind == kLearnedMS ? JBP_LEARNED_MIN_SUM : JBP_LEARNED_SUM_PRODUCT;
+    return !m ? -1
+           : m->kind == kLearnedMS  ? JBP_LEARNED_MIN_SUM
+           : m->kind == kLearnedLMS ? JBP_LEARNED_LAYERED_MIN_SUM
+                                    : JBP_LEARNED_SUM_PRODUCT;
 }
 
 int jbp_learned_set_params(jbp_learned *m, const double *params, int use_mlp, char *err,
@@ -236,6 +243,10 @@ int jbp_learned_set_params(jbp_learned *m, const double *params, int use_mlp, ch
     if (!m || !params) {
         set_err(err, errlen, "bad arguments");
         return JBP_ERR_ARG;
+    }
+    if (use_mlp && m->kind == kLearnedLMS) {
+        set_err(err, errlen, "the layered min-sum model has no damping (use_mlp must be 0)");
+        return JBP_ERR_UNSUPPORTED;
     }
     HIPCHK(hipSetDevice(m->ctx->device));
     m->params.assign(params, params + m->count);
@@ -377,9 +388,10 @@ int jbp_simulate_learned(jbp_ctx *ctx, const jbp_learned *m, const jbp_sim_cfg *
     li.T = m->T;
     li.use_mlp = m->use_mlp;
     li.kind = m->kind;
-    if (m->kind == kLearnedMS) {
-        // variable pass k: (wc X, wc Z) of iteration k-1 per row block, then
-        // (wv X, wv Z) of iteration k, FP16 pairs (1 where there is none)
+    if (m->kind == kLearnedMS || m->kind == kLearnedLMS) {
+        // min-sum: variable pass k: (wc X, wc Z) of iteration k-1 per row
+        // block, then (wv X, wv Z) of iteration k, FP16 pairs (1 where there
+        // is none).  Layered: record k = iteration k+1's own wc and wv.
         if (ctx->qcp.J != 3 || ctx->qcp.L != 12) {
             set_err(err, errlen, "the min-sum decoder is built for 3 x 12 blocks");
             return JBP_ERR_UNSUPPORTED;
@@ -391,17 +403,22 @@ int jbp_simulate_learned(jbp_ctx *ctx, const jbp_learned *m, const jbp_sim_cfg *
         };
         auto pair = [&](double x, double z) { return h16(x) | (h16(z) << 16); };
         std::vector<uint32_t> rec(static_cast<size_t>(T + 1) * kMsRec, 0u);
+        const bool layered = m->kind == kLearnedLMS;
+        auto par = [&](int part, int b, int t) {
+            return m->params[(static_cast<size_t>(part) * nb + b) * T + t];
+        };
         for (int k = 0; k <= T; ++k)
             for (int cc = 0; cc < L; ++cc)
                 for (int r = 0; r < J; ++r) {
                     const int b = r * L + cc;
                     uint32_t *w = &rec[static_cast<size_t>(k) * kMsRec + cc * kMsRecCC];
-                    w[r] = k == 0 ? pair(1.0, 1.0)
-                                  : pair(m->params[(static_cast<size_t>(0) * nb + b) * T + k - 1],
-                                         m->params[(static_cast<size_t>(1) * nb + b) * T + k - 1]);
-                    w[3 + r] = k == T ? pair(1.0, 1.0)
-                                      : pair(m->params[(static_cast<size_t>(2) * nb + b) * T + k],
-                                             m->params[(static_cast<size_t>(3) * nb + b) * T + k]);
+                    if (layered) {
+                        w[r] = k == T ? pair(1.0, 1.0) : pair(par(0, b, k), par(1, b, k));
+                        w[3 + r] = k == T ? pair(1.0, 1.0) : pair(par(2, b, k), par(3, b, k));
+                        continue;
+                    }
+                    w[r] = k == 0 ? pair(1.0, 1.0) : pair(par(0, b, k - 1), par(1, b, k - 1));
+                    w[3 + r] = k == T ? pair(1.0, 1.0) : pair(par(2, b, k), par(3, b, k));
                 }
         HIPCHK(hipMemcpy(m->d_ms_rec, rec.data(), rec.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         li.ms_rec = m->d_ms_rec;

@@ -27,7 +27,7 @@ def train(decoder, T=25, steps=200, batch=4096, lr=1e-2, p_lo=0.03, p_hi=0.05, s
     params = model.get_params()
     if init is not None:
         params = np.asarray(init, dtype=np.float64).copy()
-    model.set_params(params, use_mlp=use_mlp)
+    model.set_params(params, use_mlp=use_mlp and kind != 2)  # the layered kind has no damping
     history = []
     t0 = time.time()
     nxt = first_trial
@@ -98,25 +98,46 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--no-mlp", action="store_true", help="message weights only, no damping MLP")
-    ap.add_argument("--kind", choices=("sum-product", "min-sum"), default="min-sum",
-                    help="check / variable updates of the learned decoder")
+    ap.add_argument("--kind", choices=("sum-product", "min-sum", "layered-min-sum"),
+                    default="layered-min-sum",
+                    help="updates and schedule of the learned decoder (layered: no damping)")
     ap.add_argument("--init-weight", type=float, default=1.0,
                     help="starting value of every message weight")
+    ap.add_argument("--init-var-weight", type=float, default=None,
+                    help="starting value of the variable-message weights (default: --init-weight)")
+    ap.add_argument("--first-trial", type=int, default=1,
+                    help="trial index of the first training sample")
+    ap.add_argument("--init-schedule", default="",
+                    help="comma-separated starting check-message weights of the first "
+                         "iterations (the rest start at --init-weight)")
     a = ap.parse_args(argv)
     dec = Decoder(Code.from_params(a.params), a.device)
-    kind = 1 if a.kind == "min-sum" else 0
+    kind = {"sum-product": 0, "min-sum": 1, "layered-min-sum": 2}[a.kind]
     probe = LearnedModel(dec, a.T, kind=kind)
     init = probe.get_params()
     probe.destroy()
-    init[:4 * probe.nb * a.T] = a.init_weight
+    nw = 2 * probe.nb * a.T
+    init[:nw] = a.init_weight
+    init[nw:2 * nw] = a.init_weight if a.init_var_weight is None else a.init_var_weight
+    sched = [float(x) for x in a.init_schedule.split(",") if x.strip()]
+    if len(sched) > a.T:
+        ap.error("--init-schedule has more entries than --T")
+    wc = init[:2 * probe.nb * a.T].reshape(2, probe.nb, a.T)  # check weights, a view
+    for k, w in enumerate(sched):
+        wc[:, :, k] = w
     model, hist = train(dec, T=a.T, steps=a.steps, batch=a.batch, lr=a.lr, p_lo=a.p_lo,
-                        p_hi=a.p_hi, seed=a.seed, lr_decay=a.lr_decay,
-                        log=lambda m: print(m, flush=True), use_mlp=not a.no_mlp, kind=kind,
+                        p_hi=a.p_hi, seed=a.seed, first_trial=a.first_trial, lr_decay=a.lr_decay,
+                        log=lambda m: print(m, flush=True), use_mlp=not a.no_mlp and kind != 2,
+                        kind=kind,
                         init=init)
+    loss = hist[-1][1] if hist else float("nan")
     save(model, a.out, meta={"steps": a.steps, "batch": a.batch, "lr": a.lr, "p_lo": a.p_lo,
-                             "p_hi": a.p_hi, "seed": a.seed, "loss": hist[-1][1],
-                             "init_weight": a.init_weight})
-    print("wrote %s (final loss %.6f, %.1f s)" % (a.out, hist[-1][1], hist[-1][2]), flush=True)
+                             "p_hi": a.p_hi, "seed": a.seed, "loss": loss,
+                             "init_weight": a.init_weight, "init_schedule": a.init_schedule,
+                             "init_var_weight": (a.init_weight if a.init_var_weight is None
+                                                 else a.init_var_weight),
+                             "first_trial": a.first_trial})
+    print("wrote %s (%d steps, final loss %.6f)" % (a.out, a.steps, loss), flush=True)
     model.destroy()
     dec.destroy()
     return 0

@@ -35,6 +35,23 @@
 // discrete choice the forward pass made (arg-minima, signs, clamps, max
 // branches, estimates), so a finite-difference test can confirm that both
 // of its points lie on the same smooth piece.
+//
+// Third kind (kind = 2, layered min-sum): the min-sum model above on a
+// layered schedule.  The checks of each side are split into layers (the
+// permutation-block code: layer r = block row r, both sides' layer r
+// together); an iteration updates the layers in order, each from the
+// messages as the earlier layers of the same iteration left them:
+//   S_g(v) = sum of v's current check messages on side g
+//   mu_e   = wv[g][blk][t] * (S_g(v) - lambda_e + F_g(S_other(v)))   (e in layer r)
+//   lambda_e <- wc[g][blk][t] * CN_e(mu)                            (checks of layer r)
+// The loss is taken on the marginals at the end of each iteration.  The
+// estimate tested for convergence at iteration t is the one each layer's
+// update starts from: the decode stops at the first t whose layers all start
+// from the same estimate and whose rows all accept it (the rule the device
+// decoder can test inside the iteration).  The adjoint walks the layer steps
+// backwards; the state before step (t, r) is iteration t's messages on
+// layers < r and iteration t-1's on the rest, so the per-iteration tape of
+// the other kinds serves it too.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -52,6 +69,8 @@ struct Graph {
     int m[2] = {0, 0};
     int nb = 0;  // weight blocks per side
     std::vector<int> chk_off[2], edge_var[2], var_off[2], var_edge[2], blk[2];
+    int nlay = 0;                 // layers (kind 2)
+    std::vector<int> edge_lay[2]; // layer of each edge's check (kind 2)
 };
 
 inline double sigmoid(double x) {
@@ -73,7 +92,7 @@ struct Params {
     const double *wv;   // [2][nb][T]
     const double *mlp;  // W1[16][2] | b1[16] | W2[16] | b2
     bool use_mlp;
-    int kind;           // 0 sum-product, 1 min-sum
+    int kind;           // 0 sum-product, 1 min-sum, 2 layered min-sum
 };
 
 // Hash of the discrete choices of a min-sum forward pass (FNV-1a).
@@ -98,10 +117,10 @@ inline double fterm_ms_d(double L, const double k[4]) {
            (std::log(k[2]) + L > std::log(k[3]) ? 1.0 : 0.0);
 }
 inline double fterm_k(int kind, double L, const double k[4], Sig *sig = nullptr) {
-    return kind == 1 ? fterm_ms(L, k, sig) : fterm(L, k);
+    return kind >= 1 ? fterm_ms(L, k, sig) : fterm(L, k);
 }
 inline double fterm_kd(int kind, double L, const double k[4]) {
-    return kind == 1 ? fterm_ms_d(L, k) : fterm_d(L, k);
+    return kind >= 1 ? fterm_ms_d(L, k) : fterm_d(L, k);
 }
 
 // Min-sum check update, written from its definition: for each edge the
@@ -450,6 +469,224 @@ void trial_backward(const Graph &g, const Params &P, int T, const int *err, cons
     }
 }
 
+// ---- kind 2: layered min-sum ----
+
+std::vector<int> trial_syndromes(const Graph &g, const int *err, int s) {
+    std::vector<int> syn(g.m[s], 0);
+    for (int c = 0; c < g.m[s]; ++c) {
+        int par = 0;
+        for (int e = g.chk_off[s][c]; e < g.chk_off[s][c + 1]; ++e) {
+            int st = err[g.edge_var[s][e]];
+            par ^= s == 0 ? (st & 1) : (st >> 1);
+        }
+        syn[c] = par;
+    }
+    return syn;
+}
+
+// Per-variable sums of a message state.
+void var_sums(const Graph &g, const std::vector<double> *lam, std::vector<double> *S) {
+    for (int s = 0; s < 2; ++s) {
+        S[s].assign(g.n, 0.0);
+        for (int v = 0; v < g.n; ++v)
+            for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) S[s][v] += lam[s][g.var_edge[s][k]];
+    }
+}
+
+// Layer step (t, r) from the state `cur`: a and mu of layer r's edges (other
+// edges untouched), raw and clamped check outputs of layer r's checks.
+void layer_step(const Graph &g, const Params &P, int T, int t, int r, const std::vector<int> *syn,
+                const double *fx, const double *fz, const std::vector<double> *cur,
+                std::vector<double> *S, std::vector<double> *a, std::vector<double> *mu,
+                std::vector<double> *raw, std::vector<double> *out, Sig *sig) {
+    var_sums(g, cur, S);
+    for (int s = 0; s < 2; ++s) {
+        for (int v = 0; v < g.n; ++v) {
+            bool any = false;
+            for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1] && !any; ++k)
+                any = g.edge_lay[s][g.var_edge[s][k]] == r;
+            if (!any) continue;
+            const double f = s == 0 ? fterm_ms(S[1][v], fx, sig) : fterm_ms(S[0][v], fz, sig);
+            for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) {
+                const int e = g.var_edge[s][k];
+                if (g.edge_lay[s][e] != r) continue;
+                a[s][e] = S[s][v] - cur[s][e] + f;
+                mu[s][e] = P.wv[(s * g.nb + g.blk[s][e]) * T + (t - 1)] * a[s][e];
+            }
+        }
+        for (int c = 0; c < g.m[s]; ++c) {
+            const int off = g.chk_off[s][c], deg = g.chk_off[s][c + 1] - off;
+            if (deg == 0 || g.edge_lay[s][off] != r) continue;
+            check_node_ms(&mu[s][off], deg, syn[s][c], &raw[s][off], &out[s][off], sig);
+        }
+    }
+}
+
+double trial_forward_layered(const Graph &g, const Params &P, int T, const int *err,
+                             const double *prior, TrialTape &tape, Sig *sig) {
+    const double fx[4] = {prior[0], prior[2], prior[1], prior[3]};
+    const double fz[4] = {prior[0], prior[1], prior[2], prior[3]};
+    std::vector<int> syn[2] = {trial_syndromes(g, err, 0), trial_syndromes(g, err, 1)};
+    for (int s = 0; s < 2; ++s) tape.lam[s].assign(T + 1, std::vector<double>(g.edge_var[s].size(), 0.0));
+    tape.d.assign(T, 0.0);
+    tape.x0.assign(T, 0.0);
+    tape.x1.assign(T, 0.0);
+    tape.converged_at = 0;
+    tape.marg.assign(2 * static_cast<size_t>(g.n), 0.0);
+    std::vector<double> cur[2], S[2], a[2], mu[2], raw[2], out[2];
+    for (int s = 0; s < 2; ++s) {
+        cur[s].assign(g.edge_var[s].size(), 0.0);
+        a[s].assign(g.edge_var[s].size(), 0.0);
+        mu[s].assign(g.edge_var[s].size(), 0.0);
+        raw[s].assign(g.edge_var[s].size(), 0.0);
+        out[s].assign(g.edge_var[s].size(), 0.0);
+    }
+    std::vector<int> est(g.n, 0), est0(g.n, 0);
+    double loss = 0.0;
+    for (int t = 1; t <= T; ++t) {
+        bool stable = true, rows_ok = true;
+        for (int r = 0; r < g.nlay; ++r) {
+            // the estimate this layer starts from, and its rows' parities
+            var_sums(g, cur, S);
+            for (int v = 0; v < g.n; ++v) {
+                const double lx = S[0][v] + fterm_ms(S[1][v], fx), lz = S[1][v] + fterm_ms(S[0][v], fz);
+                const int e = (lx < 0 ? 1 : 0) | (lz < 0 ? 2 : 0);
+                if (r == 0) est0[v] = e;
+                else if (e != est0[v]) stable = false;
+                est[v] = e;
+            }
+            for (int s = 0; s < 2; ++s)
+                for (int c = 0; c < g.m[s]; ++c) {
+                    const int off = g.chk_off[s][c];
+                    if (g.chk_off[s][c + 1] == off || g.edge_lay[s][off] != r) continue;
+                    int par = 0;
+                    for (int e = off; e < g.chk_off[s][c + 1]; ++e) {
+                        const int st = est[g.edge_var[s][e]];
+                        par ^= s == 0 ? (st & 1) : (st >> 1);
+                    }
+                    if (par != syn[s][c]) rows_ok = false;
+                }
+            layer_step(g, P, T, t, r, syn, fx, fz, cur, S, a, mu, raw, out, sig);
+            for (int s = 0; s < 2; ++s)
+                for (size_t e = 0; e < cur[s].size(); ++e)
+                    if (g.edge_lay[s][e] == r)
+                        cur[s][e] = P.wc[(s * g.nb + g.blk[s][e]) * T + (t - 1)] * out[s][e];
+        }
+        if (stable && rows_ok && tape.converged_at == 0) tape.converged_at = t;
+        for (int s = 0; s < 2; ++s) tape.lam[s][t] = cur[s];
+        var_sums(g, cur, S);
+        for (int v = 0; v < g.n; ++v) {
+            const double lx = S[0][v] + fterm_ms(S[1][v], fx, sig);
+            const double lz = S[1][v] + fterm_ms(S[0][v], fz, sig);
+            const int yx = err[v] & 1, yz = err[v] >> 1;
+            loss += softplus((2 * yx - 1) * lx) + softplus((2 * yz - 1) * lz);
+            tape.marg[v] = lx;
+            tape.marg[g.n + v] = lz;
+            est[v] = (lx < 0 ? 1 : 0) | (lz < 0 ? 2 : 0);
+            if (sig) sig->add(static_cast<uint64_t>(est[v]));
+        }
+    }
+    tape.est = est;
+    return loss;
+}
+
+void trial_backward_layered(const Graph &g, const Params &P, int T, const int *err,
+                            const double *prior, const TrialTape &tape, double scale, double *gwc,
+                            double *gwv) {
+    const double fx[4] = {prior[0], prior[2], prior[1], prior[3]};
+    const double fz[4] = {prior[0], prior[1], prior[2], prior[3]};
+    std::vector<int> syn[2] = {trial_syndromes(g, err, 0), trial_syndromes(g, err, 1)};
+    std::vector<double> G[2], Gn[2], pre[2], S[2], a[2], mu[2], raw[2], out[2], Gmu[2], Ga[2];
+    for (int s = 0; s < 2; ++s) {
+        const size_t E = g.edge_var[s].size();
+        G[s].assign(E, 0.0);
+        a[s].assign(E, 0.0);
+        mu[s].assign(E, 0.0);
+        raw[s].assign(E, 0.0);
+        out[s].assign(E, 0.0);
+    }
+    for (int t = T; t >= 1; --t) {
+        // loss term on the marginals at the end of iteration t
+        const std::vector<double> endv[2] = {tape.lam[0][t], tape.lam[1][t]};
+        var_sums(g, endv, S);
+        for (int v = 0; v < g.n; ++v) {
+            const double lx = S[0][v] + fterm_ms(S[1][v], fx), lz = S[1][v] + fterm_ms(S[0][v], fz);
+            const double kx = 2 * (err[v] & 1) - 1, kz = 2 * (err[v] >> 1) - 1;
+            const double glx = scale * kx * sigmoid(kx * lx), glz = scale * kz * sigmoid(kz * lz);
+            const double gsx = glx + glz * fterm_ms_d(S[0][v], fz);
+            const double gsz = glz + glx * fterm_ms_d(S[1][v], fx);
+            for (int k = g.var_off[0][v]; k < g.var_off[0][v + 1]; ++k) G[0][g.var_edge[0][k]] += gsx;
+            for (int k = g.var_off[1][v]; k < g.var_off[1][v + 1]; ++k) G[1][g.var_edge[1][k]] += gsz;
+        }
+        for (int r = g.nlay - 1; r >= 0; --r) {
+            // the state before step (t, r)
+            for (int s = 0; s < 2; ++s) {
+                pre[s].resize(g.edge_var[s].size());
+                for (size_t e = 0; e < pre[s].size(); ++e)
+                    pre[s][e] = g.edge_lay[s][e] < r ? tape.lam[s][t][e] : tape.lam[s][t - 1][e];
+            }
+            layer_step(g, P, T, t, r, syn, fx, fz, pre, S, a, mu, raw, out, nullptr);
+            // lambda_e = wc * clamp(raw_e) on layer r
+            for (int s = 0; s < 2; ++s) {
+                Gmu[s].assign(g.edge_var[s].size(), 0.0);
+                std::vector<double> Gout(g.edge_var[s].size(), 0.0);
+                for (size_t e = 0; e < G[s].size(); ++e) {
+                    if (g.edge_lay[s][e] != r) continue;
+                    const size_t wi = (s * g.nb + g.blk[s][e]) * T + (t - 1);
+                    gwc[wi] += G[s][e] * out[s][e];
+                    Gout[e] = std::fabs(raw[s][e]) > kLlrMax ? 0.0 : P.wc[wi] * G[s][e];
+                }
+                // min-sum check adjoint: each edge's gradient to the input
+                // attaining the minimum over the other edges
+                for (int c = 0; c < g.m[s]; ++c) {
+                    const int off = g.chk_off[s][c], deg = g.chk_off[s][c + 1] - off;
+                    if (deg == 0 || g.edge_lay[s][off] != r) continue;
+                    for (int e = 0; e < deg; ++e) {
+                        const double ge = Gout[off + e];
+                        if (ge == 0.0) continue;
+                        int js = -1;
+                        for (int j = 0; j < deg; ++j)
+                            if (j != e && (js < 0 || std::fabs(mu[s][off + j]) < std::fabs(mu[s][off + js]))) js = j;
+                        const double se = raw[s][off + e] < 0 ? -1.0 : 1.0;
+                        const double sj = mu[s][off + js] < 0 ? -1.0 : 1.0;
+                        Gmu[s][off + js] += ge * se * sj;
+                    }
+                }
+            }
+            // variable side: a_e = S_g(v) - lambda_e + F_g(S_o(v)), mu = wv a
+            std::vector<double> A[2];
+            for (int s = 0; s < 2; ++s) {
+                Ga[s].assign(g.edge_var[s].size(), 0.0);
+                A[s].assign(g.n, 0.0);
+                for (int v = 0; v < g.n; ++v)
+                    for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) {
+                        const int e = g.var_edge[s][k];
+                        if (g.edge_lay[s][e] != r) continue;
+                        const size_t wi = (s * g.nb + g.blk[s][e]) * T + (t - 1);
+                        gwv[wi] += Gmu[s][e] * a[s][e];
+                        Ga[s][e] = P.wv[wi] * Gmu[s][e];
+                        A[s][v] += Ga[s][e];
+                    }
+            }
+            for (int s = 0; s < 2; ++s) {
+                const int o = 1 - s;
+                const double *kf = s == 0 ? fz : fx;  // side o's term F_o(S_s)
+                Gn[s].assign(g.edge_var[s].size(), 0.0);
+                for (int v = 0; v < g.n; ++v) {
+                    const double dS = A[s][v] + A[o][v] * fterm_ms_d(S[s][v], kf);
+                    for (int k = g.var_off[s][v]; k < g.var_off[s][v + 1]; ++k) {
+                        const int e = g.var_edge[s][k];
+                        const bool in = g.edge_lay[s][e] == r;
+                        Gn[s][e] = (in ? 0.0 : G[s][e]) + dS - (in ? Ga[s][e] : 0.0);
+                    }
+                }
+            }
+            G[0].swap(Gn[0]);
+            G[1].swap(Gn[1]);
+        }
+    }
+}
+
 Graph make_graph(int n, int mx, int mz, const int *xoff, const int *xvar, const int *zoff,
                  const int *zvar, const int *xblk, const int *zblk, int nb) {
     Graph g;
@@ -490,18 +727,30 @@ int orl_mlp_params() { return kMlpParams; }
 // when grad_* are non-null, its gradient.  Per trial, conv_out (may be
 // null) receives the first iteration whose estimate satisfies both
 // syndromes (0 = none) and est_out (may be null) the final estimate.
-// kind 0: sum-product, 1: min-sum (header).  signature (may be null): hash
+// kind 0: sum-product, 1: min-sum, 2: layered min-sum (header; orl_loss3,
+// with each side's check layers xlay / zlay, nlay layers).  signature (may be null): hash
 // of the min-sum forward pass's discrete choices over the whole batch.
 // marg_out (may be null): per trial the last iteration's per-qubit X then Z
 // marginal log-likelihood ratios (batch x 2n).
-double orl_loss2(int kind, int n, int mx, int mz, const int *xoff, const int *xvar,
-                 const int *zoff, const int *zvar, const int *xblk, const int *zblk, int nb, int T,
-                 const double *wc, const double *wv, const double *mlp, int use_mlp, int batch,
-                 const int *err, const double *p, double *grad_wc, double *grad_wv,
-                 double *grad_mlp, int *conv_out, int *est_out, uint64_t *signature,
-                 double *marg_out) {
+double orl_loss3(int kind, int n, int mx, int mz, const int *xoff, const int *xvar,
+                 const int *zoff, const int *zvar, const int *xblk, const int *zblk, int nb,
+                 const int *xlay, const int *zlay, int nlay, int T, const double *wc,
+                 const double *wv, const double *mlp, int use_mlp, int batch, const int *err,
+                 const double *p, double *grad_wc, double *grad_wv, double *grad_mlp,
+                 int *conv_out, int *est_out, uint64_t *signature, double *marg_out) {
     Graph g = make_graph(n, mx, mz, xoff, xvar, zoff, zvar, xblk, zblk, nb);
-    Params P{wc, wv, mlp, use_mlp != 0, kind == 1 ? 1 : 0};
+    Params P{wc, wv, mlp, use_mlp != 0, kind == 1 ? 1 : (kind == 2 ? 2 : 0)};
+    if (P.kind == 2) {
+        // layers per check -> per edge; the layered kind has no damping
+        g.nlay = nlay;
+        const int *lay[2] = {xlay, zlay};
+        for (int s = 0; s < 2; ++s) {
+            g.edge_lay[s].assign(g.edge_var[s].size(), 0);
+            for (int c = 0; c < g.m[s]; ++c)
+                for (int e = g.chk_off[s][c]; e < g.chk_off[s][c + 1]; ++e) g.edge_lay[s][e] = lay[s][c];
+        }
+        P.use_mlp = false;
+    }
     Sig sig;
     const double scale = 1.0 / (static_cast<double>(batch) * T * 2.0 * n);
     if (grad_wc) std::memset(grad_wc, 0, sizeof(double) * 2 * nb * T);
@@ -513,18 +762,32 @@ double orl_loss2(int kind, int n, int mx, int mz, const int *xoff, const int *xv
         double prior[4];
         prior_of(p[b], prior);
         TrialTape tape;
-        total += trial_forward(g, P, T, err + static_cast<size_t>(b) * n, prior, tape,
-                               signature ? &sig : nullptr);
+        const int *eb = err + static_cast<size_t>(b) * n;
+        if (P.kind == 2) total += trial_forward_layered(g, P, T, eb, prior, tape, signature ? &sig : nullptr);
+        else total += trial_forward(g, P, T, eb, prior, tape, signature ? &sig : nullptr);
         if (conv_out) conv_out[b] = tape.converged_at;
         if (est_out) std::copy(tape.est.begin(), tape.est.end(), est_out + static_cast<size_t>(b) * n);
         if (marg_out) std::copy(tape.marg.begin(), tape.marg.end(), marg_out + static_cast<size_t>(b) * 2 * n);
-        if (grad_wc || grad_wv || grad_mlp)
-            trial_backward(g, P, T, err + static_cast<size_t>(b) * n, prior, tape, scale,
-                           grad_wc ? grad_wc : dummy_wc.data(), grad_wv ? grad_wv : dummy_wv.data(),
-                           grad_mlp ? grad_mlp : dummy_mlp.data());
+        if (grad_wc || grad_wv || grad_mlp) {
+            double *gc = grad_wc ? grad_wc : dummy_wc.data(), *gv = grad_wv ? grad_wv : dummy_wv.data();
+            if (P.kind == 2) trial_backward_layered(g, P, T, eb, prior, tape, scale, gc, gv);
+            else trial_backward(g, P, T, eb, prior, tape, scale, gc, gv, grad_mlp ? grad_mlp : dummy_mlp.data());
+        }
     }
     if (signature) *signature = sig.h;
     return total * scale;
+}
+
+double orl_loss2(int kind, int n, int mx, int mz, const int *xoff, const int *xvar,
+                 const int *zoff, const int *zvar, const int *xblk, const int *zblk, int nb, int T,
+                 const double *wc, const double *wv, const double *mlp, int use_mlp, int batch,
+                 const int *err, const double *p, double *grad_wc, double *grad_wv,
+                 double *grad_mlp, int *conv_out, int *est_out, uint64_t *signature,
+                 double *marg_out) {
+    if (kind == 2) return 0.0;  // the layered kind needs its layers: orl_loss3
+    return orl_loss3(kind, n, mx, mz, xoff, xvar, zoff, zvar, xblk, zblk, nb, nullptr, nullptr, 0, T,
+                     wc, wv, mlp, use_mlp, batch, err, p, grad_wc, grad_wv, grad_mlp, conv_out,
+                     est_out, signature, marg_out);
 }
 
 double orl_loss(int n, int mx, int mz, const int *xoff, const int *xvar, const int *zoff,

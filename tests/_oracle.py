@@ -139,10 +139,12 @@ class Oracle:
 
 class LearnedOracle:
     """FP64 learned-BP forward / loss / adjoint (oracle/learned_oracle.cpp).
-    kind 0: sum-product, 1: min-sum.  Weight blocks: edge at position i of
-    check r*P + x is block r*L + i, or explicit per-edge blocks (blk, nb)."""
+    kind 0: sum-product, 1: min-sum, 2: layered min-sum.  Weight blocks: edge
+    at position i of check r*P + x is block r*L + i, or explicit per-edge
+    blocks (blk, nb).  Layers (kind 2): check r*P + x is in layer r, or
+    explicit per-check layers (lay, nlay)."""
 
-    def __init__(self, code, P=768, L=12, path=ORACLE_SO, blk=None, nb=None):
+    def __init__(self, code, P=768, L=12, path=ORACLE_SO, blk=None, nb=None, lay=None, nlay=None):
         L_ = C.CDLL(path)
         vp = C.c_void_p
         L_.orl_loss.restype = C.c_double
@@ -152,10 +154,19 @@ class LearnedOracle:
         L_.orl_loss2.argtypes = [C.c_int] * 4 + [vp] * 6 + [C.c_int, C.c_int, vp, vp, vp, C.c_int,
                                                             C.c_int, vp, vp, vp, vp, vp, vp, vp,
                                                             vp, vp]
+        L_.orl_loss3.restype = C.c_double
+        L_.orl_loss3.argtypes = ([C.c_int] * 4 + [vp] * 6 + [C.c_int, vp, vp, C.c_int, C.c_int] +
+                                 [vp] * 3 + [C.c_int, C.c_int] + [vp] * 9)
         self.lib = L_
         self.code = code
         self.xoff, self.xvar = code.csr(0)
         self.zoff, self.zvar = code.csr(1)
+        if lay is not None:
+            self.lay = [np.ascontiguousarray(x, dtype=np.int32) for x in lay]
+            self.nlay = int(nlay)
+        else:
+            self.lay = [np.arange(m, dtype=np.int32) // P for m in (code.mx, code.mz)]
+            self.nlay = max(1, code.mx // P)
         if blk is not None:
             self.blk = [np.ascontiguousarray(b, dtype=np.int32) for b in blk]
             self.nb = int(nb)
@@ -183,11 +194,11 @@ class LearnedOracle:
         sig = np.zeros(1, np.uint64)
         marg = np.zeros((err.shape[0], 2 * err.shape[1]), np.float64)
         c = self.code
-        val = self.lib.orl_loss2(int(kind), c.n, c.mx, c.mz, _p(self.xoff), _p(self.xvar),
+        val = self.lib.orl_loss3(int(kind), c.n, c.mx, c.mz, _p(self.xoff), _p(self.xvar),
                                  _p(self.zoff), _p(self.zvar), _p(self.blk[0]), _p(self.blk[1]),
-                                 self.nb, T, _p(wc), _p(wv), _p(mlp), int(use_mlp), err.shape[0],
-                                 _p(err), _p(p), _p(g[0]), _p(g[1]), _p(g[2]), _p(conv), _p(est),
-                                 _p(sig), _p(marg))
+                                 self.nb, _p(self.lay[0]), _p(self.lay[1]), self.nlay, T, _p(wc),
+                                 _p(wv), _p(mlp), int(use_mlp), err.shape[0], _p(err), _p(p),
+                                 _p(g[0]), _p(g[1]), _p(g[2]), _p(conv), _p(est), _p(sig), _p(marg))
         out = (val, g, conv, est) if grad else (val, conv, est)
         if extra:
             out = out + ({"signature": int(sig[0]), "marginals": marg},)

@@ -172,3 +172,108 @@ def test_minsum_adjoint_matches_central_differences_on_smooth_pieces(oracle):
         assert rel < 1e-3, (name, ix, g[ix], fd)
     print("smooth pieces checked", found, "worst relative error %.2e" % worst)
     assert all(v >= 4 for v in found.values()), found
+
+
+# ---------------------------------------------------------------------------
+# Layered min-sum kind (kind = 2).
+
+def small_hgp_oracle(oracle, nlay=3):
+    import joint_bp_plus_pp_amd as jbp
+    c = jbp.Code.hgp(15, 20, 3, 4, seed=2)
+    oc = oracle.code_from_checks(c.n, c.checks(0), c.checks(1))
+    blk = []
+    for s in (0, 1):
+        off = oc.csr(s)[0]
+        b = np.zeros(off[-1], np.int32)
+        for k in range(len(off) - 1):
+            b[off[k]:off[k + 1]] = np.arange(off[k + 1] - off[k])
+        blk.append(b)
+    nb = int(max(b.max() for b in blk)) + 1
+    # layers: contiguous ranges of check indices
+    lay = [(np.arange(m) * nlay // m).astype(np.int32) for m in (oc.mx, oc.mz)]
+    return oc, LearnedOracle_(oc, blk=blk, nb=nb, lay=lay, nlay=nlay), nb
+
+
+def test_layered_with_one_layer_is_the_flooding_min_sum(oracle):
+    # one layer: every check updates from the same state, i.e. the flooding
+    # schedule of kind 1 (whose adjoint and tree exactness are pinned above):
+    # loss and gradient agree to rounding
+    oc, lo, nb = small_hgp_oracle(oracle, nlay=1)
+    T = 5
+    rng = np.random.default_rng(3)
+    wc = 0.8 + 0.1 * rng.standard_normal((2, nb, T))
+    wv = 1 + 0.1 * rng.standard_normal((2, nb, T))
+    errs = np.stack([oc.sample(7, t, 0.05) for t in (1, 2, 3, 4)])
+    p = np.array([0.05, 0.04, 0.06, 0.05])
+    l1, (gc1, gv1, _), _, _ = lo.loss(T, wc, wv, np.zeros(65), 0, errs, p, grad=True, kind=1)
+    l2, (gc2, gv2, _), _, _ = lo.loss(T, wc, wv, np.zeros(65), 0, errs, p, grad=True, kind=2)
+    assert abs(l1 - l2) <= 1e-12 * abs(l1)
+    assert np.allclose(gc1, gc2, rtol=1e-10, atol=1e-15) and np.allclose(gv1, gv2, rtol=1e-10, atol=1e-15)
+
+
+@pytest.mark.parametrize("nlay", [2, 3])
+def test_layered_unit_weights_are_exact_max_product_on_a_tree(oracle, nlay):
+    # on a cycle-free graph layered max-product also reaches the exact
+    # max-marginals
+    code, xs, zs = tree_code(oracle)
+    offs = [code.csr(s)[0] for s in (0, 1)]
+    blk = [np.zeros(off[-1], np.int32) for off in offs]
+    lay = [np.arange(2, dtype=np.int32) % nlay for _ in range(2)]
+    lo = LearnedOracle_(code, blk=blk, nb=1, lay=lay, nlay=nlay)
+    T = 8
+    ones = np.ones((2, 1, T))
+    rng = np.random.default_rng(6)
+    for trial in range(12):
+        err = rng.integers(0, 4, size=8).astype(np.int32)
+        err[rng.random(8) < 0.5] = 0
+        p = [0.05, 0.12, 0.2][trial % 3]
+        sx, sz = code.syndrome(err)
+        _, conv, est, ex = lo.loss(T, ones, ones, np.zeros(65), 0, err[None], p, kind=2, extra=True)
+        want = brute_force_max_marginals(8, xs, zs, sx, sz, oracle.prior(p))
+        assert np.allclose(ex["marginals"][0], want, rtol=0, atol=1e-9), (trial, err)
+
+
+def test_layered_adjoint_matches_central_differences_on_smooth_pieces(oracle):
+    # as for kind 1: central differences where the forward pass makes the
+    # same discrete choices at theta - eps, theta and theta + eps, for at
+    # least 5 check weights and 5 variable weights, over 3 layers (a variable
+    # weight moves every message it scales, so fewer of them sit on a smooth
+    # piece at this eps: more draws)
+    oc, lo, nb = small_hgp_oracle(oracle, nlay=3)
+    T = 5
+    rng = np.random.default_rng(11)
+    wc = 0.8 + 0.1 * rng.standard_normal((2, nb, T))
+    wv = 1 + 0.1 * rng.standard_normal((2, nb, T))
+    mlp = np.zeros(65)
+    errs = np.stack([oc.sample(7, t, 0.05) for t in (1, 2, 3)])
+    p = np.array([0.05, 0.04, 0.06])
+    _, (gwc, gwv, _), _, _, ex = lo.loss(T, wc, wv, mlp, 0, errs, p, grad=True, kind=2, extra=True)
+    eps = 1e-3
+    found = {"wc": 0, "wv": 0}
+    worst = 0.0
+    for attempt in range(2000):
+        name = ("wc", "wv")[attempt % 2]
+        if found[name] >= 5:
+            continue
+        arr = {"wc": wc, "wv": wv}[name]
+        g = {"wc": gwc, "wv": gwv}[name]
+        ix = tuple(int(rng.integers(0, s)) for s in arr.shape)
+        old = arr[ix]
+        arr[ix] = old + eps
+        lp, _, _, ep = lo.loss(T, wc, wv, mlp, 0, errs, p, kind=2, extra=True)
+        arr[ix] = old - eps
+        lm, _, _, em = lo.loss(T, wc, wv, mlp, 0, errs, p, kind=2, extra=True)
+        arr[ix] = old
+        if not (ep["signature"] == em["signature"] == ex["signature"]):
+            continue
+        fd = (lp - lm) / (2 * eps)
+        if abs(fd) < 1e-9:
+            # weights whose effect is below the differences' rounding (a
+            # loss of ~0.05 differenced at eps 1e-3 resolves ~1e-13)
+            continue
+        rel = abs(g[ix] - fd) / abs(fd)
+        worst = max(worst, rel)
+        found[name] += 1
+        assert rel < 1e-3, (name, ix, g[ix], fd)
+    print("smooth pieces checked", found, "worst relative error %.2e" % worst)
+    assert all(v >= 5 for v in found.values()), found
