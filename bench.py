@@ -18,7 +18,7 @@ the learned FER no higher than the upper end of plain BP's 95% interval;
 "fer_below_plain_ci" when it is under the interval's lower end) beside it.
 
 Prints one JSON line (last line of stdout).  `roofline` is for the learned
-decoder's kernel (decode_qc_ms_kernel): it keeps every message on chip, so
+decoder's kernel (decode_qc_lms_kernel): it keeps every message on chip, so
 HBM does not bind it; the vector ALU does.  Its per-trial-iteration issue
 slots (wave64 VALU instructions at their issue cost) come from the rocprofv3
 counter profile committed for this kernel source (profiles/, checked by the
@@ -440,11 +440,12 @@ def copy_calibration(mib):
 
 
 def ms_kernel_name(model):
-    """Instance of decode_qc_ms_kernel<P, J, L, DAMPED, LAYERED> that runs a
-    learned model on the included code."""
-    return "decode_qc_ms_kernel<768, 3, 12, %s, %s>" % (
-        "true" if (model.kind == 1 and model.use_mlp) else "false",
-        "true" if model.kind == 2 else "false")
+    """Kernel instance that runs a learned min-sum model on the included code:
+    decode_qc_lms_kernel<P, J, L> (layered) or decode_qc_ms_kernel<P, J, L,
+    DAMPED> (flooding)."""
+    if model.kind == 2:
+        return "decode_qc_lms_kernel<768, 3, 12>"
+    return "decode_qc_ms_kernel<768, 3, 12, %s>" % ("true" if model.use_mlp else "false")
 
 
 def profile_run(jbp, args):

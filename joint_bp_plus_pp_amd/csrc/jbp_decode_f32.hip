@@ -571,117 +571,232 @@ __device__ __forceinline__ int qc_phase_ms(uint32_t *Bw, int *s_cnt, float *s_d,
 }
 
 // Layered learned min-sum phase (kind kLearnedLMS, the training kernels'
-// learned_*_lms_kernel in base 2): block row r of both sides is layer r;
-// iteration k updates the layers in turn.  Thread t keeps the weighted check
-// messages of its variables (cc, t) for all three rows in registers (36
-// (X, Z) pairs); layer r is
-//   var step:  S = sum_r' lambda[r'],  G = S + F(S'),  est = sign bits of G,
-//              mu = wv[k][r] * (G - lambda[r]) written to row r's slots
-//   check step (row r, thread t's X and Z checks): min-sum as above
-//   read back: lambda[r] = wc[k][r] * (row r's new check outputs)
-// so each layer starts from the messages the layers before it left.  An
-// iteration converges when its three layers start from the same estimate
-// (est) and every row's parity (taken from the estimate bits the layer's mu
-// carry) matches the syndrome.  The packed slot addresses are laundered at
-// each layer so that only the current layer's are unpacked at a time.
+// learned_*_lms_kernel in base 2): block row r of both sides is layer r, and
+// iteration k updates the layers in turn, each from the messages the layers
+// before it left.  Check-centric form: thread t owns the X check (r, t) and
+// the Z check (r, t) of every row and keeps their weighted outgoing messages
+// lambda (36 (X, Z) FP16 pairs) in registers; the per-variable sums
+//   S(v) = (S_x(v), S_z(v)) = sums of v's current weighted check messages
+// live in LDS as FP16 pairs, in three buffers: layer r reads buffer r and
+// writes buffer r+1 mod 3, so every LDS offset is a constant.  Layer
+// r, for the X check's variable v and the Z check's variable v' at each
+// position cc (one packed lane each):
+//   U = (S_x(v), S_z(v')),  W = (S_z(v), S_x(v'))      (from the read buffer)
+//   F = the joint channel term of W (a clamp, LmsChan),  G = U + F
+//   mu = wv[k][r][cc] * (U - lambda + F),  lambda' = wc[k][r][cc] * CN(mu)
+//   U - lambda + lambda' -> the write buffer's S_x(v) and S_z(v')
+// Every variable meets exactly one X and one Z check of a block row, so each
+// half of each sum is written once per layer, and one barrier separates
+// layers.  (The sums are updated in place rather than re-added from the
+// three rows; the difference is FP16 rounding.)
+//
+// Convergence.  The sign bits of G are the estimate this layer starts from;
+// their parity over a check is compared with its syndrome bit.  After three
+// consecutive layers whose rows all accept, the estimate of the current sums
+// is verified against every row (verify) and, if it holds, the trial stops
+// with that estimate; iterations = the number begun.  A trial whose layers
+// settle on a valid estimate thus stops within the same iteration as a test
+// at iteration boundaries would, without running the remaining layers.
+struct LmsChan {
+    h2 s, k, d, lo;  // c = clamp01(s * W + k), F = d * c + lo, per half
+};
+// one half: F(L) = max(a + L, b) - max(c + L, d) = min(max(sl*L + k, lo), hi)
+__device__ __forceinline__ void lms_chan_half(float a, float b, float c, float d, float (&v)[4]) {
+    a = fmaxf(a, -100.0f);
+    b = fmaxf(b, -100.0f);
+    c = fmaxf(c, -100.0f);
+    d = fmaxf(d, -100.0f);
+    const bool rising = b - a <= d - c;
+    const float sl = rising ? 1.0f : -1.0f;
+    const float k = rising ? a - d : b - c;
+    const float lo = rising ? b - d : a - c;
+    const float hi = rising ? a - c : b - d;
+    const float w = hi - lo;
+    const bool flat = !(w > 1e-3f);
+    v[0] = flat ? 0.0f : sl / w;
+    v[1] = flat ? 0.0f : (k - lo) / w;
+    v[2] = flat ? 0.0f : w;
+    v[3] = lo;
+}
+__device__ __forceinline__ LmsChan lms_chan(const PriorF &pr) {
+    float x[4], z[4];
+    lms_chan_half(pr.lp[0], pr.lp[2], pr.lp[1], pr.lp[3], x);  // X half: (I, Z, X, Y)
+    lms_chan_half(pr.lp[0], pr.lp[1], pr.lp[2], pr.lp[3], z);  // Z half: (I, X, Z, Y)
+    LmsChan m;
+    m.s = h2{(_Float16)x[0], (_Float16)z[0]};
+    m.k = h2{(_Float16)x[1], (_Float16)z[1]};
+    m.d = h2{(_Float16)x[2], (_Float16)z[2]};
+    m.lo = h2{(_Float16)x[3], (_Float16)z[3]};
+    return m;
+}
+// G of the packed lanes whose own-side sums are U and other-side sums W
+__device__ __forceinline__ h2 lms_F(h2 W, const LmsChan &ch) {
+    const h2 c = __builtin_elementwise_min(
+        __builtin_elementwise_max(__builtin_elementwise_fma(W, ch.s, ch.k), h2{(_Float16)0.0f, (_Float16)0.0f}),
+        h2{(_Float16)1.0f, (_Float16)1.0f});
+    return __builtin_elementwise_fma(c, ch.d, ch.lo);
+}
+
+typedef __attribute__((address_space(4))) const uint32_t cu32;
+
+// LDS layout of the sums: variable (cc, y) of buffer bf is word
+// cc*3P + bf*P + y.  A thread's packed slot address (16 bits) holds the LDS
+// base of the sums plus 4*y plus lms_split(cc), so that the rest of the
+// offset, lms_off(bf, cc), fits the LDS instruction's 16-bit immediate.
+template <int P>
+__device__ __forceinline__ constexpr uint32_t lms_split(int cc) {
+    return cc * 3 * P * 4 > 45056 ? static_cast<uint32_t>(cc * 3 * P * 4 - 45056) : 0u;
+}
+template <int P>
+__device__ __forceinline__ constexpr uint32_t lms_off(int bf, int cc) {
+    return static_cast<uint32_t>((cc * 3 * P + bf * P) * 4) - lms_split<P>(cc);
+}
+
 template <int P, int J, int L>
-__device__ __forceinline__ int qc_phase_lms(uint32_t *Bw, uint32_t *s_w, uint32_t (&aw)[2][J][(L + 1) / 2],
-                                            uint32_t synx, uint32_t synz, const PriorF &pr, int kmax,
-                                            const LearnedInfer &li, uint32_t &estx, uint32_t &estz,
-                                            int &matched) {
-    constexpr int HB = J * L / 2;
+__device__ __forceinline__ int qc_phase_lms(uint32_t sb, uint32_t (&aw)[J][L], uint32_t synx,
+                                            uint32_t synz, const PriorF &pr, int kmax, const cu32 *rec,
+                                            int *s_flag, uint32_t &estx, uint32_t &estz, int &matched) {
+    static_assert(lms_off<P>(2, L - 1) + 2 < 65536 && (P - 1) * 4 + lms_split<P>(L - 1) + 1024 < 65536,
+                  "sum-buffer offsets must fit the LDS immediate and 16-bit slot addresses");
     const int t = threadIdx.x;
-    auto launder = [&]() {
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-            for (int r = 0; r < J; ++r)
-#pragma unroll
-                for (int w = 0; w < (L + 1) / 2; ++w) asm volatile("" : "+v"(aw[s][r][w]));
-    };
-    auto at = [&](int s, int r, int cc) -> uint32_t {
-        const uint32_t a16 = (cc & 1) ? (aw[s][r][cc >> 1] >> 16) : (aw[s][r][cc >> 1] & 0xffffu);
-        return a16 + static_cast<uint32_t>(((r * L + cc) % HB) * P * 4) + (s ? 2u : 0u);
-    };
-    auto ldh = [](uint32_t a) -> _Float16 { return *reinterpret_cast<lds_h16 *>(static_cast<uintptr_t>(a)); };
+    const bool lane0 = (t & 63) == 0;
+    constexpr uint32_t kSign = 0x80008000u, kMag = 0x7fff7fffu;
+    auto lds32 = [](uint32_t a) -> lds_u32 * { return reinterpret_cast<lds_u32 *>(static_cast<uintptr_t>(a)); };
     auto sth = [](uint32_t a, _Float16 v) { *reinterpret_cast<lds_h16 *>(static_cast<uintptr_t>(a)) = v; };
-    auto stage = [&](int kk) {
-        if (t < kMsRec && kk < kmax) s_w[(kk & 1) * kMsRec + t] = li.ms_rec[kk * kMsRec + t];
+    auto launder = [&](int r) {
+#pragma unroll
+        for (int cc = 0; cc < L; ++cc) asm volatile("" : "+v"(aw[r][cc]));
     };
-    const MsChan ch = ms_chan(pr);
+    const LmsChan ch = lms_chan(pr);
+    uint32_t synw[J];
+#pragma unroll
+    for (int r = 0; r < J; ++r) synw[r] = (((synx >> r) & 1u) << 15) | (((synz >> r) & 1u) << 31);
     h2 lam[J][L];
 #pragma unroll
     for (int r = 0; r < J; ++r)
 #pragma unroll
         for (int cc = 0; cc < L; ++cc) lam[r][cc] = h2{(_Float16)0.0f, (_Float16)0.0f};
-    stage(0);
-    int k = 0;
+#pragma unroll
+    for (int cc = 0; cc < L; ++cc) *lds32(sb + (cc * 3 * P + t) * 4) = 0u;
     __syncthreads();
-    for (;;) {
-        stage(k + 1);
-        const uint32_t *rec = s_w + (k & 1) * kMsRec;
-        uint32_t est0 = 0, estchg = 0, bad = 0;
+
+    // parity of the estimate of buffer bf's sums over every row, vs the syndrome
+    auto verify = [&](int bf) -> uint32_t {
+        uint32_t vb = 0;
 #pragma unroll
         for (int r = 0; r < J; ++r) {
-            uint32_t est_r = 0;
-            launder();
+            launder(r);
+            uint32_t wx[L], wz[L];
 #pragma unroll
             for (int cc = 0; cc < L; ++cc) {
-                const h2 wv = as_h2(rec[cc * kMsRecCC + 3 + r]);
-                const h2 S = lam[0][cc] + lam[1][cc] + lam[2][cc];
-                const h2 G = ms_marginal(S, ch);
-                const uint32_t eb = (h2_bits(G) >> 15) & 0x00010001u;
-                est_r |= eb << cc;
-                const h2 mb = as_h2((h2_bits(wv * (G - lam[r][cc])) & 0xfffefffeu) | eb);
-                sth(at(0, r, cc), mb.x);
-                sth(at(1, r, cc), mb.y);
+                wx[cc] = *lds32((aw[r][cc] & 0xffffu) + lms_off<P>(bf, cc));
+                wz[cc] = *lds32((aw[r][cc] >> 16) + lms_off<P>(bf, cc));
             }
-            if (r == 0) est0 = est_r;
-            else estchg |= est_r ^ est0;
-            __syncthreads();
+            uint32_t acc = 0;
+#pragma unroll
+            for (int cc = 0; cc < L; ++cc) {
+                const h2 U = as_h2((wx[cc] & 0xffffu) | (wz[cc] & 0xffff0000u));
+                const h2 W = as_h2(__builtin_amdgcn_alignbit(wz[cc], wx[cc], 16));
+                acc ^= h2_bits(U + lms_F(W, ch));
+            }
+            vb |= (acc ^ synw[r]) & kSign;
+        }
+        return vb;
+    };
+
+    int nv = 0;
+    uint32_t hist = 7;  // bit i: a row rejected the estimate of the layer i steps back
+    // weights of layer (k, r): (wc X, wc Z) per column block at w[cc], (wv X,
+    // wv Z) at w[12 + cc]; loaded one layer ahead into scalar registers
+    uint32_t wnext[2 * L];
+#pragma unroll
+    for (int i = 0; i < 2 * L; ++i) wnext[i] = rec[i];
+    for (int k = 0; k < kmax; ++k) {
+#pragma unroll
+        for (int r = 0; r < J; ++r) {
+            const int j = k * J + r;
+            uint32_t wcur[2 * L];
+#pragma unroll
+            for (int i = 0; i < 2 * L; ++i) wcur[i] = wnext[i];
+            launder(r);
+            uint32_t wx[L], wz[L];
+#pragma unroll
+            for (int cc = 0; cc < L; ++cc) {
+                wx[cc] = *lds32((aw[r][cc] & 0xffffu) + lms_off<P>(r, cc));
+                wz[cc] = *lds32((aw[r][cc] >> 16) + lms_off<P>(r, cc));
+            }
+            h2 E[L], mu[L];
+            u16x2 pre[L];
+            u16x2 run = as_u16x2(kMsClampPair);
+            uint32_t acc = 0, sg = 0;
+#pragma unroll
+            for (int cc = 0; cc < L; ++cc) {
+                const h2 U = as_h2((wx[cc] & 0xffffu) | (wz[cc] & 0xffff0000u));
+                const h2 W = as_h2(__builtin_amdgcn_alignbit(wz[cc], wx[cc], 16));
+                const h2 F = lms_F(W, ch);
+                E[cc] = U - lam[r][cc];
+                acc ^= h2_bits(U + F);
+                mu[cc] = as_h2(wcur[L + cc]) * (E[cc] + F);
+                sg ^= h2_bits(mu[cc]);
+                pre[cc] = run;
+                run = umin2(run, as_u16x2(h2_bits(mu[cc]) & kMag));
+            }
             {
-                lds_u32 *row = reinterpret_cast<lds_u32 *>(reinterpret_cast<uintptr_t>(Bw)) + r * L * P + t;
-                const uint32_t synw = ((synx >> r) & 1u) | (((synz >> r) & 1u) << 16);
-                uint32_t u[L];
-                u16x2 pre[L];
-                u16x2 run = as_u16x2(kMsClampPair);
-                uint32_t acc = 0;
+                // the next layer's record (issued after this layer's reads;
+                // record T, past the last layer, exists)
+                const cu32 *wn = rec + (j + 1) * 32;
 #pragma unroll
-                for (int i = 0; i < L; ++i) {
-                    u[i] = row[i * P];
-                    acc ^= u[i];
-                    pre[i] = run;
-                    run = umin2(run, as_u16x2(u[i] & 0x7fff7fffu));
-                }
-                bad |= (acc ^ synw) & 0x00010001u;  // estimate parity vs syndrome
-                const uint32_t sall = acc ^ (synw << 15);
-                u16x2 suf = as_u16x2(kMsClampPair);
-#pragma unroll
-                for (int i = L - 1; i >= 0; --i) {
-                    const uint32_t mag = u16x2_bits(umin2(pre[i], suf));
-                    suf = umin2(suf, as_u16x2(u[i] & 0x7fff7fffu));
-                    row[i * P] = mag | ((sall ^ u[i]) & 0x80008000u);
-                }
+                for (int i = 0; i < 2 * L; ++i) wnext[i] = wn[i];
             }
-            __syncthreads();
-            launder();
+            const uint32_t bad = (acc ^ synw[r]) & kSign;
+            const uint32_t sall = (sg ^ synw[r]) & kSign;
+            const int wb = (r + 1) % J;  // the buffer this layer writes
+            u16x2 suf = as_u16x2(kMsClampPair);
 #pragma unroll
-            for (int cc = 0; cc < L; ++cc)
-                lam[r][cc] = as_h2(rec[cc * kMsRecCC + r]) * h2{ldh(at(0, r, cc)), ldh(at(1, r, cc))};
-        }
-        ++k;
-        const int any_bad = __syncthreads_or(static_cast<int>(bad | estchg));
-        estx = est0 & 0xffffu;
-        estz = est0 >> 16;
-        if (!any_bad) {
-            matched = 1;
-            return k;
-        }
-        if (k >= kmax) {
-            matched = 0;
-            return k;
+            for (int cc = L - 1; cc >= 0; --cc) {
+                const uint32_t m = h2_bits(mu[cc]) & kMag;
+                const uint32_t mag = u16x2_bits(umin2(pre[cc], suf));
+                suf = umin2(suf, as_u16x2(m));
+                const h2 p = as_h2(wcur[cc]) * as_h2(mag);
+                const h2 lnew = as_h2(h2_bits(p) ^ ((h2_bits(mu[cc]) ^ m) ^ sall));
+                const h2 S = E[cc] + lnew;
+                lam[r][cc] = lnew;
+                sth((aw[r][cc] & 0xffffu) + lms_off<P>(wb, cc), S.x);
+                sth((aw[r][cc] >> 16) + lms_off<P>(wb, cc) + 2, S.y);
+            }
+            if (__ballot(bad != 0) != 0 && lane0) s_flag[j & 3] = 1;
+            if (t == 0) s_flag[(j + 2) & 3] = 0;
+            __syncthreads();
+            const int f = __builtin_amdgcn_readfirstlane(s_flag[j & 3]);
+            hist = (hist << 1) | (f ? 1u : 0u);
+            if ((hist & 7u) == 0) {
+                const uint32_t vb = verify(wb);
+                const int vi = 4 + (nv & 1);
+                if (t == 0) s_flag[4 + ((nv + 1) & 1)] = 0;
+                if (__ballot(vb != 0) != 0 && lane0) s_flag[vi] = 1;
+                __syncthreads();
+                const int vf = __builtin_amdgcn_readfirstlane(s_flag[vi]);
+                ++nv;
+                if (!vf) {
+                    // the estimate of the variables (cc, t) from the same sums
+                    estx = 0;
+                    estz = 0;
+#pragma unroll
+                    for (int cc = 0; cc < L; ++cc) {
+                        const h2 S = as_h2(*lds32(sb + (cc * 3 * P + wb * P + t) * 4));
+                        const uint32_t g = h2_bits(S + lms_F(__builtin_shufflevector(S, S, 1, 0), ch));
+                        estx |= ((g >> 15) & 1u) << cc;
+                        estz |= (g >> 31) << cc;
+                    }
+                    matched = 1;
+                    return k + 1;
+                }
+                hist = 1;
+            }
         }
     }
+    matched = 0;
+    return kmax;
 }
 
 // Exchange-slot addresses of thread t's variables (cc, t): row index
@@ -857,12 +972,12 @@ __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, 
     }
 }
 
-// The learned min-sum decoder (li.T iterations of qc_phase_ms).  A trial
+// The learned flooding min-sum decoder (li.T iterations of qc_phase_ms).  A trial
 // that converges is finished here; the others are appended to fb_list
 // (fb_ctr[0] of them) for plain BP, launched next on the same stream, which
 // decodes them from the channel prior (reported iterations li.T + its own).
 // Its exchange buffer is B only (J*L*P words).
-template <int P, int J, int L, bool DAMPED, bool LAYERED>
+template <int P, int J, int L, bool DAMPED>
 __global__ void __launch_bounds__(P) decode_qc_ms_kernel(DevGraph g, QcDev qc, TrialPlanes tp,
                                                         int count, NoiseTab nt, int *counter,
                                                         LearnedInfer li, int *fb_list, int *fb_ctr) {
@@ -901,13 +1016,76 @@ __global__ void __launch_bounds__(P) decode_qc_ms_kernel(DevGraph g, QcDev qc, T
         }
         uint32_t estx = 0, estz = 0;
         int matched = 0;
-        int k;
-        if constexpr (LAYERED)
-            k = qc_phase_lms<P, J, L>(reinterpret_cast<uint32_t *>(B), s_w, aw, synx, synz, pr, li.T, li,
-                                      estx, estz, matched);
-        else
-            k = qc_phase_ms<P, J, L, DAMPED>(reinterpret_cast<uint32_t *>(B), s_cnt, &s_d, s_w, aw, synx,
-                                             synz, pr, li.T, li, u0, estx, estz, matched);
+        const int k = qc_phase_ms<P, J, L, DAMPED>(reinterpret_cast<uint32_t *>(B), s_cnt, &s_d, s_w, aw,
+                                                   synx, synz, pr, li.T, li, u0, estx, estz, matched);
+        if (matched) {
+            write_outcome<P, L>(g, tp, trial, estx, estz, k, 1);
+        } else if (t == 0) {
+            fb_list[atomicAdd(fb_ctr, 1)] = trial;
+        }
+        __syncthreads();
+    }
+}
+
+// Sum-buffer slot addresses of thread t's checks: position cc of the X
+// check (r, t) is variable (cc, a*t + b mod P) of block (X, r, cc); likewise
+// the Z check.  One register per (r, cc): the X check's in the low 16 bits,
+// the Z check's in the high 16 (see lms_off).
+template <int P, int J, int L>
+__device__ __forceinline__ void lms_slot_addresses(const int *qc_tab, uint32_t sb,
+                                                   uint32_t (&aw)[J][L]) {
+        const int t = threadIdx.x;
+    constexpr int JL = J * L;
+#pragma unroll
+    for (int r = 0; r < J; ++r)
+#pragma unroll
+        for (int cc = 0; cc < L; ++cc) {
+            uint32_t a16[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int a = qc_tab[(3 * s + 0) * JL + r * L + cc];
+                const int b = qc_tab[(3 * s + 1) * JL + r * L + cc];
+                const uint32_t y = static_cast<uint32_t>((a * t + b) % P);
+                a16[s] = sb + y * 4u + lms_split<P>(cc);
+            }
+            aw[r][cc] = a16[0] | (a16[1] << 16);
+        }
+}
+
+// The layered learned min-sum decoder (li.T iterations of qc_phase_lms).
+// A trial that converges is finished here; the others are appended to
+// fb_list (fb_ctr[0] of them) for plain BP, launched next, which decodes
+// them from the channel prior (reported iterations li.T + its own).  LDS:
+// the three sum buffers (3*L*P words).
+template <int P, int J, int L>
+__global__ void __launch_bounds__(P) decode_qc_lms_kernel(DevGraph g, TrialPlanes tp, int count,
+                                                         NoiseTab nt, int *counter, LearnedInfer li,
+                                                         int *fb_list, int *fb_ctr) {
+    static_assert(P % 64 == 0 && P <= 1024, "one thread per check of a block row, whole wavefronts");
+    extern __shared__ uint32_t Sbuf[];
+    __shared__ int s_trial;
+    __shared__ int s_flag[6];
+    const int t = threadIdx.x;
+    const uint32_t sb = static_cast<uint32_t>((uintptr_t)((lds_u32 *)Sbuf));  // LDS address of the sums
+    uint32_t aw[J][L];
+    lms_slot_addresses<P, J, L>(g.qc_tab, sb, aw);
+    const cu32 *rec = (const cu32 *)li.ms_rec;  // scalar loads
+    for (;;) {
+        if (t == 0) {
+            const int k = atomicAdd(counter, 1);
+            s_trial = k < count ? (tp.order ? tp.order[k] : k) : -1;
+        }
+        __syncthreads();
+        const int trial = s_trial;
+        if (t < 6) s_flag[t] = 0;
+        __syncthreads();
+        if (trial < 0) return;
+        const PriorF pr = trial_prior(nt, trial);
+        uint32_t synx, synz;
+        trial_syndromes<P, J>(g, tp, trial, synx, synz);
+        uint32_t estx = 0, estz = 0;
+        int matched = 0;
+        const int k = qc_phase_lms<P, J, L>(sb, aw, synx, synz, pr, li.T, rec, s_flag, estx, estz, matched);
         if (matched) {
             write_outcome<P, L>(g, tp, trial, estx, estz, k, 1);
         } else if (t == 0) {
@@ -994,9 +1172,18 @@ void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_ta
             hipLaunchKernelGGL(kern, dim3(grid), dim3(768), ms_lds, st, g, dev, tp, count, nt, counter, li,
                                fb + 4, fb);
         };
-        if (learned->kind == kLearnedLMS) ms(decode_qc_ms_kernel<768, 3, 12, false, true>);
-        else if (learned->use_mlp) ms(decode_qc_ms_kernel<768, 3, 12, true, false>);
-        else ms(decode_qc_ms_kernel<768, 3, 12, false, false>);
+        if (learned->kind == kLearnedLMS) {
+            const size_t lms_lds = static_cast<size_t>(3) * qc.L * qc.P * sizeof(uint32_t);
+            auto kern = decode_qc_lms_kernel<768, 3, 12>;
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lms_lds));
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(768), lms_lds, st, g, tp, count, nt, counter, li, fb + 4,
+                               fb);
+        } else if (learned->use_mlp) {
+            ms(decode_qc_ms_kernel<768, 3, 12, true>);
+        } else {
+            ms(decode_qc_ms_kernel<768, 3, 12, false>);
+        }
         second(learned->T);
     } else if (learned && learned->T > 0) {
         if (learned->use_mlp)

@@ -63,7 +63,8 @@ struct LearnedInfer {
     // for r = 0..J-1, then (wv X, wv Z) of iteration k (1 at k = T), as FP16
     // pairs, two words of padding (device, (T+1) x kMsRec uint32).
     // Layered min-sum kind: record k = 0..T-1 holds iteration k+1's own
-    // (wc X, wc Z) and (wv X, wv Z) per (r, cc) in the same places.
+    // weights as one 32-word group per block row r: (wc X, wc Z) for column
+    // blocks 0..11, then (wv X, wv Z) for 0..11, then 8 words of padding.
     const uint32_t *ms_rec = nullptr;
 };
 // Words of one min-sum weight record: 12 column blocks x (3 + 3 + 2).

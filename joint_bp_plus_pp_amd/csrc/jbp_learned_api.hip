@@ -391,7 +391,9 @@ int jbp_simulate_learned(jbp_ctx *ctx, const jbp_learned *m, const jbp_sim_cfg *
     if (m->kind == kLearnedMS || m->kind == kLearnedLMS) {
         // min-sum: variable pass k: (wc X, wc Z) of iteration k-1 per row
         // block, then (wv X, wv Z) of iteration k, FP16 pairs (1 where there
-        // is none).  Layered: record k = iteration k+1's own wc and wv.
+        // is none).  Layered: record k = iteration k+1's own weights, one
+        // 32-word group per block row r: (wc X, wc Z) per column block, then
+        // (wv X, wv Z) per column block, then padding.
         if (ctx->qcp.J != 3 || ctx->qcp.L != 12) {
             set_err(err, errlen, "the min-sum decoder is built for 3 x 12 blocks");
             return JBP_ERR_UNSUPPORTED;
@@ -413,8 +415,10 @@ int jbp_simulate_learned(jbp_ctx *ctx, const jbp_learned *m, const jbp_sim_cfg *
                     const int b = r * L + cc;
                     uint32_t *w = &rec[static_cast<size_t>(k) * kMsRec + cc * kMsRecCC];
                     if (layered) {
-                        w[r] = k == T ? pair(1.0, 1.0) : pair(par(0, b, k), par(1, b, k));
-                        w[3 + r] = k == T ? pair(1.0, 1.0) : pair(par(2, b, k), par(3, b, k));
+                        if (k == T) continue;
+                        uint32_t *lw = &rec[static_cast<size_t>(k) * kMsRec + r * 32];
+                        lw[cc] = pair(par(0, b, k), par(1, b, k));
+                        lw[12 + cc] = pair(par(2, b, k), par(3, b, k));
                         continue;
                     }
                     w[r] = k == 0 ? pair(1.0, 1.0) : pair(par(0, b, k - 1), par(1, b, k - 1));
