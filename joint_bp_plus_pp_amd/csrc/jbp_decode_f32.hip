@@ -728,16 +728,25 @@ __device__ __forceinline__ int qc_phase_lms(uint32_t sb, uint32_t (&aw)[J][L], u
             h2 E[L], mu[L];
             u16x2 pre[L];
             u16x2 run = as_u16x2(kMsClampPair);
-            uint32_t acc = 0, sg = 0;
+            // acc: sign parity of G (the estimate), sg: of mu; folded two
+            // positions at a time (three-input XORs)
+            uint32_t acc = 0, sg = 0, gprev = 0, mprev = 0;
 #pragma unroll
             for (int cc = 0; cc < L; ++cc) {
                 const h2 U = as_h2((wx[cc] & 0xffffu) | (wz[cc] & 0xffff0000u));
                 const h2 W = as_h2(__builtin_amdgcn_alignbit(wz[cc], wx[cc], 16));
                 const h2 F = lms_F(W, ch);
                 E[cc] = U - lam[r][cc];
-                acc ^= h2_bits(U + F);
+                const uint32_t gb = h2_bits(U + F);
                 mu[cc] = as_h2(wcur[L + cc]) * (E[cc] + F);
-                sg ^= h2_bits(mu[cc]);
+                const uint32_t mb = h2_bits(mu[cc]);
+                if (cc & 1) {
+                    acc = __builtin_amdgcn_bitop3_b32(acc, gprev, gb, 0x96);  // a ^ b ^ c
+                    sg = __builtin_amdgcn_bitop3_b32(sg, mprev, mb, 0x96);
+                } else {
+                    gprev = gb;
+                    mprev = mb;
+                }
                 pre[cc] = run;
                 run = umin2(run, as_u16x2(h2_bits(mu[cc]) & kMag));
             }
@@ -1034,8 +1043,18 @@ __global__ void __launch_bounds__(P) decode_qc_ms_kernel(DevGraph g, QcDev qc, T
 template <int P, int J, int L>
 __device__ __forceinline__ void lms_slot_addresses(const int *qc_tab, uint32_t sb,
                                                    uint32_t (&aw)[J][L]) {
-        const int t = threadIdx.x;
+    // the (a, b) tables of both sides pass through LDS (the sum buffers are
+    // not in use yet), so they are read per thread rather than held in
+    // scalar registers
+    const int t = threadIdx.x;
     constexpr int JL = J * L;
+    static_assert(4 * JL <= P, "one thread per table entry");
+    if (t < 4 * JL) {
+        const int s = t / (2 * JL), rest = t % (2 * JL);  // rest: a (0..JL-1) then b
+        *reinterpret_cast<lds_u32 *>(static_cast<uintptr_t>(sb + t * 4)) =
+            static_cast<uint32_t>(qc_tab[(3 * s + rest / JL) * JL + rest % JL]);
+    }
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < J; ++r)
 #pragma unroll
@@ -1043,13 +1062,15 @@ __device__ __forceinline__ void lms_slot_addresses(const int *qc_tab, uint32_t s
             uint32_t a16[2];
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
-                const int a = qc_tab[(3 * s + 0) * JL + r * L + cc];
-                const int b = qc_tab[(3 * s + 1) * JL + r * L + cc];
+                const lds_u32 *row = reinterpret_cast<lds_u32 *>(static_cast<uintptr_t>(sb + s * 2 * JL * 4));
+                const int a = static_cast<int>(row[r * L + cc]);
+                const int b = static_cast<int>(row[JL + r * L + cc]);
                 const uint32_t y = static_cast<uint32_t>((a * t + b) % P);
                 a16[s] = sb + y * 4u + lms_split<P>(cc);
             }
             aw[r][cc] = a16[0] | (a16[1] << 16);
         }
+    __syncthreads();
 }
 
 // The layered learned min-sum decoder (li.T iterations of qc_phase_lms).
@@ -1058,7 +1079,7 @@ __device__ __forceinline__ void lms_slot_addresses(const int *qc_tab, uint32_t s
 // them from the channel prior (reported iterations li.T + its own).  LDS:
 // the three sum buffers (3*L*P words).
 template <int P, int J, int L>
-__global__ void __launch_bounds__(P) decode_qc_lms_kernel(DevGraph g, TrialPlanes tp, int count,
+__global__ void __launch_bounds__(P) __attribute__((amdgpu_num_vgpr(80))) decode_qc_lms_kernel(DevGraph g, TrialPlanes tp, int count,
                                                          NoiseTab nt, int *counter, LearnedInfer li,
                                                          int *fb_list, int *fb_ctr) {
     static_assert(P % 64 == 0 && P <= 1024, "one thread per check of a block row, whole wavefronts");

@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(64) seed_kernel(uint64_t base_seed, int64_t fi
 // (state row blockIdx.x); grid = trials, block = 64 (one wavefront),
 // dynamic LDS = 2*(wn+1) words.
 template <int CW>
-__global__ void __launch_bounds__(64) sample_kernel(DevGraph g, TrialPlanes tp, const uint64_t *state,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_num_vgpr(16))) sample_kernel(DevGraph g, TrialPlanes tp, const uint64_t *state,
                                                     int trial0, NoiseTab nt) {
     __shared__ uint64_t mt[kMtN];
     extern __shared__ uint32_t planes[];  // err_x | err_z, wn + 1 words each
@@ -325,8 +325,9 @@ __global__ void __launch_bounds__(64) weight_kernel(DevGraph g, TrialPlanes tp, 
 
 // One block: counting sort of the keys, largest first.  A pipelined run
 // orders chunk j+1 between the end of its sampling and the start of its
-// decoder launch, so the block is wide (16 wavefronts) to be short.
-constexpr int kOrderBlock = 1024;
+// decoder launch; four wavefronts (one per SIMD) with few registers, so the
+// block fits beside a resident decoder workgroup.
+constexpr int kOrderBlock = 256;
 constexpr int kOrderKeys = 4096 / kOrderBlock;  // bins per thread in the scan
 __global__ void __launch_bounds__(kOrderBlock) order_kernel(const int *keys, int count, int *order) {
     __shared__ int bins[4096];
