@@ -597,8 +597,25 @@ __device__ __forceinline__ int qc_phase_ms(uint32_t *Bw, int *s_cnt, float *s_d,
 // with that estimate; iterations = the number begun.  A trial whose layers
 // settle on a valid estimate thus stops within the same iteration as a test
 // at iteration boundaries would, without running the remaining layers.
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS (and
+// scalar) operations, not for its global stores, which no thread of the
+// workgroup reads back (__syncthreads would wait for them: trial outcomes
+// would then stall every trial's end).  The empty asm statements keep the
+// compiler from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt not waited for
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// The joint channel term of one half is F(W) = lo + d * clamp01(s*W + k0).
+// The sums are stored offset by lo of their own side (S' = S + lo), so
+// that G = U' + d*c and mu' = (U' - lambda) + d*c are one FMA each; the
+// clamp argument, which reads the other side's offset sums W', uses
+// k = k0 - s * (lo of the other side).
 struct LmsChan {
-    h2 s, k, d, lo;  // c = clamp01(s * W + k), F = d * c + lo, per half
+    h2 s, k, d, lo;  // per half (X low, Z high)
 };
 // one half: F(L) = max(a + L, b) - max(c + L, d) = min(max(sl*L + k, lo), hi)
 __device__ __forceinline__ void lms_chan_half(float a, float b, float c, float d, float (&v)[4]) {
@@ -624,17 +641,21 @@ __device__ __forceinline__ LmsChan lms_chan(const PriorF &pr) {
     lms_chan_half(pr.lp[0], pr.lp[1], pr.lp[2], pr.lp[3], z);  // Z half: (I, X, Z, Y)
     LmsChan m;
     m.s = h2{(_Float16)x[0], (_Float16)z[0]};
-    m.k = h2{(_Float16)x[1], (_Float16)z[1]};
+    // the X half's clamp reads Z sums (offset by the Z lo), and vice versa
+    m.k = h2{(_Float16)(x[1] - x[0] * z[3]), (_Float16)(z[1] - z[0] * x[3])};
     m.d = h2{(_Float16)x[2], (_Float16)z[2]};
     m.lo = h2{(_Float16)x[3], (_Float16)z[3]};
     return m;
 }
-// G of the packed lanes whose own-side sums are U and other-side sums W
-__device__ __forceinline__ h2 lms_F(h2 W, const LmsChan &ch) {
-    const h2 c = __builtin_elementwise_min(
+// clamp01(s * W' + k) of the packed lanes whose other-side offset sums are W'
+__device__ __forceinline__ h2 lms_c(h2 W, const LmsChan &ch) {
+    return __builtin_elementwise_min(
         __builtin_elementwise_max(__builtin_elementwise_fma(W, ch.s, ch.k), h2{(_Float16)0.0f, (_Float16)0.0f}),
         h2{(_Float16)1.0f, (_Float16)1.0f});
-    return __builtin_elementwise_fma(c, ch.d, ch.lo);
+}
+// the marginal G of lanes with own-side offset sums U' and other-side W'
+__device__ __forceinline__ h2 lms_G(h2 U, h2 W, const LmsChan &ch) {
+    return __builtin_elementwise_fma(lms_c(W, ch), ch.d, U);
 }
 
 typedef __attribute__((address_space(4))) const uint32_t cu32;
@@ -677,8 +698,8 @@ __device__ __forceinline__ int qc_phase_lms(uint32_t sb, uint32_t (&aw)[J][L], u
 #pragma unroll
         for (int cc = 0; cc < L; ++cc) lam[r][cc] = h2{(_Float16)0.0f, (_Float16)0.0f};
 #pragma unroll
-    for (int cc = 0; cc < L; ++cc) *lds32(sb + (cc * 3 * P + t) * 4) = 0u;
-    __syncthreads();
+    for (int cc = 0; cc < L; ++cc) *lds32(sb + (cc * 3 * P + t) * 4) = h2_bits(ch.lo);  // S' = 0 + lo
+    lds_barrier();
 
     // parity of the estimate of buffer bf's sums over every row, vs the syndrome
     auto verify = [&](int bf) -> uint32_t {
@@ -697,7 +718,7 @@ __device__ __forceinline__ int qc_phase_lms(uint32_t sb, uint32_t (&aw)[J][L], u
             for (int cc = 0; cc < L; ++cc) {
                 const h2 U = as_h2((wx[cc] & 0xffffu) | (wz[cc] & 0xffff0000u));
                 const h2 W = as_h2(__builtin_amdgcn_alignbit(wz[cc], wx[cc], 16));
-                acc ^= h2_bits(U + lms_F(W, ch));
+                acc ^= h2_bits(lms_G(U, W, ch));
             }
             vb |= (acc ^ synw[r]) & kSign;
         }
@@ -735,10 +756,10 @@ __device__ __forceinline__ int qc_phase_lms(uint32_t sb, uint32_t (&aw)[J][L], u
             for (int cc = 0; cc < L; ++cc) {
                 const h2 U = as_h2((wx[cc] & 0xffffu) | (wz[cc] & 0xffff0000u));
                 const h2 W = as_h2(__builtin_amdgcn_alignbit(wz[cc], wx[cc], 16));
-                const h2 F = lms_F(W, ch);
+                const h2 c = lms_c(W, ch);
                 E[cc] = U - lam[r][cc];
-                const uint32_t gb = h2_bits(U + F);
-                mu[cc] = as_h2(wcur[L + cc]) * (E[cc] + F);
+                const uint32_t gb = h2_bits(__builtin_elementwise_fma(c, ch.d, U));
+                mu[cc] = as_h2(wcur[L + cc]) * __builtin_elementwise_fma(c, ch.d, E[cc]);
                 const uint32_t mb = h2_bits(mu[cc]);
                 if (cc & 1) {
                     acc = __builtin_amdgcn_bitop3_b32(acc, gprev, gb, 0x96);  // a ^ b ^ c
@@ -775,7 +796,7 @@ __device__ __forceinline__ int qc_phase_lms(uint32_t sb, uint32_t (&aw)[J][L], u
             }
             if (__ballot(bad != 0) != 0 && lane0) s_flag[j & 3] = 1;
             if (t == 0) s_flag[(j + 2) & 3] = 0;
-            __syncthreads();
+            lds_barrier();
             const int f = __builtin_amdgcn_readfirstlane(s_flag[j & 3]);
             hist = (hist << 1) | (f ? 1u : 0u);
             if ((hist & 7u) == 0) {
@@ -783,7 +804,7 @@ __device__ __forceinline__ int qc_phase_lms(uint32_t sb, uint32_t (&aw)[J][L], u
                 const int vi = 4 + (nv & 1);
                 if (t == 0) s_flag[4 + ((nv + 1) & 1)] = 0;
                 if (__ballot(vb != 0) != 0 && lane0) s_flag[vi] = 1;
-                __syncthreads();
+                lds_barrier();
                 const int vf = __builtin_amdgcn_readfirstlane(s_flag[vi]);
                 ++nv;
                 if (!vf) {
@@ -793,7 +814,7 @@ __device__ __forceinline__ int qc_phase_lms(uint32_t sb, uint32_t (&aw)[J][L], u
 #pragma unroll
                     for (int cc = 0; cc < L; ++cc) {
                         const h2 S = as_h2(*lds32(sb + (cc * 3 * P + wb * P + t) * 4));
-                        const uint32_t g = h2_bits(S + lms_F(__builtin_shufflevector(S, S, 1, 0), ch));
+                        const uint32_t g = h2_bits(lms_G(S, __builtin_shufflevector(S, S, 1, 0), ch));
                         estx |= ((g >> 15) & 1u) << cc;
                         estz |= (g >> 31) << cc;
                     }
@@ -868,18 +889,18 @@ __device__ __forceinline__ void write_outcome(const DevGraph &g, const TrialPlan
                                               int matched) {
     const int t = threadIdx.x;
     const int lane = t & 63;
-    uint32_t *ox = tp.est_x + static_cast<size_t>(trial) * g.wn;
-    uint32_t *oz = tp.est_z + static_cast<size_t>(trial) * g.wn;
+    uint64_t *ox = reinterpret_cast<uint64_t *>(tp.est_x + static_cast<size_t>(trial) * g.wn);
+    uint64_t *oz = reinterpret_cast<uint64_t *>(tp.est_z + static_cast<size_t>(trial) * g.wn);
 #pragma unroll
     for (int cc = 0; cc < L; ++cc) {
-        uint64_t bx = __ballot((estx >> cc) & 1u);
-        uint64_t bz = __ballot((estz >> cc) & 1u);
+        const uint64_t bx = __ballot((estx >> cc) & 1u);
+        const uint64_t bz = __ballot((estz >> cc) & 1u);
         if (lane == 0) {
-            int w = (cc * P + t) >> 5;
-            ox[w] = static_cast<uint32_t>(bx);
-            ox[w + 1] = static_cast<uint32_t>(bx >> 32);
-            oz[w] = static_cast<uint32_t>(bz);
-            oz[w + 1] = static_cast<uint32_t>(bz >> 32);
+            // one 64-bit store per plane (a store costs about as much with
+            // one active lane as with all of them)
+            const int w = (cc * P + t) >> 6;
+            ox[w] = bx;
+            oz[w] = bz;
         }
     }
     if (t == 0) {
@@ -1054,7 +1075,7 @@ __device__ __forceinline__ void lms_slot_addresses(const int *qc_tab, uint32_t s
         *reinterpret_cast<lds_u32 *>(static_cast<uintptr_t>(sb + t * 4)) =
             static_cast<uint32_t>(qc_tab[(3 * s + rest / JL) * JL + rest % JL]);
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int r = 0; r < J; ++r)
 #pragma unroll
@@ -1070,7 +1091,7 @@ __device__ __forceinline__ void lms_slot_addresses(const int *qc_tab, uint32_t s
             }
             aw[r][cc] = a16[0] | (a16[1] << 16);
         }
-    __syncthreads();
+    lds_barrier();
 }
 
 // The layered learned min-sum decoder (li.T iterations of qc_phase_lms).
@@ -1096,10 +1117,10 @@ __global__ void __launch_bounds__(P) __attribute__((amdgpu_num_vgpr(80))) decode
             const int k = atomicAdd(counter, 1);
             s_trial = k < count ? (tp.order ? tp.order[k] : k) : -1;
         }
-        __syncthreads();
+        lds_barrier();
         const int trial = s_trial;
         if (t < 6) s_flag[t] = 0;
-        __syncthreads();
+        lds_barrier();
         if (trial < 0) return;
         const PriorF pr = trial_prior(nt, trial);
         uint32_t synx, synz;
@@ -1112,7 +1133,7 @@ __global__ void __launch_bounds__(P) __attribute__((amdgpu_num_vgpr(80))) decode
         } else if (t == 0) {
             fb_list[atomicAdd(fb_ctr, 1)] = trial;
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 
