@@ -496,7 +496,7 @@ def summarize_counters(args):
     with open(args.profile_log) as f:
         log = json.load(f)
     kernels = {key: d["kernel"] for key, d in log["learned"].items()}
-    kernels["plain"] = "decode_qc_f32_kernel<768, 3, 12, false, false>"
+    kernels["plain"] = "decode_qc_plain_kernel<768, 3, 12>"
     iters = {key: d["trial_iterations"] for key, d in log["learned"].items()}
     iters["plain"] = log["plain_trial_iterations"]
     tot = {key: collections.defaultdict(float) for key in kernels}
@@ -725,7 +725,7 @@ def main():
     # launch time: what stragglers and the sampling/residual work around the
     # decoder cost the long run
     med_rate = args.chunk / (float(np.median(pl_ms)) / 1e3)
-    plain_roof = {"kernel": "decode_qc_f32_kernel (plain sum-product, FP32)",
+    plain_roof = {"kernel": "decode_qc_plain_kernel<768, 3, 12> (plain sum-product, FP32)",
                   "kernel_ms_per_launch": round(sum(pl_ms) / n_r, 4),
                   "median_launch_ms": round(float(np.median(pl_ms)), 4),
                   "max_launch_ms": round(float(np.max(pl_ms)), 4),

@@ -935,11 +935,10 @@ __device__ __forceinline__ void trial_syndromes(const DevGraph &g, const TrialPl
 // appended there (fb_ctr[0] of them) instead of finishing, for a later
 // launch with a larger max_iter to decode again from the prior.
 template <int P, int J, int L, bool LEARNED, bool DAMPED>
-__global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, TrialPlanes tp,
-                                                         int count, NoiseTab nt, int max_iter,
-                                                         int warmup, int interval, int *counter,
-                                                         LearnedInfer li, const int *count_ptr,
-                                                         int iter_offset, int *fb_list, int *fb_ctr) {
+__device__ __forceinline__ void qc_f32_body(DevGraph g, QcDev qc, TrialPlanes tp, int count, NoiseTab nt,
+                                            int max_iter, int warmup, int interval, int *counter,
+                                            LearnedInfer li, const int *count_ptr, int iter_offset,
+                                            int *fb_list, int *fb_ctr) {
     static_assert(P % 64 == 0 && P <= 1024, "one thread per block row, whole wavefronts");
     extern __shared__ float B[];  // [J][L][P] exchange buffer, then GZ[L][P]
     float *GZ = B + J * L * P;   // Z-side extrinsic base per variable, parked between passes
@@ -1000,6 +999,27 @@ __global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, 
         write_outcome<P, L>(g, tp, trial, estx, estz, iterations, matched);
         __syncthreads();
     }
+}
+
+// Plain BP: at most 160 VGPRs (three 768-thread wavefronts' worth per SIMD,
+// the few spills outside the iteration loop), which leaves room for a
+// sampling wavefront of the next chunk beside a resident workgroup.
+template <int P, int J, int L>
+__global__ void __launch_bounds__(P) __attribute__((amdgpu_num_vgpr(80)))
+decode_qc_plain_kernel(DevGraph g, QcDev qc, TrialPlanes tp, int count, NoiseTab nt, int max_iter, int warmup,
+                       int interval, int *counter, LearnedInfer li, const int *count_ptr, int iter_offset,
+                       int *fb_list, int *fb_ctr) {
+    qc_f32_body<P, J, L, false, false>(g, qc, tp, count, nt, max_iter, warmup, interval, counter, li, count_ptr,
+                                       iter_offset, fb_list, fb_ctr);
+}
+// The learned sum-product kinds (LEARNED = true).
+template <int P, int J, int L, bool LEARNED, bool DAMPED>
+__global__ void __launch_bounds__(P) decode_qc_f32_kernel(DevGraph g, QcDev qc, TrialPlanes tp, int count,
+                                                         NoiseTab nt, int max_iter, int warmup, int interval,
+                                                         int *counter, LearnedInfer li, const int *count_ptr,
+                                                         int iter_offset, int *fb_list, int *fb_ctr) {
+    qc_f32_body<P, J, L, LEARNED, DAMPED>(g, qc, tp, count, nt, max_iter, warmup, interval, counter, li,
+                                          count_ptr, iter_offset, fb_list, fb_ctr);
 }
 
 // The learned flooding min-sum decoder (li.T iterations of qc_phase_ms).  A trial
@@ -1201,7 +1221,7 @@ void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_ta
         TrialPlanes fbp = tp;
         fbp.order = fb + 4;
         li = LearnedInfer();
-        launch(decode_qc_f32_kernel<768, 3, 12, false, false>, fb, offset, fb + 1, fbp, on, max_iter,
+        launch(decode_qc_plain_kernel<768, 3, 12>, fb, offset, fb + 1, fbp, on, max_iter,
                nullptr, nullptr);
     };
     if (learned && learned->T > 0 && (learned->kind == kLearnedMS || learned->kind == kLearnedLMS)) {
@@ -1241,11 +1261,11 @@ void launch_decode_qc_f32(const DevGraph &g, const QcParams &qc, const int *d_ta
         // ends exactly as in one launch; the first launch ends at the pace of
         // the bulk instead of its slowest trial.
         launch_fill_i32(fb, 0, 2, st);
-        launch(decode_qc_f32_kernel<768, 3, 12, false, false>, nullptr, 0, counter, tp, st, cap_iter,
+        launch(decode_qc_plain_kernel<768, 3, 12>, nullptr, 0, counter, tp, st, cap_iter,
                fb + 4, fb);
         second(0);
     } else {
-        launch(decode_qc_f32_kernel<768, 3, 12, false, false>, nullptr, 0, counter, tp, st, max_iter,
+        launch(decode_qc_plain_kernel<768, 3, 12>, nullptr, 0, counter, tp, st, max_iter,
                nullptr, nullptr);
     }
 }
