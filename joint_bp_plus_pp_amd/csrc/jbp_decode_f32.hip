@@ -889,8 +889,11 @@ __device__ __forceinline__ void write_outcome(const DevGraph &g, const TrialPlan
                                               int matched) {
     const int t = threadIdx.x;
     const int lane = t & 63;
-    uint64_t *ox = reinterpret_cast<uint64_t *>(tp.est_x + static_cast<size_t>(trial) * g.wn);
-    uint64_t *oz = reinterpret_cast<uint64_t *>(tp.est_z + static_cast<size_t>(trial) * g.wn);
+    // 64-bit word (cc*P + t)/64 = cc*P/64 + t/64: one base per wavefront,
+    // the column block's part in the store's immediate offset
+    const size_t base = static_cast<size_t>(trial) * g.wn / 2 + (t >> 6);
+    uint64_t *ox = reinterpret_cast<uint64_t *>(tp.est_x) + base;
+    uint64_t *oz = reinterpret_cast<uint64_t *>(tp.est_z) + base;
 #pragma unroll
     for (int cc = 0; cc < L; ++cc) {
         const uint64_t bx = __ballot((estx >> cc) & 1u);
@@ -898,9 +901,8 @@ __device__ __forceinline__ void write_outcome(const DevGraph &g, const TrialPlan
         if (lane == 0) {
             // one 64-bit store per plane (a store costs about as much with
             // one active lane as with all of them)
-            const int w = (cc * P + t) >> 6;
-            ox[w] = bx;
-            oz[w] = bz;
+            ox[cc * (P / 64)] = bx;
+            oz[cc * (P / 64)] = bz;
         }
     }
     if (t == 0) {
