@@ -597,6 +597,17 @@ __device__ __forceinline__ int qc_phase_ms(uint32_t *Bw, int *s_cnt, float *s_d,
 // with that estimate; iterations = the number begun.  A trial whose layers
 // settle on a valid estimate thus stops within the same iteration as a test
 // at iteration boundaries would, without running the remaining layers.
+// This lane's index in its wavefront, computed where it is used (volatile,
+// so that neither it nor anything derived from it is hoisted into a
+// long-lived register: the layered kernel's registers are all taken by its
+// messages and slot addresses, and a hoisted per-thread constant ends up
+// spilled to scratch and reloaded with a memory round trip).
+__device__ __forceinline__ uint32_t lane_now() {
+    uint32_t lane;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane));
+    return lane;
+}
+
 // Workgroup barrier for LDS traffic only: waits for this wave's LDS (and
 // scalar) operations, not for its global stores, which no thread of the
 // workgroup reads back (__syncthreads would wait for them: trial outcomes
@@ -674,13 +685,12 @@ __device__ __forceinline__ constexpr uint32_t lms_off(int bf, int cc) {
 }
 
 template <int P, int J, int L>
-__device__ __forceinline__ int qc_phase_lms(uint32_t sb, uint32_t (&aw)[J][L], uint32_t synx,
+__device__ __forceinline__ int qc_phase_lms(uint32_t sb, int t0, uint32_t (&aw)[J][L], uint32_t synx,
                                             uint32_t synz, const PriorF &pr, int kmax, const cu32 *rec,
                                             int *s_flag, uint32_t &estx, uint32_t &estz, int &matched) {
     static_assert(lms_off<P>(2, L - 1) + 2 < 65536 && (P - 1) * 4 + lms_split<P>(L - 1) + 1024 < 65536,
                   "sum-buffer offsets must fit the LDS immediate and 16-bit slot addresses");
-    const int t = threadIdx.x;
-    const bool lane0 = (t & 63) == 0;
+    // t0: the wavefront's first thread (uniform); thread t0 + lane_now()
     constexpr uint32_t kSign = 0x80008000u, kMag = 0x7fff7fffu;
     auto lds32 = [](uint32_t a) -> lds_u32 * { return reinterpret_cast<lds_u32 *>(static_cast<uintptr_t>(a)); };
     auto sth = [](uint32_t a, _Float16 v) { *reinterpret_cast<lds_h16 *>(static_cast<uintptr_t>(a)) = v; };
@@ -689,16 +699,17 @@ __device__ __forceinline__ int qc_phase_lms(uint32_t sb, uint32_t (&aw)[J][L], u
         for (int cc = 0; cc < L; ++cc) asm volatile("" : "+v"(aw[r][cc]));
     };
     const LmsChan ch = lms_chan(pr);
-    uint32_t synw[J];
-#pragma unroll
-    for (int r = 0; r < J; ++r) synw[r] = (((synx >> r) & 1u) << 15) | (((synz >> r) & 1u) << 31);
+    // syndrome bits of the checks (r, t): X at bit r, Z at bit 16 + r; row
+    // r's pair moved to the sign positions (bits 15 and 31) where used
+    const uint32_t syn2 = synx | (synz << 16);
+    auto synw = [&](int r) { return (syn2 << (15 - r)) & kSign; };
     h2 lam[J][L];
 #pragma unroll
     for (int r = 0; r < J; ++r)
 #pragma unroll
         for (int cc = 0; cc < L; ++cc) lam[r][cc] = h2{(_Float16)0.0f, (_Float16)0.0f};
 #pragma unroll
-    for (int cc = 0; cc < L; ++cc) *lds32(sb + (cc * 3 * P + t) * 4) = h2_bits(ch.lo);  // S' = 0 + lo
+    for (int cc = 0; cc < L; ++cc) *lds32(sb + (cc * 3 * P + t0 + lane_now()) * 4) = h2_bits(ch.lo);  // S' = lo
     lds_barrier();
 
     // parity of the estimate of buffer bf's sums over every row, vs the syndrome
@@ -720,7 +731,7 @@ __device__ __forceinline__ int qc_phase_lms(uint32_t sb, uint32_t (&aw)[J][L], u
                 const h2 W = as_h2(__builtin_amdgcn_alignbit(wz[cc], wx[cc], 16));
                 acc ^= h2_bits(lms_G(U, W, ch));
             }
-            vb |= (acc ^ synw[r]) & kSign;
+            vb |= (acc & kSign) ^ synw(r);
         }
         return vb;
     };
@@ -778,8 +789,8 @@ __device__ __forceinline__ int qc_phase_lms(uint32_t sb, uint32_t (&aw)[J][L], u
 #pragma unroll
                 for (int i = 0; i < 2 * L; ++i) wnext[i] = wn[i];
             }
-            const uint32_t bad = (acc ^ synw[r]) & kSign;
-            const uint32_t sall = (sg ^ synw[r]) & kSign;
+            const uint32_t bad = (acc & kSign) ^ synw(r);
+            const uint32_t sall = (sg & kSign) ^ synw(r);
             const int wb = (r + 1) % J;  // the buffer this layer writes
             u16x2 suf = as_u16x2(kMsClampPair);
 #pragma unroll
@@ -794,16 +805,16 @@ __device__ __forceinline__ int qc_phase_lms(uint32_t sb, uint32_t (&aw)[J][L], u
                 sth((aw[r][cc] & 0xffffu) + lms_off<P>(wb, cc), S.x);
                 sth((aw[r][cc] >> 16) + lms_off<P>(wb, cc) + 2, S.y);
             }
-            if (__ballot(bad != 0) != 0 && lane0) s_flag[j & 3] = 1;
-            if (t == 0) s_flag[(j + 2) & 3] = 0;
+            if (__ballot(bad != 0) != 0 && lane_now() == 0) s_flag[j & 3] = 1;
+            if (t0 == 0 && lane_now() == 0) s_flag[(j + 2) & 3] = 0;
             lds_barrier();
             const int f = __builtin_amdgcn_readfirstlane(s_flag[j & 3]);
             hist = (hist << 1) | (f ? 1u : 0u);
             if ((hist & 7u) == 0) {
                 const uint32_t vb = verify(wb);
                 const int vi = 4 + (nv & 1);
-                if (t == 0) s_flag[4 + ((nv + 1) & 1)] = 0;
-                if (__ballot(vb != 0) != 0 && lane0) s_flag[vi] = 1;
+                if (t0 == 0 && lane_now() == 0) s_flag[4 + ((nv + 1) & 1)] = 0;
+                if (__ballot(vb != 0) != 0 && lane_now() == 0) s_flag[vi] = 1;
                 lds_barrier();
                 const int vf = __builtin_amdgcn_readfirstlane(s_flag[vi]);
                 ++nv;
@@ -811,9 +822,10 @@ __device__ __forceinline__ int qc_phase_lms(uint32_t sb, uint32_t (&aw)[J][L], u
                     // the estimate of the variables (cc, t) from the same sums
                     estx = 0;
                     estz = 0;
+                    const uint32_t own = sb + (wb * P + t0 + lane_now()) * 4;
 #pragma unroll
                     for (int cc = 0; cc < L; ++cc) {
-                        const h2 S = as_h2(*lds32(sb + (cc * 3 * P + wb * P + t) * 4));
+                        const h2 S = as_h2(*lds32(own + cc * 3 * P * 4));
                         const uint32_t g = h2_bits(lms_G(S, __builtin_shufflevector(S, S, 1, 0), ch));
                         estx |= ((g >> 15) & 1u) << cc;
                         estz |= (g >> 31) << cc;
@@ -884,10 +896,9 @@ __device__ __forceinline__ PriorF trial_prior(const NoiseTab &nt, int trial) {
 // cc*P + t; P % 64 == 0 makes each wavefront's 64 variables one aligned
 // 64-bit span.
 template <int P, int L>
-__device__ __forceinline__ void write_outcome(const DevGraph &g, const TrialPlanes &tp, int trial,
+__device__ __forceinline__ void write_outcome(const DevGraph &g, const TrialPlanes &tp, int trial, int t,
                                               uint32_t estx, uint32_t estz, int iterations,
                                               int matched) {
-    const int t = threadIdx.x;
     const int lane = t & 63;
     // 64-bit word (cc*P + t)/64 = cc*P/64 + t/64: one base per wavefront,
     // the column block's part in the store's immediate offset
@@ -913,9 +924,8 @@ __device__ __forceinline__ void write_outcome(const DevGraph &g, const TrialPlan
 
 // Syndrome bits of checks (r, t): bit r.
 template <int P, int J>
-__device__ __forceinline__ void trial_syndromes(const DevGraph &g, const TrialPlanes &tp, int trial,
+__device__ __forceinline__ void trial_syndromes(const DevGraph &g, const TrialPlanes &tp, int trial, int t,
                                                 uint32_t &synx, uint32_t &synz) {
-    const int t = threadIdx.x;
     synx = 0;
     synz = 0;
 #pragma unroll
@@ -966,7 +976,7 @@ __device__ __forceinline__ void qc_f32_body(DevGraph g, QcDev qc, TrialPlanes tp
         if (trial < 0) return;
         const PriorF pr = trial_prior(nt, trial);
         uint32_t synx, synz;  // bit r: syndrome of check (r, t)
-        trial_syndromes<P, J>(g, tp, trial, synx, synz);
+        trial_syndromes<P, J>(g, tp, trial, static_cast<int>(threadIdx.x), synx, synz);
         float lam[2][J][L];
         uint32_t estx = 0, estz = 0;  // bit cc: hard decision of variable (cc, t)
         int matched = 0, iterations = 0;
@@ -998,7 +1008,7 @@ __device__ __forceinline__ void qc_f32_body(DevGraph g, QcDev qc, TrialPlanes tp
                 continue;
             }
         }
-        write_outcome<P, L>(g, tp, trial, estx, estz, iterations, matched);
+        write_outcome<P, L>(g, tp, trial, static_cast<int>(threadIdx.x), estx, estz, iterations, matched);
         __syncthreads();
     }
 }
@@ -1055,7 +1065,7 @@ __global__ void __launch_bounds__(P) decode_qc_ms_kernel(DevGraph g, QcDev qc, T
         if (trial < 0) return;
         const PriorF pr = trial_prior(nt, trial);
         uint32_t synx, synz;
-        trial_syndromes<P, J>(g, tp, trial, synx, synz);
+        trial_syndromes<P, J>(g, tp, trial, static_cast<int>(threadIdx.x), synx, synz);
         float u0 = 0.0f;
         if (DAMPED) {
             // unsatisfied fraction of est_0 (all identity): the syndrome weight
@@ -1071,7 +1081,7 @@ __global__ void __launch_bounds__(P) decode_qc_ms_kernel(DevGraph g, QcDev qc, T
         const int k = qc_phase_ms<P, J, L, DAMPED>(reinterpret_cast<uint32_t *>(B), s_cnt, &s_d, s_w, aw,
                                                    synx, synz, pr, li.T, li, u0, estx, estz, matched);
         if (matched) {
-            write_outcome<P, L>(g, tp, trial, estx, estz, k, 1);
+            write_outcome<P, L>(g, tp, trial, static_cast<int>(threadIdx.x), estx, estz, k, 1);
         } else if (t == 0) {
             fb_list[atomicAdd(fb_ctr, 1)] = trial;
         }
@@ -1129,30 +1139,31 @@ __global__ void __launch_bounds__(P) __attribute__((amdgpu_num_vgpr(80))) decode
     extern __shared__ uint32_t Sbuf[];
     __shared__ int s_trial;
     __shared__ int s_flag[6];
-    const int t = threadIdx.x;
+    const int t0 = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) & ~63);  // wavefront base
     const uint32_t sb = static_cast<uint32_t>((uintptr_t)((lds_u32 *)Sbuf));  // LDS address of the sums
     uint32_t aw[J][L];
     lms_slot_addresses<P, J, L>(g.qc_tab, sb, aw);
+    const bool first = t0 == 0;
     const cu32 *rec = (const cu32 *)li.ms_rec;  // scalar loads
     for (;;) {
-        if (t == 0) {
+        if (first && lane_now() == 0) {
             const int k = atomicAdd(counter, 1);
             s_trial = k < count ? (tp.order ? tp.order[k] : k) : -1;
         }
         lds_barrier();
-        const int trial = s_trial;
-        if (t < 6) s_flag[t] = 0;
+        const int trial = __builtin_amdgcn_readfirstlane(s_trial);
+        if (first && lane_now() < 6) s_flag[lane_now()] = 0;
         lds_barrier();
         if (trial < 0) return;
         const PriorF pr = trial_prior(nt, trial);
         uint32_t synx, synz;
-        trial_syndromes<P, J>(g, tp, trial, synx, synz);
+        trial_syndromes<P, J>(g, tp, trial, t0 + static_cast<int>(lane_now()), synx, synz);
         uint32_t estx = 0, estz = 0;
         int matched = 0;
-        const int k = qc_phase_lms<P, J, L>(sb, aw, synx, synz, pr, li.T, rec, s_flag, estx, estz, matched);
+        const int k = qc_phase_lms<P, J, L>(sb, t0, aw, synx, synz, pr, li.T, rec, s_flag, estx, estz, matched);
         if (matched) {
-            write_outcome<P, L>(g, tp, trial, estx, estz, k, 1);
-        } else if (t == 0) {
+            write_outcome<P, L>(g, tp, trial, t0 + static_cast<int>(lane_now()), estx, estz, k, 1);
+        } else if (first && lane_now() == 0) {
             fb_list[atomicAdd(fb_ctr, 1)] = trial;
         }
         lds_barrier();
