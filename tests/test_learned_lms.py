@@ -189,3 +189,29 @@ def test_training_gain_over_hand_set_weights(main_decoder, shipped):
     assert out["trained"] <= 1.002 * out["schedule"]
     const.destroy()
     sched.destroy()
+
+
+@pytest.mark.parametrize("p", [0.0, 0.01, 0.05, 0.07])
+def test_layered_stops_on_estimates_that_match_the_syndrome(main_decoder, main_code_oracle, shipped, p):
+    # The layered phase stops a trial only after a verification pass has
+    # found every check accepting the estimate it returns: recompute the
+    # syndromes of those estimates with the oracle.  Trials it hands over
+    # finish as plain BP does; without noise every trial stops in its first
+    # iteration with the exact (identity) estimate.
+    n = 512
+    print("layered decoder at p=%g" % p, flush=True)
+    sl, rl = main_decoder.simulate_learned(shipped, 106, n, p, max_iter=200, want_err=True, want_est=True)
+    sp, rp = main_decoder.simulate(106, n, p, max_iter=200, precision=jbp.FP32)
+    fb = rl["iterations"] > shipped.T
+    stopped = np.flatnonzero(~fb)
+    print("stopped in the layered phase: %d of %d" % (stopped.size, n), flush=True)
+    assert np.all(rl["syn_match"][stopped] == 1)
+    for t in stopped:
+        sx, sz = main_code_oracle.syndrome(rl["est"][t])
+        ex, ez = main_code_oracle.syndrome(rl["err"][t])
+        assert np.array_equal(sx, ex) and np.array_equal(sz, ez), t
+    assert np.array_equal(rl["iterations"][fb] - shipped.T, rp["iterations"][fb])
+    for k in ("success", "syn_match", "exact"):
+        assert np.array_equal(rl[k][fb], rp[k][fb]), k
+    if p == 0.0:
+        assert np.all(rl["iterations"] == 1) and np.all(rl["exact"] == 1)
