@@ -42,17 +42,35 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     return x ^ (x >> 31);
 }
 
+// The twist and the tempering on 32-bit halves: a ^ (b & K) is one
+// three-input bit operation (table 0x78 over (a, b, K)), a 64-bit shift an
+// align-bit of the halves.  Same bits as
+//   y = (cur & kUpper) | (next & kLower),
+//   far ^ (y >> 1) ^ ((y & 1) ? kMtA : 0)
+// and the standard tempering (shifts 29, 17, 37, 43).
+__device__ __forceinline__ uint32_t lo32(uint64_t x) { return static_cast<uint32_t>(x); }
+__device__ __forceinline__ uint32_t hi32(uint64_t x) { return static_cast<uint32_t>(x >> 32); }
+__device__ __forceinline__ uint32_t xor_and(uint32_t a, uint32_t b, uint32_t k) {
+    return __builtin_amdgcn_bitop3_b32(a, b, k, 0x78);  // a ^ (b & k)
+}
 __device__ __forceinline__ uint64_t twist_word(uint64_t cur, uint64_t next, uint64_t far) {
-    uint64_t y = (cur & kUpper) | (next & kLower);
-    return far ^ (y >> 1) ^ ((y & 1ull) ? kMtA : 0ull);
+    static_assert(kUpper == 0xffffffff80000000ull && kLower == 0x7fffffffull, "MT19937-64 masks");
+    const uint32_t ch = hi32(cur), yl = (lo32(cur) & 0x80000000u) | (lo32(next) & 0x7fffffffu);
+    const uint32_t m = 0u - (lo32(next) & 1u);  // all ones when y is odd
+    const uint32_t lo = xor_and(__builtin_amdgcn_alignbit(ch, yl, 1), m, lo32(kMtA)) ^ lo32(far);
+    const uint32_t hi = xor_and(ch >> 1, m, hi32(kMtA)) ^ hi32(far);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
 __device__ __forceinline__ uint64_t temper(uint64_t z) {
-    z ^= (z >> 29) & 0x5555555555555555ull;
-    z ^= (z << 17) & 0x71d67fffeda60000ull;
-    z ^= (z << 37) & 0xfff7eee000000000ull;
-    z ^= (z >> 43);
-    return z;
+    const uint32_t hi = hi32(z), lo = lo32(z);
+    const uint32_t lo1 = xor_and(lo, __builtin_amdgcn_alignbit(hi, lo, 29), 0x55555555u);  // >> 29
+    const uint32_t hi1 = xor_and(hi, hi >> 29, 0x55555555u);
+    const uint32_t lo2 = xor_and(lo1, lo1 << 17, 0xeda60000u);                            // << 17
+    const uint32_t hi2 = xor_and(hi1, __builtin_amdgcn_alignbit(hi1, lo1, 15), 0x71d67fffu);
+    const uint32_t hi3 = xor_and(hi2, lo2 << 5, 0xfff7eee0u);                             // << 37
+    const uint32_t lo4 = lo2 ^ (hi3 >> 11);                                                // >> 43
+    return (static_cast<uint64_t>(hi3) << 32) | lo4;
 }
 
 // Syndromes (jointbp_ets.cpp:1834-1849) of one trial from its error bit
