@@ -44,6 +44,17 @@ void launch_learned_train(const DevGraph &g, const LearnedDevParams &lp, const R
                           R *tape_u, const double *prior, R *loss, R *grad, int *conv,
                           int *counters, bool backward, hipStream_t st);
 
+// FP32 training of the flooding min-sum kind on the permutation-block code
+// with P = 768, J = 3, L = 12 (jbp_learned_qc.hip): the same loss, gradient
+// and convergence iterations as launch_learned_train<float>, with the
+// messages on chip and the tape as [trial][T+1][72 edges][768] floats in
+// act (the same size).
+bool learned_qc_train_supported(const DevGraph &g, const LearnedDevParams &lp);
+void launch_learned_train_qc(const DevGraph &g, const LearnedDevParams &lp, const float *params,
+                             TrialPlanes tp, int batch, float *act, float *tape_d, float *tape_u,
+                             const double *prior, float *loss, float *grad, int *conv, int *counters,
+                             bool backward, hipStream_t st);
+
 void launch_adam(double *p, const double *grad, double *m, double *v, float *pf, int count,
                  double lr, int step, hipStream_t st);
 void launch_to_double(const float *src, double *dst, int count, hipStream_t st);

@@ -122,7 +122,17 @@ int run_train(jbp_learned *m, int batch, bool backward, char *err, size_t errlen
     const R *params = params_of(m, static_cast<R *>(nullptr));
     HIPCHK(hipMemsetAsync(m->d_loss, 0, sizeof(R), ctx->stream));
     HIPCHK(hipMemsetAsync(m->d_grad, 0, m->count * sizeof(R), ctx->stream));
-    launch_learned_train<R>(ctx->dg, dev_params(m), params, ctx->tp, batch, static_cast<R *>(m->d_act),
+    const LearnedDevParams lp = dev_params(m);
+    if (sizeof(R) == sizeof(float) && learned_qc_train_supported(ctx->dg, lp)) {
+        launch_learned_train_qc(ctx->dg, lp, reinterpret_cast<const float *>(params), ctx->tp, batch,
+                                static_cast<float *>(m->d_act), static_cast<float *>(m->d_tape_d),
+                                static_cast<float *>(m->d_tape_u), m->d_prior, static_cast<float *>(m->d_loss),
+                                static_cast<float *>(m->d_grad), m->d_conv, m->d_counters, backward,
+                                ctx->stream);
+        HIPCHK(hipGetLastError());
+        return JBP_OK;
+    }
+    launch_learned_train<R>(ctx->dg, lp, params, ctx->tp, batch, static_cast<R *>(m->d_act),
                             static_cast<R *>(m->d_scratch), m->slots, static_cast<R *>(m->d_tape_d),
                             static_cast<R *>(m->d_tape_u), m->d_prior, static_cast<R *>(m->d_loss),
                             static_cast<R *>(m->d_grad), m->d_conv, m->d_counters, backward,

@@ -1,0 +1,828 @@
+// Learned flooding min-sum training (kind 1, FP32) for the permutation-block
+// code of the params files (P = 768, J = 3, L = 12): forward pass with its
+// tape and adjoint pass, on chip.  No counterpart in the reference program.
+//
+// The model is the generic training kernels' (jbp_learned.hip, KIND 1):
+//   mu_e     = wv[g][b][t] * (S_g(v) - lambda_e + F_g(S_other(v)))
+//   lambda_e = (1 - d_t) * wc[g][b][t] * CN_e(mu) + d_t * lambda_e(t-1)
+// with the min-sum check node clamped at 64 ln 2, F(L) = max(lI + L, lZ) -
+// max(lX + L, lY) in natural logs, d_t from the 2-16-1 MLP of (t / T,
+// unsatisfied fraction of the estimate two iterations back), and the BCE of
+// the X / Z marginals averaged over iterations 1..T.
+//
+// Layout: one 768-thread workgroup per trial; thread t owns the variables
+// (cc, t) and the checks (r, t).  The variable owner keeps the 72 messages
+// lambda of its variables' edges in registers (forward) or their gradients
+// (adjoint).  The two views meet in one LDS exchange buffer, variable-major:
+// the edge of row r and column block cc at variable (cc, y) sits in
+// B[r][cc][y], so the variable owner's slots are fixed offsets from one
+// base, and the check owner (r, x) forms the address of its input i from the
+// block's permutation, y = (a x + b) mod P, when it needs it (four
+// instructions per input, in place of 36 registers of addresses).  The
+// buffer serves one side at a time.  The tape is [trial][k = 0..T][72
+// edges][768 threads] floats, written as whole wavefront rows.  Per
+// iteration and side the forward exchanges mu (check owners read it and
+// write the clamped outputs back); the adjoint exchanges mu (the check
+// recomputed; its raw outputs back), the gradient of the raw outputs, and
+// the gradient of mu.  The estimate bit of a variable rides in the least
+// significant bit of every mu it sends, so the check owners count
+// unsatisfied checks while reading their rows (the forward and the adjoint
+// form mu bit for bit alike, so they take the same minima).  Weight
+// gradients are summed over the wavefront (DPP) and accumulated per block
+// and iteration in LDS, then added to the global gradient once per
+// workgroup.
+#include "jbp_common.h"
+#include "jbp_learned.h"
+
+namespace jbp {
+namespace {
+
+constexpr int QP = 768, QJ = 3, QL = 12, QNB = QJ * QL, QHB = QNB / 2, QK = 2 * QNB;
+constexpr float kLmaxQ = 44.3614195558365f;  // 64 ln 2
+
+typedef __attribute__((address_space(3))) float lds_f32;
+
+__device__ __forceinline__ float ldsf(uint32_t a) {
+    return *reinterpret_cast<lds_f32 *>(static_cast<uintptr_t>(a));
+}
+__device__ __forceinline__ void stf(uint32_t a, float v) {
+    *reinterpret_cast<lds_f32 *>(static_cast<uintptr_t>(a)) = v;
+}
+__device__ __forceinline__ float with_lsb(float v, uint32_t bit) {
+    return __uint_as_float((__float_as_uint(v) & ~1u) | bit);
+}
+// F(L) = max(lk0 + L, lk1) - max(lk2 + L, lk3) and its derivative
+__device__ __forceinline__ float fterm(float L, const float *lk) {
+    const float u = lk[0] + L, v = lk[2] + L;
+    return (u > lk[1] ? u : lk[1]) - (v > lk[3] ? v : lk[3]);
+}
+__device__ __forceinline__ float fterm_d(float L, const float *lk) {
+    return (lk[0] + L > lk[1] ? 1.0f : 0.0f) - (lk[2] + L > lk[3] ? 1.0f : 0.0f);
+}
+__device__ __forceinline__ float sigmf(float x) {
+    return x >= 0.0f ? 1.0f / (1.0f + expf(-x)) : expf(x) / (1.0f + expf(x));
+}
+__device__ __forceinline__ float softplusf(float x) {
+    return x > 0.0f ? x + logf(1.0f + expf(-x)) : logf(1.0f + expf(x));
+}
+
+// Sum over the wavefront, in every lane (row sums by DPP, then the four
+// rows' sums by readlane).
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float wave_sum(float v) {
+    v += dpp_f<0xb1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f<0x4e>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f<0x141>(v);  // row_half_mirror
+    v += dpp_f<0x140>(v);  // row_mirror: every lane holds its row's sum
+    const int b = __float_as_int(v);
+    return (__int_as_float(__builtin_amdgcn_readlane(b, 0)) + __int_as_float(__builtin_amdgcn_readlane(b, 16))) +
+           (__int_as_float(__builtin_amdgcn_readlane(b, 32)) + __int_as_float(__builtin_amdgcn_readlane(b, 48)));
+}
+
+// The variable owner's slots: B[r][cc][t] from a base opaque to the
+// optimiser (re-formed at each pass, so that per-slot addresses are not
+// kept live); blocks of the upper half from a second base, so every offset
+// fits the 16-bit immediate.
+struct VarSlots {
+    uint32_t lo, hi;
+};
+__device__ __forceinline__ VarSlots var_slots(uint32_t bbase) {
+    uint32_t a = bbase + static_cast<uint32_t>(threadIdx.x) * 4u;
+    asm volatile("" : "+v"(a));
+    return {a, a + static_cast<uint32_t>(QHB * QP * 4)};
+}
+__device__ __forceinline__ uint32_t vslot(const VarSlots &v, int r, int cc) {
+    const int blk = r * QL + cc;
+    return (blk < QHB ? v.lo : v.hi) + static_cast<uint32_t>((blk % QHB) * QP * 4);
+}
+// The check owner's input i of row r (side s): variable i*P + y with
+// y = (a x + b) mod P, a and b of block r*L + i from the code's table
+// ([side][a | b | ainv][r*L + cc]).  a x + b < 2^20, where
+// floor(y / 768) = mulhi(y, 5592406) exactly.
+__device__ __forceinline__ uint32_t check_slot(uint32_t bbase, const int *tab, int s, int r, int i, int x) {
+    constexpr int JL = QJ * QL;
+    const int blk = r * QL + i;
+    const uint32_t y = static_cast<uint32_t>(tab[(3 * s) * JL + blk] * x + tab[(3 * s + 1) * JL + blk]);
+    const uint32_t q = __umulhi(y, 5592406u);
+    return bbase + static_cast<uint32_t>(blk * QP * 4) + (y - q * static_cast<uint32_t>(QP)) * 4u;
+}
+static_assert(QP == 768, "check_slot's division constant is for P = 768");
+
+struct QcTrial {
+    float lx[4], lz[4];     // log prior terms: X side (I, Z, X, Y), Z side (I, X, Z, Y)
+    uint32_t ebx, ebz;      // error bits of the variables (cc, t): bit cc
+    uint32_t synx, synz;    // syndrome bits of the checks (r, t): bit r
+};
+__device__ __forceinline__ QcTrial qc_trial(const DevGraph &g, const TrialPlanes &tp, const double *prior, int b) {
+    QcTrial q;
+    const double *p4 = prior + 4 * static_cast<size_t>(b);
+    const int ox[4] = {0, 2, 1, 3};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        q.lx[i] = static_cast<float>(log(p4[ox[i]]));
+        q.lz[i] = static_cast<float>(log(p4[i]));
+    }
+    // an opaque thread index, so that the addresses below are formed here,
+    // once per trial, rather than hoisted out of the trial loop and spilled
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    q.ebx = q.ebz = q.synx = q.synz = 0;
+#pragma unroll
+    for (int cc = 0; cc < QL; ++cc) {
+        const int v = cc * QP + t;
+        const size_t w = static_cast<size_t>(b) * g.wn + (v >> 5);
+        q.ebx |= ((tp.err_x[w] >> (v & 31)) & 1u) << cc;
+        q.ebz |= ((tp.err_z[w] >> (v & 31)) & 1u) << cc;
+    }
+#pragma unroll
+    for (int r = 0; r < QJ; ++r) {
+        const int c = r * QP + t;
+        q.synx |= ((tp.syn_x[static_cast<size_t>(b) * g.wm[0] + (c >> 5)] >> (c & 31)) & 1u) << r;
+        q.synz |= ((tp.syn_z[static_cast<size_t>(b) * g.wm[1] + (c >> 5)] >> (c & 31)) & 1u) << r;
+    }
+    return q;
+}
+
+__device__ float mlp_fwd(const float *mlp, float x0, float x1, float *h) {
+    float z = mlp[4 * kMlpHidden];
+    for (int i = 0; i < kMlpHidden; ++i) {
+        h[i] = tanhf(mlp[2 * i] * x0 + mlp[2 * i + 1] * x1 + mlp[2 * kMlpHidden + i]);
+        z += mlp[3 * kMlpHidden + i] * h[i];
+    }
+    return sigmf(z);
+}
+
+// The tape of one trial through a buffer resource: element (edge e, thread
+// t) of row k sits at byte k * 72 * 768 * 4 + e * 768 * 4 + t * 4; the
+// thread's part is the one vector offset, the rest is uniform (scalar
+// offset), so no per-edge addresses are held in vector registers.
+typedef __amdgpu_buffer_rsrc_t tape_rsrc;
+__device__ __forceinline__ tape_rsrc make_tape(const float *p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), 0, 0x7fffffff, 0x00020000);
+}
+constexpr uint32_t kRowBytes = static_cast<uint32_t>(QK) * QP * 4;
+__device__ __forceinline__ uint32_t tape_off(int k, int s, int r, int cc) {
+    return static_cast<uint32_t>(k) * kRowBytes + static_cast<uint32_t>(((s * QJ + r) * QL + cc) * QP * 4);
+}
+// Row k's scalar offset, opaque to the optimiser: each pass forms its
+// edges' offsets from it (one scalar add each) instead of sharing 72
+// offsets across passes, which would be kept in scalar registers and
+// spilled.
+__device__ __forceinline__ uint32_t tape_row(int k) {
+    uint32_t o = static_cast<uint32_t>(k) * kRowBytes;
+    asm volatile("" : "+s"(o));
+    return o;
+}
+__device__ __forceinline__ uint32_t edge_off(int s, int r, int cc) {
+    return static_cast<uint32_t>(((s * QJ + r) * QL + cc) * QP * 4);
+}
+__device__ __forceinline__ float tape_ld(tape_rsrc rs, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+}
+__device__ __forceinline__ void tape_st(tape_rsrc rs, uint32_t voff, uint32_t soff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rs, voff, soff, 0);
+}
+
+// A scheduling fence between the tape reads of successive column blocks: it
+// bounds the loads in flight (and their registers) to a few blocks' worth.
+__device__ __forceinline__ void tape_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+// Pin one side's messages: the update that forms them is completed here,
+// not sunk past the next barrier with its LDS operands held in registers.
+__device__ __forceinline__ void pin_side(float (&m)[QJ][QL]) {
+#pragma unroll
+    for (int r = 0; r < QJ; ++r)
+#pragma unroll
+        for (int cc = 0; cc < QL; ++cc) asm volatile("" : "+v"(m[r][cc]));
+}
+
+// The weights of one iteration staged in LDS: wc X | wc Z | wv X | wv Z, 36
+// blocks each (the parameter vector holds [4][36][T]).
+constexpr int QW = 4 * QNB;
+__device__ __forceinline__ void stage_weights(float *dst, const float *params, int T, int it) {
+    const int i = threadIdx.x;
+    if (i < QW && it >= 1 && it <= T) dst[i] = params[static_cast<size_t>(i) * T + it - 1];
+}
+
+// Check rows of side s: min-sum on the 12 inputs of each check (r, t), in
+// place; returns the number of those checks whose inputs' estimate bits
+// (their least significant bits) disagree with the syndrome.
+__device__ __forceinline__ int qc_check_rows_fwd(uint32_t bbase, const int *tab, int s, uint32_t syn) {
+    int x = threadIdx.x;
+    asm volatile("" : "+v"(x));
+    int unsat = 0;
+#pragma unroll 1
+    for (int r = 0; r < QJ; ++r) {
+        uint32_t ad[QL];
+        float mu[QL];
+        uint32_t par = 0, neg = 0, negall = (syn >> r) & 1u;
+        float m1 = HUGE_VALF, m2 = HUGE_VALF;
+        int i1 = -1;
+#pragma unroll
+        for (int i = 0; i < QL; ++i) {
+            ad[i] = check_slot(bbase, tab, s, r, i, x);
+            mu[i] = ldsf(ad[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < QL; ++i) {
+            par ^= __float_as_uint(mu[i]) & 1u;
+            const float a = fabsf(mu[i]);
+            const uint32_t ng = mu[i] < 0.0f ? 1u : 0u;
+            neg |= ng << i;
+            negall ^= ng;
+            if (a < m1) {
+                m2 = m1;
+                m1 = a;
+                i1 = i;
+            } else if (a < m2) {
+                m2 = a;
+            }
+        }
+        unsat += (par != ((syn >> r) & 1u)) ? 1 : 0;
+#pragma unroll
+        for (int i = 0; i < QL; ++i) {
+            const float mag = i == i1 ? m2 : m1;
+            const float out = mag < kLmaxQ ? mag : kLmaxQ;
+            stf(ad[i], ((negall ^ (neg >> i)) & 1u) ? -out : out);
+        }
+    }
+    return unsat;
+}
+
+// mu of the variable (cc, t)'s edges on side s from its messages (the
+// forward's registers or the tape) and the staged weights w; the adjoint
+// forms it the same way, bit for bit.  The estimate bit of the side rides in
+// the least significant bit.
+__device__ __forceinline__ void qc_mu_side(int s, const float (&lx3)[QJ], const float (&lz3)[QJ], const QcTrial &q,
+                                           const float *w, int cc, float (&mu)[QJ]) {
+    const float sx = (lx3[0] + lx3[1]) + lx3[2];
+    const float sz = (lz3[0] + lz3[1]) + lz3[2];
+    const float fx = fterm(sz, q.lx), fz = fterm(sx, q.lz);
+    const uint32_t ex = (sx + fx < 0.0f) ? 1u : 0u, ez = (sz + fz < 0.0f) ? 1u : 0u;
+#pragma unroll
+    for (int r = 0; r < QJ; ++r) {
+        const float a = s == 0 ? (sx - lx3[r]) + fx : (sz - lz3[r]) + fz;
+        mu[r] = with_lsb(w[2 * QNB + s * QNB + r * QL + cc] * a, s == 0 ? ex : ez);
+    }
+}
+
+__global__ void __launch_bounds__(QP) learned_qc_fwd_kernel(DevGraph g, const float *params, int T, int use_mlp,
+                                                           TrialPlanes tp, float *act, float *tape_d, float *tape_u,
+                                                           const double *prior, float *loss, int *conv_out,
+                                                           int batch, int *counter) {
+    extern __shared__ float B[];  // [J][L][P] | GZ [L][P]
+    float *GZ = B + QNB * QP;     // the Z side's channel term of each variable, kept over the X update
+    __shared__ int s_trial;
+    __shared__ int s_cnt[2];
+    __shared__ float s_d, s_loss;
+    __shared__ float s_ubar[kMaxLearnedT + 1];
+    __shared__ float s_w[2][QW];
+    const int t = threadIdx.x, lane = t & 63;
+    const uint32_t bbase = static_cast<uint32_t>((uintptr_t)((lds_f32 *)B));
+    const float *mlp = params + 4 * QNB * T;
+    const size_t row = static_cast<size_t>(QK) * QP;
+    const float scale = 1.0f / (static_cast<float>(batch) * static_cast<float>(T) * static_cast<float>(2 * g.n));
+    const float mtot = static_cast<float>(2 * QJ * QP);
+    for (;;) {
+        if (t == 0) s_trial = atomicAdd(counter, 1);
+        __syncthreads();
+        const int b = __builtin_amdgcn_readfirstlane(s_trial);
+        __syncthreads();
+        if (b >= batch) return;
+        QcTrial q = qc_trial(g, tp, prior, b);
+        const tape_rsrc tape = make_tape(act + static_cast<size_t>(b) * (T + 1) * row);
+        const uint32_t voff = static_cast<uint32_t>(t) * 4u;
+        float lam[2][QJ][QL];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int r = 0; r < QJ; ++r)
+#pragma unroll
+                for (int cc = 0; cc < QL; ++cc) {
+                    lam[s][r][cc] = 0.0f;
+                    tape_st(tape, voff, tape_off(0, s, r, cc), 0.0f);
+                }
+        stage_weights(s_w[1], params, T, 1);
+        if (t == 0) {
+            s_cnt[0] = s_cnt[1] = 0;
+            s_loss = 0.0f;
+        }
+        __syncthreads();
+        {
+            const float w0 = wave_sum(static_cast<float>(__popc(q.synx) + __popc(q.synz)));
+            if (lane == 0) atomicAdd(&s_cnt[0], static_cast<int>(w0));
+        }
+        __syncthreads();
+        if (t == 0) s_ubar[0] = static_cast<float>(s_cnt[0]) / mtot;
+        int conv = 0;
+        float lsum = 0.0f;
+        for (int it = 1; it <= T; ++it) {
+            VarSlots vs = var_slots(bbase);
+            const float *w = s_w[it & 1];
+            stage_weights(s_w[(it + 1) & 1], params, T, it + 1);
+            // counter it & 1 counts the estimate of iteration it - 1; the
+            // other one was last read before this iteration's first barrier
+            if (t == 0) {
+                float h[kMlpHidden];
+                const float x0 = static_cast<float>(it) / static_cast<float>(T);
+                const float x1 = s_ubar[it >= 2 ? it - 2 : 0];
+                const float d = use_mlp ? mlp_fwd(mlp, x0, x1, h) : 0.0f;
+                s_d = d;
+                tape_d[static_cast<size_t>(b) * T + it - 1] = d;
+                tape_u[static_cast<size_t>(b) * T + it - 1] = x1;
+                s_cnt[it & 1] = 0;
+            }
+            // variable pass, X side (the Z side's channel terms and estimate
+            // bits kept for after the X update)
+            uint32_t ez_bits = 0;
+#pragma unroll
+            for (int cc = 0; cc < QL; ++cc) {
+                const float lx3[QJ] = {lam[0][0][cc], lam[0][1][cc], lam[0][2][cc]};
+                const float lz3[QJ] = {lam[1][0][cc], lam[1][1][cc], lam[1][2][cc]};
+                float mu[QJ];
+                qc_mu_side(0, lx3, lz3, q, w, cc, mu);
+#pragma unroll
+                for (int r = 0; r < QJ; ++r) stf(vslot(vs, r, cc), mu[r]);
+                const float sx = (lx3[0] + lx3[1]) + lx3[2];
+                const float sz = (lz3[0] + lz3[1]) + lz3[2];
+                const float fz = fterm(sx, q.lz);
+                ez_bits |= ((sz + fz < 0.0f) ? 1u : 0u) << cc;
+                GZ[cc * QP + t] = fz;
+            }
+            asm volatile("" : "+v"(ez_bits));
+            __syncthreads();
+            const float d = s_d;
+            int unsat = qc_check_rows_fwd(bbase, g.qc_tab, 0, q.synx);
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < QJ; ++r)
+#pragma unroll
+                for (int cc = 0; cc < QL; ++cc) {
+                    const float out = ldsf(vslot(vs, r, cc));
+                    lam[0][r][cc] = (1.0f - d) * (w[r * QL + cc] * out) + d * lam[0][r][cc];
+                }
+            pin_side(lam[0]);
+            __syncthreads();
+            vs = var_slots(bbase);
+            // Z side, from the Z messages (unchanged) and the kept terms
+#pragma unroll
+            for (int cc = 0; cc < QL; ++cc) {
+                const float sz = (lam[1][0][cc] + lam[1][1][cc]) + lam[1][2][cc];
+                const uint32_t ez = (ez_bits >> cc) & 1u;
+                const float fz = GZ[cc * QP + t];
+#pragma unroll
+                for (int r = 0; r < QJ; ++r) {
+                    const float a = (sz - lam[1][r][cc]) + fz;
+                    stf(vslot(vs, r, cc), with_lsb(w[3 * QNB + r * QL + cc] * a, ez));
+                }
+            }
+            __syncthreads();
+            unsat += qc_check_rows_fwd(bbase, g.qc_tab, 1, q.synz);
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < QJ; ++r)
+#pragma unroll
+                for (int cc = 0; cc < QL; ++cc) {
+                    const float out = ldsf(vslot(vs, r, cc));
+                    lam[1][r][cc] = (1.0f - d) * (w[QNB + r * QL + cc] * out) + d * lam[1][r][cc];
+                }
+            pin_side(lam[1]);
+            {
+                const float wsum = wave_sum(static_cast<float>(unsat));
+                if (lane == 0 && wsum != 0.0f) atomicAdd(&s_cnt[it & 1], static_cast<int>(wsum));
+            }
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int r = 0; r < QJ; ++r)
+#pragma unroll
+                    for (int cc = 0; cc < QL; ++cc) tape_st(tape, voff, tape_off(it, s, r, cc), lam[s][r][cc]);
+            // loss on the marginals after iteration it
+            asm volatile("" : "+v"(q.ebx), "+v"(q.ebz));
+#pragma unroll
+            for (int cc = 0; cc < QL; ++cc) {
+                const float sx = (lam[0][0][cc] + lam[0][1][cc]) + lam[0][2][cc];
+                const float sz = (lam[1][0][cc] + lam[1][1][cc]) + lam[1][2][cc];
+                const float mx = sx + fterm(sz, q.lx), mz = sz + fterm(sx, q.lz);
+                const float kx = static_cast<float>(2 * static_cast<int>((q.ebx >> cc) & 1u) - 1);
+                const float kz = static_cast<float>(2 * static_cast<int>((q.ebz >> cc) & 1u) - 1);
+                lsum += softplusf(kx * mx) + softplusf(kz * mz);
+            }
+            __syncthreads();
+            const int cnt = s_cnt[it & 1];  // unsatisfied checks of the estimate of iteration it - 1
+            if (t == 0 && it >= 2) s_ubar[it - 1] = static_cast<float>(cnt) / mtot;
+            if (it >= 2 && conv == 0 && cnt == 0) conv = it - 1;
+        }
+        // the estimate after the last iteration: its bits through the exchange
+        // buffer, counted by the check owners
+        if (conv == 0) {
+            int unsat = 0;
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const VarSlots vs = var_slots(bbase);
+#pragma unroll
+                for (int cc = 0; cc < QL; ++cc) {
+                    const float sx = (lam[0][0][cc] + lam[0][1][cc]) + lam[0][2][cc];
+                    const float sz = (lam[1][0][cc] + lam[1][1][cc]) + lam[1][2][cc];
+                    const uint32_t e = s == 0 ? ((sx + fterm(sz, q.lx) < 0.0f) ? 1u : 0u)
+                                              : ((sz + fterm(sx, q.lz) < 0.0f) ? 1u : 0u);
+#pragma unroll
+                    for (int r = 0; r < QJ; ++r) stf(vslot(vs, r, cc), __uint_as_float(e));
+                }
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < QJ; ++r) {
+                    uint32_t par = 0;
+#pragma unroll
+                    for (int i = 0; i < QL; ++i)
+                        par ^= __float_as_uint(ldsf(check_slot(bbase, g.qc_tab, s, r, i, t))) & 1u;
+                    unsat += (par != (((s == 0 ? q.synx : q.synz) >> r) & 1u)) ? 1 : 0;
+                }
+                __syncthreads();
+            }
+            if (t == 0) s_cnt[(T + 1) & 1] = 0;
+            __syncthreads();
+            const float wsum = wave_sum(static_cast<float>(unsat));
+            if (lane == 0 && wsum != 0.0f) atomicAdd(&s_cnt[(T + 1) & 1], static_cast<int>(wsum));
+            __syncthreads();
+            if (s_cnt[(T + 1) & 1] == 0) conv = T;
+        }
+        {
+            const float ws = wave_sum(lsum);
+            if (lane == 0) atomicAdd(&s_loss, ws);
+        }
+        __syncthreads();
+        if (t == 0) {
+            atomicAdd(loss, s_loss * scale);
+            conv_out[b] = conv;
+        }
+        __syncthreads();
+    }
+}
+
+// One side of the adjoint of iteration it (steps 2 to 6 below), with that
+// side's gradients g = dL/dlambda_it in registers: on return g holds the
+// side's part of dL/dlambda_(it-1) without the sums' cross terms, acc[cc]
+// the side's share of its sum's gradient (sum over the column's 3 edges of
+// wv * dL/dmu), and gd has the side's share of dL/dd added.
+template <int S>
+__device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&acc)[QL], float &gd, uint32_t bbase,
+                                            const int *tab, const QcTrial &q, tape_rsrc tape, uint32_t voff, int it,
+                                            float d, const float *w, float *gw_it, int lane) {
+    uint32_t prow = tape_row(it - 1);
+    auto ld = [&](int s, int r, int cc) { return tape_ld(tape, voff, prow + edge_off(s, r, cc)); };
+    VarSlots vs = var_slots(bbase);
+    // (2) mu of the side from the messages before the iteration
+#pragma unroll
+    for (int cc = 0; cc < QL; ++cc) {
+        const float lx3[QJ] = {ld(0, 0, cc), ld(0, 1, cc), ld(0, 2, cc)};
+        const float lz3[QJ] = {ld(1, 0, cc), ld(1, 1, cc), ld(1, 2, cc)};
+        float mu[QJ];
+        qc_mu_side(S, lx3, lz3, q, w, cc, mu);
+#pragma unroll
+        for (int r = 0; r < QJ; ++r) stf(vslot(vs, r, cc), mu[r]);
+        tape_fence();
+    }
+    __syncthreads();
+    // (3) the check update again, raw outputs in place; per check the first
+    // index of the smallest |mu| (i1), the first of the rest (i2), and which
+    // inputs and raw outputs are negative
+    const uint32_t syn = S == 0 ? q.synx : q.synz;
+    uint32_t info0 = 0, info1 = 0, info2 = 0;
+    int x = threadIdx.x;
+    asm volatile("" : "+v"(x));
+#pragma unroll 1
+    for (int r = 0; r < QJ; ++r) {
+        uint32_t ad[QL];
+        float mu[QL];
+        uint32_t neg = 0, negall = (syn >> r) & 1u;
+        float m1 = HUGE_VALF, m2 = HUGE_VALF;
+        int i1 = -1;
+#pragma unroll
+        for (int i = 0; i < QL; ++i) {
+            ad[i] = check_slot(bbase, tab, S, r, i, x);
+            mu[i] = ldsf(ad[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < QL; ++i) {
+            const float a = fabsf(mu[i]);
+            const uint32_t ng = mu[i] < 0.0f ? 1u : 0u;
+            neg |= ng << i;
+            negall ^= ng;
+            if (a < m1) {
+                m2 = m1;
+                m1 = a;
+                i1 = i;
+            } else if (a < m2) {
+                m2 = a;
+            }
+        }
+        // the first index of the smallest |mu| but i1's
+        int i2 = i1 == 0 ? 1 : 0;
+        float a2 = fabsf(i1 == 0 ? mu[1] : mu[0]);
+#pragma unroll
+        for (int i = 0; i < QL; ++i) {
+            const float a = fabsf(mu[i]);
+            if (i != i1 && a < a2) {
+                i2 = i;
+                a2 = a;
+            }
+        }
+        uint32_t rawneg = 0;
+#pragma unroll
+        for (int i = 0; i < QL; ++i) {
+            const float mag = i == i1 ? m2 : m1;
+            const float raw = ((negall ^ (neg >> i)) & 1u) ? -mag : mag;
+            rawneg |= (raw < 0.0f ? 1u : 0u) << i;
+            stf(ad[i], raw);
+        }
+        const uint32_t info =
+            static_cast<uint32_t>(i1) | (static_cast<uint32_t>(i2) << 4) | (neg << 8) | (rawneg << 20);
+        info0 = r == 0 ? info : info0;
+        info1 = r == 1 ? info : info1;
+        info2 = r == 2 ? info : info2;
+    }
+    __syncthreads();
+    vs = var_slots(bbase);
+    prow = tape_row(it - 1);
+    // (4) damping and check weights, per edge of the variable owner
+#pragma unroll
+    for (int r = 0; r < QJ; ++r)
+#pragma unroll
+        for (int cc = 0; cc < QL; ++cc) {
+            const float raw = ldsf(vslot(vs, r, cc));
+            const float mag = fabsf(raw);
+            const float cl = mag < kLmaxQ ? mag : kLmaxQ;
+            const float out = raw < 0.0f ? -cl : cl;
+            const float wc = w[S * QNB + r * QL + cc];
+            const float ge = g[r][cc];
+            gd += ge * (ld(S, r, cc) - wc * out);
+            // formed here: the branch below splits the pass into blocks, and
+            // sinking this sum to its end would hold every edge's terms
+            asm volatile("" : "+v"(gd));
+            const float gtil = (1.0f - d) * ge;
+            const float cw = wave_sum(gtil * out);
+            if (lane == 0 && cw != 0.0f) atomicAdd(&gw_it[S * QNB + r * QL + cc], cw);
+            g[r][cc] = d * ge;
+            asm volatile("" : "+v"(g[r][cc]));
+            stf(vslot(vs, r, cc), mag > kLmaxQ ? 0.0f : wc * gtil);
+            if ((cc & 3) == 3) tape_fence();
+        }
+    pin_side(g);
+    asm volatile("" : "+v"(gd));  // summed here, not sunk past the next passes with its terms held
+    __syncthreads();
+    // (5) check adjoint: the gradient of each output goes to the input
+    // attaining the minimum over the others
+#pragma unroll 1
+    for (int r = 0; r < QJ; ++r) {
+        const uint32_t inf = r == 0 ? info0 : (r == 1 ? info1 : info2);
+        uint32_t ad[QL];
+#pragma unroll
+        for (int i = 0; i < QL; ++i) ad[i] = check_slot(bbase, tab, S, r, i, x);
+        const int i1 = static_cast<int>(inf & 15u), i2 = static_cast<int>((inf >> 4) & 15u);
+        float to1 = 0.0f, to2 = 0.0f;
+#pragma unroll
+        for (int i = 0; i < QL; ++i) {
+            const float ge = ldsf(ad[i]);
+            const float se = ((inf >> (20 + i)) & 1u) ? -ge : ge;
+            if (i == i1) to2 += se;
+            else to1 += se;
+        }
+#pragma unroll
+        for (int i = 0; i < QL; ++i) {
+            float o = 0.0f;
+            if (i == i1) o = ((inf >> (8 + i)) & 1u) ? -to1 : to1;
+            else if (i == i2) o = ((inf >> (8 + i)) & 1u) ? -to2 : to2;
+            stf(ad[i], o);
+        }
+    }
+    __syncthreads();
+    vs = var_slots(bbase);
+    prow = tape_row(it - 1);
+    // (6) the variable update's adjoint: wv gradients and the own message's
+    // share; the sums' shares are the caller's
+#pragma unroll
+    for (int cc = 0; cc < QL; ++cc) {
+        const float lx3[QJ] = {ld(0, 0, cc), ld(0, 1, cc), ld(0, 2, cc)};
+        const float lz3[QJ] = {ld(1, 0, cc), ld(1, 1, cc), ld(1, 2, cc)};
+        const float sx = (lx3[0] + lx3[1]) + lx3[2];
+        const float sz = (lz3[0] + lz3[1]) + lz3[2];
+        const float f = S == 0 ? fterm(sz, q.lx) : fterm(sx, q.lz);
+        float sum = 0.0f;
+#pragma unroll
+        for (int r = 0; r < QJ; ++r) {
+            const float gmu = ldsf(vslot(vs, r, cc));
+            const float a = S == 0 ? (sx - lx3[r]) + f : (sz - lz3[r]) + f;
+            const float cw = wave_sum(gmu * a);
+            if (lane == 0 && cw != 0.0f) atomicAdd(&gw_it[2 * QNB + S * QNB + r * QL + cc], cw);
+            const float ga = w[2 * QNB + S * QNB + r * QL + cc] * gmu;
+            sum += ga;
+            g[r][cc] -= ga;
+            asm volatile("" : "+v"(g[r][cc]), "+v"(sum));
+        }
+        acc[cc] = sum;
+        tape_fence();
+    }
+    pin_side(g);
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(QP) learned_qc_bwd_kernel(DevGraph g, const float *params, int T, int use_mlp,
+                                                           TrialPlanes tp, const float *act, const float *tape_d,
+                                                           const float *tape_u, const double *prior, float *grad,
+                                                           int batch, int *counter) {
+    extern __shared__ float dyn[];  // B [J][L][P] | weight gradients [T][4][nb]
+    float *s_gw = dyn + QNB * QP;
+    __shared__ int s_trial;
+    __shared__ float s_gd;
+    __shared__ float s_gmlp[kMlpParams];
+    __shared__ float s_gz[kMaxLearnedT];
+    __shared__ float s_w[2][QW];
+    const int t = threadIdx.x, lane = t & 63;
+    const uint32_t bbase = static_cast<uint32_t>((uintptr_t)((lds_f32 *)dyn));
+    const float *mlp = params + 4 * QNB * T;
+    const size_t row = static_cast<size_t>(QK) * QP;
+    const float scale = 1.0f / (static_cast<float>(batch) * static_cast<float>(T) * static_cast<float>(2 * g.n));
+    for (int i = t; i < QW * T; i += QP) s_gw[i] = 0.0f;
+    for (int i = t; i < kMlpParams; i += QP) s_gmlp[i] = 0.0f;
+    if (t == 0) s_gd = 0.0f;
+    __syncthreads();
+    for (;;) {
+        if (t == 0) s_trial = atomicAdd(counter, 1);
+        __syncthreads();
+        const int b = __builtin_amdgcn_readfirstlane(s_trial);
+        __syncthreads();
+        if (b >= batch) break;
+        QcTrial q = qc_trial(g, tp, prior, b);
+        // The tape is walked from row T down; once row k's loss term is
+        // read (at iteration k) the row parks the side not in registers:
+        // dL/dlambda_(k-1) of the X side (without its pending cross term,
+        // held in cross) while the Z side is undone, then the Z side's for
+        // iteration k-1.
+        const tape_rsrc tape = make_tape(act + static_cast<size_t>(b) * (T + 1) * row);
+        const uint32_t voff = static_cast<uint32_t>(t) * 4u;
+        float cross[QL];  // the X side's pending share of the sums' gradients, per column block
+#pragma unroll
+        for (int cc = 0; cc < QL; ++cc) cross[cc] = 0.0f;
+        stage_weights(s_w[T & 1], params, T, T);
+        __syncthreads();
+        for (int it = T; it >= 1; --it) {
+            const float *w = s_w[it & 1];
+            float *gw_it = s_gw + (it - 1) * QW;  // this iteration's weight gradients, ordered as w
+            stage_weights(s_w[(it - 1) & 1], params, T, it - 1);
+            const float d = tape_d[static_cast<size_t>(b) * T + it - 1];
+            float gd = 0.0f;
+            float gv[QJ][QL];  // the resident side's dL/dlambda
+            // X side: parked gradient plus its cross term, plus the loss term
+            // of iteration it (whose Z part is held for the Z side)
+            // (at it = T nothing is parked: row T is read and discarded)
+            const bool parked = it < T;
+            uint32_t krow = tape_row(parked ? it + 1 : it);
+#pragma unroll
+            for (int r = 0; r < QJ; ++r)
+#pragma unroll
+                for (int cc = 0; cc < QL; ++cc) {
+                    const float v = tape_ld(tape, voff, krow + edge_off(0, r, cc));
+                    gv[r][cc] = (parked ? v : 0.0f) + cross[cc];
+                }
+            float gsz[QL];
+            asm volatile("" : "+v"(q.ebx), "+v"(q.ebz));
+            krow = tape_row(it);
+#pragma unroll
+            for (int cc = 0; cc < QL; ++cc) {
+                auto ldc = [&](int s, int r) { return tape_ld(tape, voff, krow + edge_off(s, r, cc)); };
+                const float sx = (ldc(0, 0) + ldc(0, 1)) + ldc(0, 2);
+                const float sz = (ldc(1, 0) + ldc(1, 1)) + ldc(1, 2);
+                const float mx = sx + fterm(sz, q.lx), mz = sz + fterm(sx, q.lz);
+                const float kx = static_cast<float>(2 * static_cast<int>((q.ebx >> cc) & 1u) - 1);
+                const float kz = static_cast<float>(2 * static_cast<int>((q.ebz >> cc) & 1u) - 1);
+                const float glx = scale * kx * sigmf(kx * mx), glz = scale * kz * sigmf(kz * mz);
+                const float gsx = glx + glz * fterm_d(sx, q.lz);
+                gsz[cc] = glz + glx * fterm_d(sz, q.lx);
+#pragma unroll
+                for (int r = 0; r < QJ; ++r) gv[r][cc] += gsx;
+                tape_fence();
+            }
+            pin_side(gv);
+            float accx[QL];
+            qc_bwd_side<0>(gv, accx, gd, bbase, g.qc_tab, q, tape, voff, it, d, w, gw_it, lane);
+            krow = tape_row(it);
+#pragma unroll
+            for (int r = 0; r < QJ; ++r)
+#pragma unroll
+                for (int cc = 0; cc < QL; ++cc) tape_st(tape, voff, krow + edge_off(0, r, cc), gv[r][cc]);
+            // Z side: parked gradient plus the loss term
+            krow = tape_row(parked ? it + 1 : it);
+#pragma unroll
+            for (int r = 0; r < QJ; ++r)
+#pragma unroll
+                for (int cc = 0; cc < QL; ++cc) {
+                    const float v = tape_ld(tape, voff, krow + edge_off(1, r, cc));
+                    gv[r][cc] = (parked ? v : 0.0f) + gsz[cc];
+                }
+            pin_side(gv);
+            float accz[QL];
+            qc_bwd_side<1>(gv, accz, gd, bbase, g.qc_tab, q, tape, voff, it, d, w, gw_it, lane);
+            // the sums' shares: S_x reaches the X messages directly and the
+            // Z side's channel term, S_z likewise
+            krow = tape_row(it - 1);
+#pragma unroll
+            for (int cc = 0; cc < QL; ++cc) {
+                auto ldp = [&](int s, int r) { return tape_ld(tape, voff, krow + edge_off(s, r, cc)); };
+                const float sx = (ldp(0, 0) + ldp(0, 1)) + ldp(0, 2);
+                const float sz = (ldp(1, 0) + ldp(1, 1)) + ldp(1, 2);
+                const float fdx = fterm_d(sz, q.lx);  // d a_x / d S_z
+                const float fdz = fterm_d(sx, q.lz);  // d a_z / d S_x
+                cross[cc] = accx[cc] + accz[cc] * fdz;
+#pragma unroll
+                for (int r = 0; r < QJ; ++r) gv[r][cc] += accz[cc] + accx[cc] * fdx;
+                tape_fence();
+            }
+            krow = tape_row(it);
+#pragma unroll
+            for (int r = 0; r < QJ; ++r)
+#pragma unroll
+                for (int cc = 0; cc < QL; ++cc) tape_st(tape, voff, krow + edge_off(1, r, cc), gv[r][cc]);
+            // (7) the damping's gradient, dL/dz of the MLP output, for the
+            // MLP's adjoint after the trial
+            if (use_mlp) {
+                const float ws = wave_sum(gd);
+                if (lane == 0) atomicAdd(&s_gd, ws);
+                __syncthreads();
+                if (t == 0) {
+                    s_gz[it - 1] = s_gd * d * (1.0f - d);
+                    s_gd = 0.0f;
+                }
+            }
+            __syncthreads();
+        }
+        // the MLP's adjoint: lane j of the first wavefront takes iteration
+        // j + 1 and the 65 parameter gradients are summed over the lanes
+        if (use_mlp && t < 64) {
+            const bool on = t < T;
+            const float x0 = static_cast<float>(t + 1) / static_cast<float>(T);
+            const float x1 = on ? tape_u[static_cast<size_t>(b) * T + t] : 0.0f;
+            const float gz = on ? s_gz[t] : 0.0f;
+            float h[kMlpHidden];
+            mlp_fwd(mlp, x0, x1, h);
+            const float v = wave_sum(gz);
+            if (t == 0) s_gmlp[4 * kMlpHidden] += v;
+            for (int i = 0; i < kMlpHidden; ++i) {
+                const float ga = gz * mlp[3 * kMlpHidden + i] * (1.0f - h[i] * h[i]);
+                const float g3 = wave_sum(gz * h[i]), g0 = wave_sum(ga * x0), g1 = wave_sum(ga * x1),
+                            g2 = wave_sum(ga);
+                if (t == 0) {
+                    s_gmlp[3 * kMlpHidden + i] += g3;
+                    s_gmlp[2 * i] += g0;
+                    s_gmlp[2 * i + 1] += g1;
+                    s_gmlp[2 * kMlpHidden + i] += g2;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // the gradient vector is [wc X | wc Z | wv X | wv Z][36][T]: entry
+    // kk*T + j is s_gw[j][kk]
+    float *gmlp = grad + 4 * QNB * T;
+    for (int i = t; i < QW * T; i += QP) {
+        const int kk = i / T, j = i - kk * T;
+        const float v = s_gw[j * QW + kk];
+        if (v != 0.0f) atomicAdd(&grad[i], v);
+    }
+    for (int i = t; i < kMlpParams; i += QP)
+        if (use_mlp && s_gmlp[i] != 0.0f) atomicAdd(&gmlp[i], s_gmlp[i]);
+}
+
+}  // namespace
+
+bool learned_qc_train_supported(const DevGraph &g, const LearnedDevParams &lp) {
+    return lp.kind == kLearnedMS && g.qc_tab && g.qc_P == QP && g.qc_J == QJ && g.qc_L == QL &&
+           lp.nb == QNB && lp.T >= 1 && lp.T <= kMaxLearnedT;
+}
+
+void launch_learned_train_qc(const DevGraph &g, const LearnedDevParams &lp, const float *params, TrialPlanes tp,
+                             int batch, float *act, float *tape_d, float *tape_u, const double *prior, float *loss,
+                             float *grad, int *conv, int *counters, bool backward, hipStream_t st) {
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int grid = std::max(1, std::min(batch, cus > 0 ? cus : 256));
+    const size_t fwd_lds = static_cast<size_t>(QNB + QL) * QP * sizeof(float);
+    const size_t bwd_lds = static_cast<size_t>(QNB) * QP * sizeof(float) +
+                           static_cast<size_t>(4) * QNB * lp.T * sizeof(float);
+    launch_fill_i32(counters, 0, 2, st);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(learned_qc_fwd_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(fwd_lds));
+    hipLaunchKernelGGL(learned_qc_fwd_kernel, dim3(grid), dim3(QP), fwd_lds, st, g, params, lp.T, lp.use_mlp, tp,
+                       act, tape_d, tape_u, prior, loss, conv, batch, counters);
+    if (!backward) return;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(learned_qc_bwd_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bwd_lds));
+    hipLaunchKernelGGL(learned_qc_bwd_kernel, dim3(grid), dim3(QP), bwd_lds, st, g, params, lp.T, lp.use_mlp, tp,
+                       act, tape_d, tape_u, prior, grad, batch, counters + 1);
+}
+
+}  // namespace jbp
