@@ -120,10 +120,11 @@ __device__ __forceinline__ QcTrial qc_trial(const DevGraph &g, const TrialPlanes
     QcTrial q;
     const double *p4 = prior + 4 * static_cast<size_t>(b);
     const int ox[4] = {0, 2, 1, 3};
+    // per-trial constants, uniform over the workgroup (scalar registers)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        q.lx[i] = static_cast<float>(log(p4[ox[i]]));
-        q.lz[i] = static_cast<float>(log(p4[i]));
+        q.lx[i] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(static_cast<float>(log(p4[ox[i]])))));
+        q.lz[i] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(static_cast<float>(log(p4[i])))));
     }
     // an opaque thread index, so that the addresses below are formed here,
     // once per trial, rather than hoisted out of the trial loop and spilled
@@ -197,6 +198,11 @@ __device__ __forceinline__ void pin_side(float (&m)[QJ][QL]) {
     for (int r = 0; r < QJ; ++r)
 #pragma unroll
         for (int cc = 0; cc < QL; ++cc) asm volatile("" : "+v"(m[r][cc]));
+}
+
+__device__ __forceinline__ void pin_row(float (&m)[QL]) {
+#pragma unroll
+    for (int cc = 0; cc < QL; ++cc) asm volatile("" : "+v"(m[cc]));
 }
 
 // The weights of one iteration staged in LDS: wc X | wc Z | wv X | wv Z, 36
@@ -469,22 +475,35 @@ __global__ void __launch_bounds__(QP) learned_qc_fwd_kernel(DevGraph g, const fl
 // the side's share of its sum's gradient (sum over the column's 3 edges of
 // wv * dL/dmu), and gd has the side's share of dL/dd added.
 template <int S>
-__device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&acc)[QL], float &gd, uint32_t bbase,
-                                            const int *tab, const QcTrial &q, tape_rsrc tape, uint32_t voff, int it,
-                                            float d, const float *w, float *gw_it, int lane) {
-    uint32_t prow = tape_row(it - 1);
-    auto ld = [&](int s, int r, int cc) { return tape_ld(tape, voff, prow + edge_off(s, r, cc)); };
+__device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&l)[QJ][QL], float (&acc)[QL], float &gd,
+                                            float (&sx)[QL], float (&sz)[QL], uint32_t bbase, const int *tab,
+                                            const QcTrial &q, tape_rsrc tape, uint32_t voff, int it,
+                                            const float *w, float *gw_it, float d, int lane) {
+    // l: the side's messages before the iteration (tape row it-1), loaded by
+    // the caller for (2); (4) and (6) load them again, issued at the end of
+    // the check pass before them so that they arrive during its barrier
+    auto ld = [&](int r, int cc) { return l[r][cc]; };
+    auto reload = [&]() {
+        const uint32_t prow = tape_row(it - 1);
+#pragma unroll
+        for (int r = 0; r < QJ; ++r)
+#pragma unroll
+            for (int cc = 0; cc < QL; ++cc) l[r][cc] = tape_ld(tape, voff, prow + edge_off(S, r, cc));
+    };
     VarSlots vs = var_slots(bbase);
-    // (2) mu of the side from the messages before the iteration
+    // (2) mu of the side from the messages before the iteration (the side's
+    // own messages from the tape, both sums from the caller)
 #pragma unroll
     for (int cc = 0; cc < QL; ++cc) {
-        const float lx3[QJ] = {ld(0, 0, cc), ld(0, 1, cc), ld(0, 2, cc)};
-        const float lz3[QJ] = {ld(1, 0, cc), ld(1, 1, cc), ld(1, 2, cc)};
-        float mu[QJ];
-        qc_mu_side(S, lx3, lz3, q, w, cc, mu);
+        const float l3[QJ] = {ld(0, cc), ld(1, cc), ld(2, cc)};
+        const float f = S == 0 ? fterm(sz[cc], q.lx) : fterm(sx[cc], q.lz);
+        const float e = S == 0 ? sx[cc] + f : sz[cc] + f;
+        const uint32_t est = e < 0.0f ? 1u : 0u;
 #pragma unroll
-        for (int r = 0; r < QJ; ++r) stf(vslot(vs, r, cc), mu[r]);
-        tape_fence();
+        for (int r = 0; r < QJ; ++r) {
+            const float a = ((S == 0 ? sx[cc] : sz[cc]) - l3[r]) + f;
+            stf(vslot(vs, r, cc), with_lsb(w[2 * QNB + S * QNB + r * QL + cc] * a, est));
+        }
     }
     __syncthreads();
     // (3) the check update again, raw outputs in place; per check the first
@@ -496,16 +515,12 @@ __device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&acc)[QL]
     asm volatile("" : "+v"(x));
 #pragma unroll 1
     for (int r = 0; r < QJ; ++r) {
-        uint32_t ad[QL];
         float mu[QL];
         uint32_t neg = 0, negall = (syn >> r) & 1u;
         float m1 = HUGE_VALF, m2 = HUGE_VALF;
         int i1 = -1;
 #pragma unroll
-        for (int i = 0; i < QL; ++i) {
-            ad[i] = check_slot(bbase, tab, S, r, i, x);
-            mu[i] = ldsf(ad[i]);
-        }
+        for (int i = 0; i < QL; ++i) mu[i] = ldsf(check_slot(bbase, tab, S, r, i, x));
 #pragma unroll
         for (int i = 0; i < QL; ++i) {
             const float a = fabsf(mu[i]);
@@ -537,7 +552,7 @@ __device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&acc)[QL]
             const float mag = i == i1 ? m2 : m1;
             const float raw = ((negall ^ (neg >> i)) & 1u) ? -mag : mag;
             rawneg |= (raw < 0.0f ? 1u : 0u) << i;
-            stf(ad[i], raw);
+            stf(check_slot(bbase, tab, S, r, i, x), raw);
         }
         const uint32_t info =
             static_cast<uint32_t>(i1) | (static_cast<uint32_t>(i2) << 4) | (neg << 8) | (rawneg << 20);
@@ -545,9 +560,13 @@ __device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&acc)[QL]
         info1 = r == 1 ? info : info1;
         info2 = r == 2 ? info : info2;
     }
+    reload();
     __syncthreads();
     vs = var_slots(bbase);
-    prow = tape_row(it - 1);
+    // the sums re-enter opaque, so that (6) re-forms what (2) formed
+    // instead of keeping (2)'s values live through (3) to (5)
+    pin_row(sx);
+    pin_row(sz);
     // (4) damping and check weights, per edge of the variable owner
 #pragma unroll
     for (int r = 0; r < QJ; ++r)
@@ -559,7 +578,7 @@ __device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&acc)[QL]
             const float out = raw < 0.0f ? -cl : cl;
             const float wc = w[S * QNB + r * QL + cc];
             const float ge = g[r][cc];
-            gd += ge * (ld(S, r, cc) - wc * out);
+            gd += ge * (ld(r, cc) - wc * out);
             // formed here: the branch below splits the pass into blocks, and
             // sinking this sum to its end would hold every edge's terms
             asm volatile("" : "+v"(gd));
@@ -569,7 +588,6 @@ __device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&acc)[QL]
             g[r][cc] = d * ge;
             asm volatile("" : "+v"(g[r][cc]));
             stf(vslot(vs, r, cc), mag > kLmaxQ ? 0.0f : wc * gtil);
-            if ((cc & 3) == 3) tape_fence();
         }
     pin_side(g);
     asm volatile("" : "+v"(gd));  // summed here, not sunk past the next passes with its terms held
@@ -599,23 +617,20 @@ __device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&acc)[QL]
             stf(ad[i], o);
         }
     }
+    reload();
     __syncthreads();
     vs = var_slots(bbase);
-    prow = tape_row(it - 1);
     // (6) the variable update's adjoint: wv gradients and the own message's
     // share; the sums' shares are the caller's
 #pragma unroll
     for (int cc = 0; cc < QL; ++cc) {
-        const float lx3[QJ] = {ld(0, 0, cc), ld(0, 1, cc), ld(0, 2, cc)};
-        const float lz3[QJ] = {ld(1, 0, cc), ld(1, 1, cc), ld(1, 2, cc)};
-        const float sx = (lx3[0] + lx3[1]) + lx3[2];
-        const float sz = (lz3[0] + lz3[1]) + lz3[2];
-        const float f = S == 0 ? fterm(sz, q.lx) : fterm(sx, q.lz);
+        const float l3[QJ] = {ld(0, cc), ld(1, cc), ld(2, cc)};
+        const float f = S == 0 ? fterm(sz[cc], q.lx) : fterm(sx[cc], q.lz);
         float sum = 0.0f;
 #pragma unroll
         for (int r = 0; r < QJ; ++r) {
             const float gmu = ldsf(vslot(vs, r, cc));
-            const float a = S == 0 ? (sx - lx3[r]) + f : (sz - lz3[r]) + f;
+            const float a = ((S == 0 ? sx[cc] : sz[cc]) - l3[r]) + f;
             const float cw = wave_sum(gmu * a);
             if (lane == 0 && cw != 0.0f) atomicAdd(&gw_it[2 * QNB + S * QNB + r * QL + cc], cw);
             const float ga = w[2 * QNB + S * QNB + r * QL + cc] * gmu;
@@ -624,7 +639,6 @@ __device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&acc)[QL]
             asm volatile("" : "+v"(g[r][cc]), "+v"(sum));
         }
         acc[cc] = sum;
-        tape_fence();
     }
     pin_side(g);
     __syncthreads();
@@ -667,6 +681,20 @@ __global__ void __launch_bounds__(QP) learned_qc_bwd_kernel(DevGraph g, const fl
         float cross[QL];  // the X side's pending share of the sums' gradients, per column block
 #pragma unroll
         for (int cc = 0; cc < QL; ++cc) cross[cc] = 0.0f;
+        // the sums S_x, S_z of each variable (cc, t) over the messages of tape
+        // row k: row T's here, then each iteration forms row it-1's, which
+        // the next iteration's loss term reads
+        float sxc[QL], szc[QL];
+        {
+            const uint32_t krow = tape_row(T);
+#pragma unroll
+            for (int cc = 0; cc < QL; ++cc) {
+                auto ldc = [&](int s, int r) { return tape_ld(tape, voff, krow + edge_off(s, r, cc)); };
+                sxc[cc] = (ldc(0, 0) + ldc(0, 1)) + ldc(0, 2);
+                szc[cc] = (ldc(1, 0) + ldc(1, 1)) + ldc(1, 2);
+                tape_fence();
+            }
+        }
         stage_weights(s_w[T & 1], params, T, T);
         __syncthreads();
         for (int it = T; it >= 1; --it) {
@@ -677,25 +705,34 @@ __global__ void __launch_bounds__(QP) learned_qc_bwd_kernel(DevGraph g, const fl
             float gd = 0.0f;
             float gv[QJ][QL];  // the resident side's dL/dlambda
             // X side: parked gradient plus its cross term, plus the loss term
-            // of iteration it (whose Z part is held for the Z side)
+            // of iteration it (whose Z part is held for the Z side); row it-1
+            // in the same round trip: the X messages (kept for the X side's
+            // passes) and the sums, for both sides and the next loss term
             // (at it = T nothing is parked: row T is read and discarded)
             const bool parked = it < T;
-            uint32_t krow = tape_row(parked ? it + 1 : it);
+            float lr[QJ][QL];
+            float zr[QJ][QL];
+            {
+                const uint32_t krow = tape_row(parked ? it + 1 : it);
+                const uint32_t prow = tape_row(it - 1);
+#pragma unroll
+                for (int r = 0; r < QJ; ++r)
+#pragma unroll
+                    for (int cc = 0; cc < QL; ++cc) {
+                        gv[r][cc] = tape_ld(tape, voff, krow + edge_off(0, r, cc));
+                        lr[r][cc] = tape_ld(tape, voff, prow + edge_off(0, r, cc));
+                        zr[r][cc] = tape_ld(tape, voff, prow + edge_off(1, r, cc));
+                    }
+            }
 #pragma unroll
             for (int r = 0; r < QJ; ++r)
 #pragma unroll
-                for (int cc = 0; cc < QL; ++cc) {
-                    const float v = tape_ld(tape, voff, krow + edge_off(0, r, cc));
-                    gv[r][cc] = (parked ? v : 0.0f) + cross[cc];
-                }
+                for (int cc = 0; cc < QL; ++cc) gv[r][cc] = (parked ? gv[r][cc] : 0.0f) + cross[cc];
             float gsz[QL];
             asm volatile("" : "+v"(q.ebx), "+v"(q.ebz));
-            krow = tape_row(it);
 #pragma unroll
             for (int cc = 0; cc < QL; ++cc) {
-                auto ldc = [&](int s, int r) { return tape_ld(tape, voff, krow + edge_off(s, r, cc)); };
-                const float sx = (ldc(0, 0) + ldc(0, 1)) + ldc(0, 2);
-                const float sz = (ldc(1, 0) + ldc(1, 1)) + ldc(1, 2);
+                const float sx = sxc[cc], sz = szc[cc];
                 const float mx = sx + fterm(sz, q.lx), mz = sz + fterm(sx, q.lz);
                 const float kx = static_cast<float>(2 * static_cast<int>((q.ebx >> cc) & 1u) - 1);
                 const float kz = static_cast<float>(2 * static_cast<int>((q.ebz >> cc) & 1u) - 1);
@@ -704,48 +741,70 @@ __global__ void __launch_bounds__(QP) learned_qc_bwd_kernel(DevGraph g, const fl
                 gsz[cc] = glz + glx * fterm_d(sz, q.lx);
 #pragma unroll
                 for (int r = 0; r < QJ; ++r) gv[r][cc] += gsx;
-                tape_fence();
             }
             pin_side(gv);
-            float accx[QL];
-            qc_bwd_side<0>(gv, accx, gd, bbase, g.qc_tab, q, tape, voff, it, d, w, gw_it, lane);
-            krow = tape_row(it);
-#pragma unroll
-            for (int r = 0; r < QJ; ++r)
-#pragma unroll
-                for (int cc = 0; cc < QL; ++cc) tape_st(tape, voff, krow + edge_off(0, r, cc), gv[r][cc]);
-            // Z side: parked gradient plus the loss term
-            krow = tape_row(parked ? it + 1 : it);
-#pragma unroll
-            for (int r = 0; r < QJ; ++r)
-#pragma unroll
-                for (int cc = 0; cc < QL; ++cc) {
-                    const float v = tape_ld(tape, voff, krow + edge_off(1, r, cc));
-                    gv[r][cc] = (parked ? v : 0.0f) + gsz[cc];
-                }
-            pin_side(gv);
-            float accz[QL];
-            qc_bwd_side<1>(gv, accz, gd, bbase, g.qc_tab, q, tape, voff, it, d, w, gw_it, lane);
-            // the sums' shares: S_x reaches the X messages directly and the
-            // Z side's channel term, S_z likewise
-            krow = tape_row(it - 1);
 #pragma unroll
             for (int cc = 0; cc < QL; ++cc) {
-                auto ldp = [&](int s, int r) { return tape_ld(tape, voff, krow + edge_off(s, r, cc)); };
-                const float sx = (ldp(0, 0) + ldp(0, 1)) + ldp(0, 2);
-                const float sz = (ldp(1, 0) + ldp(1, 1)) + ldp(1, 2);
-                const float fdx = fterm_d(sz, q.lx);  // d a_x / d S_z
-                const float fdz = fterm_d(sx, q.lz);  // d a_z / d S_x
+                sxc[cc] = (lr[0][cc] + lr[1][cc]) + lr[2][cc];
+                szc[cc] = (zr[0][cc] + zr[1][cc]) + zr[2][cc];
+            }
+            // the loss term's Z part waits in row it (free from here on: the
+            // loss term read its sums from registers)
+            {
+                const uint32_t krow = tape_row(it);
+#pragma unroll
+                for (int cc = 0; cc < QL; ++cc) tape_st(tape, voff, krow + edge_off(1, 0, cc), gsz[cc]);
+            }
+            float accx[QL];
+            qc_bwd_side<0>(gv, lr, accx, gd, sxc, szc, bbase, g.qc_tab, q, tape, voff, it, w, gw_it, d, lane);
+            {
+                const uint32_t krow = tape_row(it);
+#pragma unroll
+                for (int r = 0; r < QJ; ++r)
+#pragma unroll
+                    for (int cc = 0; cc < QL; ++cc) tape_st(tape, voff, krow + edge_off(0, r, cc), gv[r][cc]);
+            }
+            // Z side: parked gradient plus the loss term, and the Z messages
+            // of row it-1, in one round trip
+            {
+                float lz[QL];
+                const uint32_t jrow = tape_row(it);
+                const uint32_t krow = tape_row(parked ? it + 1 : it);
+                const uint32_t prow = tape_row(it - 1);
+#pragma unroll
+                for (int cc = 0; cc < QL; ++cc) lz[cc] = tape_ld(tape, voff, jrow + edge_off(1, 0, cc));
+#pragma unroll
+                for (int r = 0; r < QJ; ++r)
+#pragma unroll
+                    for (int cc = 0; cc < QL; ++cc) {
+                        gv[r][cc] = tape_ld(tape, voff, krow + edge_off(1, r, cc));
+                        lr[r][cc] = tape_ld(tape, voff, prow + edge_off(1, r, cc));
+                    }
+#pragma unroll
+                for (int r = 0; r < QJ; ++r)
+#pragma unroll
+                    for (int cc = 0; cc < QL; ++cc) gv[r][cc] = (parked ? gv[r][cc] : 0.0f) + lz[cc];
+            }
+            pin_side(gv);
+            float accz[QL];
+            qc_bwd_side<1>(gv, lr, accz, gd, sxc, szc, bbase, g.qc_tab, q, tape, voff, it, w, gw_it, d, lane);
+            // the sums' shares: S_x reaches the X messages directly and the
+            // Z side's channel term, S_z likewise
+#pragma unroll
+            for (int cc = 0; cc < QL; ++cc) {
+                const float fdx = fterm_d(szc[cc], q.lx);  // d a_x / d S_z
+                const float fdz = fterm_d(sxc[cc], q.lz);  // d a_z / d S_x
                 cross[cc] = accx[cc] + accz[cc] * fdz;
 #pragma unroll
                 for (int r = 0; r < QJ; ++r) gv[r][cc] += accz[cc] + accx[cc] * fdx;
-                tape_fence();
             }
-            krow = tape_row(it);
+            {
+                const uint32_t krow = tape_row(it);
 #pragma unroll
-            for (int r = 0; r < QJ; ++r)
+                for (int r = 0; r < QJ; ++r)
 #pragma unroll
-                for (int cc = 0; cc < QL; ++cc) tape_st(tape, voff, krow + edge_off(1, r, cc), gv[r][cc]);
+                    for (int cc = 0; cc < QL; ++cc) tape_st(tape, voff, krow + edge_off(1, r, cc), gv[r][cc]);
+            }
             // (7) the damping's gradient, dL/dz of the MLP output, for the
             // MLP's adjoint after the trial
             if (use_mlp) {
