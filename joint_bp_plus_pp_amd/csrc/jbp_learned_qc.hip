@@ -82,6 +82,54 @@ __device__ __forceinline__ float wave_sum(float v) {
            (__int_as_float(__builtin_amdgcn_readlane(b, 32)) + __int_as_float(__builtin_amdgcn_readlane(b, 48)));
 }
 
+// Sums over the wavefront of 12 per-lane values at once (a transposing
+// reduction): each step halves the values a lane holds, the lanes on one
+// side of a lane-index bit keeping the first half summed with their
+// partners', the others the second half.  Bits 5 and 4 use the lane-swap
+// instructions (permlane32 / permlane16 swap: the two results' sum holds the
+// first operand's pair sums in the lower lanes and the second's in the
+// upper), bits 3 and 2 two masked row shifts, bits 1 and 0 a butterfly.  On
+// return a lane holds the sum of value slot12(lane) (when that is >= 0; the
+// four lanes of a quad hold the same sum).
+__device__ __forceinline__ float swap_sum32(float a, float b) {
+    const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
+__device__ __forceinline__ float swap_sum16(float a, float b) {
+    const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
+// lanes with the bit clear keep a plus the partner's a (row_shl), the others
+// b plus the partner's b (row_shr); LO / HI are the banks of each side
+template <int SHL, int SHR, int LO, int HI>
+__device__ __forceinline__ float shift_sum(float a, float b, bool hi) {
+    float t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a), SHL, 0xf, LO, false));
+    t = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(t), __float_as_int(b), SHR, 0xf, HI, false));
+    return (hi ? b : a) + t;
+}
+__device__ __forceinline__ float reduce12(const float (&v)[12], uint32_t lane) {
+    float w[6], x[3];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) w[k] = swap_sum32(v[k], v[k + 6]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) x[k] = swap_sum16(w[k], w[k + 3]);
+    const bool b3 = (lane >> 3) & 1u, b2 = (lane >> 2) & 1u;
+    // bit 3: pairs (0, 2) and (1, padding)
+    const float y0 = shift_sum<0x108, 0x118, 0x3, 0xc>(x[0], x[2], b3);
+    const float y1 = shift_sum<0x108, 0x118, 0x3, 0xc>(x[1], 0.0f, b3);
+    // bit 2: the pair (y0, y1)
+    float z = shift_sum<0x104, 0x114, 0x5, 0xa>(y0, y1, b2);
+    z += dpp_f<0xb1>(z);  // quad_perm [1,0,3,2]
+    z += dpp_f<0x4e>(z);  // quad_perm [2,3,0,1]
+    return z;
+}
+// The value slot whose sum reduce12 leaves in this lane, or -1 (padding).
+__device__ __forceinline__ int slot12(uint32_t lane) {
+    const int b2 = (lane >> 2) & 1, b3 = (lane >> 3) & 1, b4 = (lane >> 4) & 1, b5 = (lane >> 5) & 1;
+    const int s3 = b2 + 2 * b3;  // slot among x[0..2] (3 = padding)
+    return s3 < 3 ? s3 + 3 * b4 + 6 * b5 : -1;
+}
+
 // The variable owner's slots: B[r][cc][t] from a base opaque to the
 // optimiser (re-formed at each pass, so that per-slot addresses are not
 // kept live); blocks of the upper half from a second base, so every offset
@@ -567,9 +615,12 @@ __device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&l)[QJ][Q
     // instead of keeping (2)'s values live through (3) to (5)
     pin_row(sx);
     pin_row(sz);
-    // (4) damping and check weights, per edge of the variable owner
+    // (4) damping and check weights, per edge of the variable owner; the wc
+    // gradients of a block row are summed over the wavefront together
+    const int wslot = (lane & 3) == 0 ? slot12(static_cast<uint32_t>(lane)) : -1;
 #pragma unroll
-    for (int r = 0; r < QJ; ++r)
+    for (int r = 0; r < QJ; ++r) {
+        float pw[QL];
 #pragma unroll
         for (int cc = 0; cc < QL; ++cc) {
             const float raw = ldsf(vslot(vs, r, cc));
@@ -579,17 +630,19 @@ __device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&l)[QJ][Q
             const float wc = w[S * QNB + r * QL + cc];
             const float ge = g[r][cc];
             gd += ge * (ld(r, cc) - wc * out);
-            // formed here: the branch below splits the pass into blocks, and
-            // sinking this sum to its end would hold every edge's terms
+            // formed here, not sunk to the pass's end with every edge's terms
             asm volatile("" : "+v"(gd));
             const float gtil = (1.0f - d) * ge;
-            const float cw = wave_sum(gtil * out);
-            if (lane == 0 && cw != 0.0f) atomicAdd(&gw_it[S * QNB + r * QL + cc], cw);
+            pw[cc] = gtil * out;
             g[r][cc] = d * ge;
             asm volatile("" : "+v"(g[r][cc]));
             stf(vslot(vs, r, cc), mag > kLmaxQ ? 0.0f : wc * gtil);
         }
+        const float sum = reduce12(pw, static_cast<uint32_t>(lane));
+        if (wslot >= 0) atomicAdd(&gw_it[S * QNB + r * QL + wslot], sum);
+    }
     pin_side(g);
+    asm volatile("" : "+v"(gd));
     asm volatile("" : "+v"(gd));  // summed here, not sunk past the next passes with its terms held
     __syncthreads();
     // (5) check adjoint: the gradient of each output goes to the input
@@ -620,27 +673,30 @@ __device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&l)[QJ][Q
     reload();
     __syncthreads();
     vs = var_slots(bbase);
-    // (6) the variable update's adjoint: wv gradients and the own message's
-    // share; the sums' shares are the caller's
+    // (6) the variable update's adjoint: wv gradients (a block row's summed
+    // over the wavefront together) and the own message's share; the sums'
+    // shares are the caller's
 #pragma unroll
-    for (int cc = 0; cc < QL; ++cc) {
-        const float l3[QJ] = {ld(0, cc), ld(1, cc), ld(2, cc)};
-        const float f = S == 0 ? fterm(sz[cc], q.lx) : fterm(sx[cc], q.lz);
-        float sum = 0.0f;
+    for (int cc = 0; cc < QL; ++cc) acc[cc] = 0.0f;
 #pragma unroll
-        for (int r = 0; r < QJ; ++r) {
+    for (int r = 0; r < QJ; ++r) {
+        float pv[QL];
+#pragma unroll
+        for (int cc = 0; cc < QL; ++cc) {
+            const float f = S == 0 ? fterm(sz[cc], q.lx) : fterm(sx[cc], q.lz);
             const float gmu = ldsf(vslot(vs, r, cc));
-            const float a = ((S == 0 ? sx[cc] : sz[cc]) - l3[r]) + f;
-            const float cw = wave_sum(gmu * a);
-            if (lane == 0 && cw != 0.0f) atomicAdd(&gw_it[2 * QNB + S * QNB + r * QL + cc], cw);
+            const float a = ((S == 0 ? sx[cc] : sz[cc]) - ld(r, cc)) + f;
+            pv[cc] = gmu * a;
             const float ga = w[2 * QNB + S * QNB + r * QL + cc] * gmu;
-            sum += ga;
+            acc[cc] += ga;
             g[r][cc] -= ga;
-            asm volatile("" : "+v"(g[r][cc]), "+v"(sum));
+            asm volatile("" : "+v"(g[r][cc]), "+v"(acc[cc]));
         }
-        acc[cc] = sum;
+        const float sum = reduce12(pv, static_cast<uint32_t>(lane));
+        if (wslot >= 0) atomicAdd(&gw_it[2 * QNB + S * QNB + r * QL + wslot], sum);
     }
     pin_side(g);
+    pin_row(acc);
     __syncthreads();
 }
 
