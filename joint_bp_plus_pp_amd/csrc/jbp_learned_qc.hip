@@ -59,11 +59,19 @@ __device__ __forceinline__ float fterm(float L, const float *lk) {
 __device__ __forceinline__ float fterm_d(float L, const float *lk) {
     return (lk[0] + L > lk[1] ? 1.0f : 0.0f) - (lk[2] + L > lk[3] ? 1.0f : 0.0f);
 }
+// The logistic function and softplus from one base-2 exponential of -|x|
+// (hardware exp2 / log2 / reciprocal, within about an ulp), without
+// branches: sigma(x) = 1 / (1 + e) for x >= 0 and e / (1 + e) below, with
+// e = exp(-|x|); softplus(x) = max(x, 0) + ln(1 + e).
+constexpr float kLog2e = 1.44269504088896341f, kLn2 = 0.693147180559945309f;
 __device__ __forceinline__ float sigmf(float x) {
-    return x >= 0.0f ? 1.0f / (1.0f + expf(-x)) : expf(x) / (1.0f + expf(x));
+    const float e = __builtin_amdgcn_exp2f(-fabsf(x) * kLog2e);
+    const float s = __builtin_amdgcn_rcpf(1.0f + e);
+    return x >= 0.0f ? s : e * s;
 }
 __device__ __forceinline__ float softplusf(float x) {
-    return x > 0.0f ? x + logf(1.0f + expf(-x)) : logf(1.0f + expf(x));
+    const float e = __builtin_amdgcn_exp2f(-fabsf(x) * kLog2e);
+    return fmaxf(x, 0.0f) + __builtin_amdgcn_logf(1.0f + e) * kLn2;
 }
 
 // Sum over the wavefront, in every lane (row sums by DPP, then the four
@@ -287,13 +295,12 @@ __device__ __forceinline__ int qc_check_rows_fwd(uint32_t bbase, const int *tab,
             const uint32_t ng = mu[i] < 0.0f ? 1u : 0u;
             neg |= ng << i;
             negall ^= ng;
-            if (a < m1) {
-                m2 = m1;
-                m1 = a;
-                i1 = i;
-            } else if (a < m2) {
-                m2 = a;
-            }
+            // the two smallest magnitudes and the first index of the
+            // smallest, without branches
+            const bool lt1 = a < m1;
+            m2 = lt1 ? m1 : (a < m2 ? a : m2);
+            i1 = lt1 ? i : i1;
+            m1 = lt1 ? a : m1;
         }
         unsat += (par != ((syn >> r) & 1u)) ? 1 : 0;
 #pragma unroll
@@ -465,6 +472,7 @@ __global__ void __launch_bounds__(QP) learned_qc_fwd_kernel(DevGraph g, const fl
                 const float kz = static_cast<float>(2 * static_cast<int>((q.ebz >> cc) & 1u) - 1);
                 lsum += softplusf(kx * mx) + softplusf(kz * mz);
             }
+            asm volatile("" : "+v"(lsum));  // summed here, not carried into the next iteration's passes
             __syncthreads();
             const int cnt = s_cnt[it & 1];  // unsatisfied checks of the estimate of iteration it - 1
             if (t == 0 && it >= 2) s_ubar[it - 1] = static_cast<float>(cnt) / mtot;
@@ -575,13 +583,12 @@ __device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&l)[QJ][Q
             const uint32_t ng = mu[i] < 0.0f ? 1u : 0u;
             neg |= ng << i;
             negall ^= ng;
-            if (a < m1) {
-                m2 = m1;
-                m1 = a;
-                i1 = i;
-            } else if (a < m2) {
-                m2 = a;
-            }
+            // the two smallest magnitudes and the first index of the
+            // smallest, without branches
+            const bool lt1 = a < m1;
+            m2 = lt1 ? m1 : (a < m2 ? a : m2);
+            i1 = lt1 ? i : i1;
+            m1 = lt1 ? a : m1;
         }
         // the first index of the smallest |mu| but i1's
         int i2 = i1 == 0 ? 1 : 0;
@@ -589,10 +596,9 @@ __device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&l)[QJ][Q
 #pragma unroll
         for (int i = 0; i < QL; ++i) {
             const float a = fabsf(mu[i]);
-            if (i != i1 && a < a2) {
-                i2 = i;
-                a2 = a;
-            }
+            const bool lt = i != i1 && a < a2;
+            i2 = lt ? i : i2;
+            a2 = lt ? a : a2;
         }
         uint32_t rawneg = 0;
 #pragma unroll
