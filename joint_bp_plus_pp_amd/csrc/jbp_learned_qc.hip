@@ -158,10 +158,16 @@ __device__ __forceinline__ uint32_t vslot(const VarSlots &v, int r, int cc) {
 // y = (a x + b) mod P, a and b of block r*L + i from the code's table
 // ([side][a | b | ainv][r*L + cc]).  a x + b < 2^20, where
 // floor(y / 768) = mulhi(y, 5592406) exactly.
+// The table is read through the constant address space: it is never
+// written while the kernels run, so its uniform reads are scalar loads
+// (vector loads would share the vector memory counter with the tape's
+// loads and stores, and waiting for them would wait for those too).
+typedef __attribute__((address_space(4))) const int ctab_i32;
 __device__ __forceinline__ uint32_t check_slot(uint32_t bbase, const int *tab, int s, int r, int i, int x) {
     constexpr int JL = QJ * QL;
     const int blk = r * QL + i;
-    const uint32_t y = static_cast<uint32_t>(tab[(3 * s) * JL + blk] * x + tab[(3 * s + 1) * JL + blk]);
+    const ctab_i32 *ct = (const ctab_i32 *)tab;
+    const uint32_t y = static_cast<uint32_t>(ct[(3 * s) * JL + blk] * x + ct[(3 * s + 1) * JL + blk]);
     const uint32_t q = __umulhi(y, 5592406u);
     return bbase + static_cast<uint32_t>(blk * QP * 4) + (y - q * static_cast<uint32_t>(QP)) * 4u;
 }
