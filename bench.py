@@ -232,17 +232,18 @@ def sp_learned_leg(jbp, dec, args, first):
 
 def training_leg(jbp, dec, args):
     """The training workload: --train-steps steps (after one untimed) of the
-    min-sum model at T = 25, batch --train-batch, p uniform in [0.03, 0.05],
-    each a device-sampled batch, the forward pass with its tape, the adjoint
-    pass and the Adam update.  The forward tape (the check messages of every
-    iteration, (T + 1) x 55,296 FP32 values per trial) is written once and
-    read once per step: that is the step's streaming traffic."""
+    min-sum model with its MLP damping at T = 25, batch --train-batch, p
+    uniform in [0.03, 0.05], each a device-sampled batch, the forward pass
+    with its tape, the adjoint pass and the Adam update.  The forward tape
+    (the check messages of every iteration, (T + 1) x 55,296 FP32 values per
+    trial) is written once and read once per step: that is the step's
+    streaming traffic."""
     T, B = 25, args.train_batch
     model = jbp.LearnedModel(dec, T, kind=jbp.LEARNED_MIN_SUM)
     params = model.get_params()
     params[:4 * model.nb * T] = 1.0
     params[:2 * model.nb * T] = 0.75
-    model.set_params(params, use_mlp=False)
+    model.set_params(params, use_mlp=True)
     model.train_step(args.seed, 1, B, 0.03, 0.05, lr=1e-3)
     t0 = time.perf_counter()
     losses = []
@@ -251,8 +252,8 @@ def training_leg(jbp, dec, args):
     el = (time.perf_counter() - t0) / args.train_steps
     tape = B * (T + 1) * EDGES_PER_ITER * 4
     model.destroy()
-    return {"workload": "min-sum model, T=%d, batch %d, p ~ U[0.03, 0.05], FP32 forward + adjoint + "
-                        "Adam" % (T, B),
+    return {"workload": "min-sum model with MLP damping, T=%d, batch %d, p ~ U[0.03, 0.05], FP32 forward "
+                        "+ adjoint + Adam" % (T, B),
             "steps": args.train_steps, "ms_per_step": round(1e3 * el, 2),
             "trial_iterations_per_s": round(B * T / el, 1),
             "tape_bytes_per_step": tape,

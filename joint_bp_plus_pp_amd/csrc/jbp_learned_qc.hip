@@ -27,10 +27,12 @@
 // the gradient of mu.  The estimate bit of a variable rides in the least
 // significant bit of every mu it sends, so the check owners count
 // unsatisfied checks while reading their rows (the forward and the adjoint
-// form mu bit for bit alike, so they take the same minima).  Weight
-// gradients are summed over the wavefront (DPP) and accumulated per block
-// and iteration in LDS, then added to the global gradient once per
-// workgroup.
+// form mu bit for bit alike, so they take the same minima).  The adjoint
+// holds one side's gradients in registers and parks the other's in the
+// tape row its loss term has consumed.  Weight gradients of a block row are
+// summed over the wavefront in one transposing reduction and accumulated
+// per iteration and block in LDS, then added to the global gradient once
+// per workgroup.
 #include "jbp_common.h"
 #include "jbp_learned.h"
 
