@@ -925,11 +925,462 @@ __global__ void __launch_bounds__(QP) learned_qc_bwd_kernel(DevGraph g, const fl
         if (use_mlp && s_gmlp[i] != 0.0f) atomicAdd(&gmlp[i], s_gmlp[i]);
 }
 
+// ---- layered min-sum (kind 2): block row r of both sides is layer r ----
+//
+// Iteration t updates the layers in turn: layer r forms every variable's
+// sums from the current messages (rows before r already of iteration t),
+// sends mu = wv * (S - lambda + F) on row r of both sides, and replaces row
+// r's messages by wc * CN(mu).  The iteration converges when every layer
+// starts from the same estimate as layer 0 and every check accepts that
+// estimate (jbp_learned.hip's layered kernels; no damping).  The exchange
+// buffer holds one layer: B[side][cc][y] for row r (73.7 KB).  Layer 0's
+// estimate bits ride in mu's least significant bit at every layer, so each
+// layer's check owners count its checks against that estimate.
+
+__device__ __forceinline__ VarSlots var_slots_l(uint32_t bbase) {
+    uint32_t a = bbase + static_cast<uint32_t>(threadIdx.x) * 4u;
+    asm volatile("" : "+v"(a));
+    return {a, a + static_cast<uint32_t>(QL * QP * 4)};
+}
+// variable (cc, t)'s slot of side s in the layer buffer
+__device__ __forceinline__ uint32_t lslot(const VarSlots &v, int s, int cc) {
+    return (s == 0 ? v.lo : v.hi) + static_cast<uint32_t>(cc * QP * 4);
+}
+// check (r, x)'s input i of side s in the layer buffer
+__device__ __forceinline__ uint32_t lcheck_slot(uint32_t bbase, const int *tab, int s, int r, int i, int x) {
+    constexpr int JL = QJ * QL;
+    const int blk = r * QL + i;
+    const ctab_i32 *ct = (const ctab_i32 *)tab;
+    const uint32_t y = static_cast<uint32_t>(ct[(3 * s) * JL + blk] * x + ct[(3 * s + 1) * JL + blk]);
+    const uint32_t q = __umulhi(y, 5592406u);
+    return bbase + static_cast<uint32_t>((s * QL + i) * QP * 4) + (y - q * static_cast<uint32_t>(QP)) * 4u;
+}
+
+// One check row of the layer (side s): min-sum in place, clamped outputs;
+// returns 1 when its inputs' least significant bits disagree with the
+// syndrome bit.
+__device__ __forceinline__ uint32_t lms_check_row_fwd(uint32_t bbase, const int *tab, int s, int r, int x,
+                                                      uint32_t synbit) {
+    float mu[QL];
+    uint32_t par = 0, neg = 0, negall = synbit;
+    float m1 = HUGE_VALF, m2 = HUGE_VALF;
+    int i1 = -1;
+#pragma unroll
+    for (int i = 0; i < QL; ++i) mu[i] = ldsf(lcheck_slot(bbase, tab, s, r, i, x));
+#pragma unroll
+    for (int i = 0; i < QL; ++i) {
+        par ^= __float_as_uint(mu[i]) & 1u;
+        const float a = fabsf(mu[i]);
+        const uint32_t ng = mu[i] < 0.0f ? 1u : 0u;
+        neg |= ng << i;
+        negall ^= ng;
+        const bool lt1 = a < m1;
+        m2 = lt1 ? m1 : (a < m2 ? a : m2);
+        i1 = lt1 ? i : i1;
+        m1 = lt1 ? a : m1;
+    }
+#pragma unroll
+    for (int i = 0; i < QL; ++i) {
+        const float mag = i == i1 ? m2 : m1;
+        const float out = mag < kLmaxQ ? mag : kLmaxQ;
+        stf(lcheck_slot(bbase, tab, s, r, i, x), ((negall ^ (neg >> i)) & 1u) ? -out : out);
+    }
+    return par != synbit ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(QP) learned_qc_lms_fwd_kernel(DevGraph g, const float *params, int T,
+                                                               TrialPlanes tp, float *act, const double *prior,
+                                                               float *loss, int *conv_out, int batch,
+                                                               int *counter) {
+    extern __shared__ float B[];  // two layer buffers [2][side][L][P]
+    __shared__ int s_trial;
+    __shared__ float s_loss;
+    __shared__ float s_w[2][QW];
+    const int t = threadIdx.x, lane = t & 63;
+    const uint32_t bbase0 = static_cast<uint32_t>((uintptr_t)((lds_f32 *)B));
+    const size_t row = static_cast<size_t>(QK) * QP;
+    const float scale = 1.0f / (static_cast<float>(batch) * static_cast<float>(T) * static_cast<float>(2 * g.n));
+    for (;;) {
+        if (t == 0) s_trial = atomicAdd(counter, 1);
+        __syncthreads();
+        const int b = __builtin_amdgcn_readfirstlane(s_trial);
+        __syncthreads();
+        if (b >= batch) return;
+        QcTrial q = qc_trial(g, tp, prior, b);
+        const tape_rsrc tape = make_tape(act + static_cast<size_t>(b) * (T + 1) * row);
+        const uint32_t voff = static_cast<uint32_t>(t) * 4u;
+        float lam[2][QJ][QL];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int r = 0; r < QJ; ++r)
+#pragma unroll
+                for (int cc = 0; cc < QL; ++cc) {
+                    lam[s][r][cc] = 0.0f;
+                    tape_st(tape, voff, tape_off(0, s, r, cc), 0.0f);
+                }
+        stage_weights(s_w[1], params, T, 1);
+        if (t == 0) s_loss = 0.0f;
+        __syncthreads();
+        int conv = 0;
+        float lsum = 0.0f;
+        for (int it = 1; it <= T; ++it) {
+            const float *w = s_w[it & 1];
+            stage_weights(s_w[(it + 1) & 1], params, T, it + 1);
+            uint32_t e0x = 0, e0z = 0;  // layer 0's estimate
+            uint32_t flag = 0;          // an unstable estimate or an unsatisfied check
+#pragma unroll
+            for (int r = 0; r < QJ; ++r) {
+                const uint32_t bbase = bbase0 + static_cast<uint32_t>((r & 1) * 2 * QL * QP * 4);
+                const VarSlots vs = var_slots_l(bbase);
+#pragma unroll
+                for (int cc = 0; cc < QL; ++cc) {
+                    const float sx = (lam[0][0][cc] + lam[0][1][cc]) + lam[0][2][cc];
+                    const float sz = (lam[1][0][cc] + lam[1][1][cc]) + lam[1][2][cc];
+                    const float fx = fterm(sz, q.lx), fz = fterm(sx, q.lz);
+                    const uint32_t ex = (sx + fx < 0.0f) ? 1u : 0u, ez = (sz + fz < 0.0f) ? 1u : 0u;
+                    if (r == 0) {
+                        e0x |= ex << cc;
+                        e0z |= ez << cc;
+                    } else {
+                        flag |= (ex ^ ((e0x >> cc) & 1u)) | (ez ^ ((e0z >> cc) & 1u));
+                    }
+                    const float ax = (sx - lam[0][r][cc]) + fx, az = (sz - lam[1][r][cc]) + fz;
+                    stf(lslot(vs, 0, cc), with_lsb(w[2 * QNB + r * QL + cc] * ax, (e0x >> cc) & 1u));
+                    stf(lslot(vs, 1, cc), with_lsb(w[3 * QNB + r * QL + cc] * az, (e0z >> cc) & 1u));
+                }
+                asm volatile("" : "+v"(e0x), "+v"(e0z), "+v"(flag));
+                __syncthreads();
+                {
+                    int x = t;
+                    asm volatile("" : "+v"(x));
+                    flag |= lms_check_row_fwd(bbase, g.qc_tab, 0, r, x, (q.synx >> r) & 1u);
+                    flag |= lms_check_row_fwd(bbase, g.qc_tab, 1, r, x, (q.synz >> r) & 1u);
+                }
+                __syncthreads();
+                const VarSlots vr = var_slots_l(bbase);
+#pragma unroll
+                for (int cc = 0; cc < QL; ++cc) {
+                    lam[0][r][cc] = w[r * QL + cc] * ldsf(lslot(vr, 0, cc));
+                    lam[1][r][cc] = w[QNB + r * QL + cc] * ldsf(lslot(vr, 1, cc));
+                }
+                pin_side(lam[0]);
+                pin_side(lam[1]);
+            }
+            const int any = __syncthreads_or(static_cast<int>(flag));
+            if (any == 0 && conv == 0) conv = it;
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int r = 0; r < QJ; ++r)
+#pragma unroll
+                    for (int cc = 0; cc < QL; ++cc) tape_st(tape, voff, tape_off(it, s, r, cc), lam[s][r][cc]);
+            // loss on the marginals at the end of the iteration
+            asm volatile("" : "+v"(q.ebx), "+v"(q.ebz));
+#pragma unroll
+            for (int cc = 0; cc < QL; ++cc) {
+                const float sx = (lam[0][0][cc] + lam[0][1][cc]) + lam[0][2][cc];
+                const float sz = (lam[1][0][cc] + lam[1][1][cc]) + lam[1][2][cc];
+                const float mx = sx + fterm(sz, q.lx), mz = sz + fterm(sx, q.lz);
+                const float kx = static_cast<float>(2 * static_cast<int>((q.ebx >> cc) & 1u) - 1);
+                const float kz = static_cast<float>(2 * static_cast<int>((q.ebz >> cc) & 1u) - 1);
+                lsum += softplusf(kx * mx) + softplusf(kz * mz);
+            }
+            asm volatile("" : "+v"(lsum));
+        }
+        {
+            const float ws = wave_sum(lsum);
+            if (lane == 0) atomicAdd(&s_loss, ws);
+        }
+        __syncthreads();
+        if (t == 0) {
+            atomicAdd(loss, s_loss * scale);
+            conv_out[b] = conv;
+        }
+        __syncthreads();
+    }
+}
+
+// The layered adjoint.  Iteration t is undone layer by layer (r = 2, 1, 0)
+// with the gradients of all 72 messages in registers.  The state before step
+// (t, r) — rows before r from tape row t, the rest from row t-1 — is read
+// again for each layer (its sums are kept through the layer's passes, row r's
+// own messages read again before the last pass); layer 0's estimate, which
+// rode in every mu of the iteration, comes from row t-1's sums.
+__global__ void __launch_bounds__(QP) learned_qc_lms_bwd_kernel(DevGraph g, const float *params, int T,
+                                                               TrialPlanes tp, float *act, const double *prior,
+                                                               float *grad, int batch, int *counter) {
+    extern __shared__ float dyn[];  // layer buffer [side][L][P] | weight gradients [T][4][nb]
+    float *s_gw = dyn + 2 * QL * QP;
+    __shared__ int s_trial;
+    __shared__ float s_w[2][QW];
+    const int t = threadIdx.x, lane = t & 63;
+    const uint32_t bbase = static_cast<uint32_t>((uintptr_t)((lds_f32 *)dyn));
+    const size_t row = static_cast<size_t>(QK) * QP;
+    const float scale = 1.0f / (static_cast<float>(batch) * static_cast<float>(T) * static_cast<float>(2 * g.n));
+    for (int i = t; i < QW * T; i += QP) s_gw[i] = 0.0f;
+    __syncthreads();
+    for (;;) {
+        if (t == 0) s_trial = atomicAdd(counter, 1);
+        __syncthreads();
+        const int b = __builtin_amdgcn_readfirstlane(s_trial);
+        __syncthreads();
+        if (b >= batch) break;
+        QcTrial q = qc_trial(g, tp, prior, b);
+        const tape_rsrc tape = make_tape(act + static_cast<size_t>(b) * (T + 1) * row);
+        const uint32_t voff = static_cast<uint32_t>(t) * 4u;
+        float G[2][QJ][QL];  // dL / d lambda of the state after the step being undone
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int r = 0; r < QJ; ++r)
+#pragma unroll
+                for (int cc = 0; cc < QL; ++cc) G[s][r][cc] = 0.0f;
+        // sums of tape row k for the loss term of iteration k: row T's here,
+        // then each iteration's layer 0 forms row t-1's
+        float sxc[QL], szc[QL];
+        {
+            const uint32_t krow = tape_row(T);
+#pragma unroll
+            for (int cc = 0; cc < QL; ++cc) {
+                auto ldc = [&](int s, int r) { return tape_ld(tape, voff, krow + edge_off(s, r, cc)); };
+                sxc[cc] = (ldc(0, 0) + ldc(0, 1)) + ldc(0, 2);
+                szc[cc] = (ldc(1, 0) + ldc(1, 1)) + ldc(1, 2);
+                if ((cc & 3) == 3) tape_fence();
+            }
+        }
+        stage_weights(s_w[T & 1], params, T, T);
+        __syncthreads();
+        const int wslot = (lane & 3) == 0 ? slot12(static_cast<uint32_t>(lane)) : -1;
+        for (int it = T; it >= 1; --it) {
+            const float *w = s_w[it & 1];
+            float *gw_it = s_gw + (it - 1) * QW;
+            stage_weights(s_w[(it - 1) & 1], params, T, it - 1);
+            // the loss term at the end of iteration it
+            asm volatile("" : "+v"(q.ebx), "+v"(q.ebz));
+#pragma unroll
+            for (int cc = 0; cc < QL; ++cc) {
+                const float sx = sxc[cc], sz = szc[cc];
+                const float mx = sx + fterm(sz, q.lx), mz = sz + fterm(sx, q.lz);
+                const float kx = static_cast<float>(2 * static_cast<int>((q.ebx >> cc) & 1u) - 1);
+                const float kz = static_cast<float>(2 * static_cast<int>((q.ebz >> cc) & 1u) - 1);
+                const float glx = scale * kx * sigmf(kx * mx), glz = scale * kz * sigmf(kz * mz);
+                const float gsx = glx + glz * fterm_d(sx, q.lz);
+                const float gsz = glz + glx * fterm_d(sz, q.lx);
+#pragma unroll
+                for (int r = 0; r < QJ; ++r) {
+                    G[0][r][cc] += gsx;
+                    G[1][r][cc] += gsz;
+                }
+            }
+            pin_side(G[0]);
+            pin_side(G[1]);
+            // layer 0's estimate of iteration it, from row it-1's sums
+            uint32_t e0x = 0, e0z = 0;
+            {
+                const uint32_t krow = tape_row(it - 1);
+#pragma unroll
+                for (int cc = 0; cc < QL; ++cc) {
+                    auto ldp = [&](int s, int r) { return tape_ld(tape, voff, krow + edge_off(s, r, cc)); };
+                    const float sx = (ldp(0, 0) + ldp(0, 1)) + ldp(0, 2);
+                    const float sz = (ldp(1, 0) + ldp(1, 1)) + ldp(1, 2);
+                    e0x |= (sx + fterm(sz, q.lx) < 0.0f ? 1u : 0u) << cc;
+                    e0z |= (sz + fterm(sx, q.lz) < 0.0f ? 1u : 0u) << cc;
+                    if ((cc & 3) == 3) tape_fence();
+                }
+            }
+            asm volatile("" : "+v"(e0x), "+v"(e0z));
+#pragma unroll
+            for (int r = QJ - 1; r >= 0; --r) {
+                // (2) the state before step (it, r), its sums, and mu of row r
+                float Sx[QL], Sz[QL];
+                {
+                    const uint32_t crow = tape_row(it), prow = tape_row(it - 1);
+                    VarSlots vs = var_slots_l(bbase);
+#pragma unroll
+                    for (int cc = 0; cc < QL; ++cc) {
+                        float x[2][QJ];
+#pragma unroll
+                        for (int s = 0; s < 2; ++s)
+#pragma unroll
+                            for (int rr = 0; rr < QJ; ++rr)
+                                x[s][rr] = tape_ld(tape, voff, (rr < r ? crow : prow) + edge_off(s, rr, cc));
+                        Sx[cc] = (x[0][0] + x[0][1]) + x[0][2];
+                        Sz[cc] = (x[1][0] + x[1][1]) + x[1][2];
+                        const float fx = fterm(Sz[cc], q.lx), fz = fterm(Sx[cc], q.lz);
+                        const float ax = (Sx[cc] - x[0][r]) + fx, az = (Sz[cc] - x[1][r]) + fz;
+                        stf(lslot(vs, 0, cc), with_lsb(w[2 * QNB + r * QL + cc] * ax, (e0x >> cc) & 1u));
+                        stf(lslot(vs, 1, cc), with_lsb(w[3 * QNB + r * QL + cc] * az, (e0z >> cc) & 1u));
+                        if ((cc & 3) == 3) tape_fence();
+                    }
+                }
+                pin_row(Sx);
+                pin_row(Sz);
+                __syncthreads();
+                // (3) row r's checks of both sides again, raw outputs in place
+                uint32_t info[2];
+                {
+                    int x = t;
+                    asm volatile("" : "+v"(x));
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        const uint32_t syn = ((s == 0 ? q.synx : q.synz) >> r) & 1u;
+                        float mu[QL];
+                        uint32_t neg = 0, negall = syn;
+                        float m1 = HUGE_VALF, m2 = HUGE_VALF;
+                        int i1 = -1;
+#pragma unroll
+                        for (int i = 0; i < QL; ++i) mu[i] = ldsf(lcheck_slot(bbase, g.qc_tab, s, r, i, x));
+#pragma unroll
+                        for (int i = 0; i < QL; ++i) {
+                            const float a = fabsf(mu[i]);
+                            const uint32_t ng = mu[i] < 0.0f ? 1u : 0u;
+                            neg |= ng << i;
+                            negall ^= ng;
+                            const bool lt1 = a < m1;
+                            m2 = lt1 ? m1 : (a < m2 ? a : m2);
+                            i1 = lt1 ? i : i1;
+                            m1 = lt1 ? a : m1;
+                        }
+                        int i2 = i1 == 0 ? 1 : 0;
+                        float a2 = fabsf(i1 == 0 ? mu[1] : mu[0]);
+#pragma unroll
+                        for (int i = 0; i < QL; ++i) {
+                            const float a = fabsf(mu[i]);
+                            const bool lt = i != i1 && a < a2;
+                            i2 = lt ? i : i2;
+                            a2 = lt ? a : a2;
+                        }
+                        uint32_t rawneg = 0;
+#pragma unroll
+                        for (int i = 0; i < QL; ++i) {
+                            const float mag = i == i1 ? m2 : m1;
+                            const float raw = ((negall ^ (neg >> i)) & 1u) ? -mag : mag;
+                            rawneg |= (raw < 0.0f ? 1u : 0u) << i;
+                            stf(lcheck_slot(bbase, g.qc_tab, s, r, i, x), raw);
+                        }
+                        info[s] = static_cast<uint32_t>(i1) | (static_cast<uint32_t>(i2) << 4) | (neg << 8) |
+                                  (rawneg << 20);
+                    }
+                }
+                __syncthreads();
+                // (4) check weights: their gradient, and the gradient of the
+                // raw check outputs
+                {
+                    const VarSlots vs = var_slots_l(bbase);
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        float pw[QL];
+#pragma unroll
+                        for (int cc = 0; cc < QL; ++cc) {
+                            const float raw = ldsf(lslot(vs, s, cc));
+                            const float mag = fabsf(raw);
+                            const float cl = mag < kLmaxQ ? mag : kLmaxQ;
+                            const float out = raw < 0.0f ? -cl : cl;
+                            const float wc = w[s * QNB + r * QL + cc];
+                            const float ge = G[s][r][cc];
+                            pw[cc] = ge * out;
+                            stf(lslot(vs, s, cc), mag > kLmaxQ ? 0.0f : wc * ge);
+                        }
+                        const float sum = reduce12(pw, static_cast<uint32_t>(lane));
+                        if (wslot >= 0) atomicAdd(&gw_it[s * QNB + r * QL + wslot], sum);
+                    }
+                }
+                __syncthreads();
+                // (5) check adjoint
+                {
+                    int x = t;
+                    asm volatile("" : "+v"(x));
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        const uint32_t inf = info[s];
+                        uint32_t ad[QL];
+#pragma unroll
+                        for (int i = 0; i < QL; ++i) ad[i] = lcheck_slot(bbase, g.qc_tab, s, r, i, x);
+                        const int i1 = static_cast<int>(inf & 15u), i2 = static_cast<int>((inf >> 4) & 15u);
+                        float to1 = 0.0f, to2 = 0.0f;
+#pragma unroll
+                        for (int i = 0; i < QL; ++i) {
+                            const float ge = ldsf(ad[i]);
+                            const float se = ((inf >> (20 + i)) & 1u) ? -ge : ge;
+                            if (i == i1) to2 += se;
+                            else to1 += se;
+                        }
+#pragma unroll
+                        for (int i = 0; i < QL; ++i) {
+                            float o = 0.0f;
+                            if (i == i1) o = ((inf >> (8 + i)) & 1u) ? -to1 : to1;
+                            else if (i == i2) o = ((inf >> (8 + i)) & 1u) ? -to2 : to2;
+                            stf(ad[i], o);
+                        }
+                    }
+                }
+                // row r's own messages before the step, for a (arriving during
+                // the barrier)
+                float pl[2][QL];
+                {
+                    const uint32_t prow = tape_row(it - 1);
+#pragma unroll
+                    for (int s = 0; s < 2; ++s)
+#pragma unroll
+                        for (int cc = 0; cc < QL; ++cc) pl[s][cc] = tape_ld(tape, voff, prow + edge_off(s, r, cc));
+                }
+                __syncthreads();
+                // (6) the variable side: wv gradients; the gradient of the state
+                // before the step (row r's messages were replaced: their
+                // gradient restarts from the sums' shares less their own share
+                // of a; the other rows gain the sums' shares)
+                {
+                    const VarSlots vs = var_slots_l(bbase);
+                    float pvx[QL], pvz[QL];
+#pragma unroll
+                    for (int cc = 0; cc < QL; ++cc) {
+                        const float gmx = ldsf(lslot(vs, 0, cc)), gmz = ldsf(lslot(vs, 1, cc));
+                        const float fx = fterm(Sz[cc], q.lx), fz = fterm(Sx[cc], q.lz);
+                        const float ax = (Sx[cc] - pl[0][cc]) + fx, az = (Sz[cc] - pl[1][cc]) + fz;
+                        pvx[cc] = gmx * ax;
+                        pvz[cc] = gmz * az;
+                        const float gax = w[2 * QNB + r * QL + cc] * gmx, gaz = w[3 * QNB + r * QL + cc] * gmz;
+                        const float dsx = gax + gaz * fterm_d(Sx[cc], q.lz);  // d S_x: a_x, and a_z through F_z
+                        const float dsz = gaz + gax * fterm_d(Sz[cc], q.lx);
+#pragma unroll
+                        for (int rr = 0; rr < QJ; ++rr) {
+                            G[0][rr][cc] = rr == r ? dsx - gax : G[0][rr][cc] + dsx;
+                            G[1][rr][cc] = rr == r ? dsz - gaz : G[1][rr][cc] + dsz;
+                        }
+                    }
+                    const float sx_ = reduce12(pvx, static_cast<uint32_t>(lane));
+                    if (wslot >= 0) atomicAdd(&gw_it[2 * QNB + r * QL + wslot], sx_);
+                    const float sz_ = reduce12(pvz, static_cast<uint32_t>(lane));
+                    if (wslot >= 0) atomicAdd(&gw_it[3 * QNB + r * QL + wslot], sz_);
+                }
+                pin_side(G[0]);
+                pin_side(G[1]);
+                if (r == 0) {
+#pragma unroll
+                    for (int cc = 0; cc < QL; ++cc) {
+                        sxc[cc] = Sx[cc];
+                        szc[cc] = Sz[cc];
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < QW * T; i += QP) {
+        const int kk = i / T, j = i - kk * T;
+        const float v = s_gw[j * QW + kk];
+        if (v != 0.0f) atomicAdd(&grad[i], v);
+    }
+}
+
 }  // namespace
 
 bool learned_qc_train_supported(const DevGraph &g, const LearnedDevParams &lp) {
-    return lp.kind == kLearnedMS && g.qc_tab && g.qc_P == QP && g.qc_J == QJ && g.qc_L == QL &&
-           lp.nb == QNB && lp.T >= 1 && lp.T <= kMaxLearnedT;
+    const bool kind_ok = lp.kind == kLearnedMS || (lp.kind == kLearnedLMS && lp.lpb == QL && !lp.use_mlp);
+    return kind_ok && g.qc_tab && g.qc_P == QP && g.qc_J == QJ && g.qc_L == QL && lp.nb == QNB && lp.T >= 1 &&
+           lp.T <= kMaxLearnedT;
 }
 
 void launch_learned_train_qc(const DevGraph &g, const LearnedDevParams &lp, const float *params, TrialPlanes tp,
@@ -943,6 +1394,21 @@ void launch_learned_train_qc(const DevGraph &g, const LearnedDevParams &lp, cons
     const size_t bwd_lds = static_cast<size_t>(QNB) * QP * sizeof(float) +
                            static_cast<size_t>(4) * QNB * lp.T * sizeof(float);
     launch_fill_i32(counters, 0, 2, st);
+    if (lp.kind == kLearnedLMS) {
+        const size_t lfwd_lds = static_cast<size_t>(2) * 2 * QL * QP * sizeof(float);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(learned_qc_lms_fwd_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lfwd_lds));
+        hipLaunchKernelGGL(learned_qc_lms_fwd_kernel, dim3(grid), dim3(QP), lfwd_lds, st, g, params, lp.T, tp, act,
+                           prior, loss, conv, batch, counters);
+        if (!backward) return;
+        const size_t lbwd_lds = static_cast<size_t>(2) * QL * QP * sizeof(float) +
+                                static_cast<size_t>(QW) * lp.T * sizeof(float);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(learned_qc_lms_bwd_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lbwd_lds));
+        hipLaunchKernelGGL(learned_qc_lms_bwd_kernel, dim3(grid), dim3(QP), lbwd_lds, st, g, params, lp.T, tp, act,
+                           prior, grad, batch, counters + 1);
+        return;
+    }
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(learned_qc_fwd_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(fwd_lds));
     hipLaunchKernelGGL(learned_qc_fwd_kernel, dim3(grid), dim3(QP), fwd_lds, st, g, params, lp.T, lp.use_mlp, tp,

@@ -83,6 +83,33 @@ def test_fp64_loss_and_gradient_match_oracle(main_decoder, main_code_oracle, lo)
     model.destroy()
 
 
+def test_fp32_trainer_follows_fp64_at_t25(main_decoder, main_code_oracle):
+    # the training workload's depth (T = 25) on 24 trials at p in
+    # [0.03, 0.05], weights spread around the shipped schedule: the FP32
+    # trainer (messages on chip, for this code) against the FP64 one, which
+    # the oracle pins
+    T = 25
+    model = jbp.LearnedModel(main_decoder, T, kind=LMS)
+    rng = np.random.default_rng(17)
+    params = schedule_params(model)
+    k = 2 * model.nb * T
+    params[:2 * k] *= 1.0 + 0.05 * rng.standard_normal(2 * k)
+    model.set_params(params, use_mlp=False)
+    n = 24
+    p = np.linspace(0.03, 0.05, n)
+    errs = np.stack([main_code_oracle.sample(2024, 1 + t, float(p[t])) for t in range(n)])
+    l64, g64, c64 = model.loss_grad(errs, p, precision=jbp.FP64)
+    l32, g32, c32 = model.loss_grad(errs, p, precision=jbp.FP32)
+    cos = g32 @ g64 / (np.linalg.norm(g32) * np.linalg.norm(g64))
+    print("layered T=25: loss fp32 %.9g fp64 %.9g; gradient cosine %.7f; convergence agreement %.3f"
+          % (l32, l64, cos, float(np.mean(c32 == c64))), flush=True)
+    assert np.isfinite(l32) and np.all(np.isfinite(g32))
+    assert abs(l32 - l64) <= 1e-4 * abs(l64)
+    assert cos > 0.9999
+    assert np.mean(c32 == c64) >= 0.9
+    model.destroy()
+
+
 def test_fp16_inference_follows_the_training_forward_pass(main_decoder):
     # the inference phase (FP16 pairs, base 2) and the trainer's forward pass
     # (FP32, natural log) are the same model with the same convergence rule:
