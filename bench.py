@@ -237,7 +237,8 @@ def training_leg(jbp, dec, args):
     with its tape, the adjoint pass and the Adam update.  The forward tape
     (the check messages of every iteration, (T + 1) x 55,296 FP32 values per
     trial) is written once and read once per step: that is the step's
-    streaming traffic."""
+    streaming traffic.  The same number of steps of the headline decoder's
+    model (layered min-sum, no damping) is timed after it."""
     T, B = 25, args.train_batch
     model = jbp.LearnedModel(dec, T, kind=jbp.LEARNED_MIN_SUM)
     params = model.get_params()
@@ -252,8 +253,24 @@ def training_leg(jbp, dec, args):
     el = (time.perf_counter() - t0) / args.train_steps
     tape = B * (T + 1) * EDGES_PER_ITER * 4
     model.destroy()
+    # the headline decoder's model (layered min-sum, no damping), same steps
+    lm = jbp.LearnedModel(dec, T, kind=jbp.LEARNED_LAYERED_MIN_SUM)
+    lp = lm.get_params()
+    lp[:4 * lm.nb * T] = 1.0
+    lp[:2 * lm.nb * T] = 0.75
+    lm.set_params(lp, use_mlp=False)
+    lm.train_step(args.seed, 1, B, 0.03, 0.05, lr=1e-3)
+    t1 = time.perf_counter()
+    llosses = []
+    for k in range(args.train_steps):
+        llosses.append(lm.train_step(args.seed, 1 + (k + 1) * B, B, 0.03, 0.05, lr=1e-3))
+    lel = (time.perf_counter() - t1) / args.train_steps
+    lm.destroy()
     return {"workload": "min-sum model with MLP damping, T=%d, batch %d, p ~ U[0.03, 0.05], FP32 forward "
                         "+ adjoint + Adam" % (T, B),
+            "layered_min_sum": {"ms_per_step": round(1e3 * lel, 2),
+                                "trial_iterations_per_s": round(B * T / lel, 1),
+                                "loss": [round(x, 6) for x in llosses]},
             "steps": args.train_steps, "ms_per_step": round(1e3 * el, 2),
             "trial_iterations_per_s": round(B * T / el, 1),
             "tape_bytes_per_step": tape,

@@ -1218,11 +1218,11 @@ __global__ void __launch_bounds__(QP) learned_qc_lms_bwd_kernel(DevGraph g, cons
                 pin_row(Sz);
                 __syncthreads();
                 // (3) row r's checks of both sides again, raw outputs in place
-                uint32_t info[2];
+                uint32_t info0 = 0, info1 = 0;
                 {
                     int x = t;
                     asm volatile("" : "+v"(x));
-#pragma unroll
+#pragma unroll 1
                     for (int s = 0; s < 2; ++s) {
                         const uint32_t syn = ((s == 0 ? q.synx : q.synz) >> r) & 1u;
                         float mu[QL];
@@ -1259,8 +1259,10 @@ __global__ void __launch_bounds__(QP) learned_qc_lms_bwd_kernel(DevGraph g, cons
                             rawneg |= (raw < 0.0f ? 1u : 0u) << i;
                             stf(lcheck_slot(bbase, g.qc_tab, s, r, i, x), raw);
                         }
-                        info[s] = static_cast<uint32_t>(i1) | (static_cast<uint32_t>(i2) << 4) | (neg << 8) |
-                                  (rawneg << 20);
+                        const uint32_t info = static_cast<uint32_t>(i1) | (static_cast<uint32_t>(i2) << 4) |
+                                              (neg << 8) | (rawneg << 20);
+                        info0 = s == 0 ? info : info0;
+                        info1 = s == 1 ? info : info1;
                     }
                 }
                 __syncthreads();
@@ -1291,9 +1293,9 @@ __global__ void __launch_bounds__(QP) learned_qc_lms_bwd_kernel(DevGraph g, cons
                 {
                     int x = t;
                     asm volatile("" : "+v"(x));
-#pragma unroll
+#pragma unroll 1
                     for (int s = 0; s < 2; ++s) {
-                        const uint32_t inf = info[s];
+                        const uint32_t inf = s == 0 ? info0 : info1;
                         uint32_t ad[QL];
 #pragma unroll
                         for (int i = 0; i < QL; ++i) ad[i] = lcheck_slot(bbase, g.qc_tab, s, r, i, x);
@@ -1332,14 +1334,24 @@ __global__ void __launch_bounds__(QP) learned_qc_lms_bwd_kernel(DevGraph g, cons
                 // of a; the other rows gain the sums' shares)
                 {
                     const VarSlots vs = var_slots_l(bbase);
-                    float pvx[QL], pvz[QL];
+                    // wv gradients, one side at a time
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        float pv[QL];
+#pragma unroll
+                        for (int cc = 0; cc < QL; ++cc) {
+                            const float gm = ldsf(lslot(vs, s, cc));
+                            const float f = s == 0 ? fterm(Sz[cc], q.lx) : fterm(Sx[cc], q.lz);
+                            const float a = ((s == 0 ? Sx[cc] : Sz[cc]) - pl[s][cc]) + f;
+                            pv[cc] = gm * a;
+                        }
+                        const float sum = reduce12(pv, static_cast<uint32_t>(lane));
+                        if (wslot >= 0) atomicAdd(&gw_it[2 * QNB + s * QNB + r * QL + wslot], sum);
+                    }
+                    // the gradient of the state before the step
 #pragma unroll
                     for (int cc = 0; cc < QL; ++cc) {
                         const float gmx = ldsf(lslot(vs, 0, cc)), gmz = ldsf(lslot(vs, 1, cc));
-                        const float fx = fterm(Sz[cc], q.lx), fz = fterm(Sx[cc], q.lz);
-                        const float ax = (Sx[cc] - pl[0][cc]) + fx, az = (Sz[cc] - pl[1][cc]) + fz;
-                        pvx[cc] = gmx * ax;
-                        pvz[cc] = gmz * az;
                         const float gax = w[2 * QNB + r * QL + cc] * gmx, gaz = w[3 * QNB + r * QL + cc] * gmz;
                         const float dsx = gax + gaz * fterm_d(Sx[cc], q.lz);  // d S_x: a_x, and a_z through F_z
                         const float dsz = gaz + gax * fterm_d(Sz[cc], q.lx);
@@ -1349,10 +1361,6 @@ __global__ void __launch_bounds__(QP) learned_qc_lms_bwd_kernel(DevGraph g, cons
                             G[1][rr][cc] = rr == r ? dsz - gaz : G[1][rr][cc] + dsz;
                         }
                     }
-                    const float sx_ = reduce12(pvx, static_cast<uint32_t>(lane));
-                    if (wslot >= 0) atomicAdd(&gw_it[2 * QNB + r * QL + wslot], sx_);
-                    const float sz_ = reduce12(pvz, static_cast<uint32_t>(lane));
-                    if (wslot >= 0) atomicAdd(&gw_it[3 * QNB + r * QL + wslot], sz_);
                 }
                 pin_side(G[0]);
                 pin_side(G[1]);
