@@ -49,11 +49,15 @@ void launch_learned_train(const DevGraph &g, const LearnedDevParams &lp, const R
 // and convergence iterations as launch_learned_train<float>, with the
 // messages on chip and the tape as [trial][T+1][72 edges][768] floats in
 // act (the same size).
+// chk: the forward pass's check records for the adjoint (the two smallest
+// input magnitudes, where they are, and the signs of every check row of
+// every iteration), learned_qc_chk_bytes(batch, T) bytes.
 bool learned_qc_train_supported(const DevGraph &g, const LearnedDevParams &lp);
+size_t learned_qc_chk_bytes(int batch, int T);
 void launch_learned_train_qc(const DevGraph &g, const LearnedDevParams &lp, const float *params,
-                             TrialPlanes tp, int batch, float *act, float *tape_d, float *tape_u,
-                             const double *prior, float *loss, float *grad, int *conv, int *counters,
-                             bool backward, hipStream_t st);
+                             TrialPlanes tp, int batch, float *act, uint32_t *chk, float *tape_d,
+                             float *tape_u, const double *prior, float *loss, float *grad, int *conv,
+                             int *counters, bool backward, hipStream_t st);
 
 void launch_adam(double *p, const double *grad, double *m, double *v, float *pf, int count,
                  double lr, int step, hipStream_t st);

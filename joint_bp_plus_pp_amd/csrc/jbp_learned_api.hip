@@ -27,6 +27,7 @@ struct jbp_learned {
     int batch_cap = 0, prec_cap = 0;
     std::vector<void *> bufs;
     void *d_act = nullptr, *d_scratch = nullptr, *d_tape_d = nullptr, *d_tape_u = nullptr;
+    void *d_chk = nullptr;             // check records of the on-chip FP32 trainer
     void *d_loss = nullptr, *d_grad = nullptr;
     double *d_grad64 = nullptr, *d_prior = nullptr;
     DrawThresholds *d_th = nullptr;
@@ -56,6 +57,8 @@ void free_bufs(jbp_learned *m) {
     m->prec_cap = 0;
 }
 
+LearnedDevParams dev_params(const jbp_learned *m);
+
 int ensure_train(jbp_learned *m, int batch, int precision, char *err, size_t errlen) {
     if (batch <= m->batch_cap && precision == m->prec_cap) return JBP_OK;
     free_bufs(m);
@@ -72,7 +75,16 @@ int ensure_train(jbp_learned *m, int batch, int precision, char *err, size_t err
     int rc = JBP_OK;
     void *p = nullptr;
     if ((rc = alloc(b * (m->T + 1) * Et * R, &m->d_act))) return rc;
-    if ((rc = alloc(static_cast<size_t>(m->slots) * learned_slot_elems(ctx->dg) * R, &m->d_scratch))) return rc;
+    if (precision != JBP_FP64 && learned_qc_train_supported(ctx->dg, dev_params(m))) {
+        // the on-chip trainer keeps no per-slot scratch, and records its
+        // check rows for the adjoint
+        m->d_scratch = nullptr;
+        if ((rc = alloc(learned_qc_chk_bytes(batch, m->T), &m->d_chk))) return rc;
+    } else {
+        m->d_chk = nullptr;
+        if ((rc = alloc(static_cast<size_t>(m->slots) * learned_slot_elems(ctx->dg) * R, &m->d_scratch)))
+            return rc;
+    }
     if ((rc = alloc(b * m->T * R, &m->d_tape_d))) return rc;
     if ((rc = alloc(b * m->T * R, &m->d_tape_u))) return rc;
     if ((rc = alloc(R, &m->d_loss))) return rc;
@@ -125,7 +137,8 @@ int run_train(jbp_learned *m, int batch, bool backward, char *err, size_t errlen
     const LearnedDevParams lp = dev_params(m);
     if (sizeof(R) == sizeof(float) && learned_qc_train_supported(ctx->dg, lp)) {
         launch_learned_train_qc(ctx->dg, lp, reinterpret_cast<const float *>(params), ctx->tp, batch,
-                                static_cast<float *>(m->d_act), static_cast<float *>(m->d_tape_d),
+                                static_cast<float *>(m->d_act), static_cast<uint32_t *>(m->d_chk),
+                                static_cast<float *>(m->d_tape_d),
                                 static_cast<float *>(m->d_tape_u), m->d_prior, static_cast<float *>(m->d_loss),
                                 static_cast<float *>(m->d_grad), m->d_conv, m->d_counters, backward,
                                 ctx->stream);

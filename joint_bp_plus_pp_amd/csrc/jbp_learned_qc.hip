@@ -255,6 +255,61 @@ __device__ __forceinline__ void tape_st(tape_rsrc rs, uint32_t voff, uint32_t so
 // bounds the loads in flight (and their registers) to a few blocks' worth.
 __device__ __forceinline__ void tape_fence() { __builtin_amdgcn_sched_barrier(0); }
 
+// Check records: for every iteration, side and block row, the forward
+// pass's check (r, x) leaves three words for the adjoint — where the
+// smallest input magnitude is (i1), where the smallest of the others is
+// (i2), which inputs and raw outputs are negative (bits 8..19, 20..31), and
+// the two smallest magnitudes — so that the adjoint writes the raw outputs
+// without forming mu again.  Per trial [iteration - 1][side * 3 + row][word]
+// [768], coalesced like the tape.
+constexpr uint32_t kChkIterBytes = 6u * 3u * QP * 4u;
+__device__ __forceinline__ uint32_t chk_word(int s, int r, int word) {
+    return static_cast<uint32_t>(((s * QJ + r) * 3 + word) * QP * 4);
+}
+__device__ __forceinline__ uint32_t chk_iter(int it) {
+    uint32_t o = static_cast<uint32_t>(it - 1) * kChkIterBytes;
+    asm volatile("" : "+s"(o));
+    return o;
+}
+__device__ __forceinline__ tape_rsrc make_chk(uint32_t *p) {
+    return __builtin_amdgcn_make_buffer_rsrc(p, 0, 0x7fffffff, 0x00020000);
+}
+__device__ __forceinline__ uint32_t chk_ld(tape_rsrc rs, uint32_t voff, uint32_t soff) {
+    return __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0);
+}
+__device__ __forceinline__ void chk_st(tape_rsrc rs, uint32_t voff, uint32_t soff, uint32_t v) {
+    __builtin_amdgcn_raw_buffer_store_b32(v, rs, voff, soff, 0);
+}
+// the record of one check row: i1, i2, signs of the inputs and of the raw
+// outputs (raw_i < 0), from the inputs' signs and the two minima
+__device__ __forceinline__ uint32_t chk_info(int i1, int i2, uint32_t neg, uint32_t negall, float m1, float m2) {
+    // raw_i = (-1)^(negall ^ neg_i) * (i == i1 ? m2 : m1) is negative when
+    // that sign is and the magnitude is not zero
+    constexpr uint32_t kAll = (1u << QL) - 1u;
+    const uint32_t sgn = (negall ? ~neg : neg) & kAll;
+    const uint32_t b1 = 1u << i1;
+    const uint32_t nz = ((m1 > 0.0f ? kAll : 0u) & ~b1) | (m2 > 0.0f ? b1 : 0u);
+    return static_cast<uint32_t>(i1) | (static_cast<uint32_t>(i2) << 4) | (neg << 8) | ((sgn & nz) << 20);
+}
+// raw output i of a recorded check row
+__device__ __forceinline__ float chk_raw(uint32_t info, float m1, float m2, int i) {
+    const float mag = i == static_cast<int>(info & 15u) ? m2 : m1;
+    return ((info >> (20 + i)) & 1u) ? -mag : mag;
+}
+// the first index of the smallest magnitude other than i1's
+__device__ __forceinline__ int second_index(const float (&mu)[QL], int i1) {
+    int i2 = i1 == 0 ? 1 : 0;
+    float a2 = fabsf(i1 == 0 ? mu[1] : mu[0]);
+#pragma unroll
+    for (int i = 0; i < QL; ++i) {
+        const float a = fabsf(mu[i]);
+        const bool lt = i != i1 && a < a2;
+        i2 = lt ? i : i2;
+        a2 = lt ? a : a2;
+    }
+    return i2;
+}
+
 // Pin one side's messages: the update that forms them is completed here,
 // not sunk past the next barrier with its LDS operands held in registers.
 __device__ __forceinline__ void pin_side(float (&m)[QJ][QL]) {
@@ -280,22 +335,19 @@ __device__ __forceinline__ void stage_weights(float *dst, const float *params, i
 // Check rows of side s: min-sum on the 12 inputs of each check (r, t), in
 // place; returns the number of those checks whose inputs' estimate bits
 // (their least significant bits) disagree with the syndrome.
-__device__ __forceinline__ int qc_check_rows_fwd(uint32_t bbase, const int *tab, int s, uint32_t syn) {
+__device__ __forceinline__ int qc_check_rows_fwd(uint32_t bbase, const int *tab, int s, uint32_t syn,
+                                                 tape_rsrc chk, uint32_t voff, uint32_t crow) {
     int x = threadIdx.x;
     asm volatile("" : "+v"(x));
     int unsat = 0;
 #pragma unroll 1
     for (int r = 0; r < QJ; ++r) {
-        uint32_t ad[QL];
         float mu[QL];
         uint32_t par = 0, neg = 0, negall = (syn >> r) & 1u;
         float m1 = HUGE_VALF, m2 = HUGE_VALF;
         int i1 = -1;
 #pragma unroll
-        for (int i = 0; i < QL; ++i) {
-            ad[i] = check_slot(bbase, tab, s, r, i, x);
-            mu[i] = ldsf(ad[i]);
-        }
+        for (int i = 0; i < QL; ++i) mu[i] = ldsf(check_slot(bbase, tab, s, r, i, x));
 #pragma unroll
         for (int i = 0; i < QL; ++i) {
             par ^= __float_as_uint(mu[i]) & 1u;
@@ -311,11 +363,14 @@ __device__ __forceinline__ int qc_check_rows_fwd(uint32_t bbase, const int *tab,
             m1 = lt1 ? a : m1;
         }
         unsat += (par != ((syn >> r) & 1u)) ? 1 : 0;
+        chk_st(chk, voff, crow + chk_word(s, r, 0), chk_info(i1, second_index(mu, i1), neg, negall, m1, m2));
+        chk_st(chk, voff, crow + chk_word(s, r, 1), __float_as_uint(m1));
+        chk_st(chk, voff, crow + chk_word(s, r, 2), __float_as_uint(m2));
 #pragma unroll
         for (int i = 0; i < QL; ++i) {
             const float mag = i == i1 ? m2 : m1;
             const float out = mag < kLmaxQ ? mag : kLmaxQ;
-            stf(ad[i], ((negall ^ (neg >> i)) & 1u) ? -out : out);
+            stf(check_slot(bbase, tab, s, r, i, x), ((negall ^ (neg >> i)) & 1u) ? -out : out);
         }
     }
     return unsat;
@@ -339,7 +394,8 @@ __device__ __forceinline__ void qc_mu_side(int s, const float (&lx3)[QJ], const 
 }
 
 __global__ void __launch_bounds__(QP) learned_qc_fwd_kernel(DevGraph g, const float *params, int T, int use_mlp,
-                                                           TrialPlanes tp, float *act, float *tape_d, float *tape_u,
+                                                           TrialPlanes tp, float *act, uint32_t *chk,
+                                                           float *tape_d, float *tape_u,
                                                            const double *prior, float *loss, int *conv_out,
                                                            int batch, int *counter) {
     extern __shared__ float B[];  // [J][L][P] | GZ [L][P]
@@ -363,6 +419,7 @@ __global__ void __launch_bounds__(QP) learned_qc_fwd_kernel(DevGraph g, const fl
         if (b >= batch) return;
         QcTrial q = qc_trial(g, tp, prior, b);
         const tape_rsrc tape = make_tape(act + static_cast<size_t>(b) * (T + 1) * row);
+        const tape_rsrc chkr = make_chk(chk + static_cast<size_t>(b) * T * (kChkIterBytes / 4));
         const uint32_t voff = static_cast<uint32_t>(t) * 4u;
         float lam[2][QJ][QL];
 #pragma unroll
@@ -424,7 +481,7 @@ __global__ void __launch_bounds__(QP) learned_qc_fwd_kernel(DevGraph g, const fl
             asm volatile("" : "+v"(ez_bits));
             __syncthreads();
             const float d = s_d;
-            int unsat = qc_check_rows_fwd(bbase, g.qc_tab, 0, q.synx);
+            int unsat = qc_check_rows_fwd(bbase, g.qc_tab, 0, q.synx, chkr, voff, chk_iter(it));
             __syncthreads();
 #pragma unroll
             for (int r = 0; r < QJ; ++r)
@@ -449,7 +506,7 @@ __global__ void __launch_bounds__(QP) learned_qc_fwd_kernel(DevGraph g, const fl
                 }
             }
             __syncthreads();
-            unsat += qc_check_rows_fwd(bbase, g.qc_tab, 1, q.synz);
+            unsat += qc_check_rows_fwd(bbase, g.qc_tab, 1, q.synz, chkr, voff, chk_iter(it));
             __syncthreads();
 #pragma unroll
             for (int r = 0; r < QJ; ++r)
@@ -539,13 +596,14 @@ __global__ void __launch_bounds__(QP) learned_qc_fwd_kernel(DevGraph g, const fl
 // the side's share of its sum's gradient (sum over the column's 3 edges of
 // wv * dL/dmu), and gd has the side's share of dL/dd added.
 template <int S>
-__device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&l)[QJ][QL], float (&acc)[QL], float &gd,
-                                            float (&sx)[QL], float (&sz)[QL], uint32_t bbase, const int *tab,
-                                            const QcTrial &q, tape_rsrc tape, uint32_t voff, int it,
+__device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&acc)[QL], float &gd, float (&sx)[QL],
+                                            float (&sz)[QL], uint32_t bbase, const int *tab, const QcTrial &q,
+                                            tape_rsrc tape, tape_rsrc chkr, uint32_t voff, int it,
                                             const float *w, float *gw_it, float d, int lane) {
-    // l: the side's messages before the iteration (tape row it-1), loaded by
-    // the caller for (2); (4) and (6) load them again, issued at the end of
-    // the check pass before them so that they arrive during its barrier
+    // l: the side's messages before the iteration (tape row it-1), loaded
+    // for (4) and again for (6), each issued before the barrier ahead of the
+    // pass so that they arrive during it
+    float l[QJ][QL];
     auto ld = [&](int r, int cc) { return l[r][cc]; };
     auto reload = [&]() {
         const uint32_t prow = tape_row(it - 1);
@@ -554,79 +612,30 @@ __device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&l)[QJ][Q
 #pragma unroll
             for (int cc = 0; cc < QL; ++cc) l[r][cc] = tape_ld(tape, voff, prow + edge_off(S, r, cc));
     };
-    VarSlots vs = var_slots(bbase);
-    // (2) mu of the side from the messages before the iteration (the side's
-    // own messages from the tape, both sums from the caller)
-#pragma unroll
-    for (int cc = 0; cc < QL; ++cc) {
-        const float l3[QJ] = {ld(0, cc), ld(1, cc), ld(2, cc)};
-        const float f = S == 0 ? fterm(sz[cc], q.lx) : fterm(sx[cc], q.lz);
-        const float e = S == 0 ? sx[cc] + f : sz[cc] + f;
-        const uint32_t est = e < 0.0f ? 1u : 0u;
+    // (3) the check rows' raw outputs, from the forward pass's records of
+    // them (the same minima, signs and positions it used)
+    uint32_t info[QJ];
+    {
+        float m1r[QJ], m2r[QJ];
+        const uint32_t crow = chk_iter(it);
 #pragma unroll
         for (int r = 0; r < QJ; ++r) {
-            const float a = ((S == 0 ? sx[cc] : sz[cc]) - l3[r]) + f;
-            stf(vslot(vs, r, cc), with_lsb(w[2 * QNB + S * QNB + r * QL + cc] * a, est));
+            info[r] = chk_ld(chkr, voff, crow + chk_word(S, r, 0));
+            m1r[r] = __uint_as_float(chk_ld(chkr, voff, crow + chk_word(S, r, 1)));
+            m2r[r] = __uint_as_float(chk_ld(chkr, voff, crow + chk_word(S, r, 2)));
         }
-    }
-    __syncthreads();
-    // (3) the check update again, raw outputs in place; per check the first
-    // index of the smallest |mu| (i1), the first of the rest (i2), and which
-    // inputs and raw outputs are negative
-    const uint32_t syn = S == 0 ? q.synx : q.synz;
-    uint32_t info0 = 0, info1 = 0, info2 = 0;
-    int x = threadIdx.x;
-    asm volatile("" : "+v"(x));
-#pragma unroll 1
-    for (int r = 0; r < QJ; ++r) {
-        float mu[QL];
-        uint32_t neg = 0, negall = (syn >> r) & 1u;
-        float m1 = HUGE_VALF, m2 = HUGE_VALF;
-        int i1 = -1;
+        int x = threadIdx.x;
+        asm volatile("" : "+v"(x));
 #pragma unroll
-        for (int i = 0; i < QL; ++i) mu[i] = ldsf(check_slot(bbase, tab, S, r, i, x));
+        for (int r = 0; r < QJ; ++r)
 #pragma unroll
-        for (int i = 0; i < QL; ++i) {
-            const float a = fabsf(mu[i]);
-            const uint32_t ng = mu[i] < 0.0f ? 1u : 0u;
-            neg |= ng << i;
-            negall ^= ng;
-            // the two smallest magnitudes and the first index of the
-            // smallest, without branches
-            const bool lt1 = a < m1;
-            m2 = lt1 ? m1 : (a < m2 ? a : m2);
-            i1 = lt1 ? i : i1;
-            m1 = lt1 ? a : m1;
-        }
-        // the first index of the smallest |mu| but i1's
-        int i2 = i1 == 0 ? 1 : 0;
-        float a2 = fabsf(i1 == 0 ? mu[1] : mu[0]);
-#pragma unroll
-        for (int i = 0; i < QL; ++i) {
-            const float a = fabsf(mu[i]);
-            const bool lt = i != i1 && a < a2;
-            i2 = lt ? i : i2;
-            a2 = lt ? a : a2;
-        }
-        uint32_t rawneg = 0;
-#pragma unroll
-        for (int i = 0; i < QL; ++i) {
-            const float mag = i == i1 ? m2 : m1;
-            const float raw = ((negall ^ (neg >> i)) & 1u) ? -mag : mag;
-            rawneg |= (raw < 0.0f ? 1u : 0u) << i;
-            stf(check_slot(bbase, tab, S, r, i, x), raw);
-        }
-        const uint32_t info =
-            static_cast<uint32_t>(i1) | (static_cast<uint32_t>(i2) << 4) | (neg << 8) | (rawneg << 20);
-        info0 = r == 0 ? info : info0;
-        info1 = r == 1 ? info : info1;
-        info2 = r == 2 ? info : info2;
+            for (int i = 0; i < QL; ++i) stf(check_slot(bbase, tab, S, r, i, x), chk_raw(info[r], m1r[r], m2r[r], i));
     }
     reload();
     __syncthreads();
-    vs = var_slots(bbase);
-    // the sums re-enter opaque, so that (6) re-forms what (2) formed
-    // instead of keeping (2)'s values live through (3) to (5)
+    VarSlots vs = var_slots(bbase);
+    // the sums re-enter opaque, so that (6) does not keep values from the
+    // caller's passes live through (3) to (5)
     pin_row(sx);
     pin_row(sz);
     // (4) damping and check weights, per edge of the variable owner; the wc
@@ -656,14 +665,15 @@ __device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&l)[QJ][Q
         if (wslot >= 0) atomicAdd(&gw_it[S * QNB + r * QL + wslot], sum);
     }
     pin_side(g);
-    asm volatile("" : "+v"(gd));
     asm volatile("" : "+v"(gd));  // summed here, not sunk past the next passes with its terms held
     __syncthreads();
     // (5) check adjoint: the gradient of each output goes to the input
     // attaining the minimum over the others
+    int x = threadIdx.x;
+    asm volatile("" : "+v"(x));
 #pragma unroll 1
     for (int r = 0; r < QJ; ++r) {
-        const uint32_t inf = r == 0 ? info0 : (r == 1 ? info1 : info2);
+        const uint32_t inf = r == 0 ? info[0] : (r == 1 ? info[1] : info[2]);
         uint32_t ad[QL];
 #pragma unroll
         for (int i = 0; i < QL; ++i) ad[i] = check_slot(bbase, tab, S, r, i, x);
@@ -715,7 +725,8 @@ __device__ __forceinline__ void qc_bwd_side(float (&g)[QJ][QL], float (&l)[QJ][Q
 }
 
 __global__ void __launch_bounds__(QP) learned_qc_bwd_kernel(DevGraph g, const float *params, int T, int use_mlp,
-                                                           TrialPlanes tp, const float *act, const float *tape_d,
+                                                           TrialPlanes tp, const float *act, uint32_t *chk,
+                                                           const float *tape_d,
                                                            const float *tape_u, const double *prior, float *grad,
                                                            int batch, int *counter) {
     extern __shared__ float dyn[];  // B [J][L][P] | weight gradients [T][4][nb]
@@ -747,6 +758,7 @@ __global__ void __launch_bounds__(QP) learned_qc_bwd_kernel(DevGraph g, const fl
         // held in cross) while the Z side is undone, then the Z side's for
         // iteration k-1.
         const tape_rsrc tape = make_tape(act + static_cast<size_t>(b) * (T + 1) * row);
+        const tape_rsrc chkr = make_chk(chk + static_cast<size_t>(b) * T * (kChkIterBytes / 4));
         const uint32_t voff = static_cast<uint32_t>(t) * 4u;
         float cross[QL];  // the X side's pending share of the sums' gradients, per column block
 #pragma unroll
@@ -794,6 +806,7 @@ __global__ void __launch_bounds__(QP) learned_qc_bwd_kernel(DevGraph g, const fl
                         zr[r][cc] = tape_ld(tape, voff, prow + edge_off(1, r, cc));
                     }
             }
+            // (the X messages and the Z messages serve the sums only)
 #pragma unroll
             for (int r = 0; r < QJ; ++r)
 #pragma unroll
@@ -826,7 +839,7 @@ __global__ void __launch_bounds__(QP) learned_qc_bwd_kernel(DevGraph g, const fl
                 for (int cc = 0; cc < QL; ++cc) tape_st(tape, voff, krow + edge_off(1, 0, cc), gsz[cc]);
             }
             float accx[QL];
-            qc_bwd_side<0>(gv, lr, accx, gd, sxc, szc, bbase, g.qc_tab, q, tape, voff, it, w, gw_it, d, lane);
+            qc_bwd_side<0>(gv, accx, gd, sxc, szc, bbase, g.qc_tab, q, tape, chkr, voff, it, w, gw_it, d, lane);
             {
                 const uint32_t krow = tape_row(it);
 #pragma unroll
@@ -834,22 +847,17 @@ __global__ void __launch_bounds__(QP) learned_qc_bwd_kernel(DevGraph g, const fl
 #pragma unroll
                     for (int cc = 0; cc < QL; ++cc) tape_st(tape, voff, krow + edge_off(0, r, cc), gv[r][cc]);
             }
-            // Z side: parked gradient plus the loss term, and the Z messages
-            // of row it-1, in one round trip
+            // Z side: parked gradient plus the loss term, in one round trip
             {
                 float lz[QL];
                 const uint32_t jrow = tape_row(it);
                 const uint32_t krow = tape_row(parked ? it + 1 : it);
-                const uint32_t prow = tape_row(it - 1);
 #pragma unroll
                 for (int cc = 0; cc < QL; ++cc) lz[cc] = tape_ld(tape, voff, jrow + edge_off(1, 0, cc));
 #pragma unroll
                 for (int r = 0; r < QJ; ++r)
 #pragma unroll
-                    for (int cc = 0; cc < QL; ++cc) {
-                        gv[r][cc] = tape_ld(tape, voff, krow + edge_off(1, r, cc));
-                        lr[r][cc] = tape_ld(tape, voff, prow + edge_off(1, r, cc));
-                    }
+                    for (int cc = 0; cc < QL; ++cc) gv[r][cc] = tape_ld(tape, voff, krow + edge_off(1, r, cc));
 #pragma unroll
                 for (int r = 0; r < QJ; ++r)
 #pragma unroll
@@ -857,7 +865,7 @@ __global__ void __launch_bounds__(QP) learned_qc_bwd_kernel(DevGraph g, const fl
             }
             pin_side(gv);
             float accz[QL];
-            qc_bwd_side<1>(gv, lr, accz, gd, sxc, szc, bbase, g.qc_tab, q, tape, voff, it, w, gw_it, d, lane);
+            qc_bwd_side<1>(gv, accz, gd, sxc, szc, bbase, g.qc_tab, q, tape, chkr, voff, it, w, gw_it, d, lane);
             // the sums' shares: S_x reaches the X messages directly and the
             // Z side's channel term, S_z likewise
 #pragma unroll
@@ -960,7 +968,8 @@ __device__ __forceinline__ uint32_t lcheck_slot(uint32_t bbase, const int *tab, 
 // returns 1 when its inputs' least significant bits disagree with the
 // syndrome bit.
 __device__ __forceinline__ uint32_t lms_check_row_fwd(uint32_t bbase, const int *tab, int s, int r, int x,
-                                                      uint32_t synbit) {
+                                                      uint32_t synbit, tape_rsrc chk, uint32_t voff,
+                                                      uint32_t crow) {
     float mu[QL];
     uint32_t par = 0, neg = 0, negall = synbit;
     float m1 = HUGE_VALF, m2 = HUGE_VALF;
@@ -979,6 +988,9 @@ __device__ __forceinline__ uint32_t lms_check_row_fwd(uint32_t bbase, const int 
         i1 = lt1 ? i : i1;
         m1 = lt1 ? a : m1;
     }
+    chk_st(chk, voff, crow + chk_word(s, r, 0), chk_info(i1, second_index(mu, i1), neg, negall, m1, m2));
+    chk_st(chk, voff, crow + chk_word(s, r, 1), __float_as_uint(m1));
+    chk_st(chk, voff, crow + chk_word(s, r, 2), __float_as_uint(m2));
 #pragma unroll
     for (int i = 0; i < QL; ++i) {
         const float mag = i == i1 ? m2 : m1;
@@ -989,7 +1001,8 @@ __device__ __forceinline__ uint32_t lms_check_row_fwd(uint32_t bbase, const int 
 }
 
 __global__ void __launch_bounds__(QP) learned_qc_lms_fwd_kernel(DevGraph g, const float *params, int T,
-                                                               TrialPlanes tp, float *act, const double *prior,
+                                                               TrialPlanes tp, float *act, uint32_t *chk,
+                                                               const double *prior,
                                                                float *loss, int *conv_out, int batch,
                                                                int *counter) {
     extern __shared__ float B[];  // two layer buffers [2][side][L][P]
@@ -1008,6 +1021,7 @@ __global__ void __launch_bounds__(QP) learned_qc_lms_fwd_kernel(DevGraph g, cons
         if (b >= batch) return;
         QcTrial q = qc_trial(g, tp, prior, b);
         const tape_rsrc tape = make_tape(act + static_cast<size_t>(b) * (T + 1) * row);
+        const tape_rsrc chkr = make_chk(chk + static_cast<size_t>(b) * T * (kChkIterBytes / 4));
         const uint32_t voff = static_cast<uint32_t>(t) * 4u;
         float lam[2][QJ][QL];
 #pragma unroll
@@ -1054,8 +1068,9 @@ __global__ void __launch_bounds__(QP) learned_qc_lms_fwd_kernel(DevGraph g, cons
                 {
                     int x = t;
                     asm volatile("" : "+v"(x));
-                    flag |= lms_check_row_fwd(bbase, g.qc_tab, 0, r, x, (q.synx >> r) & 1u);
-                    flag |= lms_check_row_fwd(bbase, g.qc_tab, 1, r, x, (q.synz >> r) & 1u);
+                    const uint32_t crow = chk_iter(it);
+                    flag |= lms_check_row_fwd(bbase, g.qc_tab, 0, r, x, (q.synx >> r) & 1u, chkr, voff, crow);
+                    flag |= lms_check_row_fwd(bbase, g.qc_tab, 1, r, x, (q.synz >> r) & 1u, chkr, voff, crow);
                 }
                 __syncthreads();
                 const VarSlots vr = var_slots_l(bbase);
@@ -1108,8 +1123,9 @@ __global__ void __launch_bounds__(QP) learned_qc_lms_fwd_kernel(DevGraph g, cons
 // own messages read again before the last pass); layer 0's estimate, which
 // rode in every mu of the iteration, comes from row t-1's sums.
 __global__ void __launch_bounds__(QP) learned_qc_lms_bwd_kernel(DevGraph g, const float *params, int T,
-                                                               TrialPlanes tp, float *act, const double *prior,
-                                                               float *grad, int batch, int *counter) {
+                                                               TrialPlanes tp, float *act, uint32_t *chk,
+                                                               const double *prior, float *grad, int batch,
+                                                               int *counter) {
     extern __shared__ float dyn[];  // layer buffer [side][L][P] | weight gradients [T][4][nb]
     float *s_gw = dyn + 2 * QL * QP;
     __shared__ int s_trial;
@@ -1128,6 +1144,7 @@ __global__ void __launch_bounds__(QP) learned_qc_lms_bwd_kernel(DevGraph g, cons
         if (b >= batch) break;
         QcTrial q = qc_trial(g, tp, prior, b);
         const tape_rsrc tape = make_tape(act + static_cast<size_t>(b) * (T + 1) * row);
+        const tape_rsrc chkr = make_chk(chk + static_cast<size_t>(b) * T * (kChkIterBytes / 4));
         const uint32_t voff = static_cast<uint32_t>(t) * 4u;
         float G[2][QJ][QL];  // dL / d lambda of the state after the step being undone
 #pragma unroll
@@ -1175,28 +1192,22 @@ __global__ void __launch_bounds__(QP) learned_qc_lms_bwd_kernel(DevGraph g, cons
             }
             pin_side(G[0]);
             pin_side(G[1]);
-            // layer 0's estimate of iteration it, from row it-1's sums
-            uint32_t e0x = 0, e0z = 0;
-            {
-                const uint32_t krow = tape_row(it - 1);
-#pragma unroll
-                for (int cc = 0; cc < QL; ++cc) {
-                    auto ldp = [&](int s, int r) { return tape_ld(tape, voff, krow + edge_off(s, r, cc)); };
-                    const float sx = (ldp(0, 0) + ldp(0, 1)) + ldp(0, 2);
-                    const float sz = (ldp(1, 0) + ldp(1, 1)) + ldp(1, 2);
-                    e0x |= (sx + fterm(sz, q.lx) < 0.0f ? 1u : 0u) << cc;
-                    e0z |= (sz + fterm(sx, q.lz) < 0.0f ? 1u : 0u) << cc;
-                    if ((cc & 3) == 3) tape_fence();
-                }
-            }
-            asm volatile("" : "+v"(e0x), "+v"(e0z));
 #pragma unroll
             for (int r = QJ - 1; r >= 0; --r) {
-                // (2) the state before step (it, r), its sums, and mu of row r
+                // (2) the sums of the state before step (it, r); (3) row r's
+                // raw check outputs from the forward pass's records
                 float Sx[QL], Sz[QL];
+                uint32_t info0, info1;
                 {
+                    float m1r[2], m2r[2];
+                    const uint32_t hrow = chk_iter(it);
+                    info0 = chk_ld(chkr, voff, hrow + chk_word(0, r, 0));
+                    m1r[0] = __uint_as_float(chk_ld(chkr, voff, hrow + chk_word(0, r, 1)));
+                    m2r[0] = __uint_as_float(chk_ld(chkr, voff, hrow + chk_word(0, r, 2)));
+                    info1 = chk_ld(chkr, voff, hrow + chk_word(1, r, 0));
+                    m1r[1] = __uint_as_float(chk_ld(chkr, voff, hrow + chk_word(1, r, 1)));
+                    m2r[1] = __uint_as_float(chk_ld(chkr, voff, hrow + chk_word(1, r, 2)));
                     const uint32_t crow = tape_row(it), prow = tape_row(it - 1);
-                    VarSlots vs = var_slots_l(bbase);
 #pragma unroll
                     for (int cc = 0; cc < QL; ++cc) {
                         float x[2][QJ];
@@ -1207,64 +1218,18 @@ __global__ void __launch_bounds__(QP) learned_qc_lms_bwd_kernel(DevGraph g, cons
                                 x[s][rr] = tape_ld(tape, voff, (rr < r ? crow : prow) + edge_off(s, rr, cc));
                         Sx[cc] = (x[0][0] + x[0][1]) + x[0][2];
                         Sz[cc] = (x[1][0] + x[1][1]) + x[1][2];
-                        const float fx = fterm(Sz[cc], q.lx), fz = fterm(Sx[cc], q.lz);
-                        const float ax = (Sx[cc] - x[0][r]) + fx, az = (Sz[cc] - x[1][r]) + fz;
-                        stf(lslot(vs, 0, cc), with_lsb(w[2 * QNB + r * QL + cc] * ax, (e0x >> cc) & 1u));
-                        stf(lslot(vs, 1, cc), with_lsb(w[3 * QNB + r * QL + cc] * az, (e0z >> cc) & 1u));
                         if ((cc & 3) == 3) tape_fence();
+                    }
+                    int x = t;
+                    asm volatile("" : "+v"(x));
+#pragma unroll
+                    for (int i = 0; i < QL; ++i) {
+                        stf(lcheck_slot(bbase, g.qc_tab, 0, r, i, x), chk_raw(info0, m1r[0], m2r[0], i));
+                        stf(lcheck_slot(bbase, g.qc_tab, 1, r, i, x), chk_raw(info1, m1r[1], m2r[1], i));
                     }
                 }
                 pin_row(Sx);
                 pin_row(Sz);
-                __syncthreads();
-                // (3) row r's checks of both sides again, raw outputs in place
-                uint32_t info0 = 0, info1 = 0;
-                {
-                    int x = t;
-                    asm volatile("" : "+v"(x));
-#pragma unroll 1
-                    for (int s = 0; s < 2; ++s) {
-                        const uint32_t syn = ((s == 0 ? q.synx : q.synz) >> r) & 1u;
-                        float mu[QL];
-                        uint32_t neg = 0, negall = syn;
-                        float m1 = HUGE_VALF, m2 = HUGE_VALF;
-                        int i1 = -1;
-#pragma unroll
-                        for (int i = 0; i < QL; ++i) mu[i] = ldsf(lcheck_slot(bbase, g.qc_tab, s, r, i, x));
-#pragma unroll
-                        for (int i = 0; i < QL; ++i) {
-                            const float a = fabsf(mu[i]);
-                            const uint32_t ng = mu[i] < 0.0f ? 1u : 0u;
-                            neg |= ng << i;
-                            negall ^= ng;
-                            const bool lt1 = a < m1;
-                            m2 = lt1 ? m1 : (a < m2 ? a : m2);
-                            i1 = lt1 ? i : i1;
-                            m1 = lt1 ? a : m1;
-                        }
-                        int i2 = i1 == 0 ? 1 : 0;
-                        float a2 = fabsf(i1 == 0 ? mu[1] : mu[0]);
-#pragma unroll
-                        for (int i = 0; i < QL; ++i) {
-                            const float a = fabsf(mu[i]);
-                            const bool lt = i != i1 && a < a2;
-                            i2 = lt ? i : i2;
-                            a2 = lt ? a : a2;
-                        }
-                        uint32_t rawneg = 0;
-#pragma unroll
-                        for (int i = 0; i < QL; ++i) {
-                            const float mag = i == i1 ? m2 : m1;
-                            const float raw = ((negall ^ (neg >> i)) & 1u) ? -mag : mag;
-                            rawneg |= (raw < 0.0f ? 1u : 0u) << i;
-                            stf(lcheck_slot(bbase, g.qc_tab, s, r, i, x), raw);
-                        }
-                        const uint32_t info = static_cast<uint32_t>(i1) | (static_cast<uint32_t>(i2) << 4) |
-                                              (neg << 8) | (rawneg << 20);
-                        info0 = s == 0 ? info : info0;
-                        info1 = s == 1 ? info : info1;
-                    }
-                }
                 __syncthreads();
                 // (4) check weights: their gradient, and the gradient of the
                 // raw check outputs
@@ -1391,9 +1356,14 @@ bool learned_qc_train_supported(const DevGraph &g, const LearnedDevParams &lp) {
            lp.T <= kMaxLearnedT;
 }
 
+size_t learned_qc_chk_bytes(int batch, int T) {
+    return static_cast<size_t>(batch) * static_cast<size_t>(T) * kChkIterBytes;
+}
+
 void launch_learned_train_qc(const DevGraph &g, const LearnedDevParams &lp, const float *params, TrialPlanes tp,
-                             int batch, float *act, float *tape_d, float *tape_u, const double *prior, float *loss,
-                             float *grad, int *conv, int *counters, bool backward, hipStream_t st) {
+                             int batch, float *act, uint32_t *chk, float *tape_d, float *tape_u,
+                             const double *prior, float *loss, float *grad, int *conv, int *counters, bool backward,
+                             hipStream_t st) {
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -1407,25 +1377,25 @@ void launch_learned_train_qc(const DevGraph &g, const LearnedDevParams &lp, cons
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(learned_qc_lms_fwd_kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lfwd_lds));
         hipLaunchKernelGGL(learned_qc_lms_fwd_kernel, dim3(grid), dim3(QP), lfwd_lds, st, g, params, lp.T, tp, act,
-                           prior, loss, conv, batch, counters);
+                           chk, prior, loss, conv, batch, counters);
         if (!backward) return;
         const size_t lbwd_lds = static_cast<size_t>(2) * QL * QP * sizeof(float) +
                                 static_cast<size_t>(QW) * lp.T * sizeof(float);
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(learned_qc_lms_bwd_kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lbwd_lds));
         hipLaunchKernelGGL(learned_qc_lms_bwd_kernel, dim3(grid), dim3(QP), lbwd_lds, st, g, params, lp.T, tp, act,
-                           prior, grad, batch, counters + 1);
+                           chk, prior, grad, batch, counters + 1);
         return;
     }
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(learned_qc_fwd_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(fwd_lds));
     hipLaunchKernelGGL(learned_qc_fwd_kernel, dim3(grid), dim3(QP), fwd_lds, st, g, params, lp.T, lp.use_mlp, tp,
-                       act, tape_d, tape_u, prior, loss, conv, batch, counters);
+                       act, chk, tape_d, tape_u, prior, loss, conv, batch, counters);
     if (!backward) return;
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(learned_qc_bwd_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bwd_lds));
     hipLaunchKernelGGL(learned_qc_bwd_kernel, dim3(grid), dim3(QP), bwd_lds, st, g, params, lp.T, lp.use_mlp, tp,
-                       act, tape_d, tape_u, prior, grad, batch, counters + 1);
+                       act, chk, tape_d, tape_u, prior, grad, batch, counters + 1);
 }
 
 }  // namespace jbp
