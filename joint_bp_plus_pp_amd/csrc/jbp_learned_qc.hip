@@ -1,38 +1,40 @@
-// Learned flooding min-sum training (kind 1, FP32) for the permutation-block
-// code of the params files (P = 768, J = 3, L = 12): forward pass with its
-// tape and adjoint pass, on chip.  No counterpart in the reference program.
+// Learned min-sum training in FP32 — the flooding kind (kind 1, with its MLP
+// damping) and the layered kind (kind 2, the headline decoder's model) — for
+// the permutation-block code of the params files (P = 768, J = 3, L = 12):
+// forward passes with their tape and adjoint passes, on chip.  No
+// counterpart in the reference program.
 //
-// The model is the generic training kernels' (jbp_learned.hip, KIND 1):
+// The models are the generic training kernels' (jbp_learned.hip).  Flooding:
 //   mu_e     = wv[g][b][t] * (S_g(v) - lambda_e + F_g(S_other(v)))
 //   lambda_e = (1 - d_t) * wc[g][b][t] * CN_e(mu) + d_t * lambda_e(t-1)
 // with the min-sum check node clamped at 64 ln 2, F(L) = max(lI + L, lZ) -
 // max(lX + L, lY) in natural logs, d_t from the 2-16-1 MLP of (t / T,
 // unsatisfied fraction of the estimate two iterations back), and the BCE of
-// the X / Z marginals averaged over iterations 1..T.
+// the X / Z marginals averaged over iterations 1..T.  The layered kind runs
+// the same updates layer by layer (below).
 //
 // Layout: one 768-thread workgroup per trial; thread t owns the variables
 // (cc, t) and the checks (r, t).  The variable owner keeps the 72 messages
 // lambda of its variables' edges in registers (forward) or their gradients
-// (adjoint).  The two views meet in one LDS exchange buffer, variable-major:
-// the edge of row r and column block cc at variable (cc, y) sits in
-// B[r][cc][y], so the variable owner's slots are fixed offsets from one
-// base, and the check owner (r, x) forms the address of its input i from the
-// block's permutation, y = (a x + b) mod P, when it needs it (four
-// instructions per input, in place of 36 registers of addresses).  The
-// buffer serves one side at a time.  The tape is [trial][k = 0..T][72
-// edges][768 threads] floats, written as whole wavefront rows.  Per
-// iteration and side the forward exchanges mu (check owners read it and
-// write the clamped outputs back); the adjoint exchanges mu (the check
-// recomputed; its raw outputs back), the gradient of the raw outputs, and
-// the gradient of mu.  The estimate bit of a variable rides in the least
-// significant bit of every mu it sends, so the check owners count
-// unsatisfied checks while reading their rows (the forward and the adjoint
-// form mu bit for bit alike, so they take the same minima).  The adjoint
-// holds one side's gradients in registers and parks the other's in the
-// tape row its loss term has consumed.  Weight gradients of a block row are
-// summed over the wavefront in one transposing reduction and accumulated
-// per iteration and block in LDS, then added to the global gradient once
-// per workgroup.
+// (adjoint).  The two views meet in an LDS exchange buffer, variable-major:
+// the edge of row r and column block cc at variable (cc, y) sits at [r][cc]
+// [y], so the variable owner's slots are fixed offsets from one base, and
+// the check owner (r, x) forms the address of its input i from the block's
+// permutation, y = (a x + b) mod P, when it needs it (four instructions per
+// input, in place of 36 registers of addresses).  The tape is [trial][k =
+// 0..T][72 edges][768 threads] floats, written as whole wavefront rows.  The
+// forward exchanges mu (check owners read it and write the clamped outputs
+// back), with the estimate bit of a variable in the least significant bit
+// of every mu it sends, so the check owners count unsatisfied checks while
+// reading their rows; it also records every check row (where its two
+// smallest input magnitudes are, the signs, the two magnitudes).  The
+// adjoint writes each check's raw outputs from its record, then exchanges
+// the gradient of the raw outputs and the gradient of mu.  The flooding
+// adjoint holds one side's gradients in registers and parks the other's in
+// the tape row its loss term has consumed.  Weight gradients of a block row
+// are summed over the wavefront in one transposing reduction and
+// accumulated per iteration and block in LDS, then added to the global
+// gradient once per workgroup.
 #include "jbp_common.h"
 #include "jbp_learned.h"
 
