@@ -100,6 +100,30 @@ def test_fp32_trainer_follows_fp64_at_t25(main_decoder, main_code_oracle, use_ml
     model.destroy()
 
 
+@pytest.mark.parametrize("kind,T", [(jbp.LEARNED_MIN_SUM, 1), (jbp.LEARNED_MIN_SUM, 64),
+                                    (jbp.LEARNED_LAYERED_MIN_SUM, 1), (jbp.LEARNED_LAYERED_MIN_SUM, 64)])
+def test_fp32_trainer_at_the_ends_of_its_range(main_decoder, main_code_oracle, kind, T):
+    # the on-chip FP32 trainer at T = 1 and at the largest T (its LDS
+    # accumulators are sized by T), on 5 trials (fewer trials than
+    # workgroups), against the FP64 trainer
+    model = jbp.LearnedModel(main_decoder, T, kind=kind)
+    params = ms_params(model, 23)
+    use_mlp = kind == jbp.LEARNED_MIN_SUM
+    model.set_params(params, use_mlp=use_mlp)
+    p = np.array([0.03, 0.035, 0.04, 0.045, 0.05])
+    errs = np.stack([main_code_oracle.sample(77, 1 + t, float(p[t])) for t in range(len(p))])
+    l64, g64, c64 = model.loss_grad(errs, p, precision=jbp.FP64)
+    l32, g32, c32 = model.loss_grad(errs, p, precision=jbp.FP32)
+    cos = g32 @ g64 / (np.linalg.norm(g32) * np.linalg.norm(g64))
+    print("kind %d T=%d: loss fp32 %.9g fp64 %.9g; gradient cosine %.7f; conv %s / %s"
+          % (kind, T, l32, l64, cos, c32, c64), flush=True)
+    assert np.isfinite(l32) and np.all(np.isfinite(g32))
+    assert abs(l32 - l64) <= 1e-4 * abs(l64)
+    assert cos > 0.9999
+    assert np.array_equal(c32, c64)
+    model.destroy()
+
+
 def test_fp16_inference_follows_the_training_forward_pass(main_decoder):
     # the inference phase (FP16 pairs, base 2) and the trainer's forward
     # pass (FP32, natural log) are the same model: the first iteration whose
