@@ -453,15 +453,28 @@ __global__ void __launch_bounds__(QP) learned_qc_fwd_kernel(DevGraph g, const fl
             stage_weights(s_w[(it + 1) & 1], params, T, it + 1);
             // counter it & 1 counts the estimate of iteration it - 1; the
             // other one was last read before this iteration's first barrier
-            if (t == 0) {
-                float h[kMlpHidden];
+            // the damping from the MLP on the first wavefront, one hidden
+            // unit per lane (the same units as the adjoint's; their sum
+            // over the lanes)
+            if (t < 64) {
                 const float x0 = static_cast<float>(it) / static_cast<float>(T);
                 const float x1 = s_ubar[it >= 2 ? it - 2 : 0];
-                const float d = use_mlp ? mlp_fwd(mlp, x0, x1, h) : 0.0f;
-                s_d = d;
-                tape_d[static_cast<size_t>(b) * T + it - 1] = d;
-                tape_u[static_cast<size_t>(b) * T + it - 1] = x1;
-                s_cnt[it & 1] = 0;
+                float d = 0.0f;
+                if (use_mlp) {
+                    float v = 0.0f;
+                    if (t < kMlpHidden)
+                        v = mlp[3 * kMlpHidden + t] *
+                            tanhf(mlp[2 * t] * x0 + mlp[2 * t + 1] * x1 + mlp[2 * kMlpHidden + t]);
+#pragma unroll
+                    for (int m = kMlpHidden / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+                    d = sigmf(mlp[4 * kMlpHidden] + v);
+                }
+                if (t == 0) {
+                    s_d = d;
+                    tape_d[static_cast<size_t>(b) * T + it - 1] = d;
+                    tape_u[static_cast<size_t>(b) * T + it - 1] = x1;
+                    s_cnt[it & 1] = 0;
+                }
             }
             // variable pass, X side (the Z side's channel terms and estimate
             // bits kept for after the X update)
