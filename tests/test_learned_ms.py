@@ -124,6 +124,54 @@ def test_fp32_trainer_at_the_ends_of_its_range(main_decoder, main_code_oracle, k
     model.destroy()
 
 
+@pytest.mark.parametrize("kind,eps,weights_only", [(jbp.LEARNED_SUM_PRODUCT, 1e-3, True),
+                                                   (jbp.LEARNED_SUM_PRODUCT, 1e-6, False),
+                                                   (jbp.LEARNED_MIN_SUM, 1e-6, False)])
+def test_fp64_gradient_central_differences_at_t25(main_decoder, main_code_oracle, kind, eps, weights_only):
+    # T = 25 with the MLP damping: the FP64 adjoint against central
+    # differences, relative error < 1e-3.  Both losses are piecewise smooth:
+    # the MLP's second input (the unsatisfied fraction of a hard-decision
+    # estimate) moves in steps, and the min-sum loss also where an argument
+    # minimum changes.  A parameter counts only where its two one-sided
+    # differences agree within 1% (curvature moves them apart by under 0.5%
+    # at eps = 1e-3); at least 8 of those probed must.  eps = 1e-3 on the
+    # sum-product model's weights (8 of them: early and late iterations, both
+    # sides and kinds); eps = 1e-6 on those and on MLP parameters of both
+    # kinds (an MLP parameter shifts the damping of every iteration, and the
+    # steps around it are too dense for eps = 1e-3).
+    T = 25
+    model = jbp.LearnedModel(main_decoder, T, kind=kind)
+    params = ms_params(model, 29)
+    model.set_params(params, use_mlp=True)
+    p = np.array([0.04, 0.05])
+    errs = np.stack([main_code_oracle.sample(5, t, float(p[t - 1])) for t in (1, 2)])
+    l0, grad, _ = model.loss_grad(errs, p, precision=jbp.FP64)
+    nb = model.nb
+    k = 2 * nb * T
+    picks = [0, 7 * T + 3, nb * T + 1, 2 * nb * T - 2, k, k + 5 * T, k + nb * T + 2, 2 * k - 1]
+    if not weights_only:
+        picks += [2 * k, 2 * k + 1, 2 * k + 17, 2 * k + 33, 2 * k + 40, 2 * k + 48, 2 * k + 55, model.count - 1]
+    smooth = 0
+    for i in picks:
+        q = params.copy()
+        q[i] += eps
+        model.set_params(q, use_mlp=True)
+        lp = model.loss_grad(errs, p, precision=jbp.FP64, grad=False)[0]
+        q[i] -= 2 * eps
+        model.set_params(q, use_mlp=True)
+        lm = model.loss_grad(errs, p, precision=jbp.FP64, grad=False)[0]
+        fp, fm, fd = (lp - l0) / eps, (l0 - lm) / eps, (lp - lm) / (2 * eps)
+        agree = abs(fp - fm) <= 0.01 * abs(fd)
+        rel = abs(grad[i] - fd) / max(abs(fd), 1e-12)
+        print("kind %d eps %g param %d: adjoint %.9g central %.9g (one-sided %.9g, %.9g) rel %.2e %s"
+              % (kind, eps, i, grad[i], fd, fp, fm, rel, "smooth" if agree else "step"), flush=True)
+        if agree:
+            smooth += 1
+            assert rel < 1e-3 or abs(grad[i] - fd) < 1e-9
+    assert smooth >= 8
+    model.destroy()
+
+
 def test_fp16_inference_follows_the_training_forward_pass(main_decoder):
     # the inference phase (FP16 pairs, base 2) and the trainer's forward
     # pass (FP32, natural log) are the same model: the first iteration whose
