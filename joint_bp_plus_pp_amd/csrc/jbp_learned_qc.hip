@@ -298,19 +298,6 @@ __device__ __forceinline__ float chk_raw(uint32_t info, float m1, float m2, int 
     const float mag = i == static_cast<int>(info & 15u) ? m2 : m1;
     return ((info >> (20 + i)) & 1u) ? -mag : mag;
 }
-// the first index of the smallest magnitude other than i1's
-__device__ __forceinline__ int second_index(const float (&mu)[QL], int i1) {
-    int i2 = i1 == 0 ? 1 : 0;
-    float a2 = fabsf(i1 == 0 ? mu[1] : mu[0]);
-#pragma unroll
-    for (int i = 0; i < QL; ++i) {
-        const float a = fabsf(mu[i]);
-        const bool lt = i != i1 && a < a2;
-        i2 = lt ? i : i2;
-        a2 = lt ? a : a2;
-    }
-    return i2;
-}
 
 // Pin one side's messages: the update that forms them is completed here,
 // not sunk past the next barrier with its LDS operands held in registers.
@@ -347,7 +334,7 @@ __device__ __forceinline__ int qc_check_rows_fwd(uint32_t bbase, const int *tab,
         float mu[QL];
         uint32_t par = 0, neg = 0, negall = (syn >> r) & 1u;
         float m1 = HUGE_VALF, m2 = HUGE_VALF;
-        int i1 = -1;
+        int i1 = -1, i2 = -1;
 #pragma unroll
         for (int i = 0; i < QL; ++i) mu[i] = ldsf(check_slot(bbase, tab, s, r, i, x));
 #pragma unroll
@@ -359,13 +346,14 @@ __device__ __forceinline__ int qc_check_rows_fwd(uint32_t bbase, const int *tab,
             negall ^= ng;
             // the two smallest magnitudes and the first index of the
             // smallest, without branches
-            const bool lt1 = a < m1;
-            m2 = lt1 ? m1 : (a < m2 ? a : m2);
+            const bool lt1 = a < m1, lt2 = a < m2;
+            i2 = lt1 ? i1 : (lt2 ? i : i2);  // the first index of the minimum over the others
+            m2 = lt1 ? m1 : (lt2 ? a : m2);
             i1 = lt1 ? i : i1;
             m1 = lt1 ? a : m1;
         }
         unsat += (par != ((syn >> r) & 1u)) ? 1 : 0;
-        chk_st(chk, voff, crow + chk_word(s, r, 0), chk_info(i1, second_index(mu, i1), neg, negall, m1, m2));
+        chk_st(chk, voff, crow + chk_word(s, r, 0), chk_info(i1, i2, neg, negall, m1, m2));
         chk_st(chk, voff, crow + chk_word(s, r, 1), __float_as_uint(m1));
         chk_st(chk, voff, crow + chk_word(s, r, 2), __float_as_uint(m2));
 #pragma unroll
@@ -988,7 +976,7 @@ __device__ __forceinline__ uint32_t lms_check_row_fwd(uint32_t bbase, const int 
     float mu[QL];
     uint32_t par = 0, neg = 0, negall = synbit;
     float m1 = HUGE_VALF, m2 = HUGE_VALF;
-    int i1 = -1;
+    int i1 = -1, i2 = -1;
 #pragma unroll
     for (int i = 0; i < QL; ++i) mu[i] = ldsf(lcheck_slot(bbase, tab, s, r, i, x));
 #pragma unroll
@@ -998,12 +986,13 @@ __device__ __forceinline__ uint32_t lms_check_row_fwd(uint32_t bbase, const int 
         const uint32_t ng = mu[i] < 0.0f ? 1u : 0u;
         neg |= ng << i;
         negall ^= ng;
-        const bool lt1 = a < m1;
-        m2 = lt1 ? m1 : (a < m2 ? a : m2);
+        const bool lt1 = a < m1, lt2 = a < m2;
+        i2 = lt1 ? i1 : (lt2 ? i : i2);  // the first index of the minimum over the others
+        m2 = lt1 ? m1 : (lt2 ? a : m2);
         i1 = lt1 ? i : i1;
         m1 = lt1 ? a : m1;
     }
-    chk_st(chk, voff, crow + chk_word(s, r, 0), chk_info(i1, second_index(mu, i1), neg, negall, m1, m2));
+    chk_st(chk, voff, crow + chk_word(s, r, 0), chk_info(i1, i2, neg, negall, m1, m2));
     chk_st(chk, voff, crow + chk_word(s, r, 1), __float_as_uint(m1));
     chk_st(chk, voff, crow + chk_word(s, r, 2), __float_as_uint(m2));
 #pragma unroll
